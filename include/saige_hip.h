@@ -1,0 +1,218 @@
+/*
+ * saige_hip.h -- C ABI of libsaigehip, the MI355X (gfx950) implementation of
+ * SAIGE step 1's hot path: PLINK decode / QC / standardisation, the resident
+ * GRM built by an MFMA SYRK, the PCG solve on Sigma = tau0 W^-1 + tau1 K, and
+ * the block PCG that the trace, AI-REML and serving loops run on.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative SAIGE_E* code on
+ *     failure; saige_last_error() then holds a message.  No C++ exception
+ *     crosses this boundary (the reference turns its C++ exceptions into R
+ *     errors with BEGIN_RCPP/END_RCPP, src/SAIGE/src/RcppExports.cpp).
+ *   - Pointers named *_host are host memory owned by the caller; the library
+ *     copies what it needs.  Device memory is owned by the context.
+ *   - One context per device; calls on one context are synchronous and must
+ *     not overlap (the reference's state is the process-global genoClass
+ *     geno, FG.cpp:1188, driven from R's single thread).
+ *   - Vectors are FP32, length Nnomissing (the phenotyped-sample count),
+ *     in subSampleInGeno order.  Multi-column arguments are column-major
+ *     (column c starts at c * Nnomissing), like arma::fmat.
+ *
+ * FG.cpp below is src/SAIGE/src/SAIGE_fitGLMM_fast.cpp and GPU.hpp is
+ * src/SAIGE/src/gpuSymMatMult.hpp of the reference.
+ */
+#ifndef SAIGE_HIP_H
+#define SAIGE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SAIGE_OK 0
+#define SAIGE_E_ARG (-1)      /* invalid argument / size mismatch        */
+#define SAIGE_E_STATE (-2)    /* call out of order (e.g. no genotypes)   */
+#define SAIGE_E_NOMEM (-3)    /* device or pinned allocation failed      */
+#define SAIGE_E_IO (-4)       /* file could not be opened or read        */
+#define SAIGE_E_DEVICE (-5)   /* HIP runtime error                       */
+#define SAIGE_E_NOTCONV (-6)  /* PCG hit maxiter (result still written)  */
+
+/* GRM arithmetic for saige_build_grm */
+#define SAIGE_GRM_FP32 0      /* FP32 Z, FP32 MFMA (v_mfma_f32_32x32x2_f32) */
+#define SAIGE_GRM_BF16 1      /* BF16 Z, BF16 MFMA with FP32 accumulation  */
+
+typedef struct saige_ctx saige_ctx;
+
+/* ------------------------------------------------------------ lifecycle */
+
+/* Create a context on HIP device `device`.  Replaces the process-global
+ * genoClass (FG.cpp:1188) and gpuSymMatMult's constructor (GPU.hpp:26). */
+int saige_create(int device, saige_ctx** out);
+/* Free every device buffer (GPU.hpp:27, ~gpuSymMatMult; closeGenoFile_plink,
+ * FG.cpp:1191-1209). */
+int saige_destroy(saige_ctx* ctx);
+/* Message of the last failed call on this thread. */
+const char* saige_last_error(void);
+
+/* ------------------------------------------------------------- setters */
+
+/* setminMAFforGRM (FG.cpp:4923) */
+int saige_set_min_maf_for_grm(saige_ctx* ctx, float min_maf);
+/* setmaxMissingRateforGRM (FG.cpp:4928) */
+int saige_set_max_missing_rate_for_grm(saige_ctx* ctx, float max_missing);
+/* setminMAC_VarianceRatio (FG.cpp:4970) */
+int saige_set_min_mac_variance_ratio(saige_ctx* ctx, float min_mac, float max_mac,
+                                     int is_var_ratio_in_geno);
+/* The random common-marker pool for variance ratios, g_randMarkerIndforVR
+ * (FG.cpp:866-871, drawn there with arma::randi; the caller draws it here). */
+int saige_set_vr_random_markers(saige_ctx* ctx, const int32_t* idx_host, int64_t n);
+
+/* ------------------------------------------------------- genotype ingest */
+
+/* setgeno (FG.cpp:2267-2275 -> genoClass::setGenoObj, FG.cpp:739-1024).
+ * Counts samples in the .fam and markers in the .bim, reads the .bed in one
+ * pass, decodes, QCs, imputes and repacks on the device.
+ *   sub_sample_in_geno : n_sub 1-based .fam row indices of the phenotyped
+ *                        samples (subSampleInGeno)
+ *   indicator          : N flags (indicatorGenoSamplesWithPheno); NULL derives
+ *                        them from sub_sample_in_geno
+ *   is_diag_of_kin_set_as_one : isDiagofKinSetAsOne */
+int saige_setgeno(saige_ctx* ctx, const char* bed_file, const char* bim_file,
+                  const char* fam_file, const int32_t* sub_sample_in_geno, int64_t n_sub,
+                  const uint8_t* indicator, int is_diag_of_kin_set_as_one);
+/* Same from a .bed body (magic bytes stripped) already in host memory,
+ * M * ceil(N/4) bytes. */
+int saige_setgeno_host(saige_ctx* ctx, const uint8_t* bed_body_host, int64_t N, int64_t M,
+                       const int32_t* sub_sample_in_geno, int64_t n_sub,
+                       const uint8_t* indicator, int is_diag_of_kin_set_as_one);
+/* Same from a .bed body already in device memory of this context's device
+ * (not freed, not modified). */
+int saige_setgeno_device(saige_ctx* ctx, const void* bed_body_dev, int64_t N, int64_t M,
+                         const int32_t* sub_sample_in_geno, int64_t n_sub,
+                         const uint8_t* indicator, int is_diag_of_kin_set_as_one);
+
+/* ------------------------------------------------------------- getters */
+
+int64_t saige_get_nnomissing(const saige_ctx* ctx);   /* getNnomissingOut (FG.cpp:1262) */
+int64_t saige_get_total_markers(const saige_ctx* ctx);/* gettotalMarker (FG.cpp:1213)   */
+/* getMsub_MAFge_minMAFtoConstructGRM (FG.cpp:1271): QC-passing marker count */
+int64_t saige_get_mqc(const saige_ctx* ctx);
+int64_t saige_get_n_vr_markers(const saige_ctx* ctx);
+/* getAlleleFreqVec (FG.cpp:1219): Mqc floats */
+int saige_get_allele_freq(saige_ctx* ctx, float* out_host);
+/* getMACVec (FG.cpp:1224): Mqc ints */
+int saige_get_mac(saige_ctx* ctx, int32_t* out_host);
+/* invstdvVec: Mqc floats */
+int saige_get_inv_std(saige_ctx* ctx, float* out_host);
+/* getQCdMarkerIndex (FG.cpp:1249): M flags */
+int saige_get_qc_marker_index(saige_ctx* ctx, uint8_t* out_host);
+/* getMACVec_forVarRatio / getIndexVec_forVarRatio (FG.cpp:1230, 1235) */
+int saige_get_mac_vr(saige_ctx* ctx, int32_t* out_host);
+int saige_get_index_vr(saige_ctx* ctx, int32_t* out_host);
+/* Per-marker decode results for all M markers (the values
+ * Get_OneSNP_Geno_atBeginning computes, FG.cpp:326-579): allele count after
+ * imputation, missing count, MAC, imputation fill, freq, missing rate,
+ * 1/sd and the QC / VR flags (bit0 QC, bit1 VR).  Any pointer may be NULL. */
+int saige_get_marker_stats(saige_ctx* ctx, int32_t* allele_count, int32_t* num_missing,
+                           int32_t* mac, int32_t* fill, float* freq, float* missing_rate,
+                           float* inv_std, uint8_t* flags);
+/* Get_OneSNP_Geno (FG.cpp:2281): integer genotypes of QC'd marker idx */
+int saige_get_one_snp_geno(saige_ctx* ctx, int64_t idx, int32_t* out_host);
+/* Get_OneSNP_Geno_forVarRatio (FG.cpp:2291) */
+int saige_get_one_snp_geno_vr(saige_ctx* ctx, int64_t idx, int32_t* out_host);
+/* Get_OneSNP_StdGeno (FG.cpp:582-662) of QC'd marker idx */
+int saige_get_one_snp_std_geno(saige_ctx* ctx, int64_t idx, float* out_host);
+/* Get_Diagof_StdGeno (FG.cpp:665-704): sum_j z_ij^2, FP32, markers in order */
+int saige_get_diag_std(saige_ctx* ctx, float* out_host);
+/* get_DiagofKin (FG.cpp:4358-4375) / get_GRMdiagVec (FG.cpp:4913-4920) */
+int saige_get_diag_of_kin(saige_ctx* ctx, float* out_host);
+
+/* ------------------------------------------------------------------ GRM */
+
+/* Build the resident GRM S = Z Z^T (FP32 storage, full N x N) with an MFMA
+ * SYRK over marker panels.  Replaces gpuDistributeSNPs + set_matrix
+ * (FG.cpp:1893-1950, GPU.cu:30-138; GPU.hpp:29), which upload the FP32 Z
+ * instead.  K = S / Mqc. */
+int saige_build_grm(saige_ctx* ctx, int precision);
+/* Copy K[r0:r0+nr, c0:c0+nc] (= S / Mqc, FP32) to host, row-major. */
+int saige_get_grm_block(saige_ctx* ctx, int64_t r0, int64_t c0, int64_t nr, int64_t nc,
+                        float* out_host);
+/* Seconds of device time of the last saige_build_grm (HIP events). */
+double saige_last_grm_seconds(const saige_ctx* ctx);
+
+/* getCrossprodMatAndKin / parallelCrossProd (FG.cpp:1953-1981, 1669-1708):
+ * y = K b.  Replaces gpuSymMatMult::sym_sgemv (GPU.hpp:30, GPU.cu:211-252). */
+int saige_crossprod_mat_and_kin(saige_ctx* ctx, const float* b_host, float* out_host);
+/* k columns at once: Y = K B (column-major N x k). */
+int saige_grm_matvec(saige_ctx* ctx, const float* B_host, int64_t k, float* Y_host);
+
+/* getDiagOfSigma (FG.cpp:2322-2365); tau is {tau0, tau1}. */
+int saige_get_diag_of_sigma(saige_ctx* ctx, const float* w_host, const float* tau,
+                            float* out_host);
+/* getCrossprod (FG.cpp:2397-2425): Sigma b. */
+int saige_get_crossprod(saige_ctx* ctx, const float* b_host, const float* w_host,
+                        const float* tau, float* out_host);
+
+/* ------------------------------------------------------------------ PCG */
+
+/* getPCG1ofSigmaAndVector (FG.cpp:2593-2809): one right-hand side, Jacobi
+ * preconditioner, stop when ||r||^2 <= tol (absolute) or at maxiter.  FP32
+ * vectors and GRM product, FP64 dot products and convergence test.  Writes
+ * the iteration count to *iters (may be NULL).  Returns SAIGE_E_NOTCONV (x
+ * still written) if maxiter was reached, where the reference prints a
+ * warning (FG.cpp:2794-2797). */
+int saige_pcg1(saige_ctx* ctx, const float* w_host, const float* tau, const float* b_host,
+               int maxiter, float tol, float* x_host, int32_t* iters);
+/* getSigma_X (FG.cpp:3347-3366) and every other "same Sigma, k right-hand
+ * sides" solve: block PCG, one GRM GEMM per iteration for all columns; each
+ * column keeps its own alpha/beta and stopping test, so column c equals its
+ * own single solve up to the GEMM's rounding. */
+int saige_pcg_block(saige_ctx* ctx, const float* w_host, const float* tau, const float* B_host,
+                    int64_t k, int maxiter, float tol, float* X_host, int32_t* iters);
+
+/* ------------------------------------------- block-PCG slot engine
+ * The engine keeps up to `width` (<= 64) solves in flight over shared K.
+ * Each slot belongs to a "system" (w, tau, optional sample mask), so many
+ * phenotypes, probes and sample subsets advance together; a slot joins or
+ * leaves between iterations (continuous batching).  The reference has no
+ * counterpart: it solves one right-hand side at a time (FG.cpp:2593). */
+
+/* (Re)allocate for `width` slots and `n_systems` systems. */
+int saige_engine_reserve(saige_ctx* ctx, int width, int n_systems);
+/* Set system s: Sigma_s = tau0 diag(1/w) + tau1 K restricted to mask (N
+ * flags; NULL = all samples).  Samples outside the mask are held at zero. */
+int saige_engine_set_system(saige_ctx* ctx, int s, const float* w_host, const float* tau,
+                            const uint8_t* mask_host);
+/* Start solving Sigma_s x = b in slot `slot` (b: N floats on host). */
+int saige_engine_add(saige_ctx* ctx, int slot, int s, const float* b_host, int maxiter,
+                     float tol);
+/* Run `n_iter` iterations on every live slot.  done[slot] = 1 when the slot
+ * has stopped; iters[slot] = its iteration count (either may be NULL). */
+int saige_engine_step(saige_ctx* ctx, int n_iter, uint8_t* done_host, int32_t* iters_host);
+/* Copy the solution of a slot to host and free the slot. */
+int saige_engine_take(saige_ctx* ctx, int slot, float* x_host);
+/* Seconds of device time spent in engine GEMM/GEMV launches, and their count. */
+int saige_engine_stats(const saige_ctx* ctx, double* gemm_seconds, int64_t* gemm_launches);
+
+/* ------------------------------------------------------- synthetic data */
+
+/* Fill a device .bed body (M x ceil(N/4) bytes, PLINK-1 codes) with
+ * genotypes g ~ Binomial(2, p_j), p_j ~ U(maf_lo, maf_hi), each call missing
+ * with probability miss_rate, from a counter-based generator keyed by seed.
+ * If pair_frac > 0, samples [0, 2*npairs) form sibling pairs (2i, 2i+1) with
+ * npairs = floor(pair_frac * N / 2): the second copies the first's
+ * genotype at each marker with probability 1/2.  Test/benchmark input only. */
+int saige_synth_bed_device(saige_ctx* ctx, void* bed_body_dev, int64_t N, int64_t M,
+                           float maf_lo, float maf_hi, float miss_rate, float pair_frac,
+                           uint64_t seed);
+/* Device allocation helper for the above (freed by saige_free_device). */
+int saige_alloc_device(saige_ctx* ctx, int64_t bytes, void** out);
+int saige_free_device(saige_ctx* ctx, void* p);
+/* Copy rows of a device .bed body to host (for building phenotypes). */
+int saige_copy_device_to_host(saige_ctx* ctx, const void* src_dev, int64_t bytes, void* dst_host);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SAIGE_HIP_H */

@@ -1,0 +1,1083 @@
+// C ABI entry points of libsaigehip (declared in include/saige_hip.h).
+//
+// Host-side orchestration only: file ingest, compaction of the QC'd marker
+// list, panel scheduling of the SYRK and the block-PCG slot engine.  Every
+// O(N) or larger computation runs in the kernels of geno_kernels.hip,
+// grm_kernels.hip and pcg_kernels.hip.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <numeric>
+
+#include "saige_internal.hpp"
+
+using namespace saige;
+
+namespace saige {
+
+static thread_local std::string g_err;
+
+void set_error(const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+}  // namespace saige
+
+namespace {
+
+#define CHECK_CTX(ctx)                                       \
+    do {                                                     \
+        if (!(ctx)) {                                        \
+            set_error("null context");                       \
+            return SAIGE_E_ARG;                              \
+        }                                                    \
+        hipError_t _se = hipSetDevice((ctx)->device);        \
+        if (_se != hipSuccess) {                             \
+            set_error("hipSetDevice: %s", hipGetErrorString(_se)); \
+            return SAIGE_E_DEVICE;                           \
+        }                                                    \
+    } while (0)
+
+#define CHECK_LOADED(ctx)                                                      \
+    do {                                                                       \
+        if (!(ctx)->loaded) {                                                  \
+            set_error("no genotypes loaded: call saige_setgeno first");       \
+            return SAIGE_E_STATE;                                              \
+        }                                                                      \
+    } while (0)
+
+#define CHECK_GRM(ctx)                                                         \
+    do {                                                                       \
+        CHECK_LOADED(ctx);                                                     \
+        if (!(ctx)->grm_ready) {                                               \
+            set_error("GRM not built: call saige_build_grm first");           \
+            return SAIGE_E_STATE;                                              \
+        }                                                                      \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, int64_t count) {
+    *p = nullptr;
+    if (count <= 0) count = 1;
+    return hipMalloc((void**)p, sizeof(T) * (size_t)count);
+}
+
+template <typename T>
+void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+void free_engine(Engine& e) {
+    dfree(e.X); dfree(e.R); dfree(e.Z); dfree(e.P); dfree(e.AP);
+    dfree(e.winv); dfree(e.minv); dfree(e.mask); dfree(e.tau);
+    dfree(e.slot_sys); dfree(e.rz); dfree(e.sumr2); dfree(e.pap); dfree(e.alpha); dfree(e.beta);
+    dfree(e.iter); dfree(e.maxiter); dfree(e.tol); dfree(e.live); dfree(e.done);
+    dfree(e.partial); dfree(e.stage);
+    if (e.h_done) (void)hipHostFree(e.h_done);
+    if (e.h_iter) (void)hipHostFree(e.h_iter);
+    e.h_done = nullptr;
+    e.h_iter = nullptr;
+    e.h_slot_sys.clear();
+    e.h_live.clear();
+    e.width = e.nsys = e.kpad = 0;
+}
+
+void free_geno(saige_ctx* c) {
+    dfree(c->d_stats); dfree(c->d_packed); dfree(c->d_packed_vr); dfree(c->d_lut); dfree(c->d_lut_vr);
+    dfree(c->d_diag); dfree(c->d_grm); dfree(c->d_vec);
+    free_engine(c->eng);
+    free_engine(c->eng1);
+    c->h_stats.clear();
+    c->qc_index.clear();
+    c->vr_index.clear();
+    c->loaded = false;
+    c->grm_ready = false;
+    c->N = c->M = c->Nnm = c->Npad = c->Mqc = c->Mvr = c->bpm_nm = 0;
+}
+
+int64_t count_lines(const char* path, int* err) {
+    std::ifstream f(path);
+    if (!f.is_open()) {
+        *err = 1;
+        return 0;
+    }
+    int64_t n = 0;
+    std::string line;
+    while (std::getline(f, line)) n++;   // FG.cpp:766-771, 781-786
+    *err = 0;
+    return n;
+}
+
+// Allocate and fill the slot engine: kpad columns, nsys systems.
+int engine_reserve(saige_ctx* c, Engine& e, int kpad, int nsys) {
+    if (e.kpad == kpad && e.nsys >= nsys && e.X) return SAIGE_OK;
+    free_engine(e);
+    const int64_t Np = c->Npad;
+    SAIGE_HIP_TRY(dalloc(&e.X, Np * kpad));
+    SAIGE_HIP_TRY(dalloc(&e.R, Np * kpad));
+    SAIGE_HIP_TRY(dalloc(&e.Z, Np * kpad));
+    SAIGE_HIP_TRY(dalloc(&e.P, Np * kpad));
+    SAIGE_HIP_TRY(dalloc(&e.AP, Np * kpad));
+    SAIGE_HIP_TRY(dalloc(&e.winv, Np * nsys));
+    SAIGE_HIP_TRY(dalloc(&e.minv, Np * nsys));
+    SAIGE_HIP_TRY(dalloc(&e.mask, Np * nsys));
+    SAIGE_HIP_TRY(dalloc(&e.tau, 2 * nsys));
+    SAIGE_HIP_TRY(dalloc(&e.slot_sys, kpad));
+    SAIGE_HIP_TRY(dalloc(&e.rz, kpad));
+    SAIGE_HIP_TRY(dalloc(&e.sumr2, kpad));
+    SAIGE_HIP_TRY(dalloc(&e.pap, kpad));
+    SAIGE_HIP_TRY(dalloc(&e.alpha, kpad));
+    SAIGE_HIP_TRY(dalloc(&e.beta, kpad));
+    SAIGE_HIP_TRY(dalloc(&e.iter, kpad));
+    SAIGE_HIP_TRY(dalloc(&e.maxiter, kpad));
+    SAIGE_HIP_TRY(dalloc(&e.tol, kpad));
+    SAIGE_HIP_TRY(dalloc(&e.live, kpad));
+    SAIGE_HIP_TRY(dalloc(&e.done, kpad));
+    SAIGE_HIP_TRY(dalloc(&e.stage, Np));
+    int64_t np = std::max<int64_t>(gemv_max_parts(Np), gemm_max_parts(Np) * kpad);
+    np = std::max<int64_t>(np, vec_max_parts(Np, kpad) * kpad);
+    e.nparts = np;
+    SAIGE_HIP_TRY(dalloc(&e.partial, np * 2));
+    SAIGE_HIP_TRY(hipHostMalloc((void**)&e.h_done, kpad, hipHostMallocDefault));
+    SAIGE_HIP_TRY(hipHostMalloc((void**)&e.h_iter, sizeof(int32_t) * kpad, hipHostMallocDefault));
+    e.kpad = kpad;
+    e.width = kpad;
+    e.nsys = nsys;
+    e.h_slot_sys.assign(kpad, 0);
+    e.h_live.assign(kpad, 0);
+    // zero all column blocks so padded rows and idle columns read as zero
+    hipStream_t s = c->stream;
+    SAIGE_HIP_TRY(launch_fill_zero(s, e.X, Np * kpad * 4));
+    SAIGE_HIP_TRY(launch_fill_zero(s, e.R, Np * kpad * 4));
+    SAIGE_HIP_TRY(launch_fill_zero(s, e.Z, Np * kpad * 4));
+    SAIGE_HIP_TRY(launch_fill_zero(s, e.P, Np * kpad * 4));
+    SAIGE_HIP_TRY(launch_fill_zero(s, e.AP, Np * kpad * 4));
+    SAIGE_HIP_TRY(launch_fill_zero(s, e.winv, Np * nsys * 4));
+    SAIGE_HIP_TRY(launch_fill_zero(s, e.minv, Np * nsys * 4));
+    SAIGE_HIP_TRY(launch_fill_zero(s, e.mask, round_up(Np * nsys, 4)));
+    SAIGE_HIP_TRY(launch_fill_zero(s, e.stage, Np * 4));
+    std::vector<uint8_t> zeros(kpad, 0);
+    std::vector<int32_t> zi(kpad, 0);
+    SAIGE_HIP_TRY(hipMemcpyAsync(e.live, zeros.data(), kpad, hipMemcpyHostToDevice, s));
+    SAIGE_HIP_TRY(hipMemcpyAsync(e.done, zeros.data(), kpad, hipMemcpyHostToDevice, s));
+    SAIGE_HIP_TRY(hipMemcpyAsync(e.slot_sys, zi.data(), 4 * kpad, hipMemcpyHostToDevice, s));
+    SAIGE_HIP_TRY(hipStreamSynchronize(s));
+    return SAIGE_OK;
+}
+
+int engine_set_system(saige_ctx* c, Engine& e, int s, const float* w_host, const float* tau,
+                      const uint8_t* mask_host) {
+    if (s < 0 || s >= e.nsys) {
+        set_error("system %d out of range [0, %d)", s, e.nsys);
+        return SAIGE_E_ARG;
+    }
+    if (!w_host || !tau) {
+        set_error("w and tau are required");
+        return SAIGE_E_ARG;
+    }
+    hipStream_t st = c->stream;
+    const int64_t Np = c->Npad, N = c->Nnm;
+    float* d_w = c->d_vec;                         // [Npad]
+    uint8_t* d_m = reinterpret_cast<uint8_t*>(c->d_vec + Np);
+    SAIGE_HIP_TRY(hipMemcpyAsync(d_w, w_host, sizeof(float) * N, hipMemcpyHostToDevice, st));
+    if (mask_host) SAIGE_HIP_TRY(hipMemcpyAsync(d_m, mask_host, N, hipMemcpyHostToDevice, st));
+    SAIGE_HIP_TRY(hipMemcpyAsync(e.tau + 2 * s, tau, sizeof(float) * 2, hipMemcpyHostToDevice, st));
+    SAIGE_HIP_TRY(launch_sigma_setup(st, d_w, mask_host ? d_m : nullptr, c->d_diag, N, Np, tau[0], tau[1],
+                                     (float)c->Mqc, c->diag_one, e.winv + s * Np, e.minv + s * Np,
+                                     e.mask + s * Np, nullptr));
+    SAIGE_HIP_TRY(hipStreamSynchronize(st));
+    return SAIGE_OK;
+}
+
+int engine_sync_host_arrays(saige_ctx* c, Engine& e, const std::vector<int32_t>* maxiter,
+                            const std::vector<double>* tol) {
+    hipStream_t s = c->stream;
+    SAIGE_HIP_TRY(hipMemcpyAsync(e.live, e.h_live.data(), e.kpad, hipMemcpyHostToDevice, s));
+    SAIGE_HIP_TRY(hipMemcpyAsync(e.slot_sys, e.h_slot_sys.data(), 4 * e.kpad, hipMemcpyHostToDevice, s));
+    if (maxiter) SAIGE_HIP_TRY(hipMemcpyAsync(e.maxiter, maxiter->data(), 4 * e.kpad, hipMemcpyHostToDevice, s));
+    if (tol) SAIGE_HIP_TRY(hipMemcpyAsync(e.tol, tol->data(), 8 * e.kpad, hipMemcpyHostToDevice, s));
+    SAIGE_HIP_TRY(hipStreamSynchronize(s));
+    return SAIGE_OK;
+}
+
+int engine_add(saige_ctx* c, Engine& e, int slot, int s, const float* b_host, int maxiter, float tol) {
+    if (slot < 0 || slot >= e.kpad) {
+        set_error("slot %d out of range [0, %d)", slot, e.kpad);
+        return SAIGE_E_ARG;
+    }
+    if (s < 0 || s >= e.nsys) {
+        set_error("system %d out of range [0, %d)", s, e.nsys);
+        return SAIGE_E_ARG;
+    }
+    if (e.h_live[slot]) {
+        set_error("slot %d is busy", slot);
+        return SAIGE_E_STATE;
+    }
+    hipStream_t st = c->stream;
+    const int64_t Np = c->Npad, N = c->Nnm;
+    // per-slot parameters: read back the current device arrays so other
+    // slots keep theirs, then patch this slot
+    std::vector<int32_t> mi(e.kpad);
+    std::vector<double> tl(e.kpad);
+    SAIGE_HIP_TRY(hipMemcpyAsync(mi.data(), e.maxiter, 4 * e.kpad, hipMemcpyDeviceToHost, st));
+    SAIGE_HIP_TRY(hipMemcpyAsync(tl.data(), e.tol, 8 * e.kpad, hipMemcpyDeviceToHost, st));
+    SAIGE_HIP_TRY(hipStreamSynchronize(st));
+    mi[slot] = maxiter;
+    tl[slot] = (double)tol;
+    e.h_slot_sys[slot] = s;
+    e.h_live[slot] = 1;
+    int rc = engine_sync_host_arrays(c, e, &mi, &tl);
+    if (rc) return rc;
+    SAIGE_HIP_TRY(hipMemcpyAsync(e.stage, b_host, sizeof(float) * N, hipMemcpyHostToDevice, st));
+    int64_t np = 0;
+    SAIGE_HIP_TRY(launch_col_init(st, e.stage, slot, e.kpad, N, Np, e.minv + s * Np, e.mask + s * Np, e.X,
+                                  e.R, e.Z, e.P, e.partial, &np));
+    SAIGE_HIP_TRY(launch_col_init_finish(st, slot, e.kpad, e.partial, np, e.rz, e.sumr2, e.maxiter, e.tol,
+                                         e.iter, e.done, e.live));
+    SAIGE_HIP_TRY(hipStreamSynchronize(st));
+    return SAIGE_OK;
+}
+
+int engine_step(saige_ctx* c, Engine& e, int n_iter, uint8_t* done_out, int32_t* iters_out) {
+    hipStream_t st = c->stream;
+    const int64_t Np = c->Npad, N = c->Nnm;
+    SigmaArgs a;
+    a.winv = e.winv;
+    a.mask = e.mask;
+    a.tau = e.tau;
+    a.sys = e.slot_sys;
+    a.live = e.live;
+    a.inv_m = 0.0f;
+    a.mqc = (float)c->Mqc;
+    a.mode = 1;
+    SAIGE_HIP_TRY(hipMemcpyAsync(e.h_done, e.done, e.kpad, hipMemcpyDeviceToHost, st));
+    SAIGE_HIP_TRY(hipStreamSynchronize(st));
+    for (int it = 0; it < n_iter; it++) {
+        bool any = false;
+        for (int k = 0; k < e.kpad; k++) any |= (e.h_live[k] && !e.h_done[k]);
+        if (!any) break;
+        int64_t np = 0;
+        SAIGE_HIP_TRY(hipEventRecord(c->ev0, st));
+        if (e.kpad == 1)
+            SAIGE_HIP_TRY(launch_grm_gemv(st, c->d_grm, Np, N, e.P, e.AP, a, e.partial, &np));
+        else
+            SAIGE_HIP_TRY(launch_grm_gemm(st, c->d_grm, Np, N, e.P, e.AP, a, e.partial, &np));
+        SAIGE_HIP_TRY(hipEventRecord(c->ev1, st));
+        SAIGE_HIP_TRY(launch_alpha(st, e.kpad, e.partial, np, e.rz, e.done, e.live, e.alpha));
+        int64_t np2 = 0;
+        SAIGE_HIP_TRY(launch_update(st, e.kpad, N, Np, e.alpha, e.done, e.live, e.minv, e.slot_sys, e.X, e.R,
+                                    e.Z, e.P, e.AP, e.partial, &np2));
+        SAIGE_HIP_TRY(launch_beta(st, e.kpad, e.partial, np2, e.rz, e.sumr2, e.maxiter, e.tol, e.iter, e.done,
+                                  e.live, e.beta));
+        SAIGE_HIP_TRY(launch_pupdate(st, e.kpad, N, Np, e.beta, e.done, e.live, e.Z, e.P));
+        SAIGE_HIP_TRY(hipMemcpyAsync(e.h_done, e.done, e.kpad, hipMemcpyDeviceToHost, st));
+        SAIGE_HIP_TRY(hipStreamSynchronize(st));
+        float ms = 0.f;
+        SAIGE_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        e.gemm_seconds += ms * 1e-3;
+        e.gemm_launches++;
+    }
+    SAIGE_HIP_TRY(hipMemcpyAsync(e.h_iter, e.iter, 4 * e.kpad, hipMemcpyDeviceToHost, st));
+    SAIGE_HIP_TRY(hipMemcpyAsync(e.h_done, e.done, e.kpad, hipMemcpyDeviceToHost, st));
+    SAIGE_HIP_TRY(hipStreamSynchronize(st));
+    for (int k = 0; k < e.kpad; k++) {
+        if (done_out) done_out[k] = e.h_live[k] ? e.h_done[k] : 1;
+        if (iters_out) iters_out[k] = e.h_live[k] ? e.h_iter[k] : 0;
+    }
+    return SAIGE_OK;
+}
+
+int engine_take(saige_ctx* c, Engine& e, int slot, float* x_host) {
+    if (slot < 0 || slot >= e.kpad) {
+        set_error("slot %d out of range", slot);
+        return SAIGE_E_ARG;
+    }
+    hipStream_t st = c->stream;
+    if (x_host) {
+        SAIGE_HIP_TRY(launch_gather_col(st, e.X, slot, e.kpad, c->Nnm, e.stage));
+        SAIGE_HIP_TRY(hipMemcpyAsync(x_host, e.stage, sizeof(float) * c->Nnm, hipMemcpyDeviceToHost, st));
+    }
+    e.h_live[slot] = 0;
+    return engine_sync_host_arrays(c, e, nullptr, nullptr);
+}
+
+// Common tail of the three setgeno variants: decode, compact, repack.
+int setgeno_device_impl(saige_ctx* c, const uint8_t* d_bed, int64_t N, int64_t M,
+                        const int32_t* subidx, int64_t n_sub, const uint8_t* indicator, int diag_one) {
+    if (N <= 0 || M <= 0 || n_sub <= 0 || n_sub > N || !subidx) {
+        set_error("invalid sizes: N=%lld M=%lld n_sub=%lld", (long long)N, (long long)M, (long long)n_sub);
+        return SAIGE_E_ARG;
+    }
+    if (N > (int64_t)1 << 30 || n_sub * 2 > (int64_t)INT32_MAX) {
+        set_error("N=%lld exceeds the supported sample count", (long long)N);
+        return SAIGE_E_ARG;
+    }
+    bool identity = (n_sub == N);
+    for (int64_t i = 0; i < n_sub; i++) {
+        if (subidx[i] < 1 || subidx[i] > N) {
+            set_error("subSampleInGeno[%lld] = %d outside 1..%lld", (long long)i, subidx[i], (long long)N);
+            return SAIGE_E_ARG;
+        }
+        if (subidx[i] != i + 1) identity = false;
+    }
+    free_geno(c);
+    hipStream_t st = c->stream;
+    c->N = N;
+    c->M = M;
+    c->Nnm = n_sub;
+    c->Npad = round_up(n_sub, NPAD_ALIGN);
+    c->bpm_nm = (n_sub + 3) / 4;
+    c->diag_one = diag_one;
+    const int64_t bpm = (N + 3) / 4;
+
+    std::vector<uint8_t> ind(N, 0);
+    if (indicator) std::memcpy(ind.data(), indicator, N);
+    else for (int64_t i = 0; i < n_sub; i++) ind[subidx[i] - 1] = 1;
+    std::vector<uint8_t> rset(M, 0);
+    for (int32_t v : c->vr_rand)
+        if (v >= 0 && v < M) rset[v] = 1;
+
+    uint8_t *d_ind = nullptr, *d_rset = nullptr;
+    int32_t *d_sub = nullptr, *d_list = nullptr;
+    auto cleanup = [&]() { dfree(d_ind); dfree(d_rset); dfree(d_sub); dfree(d_list); };
+    int rc = SAIGE_OK;
+    do {
+        if (dalloc(&d_ind, N) || dalloc(&d_rset, M) || dalloc(&d_sub, n_sub) || dalloc(&c->d_stats, M)) {
+            set_error("device allocation failed (setgeno)");
+            rc = SAIGE_E_NOMEM;
+            break;
+        }
+        if (hipMemcpyAsync(d_ind, ind.data(), N, hipMemcpyHostToDevice, st) ||
+            hipMemcpyAsync(d_rset, rset.data(), M, hipMemcpyHostToDevice, st) ||
+            hipMemcpyAsync(d_sub, subidx, 4 * n_sub, hipMemcpyHostToDevice, st)) {
+            set_error("host to device copy failed (setgeno)");
+            rc = SAIGE_E_DEVICE;
+            break;
+        }
+        DecodeParams p;
+        p.N = N;
+        p.M = M;
+        p.Nnm = n_sub;
+        p.bpm = bpm;
+        p.min_maf = c->min_maf;
+        p.max_missing = c->max_missing;
+        p.vr_mode = c->is_var_ratio ? (c->max_mac_vr != -1.0f ? 1 : 2) : 0;
+        p.min_mac_vr = c->min_mac_vr;
+        p.max_mac_vr = c->max_mac_vr;
+        hipError_t e = launch_decode_stats(st, d_bed, d_ind, d_rset, p, c->d_stats);
+        c->h_stats.resize(M);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(c->h_stats.data(), c->d_stats, sizeof(MarkerStat) * M, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) {
+            set_error("decode failed: %s", hipGetErrorString(e));
+            rc = SAIGE_E_DEVICE;
+            break;
+        }
+        // compaction in marker order (SNPIdx_new / SNPIdx_vr, FG.cpp:923-943)
+        for (int64_t m = 0; m < M; m++) {
+            if (c->h_stats[m].pass_qc) c->qc_index.push_back((int32_t)m);
+            if (c->h_stats[m].pass_vr) c->vr_index.push_back((int32_t)m);
+        }
+        c->Mqc = (int64_t)c->qc_index.size();
+        c->Mvr = (int64_t)c->vr_index.size();
+        const int64_t nl = std::max<int64_t>(c->Mqc, c->Mvr);
+        if (dalloc(&d_list, nl) || dalloc(&c->d_packed, c->Mqc * c->bpm_nm) ||
+            dalloc(&c->d_packed_vr, c->Mvr * c->bpm_nm) || dalloc(&c->d_lut, c->Mqc) ||
+            dalloc(&c->d_lut_vr, c->Mvr) || dalloc(&c->d_diag, c->Npad) || dalloc(&c->d_vec, 4 * c->Npad)) {
+            set_error("device allocation failed (packed genotypes)");
+            rc = SAIGE_E_NOMEM;
+            break;
+        }
+        e = launch_fill_zero(st, c->d_diag, 4 * c->Npad);
+        if (e == hipSuccess && c->Mqc > 0)
+            e = hipMemcpyAsync(d_list, c->qc_index.data(), 4 * c->Mqc, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess)
+            e = launch_repack(st, d_bed, bpm, d_list, c->Mqc, c->d_stats, identity ? nullptr : d_sub, n_sub,
+                              c->bpm_nm, c->d_packed);
+        if (e == hipSuccess) e = launch_lut(st, d_list, c->Mqc, c->d_stats, c->d_lut);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e == hipSuccess && c->Mvr > 0) {
+            e = hipMemcpyAsync(d_list, c->vr_index.data(), 4 * c->Mvr, hipMemcpyHostToDevice, st);
+            if (e == hipSuccess)
+                e = launch_repack(st, d_bed, bpm, d_list, c->Mvr, c->d_stats, identity ? nullptr : d_sub,
+                                  n_sub, c->bpm_nm, c->d_packed_vr);
+            if (e == hipSuccess) e = launch_lut(st, d_list, c->Mvr, c->d_stats, c->d_lut_vr);
+        }
+        // diag(Z Z^T) in FP32 marker order (Get_Diagof_StdGeno, FG.cpp:665-704)
+        if (e == hipSuccess && c->Mqc > 0)
+            e = launch_diag_std(st, c->d_packed, c->d_lut, c->Mqc, n_sub, c->bpm_nm, c->d_diag);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) {
+            set_error("repack failed: %s", hipGetErrorString(e));
+            rc = SAIGE_E_DEVICE;
+            break;
+        }
+    } while (0);
+    cleanup();
+    if (rc != SAIGE_OK) {
+        free_geno(c);
+        return rc;
+    }
+    c->loaded = true;
+    return SAIGE_OK;
+}
+
+int read_bed_body(const char* bed_file, int64_t N, int64_t M, std::vector<uint8_t>& body) {
+    std::ifstream f(bed_file, std::ios::binary);
+    if (!f.is_open()) {
+        set_error("cannot open bed file %s", bed_file);
+        return SAIGE_E_IO;
+    }
+    unsigned char magic[3] = {0, 0, 0};
+    f.read(reinterpret_cast<char*>(magic), 3);
+    if (!f || magic[0] != 0x6c || magic[1] != 0x1b || magic[2] != 0x01) {
+        set_error("%s is not a SNP-major PLINK-1 .bed (magic 6c 1b 01)", bed_file);
+        return SAIGE_E_IO;
+    }
+    const int64_t bytes = M * ((N + 3) / 4);
+    body.resize((size_t)bytes);
+    // one bulk read instead of a seekg + read per marker (FG.cpp:902-903)
+    f.read(reinterpret_cast<char*>(body.data()), bytes);
+    if (f.gcount() != bytes) {
+        set_error("%s: expected %lld genotype bytes, read %lld", bed_file, (long long)bytes,
+                  (long long)f.gcount());
+        return SAIGE_E_IO;
+    }
+    return SAIGE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* saige_last_error(void) { return g_err.c_str(); }
+
+int saige_create(int device, saige_ctx** out) {
+    if (!out) {
+        set_error("out is null");
+        return SAIGE_E_ARG;
+    }
+    *out = nullptr;
+    int ndev = 0;
+    SAIGE_HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) {
+        set_error("device %d out of range (%d devices)", device, ndev);
+        return SAIGE_E_ARG;
+    }
+    SAIGE_HIP_TRY(hipSetDevice(device));
+    saige_ctx* c = new saige_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e != hipSuccess) {
+        set_error("stream/event creation failed: %s", hipGetErrorString(e));
+        delete c;
+        return SAIGE_E_DEVICE;
+    }
+    *out = c;
+    return SAIGE_OK;
+}
+
+int saige_destroy(saige_ctx* c) {
+    if (!c) return SAIGE_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    free_geno(c);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SAIGE_OK;
+}
+
+int saige_set_min_maf_for_grm(saige_ctx* c, float v) {
+    CHECK_CTX(c);
+    c->min_maf = v;
+    return SAIGE_OK;
+}
+
+int saige_set_max_missing_rate_for_grm(saige_ctx* c, float v) {
+    CHECK_CTX(c);
+    c->max_missing = v;
+    return SAIGE_OK;
+}
+
+int saige_set_min_mac_variance_ratio(saige_ctx* c, float mn, float mx, int is_vr) {
+    CHECK_CTX(c);
+    c->min_mac_vr = mn;
+    c->max_mac_vr = mx;
+    c->is_var_ratio = is_vr ? 1 : 0;
+    return SAIGE_OK;
+}
+
+int saige_set_vr_random_markers(saige_ctx* c, const int32_t* idx, int64_t n) {
+    CHECK_CTX(c);
+    if (n < 0 || (n > 0 && !idx)) {
+        set_error("invalid random marker list");
+        return SAIGE_E_ARG;
+    }
+    c->vr_rand.assign(idx, idx + n);
+    return SAIGE_OK;
+}
+
+int saige_setgeno(saige_ctx* c, const char* bed, const char* bim, const char* fam, const int32_t* subidx,
+                  int64_t n_sub, const uint8_t* indicator, int diag_one) {
+    CHECK_CTX(c);
+    if (!bed || !bim || !fam) {
+        set_error("bed, bim and fam paths are required");
+        return SAIGE_E_ARG;
+    }
+    int err = 0;
+    const int64_t N = count_lines(fam, &err);
+    if (err) {
+        set_error("Error! fam file not open! (%s)", fam);
+        return SAIGE_E_IO;
+    }
+    const int64_t M = count_lines(bim, &err);
+    if (err) {
+        set_error("Error! bim file not open! (%s)", bim);
+        return SAIGE_E_IO;
+    }
+    std::vector<uint8_t> body;
+    int rc = read_bed_body(bed, N, M, body);
+    if (rc) return rc;
+    return saige_setgeno_host(c, body.data(), N, M, subidx, n_sub, indicator, diag_one);
+}
+
+int saige_setgeno_host(saige_ctx* c, const uint8_t* body, int64_t N, int64_t M, const int32_t* subidx,
+                       int64_t n_sub, const uint8_t* indicator, int diag_one) {
+    CHECK_CTX(c);
+    if (!body || N <= 0 || M <= 0) {
+        set_error("invalid bed body");
+        return SAIGE_E_ARG;
+    }
+    uint8_t* d_bed = nullptr;
+    const int64_t bytes = M * ((N + 3) / 4);
+    SAIGE_HIP_TRY(dalloc(&d_bed, bytes));
+    hipError_t e = hipMemcpy(d_bed, body, (size_t)bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        dfree(d_bed);
+        set_error("bed upload failed: %s", hipGetErrorString(e));
+        return SAIGE_E_DEVICE;
+    }
+    int rc = setgeno_device_impl(c, d_bed, N, M, subidx, n_sub, indicator, diag_one);
+    dfree(d_bed);
+    return rc;
+}
+
+int saige_setgeno_device(saige_ctx* c, const void* d_bed, int64_t N, int64_t M, const int32_t* subidx,
+                         int64_t n_sub, const uint8_t* indicator, int diag_one) {
+    CHECK_CTX(c);
+    if (!d_bed) {
+        set_error("bed_body_dev is null");
+        return SAIGE_E_ARG;
+    }
+    return setgeno_device_impl(c, static_cast<const uint8_t*>(d_bed), N, M, subidx, n_sub, indicator,
+                               diag_one);
+}
+
+int64_t saige_get_nnomissing(const saige_ctx* c) { return c ? c->Nnm : 0; }
+int64_t saige_get_total_markers(const saige_ctx* c) { return c ? c->M : 0; }
+int64_t saige_get_mqc(const saige_ctx* c) { return c ? c->Mqc : 0; }
+int64_t saige_get_n_vr_markers(const saige_ctx* c) { return c ? c->Mvr : 0; }
+
+int saige_get_marker_stats(saige_ctx* c, int32_t* ac, int32_t* miss, int32_t* mac, int32_t* fill, float* freq,
+                           float* mr, float* inv_std, uint8_t* flags) {
+    CHECK_CTX(c);
+    CHECK_LOADED(c);
+    for (int64_t m = 0; m < c->M; m++) {
+        const MarkerStat& s = c->h_stats[m];
+        if (ac) ac[m] = s.allele_count;
+        if (miss) miss[m] = s.num_missing;
+        if (mac) mac[m] = s.mac;
+        if (fill) fill[m] = s.fill;
+        if (freq) freq[m] = s.freq;
+        if (mr) mr[m] = s.missing_rate;
+        if (inv_std) inv_std[m] = s.inv_std;
+        if (flags) flags[m] = (uint8_t)(s.pass_qc | (s.pass_vr << 1));
+    }
+    return SAIGE_OK;
+}
+
+int saige_get_allele_freq(saige_ctx* c, float* out) {
+    CHECK_CTX(c);
+    CHECK_LOADED(c);
+    for (int64_t j = 0; j < c->Mqc; j++) out[j] = c->h_stats[c->qc_index[j]].freq;
+    return SAIGE_OK;
+}
+
+int saige_get_mac(saige_ctx* c, int32_t* out) {
+    CHECK_CTX(c);
+    CHECK_LOADED(c);
+    for (int64_t j = 0; j < c->Mqc; j++) out[j] = c->h_stats[c->qc_index[j]].mac;
+    return SAIGE_OK;
+}
+
+int saige_get_inv_std(saige_ctx* c, float* out) {
+    CHECK_CTX(c);
+    CHECK_LOADED(c);
+    for (int64_t j = 0; j < c->Mqc; j++) out[j] = c->h_stats[c->qc_index[j]].inv_std;
+    return SAIGE_OK;
+}
+
+int saige_get_qc_marker_index(saige_ctx* c, uint8_t* out) {
+    CHECK_CTX(c);
+    CHECK_LOADED(c);
+    for (int64_t m = 0; m < c->M; m++) out[m] = c->h_stats[m].pass_qc;
+    return SAIGE_OK;
+}
+
+int saige_get_mac_vr(saige_ctx* c, int32_t* out) {
+    CHECK_CTX(c);
+    CHECK_LOADED(c);
+    for (int64_t j = 0; j < c->Mvr; j++) out[j] = c->h_stats[c->vr_index[j]].mac;
+    return SAIGE_OK;
+}
+
+int saige_get_index_vr(saige_ctx* c, int32_t* out) {
+    CHECK_CTX(c);
+    CHECK_LOADED(c);
+    for (int64_t j = 0; j < c->Mvr; j++) out[j] = c->vr_index[j];
+    return SAIGE_OK;
+}
+
+static int get_geno_impl(saige_ctx* c, const uint8_t* packed, int64_t n, int64_t idx, int32_t* out) {
+    if (idx < 0 || idx >= n) {
+        set_error("marker index %lld out of range [0, %lld)", (long long)idx, (long long)n);
+        return SAIGE_E_ARG;
+    }
+    int32_t* d = reinterpret_cast<int32_t*>(c->d_vec);
+    SAIGE_HIP_TRY(launch_get_geno(c->stream, packed + idx * c->bpm_nm, c->Nnm, d));
+    SAIGE_HIP_TRY(hipMemcpyAsync(out, d, 4 * c->Nnm, hipMemcpyDeviceToHost, c->stream));
+    SAIGE_HIP_TRY(hipStreamSynchronize(c->stream));
+    return SAIGE_OK;
+}
+
+int saige_get_one_snp_geno(saige_ctx* c, int64_t idx, int32_t* out) {
+    CHECK_CTX(c);
+    CHECK_LOADED(c);
+    return get_geno_impl(c, c->d_packed, c->Mqc, idx, out);
+}
+
+int saige_get_one_snp_geno_vr(saige_ctx* c, int64_t idx, int32_t* out) {
+    CHECK_CTX(c);
+    CHECK_LOADED(c);
+    return get_geno_impl(c, c->d_packed_vr, c->Mvr, idx, out);
+}
+
+int saige_get_one_snp_std_geno(saige_ctx* c, int64_t idx, float* out) {
+    CHECK_CTX(c);
+    CHECK_LOADED(c);
+    if (idx < 0 || idx >= c->Mqc) {
+        set_error("marker index %lld out of range", (long long)idx);
+        return SAIGE_E_ARG;
+    }
+    SAIGE_HIP_TRY(launch_std_geno(c->stream, c->d_packed + idx * c->bpm_nm, c->d_lut + idx, c->Nnm, c->d_vec));
+    SAIGE_HIP_TRY(hipMemcpyAsync(out, c->d_vec, 4 * c->Nnm, hipMemcpyDeviceToHost, c->stream));
+    SAIGE_HIP_TRY(hipStreamSynchronize(c->stream));
+    return SAIGE_OK;
+}
+
+int saige_get_diag_std(saige_ctx* c, float* out) {
+    CHECK_CTX(c);
+    CHECK_LOADED(c);
+    SAIGE_HIP_TRY(hipMemcpy(out, c->d_diag, 4 * c->Nnm, hipMemcpyDeviceToHost));
+    return SAIGE_OK;
+}
+
+int saige_get_diag_of_kin(saige_ctx* c, float* out) {
+    CHECK_CTX(c);
+    CHECK_LOADED(c);
+    if (c->diag_one) {
+        for (int64_t i = 0; i < c->Nnm; i++) out[i] = 1.0f;   // FG.cpp:4371-4372
+        return SAIGE_OK;
+    }
+    SAIGE_HIP_TRY(hipMemcpy(out, c->d_diag, 4 * c->Nnm, hipMemcpyDeviceToHost));
+    const float m = (float)c->Mqc;   // x / MminMAF (FG.cpp:4368-4369)
+    for (int64_t i = 0; i < c->Nnm; i++) out[i] = out[i] / m;
+    return SAIGE_OK;
+}
+
+int saige_build_grm(saige_ctx* c, int precision) {
+    CHECK_CTX(c);
+    CHECK_LOADED(c);
+    if (precision != SAIGE_GRM_FP32 && precision != SAIGE_GRM_BF16) {
+        set_error("unknown GRM precision %d", precision);
+        return SAIGE_E_ARG;
+    }
+    if (c->Mqc <= 0) {
+        set_error("no marker passed QC; the GRM would be empty");
+        return SAIGE_E_STATE;
+    }
+    hipStream_t st = c->stream;
+    const int64_t Np = c->Npad;
+    if (!c->d_grm) {
+        hipError_t e = dalloc(&c->d_grm, Np * Np);
+        if (e != hipSuccess) {
+            set_error("cannot allocate the %lld x %lld FP32 GRM (%.1f GB)", (long long)Np, (long long)Np,
+                      4.0 * Np * Np / 1e9);
+            return SAIGE_E_NOMEM;
+        }
+    }
+    c->grm_ready = false;
+    const int64_t esz = precision == SAIGE_GRM_BF16 ? 2 : 4;
+    const int64_t kc = precision == SAIGE_GRM_BF16 ? syrk_kchunk_bf16() : syrk_kchunk_f32();
+    size_t free_b = 0, total_b = 0;
+    SAIGE_HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    // panel budget: at most 32 GB and half of what is free
+    int64_t budget = std::min<int64_t>((int64_t)32 << 30, (int64_t)(free_b / 2));
+    int64_t pw = budget / (Np * esz);
+    pw = std::max<int64_t>(kc, pw / 64 * 64);
+    const int64_t full = round_up(c->Mqc, 64);
+    pw = std::min(pw, full);
+    // SAIGE_GRM_PANEL_MARKERS caps the panel width (tests use it to force
+    // several panels, i.e. the accumulate path of the SYRK)
+    if (const char* env = std::getenv("SAIGE_GRM_PANEL_MARKERS")) {
+        const int64_t cap = std::max<int64_t>(64, std::atoll(env) / 64 * 64);
+        pw = std::min(pw, cap);
+    }
+    void* d_panel = nullptr;
+    SAIGE_HIP_TRY(hipMalloc(&d_panel, (size_t)(Np * pw * esz)));
+    SAIGE_HIP_TRY(hipEventRecord(c->ev0, st));
+    hipError_t e = hipSuccess;
+    int first = 1;
+    for (int64_t j0 = 0; j0 < c->Mqc && e == hipSuccess; j0 += pw) {
+        const int64_t np = std::min(pw, c->Mqc - j0);
+        const int64_t ldz = round_up(np, 64);
+        if (precision == SAIGE_GRM_BF16) {
+            e = launch_std_panel_bf16(st, c->d_packed, c->d_lut, j0, np, c->Nnm, Np, c->bpm_nm, ldz,
+                                      static_cast<__hip_bfloat16*>(d_panel));
+            if (e == hipSuccess)
+                e = launch_syrk_bf16(st, static_cast<__hip_bfloat16*>(d_panel), Np, ldz, ldz, c->d_grm, !first);
+        } else {
+            e = launch_std_panel_f32(st, c->d_packed, c->d_lut, j0, np, c->Nnm, Np, c->bpm_nm, ldz,
+                                     static_cast<float*>(d_panel));
+            if (e == hipSuccess)
+                e = launch_syrk_f32(st, static_cast<float*>(d_panel), Np, ldz, ldz, c->d_grm, !first);
+        }
+        first = 0;
+    }
+    if (e == hipSuccess) e = hipEventRecord(c->ev1, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(d_panel);
+    if (e != hipSuccess) {
+        set_error("GRM build failed: %s", hipGetErrorString(e));
+        return SAIGE_E_DEVICE;
+    }
+    float ms = 0.f;
+    SAIGE_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->grm_seconds = ms * 1e-3;
+    c->grm_ready = true;
+    c->grm_precision = precision;
+    return SAIGE_OK;
+}
+
+double saige_last_grm_seconds(const saige_ctx* c) { return c ? c->grm_seconds : 0.0; }
+
+int saige_get_grm_block(saige_ctx* c, int64_t r0, int64_t c0, int64_t nr, int64_t nc, float* out) {
+    CHECK_CTX(c);
+    CHECK_GRM(c);
+    if (r0 < 0 || c0 < 0 || nr < 0 || nc < 0 || r0 + nr > c->Nnm || c0 + nc > c->Nnm) {
+        set_error("GRM block out of range");
+        return SAIGE_E_ARG;
+    }
+    SAIGE_HIP_TRY(hipMemcpy2D(out, sizeof(float) * nc, c->d_grm + r0 * c->Npad + c0, sizeof(float) * c->Npad,
+                              sizeof(float) * nc, nr, hipMemcpyDeviceToHost));
+    const float m = (float)c->Mqc;
+    for (int64_t i = 0; i < nr * nc; i++) out[i] = out[i] / m;
+    return SAIGE_OK;
+}
+
+int saige_grm_matvec(saige_ctx* c, const float* B, int64_t k, float* Y) {
+    CHECK_CTX(c);
+    CHECK_GRM(c);
+    if (k <= 0 || !B || !Y) {
+        set_error("invalid matvec arguments");
+        return SAIGE_E_ARG;
+    }
+    hipStream_t st = c->stream;
+    const int64_t Np = c->Npad, N = c->Nnm;
+    SigmaArgs a{};
+    a.mqc = (float)c->Mqc;
+    a.mode = 0;
+    if (k == 1) {
+        int rc = engine_reserve(c, c->eng1, 1, 1);
+        if (rc) return rc;
+        Engine& e = c->eng1;
+        SAIGE_HIP_TRY(hipMemcpyAsync(e.stage, B, 4 * N, hipMemcpyHostToDevice, st));
+        SAIGE_HIP_TRY(launch_grm_gemv(st, c->d_grm, Np, N, e.stage, e.AP, a, nullptr, nullptr));
+        SAIGE_HIP_TRY(hipMemcpyAsync(Y, e.AP, 4 * N, hipMemcpyDeviceToHost, st));
+        SAIGE_HIP_TRY(hipStreamSynchronize(st));
+        return SAIGE_OK;
+    }
+    // own scratch, so live engine slots are untouched
+    float *d_in = nullptr, *d_blk = nullptr, *d_out = nullptr;
+    auto release = [&]() { dfree(d_in); dfree(d_blk); dfree(d_out); };
+    if (dalloc(&d_in, N * EW) || dalloc(&d_blk, Np * EW) || dalloc(&d_out, Np * EW)) {
+        release();
+        set_error("device allocation failed (grm_matvec)");
+        return SAIGE_E_NOMEM;
+    }
+    hipError_t e = hipSuccess;
+    for (int64_t k0 = 0; k0 < k && e == hipSuccess; k0 += EW) {
+        const int kb = (int)std::min<int64_t>(EW, k - k0);
+        e = hipMemcpyAsync(d_in, B + k0 * N, 4 * N * kb, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = launch_scatter_cols(st, d_in, kb, N, Np, EW, d_blk);
+        if (e == hipSuccess) e = launch_grm_gemm(st, c->d_grm, Np, N, d_blk, d_out, a, nullptr, nullptr);
+        if (e == hipSuccess) e = launch_gather_cols(st, d_out, kb, N, EW, d_in);
+        if (e == hipSuccess) e = hipMemcpyAsync(Y + k0 * N, d_in, 4 * N * kb, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+    }
+    release();
+    if (e != hipSuccess) {
+        set_error("grm_matvec failed: %s", hipGetErrorString(e));
+        return SAIGE_E_DEVICE;
+    }
+    return SAIGE_OK;
+}
+
+int saige_crossprod_mat_and_kin(saige_ctx* c, const float* b, float* out) {
+    return saige_grm_matvec(c, b, 1, out);
+}
+
+int saige_get_diag_of_sigma(saige_ctx* c, const float* w, const float* tau, float* out) {
+    CHECK_CTX(c);
+    CHECK_LOADED(c);
+    if (!w || !tau || !out) {
+        set_error("invalid arguments");
+        return SAIGE_E_ARG;
+    }
+    hipStream_t st = c->stream;
+    const int64_t Np = c->Npad, N = c->Nnm;
+    float* d_w = c->d_vec;
+    float* d_winv = c->d_vec + Np;
+    float* d_minv = c->d_vec + 2 * Np;
+    float* d_d = c->d_vec + 3 * Np;
+    uint8_t* d_mask = nullptr;
+    SAIGE_HIP_TRY(dalloc(&d_mask, Np));
+    SAIGE_HIP_TRY(hipMemcpyAsync(d_w, w, 4 * N, hipMemcpyHostToDevice, st));
+    SAIGE_HIP_TRY(launch_sigma_setup(st, d_w, nullptr, c->d_diag, N, Np, tau[0], tau[1], (float)c->Mqc,
+                                     c->diag_one, d_winv, d_minv, d_mask, d_d));
+    SAIGE_HIP_TRY(hipMemcpyAsync(out, d_d, 4 * N, hipMemcpyDeviceToHost, st));
+    SAIGE_HIP_TRY(hipStreamSynchronize(st));
+    dfree(d_mask);
+    return SAIGE_OK;
+}
+
+int saige_get_crossprod(saige_ctx* c, const float* b, const float* w, const float* tau, float* out) {
+    CHECK_CTX(c);
+    CHECK_GRM(c);
+    if (!b || !w || !tau || !out) {
+        set_error("invalid arguments");
+        return SAIGE_E_ARG;
+    }
+    int rc = engine_reserve(c, c->eng1, 1, 1);
+    if (rc) return rc;
+    Engine& e = c->eng1;
+    rc = engine_set_system(c, e, 0, w, tau, nullptr);
+    if (rc) return rc;
+    hipStream_t st = c->stream;
+    const int64_t Np = c->Npad, N = c->Nnm;
+    e.h_slot_sys[0] = 0;
+    e.h_live[0] = 1;
+    rc = engine_sync_host_arrays(c, e, nullptr, nullptr);
+    if (rc) return rc;
+    SAIGE_HIP_TRY(hipMemcpyAsync(e.stage, b, 4 * N, hipMemcpyHostToDevice, st));
+    SigmaArgs a;
+    a.winv = e.winv;
+    a.mask = e.mask;
+    a.tau = e.tau;
+    a.sys = e.slot_sys;
+    a.live = e.live;
+    a.inv_m = 0.f;
+    a.mqc = (float)c->Mqc;
+    a.mode = 1;
+    SAIGE_HIP_TRY(launch_grm_gemv(st, c->d_grm, Np, N, e.stage, e.AP, a, e.partial, nullptr));
+    SAIGE_HIP_TRY(hipMemcpyAsync(out, e.AP, 4 * N, hipMemcpyDeviceToHost, st));
+    SAIGE_HIP_TRY(hipStreamSynchronize(st));
+    e.h_live[0] = 0;
+    return engine_sync_host_arrays(c, e, nullptr, nullptr);
+}
+
+int saige_pcg1(saige_ctx* c, const float* w, const float* tau, const float* b, int maxiter, float tol, float* x,
+               int32_t* iters) {
+    CHECK_CTX(c);
+    CHECK_GRM(c);
+    if (!w || !tau || !b || !x) {
+        set_error("invalid arguments");
+        return SAIGE_E_ARG;
+    }
+    int rc = engine_reserve(c, c->eng1, 1, 1);
+    if (rc) return rc;
+    Engine& e = c->eng1;
+    if ((rc = engine_set_system(c, e, 0, w, tau, nullptr))) return rc;
+    if ((rc = engine_add(c, e, 0, 0, b, maxiter, tol))) return rc;
+    uint8_t done = 0;
+    int32_t it = 0;
+    while (!done) {
+        if ((rc = engine_step(c, e, 64, &done, &it))) return rc;
+    }
+    if ((rc = engine_take(c, e, 0, x))) return rc;
+    if (iters) *iters = it;
+    if (it >= maxiter && maxiter > 0) {
+        set_error("pcg did not converge. You may increase maxiter number.");
+        return SAIGE_E_NOTCONV;
+    }
+    return SAIGE_OK;
+}
+
+int saige_pcg_block(saige_ctx* c, const float* w, const float* tau, const float* B, int64_t k, int maxiter,
+                    float tol, float* X, int32_t* iters) {
+    CHECK_CTX(c);
+    CHECK_GRM(c);
+    if (!w || !tau || !B || !X || k <= 0) {
+        set_error("invalid arguments");
+        return SAIGE_E_ARG;
+    }
+    int rc = engine_reserve(c, c->eng, EW, std::max(1, c->eng.nsys));
+    if (rc) return rc;
+    Engine& e = c->eng;
+    for (int k0 = 0; k0 < e.kpad; k0++)
+        if (e.h_live[k0]) {
+            set_error("engine has live slots; saige_pcg_block needs an idle engine");
+            return SAIGE_E_STATE;
+        }
+    if ((rc = engine_set_system(c, e, 0, w, tau, nullptr))) return rc;
+    const int64_t N = c->Nnm;
+    bool notconv = false;
+    std::vector<uint8_t> done(EW);
+    std::vector<int32_t> it(EW);
+    for (int64_t k0 = 0; k0 < k; k0 += EW) {
+        const int kb = (int)std::min<int64_t>(EW, k - k0);
+        for (int j = 0; j < kb; j++)
+            if ((rc = engine_add(c, e, j, 0, B + (k0 + j) * N, maxiter, tol))) return rc;
+        for (;;) {
+            if ((rc = engine_step(c, e, 64, done.data(), it.data()))) return rc;
+            bool all = true;
+            for (int j = 0; j < kb; j++) all &= done[j] != 0;
+            if (all) break;
+        }
+        for (int j = 0; j < kb; j++) {
+            if (iters) iters[k0 + j] = it[j];
+            if (it[j] >= maxiter && maxiter > 0) notconv = true;
+            if ((rc = engine_take(c, e, j, X + (k0 + j) * N))) return rc;
+        }
+    }
+    if (notconv) {
+        set_error("pcg did not converge for at least one column. You may increase maxiter number.");
+        return SAIGE_E_NOTCONV;
+    }
+    return SAIGE_OK;
+}
+
+int saige_engine_reserve(saige_ctx* c, int width, int n_systems) {
+    CHECK_CTX(c);
+    CHECK_GRM(c);
+    if (width < 1 || width > EW || n_systems < 1) {
+        set_error("engine width must be in [1, %d] and n_systems >= 1", EW);
+        return SAIGE_E_ARG;
+    }
+    return engine_reserve(c, c->eng, EW, n_systems);
+}
+
+int saige_engine_set_system(saige_ctx* c, int s, const float* w, const float* tau, const uint8_t* mask) {
+    CHECK_CTX(c);
+    CHECK_GRM(c);
+    if (!c->eng.X) {
+        set_error("call saige_engine_reserve first");
+        return SAIGE_E_STATE;
+    }
+    return engine_set_system(c, c->eng, s, w, tau, mask);
+}
+
+int saige_engine_add(saige_ctx* c, int slot, int s, const float* b, int maxiter, float tol) {
+    CHECK_CTX(c);
+    CHECK_GRM(c);
+    if (!c->eng.X || !b) {
+        set_error("engine not reserved or b is null");
+        return SAIGE_E_STATE;
+    }
+    return engine_add(c, c->eng, slot, s, b, maxiter, tol);
+}
+
+int saige_engine_step(saige_ctx* c, int n_iter, uint8_t* done, int32_t* iters) {
+    CHECK_CTX(c);
+    CHECK_GRM(c);
+    if (!c->eng.X) {
+        set_error("engine not reserved");
+        return SAIGE_E_STATE;
+    }
+    return engine_step(c, c->eng, n_iter, done, iters);
+}
+
+int saige_engine_take(saige_ctx* c, int slot, float* x) {
+    CHECK_CTX(c);
+    if (!c->eng.X) {
+        set_error("engine not reserved");
+        return SAIGE_E_STATE;
+    }
+    return engine_take(c, c->eng, slot, x);
+}
+
+int saige_engine_stats(const saige_ctx* c, double* secs, int64_t* launches) {
+    if (!c) {
+        set_error("null context");
+        return SAIGE_E_ARG;
+    }
+    if (secs) *secs = c->eng.gemm_seconds + c->eng1.gemm_seconds;
+    if (launches) *launches = c->eng.gemm_launches + c->eng1.gemm_launches;
+    return SAIGE_OK;
+}
+
+int saige_synth_bed_device(saige_ctx* c, void* d_bed, int64_t N, int64_t M, float maf_lo, float maf_hi,
+                           float miss, float pair_frac, uint64_t seed) {
+    CHECK_CTX(c);
+    if (!d_bed || N <= 0 || M <= 0) {
+        set_error("invalid synthetic bed arguments");
+        return SAIGE_E_ARG;
+    }
+    const int64_t npairs = (int64_t)std::floor((double)pair_frac * (double)N / 2.0);
+    SAIGE_HIP_TRY(launch_synth_bed(c->stream, static_cast<uint8_t*>(d_bed), N, M, maf_lo, maf_hi, miss, npairs,
+                                   seed));
+    SAIGE_HIP_TRY(hipStreamSynchronize(c->stream));
+    return SAIGE_OK;
+}
+
+int saige_alloc_device(saige_ctx* c, int64_t bytes, void** out) {
+    CHECK_CTX(c);
+    if (!out || bytes <= 0) {
+        set_error("invalid allocation request");
+        return SAIGE_E_ARG;
+    }
+    SAIGE_HIP_TRY(hipMalloc(out, (size_t)bytes));
+    return SAIGE_OK;
+}
+
+int saige_free_device(saige_ctx* c, void* p) {
+    CHECK_CTX(c);
+    if (p) SAIGE_HIP_TRY(hipFree(p));
+    return SAIGE_OK;
+}
+
+int saige_copy_device_to_host(saige_ctx* c, const void* src, int64_t bytes, void* dst) {
+    CHECK_CTX(c);
+    if (!src || !dst || bytes < 0) {
+        set_error("invalid copy");
+        return SAIGE_E_ARG;
+    }
+    SAIGE_HIP_TRY(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+    return SAIGE_OK;
+}
+
+}  // extern "C"

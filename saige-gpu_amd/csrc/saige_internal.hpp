@@ -1,0 +1,224 @@
+// Internal declarations shared by the libsaigehip translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/saige_hip.h"
+
+namespace saige {
+
+// Per-marker decode record written by the decode kernel; field meaning as in
+// genoClass::Get_OneSNP_Geno_atBeginning (FG.cpp:326-579).
+struct MarkerStat {
+    int32_t allele_count;
+    int32_t num_missing;
+    int32_t mac;
+    int32_t fill;
+    float freq;
+    float missing_rate;
+    float inv_std;
+    uint8_t pass_qc;
+    uint8_t pass_vr;
+    uint8_t pad[2];
+};
+static_assert(sizeof(MarkerStat) == 32, "MarkerStat layout");
+
+struct DecodeParams {
+    int64_t N;            // samples in the .fam
+    int64_t M;            // markers in the .bim
+    int64_t Nnm;          // phenotyped samples (Nnomissing)
+    int64_t bpm;          // bytes per marker in the .bed, ceil(N/4)
+    float min_maf;
+    float max_missing;
+    int vr_mode;          // 0 none, 1 categorical, 2 single category
+    float min_mac_vr;
+    float max_mac_vr;
+};
+
+// Engine tile width: every block-PCG launch works on EW columns.
+constexpr int EW = 64;
+// Row padding of all N-length device arrays and of the GRM.
+constexpr int64_t NPAD_ALIGN = 256;
+
+struct Engine {
+    int width = 0;        // live slot capacity (<= EW)
+    int nsys = 0;
+    int kpad = 0;         // 1 (GEMV) or EW (MFMA GEMM)
+    // [Npad][kpad] row-major column blocks
+    float* X = nullptr; float* R = nullptr; float* Z = nullptr; float* P = nullptr; float* AP = nullptr;
+    // per system [nsys][Npad]
+    float* winv = nullptr; float* minv = nullptr; uint8_t* mask = nullptr;
+    float* tau = nullptr;         // [nsys][2]
+    // per slot state
+    int32_t* slot_sys = nullptr;  // device [kpad]
+    double* rz = nullptr;         // z^T r
+    double* sumr2 = nullptr;      // r^T r
+    double* pap = nullptr;        // p^T Sigma p
+    float* alpha = nullptr;
+    float* beta = nullptr;
+    int32_t* iter = nullptr;
+    int32_t* maxiter = nullptr;
+    double* tol = nullptr;
+    uint8_t* live = nullptr;      // slot is solving
+    uint8_t* done = nullptr;      // slot stopped (device)
+    double* partial = nullptr;    // [nparts][kpad][2] partial dot products
+    int64_t nparts = 0;
+    float* stage = nullptr;       // [Npad] host-visible staging for one column
+    // pinned host mirrors
+    uint8_t* h_done = nullptr;
+    int32_t* h_iter = nullptr;
+    std::vector<int> h_slot_sys;
+    std::vector<uint8_t> h_live;
+    double gemm_seconds = 0.0;
+    int64_t gemm_launches = 0;
+};
+
+}  // namespace saige
+
+struct saige_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // settings
+    float min_maf = 0.0f;
+    float max_missing = 1.0f;
+    float min_mac_vr = 20.0f, max_mac_vr = -1.0f;
+    int is_var_ratio = 0;
+    std::vector<int32_t> vr_rand;
+    int diag_one = 0;
+    // sizes
+    int64_t N = 0, M = 0, Nnm = 0, Npad = 0, Mqc = 0, Mvr = 0, bpm_nm = 0;
+    bool loaded = false;
+    // decode outputs
+    saige::MarkerStat* d_stats = nullptr;        // [M]
+    std::vector<saige::MarkerStat> h_stats;      // [M]
+    std::vector<int32_t> qc_index, vr_index;     // original marker index of QC'd / VR markers
+    uint8_t* d_packed = nullptr;                 // [Mqc][bpm_nm] internal code
+    uint8_t* d_packed_vr = nullptr;              // [Mvr][bpm_nm]
+    float4* d_lut = nullptr;                     // [Mqc] (L0, L1, L2, 0)
+    float4* d_lut_vr = nullptr;
+    float* d_diag = nullptr;                     // [Npad] sum z^2
+    // GRM S = Z Z^T, [Npad][Npad] FP32
+    float* d_grm = nullptr;
+    bool grm_ready = false;
+    int grm_precision = -1;
+    double grm_seconds = 0.0;
+    // scratch vectors
+    float* d_vec = nullptr;                      // [4][Npad]
+    saige::Engine eng;
+    saige::Engine eng1;                          // kpad = 1 engine for saige_pcg1
+};
+
+namespace saige {
+
+void set_error(const char* fmt, ...);
+
+#define SAIGE_HIP_TRY(expr)                                                            \
+    do {                                                                               \
+        hipError_t _e = (expr);                                                        \
+        if (_e != hipSuccess) {                                                        \
+            saige::set_error("%s failed at %s:%d: %s", #expr, __FILE__, __LINE__,      \
+                             hipGetErrorString(_e));                                   \
+            return (_e == hipErrorOutOfMemory) ? SAIGE_E_NOMEM : SAIGE_E_DEVICE;       \
+        }                                                                              \
+    } while (0)
+
+inline int64_t round_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+// Correctly rounded FP32 division and square root.  The FP32 hardware
+// reciprocal/sqrt paths are not exact on gfx950; computing in FP64 and
+// rounding once is, because FP64 carries more than 2*24+2 significand bits
+// (double rounding is then innocuous for / and sqrt).
+__device__ __forceinline__ float div_rn(float a, float b) { return (float)((double)a / (double)b); }
+__device__ __forceinline__ float sqrt_rn(float a) { return (float)__builtin_sqrt((double)a); }
+
+// ---- genotype kernels (geno_kernels.hip)
+hipError_t launch_decode_stats(hipStream_t s, const uint8_t* bed, const uint8_t* indicator,
+                               const uint8_t* randset, DecodeParams p, MarkerStat* stats);
+hipError_t launch_repack(hipStream_t s, const uint8_t* bed, int64_t bpm, const int32_t* markers,
+                         int64_t nmark, const MarkerStat* stats, const int32_t* subidx,
+                         int64_t Nnm, int64_t bpm_nm, uint8_t* out);
+hipError_t launch_lut(hipStream_t s, const int32_t* markers, int64_t nmark,
+                      const MarkerStat* stats, float4* lut);
+hipError_t launch_diag_std(hipStream_t s, const uint8_t* packed, const float4* lut, int64_t Mqc,
+                           int64_t Nnm, int64_t bpm_nm, float* diag);
+hipError_t launch_get_geno(hipStream_t s, const uint8_t* packed_row, int64_t Nnm, int32_t* out);
+hipError_t launch_std_geno(hipStream_t s, const uint8_t* packed_row, const float4* lut_row,
+                           int64_t Nnm, float* out);
+hipError_t launch_synth_bed(hipStream_t s, uint8_t* bed, int64_t N, int64_t M, float maf_lo,
+                            float maf_hi, float miss, int64_t npairs, uint64_t seed);
+// Z panel [Npad][ldz] for markers [j0, j0+np) (np <= ldz; columns beyond np
+// and rows beyond Nnm are zero).  Element type float or bf16.
+hipError_t launch_std_panel_f32(hipStream_t s, const uint8_t* packed, const float4* lut, int64_t j0,
+                                int64_t np, int64_t Nnm, int64_t Npad, int64_t bpm_nm, int64_t ldz,
+                                float* Zp);
+hipError_t launch_std_panel_bf16(hipStream_t s, const uint8_t* packed, const float4* lut, int64_t j0,
+                                 int64_t np, int64_t Nnm, int64_t Npad, int64_t bpm_nm, int64_t ldz,
+                                 __hip_bfloat16* Zp);
+
+// ---- GRM kernels (grm_kernels.hip)
+// S (Npad x Npad, ld = Npad) (+)= Zp Zp^T over the lower-triangle tiles,
+// mirrored into the upper triangle.  kdim is a multiple of the kernel's k-chunk.
+hipError_t launch_syrk_f32(hipStream_t s, const float* Zp, int64_t Npad, int64_t ldz, int64_t kdim,
+                           float* S, int accumulate);
+hipError_t launch_syrk_bf16(hipStream_t s, const __hip_bfloat16* Zp, int64_t Npad, int64_t ldz,
+                            int64_t kdim, float* S, int accumulate);
+int syrk_kchunk_f32();
+int syrk_kchunk_bf16();
+
+// ---- PCG kernels (pcg_kernels.hip)
+struct SigmaArgs {
+    const float* winv;     // [nsys][Npad]
+    const uint8_t* mask;   // [nsys][Npad]
+    const float* tau;      // [nsys][2]
+    const int32_t* sys;    // [kpad] system of each column
+    const uint8_t* live;   // [kpad] column is being solved (else output zero)
+    float inv_m;           // unused; the product is divided by Mqc as float
+    float mqc;             // Mqc as float (divisor, FG.cpp:1706)
+    int mode;              // 0: out = S p / Mqc ; 1: out = Sigma p
+};
+// out[N][kpad] = epilogue(S * in), plus per-column partial p^T out dots when
+// mode == 1 (partial: [nparts][kpad][2], slot 0).  Returns nparts used.
+hipError_t launch_grm_gemv(hipStream_t s, const float* S, int64_t Npad, int64_t Nnm, const float* in,
+                           float* out, SigmaArgs a, double* partial, int64_t* nparts);
+hipError_t launch_grm_gemm(hipStream_t s, const float* S, int64_t Npad, int64_t Nnm, const float* in,
+                           float* out, SigmaArgs a, double* partial, int64_t* nparts);
+int64_t gemv_max_parts(int64_t Npad);
+int64_t gemm_max_parts(int64_t Npad);
+int64_t vec_max_parts(int64_t Npad, int kpad);
+
+hipError_t launch_sigma_setup(hipStream_t s, const float* w, const uint8_t* mask_in, const float* diag,
+                              int64_t Nnm, int64_t Npad, float tau0, float tau1, float mqc,
+                              int diag_one, float* winv, float* minv, uint8_t* mask_out, float* dout);
+// column init: X=0, R=b (staged), Z=minv*R, P=Z; partial rz/rr dots
+hipError_t launch_col_init(hipStream_t s, const float* b, int slot, int kpad, int64_t Nnm,
+                           int64_t Npad, const float* minv_sys, const uint8_t* mask_sys,
+                           float* X, float* R, float* Z, float* P, double* partial,
+                           int64_t* nparts);
+hipError_t launch_col_init_finish(hipStream_t s, int slot, int kpad, const double* partial,
+                                  int64_t nparts, double* rz, double* sumr2, const int32_t* maxiter,
+                                  const double* tol, int32_t* iter, uint8_t* done, uint8_t* live);
+hipError_t launch_alpha(hipStream_t s, int kpad, const double* partial, int64_t nparts,
+                        const double* rz, const uint8_t* done, const uint8_t* live, float* alpha);
+hipError_t launch_update(hipStream_t s, int kpad, int64_t Nnm, int64_t Npad, const float* alpha,
+                         const uint8_t* done, const uint8_t* live, const float* minv,
+                         const int32_t* sys, float* X, float* R, float* Z, const float* P,
+                         const float* AP, double* partial, int64_t* nparts);
+hipError_t launch_beta(hipStream_t s, int kpad, const double* partial, int64_t nparts, double* rz,
+                       double* sumr2, const int32_t* maxiter, const double* tol, int32_t* iter,
+                       uint8_t* done, const uint8_t* live, float* beta);
+hipError_t launch_pupdate(hipStream_t s, int kpad, int64_t Nnm, int64_t Npad, const float* beta,
+                          const uint8_t* done, const uint8_t* live, const float* Z, float* P);
+hipError_t launch_gather_col(hipStream_t s, const float* src, int slot, int kpad, int64_t Nnm,
+                             float* dst);
+hipError_t launch_scatter_cols(hipStream_t s, const float* src, int64_t k, int64_t Nnm,
+                               int64_t Npad, int kpad, float* dst);
+hipError_t launch_gather_cols(hipStream_t s, const float* src, int64_t k, int64_t Nnm, int kpad,
+                              float* dst);
+hipError_t launch_fill_zero(hipStream_t s, void* p, int64_t bytes);
+
+}  // namespace saige

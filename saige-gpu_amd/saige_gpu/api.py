@@ -1,0 +1,298 @@
+"""Host-side mirror of the reference's Rcpp exports for SAIGE step 1's hot
+path (src/SAIGE/src/SAIGE_fitGLMM_fast.cpp, FG.cpp below), over libsaigehip.
+
+A Session owns one device context; it plays the role of the reference's
+process-global `genoClass geno` (FG.cpp:1188).  Method names follow the
+Rcpp export names, argument meaning and order follow their signatures, and
+failures raise (the reference raises R errors through BEGIN_RCPP/END_RCPP).
+Vectors are numpy float32 of length Nnomissing.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, cols_f32, f32, ptr
+
+
+class Session:
+    def __init__(self, device=0):
+        self._L = _lib.load()
+        h = ctypes.c_void_p()
+        check(self._L.saige_create(int(device), ctypes.byref(h)))
+        self.ctx = h
+        self.device = device
+
+    def close(self):
+        if self.ctx:
+            self._L.saige_destroy(self.ctx)
+            self.ctx = None
+
+    closeGenoFile_plink = close  # FG.cpp:1191-1209
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ------------------------------------------------------------ setters
+    def setminMAFforGRM(self, minMAFforGRM):                      # FG.cpp:4923
+        check(self._L.saige_set_min_maf_for_grm(self.ctx, float(minMAFforGRM)))
+
+    def setmaxMissingRateforGRM(self, maxMissingforGRM):          # FG.cpp:4928
+        check(self._L.saige_set_max_missing_rate_for_grm(self.ctx, float(maxMissingforGRM)))
+
+    def setminMAC_VarianceRatio(self, t_minMACVarRatio, t_maxMACVarRatio,
+                                t_isVarianceRatioinGeno):          # FG.cpp:4970
+        check(self._L.saige_set_min_mac_variance_ratio(self.ctx, float(t_minMACVarRatio),
+                                                       float(t_maxMACVarRatio),
+                                                       int(bool(t_isVarianceRatioinGeno))))
+
+    def set_vr_random_markers(self, idx):
+        """The common-marker pool g_randMarkerIndforVR (FG.cpp:866-871)."""
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        check(self._L.saige_set_vr_random_markers(self.ctx, ptr(idx), len(idx)))
+
+    # ------------------------------------------------------------ ingest
+    def setgeno(self, bedFile, bimFile, famFile, subSampleInGeno, indicatorGenoSamplesWithPheno=None,
+                memoryChunk=2.0, isDiagofKinSetAsOne=False):
+        """setgeno (FG.cpp:2267-2275).  memoryChunk is accepted and unused, as in
+        the reference (numMarkersofEachArray = 1, FG.cpp:824)."""
+        sub = np.ascontiguousarray(subSampleInGeno, dtype=np.int32)
+        ind = None
+        if indicatorGenoSamplesWithPheno is not None:
+            ind = np.ascontiguousarray(np.asarray(indicatorGenoSamplesWithPheno).astype(np.uint8))
+        check(self._L.saige_setgeno(self.ctx, bedFile.encode(), bimFile.encode(), famFile.encode(),
+                                    ptr(sub), len(sub), ptr(ind), int(bool(isDiagofKinSetAsOne))))
+
+    def setgeno_bytes(self, bed_body, N, M, subSampleInGeno=None, indicator=None,
+                      isDiagofKinSetAsOne=False):
+        """setgeno from an in-memory .bed body (magic bytes stripped)."""
+        body = np.ascontiguousarray(bed_body, dtype=np.uint8)
+        assert body.size == M * ((N + 3) // 4)
+        sub = np.arange(1, N + 1, dtype=np.int32) if subSampleInGeno is None else \
+            np.ascontiguousarray(subSampleInGeno, dtype=np.int32)
+        ind = None if indicator is None else np.ascontiguousarray(np.asarray(indicator).astype(np.uint8))
+        check(self._L.saige_setgeno_host(self.ctx, ptr(body), int(N), int(M), ptr(sub), len(sub), ptr(ind),
+                                         int(bool(isDiagofKinSetAsOne))))
+
+    def setgeno_device(self, dev_ptr, N, M, subSampleInGeno=None, indicator=None,
+                       isDiagofKinSetAsOne=False):
+        sub = np.arange(1, N + 1, dtype=np.int32) if subSampleInGeno is None else \
+            np.ascontiguousarray(subSampleInGeno, dtype=np.int32)
+        ind = None if indicator is None else np.ascontiguousarray(np.asarray(indicator).astype(np.uint8))
+        check(self._L.saige_setgeno_device(self.ctx, dev_ptr, int(N), int(M), ptr(sub), len(sub), ptr(ind),
+                                           int(bool(isDiagofKinSetAsOne))))
+
+    # ------------------------------------------------------------ getters
+    def getNnomissingOut(self):
+        return int(self._L.saige_get_nnomissing(self.ctx))
+
+    def gettotalMarker(self):
+        return int(self._L.saige_get_total_markers(self.ctx))
+
+    def getMsub_MAFge_minMAFtoConstructGRM(self):
+        return int(self._L.saige_get_mqc(self.ctx))
+
+    def n_vr_markers(self):
+        return int(self._L.saige_get_n_vr_markers(self.ctx))
+
+    def getAlleleFreqVec(self):
+        out = np.zeros(self.getMsub_MAFge_minMAFtoConstructGRM(), np.float32)
+        check(self._L.saige_get_allele_freq(self.ctx, ptr(out)))
+        return out
+
+    def getMACVec(self):
+        out = np.zeros(self.getMsub_MAFge_minMAFtoConstructGRM(), np.int32)
+        check(self._L.saige_get_mac(self.ctx, ptr(out)))
+        return out
+
+    def getInvStdVec(self):
+        out = np.zeros(self.getMsub_MAFge_minMAFtoConstructGRM(), np.float32)
+        check(self._L.saige_get_inv_std(self.ctx, ptr(out)))
+        return out
+
+    def getQCdMarkerIndex(self):
+        out = np.zeros(self.gettotalMarker(), np.uint8)
+        check(self._L.saige_get_qc_marker_index(self.ctx, ptr(out)))
+        return out.astype(bool)
+
+    def getMACVec_forVarRatio(self):
+        out = np.zeros(self.n_vr_markers(), np.int32)
+        check(self._L.saige_get_mac_vr(self.ctx, ptr(out)))
+        return out
+
+    def getIndexVec_forVarRatio(self):
+        out = np.zeros(self.n_vr_markers(), np.int32)
+        check(self._L.saige_get_index_vr(self.ctx, ptr(out)))
+        return out
+
+    def marker_stats(self):
+        M = self.gettotalMarker()
+        out = {k: np.zeros(M, t) for k, t in [("allele_count", np.int32), ("num_missing", np.int32),
+                                              ("mac", np.int32), ("fill", np.int32),
+                                              ("freq", np.float32), ("missing_rate", np.float32),
+                                              ("inv_std", np.float32), ("flags", np.uint8)]}
+        check(self._L.saige_get_marker_stats(self.ctx, *[ptr(out[k]) for k in
+                                                         ["allele_count", "num_missing", "mac", "fill",
+                                                          "freq", "missing_rate", "inv_std", "flags"]]))
+        return out
+
+    def Get_OneSNP_Geno(self, SNPIdx):                              # FG.cpp:2281
+        out = np.zeros(self.getNnomissingOut(), np.int32)
+        check(self._L.saige_get_one_snp_geno(self.ctx, int(SNPIdx), ptr(out)))
+        return out
+
+    def Get_OneSNP_Geno_forVarRatio(self, SNPIdx):                  # FG.cpp:2291
+        out = np.zeros(self.getNnomissingOut(), np.int32)
+        check(self._L.saige_get_one_snp_geno_vr(self.ctx, int(SNPIdx), ptr(out)))
+        return out
+
+    def Get_OneSNP_StdGeno(self, SNPIdx):                           # FG.cpp:582-662
+        out = np.zeros(self.getNnomissingOut(), np.float32)
+        check(self._L.saige_get_one_snp_std_geno(self.ctx, int(SNPIdx), ptr(out)))
+        return out
+
+    def Get_Diagof_StdGeno(self):                                   # FG.cpp:665-704
+        out = np.zeros(self.getNnomissingOut(), np.float32)
+        check(self._L.saige_get_diag_std(self.ctx, ptr(out)))
+        return out
+
+    def get_DiagofKin(self):                                        # FG.cpp:4358-4375
+        out = np.zeros(self.getNnomissingOut(), np.float32)
+        check(self._L.saige_get_diag_of_kin(self.ctx, ptr(out)))
+        return out
+
+    get_GRMdiagVec = get_DiagofKin                                  # FG.cpp:4913-4920
+
+    # ------------------------------------------------------------ GRM
+    def build_grm(self, precision="fp32"):
+        prec = {"fp32": _lib.GRM_FP32, "bf16": _lib.GRM_BF16}[precision]
+        check(self._L.saige_build_grm(self.ctx, prec))
+        return float(self._L.saige_last_grm_seconds(self.ctx))
+
+    def grm_block(self, r0, c0, nr, nc):
+        out = np.zeros((nr, nc), np.float32)
+        check(self._L.saige_get_grm_block(self.ctx, int(r0), int(c0), int(nr), int(nc), ptr(out)))
+        return out
+
+    def getCrossprodMatAndKin(self, bVec):                          # FG.cpp:1953-1981
+        b = f32(bVec)
+        out = np.zeros_like(b)
+        check(self._L.saige_crossprod_mat_and_kin(self.ctx, ptr(b), ptr(out)))
+        return out
+
+    parallelCrossProd = getCrossprodMatAndKin                      # FG.cpp:1669-1708
+
+    def grm_matvec(self, B):
+        """K B for an N x k block (one GEMM launch per 64 columns)."""
+        B = np.asarray(B)
+        if B.ndim == 1:
+            return self.getCrossprodMatAndKin(B)
+        Bc = cols_f32(B)
+        out = np.zeros(Bc.shape, np.float32, order="F")
+        check(self._L.saige_grm_matvec(self.ctx, ptr(Bc), Bc.shape[1], ptr(out)))
+        return out
+
+    def getDiagOfSigma(self, wVec, tauVec):                         # FG.cpp:2322-2365
+        w = f32(wVec)
+        tau = f32(tauVec)
+        out = np.zeros_like(w)
+        check(self._L.saige_get_diag_of_sigma(self.ctx, ptr(w), ptr(tau), ptr(out)))
+        return out
+
+    def getCrossprod(self, bVec, wVec, tauVec):                     # FG.cpp:2397-2425
+        b, w, tau = f32(bVec), f32(wVec), f32(tauVec)
+        out = np.zeros_like(b)
+        check(self._L.saige_get_crossprod(self.ctx, ptr(b), ptr(w), ptr(tau), ptr(out)))
+        return out
+
+    # ------------------------------------------------------------ PCG
+    def getPCG1ofSigmaAndVector(self, wVec, tauVec, bVec, maxiterPCG=500, tolPCG=1e-5,
+                                return_iters=False):                # FG.cpp:2593-2809
+        w, tau, b = f32(wVec), f32(tauVec), f32(bVec)
+        x = np.zeros_like(b)
+        it = ctypes.c_int32(0)
+        rc = self._L.saige_pcg1(self.ctx, ptr(w), ptr(tau), ptr(b), int(maxiterPCG), float(tolPCG),
+                                ptr(x), ctypes.byref(it))
+        check(rc, allow_notconv=True)   # the reference warns and returns x (FG.cpp:2794-2797)
+        return (x, int(it.value)) if return_iters else x
+
+    def pcg_block(self, wVec, tauVec, B, maxiterPCG=500, tolPCG=1e-5):
+        """Block PCG: k right-hand sides, same Sigma, one GEMM per iteration."""
+        w, tau = f32(wVec), f32(tauVec)
+        Bc = cols_f32(B if np.ndim(B) == 2 else np.asarray(B)[:, None])
+        k = Bc.shape[1]
+        X = np.zeros(Bc.shape, np.float32, order="F")
+        iters = np.zeros(k, np.int32)
+        rc = self._L.saige_pcg_block(self.ctx, ptr(w), ptr(tau), ptr(Bc), k, int(maxiterPCG), float(tolPCG),
+                                     ptr(X), ptr(iters))
+        check(rc, allow_notconv=True)
+        return X, iters
+
+    def getSigma_X(self, wVec, tauVec, Xmat, maxiterPCG=500, tolPCG=1e-5):   # FG.cpp:3347-3366
+        X = np.asarray(Xmat, np.float32)
+        out = np.zeros(X.shape, np.float32)
+        for i in range(X.shape[1]):
+            out[:, i] = self.getPCG1ofSigmaAndVector(wVec, tauVec, X[:, i], maxiterPCG, tolPCG)
+        return out
+
+    def getSigma_G(self, wVec, tauVec, Gvec, maxiterPCG=500, tolPCG=1e-5):   # FG.cpp:3394-3398
+        return self.getPCG1ofSigmaAndVector(wVec, tauVec, Gvec, maxiterPCG, tolPCG)
+
+    # ------------------------------------------------------ engine (batched)
+    def engine_reserve(self, width, n_systems):
+        check(self._L.saige_engine_reserve(self.ctx, int(width), int(n_systems)))
+
+    def engine_set_system(self, s, w, tau, mask=None):
+        w = f32(w)
+        tau = f32(tau)
+        m = None if mask is None else np.ascontiguousarray(np.asarray(mask).astype(np.uint8))
+        check(self._L.saige_engine_set_system(self.ctx, int(s), ptr(w), ptr(tau), ptr(m)))
+
+    def engine_add(self, slot, s, b, maxiter, tol):
+        b = f32(b)
+        check(self._L.saige_engine_add(self.ctx, int(slot), int(s), ptr(b), int(maxiter), float(tol)))
+
+    def engine_step(self, n_iter, width=64):
+        done = np.zeros(width, np.uint8)
+        iters = np.zeros(width, np.int32)
+        check(self._L.saige_engine_step(self.ctx, int(n_iter), ptr(done), ptr(iters)))
+        return done, iters
+
+    def engine_take(self, slot):
+        x = np.zeros(self.getNnomissingOut(), np.float32)
+        check(self._L.saige_engine_take(self.ctx, int(slot), ptr(x)))
+        return x
+
+    def engine_stats(self):
+        s = ctypes.c_double(0.0)
+        n = ctypes.c_int64(0)
+        check(self._L.saige_engine_stats(self.ctx, ctypes.byref(s), ctypes.byref(n)))
+        return float(s.value), int(n.value)
+
+    # ------------------------------------------------------ device helpers
+    def alloc_device(self, nbytes):
+        p = ctypes.c_void_p()
+        check(self._L.saige_alloc_device(self.ctx, int(nbytes), ctypes.byref(p)))
+        return p
+
+    def free_device(self, p):
+        check(self._L.saige_free_device(self.ctx, p))
+
+    def synth_bed_device(self, dev, N, M, maf_lo, maf_hi, miss_rate, pair_frac, seed):
+        check(self._L.saige_synth_bed_device(self.ctx, dev, int(N), int(M), float(maf_lo), float(maf_hi),
+                                             float(miss_rate), float(pair_frac), int(seed)))
+
+    def copy_device_to_host(self, dev, nbytes, offset=0):
+        out = np.zeros(int(nbytes), np.uint8)
+        src = ctypes.c_void_p(dev.value + int(offset))
+        check(self._L.saige_copy_device_to_host(self.ctx, src, int(nbytes), ptr(out)))
+        return out

@@ -1,0 +1,609 @@
+"""Null-model fitting drivers: the host counterpart of the reference's R
+drivers (src/SAIGE/R/SAIGE_fitGLMM_fast.R, FG.R below) and of the
+solver-level Rcpp functions (src/SAIGE/src/SAIGE_fitGLMM_fast.cpp, FG.cpp).
+
+Each fit is written once, as a coroutine that yields work requests:
+    ("solve",  w, tau, B)  ->  X = Sigma^-1 B     (B: n x k, float64)
+    ("matvec", B)          ->  K B
+and receives the results.  Two executors run the coroutines:
+
+* PlainExecutor: every solve column goes through saige_pcg1 (single RHS,
+  GEMV on the resident GRM), one after another -- the reference's call
+  pattern (getPCG1ofSigmaAndVector per vector), i.e. the drop-in path.
+* BatchScheduler: many fits share the block-PCG slot engine (64 columns,
+  FP32 MFMA GEMM per iteration); columns join and leave between iterations
+  (continuous batching), so requests of different fits, sample subsets and
+  phases advance together.
+
+Host arithmetic (p x p inverses, projections, dot products of trace probes,
+the tau update) is FP64.  All O(N^2) work -- every GRM product inside the
+solves and every K u -- runs on the device.
+"""
+import collections
+import time
+
+import numpy as np
+
+# ----------------------------------------------------------------- families
+
+
+class Binomial:
+    """R's binomial(link = "logit") as used by FG.R."""
+    name = "binomial"
+
+    @staticmethod
+    def linkinv(eta):
+        return 1.0 / (1.0 + np.exp(-eta))
+
+    @staticmethod
+    def mu_eta(eta):
+        e = np.exp(-np.abs(eta))
+        return e / (1.0 + e) ** 2
+
+    @staticmethod
+    def variance(mu):
+        return mu * (1.0 - mu)
+
+
+class Gaussian:
+    """R's gaussian(link = "identity")."""
+    name = "gaussian"
+
+    @staticmethod
+    def linkinv(eta):
+        return eta
+
+    @staticmethod
+    def mu_eta(eta):
+        return np.ones_like(eta)
+
+    @staticmethod
+    def variance(mu):
+        return np.ones_like(mu)
+
+
+def glm_start(y, X, family, offset=None, maxit=25, epsilon=1e-8):
+    """Start values of fitNULLGLMM's fit0 = glm(...) (FG.R:1119, 1370): R's
+    glm.fit IRLS with its deviance stopping rule (glm.control defaults)."""
+    n = len(y)
+    off = np.zeros(n) if offset is None else np.asarray(offset, np.float64)
+    y = np.asarray(y, np.float64)
+    if family.name == "binomial":
+        mu = (y + 0.5) / 2.0
+        eta = np.log(mu / (1.0 - mu))
+    else:
+        mu = y.copy()
+        eta = mu.copy()
+
+    def deviance(mu):
+        if family.name == "binomial":
+            with np.errstate(divide="ignore", invalid="ignore"):
+                a = np.where(y > 0, y * np.log(y / mu), 0.0)
+                b = np.where(y < 1, (1 - y) * np.log((1 - y) / (1 - mu)), 0.0)
+            return 2.0 * np.sum(a + b)
+        return float(np.sum((y - mu) ** 2))
+
+    dev_old = deviance(mu)
+    beta = np.zeros(X.shape[1])
+    for _ in range(maxit):
+        me = family.mu_eta(eta)
+        z = (eta - off) + (y - mu) / me
+        wt = me ** 2 / family.variance(mu)
+        sw = np.sqrt(wt)
+        beta = np.linalg.lstsq(X * sw[:, None], z * sw, rcond=None)[0]
+        eta = X @ beta + off
+        mu = family.linkinv(eta)
+        dev = deviance(mu)
+        if abs(dev - dev_old) / (abs(dev) + 0.1) < epsilon:
+            break
+        dev_old = dev
+    return {"coef": beta, "eta": eta, "mu": mu, "offset": off}
+
+
+# ------------------------------------------------------------------ probes
+
+
+class RademacherProbes:
+    """Deterministic Rademacher probe vectors u_i in {-1, +1}^n.
+
+    The reference draws u = 2*rbinom(n, 1, 0.5) - 1 from R's RNG after
+    comm.set.seed(200) at every GetTrace call (FG.cpp:3114, 3133-3137), so
+    every AI iteration reuses the same probes; here probe i is a fixed
+    function of (seed, i) with the same property."""
+
+    def __init__(self, n, seed=200):
+        self.n = n
+        self.seed = seed
+        self._cache = {}
+
+    def __call__(self, i):
+        v = self._cache.get(i)
+        if v is None:
+            rng = np.random.default_rng([self.seed, i])
+            v = rng.integers(0, 2, self.n).astype(np.float64) * 2.0 - 1.0
+            self._cache[i] = v
+        return v
+
+
+# -------------------------------------------------- solver-level coroutines
+
+class Options:
+    def __init__(self, maxiter=20, tol=0.02, nrun=30, tolPCG=1e-5, maxiterPCG=500,
+                 traceCVcutoff=0.0025):
+        self.maxiter = maxiter
+        self.tol = tol
+        self.nrun = nrun
+        self.tolPCG = tolPCG
+        self.maxiterPCG = maxiterPCG
+        self.traceCVcutoff = traceCVcutoff
+
+
+def _inv_sympd(A):
+    """arma::inv_sympd(symmatu(A)) with the pinv fallback (FG.cpp:3183-3191)."""
+    A = np.triu(A) + np.triu(A, 1).T
+    try:
+        np.linalg.cholesky(A)
+        return np.linalg.inv(A)
+    except np.linalg.LinAlgError:
+        return np.linalg.pinv(A)
+
+
+def cal_cv(x):
+    """calCV (FG.cpp:3104-3110)."""
+    x = np.asarray(x, np.float64)
+    return (np.std(x, ddof=1) / np.mean(x)) / len(x)
+
+
+def get_coefficients_co(Y, X, w, tau, opt):
+    """getCoefficients (FG.cpp:3167-3199): Sigma^-1 [Y, X] in one request."""
+    sol = yield ("solve", w, tau, np.column_stack([Y, X]))
+    Sigma_iY = sol[:, 0]
+    Sigma_iX = sol[:, 1:]
+    cov = _inv_sympd(X.T @ Sigma_iX)
+    alpha = cov @ (Sigma_iX.T @ Y)
+    eta = Y - tau[0] * (Sigma_iY - Sigma_iX @ alpha) / w
+    return {"Sigma_iY": Sigma_iY, "Sigma_iX": Sigma_iX, "cov": cov, "alpha": alpha, "eta": eta}
+
+
+def get_coef_co(y, X, tau, fam, alpha0, eta0, offset, opt):
+    """Get_Coef (FG.R:2-35)."""
+    tol_coef = 0.1
+    mu = fam.linkinv(eta0)
+    me = fam.mu_eta(eta0)
+    Y = eta0 - offset + (y - mu) / me
+    W = (me / np.sqrt(fam.variance(mu))) ** 2
+    for _ in range(opt.maxiter):
+        rc = yield from get_coefficients_co(Y, X, W, tau, opt)
+        alpha = rc["alpha"]
+        eta = rc["eta"] + offset
+        mu = fam.linkinv(eta)
+        me = fam.mu_eta(eta)
+        Y = eta - offset + (y - mu) / me
+        W = (me / np.sqrt(fam.variance(mu))) ** 2
+        if np.max(np.abs(alpha - alpha0) / (np.abs(alpha) + np.abs(alpha0) + tol_coef)) < tol_coef:
+            break
+        alpha0 = alpha
+    return {"Y": Y, "alpha": alpha, "eta": eta, "W": W, "cov": rc["cov"],
+            "Sigma_iY": rc["Sigma_iY"], "Sigma_iX": rc["Sigma_iX"], "mu": mu}
+
+
+def _proj(V, Sigma_iV, Sigma_iX, cov):
+    """P v = Sigma^-1 v - Sigma^-1 X cov (Sigma^-1 X)^T v, column-wise."""
+    return Sigma_iV - Sigma_iX @ (cov @ (Sigma_iX.T @ V))
+
+
+def ai_score_co(Y, X, w, tau, Sigma_iY, Sigma_iX, cov, probes, opt, quantitative=False):
+    """getAIScore (FG.cpp:3279-3295) / getAIScore_q (FG.cpp:3479-3531) with
+    GetTrace / GetTrace_q (FG.cpp:3113-3160, 3409-3472).
+
+    The probe solves, their K u products and the PAPY (and PA0PY) solves are
+    independent, so each goes out as one request of nrun+1 (+1) columns."""
+    if quantitative:
+        cov = _inv_sympd(X.T @ Sigma_iX)   # getAIScore_q recomputes cov (FG.cpp:3487-3494)
+    PY = Sigma_iY - Sigma_iX @ (cov @ (Sigma_iX.T @ Y))
+    t1, t0 = [], []
+    start, end = 0, opt.nrun
+    APY = None
+    extra = None
+    while True:
+        U = np.column_stack([probes(i) for i in range(start, end)])
+        if APY is None:
+            KB = yield ("matvec", np.column_stack([PY, U]))
+            APY = KB[:, 0]
+            AU = KB[:, 1:]
+            rhs = [U, APY[:, None]] + ([PY[:, None]] if quantitative else [])
+            sol = yield ("solve", w, tau, np.column_stack(rhs))
+            k = U.shape[1]
+            Sigma_iU = sol[:, :k]
+            extra = sol[:, k:]
+        else:
+            AU = yield ("matvec", U)
+            Sigma_iU = yield ("solve", w, tau, U)
+        PU = _proj(U, Sigma_iU, Sigma_iX, cov)
+        t1.extend(np.einsum("ij,ij->j", AU, PU).tolist())
+        if quantitative:
+            t0.extend(np.einsum("ij,ij->j", U, PU).tolist())
+        cv = cal_cv(t1)
+        cv0 = cal_cv(t0) if quantitative else 0.0
+        if cv > opt.traceCVcutoff or (quantitative and cv0 > opt.traceCVcutoff):
+            start, end = end, end + 10    # FG.cpp:3147-3154
+            continue
+        break
+    YPAPY = PY @ APY
+    PAPY = _proj(APY, extra[:, 0], Sigma_iX, cov)
+    out = {"YPAPY": YPAPY, "PY": PY, "nprobe": len(t1)}
+    if not quantitative:
+        out["Trace"] = float(np.mean(t1))
+        out["AI"] = APY @ PAPY
+    else:
+        A0PY = PY
+        PA0PY = _proj(A0PY, extra[:, 1], Sigma_iX, cov)
+        out["YPA0PY"] = PY @ A0PY
+        out["Trace"] = np.array([np.mean(t0), np.mean(t1)])
+        out["AI"] = np.array([[A0PY @ PA0PY, A0PY @ PAPY], [A0PY @ PAPY, APY @ PAPY]])
+    return out
+
+
+def fitglmmai_rpcg_co(Y, X, w, tau, Sigma_iY, Sigma_iX, cov, probes, opt):
+    """fitglmmaiRPCG (FG.cpp:3302-3341)."""
+    re = yield from ai_score_co(Y, X, w, tau, Sigma_iY, Sigma_iX, cov, probes, opt)
+    Dtau = (re["YPAPY"] - re["Trace"]) / re["AI"]
+    tau0 = np.array(tau, np.float64)
+    tau = tau0.copy()
+    tau[1] = tau0[1] + Dtau
+    tau[tau < opt.tol] = 0.0
+    step = 1.0
+    while tau[1] < 0.0:
+        step *= 0.5
+        tau[1] = tau0[1] + step * Dtau
+    tau[tau < opt.tol] = 0.0
+    return tau, re
+
+
+def fitglmmai_rpcg_q_co(Y, X, w, tau, Sigma_iY, Sigma_iX, cov, probes, opt):
+    """fitglmmaiRPCG_q (FG.cpp:3610-3662): 2x2 AI system, GMMAT zeroing."""
+    tau = np.array(tau, np.float64)
+    zero = tau < opt.tol
+    re = yield from ai_score_co(Y, X, w, tau, Sigma_iY, Sigma_iX, cov, probes, opt, quantitative=True)
+    score = np.array([re["YPA0PY"] - re["Trace"][0], re["YPAPY"] - re["Trace"][1]])
+    Dtau = np.linalg.solve(re["AI"], score)
+    tau0 = tau.copy()
+    tau = tau0 + Dtau
+    tau[zero & (tau < opt.tol)] = 0.0
+    step = 1.0
+    while tau[0] < 0.0 or tau[1] < 0.0:
+        step *= 0.5
+        tau = tau0 + step * Dtau
+        tau[zero & (tau < opt.tol)] = 0.0
+    tau[tau < opt.tol] = 0.0
+    return tau, re
+
+
+# ------------------------------------------------------------ R drivers
+
+def fit_binary_co(y, X, probes, opt, offset=None, fit0=None, tau_init=(0.0, 0.0)):
+    """glmmkin.ai_PCG_Rcpp_Binary (FG.R:79-304) without LOCO."""
+    y = np.asarray(y, np.float64)
+    X = np.asarray(X, np.float64)
+    n = len(y)
+    offset = np.zeros(n) if offset is None else np.asarray(offset, np.float64)
+    fam = Binomial
+    if fit0 is None:
+        fit0 = glm_start(y, X, fam, offset)
+    eta = fit0["eta"]
+    alpha0 = fit0["coef"]
+    eta0 = eta
+    tau = np.array([1.0, 0.1 if tau_init[1] == 0 else float(tau_init[1])])   # FG.R:148-157
+    tau0 = tau.copy()
+    rc = yield from get_coef_co(y, X, tau, fam, alpha0, eta0, offset, opt)
+    re = yield from ai_score_co(rc["Y"], X, rc["W"], tau, rc["Sigma_iY"], rc["Sigma_iX"], rc["cov"],
+                                probes, opt)
+    tau[1] = max(0.0, tau0[1] + tau0[1] ** 2 * (re["YPAPY"] - re["Trace"]) / n)   # FG.R:165
+    history = [tau.copy()]
+    it = 0
+    for it in range(1, opt.maxiter + 1):
+        alpha0 = rc["alpha"]
+        tau0 = tau.copy()
+        eta0 = eta
+        rc = yield from get_coef_co(y, X, tau, fam, alpha0, eta0, offset, opt)
+        tau, _ = yield from fitglmmai_rpcg_co(rc["Y"], X, rc["W"], tau, rc["Sigma_iY"], rc["Sigma_iX"],
+                                              rc["cov"], probes, opt)
+        eta = rc["eta"]
+        history.append(tau.copy())
+        if tau[1] == 0:
+            break
+        if np.max(np.abs(tau - tau0) / (np.abs(tau) + np.abs(tau0) + opt.tol)) < opt.tol:   # FG.R:206
+            break
+        if np.max(tau) > opt.tol ** (-2):
+            it = opt.maxiter
+            break
+    rc = yield from get_coef_co(y, X, tau, fam, rc["alpha"], eta, offset, opt)      # FG.R:218
+    return {"theta": tau, "coefficients": rc["alpha"], "linear_predictors": rc["eta"],
+            "fitted_values": rc["mu"], "Y": rc["Y"], "cov": rc["cov"], "converged": it < opt.maxiter,
+            "history": history, "iterations": it, "traitType": "binary"}
+
+
+def fit_quantitative_co(y, X, probes, opt, offset=None, fit0=None):
+    """glmmkin.ai_PCG_Rcpp_Quantitative (FG.R:309-549) without LOCO."""
+    y = np.asarray(y, np.float64)
+    X = np.asarray(X, np.float64)
+    n = len(y)
+    offset = np.zeros(n) if offset is None else np.asarray(offset, np.float64)
+    fam = Gaussian
+    if fit0 is None:
+        fit0 = glm_start(y, X, fam, offset)
+    eta = fit0["eta"]
+    alpha = fit0["coef"]
+    alpha0 = alpha
+    eta0 = eta
+    tau = np.array([1.0, 0.0])                                   # FG.R:389-398
+    tau0 = tau.copy()
+    rc = yield from get_coef_co(y, X, tau, fam, alpha0, eta0, offset, opt)
+    re = yield from ai_score_co(rc["Y"], X, rc["W"], tau, rc["Sigma_iY"], rc["Sigma_iX"], rc["cov"],
+                                probes, opt, quantitative=True)
+    tau[1] = max(0.0, tau0[1] + tau0[1] ** 2 * (re["YPAPY"] - re["Trace"][1]) / n)   # FG.R:405-406
+    tau[0] = max(0.0, tau0[0] + tau0[0] ** 2 * (re["YPA0PY"] - re["Trace"][0]) / n)
+    history = [tau.copy()]
+    it = 0
+    for it in range(1, opt.maxiter + 1):
+        alpha0 = alpha
+        tau0 = tau.copy()
+        eta0 = eta
+        rc = yield from get_coef_co(y, X, tau, fam, alpha0, eta0, offset, opt)
+        tau, _ = yield from fitglmmai_rpcg_q_co(rc["Y"], X, rc["W"], tau, rc["Sigma_iY"], rc["Sigma_iX"],
+                                                rc["cov"], probes, opt)
+        alpha = rc["alpha"]
+        eta = rc["eta"]
+        history.append(tau.copy())
+        if tau[0] <= 0:
+            raise RuntimeError("ERROR! The first variance component parameter estimate is 0")   # FG.R:449-451
+        if tau[1] <= 0:
+            break
+        if np.max(np.abs(tau - tau0) / (np.abs(tau) + np.abs(tau0) + opt.tol)) < opt.tol:     # FG.R:456
+            break
+        if np.max(tau) > opt.tol ** (-2):
+            it = opt.maxiter
+            break
+    rc = yield from get_coef_co(y, X, tau, fam, alpha, eta, offset, opt)           # FG.R:475
+    return {"theta": tau, "coefficients": rc["alpha"], "linear_predictors": rc["eta"],
+            "fitted_values": rc["mu"], "Y": rc["Y"], "cov": rc["cov"], "converged": it < opt.maxiter,
+            "history": history, "iterations": it, "traitType": "quantitative"}
+
+
+def fit_co(trait, y, X, probes, opt, **kw):
+    if trait == "binary":
+        return fit_binary_co(y, X, probes, opt, **kw)
+    if trait == "quantitative":
+        return fit_quantitative_co(y, X, probes, opt, **kw)
+    raise ValueError(f"unknown trait type {trait!r}")
+
+
+# --------------------------------------------------------------- executors
+
+class PlainExecutor:
+    """Runs a fit coroutine the way the reference does: every right-hand side
+    is its own single-RHS PCG (saige_pcg1), every K b its own GEMV."""
+
+    def __init__(self, sess, opt):
+        self.sess = sess
+        self.opt = opt
+        self.n_solves = 0
+        self.pcg_iters = 0
+
+    def run(self, gen):
+        value = None
+        while True:
+            try:
+                req = gen.send(value)
+            except StopIteration as e:
+                return e.value
+            if req[0] == "solve":
+                _, w, tau, B = req
+                tau32 = np.asarray(tau, np.float32)
+                out = np.zeros(B.shape, np.float64)
+                for j in range(B.shape[1]):
+                    x, it = self.sess.getPCG1ofSigmaAndVector(w, tau32, B[:, j], self.opt.maxiterPCG,
+                                                              self.opt.tolPCG, return_iters=True)
+                    out[:, j] = x
+                    self.n_solves += 1
+                    self.pcg_iters += it
+                value = out
+            else:
+                _, B = req
+                value = np.column_stack([self.sess.getCrossprodMatAndKin(B[:, j])
+                                         for j in range(B.shape[1])]).astype(np.float64)
+
+
+class _Fit:
+    __slots__ = ("fid", "gen", "subset", "n", "sys", "pending", "result_buf", "remaining", "result",
+                 "error", "t_submit", "t_admit", "t_done", "tag", "iters")
+
+    def __init__(self, fid, gen, subset, tag):
+        self.fid = fid
+        self.gen = gen
+        self.subset = subset
+        self.sys = None
+        self.pending = collections.deque()
+        self.result_buf = None
+        self.remaining = 0
+        self.result = None
+        self.error = None
+        self.t_submit = None
+        self.t_admit = None
+        self.t_done = None
+        self.tag = tag
+        self.iters = 0
+
+
+class BatchScheduler:
+    """Continuous batching of PCG iterations over the shared resident GRM.
+
+    Fits are admitted while a system slot (per-fit w, tau, sample mask on the
+    device) is free; their solve columns fill the engine's slots in FIFO
+    order, join at the next iteration, and leave when they converge; a fit
+    resumes as soon as its request's last column is done.  A column's
+    arithmetic does not depend on its neighbours, so each fit gets the same
+    result alone and in the mix."""
+
+    def __init__(self, sess, opt, width=64, n_systems=16):
+        self.sess = sess
+        self.opt = opt
+        self.width = width
+        self.n_systems = n_systems
+        self.N = sess.getNnomissingOut()
+        sess.engine_reserve(width, n_systems)
+        self.free_slots = collections.deque(range(width))
+        self.free_sys = collections.deque(range(n_systems))
+        self.slot_owner = {}
+        self.waiting = collections.deque()      # submitted, not admitted
+        self.active = []                        # admitted
+        self.finished = []
+        self._next_id = 0
+        self.n_iterations = 0
+        self.n_solves = 0
+        self.busy_slot_iters = 0
+
+    # ---------------------------------------------------------------- API
+    def submit(self, gen, subset=None, tag=None, t_submit=None):
+        f = _Fit(self._next_id, gen, None if subset is None else np.asarray(subset, np.int64), tag)
+        self._next_id += 1
+        f.t_submit = time.perf_counter() if t_submit is None else t_submit
+        self.waiting.append(f)
+        return f
+
+    def idle(self):
+        return not self.waiting and not self.active
+
+    def run(self, arrivals=None):
+        """Run until every submitted fit finished.  arrivals: optional list of
+        (t_offset_seconds, gen, subset, tag) released as wall time passes."""
+        t0 = time.perf_counter()
+        arrivals = collections.deque(sorted(arrivals or [], key=lambda a: a[0]))
+        while arrivals or not self.idle():
+            now = time.perf_counter()
+            while arrivals and t0 + arrivals[0][0] <= now:
+                toff, gen, subset, tag = arrivals.popleft()
+                self.submit(gen, subset, tag, t_submit=t0 + toff)
+            self._admit()
+            self._fill()
+            if self.slot_owner:
+                self._step()
+            elif arrivals and self.idle():
+                time.sleep(max(0.0, min(0.05, t0 + arrivals[0][0] - time.perf_counter())))
+        return self.finished
+
+    # ----------------------------------------------------------- internals
+    def _admit(self):
+        while self.waiting and self.free_sys:
+            f = self.waiting.popleft()
+            f.sys = self.free_sys.popleft()
+            f.t_admit = time.perf_counter()
+            self.active.append(f)
+            self._advance(f, None)
+
+    def _expand(self, f, v):
+        if f.subset is None:
+            return v
+        out = np.zeros(self.N, v.dtype)
+        out[f.subset] = v
+        return out
+
+    def _compact(self, f, v):
+        return v if f.subset is None else v[f.subset]
+
+    def _advance(self, f, value):
+        while True:
+            try:
+                req = f.gen.send(value)
+            except StopIteration as e:
+                f.result = e.value
+                self._finish(f)
+                return
+            except Exception as e:   # a failing fit does not stop the others
+                f.error = e
+                self._finish(f)
+                return
+            if req[0] == "matvec":
+                B = req[1]
+                if f.subset is not None:
+                    Bf = np.zeros((self.N, B.shape[1]))
+                    Bf[f.subset] = B
+                    B = Bf
+                KB = self.sess.grm_matvec(B).astype(np.float64)
+                value = KB if f.subset is None else KB[f.subset]
+                continue
+            _, w, tau, B = req
+            mask = None
+            wf = w
+            if f.subset is not None:
+                wf = np.ones(self.N)
+                wf[f.subset] = w
+                mask = np.zeros(self.N, np.uint8)
+                mask[f.subset] = 1
+            self.sess.engine_set_system(f.sys, wf, np.asarray(tau, np.float32), mask)
+            f.result_buf = np.zeros(B.shape, np.float64)
+            f.remaining = B.shape[1]
+            for j in range(B.shape[1]):
+                f.pending.append((j, B[:, j]))
+            return
+
+    def _finish(self, f):
+        f.t_done = time.perf_counter()
+        self.active.remove(f)
+        self.free_sys.append(f.sys)
+        self.finished.append(f)
+
+    def _fill(self):
+        for f in self.active:
+            while f.pending and self.free_slots:
+                j, b = f.pending.popleft()
+                slot = self.free_slots.popleft()
+                self.sess.engine_add(slot, f.sys, self._expand(f, b), self.opt.maxiterPCG, self.opt.tolPCG)
+                self.slot_owner[slot] = (f, j)
+            if not self.free_slots:
+                break
+
+    def _step(self):
+        done, iters = self.sess.engine_step(1, self.width)
+        self.n_iterations += 1
+        self.busy_slot_iters += len(self.slot_owner)
+        resume = []
+        for slot in list(self.slot_owner):
+            if done[slot]:
+                f, j = self.slot_owner.pop(slot)
+                x = self.sess.engine_take(slot)
+                self.free_slots.append(slot)
+                f.result_buf[:, j] = self._compact(f, x)
+                f.iters += int(iters[slot])
+                f.remaining -= 1
+                self.n_solves += 1
+                if f.remaining == 0 and not f.pending:
+                    resume.append(f)
+        for f in resume:
+            self._advance(f, f.result_buf)
+
+
+def fit_plain(sess, trait, y, X, opt=None, probe_seed=200, **kw):
+    """One null fit through the plain single-RHS path."""
+    opt = opt or Options()
+    probes = RademacherProbes(len(y), probe_seed)
+    ex = PlainExecutor(sess, opt)
+    res = ex.run(fit_co(trait, y, X, probes, opt, **kw))
+    res["n_solves"] = ex.n_solves
+    res["pcg_iters"] = ex.pcg_iters
+    return res
+
+
+def fit_batched(sess, jobs, opt=None, width=64, n_systems=16, probe_seed=200):
+    """Many null fits through the batching scheduler.  jobs: list of dicts
+    with keys trait, y, X and optionally subset (sample indices) and tag."""
+    opt = opt or Options()
+    sch = BatchScheduler(sess, opt, width=width, n_systems=n_systems)
+    handles = []
+    for j in jobs:
+        n = len(j["y"])
+        probes = RademacherProbes(n, probe_seed)
+        gen = fit_co(j["trait"], j["y"], j["X"], probes, opt)
+        handles.append(sch.submit(gen, j.get("subset"), j.get("tag")))
+    sch.run()
+    return handles, sch

@@ -1,0 +1,204 @@
+"""GPU parity of the resident GRM (MFMA SYRK), its products, the Sigma
+operators and the PCG solvers, against the oracle (FG.cpp:1576-1708,
+2322-2425, 2593-2809) and dense FP64 linear algebra.
+
+Tolerances (stated here, derived in DESIGN.md):
+  GRM FP32 SYRK   : max |K - K64| <= 2e-6 * max|K64|
+  GRM BF16 SYRK   : max |K - K64| <= 4e-3 * max|K64|
+  K b (FP32)      : max |Kb - Kb64| <= 2e-5 * max|Kb64|
+  PCG (tol 1e-5)  : max |x - x64| <= 2e-3 * max|x64|, ||Sigma x - b||^2 <= 1.5 tol
+"""
+import numpy as np
+import pytest
+
+from fixtures import load_bed, random_bed
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def grm1000(session):
+    body, ids = load_bed("grm1000x10k", 1000)
+    session.setminMAFforGRM(0.01)
+    session.setmaxMissingRateforGRM(0.15)
+    session.setminMAC_VarianceRatio(20, -1, False)
+    session.setgeno_bytes(body, 1000, len(ids))
+    og = O.setgeno(body, 1000, len(ids), min_maf=0.01, max_missing=0.15)
+    S = O.grm_f64(og)
+    return og, S
+
+
+def _k64(og, S):
+    return S / og["Mqc"]
+
+
+@pytest.mark.parametrize("prec,tol,panel", [("fp32", 2e-6, None), ("fp32", 2e-6, "2048"),
+                                            ("bf16", 4e-3, None), ("bf16", 4e-3, "4096")])
+def test_grm_syrk(session, grm1000, monkeypatch, prec, tol, panel):
+    og, S = grm1000
+    print(f"case: SYRK {prec} panel={panel}", flush=True)
+    if panel:
+        monkeypatch.setenv("SAIGE_GRM_PANEL_MARKERS", panel)
+    else:
+        monkeypatch.delenv("SAIGE_GRM_PANEL_MARKERS", raising=False)
+    session.build_grm(prec)
+    K = session.grm_block(0, 0, 1000, 1000)
+    K64 = _k64(og, S)
+    err = np.max(np.abs(K - K64))
+    print(f"  max abs err {err:.3e}  max|K| {np.max(np.abs(K64)):.3f}")
+    assert err <= tol * np.max(np.abs(K64))
+    assert np.array_equal(K, K.T)          # mirrored tiles are exact copies
+    # diagonal against the sequential FP32 diag (independent computation)
+    assert np.allclose(np.diag(K), session.get_DiagofKin(), rtol=10 * tol)
+
+
+def test_grm_products(session, grm1000):
+    og, S = grm1000
+    print("case: GRM products", flush=True)
+    session.build_grm("fp32")
+    K64 = _k64(og, S)
+    rng = np.random.default_rng(7)
+    b = rng.standard_normal(1000).astype(np.float32)
+    kb = session.getCrossprodMatAndKin(b)
+    ref = K64 @ b.astype(np.float64)
+    assert np.max(np.abs(kb - ref)) <= 2e-5 * np.max(np.abs(ref))
+    # the oracle's per-marker FP32 product (the reference's CPU path)
+    kbo = O.crossprod(og, b)
+    assert np.max(np.abs(kb - kbo)) <= 2e-5 * np.max(np.abs(ref))
+    B = rng.standard_normal((1000, 70)).astype(np.float32)      # two GEMM launches
+    KB = session.grm_matvec(B)
+    RB = K64 @ B.astype(np.float64)
+    assert np.max(np.abs(KB - RB)) <= 2e-5 * np.max(np.abs(RB))
+    # GEMM column j equals the GEMV of column j to FP32 rounding
+    assert np.allclose(KB[:, 3], session.getCrossprodMatAndKin(B[:, 3]), rtol=1e-4, atol=1e-5)
+
+
+def test_sigma_operators(session, grm1000):
+    og, S = grm1000
+    print("case: Sigma operators", flush=True)
+    session.build_grm("fp32")
+    rng = np.random.default_rng(8)
+    w = rng.uniform(0.05, 0.25, 1000).astype(np.float32)
+    tau = np.array([1.0, 0.37], np.float32)
+    d = session.getDiagOfSigma(w, tau)
+    do = O.diag_sigma_f32(O.diag_std(og), og["Mqc"], w, tau)
+    assert np.array_equal(d.view(np.uint32), do.view(np.uint32))
+    b = rng.standard_normal(1000).astype(np.float32)
+    s = session.getCrossprod(b, w, tau)
+    so = O.sigma_mul_f32(b, O.crossprod(og, b), w, tau)
+    assert np.max(np.abs(s - so)) <= 1e-5 * np.max(np.abs(so))
+    tau0 = np.array([1.0, 0.0], np.float32)     # tau1 == 0 shortcut: bit-exact
+    s0 = session.getCrossprod(b, w, tau0)
+    assert np.array_equal(s0, O.sigma_mul_f32(b, np.zeros(1000, np.float32), w, tau0))
+
+
+def _sigma64(og, S, w, tau):
+    return tau[0] * np.diag(1.0 / w.astype(np.float64)) + tau[1] * _k64(og, S)
+
+
+def test_pcg1(session, grm1000):
+    og, S = grm1000
+    print("case: pcg1", flush=True)
+    session.build_grm("fp32")
+    rng = np.random.default_rng(9)
+    w = rng.uniform(0.05, 0.25, 1000).astype(np.float32)
+    tau = np.array([1.0, 0.3], np.float32)
+    b = rng.standard_normal(1000).astype(np.float32)
+    x, it = session.getPCG1ofSigmaAndVector(w, tau, b, 500, 1e-5, return_iters=True)
+    Sig = _sigma64(og, S, w, tau)
+    x64 = np.linalg.solve(Sig, b.astype(np.float64))
+    assert np.max(np.abs(x - x64)) <= 2e-3 * np.max(np.abs(x64))
+    r = Sig @ x.astype(np.float64) - b
+    assert r @ r <= 1.5e-5
+    # the oracle's FP32 PCG on the same problem takes about as many steps
+    xo, ito = O.pcg_f32(og, O.diag_std(og), w, tau, b, 500, 1e-5)
+    assert abs(it - ito) <= max(3, ito // 10)
+    assert np.max(np.abs(x - xo)) <= 2e-3 * np.max(np.abs(x64))
+    # maxiter reached: x is returned, the status is "not converged"
+    from saige_gpu import _lib
+    import ctypes
+    L = _lib.load()
+    xx = np.zeros(1000, np.float32)
+    itc = ctypes.c_int32(0)
+    rc = L.saige_pcg1(session.ctx, _lib.ptr(w), _lib.ptr(tau), _lib.ptr(b), 2, 1e-12, _lib.ptr(xx),
+                      ctypes.byref(itc))
+    assert rc == _lib.SAIGE_E_NOTCONV and itc.value == 2 and np.any(xx != 0)
+
+
+def test_pcg_block_and_engine(session, grm1000):
+    og, S = grm1000
+    print("case: block PCG / engine", flush=True)
+    session.build_grm("fp32")
+    rng = np.random.default_rng(10)
+    w = rng.uniform(0.05, 0.25, 1000).astype(np.float32)
+    tau = np.array([1.0, 0.5], np.float32)
+    B = rng.standard_normal((1000, 5)).astype(np.float32)
+    X, iters = session.pcg_block(w, tau, B, 500, 1e-5)
+    Sig = _sigma64(og, S, w, tau)
+    X64 = np.linalg.solve(Sig, B.astype(np.float64))
+    assert np.max(np.abs(X - X64)) <= 2e-3 * np.max(np.abs(X64))
+    for j in range(5):
+        x1 = session.getPCG1ofSigmaAndVector(w, tau, B[:, j], 500, 1e-5)
+        assert np.max(np.abs(X[:, j] - x1)) <= 2e-3 * np.max(np.abs(X64[:, j]))
+
+    # engine: a masked system solves the restricted problem
+    session.engine_reserve(64, 4)
+    sub = np.sort(rng.choice(1000, 640, replace=False))
+    mask = np.zeros(1000, np.uint8)
+    mask[sub] = 1
+    session.engine_set_system(0, w, tau)
+    session.engine_set_system(1, w, np.array([0.8, 0.2], np.float32), mask)
+    bsub = np.zeros(1000, np.float32)
+    bsub[sub] = rng.standard_normal(len(sub))
+    session.engine_add(0, 0, B[:, 0], 500, 1e-5)
+    session.engine_add(7, 1, bsub, 500, 1e-5)
+    for _ in range(1000):
+        done, it = session.engine_step(1)
+        if done[0] and done[7]:
+            break
+    x0 = session.engine_take(0)
+    x7 = session.engine_take(7)
+    assert np.max(np.abs(x0 - X64[:, 0])) <= 2e-3 * np.max(np.abs(X64[:, 0]))
+    Ss = _sigma64(og, S, w, np.array([0.8, 0.2]))[np.ix_(sub, sub)]
+    xs = np.linalg.solve(Ss, bsub[sub].astype(np.float64))
+    assert np.all(x7[mask == 0] == 0)
+    assert np.max(np.abs(x7[sub] - xs)) <= 2e-3 * np.max(np.abs(xs))
+
+    # alone vs in a mix: a column's result does not depend on its neighbours
+    session.engine_add(3, 0, B[:, 2], 500, 1e-5)
+    while not session.engine_step(50)[0][3]:
+        pass
+    alone = session.engine_take(3)
+    for slot in range(64):
+        session.engine_add(slot, slot % 2, B[:, slot % 5] * (1 + slot) if slot != 40 else B[:, 2], 500, 1e-5)
+    for _ in range(2000):
+        done, _ = session.engine_step(1)
+        if done.all():
+            break
+    mixed = session.engine_take(40)
+    for slot in range(64):
+        if slot != 40:
+            session.engine_take(slot)
+    assert np.array_equal(alone.view(np.uint32), mixed.view(np.uint32))
+
+
+def test_odd_sizes(session):
+    """N not a multiple of the tiles, a subset, several panels."""
+    N, M = 333, 250
+    print("case: odd sizes", flush=True)
+    body, G, mm = random_bed(N, M, 0.03, seed=33, maf_lo=0.02, maf_hi=0.5)
+    sub = np.arange(3, N + 1, dtype=np.int32)
+    session.setminMAFforGRM(0.01)
+    session.setmaxMissingRateforGRM(0.15)
+    session.setgeno_bytes(body, N, M, sub)
+    og = O.setgeno(body, N, M, subidx=sub, min_maf=0.01, max_missing=0.15)
+    session.build_grm("fp32")
+    K = session.grm_block(0, 0, len(sub), len(sub))
+    K64 = O.grm_f64(og) / og["Mqc"]
+    assert np.max(np.abs(K - K64)) <= 2e-6 * np.max(np.abs(K64))
+    w = np.full(len(sub), 0.2, np.float32)
+    b = np.linspace(-1, 1, len(sub)).astype(np.float32)
+    x = session.getPCG1ofSigmaAndVector(w, np.array([1.0, 0.4], np.float32), b)
+    x64 = np.linalg.solve(np.diag(np.full(len(sub), 5.0)) + 0.4 * K64, b)
+    assert np.max(np.abs(x - x64)) <= 2e-3 * np.max(np.abs(x64))

@@ -1,0 +1,202 @@
+"""Pins the oracle's FP32 restatement (standardisation, diagonal, GRM
+product, Sigma operators, PCG) and its FP64 solver layer (coefficients,
+trace, AI score, tau updates, glm start) against closed forms and dense
+linear algebra.  Each check is chosen so that a dropped term, a wrong sign
+or a transposed operand fails."""
+import numpy as np
+import pytest
+
+from fixtures import random_bed
+from oracle import oracle as O
+
+
+@pytest.fixture(scope="module")
+def small():
+    N, M = 120, 300
+    body, G, miss = random_bed(N, M, 0.02, seed=11, maf_lo=0.05, maf_hi=0.5)
+    g = O.setgeno(body, N, M, min_maf=0.01)
+    return g
+
+
+def test_std_geno_closed_form(small):
+    g = small
+    f32 = np.float32
+    for j in range(0, g["Mqc"], 37):
+        geno = O.get_geno(g["packed"][j], g["Nnm"])
+        f, s = g["freq"][j], g["inv_std"][j]
+        m2 = f32(2) * f
+        L = np.array([(f32(0) - m2) * s, (f32(1) - m2) * s, (f32(2) - m2) * s], np.float32)
+        z = O.std_geno(g["packed"][j], g["Nnm"], f, s)
+        assert np.array_equal(z.view(np.uint32), L[geno].view(np.uint32))
+        # FP64 closed form (g - 2f)/sqrt(2f(1-f)) to FP32 rounding
+        exact = (geno - 2.0 * float(f)) / np.sqrt(2.0 * float(f) * (1.0 - float(f)))
+        assert np.max(np.abs(z - exact)) < 1e-5 * max(1.0, np.max(np.abs(exact)))
+        # standardised columns have mean 0 (up to the FP32 rounding of f)
+        assert abs(np.mean(exact)) < 1e-6
+
+
+def test_diag_sequential_fp32(small):
+    g = small
+    Z = O.std_matrix(g)
+    d = np.zeros(g["Nnm"], np.float32)
+    for j in range(g["Mqc"]):
+        d = d + Z[:, j] * Z[:, j]        # numpy float32: each op rounded
+    assert np.array_equal(O.diag_std(g).view(np.uint32), d.view(np.uint32))
+    S = O.grm_f64(g)
+    assert np.allclose(O.diag_std(g), np.diag(S), rtol=1e-5)
+
+
+def test_crossprod_against_dense(small):
+    g = small
+    rng = np.random.default_rng(1)
+    b = rng.standard_normal(g["Nnm"]).astype(np.float32)
+    S = O.grm_f64(g)
+    ref = S @ b.astype(np.float64) / g["Mqc"]
+    for nt in (1, 3):
+        out = O.crossprod(g, b, nthreads=nt)
+        assert np.max(np.abs(out - ref)) < 2e-4 * np.max(np.abs(ref))
+    # a transposed or unscaled product would fail: K is symmetric PSD, K b . b >= 0
+    assert float(O.crossprod(g, b) @ b) > 0
+
+
+def test_sigma_operators_fp32():
+    rng = np.random.default_rng(2)
+    N = 50
+    w = rng.uniform(0.05, 0.25, N).astype(np.float32)
+    diag = rng.uniform(50, 150, N).astype(np.float32)
+    tau = (np.float32(1.0), np.float32(0.3))
+    d = O.diag_sigma_f32(diag, 100, w, tau)
+    f32 = np.float32
+    expect = np.maximum((tau[1] * diag) / f32(100) + tau[0] / w, f32(1e-4))
+    assert np.array_equal(d, expect.astype(np.float32))
+    d1 = O.diag_sigma_f32(diag, 100, w, tau, diag_one=True)
+    assert np.array_equal(d1, np.maximum(tau[1] + tau[0] / w, f32(1e-4)))
+    # clamp at 1e-4
+    assert np.all(O.diag_sigma_f32(diag, 100, np.full(N, 1e9, np.float32), (f32(1), f32(0))) == f32(1e-4))
+    b = rng.standard_normal(N).astype(np.float32)
+    kb = rng.standard_normal(N).astype(np.float32)
+    s = O.sigma_mul_f32(b, kb, w, tau)
+    assert np.array_equal(s, tau[0] * (b * (f32(1) / w)) + tau[1] * kb)
+    s0 = O.sigma_mul_f32(b, kb, w, (f32(1), f32(0)))
+    assert np.array_equal(s0, f32(1) * (b * (f32(1) / w)))
+
+
+def _dense_problem(seed=3, N=150, M=400):
+    body, G, miss = random_bed(N, M, 0.01, seed=seed, maf_lo=0.05, maf_hi=0.5)
+    g = O.setgeno(body, N, M, min_maf=0.01)
+    S = O.grm_f64(g)
+    grm = O.DenseGRM(S, g["Mqc"], O.diag_std(g))
+    return g, grm
+
+
+def test_pcg_fp32_and_fp64_solve_sigma():
+    g, grm = _dense_problem()
+    rng = np.random.default_rng(4)
+    N = g["Nnm"]
+    w = rng.uniform(0.1, 0.25, N)
+    tau = np.array([1.0, 0.4])
+    b = rng.standard_normal(N)
+    exact = np.linalg.solve(grm.sigma(w, tau), b)
+    x64, it64 = grm.pcg(w, tau, b, 500, 1e-10)
+    assert np.max(np.abs(x64 - exact)) < 1e-6 * np.max(np.abs(exact))
+    # residual invariant of the stopping rule ||Sigma x - b||^2 <= tol
+    x64b, _ = grm.pcg(w, tau, b, 500, 1e-5)
+    r = grm.sigma(w, tau) @ x64b - b
+    assert r @ r <= 1e-5 * 1.0001
+    diag = O.diag_std(g)
+    x32, it32 = O.pcg_f32(g, diag, w.astype(np.float32), tau.astype(np.float32),
+                          b.astype(np.float32), 500, 1e-5)
+    assert np.max(np.abs(x32 - exact)) < 1e-3 * np.max(np.abs(exact))
+    assert 0 < it32 < 500
+    # dense-K FP32 variant follows the same recurrence
+    Kd = (O.grm_f64(g) / g["Mqc"]).astype(np.float32)
+    x32d, _ = O.pcg_f32(g, diag, w.astype(np.float32), tau.astype(np.float32), b.astype(np.float32),
+                        500, 1e-5, Kdense=Kd)
+    assert np.max(np.abs(x32d - exact)) < 1e-3 * np.max(np.abs(exact))
+    # maxiter stops the loop
+    _, it1 = O.pcg_f32(g, diag, w.astype(np.float32), tau.astype(np.float32), b.astype(np.float32), 1, 1e-12)
+    assert it1 == 1
+
+
+def _P(grm, w, tau, X):
+    Si = np.linalg.inv(grm.sigma(w, tau))
+    return Si - Si @ X @ np.linalg.inv(X.T @ Si @ X) @ X.T @ Si
+
+
+def test_coefficients_trace_ai_dense():
+    g, grm = _dense_problem(seed=5)
+    N = g["Nnm"]
+    rng = np.random.default_rng(6)
+    w = rng.uniform(0.1, 0.25, N)
+    tau = np.array([1.0, 0.3])
+    X = np.column_stack([np.ones(N), rng.standard_normal(N)])
+    Y = rng.standard_normal(N)
+    rc = O.get_coefficients(grm, Y, X, w, tau, 1000, 1e-12)
+    Si = np.linalg.inv(grm.sigma(w, tau))
+    cov = np.linalg.inv(X.T @ Si @ X)
+    assert np.allclose(rc["cov"], cov, rtol=1e-7)
+    alpha = cov @ X.T @ Si @ Y
+    assert np.allclose(rc["alpha"], alpha, rtol=1e-7)
+    # eta = Y - tau0 * P Y / w  (the BLUP identity behind FG.cpp:3197)
+    P = _P(grm, w, tau, X)
+    assert np.allclose(rc["eta"], Y - tau[0] * (P @ Y) / w, atol=1e-6)
+    # trace estimator with given probes = mean(u^T K P u)
+    U = [np.where(rng.random(N) < 0.5, -1.0, 1.0) for _ in range(40)]
+    tr, n = O.get_trace(grm, rc["Sigma_iX"], X, w, tau, rc["cov"], 30, lambda i: U[i], 1000, 1e-12,
+                        cv_cutoff=1.0)
+    assert n == 30
+    assert np.isclose(tr, np.mean([u @ grm.K @ P @ u for u in U[:30]]), rtol=1e-8)
+    # and its expectation is tr(P K): the estimator is unbiased
+    assert abs(tr - np.trace(P @ grm.K)) < 0.2 * np.trace(P @ grm.K)
+    # CV growth adds probes in steps of 10 (FG.cpp:3147-3154): a cutoff just
+    # below the 30-probe CV forces exactly one extension to 40
+    t30 = [u @ grm.K @ P @ u for u in U[:30]]
+    cut = O.cal_cv(t30) * 0.99
+    _, n2 = O.get_trace(grm, rc["Sigma_iX"], X, w, tau, rc["cov"], 30, lambda i: U[i], 1000, 1e-12,
+                        cv_cutoff=cut)
+    assert n2 == 40
+    re = O.get_ai_score(grm, Y, X, w, tau, rc["Sigma_iY"], rc["Sigma_iX"], rc["cov"], 30,
+                        lambda i: U[i], 1000, 1e-12, cv_cutoff=1.0)
+    PY = P @ Y
+    assert np.isclose(re["YPAPY"], PY @ grm.K @ PY, rtol=1e-8)
+    assert np.isclose(re["AI"], PY @ grm.K @ P @ grm.K @ PY, rtol=1e-8)
+
+
+def test_tau_updates_closed_form():
+    g, grm = _dense_problem(seed=7)
+    N = g["Nnm"]
+    rng = np.random.default_rng(8)
+    w = rng.uniform(0.1, 0.25, N)
+    X = np.ones((N, 1))
+    Y = rng.standard_normal(N)
+    probes = lambda i: np.where(np.random.default_rng(i).random(N) < 0.5, -1.0, 1.0)  # noqa: E731
+    tau = np.array([1.0, 0.2])
+    rc = O.get_coefficients(grm, Y, X, w, tau, 1000, 1e-12)
+    new, re = O.fitglmmai_rpcg(grm, Y, X, w, tau, rc["Sigma_iY"], rc["Sigma_iX"], rc["cov"], 30, probes,
+                               1000, 1e-12, tol=0.02, cv_cutoff=1.0)
+    step = (re["YPAPY"] - re["Trace"]) / re["AI"]
+    cand = 0.2 + step
+    if cand >= 0.02:
+        assert np.isclose(new[1], cand)
+    assert new[0] == 1.0
+    # 2x2 update solves AI dtau = score
+    newq, req = O.fitglmmai_rpcg_q(grm, Y, X, np.ones(N), tau, rc["Sigma_iY"], rc["Sigma_iX"], rc["cov"],
+                                   30, probes, 1000, 1e-12, tol=0.02, cv_cutoff=1.0)
+    score = np.array([req["YPA0PY"] - req["Trace"][0], req["YPAPY"] - req["Trace"][1]])
+    d = np.linalg.solve(req["AI"], score)
+    if np.all(tau + d >= 0.02):
+        assert np.allclose(newq, tau + d)
+    assert np.allclose(req["AI"], req["AI"].T)
+
+
+def test_glm_start_score_equations():
+    rng = np.random.default_rng(9)
+    N = 500
+    X = np.column_stack([np.ones(N), rng.standard_normal(N), rng.random(N) < 0.5])
+    y = (rng.random(N) < 1 / (1 + np.exp(-(X @ np.array([-1.0, 0.5, 0.3]))))).astype(float)
+    fit = O.glm_fit(y, X, O.Binomial)
+    # MLE: X^T (y - mu) = 0
+    assert np.max(np.abs(X.T @ (y - fit["mu"]))) < 1e-6
+    yq = X @ np.array([1.0, 2.0, -1.0]) + rng.standard_normal(N)
+    fq = O.glm_fit(yq, X, O.Gaussian)
+    assert np.allclose(fq["coef"], np.linalg.lstsq(X, yq, rcond=None)[0])
