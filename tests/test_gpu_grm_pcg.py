@@ -3,8 +3,8 @@ operators and the PCG solvers, against the oracle (FG.cpp:1576-1708,
 2322-2425, 2593-2809) and dense FP64 linear algebra.
 
 Tolerances (stated here, derived in DESIGN.md):
-  GRM FP32 SYRK   : max |K - K64| <= 2e-6 * max|K64|
-  GRM BF16 SYRK   : max |K - K64| <= 4e-3 * max|K64|
+  GRM FP32 SYRK   : max |K - K64| <= 4 sqrt(Mqc) 2^-24 max|K64|  (FP32 sum of Mqc terms)
+  GRM BF16 SYRK   : max |K - K64| <= 4e-3 max|K64|               (BF16 z, 2^-9 per operand)
   K b (FP32)      : max |Kb - Kb64| <= 2e-5 * max|Kb64|
   PCG (tol 1e-5)  : max |x - x64| <= 2e-3 * max|x64|, ||Sigma x - b||^2 <= 1.5 tol
 """
@@ -33,7 +33,11 @@ def _k64(og, S):
     return S / og["Mqc"]
 
 
-@pytest.mark.parametrize("prec,tol,panel", [("fp32", 2e-6, None), ("fp32", 2e-6, "2048"),
+def fp32_tol(mqc):
+    return 4.0 * np.sqrt(mqc) * 2.0 ** -24
+
+
+@pytest.mark.parametrize("prec,tol,panel", [("fp32", None, None), ("fp32", None, "2048"),
                                             ("bf16", 4e-3, None), ("bf16", 4e-3, "4096")])
 def test_grm_syrk(session, grm1000, monkeypatch, prec, tol, panel):
     og, S = grm1000
@@ -45,8 +49,9 @@ def test_grm_syrk(session, grm1000, monkeypatch, prec, tol, panel):
     session.build_grm(prec)
     K = session.grm_block(0, 0, 1000, 1000)
     K64 = _k64(og, S)
+    tol = tol or fp32_tol(og["Mqc"])
     err = np.max(np.abs(K - K64))
-    print(f"  max abs err {err:.3e}  max|K| {np.max(np.abs(K64)):.3f}")
+    print(f"  max abs err {err:.3e}  max|K| {np.max(np.abs(K64)):.3f}  tol {tol:.2e}")
     assert err <= tol * np.max(np.abs(K64))
     assert np.array_equal(K, K.T)          # mirrored tiles are exact copies
     # diagonal against the sequential FP32 diag (independent computation)
@@ -196,7 +201,7 @@ def test_odd_sizes(session):
     session.build_grm("fp32")
     K = session.grm_block(0, 0, len(sub), len(sub))
     K64 = O.grm_f64(og) / og["Mqc"]
-    assert np.max(np.abs(K - K64)) <= 2e-6 * np.max(np.abs(K64))
+    assert np.max(np.abs(K - K64)) <= fp32_tol(og["Mqc"]) * np.max(np.abs(K64))
     w = np.full(len(sub), 0.2, np.float32)
     b = np.linspace(-1, 1, len(sub)).astype(np.float32)
     x = session.getPCG1ofSigmaAndVector(w, np.array([1.0, 0.4], np.float32), b)
