@@ -147,6 +147,7 @@ def main():
     gemm_flop = 2.0 * N * N * 64
     gemm_tflops = gemm_flop / gemm_avg / 1e12
     lat = sorted(h.t_done - h.t_submit for h in handles)
+    log("batched step split (s): " + ", ".join(f"{k} {v:.3f}" for k, v in sorted(sch.timing.items())))
     log(f"batched: {step_s:.2f}s/step, {fits_per_hour:.1f} fits/h, {gemm_launches} GEMMs avg "
         f"{gemm_avg * 1e3:.2f} ms ({gemm_tflops:.1f} TFLOP/s, {4.0 * N * N / gemm_avg / 1e9:.0f} GB/s), "
         f"theta {[list(np.round(h.result['theta'], 4)) for h in handles]}")
@@ -160,6 +161,7 @@ def main():
         g3, n3 = sess.engine_stats()
         gemv_avg = (g3 - g2) / max(1, n3 - n2)
         plain = {"seconds": plain_s, "fits_per_hour": 3600.0 / plain_s, "pcg_iters": pr["pcg_iters"],
+                 "grm_products": pr["pcg_iters"] + pr["n_products"],
                  "n_solves": pr["n_solves"], "gemv_ms": gemv_avg * 1e3,
                  "gemv_GBps": 4.0 * N * N / gemv_avg / 1e9, "theta": list(map(float, pr["theta"]))}
         log(f"plain: {plain_s:.1f}s per fit, {plain['fits_per_hour']:.1f} fits/h, GEMV {gemv_avg * 1e3:.2f} ms "
@@ -168,7 +170,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        n_matvec = plain["pcg_iters"] if plain else 3000
+        n_matvec = plain["grm_products"] if plain else 3000
         per_marker, ms, reps = cpu_baseline(sess, dev, N, M, n_matvec, threads, args.cpu_sample_markers)
         t_mv = per_marker * Mqc
         cpu_fph = 3600.0 / (n_matvec * t_mv)

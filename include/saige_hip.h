@@ -187,6 +187,19 @@ int saige_engine_set_system(saige_ctx* ctx, int s, const float* w_host, const fl
 /* Start solving Sigma_s x = b in slot `slot` (b: N floats on host). */
 int saige_engine_add(saige_ctx* ctx, int slot, int s, const float* b_host, int maxiter,
                      float tol);
+/* Start k solves in one call: column j of B_host (N x k, column-major)
+ * goes to slot slots[j] with system systems[j]. */
+int saige_engine_add_batch(saige_ctx* ctx, int k, const int32_t* slots, const int32_t* systems,
+                           const float* B_host, int maxiter, float tol);
+/* Start k GRM products K b in slots: column j of B_host (N x k) in slot
+ * slots[j], restricted to system systems[j]'s samples.  A product slot rides
+ * along in the next iteration's GEMM and is then done; its result is read
+ * with saige_engine_take_batch like a solution. */
+int saige_engine_add_products(saige_ctx* ctx, int k, const int32_t* slots, const int32_t* systems,
+                              const float* B_host);
+/* Copy the solutions of k slots to X_host (N x k, column-major; NULL to
+ * discard) and free the slots. */
+int saige_engine_take_batch(saige_ctx* ctx, int k, const int32_t* slots, float* X_host);
 /* Run `n_iter` iterations on every live slot.  done[slot] = 1 when the slot
  * has stopped; iters[slot] = its iteration count (either may be NULL). */
 int saige_engine_step(saige_ctx* ctx, int n_iter, uint8_t* done_host, int32_t* iters_host);
@@ -209,6 +222,9 @@ int saige_synth_bed_device(saige_ctx* ctx, void* bed_body_dev, int64_t N, int64_
 /* Device allocation helper for the above (freed by saige_free_device). */
 int saige_alloc_device(saige_ctx* ctx, int64_t bytes, void** out);
 int saige_free_device(saige_ctx* ctx, void* p);
+/* Device-to-device copy on the context's stream (the known-size copy that
+ * calibrates HBM traffic counters in profiles). */
+int saige_copy_device(saige_ctx* ctx, void* dst_dev, const void* src_dev, int64_t bytes);
 /* Copy rows of a device .bed body to host (for building phenotypes). */
 int saige_copy_device_to_host(saige_ctx* ctx, const void* src_dev, int64_t bytes, void* dst_host);
 

@@ -4,12 +4,17 @@
 // columns), so both MFMA operands are K-contiguous rows of the same matrix.
 // Each workgroup computes one 128x128 tile of the lower triangle (tile row
 // >= tile column) and writes it and its mirror into the full row-major S.
-// Tiles are enumerated so that workgroups landing on the same XCD (launch
-// order round-robins over the 8 XCDs) take consecutive tiles of the
-// triangle, which share their A row panel in that XCD's L2.
+// Tile order: the host lists the triangle's tiles grouped into 8x8-tile
+// super-blocks, and the launch index is remapped so that each XCD (launch
+// order round-robins over the 8 XCDs) walks a contiguous run of that list.
+// The ~100 tiles an XCD holds at once then span one or two super-blocks,
+// i.e. 16-24 distinct 128-row panels of Zp, whose current K chunk fits in
+// that XCD's 4 MB L2.
 //
 // Operand staging: global -> registers (prefetch of the next K chunk while
 // the current one is multiplied) -> LDS -> MFMA fragments.
+#include <algorithm>
+
 #include "saige_internal.hpp"
 
 namespace saige {
@@ -17,18 +22,11 @@ namespace saige {
 namespace {
 
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;   // native vector: stays in VGPRs
 typedef __attribute__((ext_vector_type(16))) float f32x16;
 
 constexpr int BT = 128;  // tile edge
 
-__device__ __forceinline__ void tri_tile(int64_t id, int64_t& bi, int64_t& bj) {
-    double d = sqrt(8.0 * (double)id + 1.0);
-    int64_t r = (int64_t)((d - 1.0) * 0.5);
-    while (r * (r + 1) / 2 > id) r--;
-    while ((r + 1) * (r + 2) / 2 <= id) r++;
-    bi = r;
-    bj = id - r * (r + 1) / 2;
-}
 
 // Launch index -> triangle tile index, grouping consecutive tiles per XCD.
 __device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nblk) {
@@ -75,13 +73,12 @@ constexpr int BK16 = 64;            // markers per K chunk
 constexpr int P16 = BK16 + 8;       // LDS row pitch (bf16 elements): 144 B, conflict-free b128 reads
 
 __global__ __launch_bounds__(256) void k_syrk_bf16(const __hip_bfloat16* __restrict__ Zp, int64_t ldz,
-                                                   int64_t kdim, int64_t ntile, float* __restrict__ S,
-                                                   int64_t ldS, int accumulate) {
+                                                   int64_t kdim, const int2* __restrict__ tiles, int64_t nblk,
+                                                   float* __restrict__ S, int64_t ldS, int accumulate) {
     __shared__ __attribute__((aligned(16))) __hip_bfloat16 sA[BT * P16];
     __shared__ __attribute__((aligned(16))) __hip_bfloat16 sB[BT * P16];
-    const int64_t nblk = ntile * (ntile + 1) / 2;
-    int64_t bi, bj;
-    tri_tile(xcd_remap(blockIdx.x, nblk), bi, bj);
+    const int2 t = tiles[xcd_remap(blockIdx.x, nblk)];
+    const int64_t bi = t.x, bj = t.y;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wr = wv >> 1, wc = wv & 1;
 
@@ -89,12 +86,12 @@ __global__ __launch_bounds__(256) void k_syrk_bf16(const __hip_bfloat16* __restr
     const int lrow = tid >> 3, lk = (tid & 7) * 8;
     const __hip_bfloat16* gA = Zp + (bi * BT + lrow) * ldz + lk;
     const __hip_bfloat16* gB = Zp + (bj * BT + lrow) * ldz + lk;
-    uint4 ra[4], rb[4];
+    u32x4 ra[4], rb[4];
     const int64_t nk = kdim / BK16;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        ra[q] = *reinterpret_cast<const uint4*>(gA + (int64_t)(32 * q) * ldz);
-        rb[q] = *reinterpret_cast<const uint4*>(gB + (int64_t)(32 * q) * ldz);
+        ra[q] = *reinterpret_cast<const u32x4*>(gA + (int64_t)(32 * q) * ldz);
+        rb[q] = *reinterpret_cast<const u32x4*>(gB + (int64_t)(32 * q) * ldz);
     }
     f32x16 acc[2][2];
 #pragma unroll
@@ -108,16 +105,18 @@ __global__ __launch_bounds__(256) void k_syrk_bf16(const __hip_bfloat16* __restr
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            *reinterpret_cast<uint4*>(sA + (lrow + 32 * q) * P16 + lk) = ra[q];
-            *reinterpret_cast<uint4*>(sB + (lrow + 32 * q) * P16 + lk) = rb[q];
+            *reinterpret_cast<u32x4*>(sA + (lrow + 32 * q) * P16 + lk) = ra[q];
+            *reinterpret_cast<u32x4*>(sB + (lrow + 32 * q) * P16 + lk) = rb[q];
         }
         __syncthreads();
-        if (kc + 1 < nk) {
-            const int64_t off = (kc + 1) * BK16;
+        {
+            // prefetch the next chunk (the last iteration re-reads its own chunk:
+            // unconditional loads keep ra/rb in registers)
+            const int64_t off = ((kc + 1 < nk) ? kc + 1 : kc) * BK16;
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                ra[q] = *reinterpret_cast<const uint4*>(gA + (int64_t)(32 * q) * ldz + off);
-                rb[q] = *reinterpret_cast<const uint4*>(gB + (int64_t)(32 * q) * ldz + off);
+                ra[q] = *reinterpret_cast<const u32x4*>(gA + (int64_t)(32 * q) * ldz + off);
+                rb[q] = *reinterpret_cast<const u32x4*>(gB + (int64_t)(32 * q) * ldz + off);
             }
         }
 #pragma unroll
@@ -147,13 +146,12 @@ constexpr int P32 = BK32 + 4;       // LDS row pitch (floats)
 // k = 2s + l/32 sequence over s is contiguous (four b128 reads per chunk).
 
 __global__ __launch_bounds__(256) void k_syrk_f32(const float* __restrict__ Zp, int64_t ldz, int64_t kdim,
-                                                  int64_t ntile, float* __restrict__ S, int64_t ldS,
-                                                  int accumulate) {
+                                                  const int2* __restrict__ tiles, int64_t nblk,
+                                                  float* __restrict__ S, int64_t ldS, int accumulate) {
     __shared__ __attribute__((aligned(16))) float sA[BT * P32];
     __shared__ __attribute__((aligned(16))) float sB[BT * P32];
-    const int64_t nblk = ntile * (ntile + 1) / 2;
-    int64_t bi, bj;
-    tri_tile(xcd_remap(blockIdx.x, nblk), bi, bj);
+    const int2 t = tiles[xcd_remap(blockIdx.x, nblk)];
+    const int64_t bi = t.x, bj = t.y;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wr = wv >> 1, wc = wv & 1;
 
@@ -189,8 +187,8 @@ __global__ __launch_bounds__(256) void k_syrk_f32(const float* __restrict__ Zp, 
             *reinterpret_cast<float2*>(b + 16 + hk) = make_float2(rb[q].y, rb[q].w);
         }
         __syncthreads();
-        if (kc + 1 < nk) {
-            const int64_t off = (kc + 1) * BK32;
+        {
+            const int64_t off = ((kc + 1 < nk) ? kc + 1 : kc) * BK32;
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 ra[q] = *reinterpret_cast<const float4*>(gA + (int64_t)(32 * q) * ldz + off);
@@ -225,20 +223,36 @@ __global__ __launch_bounds__(256) void k_syrk_f32(const float* __restrict__ Zp, 
 int syrk_kchunk_f32() { return BK32; }
 int syrk_kchunk_bf16() { return BK16; }
 
+int syrk_tile() { return BT; }
+
+// Lower-triangle tiles (bi >= bj) of an nt x nt tile grid, grouped into
+// g x g super-blocks (super-block row >= column, each walked row by row).
+std::vector<int2> syrk_tile_order(int64_t nt, int64_t g) {
+    std::vector<int2> out;
+    out.reserve((size_t)(nt * (nt + 1) / 2));
+    const int64_t ns = (nt + g - 1) / g;
+    for (int64_t SI = 0; SI < ns; SI++)
+        for (int64_t SJ = 0; SJ <= SI; SJ++)
+            for (int64_t bi = SI * g; bi < std::min(nt, (SI + 1) * g); bi++)
+                for (int64_t bj = SJ * g; bj < std::min(bi + 1, (SJ + 1) * g); bj++)
+                    out.push_back(make_int2((int)bi, (int)bj));
+    return out;
+}
+
 hipError_t launch_syrk_bf16(hipStream_t s, const __hip_bfloat16* Zp, int64_t Npad, int64_t ldz,
-                            int64_t kdim, float* S, int accumulate) {
+                            int64_t kdim, const int2* tiles, float* S, int accumulate) {
     const int64_t nt = Npad / BT;
     const int64_t nblk = nt * (nt + 1) / 2;
-    hipLaunchKernelGGL(k_syrk_bf16, dim3((unsigned)nblk), dim3(256), 0, s, Zp, ldz, kdim, nt, S, Npad,
+    hipLaunchKernelGGL(k_syrk_bf16, dim3((unsigned)nblk), dim3(256), 0, s, Zp, ldz, kdim, tiles, nblk, S, Npad,
                        accumulate);
     return hipGetLastError();
 }
 
 hipError_t launch_syrk_f32(hipStream_t s, const float* Zp, int64_t Npad, int64_t ldz, int64_t kdim,
-                           float* S, int accumulate) {
+                           const int2* tiles, float* S, int accumulate) {
     const int64_t nt = Npad / BT;
     const int64_t nblk = nt * (nt + 1) / 2;
-    hipLaunchKernelGGL(k_syrk_f32, dim3((unsigned)nblk), dim3(256), 0, s, Zp, ldz, kdim, nt, S, Npad,
+    hipLaunchKernelGGL(k_syrk_f32, dim3((unsigned)nblk), dim3(256), 0, s, Zp, ldz, kdim, tiles, nblk, S, Npad,
                        accumulate);
     return hipGetLastError();
 }

@@ -91,32 +91,37 @@ __global__ __launch_bounds__(256) void k_gemv(const float* __restrict__ S, int64
 }
 
 // --------------------------------------------------------------- GEMM
-constexpr int GM = 128;       // rows per workgroup
-constexpr int GKC = 32;       // S columns per K chunk
-constexpr int GP = GKC + 4;   // LDS pitch (floats), k-permuted like the FP32 SYRK
+// Block product Y = S P for EW = 64 columns, FP32 MFMA 32x32x2.
+//
+// k_gemm_part: a workgroup owns 128 rows (4 waves x 32 rows) and one of
+// GSPLIT slices of the column range of S.  Each wave streams its rows of S
+// from HBM straight into MFMA operand registers (lane l holds row l%32 and
+// the 16 consecutive columns starting at 16*(l/32) of each 32-column chunk;
+// the next chunk is prefetched while the current one is multiplied).  The
+// matching 128 x 64 superchunk of P is staged in LDS as [n][k] so each lane
+// reads its 16 B-operand values with four b128 loads.  S is read from HBM
+// exactly once; P (25.6 MB at N = 100k) is re-read per row block from L2/MALL.
+// The raw sums go to Ypart[slice][row][n].
+//
+// k_gemm_epi: Y = sum over slices in fixed order, then the Sigma epilogue
+// and the per-column FP64 partial dot products.
+constexpr int GM = 128;                 // rows per workgroup
+constexpr int GKS = 128;                // columns of S per P superchunk
+constexpr int GKP = GKS + 4;            // LDS pitch of sP (floats)
+constexpr int GSPLIT = 4;               // column slices
 
-__global__ __launch_bounds__(256) void k_gemm(const float* __restrict__ S, int64_t ld, int64_t Nnm,
-                                              const float* __restrict__ in, float* __restrict__ out,
-                                              SigmaArgs a, double* __restrict__ partial) {
-    __shared__ __attribute__((aligned(16))) float sS[GM * GP];
-    __shared__ __attribute__((aligned(16))) float sP[EW * GP];
-    __shared__ double s_dot[4][EW];
+__global__ __launch_bounds__(256) void k_gemm_part(const float* __restrict__ S, int64_t ld,
+                                                   const float* __restrict__ in, float* __restrict__ ypart) {
+    __shared__ __attribute__((aligned(16))) float sP[EW * GKP];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int64_t r0 = (int64_t)blockIdx.x * GM;
-
-    // S chunk: 16-byte chunk c = tid + 256 q -> row c/8 (+32q), k (c%8)*4
-    const int lrow = tid >> 3, lk = (tid & 7) * 4, hk = lk >> 1;
-    const float* gS = S + (r0 + lrow) * ld + lk;
-    // P chunk: [32 j][64 n]; chunk c = tid + 256 q -> j c/16, n (c%16)*4
-    const int pj0 = tid >> 4, pn = (tid & 15) * 4;
-    const float* gP = in + (int64_t)pj0 * EW + pn;
-
-    float4 rs[4], rp[2];
-    const int64_t nk = ld / GKC;
-#pragma unroll
-    for (int q = 0; q < 4; q++) rs[q] = *reinterpret_cast<const float4*>(gS + (int64_t)(32 * q) * ld);
-#pragma unroll
-    for (int q = 0; q < 2; q++) rp[q] = *reinterpret_cast<const float4*>(gP + (int64_t)(16 * q) * EW);
+    const int split = blockIdx.y;
+    const int64_t nsc = ld / GKS;                           // superchunks in a row
+    const int64_t sc0 = nsc * split / GSPLIT, sc1 = nsc * (split + 1) / GSPLIT;
+    const int hl = lane >> 5;                               // k half of this lane
+    const float* srow = S + (r0 + wv * 32 + (lane & 31)) * ld + hl * 16;
+    // P superchunk staging: thread -> column n, 32 consecutive k
+    const int pn = tid & 63, pq = tid >> 6;
 
     f32x16 acc[2];
 #pragma unroll
@@ -124,90 +129,117 @@ __global__ __launch_bounds__(256) void k_gemm(const float* __restrict__ S, int64
 #pragma unroll
         for (int r = 0; r < 16; r++) acc[ni][r] = 0.0f;
 
-    const int h = (lane >> 5) * 16;
-    for (int64_t kc = 0; kc < nk; kc++) {
+    float4 sa[4], sb[4];
+    float pr[32];
+    if (sc0 < sc1) {
+        const int64_t k0 = sc0 * GKS;
+#pragma unroll
+        for (int q = 0; q < 4; q++) sa[q] = *reinterpret_cast<const float4*>(srow + k0 + 4 * q);
+#pragma unroll
+        for (int q = 0; q < 32; q++) pr[q] = in[(k0 + pq * 32 + q) * EW + pn];
+    }
+    for (int64_t sc = sc0; sc < sc1; sc++) {
+        const int64_t k0 = sc * GKS;
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            float* d = sS + (lrow + 32 * q) * GP;
-            *reinterpret_cast<float2*>(d + hk) = make_float2(rs[q].x, rs[q].z);
-            *reinterpret_cast<float2*>(d + 16 + hk) = make_float2(rs[q].y, rs[q].w);
-        }
-#pragma unroll
-        for (int q = 0; q < 2; q++) {
-            const int j = pj0 + 16 * q;              // k index within the chunk
-            const int off = (j & 1) * 16 + (j >> 1);
-            sP[(pn + 0) * GP + off] = rp[q].x;
-            sP[(pn + 1) * GP + off] = rp[q].y;
-            sP[(pn + 2) * GP + off] = rp[q].z;
-            sP[(pn + 3) * GP + off] = rp[q].w;
-        }
+        for (int q = 0; q < 8; q++)
+            *reinterpret_cast<float4*>(sP + pn * GKP + pq * 32 + 4 * q) =
+                make_float4(pr[4 * q], pr[4 * q + 1], pr[4 * q + 2], pr[4 * q + 3]);
         __syncthreads();
-        if (kc + 1 < nk) {
-            const int64_t off = (kc + 1) * GKC;
+        if (sc + 1 < sc1) {
+            const int64_t kn = k0 + GKS;
 #pragma unroll
-            for (int q = 0; q < 4; q++)
-                rs[q] = *reinterpret_cast<const float4*>(gS + (int64_t)(32 * q) * ld + off);
-#pragma unroll
-            for (int q = 0; q < 2; q++)
-                rp[q] = *reinterpret_cast<const float4*>(gP + (off + 16 * q) * EW);
+            for (int q = 0; q < 32; q++) pr[q] = in[(kn + pq * 32 + q) * EW + pn];
         }
 #pragma unroll
-        for (int sq = 0; sq < 4; sq++) {
-            const float4 fa = *reinterpret_cast<const float4*>(sS + (wv * 32 + (lane & 31)) * GP + h + 4 * sq);
-            float4 fb[2];
+        for (int c = 0; c < GKS / 32; c++) {
+            // prefetch the next 32-column chunk of this wave's S rows
+            const int64_t kc_next = (c + 1 < GKS / 32) ? k0 + (c + 1) * 32 : ((sc + 1 < sc1) ? k0 + GKS : -1);
+            if (kc_next >= 0) {
 #pragma unroll
-            for (int ni = 0; ni < 2; ni++)
-                fb[ni] = *reinterpret_cast<const float4*>(sP + (ni * 32 + (lane & 31)) * GP + h + 4 * sq);
+                for (int q = 0; q < 4; q++) sb[q] = *reinterpret_cast<const float4*>(srow + kc_next + 4 * q);
+            }
+            const float* pb = sP + c * 32 + hl * 16;
 #pragma unroll
-            for (int e = 0; e < 4; e++)
+            for (int q = 0; q < 4; q++) {
+                const float4 b0 = *reinterpret_cast<const float4*>(pb + (lane & 31) * GKP + 4 * q);
+                const float4 b1 = *reinterpret_cast<const float4*>(pb + (32 + (lane & 31)) * GKP + 4 * q);
+                acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[q].x, b0.x, acc[0], 0, 0, 0);
+                acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[q].x, b1.x, acc[1], 0, 0, 0);
+                acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[q].y, b0.y, acc[0], 0, 0, 0);
+                acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[q].y, b1.y, acc[1], 0, 0, 0);
+                acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[q].z, b0.z, acc[0], 0, 0, 0);
+                acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[q].z, b1.z, acc[1], 0, 0, 0);
+                acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[q].w, b0.w, acc[0], 0, 0, 0);
+                acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[q].w, b1.w, acc[1], 0, 0, 0);
+            }
 #pragma unroll
-                for (int ni = 0; ni < 2; ni++)
-                    acc[ni] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[e], fb[ni][e], acc[ni], 0, 0, 0);
+            for (int q = 0; q < 4; q++) sa[q] = sb[q];
         }
     }
-
-    // epilogue: lane holds column n = ni*32 + lane%32 and rows
-    // 8*(r/4) + 4*(lane/32) + r%4 of the wave's 32-row block
+    // raw sums: lane holds column ni*32 + lane%32, rows 8*(r/4) + 4*(lane/32) + r%4
+    float* yp = ypart + (int64_t)split * (gridDim.x * (int64_t)GM) * EW;
 #pragma unroll
     for (int ni = 0; ni < 2; ni++) {
         const int n = ni * 32 + (lane & 31);
-        double dsum = 0.0;
-        int s = 0;
-        float tau0 = 0.f, tau1 = 0.f;
-        bool col_on = false;
-        if (a.mode == 1) {
-            s = a.sys[n];
-            tau0 = a.tau[2 * s];
-            tau1 = a.tau[2 * s + 1];
-            col_on = a.live[n] != 0;
-        }
 #pragma unroll
         for (int r = 0; r < 16; r++) {
             const int64_t row = r0 + wv * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            const float y = acc[ni][r];
-            float o;
-            if (a.mode == 0) {
-                o = div_rn(y, a.mqc);
-            } else {
-                const bool on = col_on && row < Nnm && a.mask[(int64_t)s * ld + row];
-                const float p = in[row * EW + n];
-                o = on ? sigma_elem(p, y, a.winv[(int64_t)s * ld + row], tau0, tau1, a.mqc) : 0.0f;
-                dsum += (double)p * (double)o;
-            }
-            out[row * EW + n] = o;
-        }
-        if (a.mode == 1) {
-            dsum += __shfl_xor(dsum, 32, 64);
-            if (lane < 32) s_dot[wv][n] = dsum;
+            yp[row * EW + n] = acc[ni][r];
         }
     }
+}
+
+constexpr int EROWS = 256;   // rows per workgroup of the epilogue
+
+__global__ __launch_bounds__(256) void k_gemm_epi(const float* __restrict__ ypart, int64_t Npad, int64_t Nnm,
+                                                  const float* __restrict__ in, float* __restrict__ out,
+                                                  SigmaArgs a, double* __restrict__ partial) {
+    __shared__ double sm[256];
+    const int tid = threadIdx.x;
+    const int n = tid & 63;
+    int s = 0;
+    float tau0 = 0.f, tau1 = 0.f;
+    bool col_on = false, product = false;
     if (a.mode == 1) {
-        __syncthreads();
-        if (tid < EW) {
-            double t = (s_dot[0][tid] + s_dot[1][tid]) + (s_dot[2][tid] + s_dot[3][tid]);
-            partial[((int64_t)blockIdx.x * EW + tid) * 2] = t;
+        s = a.sys[n];
+        tau0 = a.tau[2 * s];
+        tau1 = a.tau[2 * s + 1];
+        col_on = a.live[n] != 0;
+        product = a.kind != nullptr && a.kind[n] != 0;
+    }
+    double dsum = 0.0;
+    const int64_t rb = (int64_t)blockIdx.x * EROWS;
+    for (int rl = tid >> 6; rl < EROWS; rl += 4) {
+        const int64_t row = rb + rl;
+        const int64_t o = row * EW + n;
+        float y = ypart[o];
+#pragma unroll
+        for (int sp = 1; sp < GSPLIT; sp++) y += ypart[(int64_t)sp * Npad * EW + o];
+        float r;
+        if (a.mode == 0) {
+            r = div_rn(y, a.mqc);
+        } else if (product) {
+            // a product slot: K p on the system's samples, stored as the result
+            const bool on = col_on && row < Nnm && a.mask[(int64_t)s * Npad + row];
+            r = on ? div_rn(y, a.mqc) : 0.0f;
+            a.xout[o] = r;
+        } else {
+            const bool on = col_on && row < Nnm && a.mask[(int64_t)s * Npad + row];
+            const float p = in[o];
+            r = on ? sigma_elem(p, y, a.winv[(int64_t)s * Npad + row], tau0, tau1, a.mqc) : 0.0f;
+            dsum += (double)p * (double)r;
         }
+        out[o] = r;
+    }
+    if (a.mode == 1) {
+        sm[tid] = dsum;
+        for (int st = 128; st >= EW; st >>= 1) {
+            __syncthreads();
+            if (tid < st) sm[tid] += sm[tid + st];
+        }
+        __syncthreads();
+        if (tid < EW) partial[((int64_t)blockIdx.x * EW + tid) * 2] = sm[tid];
     }
 }
 
@@ -326,17 +358,20 @@ __global__ __launch_bounds__(256) void k_col_init_finish(int slot, const double*
 __global__ __launch_bounds__(256) void k_alpha(int kpad, const double* __restrict__ partial,
                                                int64_t nparts, const double* __restrict__ rz,
                                                const uint8_t* __restrict__ done,
-                                               const uint8_t* __restrict__ live, float* alpha) {
+                                               const uint8_t* __restrict__ live,
+                                               const uint8_t* __restrict__ kind, float* alpha) {
     __shared__ double sm[256];
     const int c = blockIdx.x;
     const double pap = sum_parts(partial, nparts, kpad, c, 0, sm);
-    if (threadIdx.x == 0) alpha[c] = (live[c] && !done[c]) ? (float)(rz[c] / pap) : 0.0f;
+    if (threadIdx.x == 0)
+        alpha[c] = (live[c] && !done[c] && !kind[c]) ? (float)(rz[c] / pap) : 0.0f;
 }
 
 // x += a p; r1 = r - a Ap; z1 = minv r1 (FG.cpp:2734-2767), partial z1.r1, r1.r1
 __global__ __launch_bounds__(256) void k_update(int kpad, int64_t Npad, const float* __restrict__ alpha,
                                                 const uint8_t* __restrict__ done,
                                                 const uint8_t* __restrict__ live,
+                                                const uint8_t* __restrict__ kind,
                                                 const float* __restrict__ minv,
                                                 const int32_t* __restrict__ sys, float* X, float* R,
                                                 float* Z, const float* __restrict__ P,
@@ -346,7 +381,7 @@ __global__ __launch_bounds__(256) void k_update(int kpad, int64_t Npad, const fl
     const int tid = threadIdx.x;
     const int c = tid % kpad;
     const int rstep = 256 / kpad;
-    const bool act = live[c] && !done[c];
+    const bool act = live[c] && !done[c] && !kind[c];
     const float a = alpha[c];
     const float* mv = minv + (int64_t)sys[c] * Npad;
     double rz = 0.0, rr = 0.0;
@@ -387,13 +422,18 @@ __global__ __launch_bounds__(256) void k_beta(int kpad, const double* __restrict
                                               const int32_t* __restrict__ maxiter,
                                               const double* __restrict__ tol, int32_t* iter,
                                               uint8_t* done, const uint8_t* __restrict__ live,
-                                              float* beta) {
+                                              const uint8_t* __restrict__ kind, float* beta) {
     __shared__ double sm[256];
     const int c = blockIdx.x;
     const double rzn = sum_parts(partial, nparts, kpad, c, 0, sm);
     const double rr = sum_parts(partial, nparts, kpad, c, 1, sm);
     if (threadIdx.x == 0) {
-        if (live[c] && !done[c]) {
+        if (live[c] && !done[c] && kind[c]) {
+            // a product slot finishes with the product it rode along in
+            beta[c] = 0.0f;
+            iter[c] = 1;
+            done[c] = 1;
+        } else if (live[c] && !done[c]) {
             beta[c] = (float)(rzn / rz[c]);
             rz[c] = rzn;
             sumr2[c] = rr;
@@ -410,12 +450,75 @@ __global__ __launch_bounds__(256) void k_beta(int kpad, const double* __restrict
 __global__ __launch_bounds__(256) void k_pupdate(int kpad, int64_t Npad, const float* __restrict__ beta,
                                                  const uint8_t* __restrict__ done,
                                                  const uint8_t* __restrict__ live,
+                                                 const uint8_t* __restrict__ kind,
                                                  const float* __restrict__ Z, float* P) {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= Npad * kpad) return;
     const int c = (int)(t % kpad);
-    if (!live[c] || done[c]) return;
+    if (!live[c] || done[c] || kind[c]) return;
     P[t] = __fadd_rn(Z[t], __fmul_rn(beta[c], P[t]));
+}
+
+// Start the listed columns together: column j of the staged block B
+// ([k][Nnm]) goes to slot slots[j] with system sys[j] (FG.cpp:2615-2690).
+// product != 0 starts GRM-product slots instead: P = b on the system's
+// samples, and the slot is done after the next GEMM pass.
+__global__ __launch_bounds__(256) void k_cols_init(const float* __restrict__ B, const int32_t* __restrict__ slots,
+                                                   const int32_t* __restrict__ sys, int kpad, int64_t Nnm,
+                                                   int64_t Npad, const float* __restrict__ minv,
+                                                   const uint8_t* __restrict__ mask, float* X, float* R, float* Z,
+                                                   float* P, double* __restrict__ partial, int product) {
+    __shared__ double sm[2][256];
+    const int tid = threadIdx.x;
+    const int j = blockIdx.y;
+    const int slot = slots[j];
+    const int64_t so = (int64_t)sys[j] * Npad;
+    const int64_t i = (int64_t)blockIdx.x * VROWS + tid;
+    const bool on = i < Nnm && mask[so + i];
+    const float r = on ? B[(int64_t)j * Nnm + i] : 0.0f;
+    const float z = __fmul_rn(minv[so + i], r);
+    const int64_t o = i * kpad + slot;
+    X[o] = 0.0f;
+    R[o] = r;
+    Z[o] = z;
+    P[o] = product ? r : z;
+    sm[0][tid] = (double)z * (double)r;
+    sm[1][tid] = (double)r * (double)r;
+    for (int s = 128; s >= 1; s >>= 1) {
+        __syncthreads();
+        if (tid < s) {
+            sm[0][tid] += sm[0][tid + s];
+            sm[1][tid] += sm[1][tid + s];
+        }
+    }
+    if (tid == 0) {
+        partial[((int64_t)blockIdx.x * kpad + slot) * 2] = sm[0][0];
+        partial[((int64_t)blockIdx.x * kpad + slot) * 2 + 1] = sm[1][0];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cols_init_finish(const int32_t* __restrict__ slots, int kpad,
+                                                          const double* __restrict__ partial, int64_t nparts,
+                                                          double* rz, double* sumr2, const int32_t* maxiter,
+                                                          const double* tol, int32_t* iter, uint8_t* done,
+                                                          int product) {
+    __shared__ double sm[256];
+    const int slot = slots[blockIdx.x];
+    const double a = sum_parts(partial, nparts, kpad, slot, 0, sm);
+    const double b = sum_parts(partial, nparts, kpad, slot, 1, sm);
+    if (threadIdx.x == 0) {
+        rz[slot] = a;
+        sumr2[slot] = b;
+        iter[slot] = 0;
+        done[slot] = product ? 0 : ((b > tol[slot] && 0 < maxiter[slot]) ? 0 : 1);   // FG.cpp:2696
+    }
+}
+
+__global__ void k_cols_gather(const float* __restrict__ src, const int32_t* __restrict__ slots, int kpad,
+                              int64_t Nnm, float* __restrict__ dst) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i < Nnm) dst[(int64_t)j * Nnm + i] = src[i * kpad + slots[j]];
 }
 
 __global__ void k_gather_col(const float* __restrict__ src, int slot, int kpad, int64_t Nnm,
@@ -452,7 +555,7 @@ inline unsigned nblk(int64_t n, int64_t b) { return (unsigned)((n + b - 1) / b);
 }  // namespace
 
 int64_t gemv_max_parts(int64_t Npad) { return Npad / GEMV_ROWS; }
-int64_t gemm_max_parts(int64_t Npad) { return Npad / GM; }
+int64_t gemm_max_parts(int64_t Npad) { return Npad / EROWS; }
 int64_t vec_max_parts(int64_t Npad, int) { return Npad / VROWS; }
 
 hipError_t launch_grm_gemv(hipStream_t s, const float* S, int64_t Npad, int64_t Nnm, const float* in,
@@ -464,12 +567,17 @@ hipError_t launch_grm_gemv(hipStream_t s, const float* S, int64_t Npad, int64_t 
 }
 
 hipError_t launch_grm_gemm(hipStream_t s, const float* S, int64_t Npad, int64_t Nnm, const float* in,
-                           float* out, SigmaArgs a, double* partial, int64_t* nparts) {
-    const int64_t nb = Npad / GM;
-    hipLaunchKernelGGL(k_gemm, dim3((unsigned)nb), dim3(256), 0, s, S, Npad, Nnm, in, out, a, partial);
+                           float* out, SigmaArgs a, double* partial, int64_t* nparts, float* ypart) {
+    hipLaunchKernelGGL(k_gemm_part, dim3((unsigned)(Npad / GM), GSPLIT), dim3(256), 0, s, S, Npad, in, ypart);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int64_t nb = Npad / EROWS;
+    hipLaunchKernelGGL(k_gemm_epi, dim3((unsigned)nb), dim3(256), 0, s, ypart, Npad, Nnm, in, out, a, partial);
     if (nparts) *nparts = nb;
     return hipGetLastError();
 }
+
+int64_t gemm_ypart_floats(int64_t Npad) { return (int64_t)GSPLIT * Npad * EW; }
 
 hipError_t launch_sigma_setup(hipStream_t s, const float* w, const uint8_t* mask_in, const float* diag,
                               int64_t Nnm, int64_t Npad, float tau0, float tau1, float mqc, int diag_one,
@@ -499,19 +607,19 @@ hipError_t launch_col_init_finish(hipStream_t s, int slot, int kpad, const doubl
 }
 
 hipError_t launch_alpha(hipStream_t s, int kpad, const double* partial, int64_t nparts, const double* rz,
-                        const uint8_t* done, const uint8_t* live, float* alpha) {
+                        const uint8_t* done, const uint8_t* live, const uint8_t* kind, float* alpha) {
     hipLaunchKernelGGL(k_alpha, dim3((unsigned)kpad), dim3(256), 0, s, kpad, partial, nparts, rz, done,
-                       live, alpha);
+                       live, kind, alpha);
     return hipGetLastError();
 }
 
 hipError_t launch_update(hipStream_t s, int kpad, int64_t Nnm, int64_t Npad, const float* alpha,
-                         const uint8_t* done, const uint8_t* live, const float* minv, const int32_t* sys,
-                         float* X, float* R, float* Z, const float* P, const float* AP, double* partial,
-                         int64_t* nparts) {
+                         const uint8_t* done, const uint8_t* live, const uint8_t* kind, const float* minv,
+                         const int32_t* sys, float* X, float* R, float* Z, const float* P, const float* AP,
+                         double* partial, int64_t* nparts) {
     (void)Nnm;
     const int64_t nb = Npad / VROWS;
-    hipLaunchKernelGGL(k_update, dim3((unsigned)nb), dim3(256), 0, s, kpad, Npad, alpha, done, live, minv,
+    hipLaunchKernelGGL(k_update, dim3((unsigned)nb), dim3(256), 0, s, kpad, Npad, alpha, done, live, kind, minv,
                        sys, X, R, Z, P, AP, partial);
     if (nparts) *nparts = nb;
     return hipGetLastError();
@@ -519,17 +627,41 @@ hipError_t launch_update(hipStream_t s, int kpad, int64_t Nnm, int64_t Npad, con
 
 hipError_t launch_beta(hipStream_t s, int kpad, const double* partial, int64_t nparts, double* rz,
                        double* sumr2, const int32_t* maxiter, const double* tol, int32_t* iter,
-                       uint8_t* done, const uint8_t* live, float* beta) {
+                       uint8_t* done, const uint8_t* live, const uint8_t* kind, float* beta) {
     hipLaunchKernelGGL(k_beta, dim3((unsigned)kpad), dim3(256), 0, s, kpad, partial, nparts, rz, sumr2,
-                       maxiter, tol, iter, done, live, beta);
+                       maxiter, tol, iter, done, live, kind, beta);
     return hipGetLastError();
 }
 
 hipError_t launch_pupdate(hipStream_t s, int kpad, int64_t Nnm, int64_t Npad, const float* beta,
-                          const uint8_t* done, const uint8_t* live, const float* Z, float* P) {
+                          const uint8_t* done, const uint8_t* live, const uint8_t* kind, const float* Z, float* P) {
     (void)Nnm;
     hipLaunchKernelGGL(k_pupdate, dim3(nblk(Npad * kpad, 256)), dim3(256), 0, s, kpad, Npad, beta, done,
-                       live, Z, P);
+                       live, kind, Z, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_cols_init(hipStream_t s, const float* B, int k, const int32_t* slots, const int32_t* sys,
+                            int kpad, int64_t Nnm, int64_t Npad, const float* minv, const uint8_t* mask, float* X,
+                            float* R, float* Z, float* P, double* partial, int64_t* nparts, int product) {
+    const int64_t nb = Npad / VROWS;
+    hipLaunchKernelGGL(k_cols_init, dim3((unsigned)nb, (unsigned)k), dim3(256), 0, s, B, slots, sys, kpad, Nnm, Npad,
+                       minv, mask, X, R, Z, P, partial, product);
+    if (nparts) *nparts = nb;
+    return hipGetLastError();
+}
+
+hipError_t launch_cols_init_finish(hipStream_t s, int k, const int32_t* slots, int kpad, const double* partial,
+                                   int64_t nparts, double* rz, double* sumr2, const int32_t* maxiter,
+                                   const double* tol, int32_t* iter, uint8_t* done, int product) {
+    hipLaunchKernelGGL(k_cols_init_finish, dim3((unsigned)k), dim3(256), 0, s, slots, kpad, partial, nparts, rz,
+                       sumr2, maxiter, tol, iter, done, product);
+    return hipGetLastError();
+}
+
+hipError_t launch_cols_gather(hipStream_t s, const float* src, int k, const int32_t* slots, int kpad, int64_t Nnm,
+                              float* dst) {
+    hipLaunchKernelGGL(k_cols_gather, dim3(nblk(Nnm, 256), (unsigned)k), dim3(256), 0, s, src, slots, kpad, Nnm, dst);
     return hipGetLastError();
 }
 

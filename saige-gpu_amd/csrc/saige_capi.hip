@@ -80,8 +80,11 @@ void free_engine(Engine& e) {
     dfree(e.X); dfree(e.R); dfree(e.Z); dfree(e.P); dfree(e.AP);
     dfree(e.winv); dfree(e.minv); dfree(e.mask); dfree(e.tau);
     dfree(e.slot_sys); dfree(e.rz); dfree(e.sumr2); dfree(e.pap); dfree(e.alpha); dfree(e.beta);
-    dfree(e.iter); dfree(e.maxiter); dfree(e.tol); dfree(e.live); dfree(e.done);
-    dfree(e.partial); dfree(e.stage);
+    dfree(e.iter); dfree(e.maxiter); dfree(e.tol); dfree(e.live); dfree(e.done); dfree(e.kind);
+    e.h_kind.clear();
+    dfree(e.partial); dfree(e.stage); dfree(e.stageB); dfree(e.d_list); dfree(e.ypart);
+    e.h_maxiter.clear();
+    e.h_tol.clear();
     if (e.h_done) (void)hipHostFree(e.h_done);
     if (e.h_iter) (void)hipHostFree(e.h_iter);
     e.h_done = nullptr;
@@ -94,6 +97,7 @@ void free_engine(Engine& e) {
 void free_geno(saige_ctx* c) {
     dfree(c->d_stats); dfree(c->d_packed); dfree(c->d_packed_vr); dfree(c->d_lut); dfree(c->d_lut_vr);
     dfree(c->d_diag); dfree(c->d_grm); dfree(c->d_vec);
+    dfree(c->d_mv_in); dfree(c->d_mv_blk); dfree(c->d_mv_out); dfree(c->d_mv_ypart);
     free_engine(c->eng);
     free_engine(c->eng1);
     c->h_stats.clear();
@@ -142,7 +146,11 @@ int engine_reserve(saige_ctx* c, Engine& e, int kpad, int nsys) {
     SAIGE_HIP_TRY(dalloc(&e.tol, kpad));
     SAIGE_HIP_TRY(dalloc(&e.live, kpad));
     SAIGE_HIP_TRY(dalloc(&e.done, kpad));
+    SAIGE_HIP_TRY(dalloc(&e.kind, kpad));
     SAIGE_HIP_TRY(dalloc(&e.stage, Np));
+    SAIGE_HIP_TRY(dalloc(&e.stageB, Np * kpad));
+    SAIGE_HIP_TRY(dalloc(&e.d_list, 2 * kpad));
+    if (kpad == EW) SAIGE_HIP_TRY(dalloc(&e.ypart, gemm_ypart_floats(Np)));
     int64_t np = std::max<int64_t>(gemv_max_parts(Np), gemm_max_parts(Np) * kpad);
     np = std::max<int64_t>(np, vec_max_parts(Np, kpad) * kpad);
     e.nparts = np;
@@ -154,6 +162,9 @@ int engine_reserve(saige_ctx* c, Engine& e, int kpad, int nsys) {
     e.nsys = nsys;
     e.h_slot_sys.assign(kpad, 0);
     e.h_live.assign(kpad, 0);
+    e.h_maxiter.assign(kpad, 0);
+    e.h_kind.assign(kpad, 0);
+    e.h_tol.assign(kpad, 0.0);
     // zero all column blocks so padded rows and idle columns read as zero
     hipStream_t s = c->stream;
     SAIGE_HIP_TRY(launch_fill_zero(s, e.X, Np * kpad * 4));
@@ -169,7 +180,10 @@ int engine_reserve(saige_ctx* c, Engine& e, int kpad, int nsys) {
     std::vector<int32_t> zi(kpad, 0);
     SAIGE_HIP_TRY(hipMemcpyAsync(e.live, zeros.data(), kpad, hipMemcpyHostToDevice, s));
     SAIGE_HIP_TRY(hipMemcpyAsync(e.done, zeros.data(), kpad, hipMemcpyHostToDevice, s));
+    SAIGE_HIP_TRY(hipMemcpyAsync(e.kind, zeros.data(), kpad, hipMemcpyHostToDevice, s));
     SAIGE_HIP_TRY(hipMemcpyAsync(e.slot_sys, zi.data(), 4 * kpad, hipMemcpyHostToDevice, s));
+    SAIGE_HIP_TRY(hipMemcpyAsync(e.maxiter, zi.data(), 4 * kpad, hipMemcpyHostToDevice, s));
+    SAIGE_HIP_TRY(hipMemcpyAsync(e.tol, e.h_tol.data(), 8 * kpad, hipMemcpyHostToDevice, s));
     SAIGE_HIP_TRY(hipStreamSynchronize(s));
     return SAIGE_OK;
 }
@@ -209,42 +223,65 @@ int engine_sync_host_arrays(saige_ctx* c, Engine& e, const std::vector<int32_t>*
     return SAIGE_OK;
 }
 
-int engine_add(saige_ctx* c, Engine& e, int slot, int s, const float* b_host, int maxiter, float tol) {
-    if (slot < 0 || slot >= e.kpad) {
-        set_error("slot %d out of range [0, %d)", slot, e.kpad);
-        return SAIGE_E_ARG;
-    }
-    if (s < 0 || s >= e.nsys) {
-        set_error("system %d out of range [0, %d)", s, e.nsys);
-        return SAIGE_E_ARG;
-    }
-    if (e.h_live[slot]) {
-        set_error("slot %d is busy", slot);
+// Start k solves at once: column j of B_host (N x k, column-major) in slot
+// slots[j] with system systems[j].  One upload, two launches, one sync.
+int engine_add_batch(saige_ctx* c, Engine& e, int k, const int32_t* slots, const int32_t* systems,
+                     const float* B_host, int maxiter, float tol, int product = 0) {
+    if (product && e.kpad != EW) {
+        set_error("GRM-product slots need the block engine");
         return SAIGE_E_STATE;
+    }
+    if (k < 1 || k > e.kpad || !slots || !systems || !B_host) {
+        set_error("invalid batch of %d columns (engine width %d)", k, e.kpad);
+        return SAIGE_E_ARG;
+    }
+    std::vector<uint8_t> seen(e.kpad, 0);
+    for (int j = 0; j < k; j++) {
+        const int slot = slots[j], sy = systems[j];
+        if (slot < 0 || slot >= e.kpad) {
+            set_error("slot %d out of range [0, %d)", slot, e.kpad);
+            return SAIGE_E_ARG;
+        }
+        if (sy < 0 || sy >= e.nsys) {
+            set_error("system %d out of range [0, %d)", sy, e.nsys);
+            return SAIGE_E_ARG;
+        }
+        if (e.h_live[slot] || seen[slot]) {
+            set_error("slot %d is busy", slot);
+            return SAIGE_E_STATE;
+        }
+        seen[slot] = 1;
+    }
+    for (int j = 0; j < k; j++) {
+        e.h_slot_sys[slots[j]] = systems[j];
+        e.h_live[slots[j]] = 1;
+        e.h_maxiter[slots[j]] = maxiter;
+        e.h_tol[slots[j]] = (double)tol;
+        e.h_kind[slots[j]] = product ? 1 : 0;
     }
     hipStream_t st = c->stream;
     const int64_t Np = c->Npad, N = c->Nnm;
-    // per-slot parameters: read back the current device arrays so other
-    // slots keep theirs, then patch this slot
-    std::vector<int32_t> mi(e.kpad);
-    std::vector<double> tl(e.kpad);
-    SAIGE_HIP_TRY(hipMemcpyAsync(mi.data(), e.maxiter, 4 * e.kpad, hipMemcpyDeviceToHost, st));
-    SAIGE_HIP_TRY(hipMemcpyAsync(tl.data(), e.tol, 8 * e.kpad, hipMemcpyDeviceToHost, st));
-    SAIGE_HIP_TRY(hipStreamSynchronize(st));
-    mi[slot] = maxiter;
-    tl[slot] = (double)tol;
-    e.h_slot_sys[slot] = s;
-    e.h_live[slot] = 1;
-    int rc = engine_sync_host_arrays(c, e, &mi, &tl);
-    if (rc) return rc;
-    SAIGE_HIP_TRY(hipMemcpyAsync(e.stage, b_host, sizeof(float) * N, hipMemcpyHostToDevice, st));
+    std::vector<int32_t> lists(2 * e.kpad, 0);
+    std::copy(slots, slots + k, lists.begin());
+    std::copy(systems, systems + k, lists.begin() + e.kpad);
+    SAIGE_HIP_TRY(hipMemcpyAsync(e.d_list, lists.data(), 8 * e.kpad, hipMemcpyHostToDevice, st));
+    SAIGE_HIP_TRY(hipMemcpyAsync(e.live, e.h_live.data(), e.kpad, hipMemcpyHostToDevice, st));
+    SAIGE_HIP_TRY(hipMemcpyAsync(e.slot_sys, e.h_slot_sys.data(), 4 * e.kpad, hipMemcpyHostToDevice, st));
+    SAIGE_HIP_TRY(hipMemcpyAsync(e.maxiter, e.h_maxiter.data(), 4 * e.kpad, hipMemcpyHostToDevice, st));
+    SAIGE_HIP_TRY(hipMemcpyAsync(e.tol, e.h_tol.data(), 8 * e.kpad, hipMemcpyHostToDevice, st));
+    SAIGE_HIP_TRY(hipMemcpyAsync(e.kind, e.h_kind.data(), e.kpad, hipMemcpyHostToDevice, st));
+    SAIGE_HIP_TRY(hipMemcpyAsync(e.stageB, B_host, sizeof(float) * N * k, hipMemcpyHostToDevice, st));
     int64_t np = 0;
-    SAIGE_HIP_TRY(launch_col_init(st, e.stage, slot, e.kpad, N, Np, e.minv + s * Np, e.mask + s * Np, e.X,
-                                  e.R, e.Z, e.P, e.partial, &np));
-    SAIGE_HIP_TRY(launch_col_init_finish(st, slot, e.kpad, e.partial, np, e.rz, e.sumr2, e.maxiter, e.tol,
-                                         e.iter, e.done, e.live));
+    SAIGE_HIP_TRY(launch_cols_init(st, e.stageB, k, e.d_list, e.d_list + e.kpad, e.kpad, N, Np, e.minv, e.mask, e.X,
+                                   e.R, e.Z, e.P, e.partial, &np, product));
+    SAIGE_HIP_TRY(launch_cols_init_finish(st, k, e.d_list, e.kpad, e.partial, np, e.rz, e.sumr2, e.maxiter, e.tol,
+                                          e.iter, e.done, product));
     SAIGE_HIP_TRY(hipStreamSynchronize(st));
     return SAIGE_OK;
+}
+
+int engine_add(saige_ctx* c, Engine& e, int slot, int s, const float* b_host, int maxiter, float tol) {
+    return engine_add_batch(c, e, 1, &slot, &s, b_host, maxiter, tol);
 }
 
 int engine_step(saige_ctx* c, Engine& e, int n_iter, uint8_t* done_out, int32_t* iters_out) {
@@ -259,6 +296,8 @@ int engine_step(saige_ctx* c, Engine& e, int n_iter, uint8_t* done_out, int32_t*
     a.inv_m = 0.0f;
     a.mqc = (float)c->Mqc;
     a.mode = 1;
+    a.kind = e.kind;
+    a.xout = e.X;
     SAIGE_HIP_TRY(hipMemcpyAsync(e.h_done, e.done, e.kpad, hipMemcpyDeviceToHost, st));
     SAIGE_HIP_TRY(hipStreamSynchronize(st));
     for (int it = 0; it < n_iter; it++) {
@@ -270,15 +309,15 @@ int engine_step(saige_ctx* c, Engine& e, int n_iter, uint8_t* done_out, int32_t*
         if (e.kpad == 1)
             SAIGE_HIP_TRY(launch_grm_gemv(st, c->d_grm, Np, N, e.P, e.AP, a, e.partial, &np));
         else
-            SAIGE_HIP_TRY(launch_grm_gemm(st, c->d_grm, Np, N, e.P, e.AP, a, e.partial, &np));
+            SAIGE_HIP_TRY(launch_grm_gemm(st, c->d_grm, Np, N, e.P, e.AP, a, e.partial, &np, e.ypart));
         SAIGE_HIP_TRY(hipEventRecord(c->ev1, st));
-        SAIGE_HIP_TRY(launch_alpha(st, e.kpad, e.partial, np, e.rz, e.done, e.live, e.alpha));
+        SAIGE_HIP_TRY(launch_alpha(st, e.kpad, e.partial, np, e.rz, e.done, e.live, e.kind, e.alpha));
         int64_t np2 = 0;
-        SAIGE_HIP_TRY(launch_update(st, e.kpad, N, Np, e.alpha, e.done, e.live, e.minv, e.slot_sys, e.X, e.R,
+        SAIGE_HIP_TRY(launch_update(st, e.kpad, N, Np, e.alpha, e.done, e.live, e.kind, e.minv, e.slot_sys, e.X, e.R,
                                     e.Z, e.P, e.AP, e.partial, &np2));
         SAIGE_HIP_TRY(launch_beta(st, e.kpad, e.partial, np2, e.rz, e.sumr2, e.maxiter, e.tol, e.iter, e.done,
-                                  e.live, e.beta));
-        SAIGE_HIP_TRY(launch_pupdate(st, e.kpad, N, Np, e.beta, e.done, e.live, e.Z, e.P));
+                                  e.live, e.kind, e.beta));
+        SAIGE_HIP_TRY(launch_pupdate(st, e.kpad, N, Np, e.beta, e.done, e.live, e.kind, e.Z, e.P));
         SAIGE_HIP_TRY(hipMemcpyAsync(e.h_done, e.done, e.kpad, hipMemcpyDeviceToHost, st));
         SAIGE_HIP_TRY(hipStreamSynchronize(st));
         float ms = 0.f;
@@ -296,18 +335,34 @@ int engine_step(saige_ctx* c, Engine& e, int n_iter, uint8_t* done_out, int32_t*
     return SAIGE_OK;
 }
 
-int engine_take(saige_ctx* c, Engine& e, int slot, float* x_host) {
-    if (slot < 0 || slot >= e.kpad) {
-        set_error("slot %d out of range", slot);
+// Copy the solutions of k slots (X_host: N x k, column-major; may be NULL)
+// and free the slots.
+int engine_take_batch(saige_ctx* c, Engine& e, int k, const int32_t* slots, float* X_host) {
+    if (k < 1 || k > e.kpad || !slots) {
+        set_error("invalid take of %d columns", k);
         return SAIGE_E_ARG;
     }
+    for (int j = 0; j < k; j++)
+        if (slots[j] < 0 || slots[j] >= e.kpad) {
+            set_error("slot %d out of range", slots[j]);
+            return SAIGE_E_ARG;
+        }
     hipStream_t st = c->stream;
-    if (x_host) {
-        SAIGE_HIP_TRY(launch_gather_col(st, e.X, slot, e.kpad, c->Nnm, e.stage));
-        SAIGE_HIP_TRY(hipMemcpyAsync(x_host, e.stage, sizeof(float) * c->Nnm, hipMemcpyDeviceToHost, st));
+    if (X_host) {
+        std::vector<int32_t> lists(2 * e.kpad, 0);
+        std::copy(slots, slots + k, lists.begin());
+        SAIGE_HIP_TRY(hipMemcpyAsync(e.d_list, lists.data(), 8 * e.kpad, hipMemcpyHostToDevice, st));
+        SAIGE_HIP_TRY(launch_cols_gather(st, e.X, k, e.d_list, e.kpad, c->Nnm, e.stageB));
+        SAIGE_HIP_TRY(hipMemcpyAsync(X_host, e.stageB, sizeof(float) * c->Nnm * k, hipMemcpyDeviceToHost, st));
     }
-    e.h_live[slot] = 0;
-    return engine_sync_host_arrays(c, e, nullptr, nullptr);
+    for (int j = 0; j < k; j++) e.h_live[slots[j]] = 0;
+    SAIGE_HIP_TRY(hipMemcpyAsync(e.live, e.h_live.data(), e.kpad, hipMemcpyHostToDevice, st));
+    SAIGE_HIP_TRY(hipStreamSynchronize(st));
+    return SAIGE_OK;
+}
+
+int engine_take(saige_ctx* c, Engine& e, int slot, float* x_host) {
+    return engine_take_batch(c, e, 1, &slot, x_host);
 }
 
 // Common tail of the three setgeno variants: decode, compact, repack.
@@ -749,6 +804,15 @@ int saige_build_grm(saige_ctx* c, int precision) {
     }
     void* d_panel = nullptr;
     SAIGE_HIP_TRY(hipMalloc(&d_panel, (size_t)(Np * pw * esz)));
+    const std::vector<int2> order = syrk_tile_order(Np / syrk_tile(), 8);
+    int2* d_tiles = nullptr;
+    if (dalloc(&d_tiles, (int64_t)order.size()) ||
+        hipMemcpy(d_tiles, order.data(), sizeof(int2) * order.size(), hipMemcpyHostToDevice)) {
+        (void)hipFree(d_panel);
+        dfree(d_tiles);
+        set_error("cannot stage the SYRK tile list");
+        return SAIGE_E_NOMEM;
+    }
     SAIGE_HIP_TRY(hipEventRecord(c->ev0, st));
     hipError_t e = hipSuccess;
     int first = 1;
@@ -759,18 +823,20 @@ int saige_build_grm(saige_ctx* c, int precision) {
             e = launch_std_panel_bf16(st, c->d_packed, c->d_lut, j0, np, c->Nnm, Np, c->bpm_nm, ldz,
                                       static_cast<__hip_bfloat16*>(d_panel));
             if (e == hipSuccess)
-                e = launch_syrk_bf16(st, static_cast<__hip_bfloat16*>(d_panel), Np, ldz, ldz, c->d_grm, !first);
+                e = launch_syrk_bf16(st, static_cast<__hip_bfloat16*>(d_panel), Np, ldz, ldz, d_tiles, c->d_grm,
+                                     !first);
         } else {
             e = launch_std_panel_f32(st, c->d_packed, c->d_lut, j0, np, c->Nnm, Np, c->bpm_nm, ldz,
                                      static_cast<float*>(d_panel));
             if (e == hipSuccess)
-                e = launch_syrk_f32(st, static_cast<float*>(d_panel), Np, ldz, ldz, c->d_grm, !first);
+                e = launch_syrk_f32(st, static_cast<float*>(d_panel), Np, ldz, ldz, d_tiles, c->d_grm, !first);
         }
         first = 0;
     }
     if (e == hipSuccess) e = hipEventRecord(c->ev1, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     (void)hipFree(d_panel);
+    dfree(d_tiles);
     if (e != hipSuccess) {
         set_error("GRM build failed: %s", hipGetErrorString(e));
         return SAIGE_E_DEVICE;
@@ -821,25 +887,26 @@ int saige_grm_matvec(saige_ctx* c, const float* B, int64_t k, float* Y) {
         SAIGE_HIP_TRY(hipStreamSynchronize(st));
         return SAIGE_OK;
     }
-    // own scratch, so live engine slots are untouched
-    float *d_in = nullptr, *d_blk = nullptr, *d_out = nullptr;
-    auto release = [&]() { dfree(d_in); dfree(d_blk); dfree(d_out); };
-    if (dalloc(&d_in, N * EW) || dalloc(&d_blk, Np * EW) || dalloc(&d_out, Np * EW)) {
-        release();
-        set_error("device allocation failed (grm_matvec)");
-        return SAIGE_E_NOMEM;
+    // own scratch (not the engine's), so live engine slots are untouched
+    if (!c->d_mv_in) {
+        if (dalloc(&c->d_mv_in, N * EW) || dalloc(&c->d_mv_blk, Np * EW) || dalloc(&c->d_mv_out, Np * EW) ||
+            dalloc(&c->d_mv_ypart, gemm_ypart_floats(Np))) {
+            dfree(c->d_mv_in); dfree(c->d_mv_blk); dfree(c->d_mv_out); dfree(c->d_mv_ypart); dfree(c->d_mv_ypart);
+            set_error("device allocation failed (grm_matvec)");
+            return SAIGE_E_NOMEM;
+        }
     }
+    float *d_in = c->d_mv_in, *d_blk = c->d_mv_blk, *d_out = c->d_mv_out;
     hipError_t e = hipSuccess;
     for (int64_t k0 = 0; k0 < k && e == hipSuccess; k0 += EW) {
         const int kb = (int)std::min<int64_t>(EW, k - k0);
         e = hipMemcpyAsync(d_in, B + k0 * N, 4 * N * kb, hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = launch_scatter_cols(st, d_in, kb, N, Np, EW, d_blk);
-        if (e == hipSuccess) e = launch_grm_gemm(st, c->d_grm, Np, N, d_blk, d_out, a, nullptr, nullptr);
+        if (e == hipSuccess) e = launch_grm_gemm(st, c->d_grm, Np, N, d_blk, d_out, a, nullptr, nullptr, c->d_mv_ypart);
         if (e == hipSuccess) e = launch_gather_cols(st, d_out, kb, N, EW, d_in);
         if (e == hipSuccess) e = hipMemcpyAsync(Y + k0 * N, d_in, 4 * N * kb, hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
     }
-    release();
     if (e != hipSuccess) {
         set_error("grm_matvec failed: %s", hipGetErrorString(e));
         return SAIGE_E_DEVICE;
@@ -903,6 +970,8 @@ int saige_get_crossprod(saige_ctx* c, const float* b, const float* w, const floa
     a.inv_m = 0.f;
     a.mqc = (float)c->Mqc;
     a.mode = 1;
+    a.kind = nullptr;
+    a.xout = nullptr;
     SAIGE_HIP_TRY(launch_grm_gemv(st, c->d_grm, Np, N, e.stage, e.AP, a, e.partial, nullptr));
     SAIGE_HIP_TRY(hipMemcpyAsync(out, e.AP, 4 * N, hipMemcpyDeviceToHost, st));
     SAIGE_HIP_TRY(hipStreamSynchronize(st));
@@ -960,8 +1029,9 @@ int saige_pcg_block(saige_ctx* c, const float* w, const float* tau, const float*
     std::vector<int32_t> it(EW);
     for (int64_t k0 = 0; k0 < k; k0 += EW) {
         const int kb = (int)std::min<int64_t>(EW, k - k0);
-        for (int j = 0; j < kb; j++)
-            if ((rc = engine_add(c, e, j, 0, B + (k0 + j) * N, maxiter, tol))) return rc;
+        std::vector<int32_t> slots(kb), sys0(kb, 0);
+        std::iota(slots.begin(), slots.end(), 0);
+        if ((rc = engine_add_batch(c, e, kb, slots.data(), sys0.data(), B + k0 * N, maxiter, tol))) return rc;
         for (;;) {
             if ((rc = engine_step(c, e, 64, done.data(), it.data()))) return rc;
             bool all = true;
@@ -971,8 +1041,8 @@ int saige_pcg_block(saige_ctx* c, const float* w, const float* tau, const float*
         for (int j = 0; j < kb; j++) {
             if (iters) iters[k0 + j] = it[j];
             if (it[j] >= maxiter && maxiter > 0) notconv = true;
-            if ((rc = engine_take(c, e, j, X + (k0 + j) * N))) return rc;
         }
+        if ((rc = engine_take_batch(c, e, kb, slots.data(), X + k0 * N))) return rc;
     }
     if (notconv) {
         set_error("pcg did not converge for at least one column. You may increase maxiter number.");
@@ -1009,6 +1079,36 @@ int saige_engine_add(saige_ctx* c, int slot, int s, const float* b, int maxiter,
         return SAIGE_E_STATE;
     }
     return engine_add(c, c->eng, slot, s, b, maxiter, tol);
+}
+
+int saige_engine_add_batch(saige_ctx* c, int k, const int32_t* slots, const int32_t* systems, const float* B,
+                           int maxiter, float tol) {
+    CHECK_CTX(c);
+    CHECK_GRM(c);
+    if (!c->eng.X) {
+        set_error("engine not reserved");
+        return SAIGE_E_STATE;
+    }
+    return engine_add_batch(c, c->eng, k, slots, systems, B, maxiter, tol);
+}
+
+int saige_engine_add_products(saige_ctx* c, int k, const int32_t* slots, const int32_t* systems, const float* B) {
+    CHECK_CTX(c);
+    CHECK_GRM(c);
+    if (!c->eng.X) {
+        set_error("engine not reserved");
+        return SAIGE_E_STATE;
+    }
+    return engine_add_batch(c, c->eng, k, slots, systems, B, 1, 0.0f, 1);
+}
+
+int saige_engine_take_batch(saige_ctx* c, int k, const int32_t* slots, float* X) {
+    CHECK_CTX(c);
+    if (!c->eng.X) {
+        set_error("engine not reserved");
+        return SAIGE_E_STATE;
+    }
+    return engine_take_batch(c, c->eng, k, slots, X);
 }
 
 int saige_engine_step(saige_ctx* c, int n_iter, uint8_t* done, int32_t* iters) {
@@ -1067,6 +1167,17 @@ int saige_alloc_device(saige_ctx* c, int64_t bytes, void** out) {
 int saige_free_device(saige_ctx* c, void* p) {
     CHECK_CTX(c);
     if (p) SAIGE_HIP_TRY(hipFree(p));
+    return SAIGE_OK;
+}
+
+int saige_copy_device(saige_ctx* c, void* dst, const void* src, int64_t bytes) {
+    CHECK_CTX(c);
+    if (!src || !dst || bytes < 0) {
+        set_error("invalid copy");
+        return SAIGE_E_ARG;
+    }
+    SAIGE_HIP_TRY(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice, c->stream));
+    SAIGE_HIP_TRY(hipStreamSynchronize(c->stream));
     return SAIGE_OK;
 }
 

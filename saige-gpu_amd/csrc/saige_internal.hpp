@@ -65,9 +65,16 @@ struct Engine {
     double* tol = nullptr;
     uint8_t* live = nullptr;      // slot is solving
     uint8_t* done = nullptr;      // slot stopped (device)
+    uint8_t* kind = nullptr;      // 0 = PCG solve, 1 = GRM product (one GEMM pass)
+    std::vector<uint8_t> h_kind;
     double* partial = nullptr;    // [nparts][kpad][2] partial dot products
     int64_t nparts = 0;
-    float* stage = nullptr;       // [Npad] host-visible staging for one column
+    float* stage = nullptr;       // [Npad] staging for one column
+    float* stageB = nullptr;      // [kpad][Npad] staging for a block of columns
+    float* ypart = nullptr;       // split-K partial products of the block GEMM
+    int32_t* d_list = nullptr;    // [2][kpad] slot / system lists of a batch call
+    std::vector<int32_t> h_maxiter;
+    std::vector<double> h_tol;
     // pinned host mirrors
     uint8_t* h_done = nullptr;
     int32_t* h_iter = nullptr;
@@ -109,6 +116,11 @@ struct saige_ctx {
     double grm_seconds = 0.0;
     // scratch vectors
     float* d_vec = nullptr;                      // [4][Npad]
+    // GRM product scratch (saige_grm_matvec), allocated on first use
+    float* d_mv_in = nullptr;                    // [EW][Nnm]
+    float* d_mv_blk = nullptr;                   // [Npad][EW]
+    float* d_mv_out = nullptr;                   // [Npad][EW]
+    float* d_mv_ypart = nullptr;                 // split-K scratch
     saige::Engine eng;
     saige::Engine eng1;                          // kpad = 1 engine for saige_pcg1
 };
@@ -163,10 +175,13 @@ hipError_t launch_std_panel_bf16(hipStream_t s, const uint8_t* packed, const flo
 // ---- GRM kernels (grm_kernels.hip)
 // S (Npad x Npad, ld = Npad) (+)= Zp Zp^T over the lower-triangle tiles,
 // mirrored into the upper triangle.  kdim is a multiple of the kernel's k-chunk.
+// tiles: the lower-triangle tile list from syrk_tile_order(Npad / syrk_tile())
 hipError_t launch_syrk_f32(hipStream_t s, const float* Zp, int64_t Npad, int64_t ldz, int64_t kdim,
-                           float* S, int accumulate);
+                           const int2* tiles, float* S, int accumulate);
 hipError_t launch_syrk_bf16(hipStream_t s, const __hip_bfloat16* Zp, int64_t Npad, int64_t ldz,
-                            int64_t kdim, float* S, int accumulate);
+                            int64_t kdim, const int2* tiles, float* S, int accumulate);
+std::vector<int2> syrk_tile_order(int64_t nt, int64_t g);
+int syrk_tile();
 int syrk_kchunk_f32();
 int syrk_kchunk_bf16();
 
@@ -180,13 +195,17 @@ struct SigmaArgs {
     float inv_m;           // unused; the product is divided by Mqc as float
     float mqc;             // Mqc as float (divisor, FG.cpp:1706)
     int mode;              // 0: out = S p / Mqc ; 1: out = Sigma p
+    const uint8_t* kind;   // [kpad] 1 = product slot (out = K p, also written to xout); may be null
+    float* xout;           // result block of product slots
 };
 // out[N][kpad] = epilogue(S * in), plus per-column partial p^T out dots when
 // mode == 1 (partial: [nparts][kpad][2], slot 0).  Returns nparts used.
 hipError_t launch_grm_gemv(hipStream_t s, const float* S, int64_t Npad, int64_t Nnm, const float* in,
                            float* out, SigmaArgs a, double* partial, int64_t* nparts);
+// ypart: gemm_ypart_floats(Npad) floats of scratch for the split-K slices
 hipError_t launch_grm_gemm(hipStream_t s, const float* S, int64_t Npad, int64_t Nnm, const float* in,
-                           float* out, SigmaArgs a, double* partial, int64_t* nparts);
+                           float* out, SigmaArgs a, double* partial, int64_t* nparts, float* ypart);
+int64_t gemm_ypart_floats(int64_t Npad);
 int64_t gemv_max_parts(int64_t Npad);
 int64_t gemm_max_parts(int64_t Npad);
 int64_t vec_max_parts(int64_t Npad, int kpad);
@@ -203,16 +222,26 @@ hipError_t launch_col_init_finish(hipStream_t s, int slot, int kpad, const doubl
                                   int64_t nparts, double* rz, double* sumr2, const int32_t* maxiter,
                                   const double* tol, int32_t* iter, uint8_t* done, uint8_t* live);
 hipError_t launch_alpha(hipStream_t s, int kpad, const double* partial, int64_t nparts,
-                        const double* rz, const uint8_t* done, const uint8_t* live, float* alpha);
+                        const double* rz, const uint8_t* done, const uint8_t* live, const uint8_t* kind,
+                        float* alpha);
 hipError_t launch_update(hipStream_t s, int kpad, int64_t Nnm, int64_t Npad, const float* alpha,
-                         const uint8_t* done, const uint8_t* live, const float* minv,
+                         const uint8_t* done, const uint8_t* live, const uint8_t* kind, const float* minv,
                          const int32_t* sys, float* X, float* R, float* Z, const float* P,
                          const float* AP, double* partial, int64_t* nparts);
 hipError_t launch_beta(hipStream_t s, int kpad, const double* partial, int64_t nparts, double* rz,
                        double* sumr2, const int32_t* maxiter, const double* tol, int32_t* iter,
-                       uint8_t* done, const uint8_t* live, float* beta);
+                       uint8_t* done, const uint8_t* live, const uint8_t* kind, float* beta);
 hipError_t launch_pupdate(hipStream_t s, int kpad, int64_t Nnm, int64_t Npad, const float* beta,
-                          const uint8_t* done, const uint8_t* live, const float* Z, float* P);
+                          const uint8_t* done, const uint8_t* live, const uint8_t* kind, const float* Z,
+                          float* P);
+hipError_t launch_cols_init(hipStream_t s, const float* B, int k, const int32_t* slots, const int32_t* sys,
+                            int kpad, int64_t Nnm, int64_t Npad, const float* minv, const uint8_t* mask, float* X,
+                            float* R, float* Z, float* P, double* partial, int64_t* nparts, int product);
+hipError_t launch_cols_init_finish(hipStream_t s, int k, const int32_t* slots, int kpad, const double* partial,
+                                   int64_t nparts, double* rz, double* sumr2, const int32_t* maxiter,
+                                   const double* tol, int32_t* iter, uint8_t* done, int product);
+hipError_t launch_cols_gather(hipStream_t s, const float* src, int k, const int32_t* slots, int kpad, int64_t Nnm,
+                              float* dst);
 hipError_t launch_gather_col(hipStream_t s, const float* src, int slot, int kpad, int64_t Nnm,
                              float* dst);
 hipError_t launch_scatter_cols(hipStream_t s, const float* src, int64_t k, int64_t Nnm,

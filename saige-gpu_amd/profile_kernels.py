@@ -1,0 +1,65 @@
+"""Profiling driver: runs the hot kernels at the benchmark workload's size
+(N = 100,000, M = 150,000 unless overridden) a fixed number of times, next
+to a device-to-device copy of known size, so that rocprofv3 kernel traces
+and HBM counters (FETCH_SIZE, WRITE_SIZE) of each can be read per dispatch:
+
+    rocprofv3 --kernel-trace --stats -d OUT -o kernels -- python3 saige-gpu_amd/profile_kernels.py
+    rocprofv3 --pmc FETCH_SIZE -d OUT -o fetch -- python3 saige-gpu_amd/profile_kernels.py
+    rocprofv3 --pmc WRITE_SIZE -d OUT -o write -- python3 saige-gpu_amd/profile_kernels.py
+
+Kernels exercised: decode/QC (k_decode_stats), repack, diag, the BF16 SYRK
+(k_std_panel + k_syrk_bf16), the 64-column block GEMM (k_gemm_part +
+k_gemm_epi, through saige_grm_matvec), the single-column GEMV (k_gemv) and
+the runtime's copy kernel on COPY_BYTES.
+"""
+import argparse
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+COPY_BYTES = 1 << 30
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--N", type=int, default=100_000)
+    ap.add_argument("--M", type=int, default=150_000)
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--grm", default="bf16")
+    args = ap.parse_args()
+
+    import numpy as np
+    from saige_gpu import Session
+
+    N, M = args.N, args.M
+    s = Session(0)
+    bpm = (N + 3) // 4
+    bed = s.alloc_device(M * bpm)
+    s.synth_bed_device(bed, N, M, 0.01, 0.5, 0.01, 0.0, 20261202 + 2000)
+    s.setminMAFforGRM(0.01)
+    s.setmaxMissingRateforGRM(0.15)
+    s.setgeno_device(bed, N, M)
+    t = s.build_grm(args.grm)
+    print(f"GRM {args.grm} {t:.3f}s", flush=True)
+    rng = np.random.default_rng(0)
+    B = rng.standard_normal((N, 64)).astype(np.float32)
+    for _ in range(args.reps):
+        s.grm_matvec(B)
+    for _ in range(args.reps):
+        s.getCrossprodMatAndKin(B[:, 0])
+    a = s.alloc_device(COPY_BYTES)
+    b = s.alloc_device(COPY_BYTES)
+    for _ in range(args.reps):
+        s.copy_device(b, a, COPY_BYTES)
+    s.free_device(a)
+    s.free_device(b)
+    s.free_device(bed)
+    s.close()
+    print("profile driver done", flush=True)
+
+
+if __name__ == "__main__":
+    main()

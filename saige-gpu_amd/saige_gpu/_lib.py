@@ -71,12 +71,16 @@ SIGNATURES = {
     "saige_engine_reserve": (_int, [_c_p, _int, _int]),
     "saige_engine_set_system": (_int, [_c_p, _int, _c_p, _c_p, _c_p]),
     "saige_engine_add": (_int, [_c_p, _int, _int, _c_p, _int, _f32]),
+    "saige_engine_add_batch": (_int, [_c_p, _int, _c_p, _c_p, _c_p, _int, _f32]),
+    "saige_engine_add_products": (_int, [_c_p, _int, _c_p, _c_p, _c_p]),
+    "saige_engine_take_batch": (_int, [_c_p, _int, _c_p, _c_p]),
     "saige_engine_step": (_int, [_c_p, _int, _c_p, _c_p]),
     "saige_engine_take": (_int, [_c_p, _int, _c_p]),
     "saige_engine_stats": (_int, [_c_p, ctypes.POINTER(_f64), ctypes.POINTER(_i64)]),
     "saige_synth_bed_device": (_int, [_c_p, _c_p, _i64, _i64, _f32, _f32, _f32, _f32, _u64]),
     "saige_alloc_device": (_int, [_c_p, _i64, ctypes.POINTER(_c_p)]),
     "saige_free_device": (_int, [_c_p, _c_p]),
+    "saige_copy_device": (_int, [_c_p, _c_p, _c_p, _i64]),
     "saige_copy_device_to_host": (_int, [_c_p, _c_p, _i64, _c_p]),
 }
 

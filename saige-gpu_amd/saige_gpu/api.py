@@ -261,6 +261,28 @@ class Session:
         b = f32(b)
         check(self._L.saige_engine_add(self.ctx, int(slot), int(s), ptr(b), int(maxiter), float(tol)))
 
+    def engine_add_batch(self, slots, systems, B, maxiter, tol):
+        """Start len(slots) solves: column j of B (N x k) in slot slots[j]."""
+        sl = np.ascontiguousarray(slots, dtype=np.int32)
+        sy = np.ascontiguousarray(systems, dtype=np.int32)
+        Bc = cols_f32(B)
+        assert Bc.shape[1] == len(sl)
+        check(self._L.saige_engine_add_batch(self.ctx, len(sl), ptr(sl), ptr(sy), ptr(Bc), int(maxiter),
+                                             float(tol)))
+
+    def engine_add_products(self, slots, systems, B):
+        """Queue GRM products K b (column j of B in slot slots[j])."""
+        sl = np.ascontiguousarray(slots, dtype=np.int32)
+        sy = np.ascontiguousarray(systems, dtype=np.int32)
+        Bc = cols_f32(B)
+        check(self._L.saige_engine_add_products(self.ctx, len(sl), ptr(sl), ptr(sy), ptr(Bc)))
+
+    def engine_take_batch(self, slots):
+        sl = np.ascontiguousarray(slots, dtype=np.int32)
+        X = np.zeros((self.getNnomissingOut(), len(sl)), np.float32, order="F")
+        check(self._L.saige_engine_take_batch(self.ctx, len(sl), ptr(sl), ptr(X)))
+        return X
+
     def engine_step(self, n_iter, width=64):
         done = np.zeros(width, np.uint8)
         iters = np.zeros(width, np.int32)
@@ -290,6 +312,9 @@ class Session:
     def synth_bed_device(self, dev, N, M, maf_lo, maf_hi, miss_rate, pair_frac, seed):
         check(self._L.saige_synth_bed_device(self.ctx, dev, int(N), int(M), float(maf_lo), float(maf_hi),
                                              float(miss_rate), float(pair_frac), int(seed)))
+
+    def copy_device(self, dst, src, nbytes):
+        check(self._L.saige_copy_device(self.ctx, dst, src, int(nbytes)))
 
     def copy_device_to_host(self, dev, nbytes, offset=0):
         out = np.zeros(int(nbytes), np.uint8)

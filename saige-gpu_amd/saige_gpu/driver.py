@@ -89,8 +89,8 @@ def glm_start(y, X, family, offset=None, maxit=25, epsilon=1e-8):
         me = family.mu_eta(eta)
         z = (eta - off) + (y - mu) / me
         wt = me ** 2 / family.variance(mu)
-        sw = np.sqrt(wt)
-        beta = np.linalg.lstsq(X * sw[:, None], z * sw, rcond=None)[0]
+        Xw = X * wt[:, None]
+        beta = np.linalg.solve(X.T @ Xw, Xw.T @ z)
         eta = X @ beta + off
         mu = family.linkinv(eta)
         dev = deviance(mu)
@@ -119,8 +119,10 @@ class RademacherProbes:
     def __call__(self, i):
         v = self._cache.get(i)
         if v is None:
-            rng = np.random.default_rng([self.seed, i])
-            v = rng.integers(0, 2, self.n).astype(np.float64) * 2.0 - 1.0
+            # one random byte per 8 entries (Philox keyed by (seed, i))
+            rng = np.random.Generator(np.random.Philox(key=[self.seed, i]))
+            bits = np.unpackbits(np.frombuffer(rng.bytes((self.n + 7) // 8), np.uint8))[: self.n]
+            v = bits.astype(np.float64) * 2.0 - 1.0
             self._cache[i] = v
         return v
 
@@ -389,6 +391,7 @@ class PlainExecutor:
         self.opt = opt
         self.n_solves = 0
         self.pcg_iters = 0
+        self.n_products = 0
 
     def run(self, gen):
         value = None
@@ -412,11 +415,12 @@ class PlainExecutor:
                 _, B = req
                 value = np.column_stack([self.sess.getCrossprodMatAndKin(B[:, j])
                                          for j in range(B.shape[1])]).astype(np.float64)
+                self.n_products += B.shape[1]
 
 
 class _Fit:
     __slots__ = ("fid", "gen", "subset", "n", "sys", "pending", "result_buf", "remaining", "result",
-                 "error", "t_submit", "t_admit", "t_done", "tag", "iters")
+                 "error", "t_submit", "t_admit", "t_done", "tag", "iters", "sys_ready")
 
     def __init__(self, fid, gen, subset, tag):
         self.fid = fid
@@ -433,6 +437,7 @@ class _Fit:
         self.t_done = None
         self.tag = tag
         self.iters = 0
+        self.sys_ready = False
 
 
 class BatchScheduler:
@@ -441,9 +446,10 @@ class BatchScheduler:
     Fits are admitted while a system slot (per-fit w, tau, sample mask on the
     device) is free; their solve columns fill the engine's slots in FIFO
     order, join at the next iteration, and leave when they converge; a fit
-    resumes as soon as its request's last column is done.  A column's
-    arithmetic does not depend on its neighbours, so each fit gets the same
-    result alone and in the mix."""
+    resumes as soon as its request's last column is done.  GRM products
+    (K u for the trace probes, K PY) are product slots that ride along in
+    the next iteration's GEMM.  A column's arithmetic does not depend on its
+    neighbours, so each fit gets the same result alone and in the mix."""
 
     def __init__(self, sess, opt, width=64, n_systems=16):
         self.sess = sess
@@ -462,6 +468,8 @@ class BatchScheduler:
         self.n_iterations = 0
         self.n_solves = 0
         self.busy_slot_iters = 0
+        # wall-time split of run(): device iterations, slot traffic, fit logic
+        self.timing = collections.Counter()
 
     # ---------------------------------------------------------------- API
     def submit(self, gen, subset=None, tag=None, t_submit=None):
@@ -499,7 +507,9 @@ class BatchScheduler:
             f.sys = self.free_sys.popleft()
             f.t_admit = time.perf_counter()
             self.active.append(f)
+            t = time.perf_counter()
             self._advance(f, None)
+            self.timing["host"] += time.perf_counter() - t
 
     def _expand(self, f, v):
         if f.subset is None:
@@ -525,27 +535,31 @@ class BatchScheduler:
                 return
             if req[0] == "matvec":
                 B = req[1]
-                if f.subset is not None:
-                    Bf = np.zeros((self.N, B.shape[1]))
-                    Bf[f.subset] = B
-                    B = Bf
-                KB = self.sess.grm_matvec(B).astype(np.float64)
-                value = KB if f.subset is None else KB[f.subset]
-                continue
-            _, w, tau, B = req
-            mask = None
-            wf = w
-            if f.subset is not None:
-                wf = np.ones(self.N)
-                wf[f.subset] = w
-                mask = np.zeros(self.N, np.uint8)
-                mask[f.subset] = 1
-            self.sess.engine_set_system(f.sys, wf, np.asarray(tau, np.float32), mask)
+                if not f.sys_ready:          # products only need the system's sample mask
+                    self._set_system(f, np.ones(B.shape[0]), (1.0, 0.0))
+                kind = "product"
+            else:
+                _, w, tau, B = req
+                self._set_system(f, w, tau)
+                kind = "solve"
             f.result_buf = np.zeros(B.shape, np.float64)
             f.remaining = B.shape[1]
             for j in range(B.shape[1]):
-                f.pending.append((j, B[:, j]))
+                f.pending.append((j, B[:, j], kind))
             return
+
+    def _set_system(self, f, w, tau):
+        mask = None
+        wf = w
+        if f.subset is not None:
+            wf = np.ones(self.N)
+            wf[f.subset] = w
+            mask = np.zeros(self.N, np.uint8)
+            mask[f.subset] = 1
+        t = time.perf_counter()
+        self.sess.engine_set_system(f.sys, wf, np.asarray(tau, np.float32), mask)
+        self.timing["set_system"] += time.perf_counter() - t
+        f.sys_ready = True
 
     def _finish(self, f):
         f.t_done = time.perf_counter()
@@ -554,33 +568,62 @@ class BatchScheduler:
         self.finished.append(f)
 
     def _fill(self):
+        groups = {"solve": ([], [], [], []), "product": ([], [], [], [])}
         for f in self.active:
             while f.pending and self.free_slots:
-                j, b = f.pending.popleft()
+                j, b, kind = f.pending.popleft()
                 slot = self.free_slots.popleft()
-                self.sess.engine_add(slot, f.sys, self._expand(f, b), self.opt.maxiterPCG, self.opt.tolPCG)
-                self.slot_owner[slot] = (f, j)
+                g = groups[kind]
+                g[0].append(slot)
+                g[1].append(f.sys)
+                g[2].append((f, b))
+                g[3].append((f, j))
             if not self.free_slots:
                 break
+        for kind, (slots, systems, cols, owners) in groups.items():
+            if not slots:
+                continue
+            t = time.perf_counter()
+            B = np.zeros((self.N, len(cols)), np.float32, order="F")   # column-major, as the ABI takes it
+            for c, (f, b) in enumerate(cols):
+                if f.subset is None:
+                    B[:, c] = b
+                else:
+                    B[f.subset, c] = b
+            if kind == "solve":
+                self.sess.engine_add_batch(slots, systems, B, self.opt.maxiterPCG, self.opt.tolPCG)
+            else:
+                self.sess.engine_add_products(slots, systems, B)
+            for slot, own in zip(slots, owners):
+                self.slot_owner[slot] = own
+            self.timing["add"] += time.perf_counter() - t
 
     def _step(self):
+        t = time.perf_counter()
         done, iters = self.sess.engine_step(1, self.width)
+        self.timing["step"] += time.perf_counter() - t
         self.n_iterations += 1
         self.busy_slot_iters += len(self.slot_owner)
+        finished = [slot for slot in self.slot_owner if done[slot]]
+        if not finished:
+            return
+        t = time.perf_counter()
+        X = self.sess.engine_take_batch(finished)
+        self.timing["take"] += time.perf_counter() - t
         resume = []
-        for slot in list(self.slot_owner):
-            if done[slot]:
-                f, j = self.slot_owner.pop(slot)
-                x = self.sess.engine_take(slot)
-                self.free_slots.append(slot)
-                f.result_buf[:, j] = self._compact(f, x)
-                f.iters += int(iters[slot])
-                f.remaining -= 1
-                self.n_solves += 1
-                if f.remaining == 0 and not f.pending:
-                    resume.append(f)
+        for c, slot in enumerate(finished):
+            f, j = self.slot_owner.pop(slot)
+            self.free_slots.append(slot)
+            f.result_buf[:, j] = self._compact(f, X[:, c])
+            f.iters += int(iters[slot])
+            f.remaining -= 1
+            self.n_solves += 1
+            if f.remaining == 0 and not f.pending:
+                resume.append(f)
+        t = time.perf_counter()
         for f in resume:
             self._advance(f, f.result_buf)
+        self.timing["host"] += time.perf_counter() - t
 
 
 def fit_plain(sess, trait, y, X, opt=None, probe_seed=200, **kw):
@@ -591,6 +634,7 @@ def fit_plain(sess, trait, y, X, opt=None, probe_seed=200, **kw):
     res = ex.run(fit_co(trait, y, X, probes, opt, **kw))
     res["n_solves"] = ex.n_solves
     res["pcg_iters"] = ex.pcg_iters
+    res["n_products"] = ex.n_products
     return res
 
 

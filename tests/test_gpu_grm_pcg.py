@@ -18,14 +18,21 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
-def grm1000(session):
+def grm1000_ref():
     body, ids = load_bed("grm1000x10k", 1000)
+    og = O.setgeno(body, 1000, len(ids), min_maf=0.01, max_missing=0.15)
+    return body, len(ids), og, O.grm_f64(og)
+
+
+@pytest.fixture
+def grm1000(session, grm1000_ref):
+    """(Re)loads the 1000 x 10k fixture into the shared session: other tests
+    load other genotypes."""
+    body, M, og, S = grm1000_ref
     session.setminMAFforGRM(0.01)
     session.setmaxMissingRateforGRM(0.15)
     session.setminMAC_VarianceRatio(20, -1, False)
-    session.setgeno_bytes(body, 1000, len(ids))
-    og = O.setgeno(body, 1000, len(ids), min_maf=0.01, max_missing=0.15)
-    S = O.grm_f64(og)
+    session.setgeno_bytes(body, 1000, M)
     return og, S
 
 
@@ -207,3 +214,39 @@ def test_odd_sizes(session):
     x = session.getPCG1ofSigmaAndVector(w, np.array([1.0, 0.4], np.float32), b)
     x64 = np.linalg.solve(np.diag(np.full(len(sub), 5.0)) + 0.4 * K64, b)
     assert np.max(np.abs(x - x64)) <= 2e-3 * np.max(np.abs(x64))
+
+
+def test_engine_product_slots(session, grm1000):
+    """GRM products ride along in the engine's GEMM next to live solves."""
+    og, S = grm1000
+    print("case: engine product slots", flush=True)
+    session.build_grm("fp32")
+    K64 = _k64(og, S)
+    rng = np.random.default_rng(12)
+    session.engine_reserve(64, 2)
+    w = rng.uniform(0.05, 0.25, 1000).astype(np.float32)
+    session.engine_set_system(0, w, np.array([1.0, 0.3], np.float32))
+    sub = np.sort(rng.choice(1000, 700, replace=False))
+    mask = np.zeros(1000, np.uint8)
+    mask[sub] = 1
+    session.engine_set_system(1, w, np.array([1.0, 0.3], np.float32), mask)
+    B = rng.standard_normal((1000, 6)).astype(np.float32)
+    session.engine_add_batch([0, 1], [0, 0], B[:, :2], 500, 1e-5)        # two solves in flight
+    session.engine_add_products([5, 9, 11], [0, 0, 1], B[:, 2:5])
+    done, it = session.engine_step(1)
+    assert done[5] and done[9] and done[11] and it[5] == 1
+    P = session.engine_take_batch([5, 9, 11])
+    ref = K64 @ B[:, 2:4].astype(np.float64)
+    assert np.max(np.abs(P[:, :2] - ref)) <= 2e-5 * np.max(np.abs(ref))
+    refm = np.zeros(1000)
+    refm[sub] = K64[np.ix_(sub, sub)] @ B[sub, 4].astype(np.float64)
+    assert np.all(P[mask == 0, 2] == 0)
+    assert np.max(np.abs(P[:, 2] - refm)) <= 2e-5 * np.max(np.abs(refm))
+    # the solves that shared the GEMM are unaffected
+    for _ in range(1000):
+        done, _ = session.engine_step(1)
+        if done[0] and done[1]:
+            break
+    X = session.engine_take_batch([0, 1])
+    X64 = np.linalg.solve(_sigma64(og, S, w, np.array([1.0, 0.3])), B[:, :2].astype(np.float64))
+    assert np.max(np.abs(X - X64)) <= 2e-3 * np.max(np.abs(X64))
