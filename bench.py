@@ -35,7 +35,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_workload(sess, N, M, maf_lo, maf_hi, miss, seed, prevalences, ncausal=1000, h2=0.3):
+def make_workload(sess, N, M, maf_lo, maf_hi, miss, seed, prevalences, ncausal=1000, h2=0.3, keep=None):
     import numpy as np
     from saige_gpu import plink, synth
 
@@ -53,7 +53,49 @@ def make_workload(sess, N, M, maf_lo, maf_hi, miss, seed, prevalences, ncausal=1
     for k, prev in enumerate(prevalences):
         g = synth.polygenic(Zc, h2, seed + 10 * k)
         phenos.append(synth.binary_phenotype(Xraw, g, prev, seed + 10 * k))
+    if keep is not None:
+        keep["Xraw"] = Xraw
+        keep["Zc"] = Zc
     return dev, X, phenos
+
+
+def serve_leg(sess, N, Xraw, Zc, n_req, rate, seed, opt):
+    """Continuous-batching serving on the resident GRM: a Poisson stream of
+    null-fit requests, each on a random 20-100% sample subset, 70% binary
+    (prevalence U(0.01, 0.3)) and 30% quantitative (h2 U(0.1, 0.6))."""
+    import numpy as np
+    from saige_gpu import driver as D
+    from saige_gpu import plink, synth
+
+    rng = np.random.default_rng(seed)
+    arrivals = []
+    t = 0.0
+    for r in range(n_req):
+        t += rng.exponential(1.0 / rate)
+        sub = np.sort(rng.choice(N, int(rng.uniform(0.2, 1.0) * N), replace=False))
+        Xs, _, _ = plink.covariate_transform(Xraw[sub])
+        if rng.random() < 0.7:
+            g = synth.polygenic(Zc[sub], 0.3, seed + r)
+            y = synth.binary_phenotype(Xraw[sub], g, rng.uniform(0.01, 0.3), seed + r)
+            trait = "binary"
+        else:
+            h2 = rng.uniform(0.1, 0.6)
+            g = synth.polygenic(Zc[sub], h2, seed + r)
+            y = synth.quantitative_phenotype(Xraw[sub], g, h2, seed + r)
+            trait = "quantitative"
+        gen = D.fit_co(trait, y, Xs, D.RademacherProbes(len(sub), 200), opt)
+        arrivals.append((t, gen, sub, (trait, len(sub))))
+    sch = D.BatchScheduler(sess, opt, width=64, n_systems=16)
+    t0 = time.perf_counter()
+    done = sch.run(arrivals)
+    wall = time.perf_counter() - t0
+    ok = [f for f in done if f.error is None]
+    lat = np.array([f.t_done - f.t_submit for f in ok])
+    return {"requests": n_req, "completed": len(ok), "errors": len(done) - len(ok),
+            "offered_rate_per_s": rate, "wall_s": wall, "fits_per_hour": len(ok) / wall * 3600.0,
+            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+            "max_concurrent_fits": sch.n_systems,
+            "slot_occupancy": sch.busy_slot_iters / max(1, sch.n_iterations * sch.width)}
 
 
 def cpu_baseline(sess, dev, N, M, n_matvec_fit, threads, sample_markers, min_seconds=10.0):
@@ -79,6 +121,78 @@ def cpu_baseline(sess, dev, N, M, n_matvec_fit, threads, sample_markers, min_sec
     return per_marker, og["Mqc"], reps
 
 
+def c4_leg(args):
+    """Large related cohort: N=200,000, M=500,000, MAF U(0.01, 0.5), 2% missing,
+    10% of samples in sibling pairs; FP32 GRM (~160 GB) resident; binary trait
+    (prevalence 0.01, h2 0.3); null fit, then variance ratios on 100 markers
+    in each of 4 MAF bins, held out of the GRM and solved as block-PCG
+    right-hand sides."""
+    import numpy as np
+    from saige_gpu import Session, plink, synth
+    from saige_gpu import driver as D
+
+    N, M = args.N, args.M
+    seed = 20261202 + 1000 * 4
+    sess = Session(0)
+    bpm = (N + 3) // 4
+    dev = sess.alloc_device(M * bpm)
+    sess.synth_bed_device(dev, N, M, 0.01, 0.5, 0.02, 0.10, seed)
+    sess.setminMAFforGRM(0.01)
+    sess.setmaxMissingRateforGRM(0.15)
+    sess.setminMAC_VarianceRatio(20, -1, False)
+    sess.setgeno_device(dev, N, M)
+    st = sess.marker_stats()
+    qc = (st["flags"] & 1).astype(bool)
+    maf = np.minimum(st["freq"], 1 - st["freq"])
+    rng = np.random.default_rng(seed)
+    bins = [(0.01, 0.05), (0.05, 0.1), (0.1, 0.25), (0.25, 0.5001)]
+    pool, cats_m = [], []
+    for lo, hi in bins:
+        cand = np.nonzero(qc & (maf >= lo) & (maf < hi) & (st["mac"] >= 20))[0]
+        pick = np.sort(rng.choice(cand, 100, replace=False))
+        pool.append(pick)
+        cats_m.append(pick)
+    pool = np.sort(np.concatenate(pool)).astype(np.int32)
+    sess.setminMAC_VarianceRatio(20, -1, True)
+    sess.set_vr_random_markers(pool)
+    sess.setgeno_device(dev, N, M)              # VR markers leave the GRM (FG.cpp:545-547)
+    Mqc = sess.getMsub_MAFge_minMAFtoConstructGRM()
+    vr_index = sess.getIndexVec_forVarRatio()
+    pos = {int(m): i for i, m in enumerate(vr_index)}
+    categories = [[pos[int(m)] for m in rng.permutation(c) if int(m) in pos] for c in cats_m]
+    causal = np.sort(rng.choice(M, 1000, replace=False))
+    rows = np.stack([sess.copy_device_to_host(dev, bpm, offset=int(j) * bpm) for j in causal])
+    sess.free_device(dev)                       # the .bed body is no longer needed: 25 GB back
+    G, mm = synth.decode_bed_rows(rows, N)
+    Zc = synth.standardized(G, mm)
+    Xraw = synth.covariates(N, seed)
+    X, _, _ = plink.covariate_transform(Xraw)
+    y = synth.binary_phenotype(Xraw, synth.polygenic(Zc, 0.3, seed), 0.01, seed)
+    del G, mm, Zc
+    grm_s = sess.build_grm(args.grm)
+    grm_tflops = N * (N + 1) * Mqc / grm_s / 1e12
+    log(f"C4: Mqc={Mqc}, VR markers {len(vr_index)}, GRM {args.grm} {grm_s:.2f}s {grm_tflops:.0f} TFLOP/s")
+    opt = D.Options(nrun=30)
+    g0, n0 = sess.engine_stats()
+    t0 = time.perf_counter()
+    handles, sch = D.fit_batched(sess, [{"trait": "binary", "y": y, "X": X}], opt)
+    if handles[0].error is not None:
+        raise handles[0].error
+    fit = handles[0].result
+    t1 = time.perf_counter()
+    vr = D.variance_ratio(sess, fit, y, X, "binary", categories, opt, vr_pool=True, num_markers=100)
+    t2 = time.perf_counter()
+    g1, n1 = sess.engine_stats()
+    gemm_avg = (g1 - g0) / max(1, n1 - n0)
+    out = {"config": "C4", "N": N, "M": M, "Mqc": Mqc, "grm_seconds": grm_s, "grm_tflops": grm_tflops,
+           "grm_GB": 4.0 * (N + 255) // 256 * 256 * ((N + 255) // 256 * 256) / 1e9,
+           "fit_seconds": t1 - t0, "theta": list(map(float, fit["theta"])), "vr_seconds": t2 - t1,
+           "variance_ratios": [row[0] for row in vr["table"]], "vr_markers": [len(r) for r in vr["ratios"]],
+           "gemm_avg_ms": gemm_avg * 1e3, "gemm_tflops": 2.0 * N * N * 64 / gemm_avg / 1e12}
+    log(f"C4: fit {t1 - t0:.1f}s theta {fit['theta']}, VR {t2 - t1:.1f}s ratios {out['variance_ratios']}")
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,12 +205,22 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample-markers", type=int, default=4000)
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--config", default="c2", choices=["c2", "c4"])
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no "
               "multi-GPU path", file=sys.stderr)
         sys.exit(2)
+
+    if args.config == "c4":
+        if args.N == 100_000:
+            args.N = 200_000
+        if args.M == 150_000:
+            args.M = 500_000
+        c4_leg(args)
+        return
 
     import numpy as np
     from saige_gpu import Session
@@ -107,7 +231,8 @@ def main():
     prevalences = [0.05, 0.1, 0.2, 0.3]
     sess = Session(0)
     t = time.perf_counter()
-    dev, X, phenos = make_workload(sess, N, M, 0.01, 0.5, 0.01, seed, prevalences)
+    kept = {}
+    dev, X, phenos = make_workload(sess, N, M, 0.01, 0.5, 0.01, seed, prevalences, keep=kept)
     log(f"workload generated in {time.perf_counter() - t:.1f}s")
     sess.setminMAFforGRM(0.01)
     sess.setmaxMissingRateforGRM(0.15)
@@ -151,6 +276,15 @@ def main():
     log(f"batched: {step_s:.2f}s/step, {fits_per_hour:.1f} fits/h, {gemm_launches} GEMMs avg "
         f"{gemm_avg * 1e3:.2f} ms ({gemm_tflops:.1f} TFLOP/s, {4.0 * N * N / gemm_avg / 1e9:.0f} GB/s), "
         f"theta {[list(np.round(h.result['theta'], 4)) for h in handles]}")
+
+    serving = None
+    if args.serve_requests > 0:
+        capacity = len(jobs) / step_s
+        serving = serve_leg(sess, N, kept["Xraw"], kept["Zc"], args.serve_requests, 1.2 * capacity,
+                            seed + 7, opt)
+        log(f"serving: {serving['completed']}/{serving['requests']} fits at offered {serving['offered_rate_per_s']:.2f}/s: "
+            f"{serving['fits_per_hour']:.0f} fits/h, p50 {serving['p50_latency_s']:.2f}s, "
+            f"p99 {serving['p99_latency_s']:.2f}s, slot occupancy {serving['slot_occupancy']:.2f}")
 
     plain = None
     if not args.no_plain:
@@ -202,8 +336,9 @@ def main():
         "grm_seconds": grm_s,
         "plain": plain,
         "batched_vs_plain": (fits_per_hour / plain["fits_per_hour"]) if plain else None,
-        "p50_fit_latency_s": float(np.percentile(lat, 50)),
-        "p99_fit_latency_s": float(np.percentile(lat, 99)),
+        "p50_fit_latency_s": serving["p50_latency_s"] if serving else float(np.percentile(lat, 50)),
+        "p99_fit_latency_s": serving["p99_latency_s"] if serving else float(np.percentile(lat, 99)),
+        "serving": serving,
         "roofline": {"bound": "mfma", "kernel": "k_gemm (block-PCG GRM x 64 RHS, FP32 MFMA)",
                      "achieved": gemm_tflops, "peak": FP32_MFMA_PEAK, "unit": "TFLOP/s",
                      "frac": gemm_tflops / FP32_MFMA_PEAK, "traffic": None,

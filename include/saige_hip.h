@@ -39,8 +39,10 @@ extern "C" {
 #define SAIGE_E_NOTCONV (-6)  /* PCG hit maxiter (result still written)  */
 
 /* GRM arithmetic for saige_build_grm */
-#define SAIGE_GRM_FP32 0      /* FP32 Z, FP32 MFMA (v_mfma_f32_32x32x2_f32) */
-#define SAIGE_GRM_BF16 1      /* BF16 Z, BF16 MFMA with FP32 accumulation  */
+#define SAIGE_GRM_FP32 0        /* FP32 Z panels, FP32 MFMA (v_mfma_f32_32x32x2_f32) */
+#define SAIGE_GRM_BF16 1        /* 2-bit genotypes standardised to BF16 inside the SYRK,
+                                   BF16 MFMA with FP32 accumulation                    */
+#define SAIGE_GRM_BF16_PANEL 2  /* same arithmetic from materialised BF16 Z panels     */
 
 typedef struct saige_ctx saige_ctx;
 
@@ -205,6 +207,10 @@ int saige_engine_take_batch(saige_ctx* ctx, int k, const int32_t* slots, float* 
 int saige_engine_step(saige_ctx* ctx, int n_iter, uint8_t* done_host, int32_t* iters_host);
 /* Copy the solution of a slot to host and free the slot. */
 int saige_engine_take(saige_ctx* ctx, int slot, float* x_host);
+/* Free and total device memory (hipMemGetInfo), for admission control. */
+int saige_device_memory(saige_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);
+/* Device bytes one engine system (w, tau, sample mask) occupies. */
+int64_t saige_engine_system_bytes(const saige_ctx* ctx);
 /* Seconds of device time spent in engine GEMM/GEMV launches, and their count. */
 int saige_engine_stats(const saige_ctx* ctx, double* gemm_seconds, int64_t* gemm_launches);
 

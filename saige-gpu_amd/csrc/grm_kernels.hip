@@ -139,6 +139,143 @@ __global__ __launch_bounds__(256) void k_syrk_bf16(const __hip_bfloat16* __restr
     store_tile(S, ldS, bi, bj, wr, wc, lane, acc, accumulate);
 }
 
+// ------------------------------------------------------- BF16, 2-bit in
+// Same tile and MFMA loop as k_syrk_bf16, but the operands arrive as 2-bit
+// genotypes and are standardised in registers: 8x fewer bytes per element
+// than a BF16 panel, so the K loop is no longer bound by L2-miss traffic.
+//
+// Input layout (built by k_pack_tiles): for each 128-sample panel p and
+// 64-marker chunk c, a 2 KB block [b][m]: byte b (samples 4b..4b+3 of the
+// panel, 2 bits each, low bits first) of marker m.  Code 0b01 marks padding
+// (samples >= Nnm, markers >= Mqc); the internal codes are 00/10/11.
+// lut64[j]: four BF16 values of marker j indexed by the internal code:
+// 00 -> L[2], 01 -> 0, 10 -> L[1], 11 -> L[0].
+//
+// Thread t decodes byte b = t/8 of markers 8*(t%8)..+7: four sample rows of
+// eight consecutive markers, i.e. four 16-byte LDS rows of the operand.
+__device__ __forceinline__ u32x4 decode_row(const uint64_t (&lut)[8], const uint8_t (&byte)[8], int j) {
+    u32x4 r;
+#pragma unroll
+    for (int h = 0; h < 4; h++) {
+        const unsigned c0 = (byte[2 * h] >> (2 * j)) & 3u;
+        const unsigned c1 = (byte[2 * h + 1] >> (2 * j)) & 3u;
+        const unsigned v0 = (unsigned)(lut[2 * h] >> (16 * c0)) & 0xFFFFu;
+        const unsigned v1 = (unsigned)(lut[2 * h + 1] >> (16 * c1)) & 0xFFFFu;
+        r[h] = v0 | (v1 << 16);
+    }
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_syrk_pk(const uint8_t* __restrict__ tp, int64_t nchunk,
+                                                 const uint64_t* __restrict__ lut64,
+                                                 const int2* __restrict__ tiles, int64_t nblk,
+                                                 float* __restrict__ S, int64_t ldS, int accumulate) {
+    __shared__ __attribute__((aligned(16))) __hip_bfloat16 sA[BT * P16];
+    __shared__ __attribute__((aligned(16))) __hip_bfloat16 sB[BT * P16];
+    const int2 t = tiles[xcd_remap(blockIdx.x, nblk)];
+    const int64_t bi = t.x, bj = t.y;
+    const bool diag = bi == bj;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wr = wv >> 1, wc = wv & 1;
+    const int o = tid & 7, b = tid >> 3;          // marker octet, byte within the panel
+    const uint64_t* gA = reinterpret_cast<const uint64_t*>(tp + (bi * nchunk * 32 + b) * 64 + 8 * o);
+    const uint64_t* gB = reinterpret_cast<const uint64_t*>(tp + (bj * nchunk * 32 + b) * 64 + 8 * o);
+    // one chunk = 2048 bytes = 256 uint64
+    uint64_t ra = gA[0], rb = gB[0];
+    const uint64_t* gl = lut64 + 8 * o;
+    uint64_t la[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) la[e] = gl[e];
+
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+        for (int ni = 0; ni < 2; ni++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[mi][ni][r] = 0.0f;
+
+    for (int64_t kc = 0; kc < nchunk; kc++) {
+        uint8_t ba[8], bb[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            ba[e] = (uint8_t)(ra >> (8 * e));
+            bb[e] = (uint8_t)(rb >> (8 * e));
+        }
+        uint64_t lc[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) lc[e] = la[e];
+        // prefetch the next chunk's bytes and LUT entries (re-read the last one at the end)
+        const int64_t kn = (kc + 1 < nchunk) ? kc + 1 : kc;
+        ra = gA[kn * 256];
+        rb = gB[kn * 256];
+#pragma unroll
+        for (int e = 0; e < 8; e++) la[e] = gl[kn * 64 + e];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            *reinterpret_cast<u32x4*>(sA + (4 * b + j) * P16 + 8 * o) = decode_row(lc, ba, j);
+            if (!diag) *reinterpret_cast<u32x4*>(sB + (4 * b + j) * P16 + 8 * o) = decode_row(lc, bb, j);
+        }
+        __syncthreads();
+        const __hip_bfloat16* pB = diag ? sA : sB;
+#pragma unroll
+        for (int ks = 0; ks < BK16 / 16; ks++) {
+            const int kk = ks * 16 + (lane >> 5) * 8;
+            bf16x8 fa[2], fb[2];
+#pragma unroll
+            for (int mi = 0; mi < 2; mi++)
+                fa[mi] = *reinterpret_cast<const bf16x8*>(sA + (wr * 64 + mi * 32 + (lane & 31)) * P16 + kk);
+#pragma unroll
+            for (int ni = 0; ni < 2; ni++)
+                fb[ni] = *reinterpret_cast<const bf16x8*>(pB + (wc * 64 + ni * 32 + (lane & 31)) * P16 + kk);
+#pragma unroll
+            for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+                for (int ni = 0; ni < 2; ni++)
+                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[mi], fb[ni], acc[mi][ni], 0, 0, 0);
+        }
+    }
+    store_tile(S, ldS, bi, bj, wr, wc, lane, acc, accumulate);
+}
+
+// Re-tile the packed genotypes for k_syrk_pk: block (p, c) byte (b, m) =
+// byte 32p + b of QC'd marker 64c + m, with padding codes 0b01.
+__global__ __launch_bounds__(256) void k_pack_tiles(const uint8_t* __restrict__ packed, int64_t Mqc, int64_t Nnm,
+                                                    int64_t bpm_nm, int64_t nchunk, uint8_t* __restrict__ tp) {
+    const int64_t p = blockIdx.x, c = blockIdx.y;
+    uint8_t* blk = tp + (p * nchunk + c) * 2048;
+    for (int i = threadIdx.x; i < 2048; i += 256) {
+        const int b = i >> 6, m = i & 63;
+        const int64_t j = c * 64 + m;
+        const int64_t byte_idx = p * 32 + b;
+        const int64_t s0 = byte_idx * 4;
+        unsigned v = 0x55u;
+        if (j < Mqc && byte_idx < bpm_nm) {
+            v = packed[j * bpm_nm + byte_idx];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (s0 + q >= Nnm) v = (v & ~(3u << (2 * q))) | (1u << (2 * q));
+        }
+        blk[i] = (uint8_t)v;
+    }
+}
+
+// lut64[j] for j < nchunk*64 (zero for padding markers)
+__global__ void k_lut64(const float4* __restrict__ lut, int64_t Mqc, int64_t n, uint64_t* __restrict__ out) {
+    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    uint64_t v = 0;
+    if (j < Mqc) {
+        const float4 L = lut[j];
+        const uint64_t l0 = __bfloat16_as_ushort(__float2bfloat16(L.x));
+        const uint64_t l1 = __bfloat16_as_ushort(__float2bfloat16(L.y));
+        const uint64_t l2 = __bfloat16_as_ushort(__float2bfloat16(L.z));
+        v = l2 | (0ull << 16) | (l1 << 32) | (l0 << 48);   // codes 00, 01, 10, 11
+    }
+    out[j] = v;
+}
+
 // ------------------------------------------------------------------ FP32
 constexpr int BK32 = 32;            // markers per K chunk
 constexpr int P32 = BK32 + 4;       // LDS row pitch (floats)
@@ -222,6 +359,28 @@ __global__ __launch_bounds__(256) void k_syrk_f32(const float* __restrict__ Zp, 
 
 int syrk_kchunk_f32() { return BK32; }
 int syrk_kchunk_bf16() { return BK16; }
+
+hipError_t launch_pack_tiles(hipStream_t s, const uint8_t* packed, const float4* lut, int64_t Mqc, int64_t Nnm,
+                             int64_t Npad, int64_t bpm_nm, uint8_t* tp, uint64_t* lut64) {
+    const int64_t nchunk = (Mqc + 63) / 64;
+    hipLaunchKernelGGL(k_pack_tiles, dim3((unsigned)(Npad / BT), (unsigned)nchunk), dim3(256), 0, s, packed, Mqc,
+                       Nnm, bpm_nm, nchunk, tp);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_lut64, dim3((unsigned)((nchunk * 64 + 255) / 256)), dim3(256), 0, s, lut, Mqc, nchunk * 64,
+                       lut64);
+    return hipGetLastError();
+}
+
+hipError_t launch_syrk_pk(hipStream_t s, const uint8_t* tp, const uint64_t* lut64, int64_t Mqc, int64_t Npad,
+                          const int2* tiles, float* S, int accumulate) {
+    const int64_t nt = Npad / BT;
+    const int64_t nblk = nt * (nt + 1) / 2;
+    const int64_t nchunk = (Mqc + 63) / 64;
+    hipLaunchKernelGGL(k_syrk_pk, dim3((unsigned)nblk), dim3(256), 0, s, tp, nchunk, lut64, tiles, nblk, S, Npad,
+                       accumulate);
+    return hipGetLastError();
+}
 
 int syrk_tile() { return BT; }
 

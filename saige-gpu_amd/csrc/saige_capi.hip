@@ -767,7 +767,7 @@ int saige_get_diag_of_kin(saige_ctx* c, float* out) {
 int saige_build_grm(saige_ctx* c, int precision) {
     CHECK_CTX(c);
     CHECK_LOADED(c);
-    if (precision != SAIGE_GRM_FP32 && precision != SAIGE_GRM_BF16) {
+    if (precision != SAIGE_GRM_FP32 && precision != SAIGE_GRM_BF16 && precision != SAIGE_GRM_BF16_PANEL) {
         set_error("unknown GRM precision %d", precision);
         return SAIGE_E_ARG;
     }
@@ -786,8 +786,41 @@ int saige_build_grm(saige_ctx* c, int precision) {
         }
     }
     c->grm_ready = false;
-    const int64_t esz = precision == SAIGE_GRM_BF16 ? 2 : 4;
-    const int64_t kc = precision == SAIGE_GRM_BF16 ? syrk_kchunk_bf16() : syrk_kchunk_f32();
+    if (precision == SAIGE_GRM_BF16) {
+        // 2-bit operands straight into the MFMA SYRK: no Z panel
+        const int64_t nchunk = (c->Mqc + 63) / 64;
+        uint8_t* d_tp = nullptr;
+        uint64_t* d_l64 = nullptr;
+        int2* d_tiles = nullptr;
+        const std::vector<int2> order = syrk_tile_order(Np / syrk_tile(), 8);
+        auto release = [&]() { dfree(d_tp); dfree(d_l64); dfree(d_tiles); };
+        if (dalloc(&d_tp, (Np / syrk_tile()) * nchunk * 2048) || dalloc(&d_l64, nchunk * 64) ||
+            dalloc(&d_tiles, (int64_t)order.size())) {
+            release();
+            set_error("cannot allocate the tiled genotypes for the SYRK");
+            return SAIGE_E_NOMEM;
+        }
+        hipError_t e = hipMemcpyAsync(d_tiles, order.data(), sizeof(int2) * order.size(), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipEventRecord(c->ev0, st);
+        if (e == hipSuccess) e = launch_pack_tiles(st, c->d_packed, c->d_lut, c->Mqc, c->Nnm, Np, c->bpm_nm, d_tp, d_l64);
+        if (e == hipSuccess) e = launch_syrk_pk(st, d_tp, d_l64, c->Mqc, Np, d_tiles, c->d_grm, 0);
+        if (e == hipSuccess) e = hipEventRecord(c->ev1, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        release();
+        if (e != hipSuccess) {
+            set_error("GRM build failed: %s", hipGetErrorString(e));
+            return SAIGE_E_DEVICE;
+        }
+        float ms = 0.f;
+        SAIGE_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->grm_seconds = ms * 1e-3;
+        c->grm_ready = true;
+        c->grm_precision = precision;
+        return SAIGE_OK;
+    }
+    const bool bf16 = precision == SAIGE_GRM_BF16_PANEL;
+    const int64_t esz = bf16 ? 2 : 4;
+    const int64_t kc = bf16 ? syrk_kchunk_bf16() : syrk_kchunk_f32();
     size_t free_b = 0, total_b = 0;
     SAIGE_HIP_TRY(hipMemGetInfo(&free_b, &total_b));
     // panel budget: at most 32 GB and half of what is free
@@ -819,7 +852,7 @@ int saige_build_grm(saige_ctx* c, int precision) {
     for (int64_t j0 = 0; j0 < c->Mqc && e == hipSuccess; j0 += pw) {
         const int64_t np = std::min(pw, c->Mqc - j0);
         const int64_t ldz = round_up(np, 64);
-        if (precision == SAIGE_GRM_BF16) {
+        if (bf16) {
             e = launch_std_panel_bf16(st, c->d_packed, c->d_lut, j0, np, c->Nnm, Np, c->bpm_nm, ldz,
                                       static_cast<__hip_bfloat16*>(d_panel));
             if (e == hipSuccess)
@@ -1138,6 +1171,20 @@ int saige_engine_stats(const saige_ctx* c, double* secs, int64_t* launches) {
     if (secs) *secs = c->eng.gemm_seconds + c->eng1.gemm_seconds;
     if (launches) *launches = c->eng.gemm_launches + c->eng1.gemm_launches;
     return SAIGE_OK;
+}
+
+int saige_device_memory(saige_ctx* c, int64_t* free_bytes, int64_t* total_bytes) {
+    CHECK_CTX(c);
+    size_t f = 0, t = 0;
+    SAIGE_HIP_TRY(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = (int64_t)f;
+    if (total_bytes) *total_bytes = (int64_t)t;
+    return SAIGE_OK;
+}
+
+int64_t saige_engine_system_bytes(const saige_ctx* c) {
+    // winv + minv (FP32) and the sample mask of one system
+    return c ? c->Npad * (2 * 4 + 1) + 8 : 0;
 }
 
 int saige_synth_bed_device(saige_ctx* c, void* d_bed, int64_t N, int64_t M, float maf_lo, float maf_hi,

@@ -181,6 +181,12 @@ hipError_t launch_syrk_f32(hipStream_t s, const float* Zp, int64_t Npad, int64_t
 hipError_t launch_syrk_bf16(hipStream_t s, const __hip_bfloat16* Zp, int64_t Npad, int64_t ldz,
                             int64_t kdim, const int2* tiles, float* S, int accumulate);
 std::vector<int2> syrk_tile_order(int64_t nt, int64_t g);
+// 2-bit-input SYRK: tp = tiled packed genotypes ((Npad/128) x nchunk x 2 KB,
+// nchunk = ceil(Mqc/64)), lut64 = nchunk*64 entries; both from launch_pack_tiles
+hipError_t launch_pack_tiles(hipStream_t s, const uint8_t* packed, const float4* lut, int64_t Mqc, int64_t Nnm,
+                             int64_t Npad, int64_t bpm_nm, uint8_t* tp, uint64_t* lut64);
+hipError_t launch_syrk_pk(hipStream_t s, const uint8_t* tp, const uint64_t* lut64, int64_t Mqc, int64_t Npad,
+                          const int2* tiles, float* S, int accumulate);
 int syrk_tile();
 int syrk_kchunk_f32();
 int syrk_kchunk_bf16();

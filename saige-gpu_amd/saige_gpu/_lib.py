@@ -22,6 +22,7 @@ SAIGE_E_NOTCONV = -6
 
 GRM_FP32 = 0
 GRM_BF16 = 1
+GRM_BF16_PANEL = 2
 
 _c_p = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -77,6 +78,8 @@ SIGNATURES = {
     "saige_engine_step": (_int, [_c_p, _int, _c_p, _c_p]),
     "saige_engine_take": (_int, [_c_p, _int, _c_p]),
     "saige_engine_stats": (_int, [_c_p, ctypes.POINTER(_f64), ctypes.POINTER(_i64)]),
+    "saige_device_memory": (_int, [_c_p, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+    "saige_engine_system_bytes": (_i64, [_c_p]),
     "saige_synth_bed_device": (_int, [_c_p, _c_p, _i64, _i64, _f32, _f32, _f32, _f32, _u64]),
     "saige_alloc_device": (_int, [_c_p, _i64, ctypes.POINTER(_c_p)]),
     "saige_free_device": (_int, [_c_p, _c_p]),

@@ -174,7 +174,7 @@ class Session:
 
     # ------------------------------------------------------------ GRM
     def build_grm(self, precision="fp32"):
-        prec = {"fp32": _lib.GRM_FP32, "bf16": _lib.GRM_BF16}[precision]
+        prec = {"fp32": _lib.GRM_FP32, "bf16": _lib.GRM_BF16, "bf16_panel": _lib.GRM_BF16_PANEL}[precision]
         check(self._L.saige_build_grm(self.ctx, prec))
         return float(self._L.saige_last_grm_seconds(self.ctx))
 
@@ -299,6 +299,15 @@ class Session:
         n = ctypes.c_int64(0)
         check(self._L.saige_engine_stats(self.ctx, ctypes.byref(s), ctypes.byref(n)))
         return float(s.value), int(n.value)
+
+    def device_memory(self):
+        f = ctypes.c_int64(0)
+        t = ctypes.c_int64(0)
+        check(self._L.saige_device_memory(self.ctx, ctypes.byref(f), ctypes.byref(t)))
+        return int(f.value), int(t.value)
+
+    def engine_system_bytes(self):
+        return int(self._L.saige_engine_system_bytes(self.ctx))
 
     # ------------------------------------------------------ device helpers
     def alloc_device(self, nbytes):

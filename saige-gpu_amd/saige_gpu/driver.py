@@ -126,6 +126,17 @@ class RademacherProbes:
             self._cache[i] = v
         return v
 
+    def block(self, start, end):
+        """Probes start..end-1 as an n x (end-start) array (cached)."""
+        key = ("block", start, end)
+        U = self._cache.get(key)
+        if U is None:
+            U = np.empty((self.n, end - start), order="F")
+            for c, i in enumerate(range(start, end)):
+                U[:, c] = self(i)
+            self._cache[key] = U
+        return U
+
 
 # -------------------------------------------------- solver-level coroutines
 
@@ -150,6 +161,23 @@ def _inv_sympd(A):
         return np.linalg.pinv(A)
 
 
+def cols(*parts):
+    """Column-major block from vectors and column blocks (the layout the
+    C ABI takes, so no transposes on the way to the device)."""
+    n = parts[0].shape[0]
+    k = sum(1 if p.ndim == 1 else p.shape[1] for p in parts)
+    out = np.empty((n, k), order="F")
+    c = 0
+    for p in parts:
+        if p.ndim == 1:
+            out[:, c] = p
+            c += 1
+        else:
+            out[:, c:c + p.shape[1]] = p
+            c += p.shape[1]
+    return out
+
+
 def cal_cv(x):
     """calCV (FG.cpp:3104-3110)."""
     x = np.asarray(x, np.float64)
@@ -158,9 +186,9 @@ def cal_cv(x):
 
 def get_coefficients_co(Y, X, w, tau, opt):
     """getCoefficients (FG.cpp:3167-3199): Sigma^-1 [Y, X] in one request."""
-    sol = yield ("solve", w, tau, np.column_stack([Y, X]))
-    Sigma_iY = sol[:, 0]
-    Sigma_iX = sol[:, 1:]
+    sol = yield ("solve", w, tau, cols(Y, X))
+    Sigma_iY = np.ascontiguousarray(sol[:, 0])
+    Sigma_iX = np.ascontiguousarray(sol[:, 1:])
     cov = _inv_sympd(X.T @ Sigma_iX)
     alpha = cov @ (Sigma_iX.T @ Y)
     eta = Y - tau[0] * (Sigma_iY - Sigma_iX @ alpha) / w
@@ -208,13 +236,14 @@ def ai_score_co(Y, X, w, tau, Sigma_iY, Sigma_iX, cov, probes, opt, quantitative
     APY = None
     extra = None
     while True:
-        U = np.column_stack([probes(i) for i in range(start, end)])
+        U = probes.block(start, end) if hasattr(probes, "block") else \
+            np.column_stack([probes(i) for i in range(start, end)])
         if APY is None:
-            KB = yield ("matvec", np.column_stack([PY, U]))
+            KB = yield ("matvec", cols(PY, U))
             APY = KB[:, 0]
             AU = KB[:, 1:]
-            rhs = [U, APY[:, None]] + ([PY[:, None]] if quantitative else [])
-            sol = yield ("solve", w, tau, np.column_stack(rhs))
+            rhs = [U, APY] + ([PY] if quantitative else [])
+            sol = yield ("solve", w, tau, cols(*rhs))
             k = U.shape[1]
             Sigma_iU = sol[:, :k]
             extra = sol[:, k:]
@@ -380,6 +409,83 @@ def fit_co(trait, y, X, probes, opt, **kw):
     raise ValueError(f"unknown trait type {trait!r}")
 
 
+# ---------------------------------------------------------- variance ratio
+
+def score_test_null(mu, mu2, y, X):
+    """ScoreTest_NULL_Model (FG.R:579-591)."""
+    V = np.asarray(mu2, np.float64)
+    XV = (X * V[:, None]).T
+    XVX = X.T @ XV.T
+    XVX_inv = np.linalg.inv(XVX)
+    XXVX_inv = X @ XVX_inv
+    return {"XV": XV, "XVX": XVX, "XXVX_inv": XXVX_inv, "XVX_inv": XVX_inv,
+            "S_a": (X * (y - mu)[:, None]).sum(0), "V": V}
+
+
+def variance_ratio_co(fit, y, X, trait, genos, categories, opt, num_markers=30, ratio_cv_cutoff=0.001):
+    """The variance-ratio core of extractVarianceRatio (FG.R:2152-2423) for a
+    fitted null model.
+
+    genos(i): integer genotypes (length n) of candidate marker i, imputed as
+    repacked (Get_OneSNP_Geno[_forVarRatio], FG.cpp:2281-2300).
+    categories: one list of candidate ids per MAC/MAF category, already in
+    the random order the reference draws with sample() (FG.R:2239-2281).
+    Each round's Sigma^-1 G solves go out as one block request.  Returns
+    the table rows (ratio, "null", k) and the per-marker ratios."""
+    y = np.asarray(y, np.float64)
+    X = np.asarray(X, np.float64)
+    fam = Binomial if trait == "binary" else Gaussian
+    eta = np.asarray(fit["linear_predictors"], np.float64)
+    mu = np.asarray(fit["fitted_values"], np.float64)
+    W = (fam.mu_eta(eta) / np.sqrt(fam.variance(mu))) ** 2            # FG.R:2194-2196
+    tau = np.asarray(fit["theta"], np.float64)
+    Sigma_iX = yield ("solve", W, tau, cols(X))                        # getSigma_X, FG.R:2200
+    noK = score_test_null(mu, mu * (1.0 - mu), y, X)
+    XtSiX_inv = np.linalg.inv(X.T @ Sigma_iX)
+    n = len(y)
+    table, per_cat = [], []
+    for k, cand in enumerate(categories):
+        ratios = []
+        idx = 0
+        target = num_markers
+        cv = ratio_cv_cutoff + 0.1
+        while cv > ratio_cv_cutoff:
+            batch = []
+            while len(ratios) + len(batch) < target and idx < len(cand):
+                G0 = np.asarray(genos(cand[idx]), np.float64)
+                idx += 1
+                if G0.sum() / (2 * n) > 0.5:                           # FG.R:2306-2308
+                    G0 = 2.0 - G0
+                AC = G0.sum()
+                G = G0 - noK["XXVX_inv"] @ (noK["XV"] @ G0)            # FG.R:2330
+                batch.append((G, AC))
+            if batch:
+                SiG = yield ("solve", W, tau, cols(*[b[0] for b in batch]))   # getSigma_G, FG.R:2338
+                for c, (G, AC) in enumerate(batch):
+                    sg = SiG[:, c]
+                    var1a = G @ sg - (G @ Sigma_iX) @ XtSiX_inv @ (X.T @ sg)    # FG.R:2341
+                    var1 = var1a / AC
+                    g = G / np.sqrt(AC)
+                    var2 = np.sum(mu * (1.0 - mu) * g * g) if trait == "binary" else g @ g   # FG.R:2360-2364
+                    ratios.append(var1 / var2)
+            cv = cal_cv(ratios)
+            if cv > ratio_cv_cutoff:
+                target += 10                                           # FG.R:2381-2384
+            if idx == len(cand):
+                cv = ratio_cv_cutoff                                   # FG.R:2389-2393
+        table.append((float(np.mean(ratios)), "null", k + 1))
+        per_cat.append(np.array(ratios))
+    return {"table": table, "ratios": per_cat}
+
+
+def write_variance_ratio(path, table):
+    """<prefix>.varianceRatio.txt as write.table(quote=F, col.names=F,
+    row.names=F) prints it (FG.R:2416-2418): "value type category"."""
+    with open(path, "w") as f:
+        for v, kind, k in table:
+            f.write(f"{v:.15g} {kind} {k}\n")
+
+
 # --------------------------------------------------------------- executors
 
 class PlainExecutor:
@@ -451,12 +557,18 @@ class BatchScheduler:
     the next iteration's GEMM.  A column's arithmetic does not depend on its
     neighbours, so each fit gets the same result alone and in the mix."""
 
-    def __init__(self, sess, opt, width=64, n_systems=16):
+    def __init__(self, sess, opt, width=64, n_systems=16, hbm_reserve_bytes=8 << 30):
         self.sess = sess
         self.opt = opt
         self.width = width
-        self.n_systems = n_systems
         self.N = sess.getNnomissingOut()
+        # admission by free HBM: never hold more systems than half of what
+        # is free beyond a reserve allows
+        free, _ = sess.device_memory()
+        per = sess.engine_system_bytes()
+        by_mem = max(1, int(max(0, free - hbm_reserve_bytes) // 2 // max(1, per)))
+        self.n_systems = n_systems = max(1, min(n_systems, by_mem))
+        self.hbm_reserve_bytes = hbm_reserve_bytes
         sess.engine_reserve(width, n_systems)
         self.free_slots = collections.deque(range(width))
         self.free_sys = collections.deque(range(n_systems))
@@ -502,6 +614,10 @@ class BatchScheduler:
 
     # ----------------------------------------------------------- internals
     def _admit(self):
+        if self.waiting and self.free_sys:
+            free, _ = self.sess.device_memory()
+            if free < self.hbm_reserve_bytes:
+                return
         while self.waiting and self.free_sys:
             f = self.waiting.popleft()
             f.sys = self.free_sys.popleft()
@@ -542,7 +658,7 @@ class BatchScheduler:
                 _, w, tau, B = req
                 self._set_system(f, w, tau)
                 kind = "solve"
-            f.result_buf = np.zeros(B.shape, np.float64)
+            f.result_buf = np.zeros(B.shape, np.float64, order="F")
             f.remaining = B.shape[1]
             for j in range(B.shape[1]):
                 f.pending.append((j, B[:, j], kind))
@@ -636,6 +752,21 @@ def fit_plain(sess, trait, y, X, opt=None, probe_seed=200, **kw):
     res["pcg_iters"] = ex.pcg_iters
     res["n_products"] = ex.n_products
     return res
+
+
+def variance_ratio(sess, fit, y, X, trait, categories, opt=None, vr_pool=True, num_markers=30,
+                   ratio_cv_cutoff=0.001, width=64):
+    """Run variance_ratio_co through the batch scheduler (all markers of a
+    round in one block PCG).  vr_pool: candidates index the markers held
+    out of the GRM (Get_OneSNP_Geno_forVarRatio) rather than GRM markers."""
+    opt = opt or Options()
+    get = sess.Get_OneSNP_Geno_forVarRatio if vr_pool else sess.Get_OneSNP_Geno
+    sch = BatchScheduler(sess, opt, width=width, n_systems=1)
+    h = sch.submit(variance_ratio_co(fit, y, X, trait, get, categories, opt, num_markers, ratio_cv_cutoff))
+    sch.run()
+    if h.error is not None:
+        raise h.error
+    return h.result
 
 
 def fit_batched(sess, jobs, opt=None, width=64, n_systems=16, probe_seed=200):

@@ -94,3 +94,49 @@ def test_subset_fit_equals_restricted_problem(session, cohort):
     gsub.kdiag = grm.kdiag[sub]
     ref = _oracle_fit(gsub, trait, y[sub], Xs, opt)
     assert np.allclose(handles[0].result["theta"], ref["theta"], rtol=1e-3, atol=1e-6)
+
+
+def test_variance_ratio_vs_oracle(session):
+    """Null fit + variance ratio (FG.R:2152-2423) with variance-ratio markers
+    held out of the GRM (FG.cpp:545-547), against the FP64 oracle on the
+    same held-out genotypes and the same null model."""
+    print("case: variance ratio", flush=True)
+    c = synth.small_cohort(1200, 3000, maf_lo=0.02, maf_hi=0.5, miss_rate=0.005, seed=synth.config_seed(4),
+                           ncausal=300)
+    rng = np.random.default_rng(11)
+    pool = np.sort(rng.choice(c["M"], 120, replace=False)).astype(np.int32)
+    flags = np.zeros(c["M"], np.uint8)
+    flags[pool] = 1
+    session.setminMAFforGRM(0.01)
+    session.setmaxMissingRateforGRM(0.15)
+    session.setminMAC_VarianceRatio(20, -1, True)
+    session.set_vr_random_markers(pool)
+    try:
+        session.setgeno_bytes(c["bed"], c["N"], c["M"])
+        session.build_grm("fp32")
+        og = O.setgeno(c["bed"], c["N"], c["M"], min_maf=0.01, max_missing=0.15, vr_mode=2, min_mac_vr=20,
+                       randset=flags)
+        assert np.array_equal(session.getIndexVec_forVarRatio(), og["index_vr"])
+        grm = O.DenseGRM(O.grm_f64(og), og["Mqc"], O.diag_std(og))
+        X, _, _ = plink.covariate_transform(c["X"])
+        y = synth.binary_phenotype(c["X"], synth.polygenic(c["Zc"], 2.0, 5), 0.25, 5)
+        opt = D.Options()
+        fit = D.fit_plain(session, "binary", y, X, opt)
+        nvr = len(og["index_vr"])
+        order = rng.permutation(nvr)
+        cats = [list(order)]
+        out = D.variance_ratio(session, fit, y, X, "binary", cats, opt, vr_pool=True, num_markers=30)
+        mu = fit["fitted_values"]
+        w = mu * (1 - mu)
+        Sigma_iX, _ = grm.pcg(w, fit["theta"], X, 500, 1e-5)
+        noK = O.score_test_null(mu, mu * (1 - mu), y, X)
+        ref = [O.variance_ratio_marker(grm, O.get_geno(og["packed_vr"][i], c["N"]), X, y, mu, w, fit["theta"],
+                                       Sigma_iX, noK["XV"], noK["XXVX_inv"], "binary", 500, 1e-5)
+               for i in order[:len(out["ratios"][0])]]
+        print("  ratio", out["table"][0][0], "oracle", np.mean(ref), "markers", len(ref))
+        assert len(ref) >= 30
+        assert np.allclose(out["ratios"][0], ref, rtol=2e-3)
+        assert 0.5 < out["table"][0][0] < 1.5
+    finally:
+        session.setminMAC_VarianceRatio(20, -1, False)
+        session.set_vr_random_markers(np.zeros(0, np.int32))

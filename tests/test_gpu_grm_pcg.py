@@ -45,7 +45,8 @@ def fp32_tol(mqc):
 
 
 @pytest.mark.parametrize("prec,tol,panel", [("fp32", None, None), ("fp32", None, "2048"),
-                                            ("bf16", 4e-3, None), ("bf16", 4e-3, "4096")])
+                                            ("bf16", 4e-3, None), ("bf16_panel", 4e-3, None),
+                                            ("bf16_panel", 4e-3, "4096")])
 def test_grm_syrk(session, grm1000, monkeypatch, prec, tol, panel):
     og, S = grm1000
     print(f"case: SYRK {prec} panel={panel}", flush=True)
@@ -63,6 +64,18 @@ def test_grm_syrk(session, grm1000, monkeypatch, prec, tol, panel):
     assert np.array_equal(K, K.T)          # mirrored tiles are exact copies
     # diagonal against the sequential FP32 diag (independent computation)
     assert np.allclose(np.diag(K), session.get_DiagofKin(), rtol=10 * tol)
+
+
+def test_bf16_two_bit_input_equals_panels(session, grm1000, monkeypatch):
+    """The 2-bit-input SYRK decodes to the same BF16 operands as the
+    materialised panel path and runs the same MFMA loop: bitwise equal."""
+    print("case: SYRK bf16 2-bit vs panel", flush=True)
+    monkeypatch.delenv("SAIGE_GRM_PANEL_MARKERS", raising=False)
+    session.build_grm("bf16_panel")
+    Kp = session.grm_block(0, 0, 1000, 1000)
+    session.build_grm("bf16")
+    Kb = session.grm_block(0, 0, 1000, 1000)
+    assert np.array_equal(Kp.view(np.uint32), Kb.view(np.uint32))
 
 
 def test_grm_products(session, grm1000):

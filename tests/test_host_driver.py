@@ -87,3 +87,39 @@ def test_pack_bed_round_trip():
     G2, m2 = decode_bed_rows(body, 23)
     assert np.array_equal(m2, miss)
     assert np.array_equal(np.where(miss, 0, G2), np.where(miss, 0, G))
+
+
+def test_variance_ratio_matches_oracle(cohort):
+    """variance_ratio_co (FG.R:2152-2423 core) against the oracle's per-marker
+    formula on the same markers, with a dense FP64 test executor."""
+    grm, X, yb, yq = cohort
+    opt = D.Options(nrun=10, traceCVcutoff=1.0, tolPCG=1e-10, maxiterPCG=2000)
+    probes = D.RademacherProbes(len(yb), 200)
+    ex = DenseExecutor(grm, opt)
+    fit = ex.run(D.fit_co("binary", yb, X, probes, opt))
+    rng = np.random.default_rng(5)
+    Gpool = rng.binomial(2, rng.uniform(0.05, 0.5, 80)[None, :], (len(yb), 80))
+    cats = [list(range(0, 40)), list(range(40, 80))]
+    out = ex.run(D.variance_ratio_co(fit, yb, X, "binary", lambda i: Gpool[:, i], cats, opt,
+                                     num_markers=12, ratio_cv_cutoff=1.0))
+    mu = fit["fitted_values"]
+    w = mu * (1 - mu)
+    Sigma_iX, _ = grm.pcg(w, fit["theta"], X, 2000, 1e-10)
+    noK = O.score_test_null(mu, mu * (1 - mu), yb, X)
+    for k, cat in enumerate(cats):
+        assert len(out["ratios"][k]) == 12
+        ref = [O.variance_ratio_marker(grm, Gpool[:, i], X, yb, mu, w, fit["theta"], Sigma_iX, noK["XV"],
+                                       noK["XXVX_inv"], "binary", 2000, 1e-10) for i in cat[:12]]
+        assert np.allclose(out["ratios"][k], ref, rtol=1e-6)
+        assert np.isclose(out["table"][k][0], np.mean(ref), rtol=1e-6)
+    # CV growth: a cutoff below the 12-marker CV adds markers in steps of 10
+    out2 = ex.run(D.variance_ratio_co(fit, yb, X, "binary", lambda i: Gpool[:, i], cats[:1], opt,
+                                      num_markers=12, ratio_cv_cutoff=0.0))
+    assert len(out2["ratios"][0]) == 40          # runs out of markers: 12, 22, 32, 40
+    import tempfile, os
+    with tempfile.TemporaryDirectory() as d:
+        pth = os.path.join(d, "vr.txt")
+        D.write_variance_ratio(pth, out["table"])
+        lines = open(pth).read().split("\n")
+        assert lines[0].split()[1:] == ["null", "1"]
+        assert abs(float(lines[0].split()[0]) - out["table"][0][0]) <= 1e-14 * abs(out["table"][0][0])
