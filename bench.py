@@ -172,6 +172,8 @@ def c4_leg(args):
     grm_s = sess.build_grm(args.grm)
     grm_tflops = N * (N + 1) * Mqc / grm_s / 1e12
     log(f"C4: Mqc={Mqc}, VR markers {len(vr_index)}, GRM {args.grm} {grm_s:.2f}s {grm_tflops:.0f} TFLOP/s")
+    tune = sess.autotune(reps=3)
+    log(f"C4 autotune: rt{tune['rt']}/s{tune['nsplit']} {tune['ms']:.2f} ms")
     opt = D.Options(nrun=30)
     g0, n0 = sess.engine_stats()
     t0 = time.perf_counter()
@@ -188,7 +190,8 @@ def c4_leg(args):
            "grm_GB": 4.0 * (N + 255) // 256 * 256 * ((N + 255) // 256 * 256) / 1e9,
            "fit_seconds": t1 - t0, "theta": list(map(float, fit["theta"])), "vr_seconds": t2 - t1,
            "variance_ratios": [row[0] for row in vr["table"]], "vr_markers": [len(r) for r in vr["ratios"]],
-           "gemm_avg_ms": gemm_avg * 1e3, "gemm_tflops": 2.0 * N * N * 64 / gemm_avg / 1e12}
+           "gemm_avg_ms": gemm_avg * 1e3, "gemm_tflops": 2.0 * N * N * 64 / gemm_avg / 1e12,
+           "gemm_autotune": tune}
     log(f"C4: fit {t1 - t0:.1f}s theta {fit['theta']}, VR {t2 - t1:.1f}s ratios {out['variance_ratios']}")
     print(json.dumps(out))
 
@@ -207,6 +210,8 @@ def main():
     ap.add_argument("--cpu-sample-markers", type=int, default=4000)
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--config", default="c2", choices=["c2", "c4"])
+    ap.add_argument("--loco-chromosomes", type=int, default=22,
+                    help="leave-one-chromosome-out fits for this many autosomes (0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -243,6 +248,9 @@ def main():
     grm_s = sess.build_grm(args.grm)
     grm_tflops = N * (N + 1) * Mqc / grm_s / 1e12
     log(f"GRM {args.grm}: {grm_s:.3f}s device time, {grm_tflops:.1f} TFLOP/s")
+    tune = sess.autotune(reps=3)
+    log("autotune block GEMM: " + ", ".join(f"rt{a}/s{b} {t:.2f}ms" for a, b, t in tune["table"]) +
+        f" -> rt{tune['rt']}/s{tune['nsplit']}")
 
     opt = D.Options(nrun=args.nrun)
     jobs = [{"trait": "binary", "y": y, "X": X, "tag": f"prev{p}"} for y, p in zip(phenos, prevalences)]
@@ -285,6 +293,33 @@ def main():
         log(f"serving: {serving['completed']}/{serving['requests']} fits at offered {serving['offered_rate_per_s']:.2f}/s: "
             f"{serving['fits_per_hour']:.0f} fits/h, p50 {serving['p50_latency_s']:.2f}s, "
             f"p99 {serving['p99_latency_s']:.2f}s, slot occupancy {serving['slot_occupancy']:.2f}")
+
+    loco = None
+    if args.loco_chromosomes > 0:
+        # the QC'd markers cut into autosomes in proportion to their lengths (Mb)
+        lengths = np.array([248, 242, 198, 190, 181, 171, 159, 145, 138, 134, 135, 133, 114, 107, 102, 90,
+                            83, 80, 59, 64, 47, 51], np.float64)
+        bounds = np.round(np.cumsum(lengths) / lengths.sum() * Mqc).astype(np.int64)
+        chr_vec = np.repeat(np.arange(1, 23), np.diff(np.r_[0, bounds]))
+        _, starts, ends = D.update_chr_start_end_index(chr_vec)
+        nc = args.loco_chromosomes
+        starts = starts[:nc] + [None] * (22 - nc)
+        ends = ends[:nc] + [None] * (22 - nc)
+        ljobs = [{"fit": h.result, "trait": "binary", "y": j["y"], "X": j["X"]} for h, j in zip(handles, jobs)]
+        import collections
+        tm = collections.Counter()
+        tl = time.perf_counter()
+        lres = D.loco(sess, ljobs, starts, ends, opt, batched=True, n_systems=len(ljobs), timing=tm)
+        loco_s = time.perf_counter() - tl
+        loco = {"chromosomes": nc, "fits": len(ljobs), "seconds": loco_s,
+                "seconds_per_chromosome": loco_s / nc,
+                "set_range_s_per_chromosome": tm["set_range"] / nc,
+                "coef_s_per_chromosome": tm["coef"] / nc,
+                "loco_fits_per_hour": len(ljobs) * nc / loco_s * 3600.0,
+                "max_abs_coef_shift": float(max(np.max(np.abs(r["coefficients"] - h.result["coefficients"]))
+                                                for rr, h in zip(lres, handles) for r in rr if r["isLOCO"]))}
+        log(f"LOCO: {nc} chromosomes x {len(ljobs)} fits in {loco_s:.1f}s "
+            f"({tm['set_range'] / nc:.3f}s range build + {tm['coef'] / nc:.3f}s Get_Coef_LOCO per chromosome)")
 
     plain = None
     if not args.no_plain:
@@ -334,11 +369,13 @@ def main():
                    "N": N, "M": M, "Mqc": Mqc, "phenotypes": len(jobs), "nrun": args.nrun, "grm": args.grm},
         "grm_tflops": grm_tflops,
         "grm_seconds": grm_s,
+        "gemm_autotune": tune,
         "plain": plain,
         "batched_vs_plain": (fits_per_hour / plain["fits_per_hour"]) if plain else None,
         "p50_fit_latency_s": serving["p50_latency_s"] if serving else float(np.percentile(lat, 50)),
         "p99_fit_latency_s": serving["p99_latency_s"] if serving else float(np.percentile(lat, 99)),
         "serving": serving,
+        "loco": loco,
         "roofline": {"bound": "mfma", "kernel": "k_gemm (block-PCG GRM x 64 RHS, FP32 MFMA)",
                      "achieved": gemm_tflops, "peak": FP32_MFMA_PEAK, "unit": "TFLOP/s",
                      "frac": gemm_tflops / FP32_MFMA_PEAK, "traffic": None,

@@ -173,6 +173,36 @@ int saige_pcg1(saige_ctx* ctx, const float* w_host, const float* tau, const floa
 int saige_pcg_block(saige_ctx* ctx, const float* w_host, const float* tau, const float* B_host,
                     int64_t k, int maxiter, float tol, float* X_host, int32_t* iters);
 
+/* ------------------------------------------------ leave one chromosome out
+ * setStartEndIndex (FG.cpp:3067-3082) together with set_Diagof_StdGeno_LOCO
+ * (FG.cpp:4934-4958) for that one range: GRM markers [start, end] (indices
+ * into the QC'd marker list, inclusive, as the reference's loops use them)
+ * are left out.  Builds, on the device, S_loco = S - Z_r Z_r^T (a copy of the
+ * resident GRM minus the range's SYRK, same kernel and precision as
+ * saige_build_grm) and diag_loco = diag - sum_{range} z^2 (FP32, marker
+ * order), with divisor M_loco = Mqc - (end - start + 1) as in
+ * parallelCrossProd_LOCO (FG.cpp:1833-1849) and getDiagOfSigma_LOCO
+ * (FG.cpp:2368-2392).  chrom is the reference's chromIndex; it is recorded
+ * and returned by saige_loco_range but the range alone defines the GRM.
+ * Needs a second Npad^2 FP32 buffer (kept until saige_destroy).
+ * SAIGE_E_STATE if the block engine has live LOCO slots. */
+int saige_set_start_end_index(saige_ctx* ctx, int64_t start, int64_t end, int chrom);
+/* The range set last: start, end, chrom, M_loco (-1s if none). */
+int saige_loco_range(const saige_ctx* ctx, int64_t* start, int64_t* end, int* chrom,
+                     int64_t* m_loco);
+/* getCrossprodMatAndKin_LOCO (FG.cpp:1989-2000): K_loco b. */
+int saige_crossprod_mat_and_kin_loco(saige_ctx* ctx, const float* b_host, float* out_host);
+/* getDiagOfSigma_LOCO (FG.cpp:2368-2392); ignores isDiagofKinSetAsOne as the
+ * reference does. */
+int saige_get_diag_of_sigma_loco(saige_ctx* ctx, const float* w_host, const float* tau,
+                                 float* out_host);
+/* getCrossprod_LOCO (FG.cpp:2430-2460): Sigma_loco b. */
+int saige_get_crossprod_loco(saige_ctx* ctx, const float* b_host, const float* w_host,
+                             const float* tau, float* out_host);
+/* getPCG1ofSigmaAndVector_LOCO (FG.cpp:2943-3030): as saige_pcg1 on Sigma_loco. */
+int saige_pcg1_loco(saige_ctx* ctx, const float* w_host, const float* tau, const float* b_host,
+                    int maxiter, float tol, float* x_host, int32_t* iters);
+
 /* ------------------------------------------- block-PCG slot engine
  * The engine keeps up to `width` (<= 64) solves in flight over shared K.
  * Each slot belongs to a "system" (w, tau, optional sample mask), so many
@@ -207,6 +237,30 @@ int saige_engine_take_batch(saige_ctx* ctx, int k, const int32_t* slots, float* 
 int saige_engine_step(saige_ctx* ctx, int n_iter, uint8_t* done_host, int32_t* iters_host);
 /* Copy the solution of a slot to host and free the slot. */
 int saige_engine_take(saige_ctx* ctx, int slot, float* x_host);
+/* Point the block engine at the LOCO GRM of saige_set_start_end_index
+ * (on = 1) or back at the full GRM (on = 0); the Get_Coef_LOCO solves of many
+ * fits (FG.R:1240-1244) then run batched.  Call with no live slots, before
+ * saige_engine_set_system (the systems' preconditioner uses diag_loco). */
+int saige_engine_use_loco(saige_ctx* ctx, int on);
+/* ------------------------------------------------------------ autotuner
+ * No reference counterpart (the reference's GRM product is one cuBLAS
+ * SGEMV per right-hand side, GPU.cu:211-252).  The block-PCG GEMM
+ * Y = S P (64 columns) has two shape parameters: rt, the 32-row tiles per
+ * wave (1 or 2), and nsplit, the column slices of S summed in a fixed order
+ * (1, 2, 4 or 8).  saige_autotune times every candidate on the resident GRM
+ * (reps launches each after one warm-up) and keeps the fastest; ms_table
+ * (may be NULL) receives the per-launch milliseconds in the order of
+ * saige_autotune_candidates.  The shape changes the summation order, so call
+ * it once before solving; results are reproducible for a fixed shape.
+ * SAIGE_E_STATE if the engine has live slots. */
+int saige_autotune(saige_ctx* ctx, int reps, double* ms_table);
+/* The candidate shapes (rt[i], nsplit[i]); returns their count (8). */
+int saige_autotune_candidates(int* rt, int* nsplit);
+/* Set the block-GEMM shape explicitly (engine idle). */
+int saige_set_gemm_config(saige_ctx* ctx, int rt, int nsplit);
+/* Current shape, and its tuned per-launch time in ms (0 if not tuned). */
+int saige_get_gemm_config(const saige_ctx* ctx, int* rt, int* nsplit, double* tuned_ms);
+
 /* Free and total device memory (hipMemGetInfo), for admission control. */
 int saige_device_memory(saige_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);
 /* Device bytes one engine system (w, tau, sample mask) occupies. */

@@ -9,7 +9,8 @@ Python side of the oracle for SAIGE step 1's hot path.  Two layers:
   drivers in src/SAIGE/R/SAIGE_fitGLMM_fast.R (FG.R), evaluated with a dense
   FP64 GRM: getCoefficients, GetTrace[_q], getAIScore[_q],
   fitglmmaiRPCG[_q], Get_Coef, glmmkin.ai_PCG_Rcpp_{Binary,Quantitative},
-  and the variance-ratio core of extractVarianceRatio.
+  the variance-ratio core of extractVarianceRatio, and the leave-one-
+  chromosome-out GRM and loop (setStartEndIndex, Get_Coef_LOCO).
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
 this module; the product path never does.
@@ -179,6 +180,35 @@ def grm_f64(g):
     lib().orc_grm_f64(_p(g["packed"]), ctypes.c_int64(g["Mqc"]), ctypes.c_int64(g["Nnm"]),
                       _p(g["freq"]), _p(g["inv_std"]), _p(S))
     return S
+
+
+def marker_subset(g, idx):
+    """The genotype dict restricted to QC'd markers idx (in that order)."""
+    idx = np.asarray(idx, np.int64)
+    return dict(g, packed=np.ascontiguousarray(g["packed"][idx]), freq=g["freq"][idx].copy(),
+                inv_std=g["inv_std"][idx].copy(), mac=g["mac"][idx].copy(), Mqc=len(idx))
+
+
+def loco_diag_std(g, start, end, diag=None):
+    """set_Diagof_StdGeno_LOCO (FG.cpp:4934-4958) for one range: the range's
+    sum of z^2 accumulated from zero in marker order (FP32), subtracted from
+    the full diagonal.  Returns (diag_loco, count)."""
+    full = diag_std(g) if diag is None else np.asarray(diag, np.float32)
+    part = diag_std(marker_subset(g, np.arange(start, end + 1)))
+    return (full - part).astype(np.float32), end - start + 1
+
+
+def loco_grm(g, start, end):
+    """The leave-out GRM of setStartEndIndex (FG.cpp:3067-3082) with
+    parallelCrossProd_LOCO (FG.cpp:1790-1851) and getDiagOfSigma_LOCO
+    (FG.cpp:2368-2392): K_loco = Z_out Z_out^T / (Mqc - (end-start+1)), the
+    FP64 product formed from the markers outside [start, end] directly (the
+    reference subtracts the range's product from the full one; the two are
+    equal in exact arithmetic), diagonal from loco_diag_std."""
+    out = np.r_[np.arange(0, start), np.arange(end + 1, g["Mqc"])]
+    S = grm_f64(marker_subset(g, out))
+    dl, cnt = loco_diag_std(g, start, end)
+    return DenseGRM(S, g["Mqc"] - cnt, diag_std=dl)
 
 
 def diag_sigma_f32(diag, Mqc, w, tau, diag_one=False):
@@ -520,6 +550,24 @@ def get_coef(grm, y, X, tau, family, alpha0, eta0, offset, maxiterPCG=500, tolPC
             "Sigma_iY": rc["Sigma_iY"], "Sigma_iX": rc["Sigma_iX"], "mu": mu}
 
 
+def loco_chromosome(grm_loco, fit, y, X, trait, offset=None, maxiterPCG=500, tolPCG=1e-5, maxiter=20):
+    """One chromosome of the LOCO loop of fitNULLGLMM (FG.R:1240-1272 binary,
+    FG.R:1486-1512 quantitative): Get_Coef_LOCO (FG.R:42-73) started at the
+    full fit's theta, coefficients and linear predictors, then the score-test
+    null object with mu2 = mu(1-mu) (binary) or 1/tau[0] (quantitative)."""
+    y = np.asarray(y, np.float64)
+    offset = np.zeros(len(y)) if offset is None else offset
+    fam = Binomial if trait == "binary" else Gaussian
+    tau = np.asarray(fit["theta"], np.float64)
+    rc = get_coef(grm_loco, y, X, tau, fam, fit["coefficients"], fit["linear_predictors"], offset,
+                  maxiterPCG, tolPCG, maxiter)
+    mu = rc["mu"]
+    mu2 = mu * (1 - mu) if trait == "binary" else np.full(len(y), 1.0 / tau[0])
+    return {"coefficients": rc["alpha"], "linear_predictors": rc["eta"], "fitted_values": mu,
+            "Y": rc["Y"], "residuals": y - mu, "cov": rc["cov"],
+            "obj_noK": score_test_null(mu, mu2, y, X)}
+
+
 def glmmkin_binary(grm, y, X, probes, offset=None, fit0=None, tau_init=(0.0, 0.0), maxiter=20,
                    tol=0.02, nrun=30, tolPCG=1e-5, maxiterPCG=500, cv_cutoff=0.0025):
     """glmmkin.ai_PCG_Rcpp_Binary (FG.R:79-304) without LOCO."""
@@ -637,4 +685,4 @@ def score_test_null(mu, mu2, y, X):
     XVX_inv = np.linalg.inv(XVX)
     XXVX_inv = X @ XVX_inv
     return {"XV": XV, "XVX": XVX, "XXVX_inv": XXVX_inv, "XVX_inv": XVX_inv,
-            "S_a": (X * (y - mu)[:, None]).sum(0), "V": V}
+            "S_a": (X * (y - mu)[:, None]).sum(0), "XVX_inv_XV": XXVX_inv * V[:, None], "V": V}

@@ -299,6 +299,12 @@ __global__ void k_synth_bed(uint8_t* __restrict__ bed, int64_t N, int64_t M, int
     bed[t] = (uint8_t)byte;
 }
 
+// out = a - b, FP32 (diag(K) minus one chromosome's part, FG.cpp:4951)
+__global__ void k_sub(const float* __restrict__ a, const float* __restrict__ b, int64_t n, float* __restrict__ out) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = __fsub_rn(a[i], b[i]);
+}
+
 inline unsigned grid_for(int64_t n, int64_t cap) { return (unsigned)(n < cap ? (n > 0 ? n : 1) : cap); }
 
 }  // namespace
@@ -331,6 +337,11 @@ hipError_t launch_diag_std(hipStream_t s, const uint8_t* packed, const float4* l
                            int64_t Nnm, int64_t bpm_nm, float* diag) {
     hipLaunchKernelGGL(k_diag_std, dim3((unsigned)((Nnm + 255) / 256)), dim3(256), 0, s, packed, lut,
                        Mqc, Nnm, bpm_nm, diag);
+    return hipGetLastError();
+}
+
+hipError_t launch_sub(hipStream_t s, const float* a, const float* b, int64_t n, float* out) {
+    hipLaunchKernelGGL(k_sub, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, b, n, out);
     return hipGetLastError();
 }
 

@@ -53,9 +53,12 @@ __device__ __forceinline__ void store_tile(float* S, int64_t ldS, int64_t bi, in
                 float v[4];
 #pragma unroll
                 for (int e = 0; e < 4; e++) v[e] = acc[mi][ni][4 * g + e];
-                if (accumulate) {
+                if (accumulate > 0) {        // S += tile
 #pragma unroll
-                    for (int e = 0; e < 4; e++) v[e] += S[(row0 + e) * ldS + col];
+                    for (int e = 0; e < 4; e++) v[e] = S[(row0 + e) * ldS + col] + v[e];
+                } else if (accumulate < 0) { // S -= tile (LOCO: remove one chromosome's markers)
+#pragma unroll
+                    for (int e = 0; e < 4; e++) v[e] = S[(row0 + e) * ldS + col] - v[e];
                 }
 #pragma unroll
                 for (int e = 0; e < 4; e++) S[(row0 + e) * ldS + col] = v[e];
@@ -166,8 +169,8 @@ __device__ __forceinline__ u32x4 decode_row(const uint64_t (&lut)[8], const uint
     return r;
 }
 
-__global__ __launch_bounds__(256) void k_syrk_pk(const uint8_t* __restrict__ tp, int64_t nchunk,
-                                                 const uint64_t* __restrict__ lut64,
+__global__ __launch_bounds__(256) void k_syrk_pk(const uint8_t* __restrict__ tp, int64_t nchunk, int64_t c0,
+                                                 int64_t c1, const uint64_t* __restrict__ lut64,
                                                  const int2* __restrict__ tiles, int64_t nblk,
                                                  float* __restrict__ S, int64_t ldS, int accumulate) {
     __shared__ __attribute__((aligned(16))) __hip_bfloat16 sA[BT * P16];
@@ -180,12 +183,12 @@ __global__ __launch_bounds__(256) void k_syrk_pk(const uint8_t* __restrict__ tp,
     const int o = tid & 7, b = tid >> 3;          // marker octet, byte within the panel
     const uint64_t* gA = reinterpret_cast<const uint64_t*>(tp + (bi * nchunk * 32 + b) * 64 + 8 * o);
     const uint64_t* gB = reinterpret_cast<const uint64_t*>(tp + (bj * nchunk * 32 + b) * 64 + 8 * o);
-    // one chunk = 2048 bytes = 256 uint64
-    uint64_t ra = gA[0], rb = gB[0];
+    // one chunk = 2048 bytes = 256 uint64; this launch covers chunks [c0, c1)
+    uint64_t ra = gA[c0 * 256], rb = gB[c0 * 256];
     const uint64_t* gl = lut64 + 8 * o;
     uint64_t la[8];
 #pragma unroll
-    for (int e = 0; e < 8; e++) la[e] = gl[e];
+    for (int e = 0; e < 8; e++) la[e] = gl[c0 * 64 + e];
 
     f32x16 acc[2][2];
 #pragma unroll
@@ -195,7 +198,7 @@ __global__ __launch_bounds__(256) void k_syrk_pk(const uint8_t* __restrict__ tp,
 #pragma unroll
             for (int r = 0; r < 16; r++) acc[mi][ni][r] = 0.0f;
 
-    for (int64_t kc = 0; kc < nchunk; kc++) {
+    for (int64_t kc = c0; kc < c1; kc++) {
         uint8_t ba[8], bb[8];
 #pragma unroll
         for (int e = 0; e < 8; e++) {
@@ -206,7 +209,7 @@ __global__ __launch_bounds__(256) void k_syrk_pk(const uint8_t* __restrict__ tp,
 #pragma unroll
         for (int e = 0; e < 8; e++) lc[e] = la[e];
         // prefetch the next chunk's bytes and LUT entries (re-read the last one at the end)
-        const int64_t kn = (kc + 1 < nchunk) ? kc + 1 : kc;
+        const int64_t kn = (kc + 1 < c1) ? kc + 1 : kc;
         ra = gA[kn * 256];
         rb = gB[kn * 256];
 #pragma unroll
@@ -262,11 +265,12 @@ __global__ __launch_bounds__(256) void k_pack_tiles(const uint8_t* __restrict__ 
 }
 
 // lut64[j] for j < nchunk*64 (zero for padding markers)
-__global__ void k_lut64(const float4* __restrict__ lut, int64_t Mqc, int64_t n, uint64_t* __restrict__ out) {
+__global__ void k_lut64(const float4* __restrict__ lut, int64_t j0, int64_t j1, int64_t n,
+                        uint64_t* __restrict__ out) {
     const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (j >= n) return;
     uint64_t v = 0;
-    if (j < Mqc) {
+    if (j >= j0 && j < j1) {
         const float4 L = lut[j];
         const uint64_t l0 = __bfloat16_as_ushort(__float2bfloat16(L.x));
         const uint64_t l1 = __bfloat16_as_ushort(__float2bfloat16(L.y));
@@ -367,18 +371,29 @@ hipError_t launch_pack_tiles(hipStream_t s, const uint8_t* packed, const float4*
                        Nnm, bpm_nm, nchunk, tp);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_lut64, dim3((unsigned)((nchunk * 64 + 255) / 256)), dim3(256), 0, s, lut, Mqc, nchunk * 64,
-                       lut64);
+    hipLaunchKernelGGL(k_lut64, dim3((unsigned)((nchunk * 64 + 255) / 256)), dim3(256), 0, s, lut, (int64_t)0, Mqc,
+                       nchunk * 64, lut64);
+    return hipGetLastError();
+}
+
+hipError_t launch_lut64_range(hipStream_t s, const float4* lut, int64_t Mqc, int64_t j0, int64_t j1,
+                              uint64_t* lut64) {
+    const int64_t nchunk = (Mqc + 63) / 64;
+    hipLaunchKernelGGL(k_lut64, dim3((unsigned)((nchunk * 64 + 255) / 256)), dim3(256), 0, s, lut, j0, j1,
+                       nchunk * 64, lut64);
     return hipGetLastError();
 }
 
 hipError_t launch_syrk_pk(hipStream_t s, const uint8_t* tp, const uint64_t* lut64, int64_t Mqc, int64_t Npad,
-                          const int2* tiles, float* S, int accumulate) {
+                          const int2* tiles, float* S, int accumulate, int64_t j0, int64_t j1) {
     const int64_t nt = Npad / BT;
     const int64_t nblk = nt * (nt + 1) / 2;
     const int64_t nchunk = (Mqc + 63) / 64;
-    hipLaunchKernelGGL(k_syrk_pk, dim3((unsigned)nblk), dim3(256), 0, s, tp, nchunk, lut64, tiles, nblk, S, Npad,
-                       accumulate);
+    if (j1 < 0) j1 = Mqc;
+    const int64_t c0 = j0 / 64, c1 = (j1 + 63) / 64;
+    if (c0 >= c1) return hipSuccess;
+    hipLaunchKernelGGL(k_syrk_pk, dim3((unsigned)nblk), dim3(256), 0, s, tp, nchunk, c0, c1, lut64, tiles, nblk, S,
+                       Npad, accumulate);
     return hipGetLastError();
 }
 

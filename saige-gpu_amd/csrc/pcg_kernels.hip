@@ -93,48 +93,58 @@ __global__ __launch_bounds__(256) void k_gemv(const float* __restrict__ S, int64
 // --------------------------------------------------------------- GEMM
 // Block product Y = S P for EW = 64 columns, FP32 MFMA 32x32x2.
 //
-// k_gemm_part: a workgroup owns 128 rows (4 waves x 32 rows) and one of
-// GSPLIT slices of the column range of S.  Each wave streams its rows of S
-// from HBM straight into MFMA operand registers (lane l holds row l%32 and
-// the 16 consecutive columns starting at 16*(l/32) of each 32-column chunk;
-// the next chunk is prefetched while the current one is multiplied).  The
-// matching 128 x 64 superchunk of P is staged in LDS as [n][k] so each lane
-// reads its 16 B-operand values with four b128 loads.  S is read from HBM
-// exactly once; P (25.6 MB at N = 100k) is re-read per row block from L2/MALL.
-// The raw sums go to Ypart[slice][row][n].
+// k_gemm_part<RT>: a workgroup owns 128*RT rows (4 waves x RT tiles of 32
+// rows) and one of gridDim.y slices of the column range of S.  Each wave
+// streams its rows of S from HBM straight into MFMA operand registers (lane
+// l holds row l%32 of each tile and the 16 consecutive columns starting at
+// 16*(l/32) of each 32-column chunk; the next chunk is prefetched while the
+// current one is multiplied).  The matching 128 x 64 superchunk of P is
+// staged in LDS as [n][k] so each lane reads its 16 B-operand values with
+// four b128 loads, each feeding 2*RT MFMAs.  S is read from HBM exactly once;
+// P (25.6 MB at N = 100k) is re-read per row block from L2/MALL, so RT = 2
+// halves that traffic at the cost of registers.  The raw sums go to
+// Ypart[slice][row][n].  RT and the slice count are picked at run time by
+// saige_autotune.
 //
 // k_gemm_epi: Y = sum over slices in fixed order, then the Sigma epilogue
 // and the per-column FP64 partial dot products.
-constexpr int GM = 128;                 // rows per workgroup
 constexpr int GKS = 128;                // columns of S per P superchunk
 constexpr int GKP = GKS + 4;            // LDS pitch of sP (floats)
-constexpr int GSPLIT = 4;               // column slices
+constexpr int GSPLIT_MAX = 8;           // most column slices
 
+template <int RT>
 __global__ __launch_bounds__(256) void k_gemm_part(const float* __restrict__ S, int64_t ld,
                                                    const float* __restrict__ in, float* __restrict__ ypart) {
+    constexpr int GM = 128 * RT;        // rows per workgroup
     __shared__ __attribute__((aligned(16))) float sP[EW * GKP];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int64_t r0 = (int64_t)blockIdx.x * GM;
-    const int split = blockIdx.y;
+    const int split = blockIdx.y, nsplit = gridDim.y;
     const int64_t nsc = ld / GKS;                           // superchunks in a row
-    const int64_t sc0 = nsc * split / GSPLIT, sc1 = nsc * (split + 1) / GSPLIT;
+    const int64_t sc0 = nsc * split / nsplit, sc1 = nsc * (split + 1) / nsplit;
     const int hl = lane >> 5;                               // k half of this lane
-    const float* srow = S + (r0 + wv * 32 + (lane & 31)) * ld + hl * 16;
+    // wave wv owns rows r0 + wv*32*RT + t*32 + [0, 32) for tiles t < RT
+    const float* srow = S + (r0 + wv * 32 * RT + (lane & 31)) * ld + hl * 16;
     // P superchunk staging: thread -> column n, 32 consecutive k
     const int pn = tid & 63, pq = tid >> 6;
 
-    f32x16 acc[2];
+    f32x16 acc[RT][2];
 #pragma unroll
-    for (int ni = 0; ni < 2; ni++)
+    for (int t = 0; t < RT; t++)
 #pragma unroll
-        for (int r = 0; r < 16; r++) acc[ni][r] = 0.0f;
+        for (int ni = 0; ni < 2; ni++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[t][ni][r] = 0.0f;
 
-    float4 sa[4], sb[4];
+    float4 sa[RT][4], sb[RT][4];
     float pr[32];
     if (sc0 < sc1) {
         const int64_t k0 = sc0 * GKS;
 #pragma unroll
-        for (int q = 0; q < 4; q++) sa[q] = *reinterpret_cast<const float4*>(srow + k0 + 4 * q);
+        for (int t = 0; t < RT; t++)
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                sa[t][q] = *reinterpret_cast<const float4*>(srow + (int64_t)t * 32 * ld + k0 + 4 * q);
 #pragma unroll
         for (int q = 0; q < 32; q++) pr[q] = in[(k0 + pq * 32 + q) * EW + pn];
     }
@@ -157,44 +167,55 @@ __global__ __launch_bounds__(256) void k_gemm_part(const float* __restrict__ S, 
             const int64_t kc_next = (c + 1 < GKS / 32) ? k0 + (c + 1) * 32 : ((sc + 1 < sc1) ? k0 + GKS : -1);
             if (kc_next >= 0) {
 #pragma unroll
-                for (int q = 0; q < 4; q++) sb[q] = *reinterpret_cast<const float4*>(srow + kc_next + 4 * q);
+                for (int t = 0; t < RT; t++)
+#pragma unroll
+                    for (int q = 0; q < 4; q++)
+                        sb[t][q] = *reinterpret_cast<const float4*>(srow + (int64_t)t * 32 * ld + kc_next + 4 * q);
             }
             const float* pb = sP + c * 32 + hl * 16;
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const float4 b0 = *reinterpret_cast<const float4*>(pb + (lane & 31) * GKP + 4 * q);
                 const float4 b1 = *reinterpret_cast<const float4*>(pb + (32 + (lane & 31)) * GKP + 4 * q);
-                acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[q].x, b0.x, acc[0], 0, 0, 0);
-                acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[q].x, b1.x, acc[1], 0, 0, 0);
-                acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[q].y, b0.y, acc[0], 0, 0, 0);
-                acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[q].y, b1.y, acc[1], 0, 0, 0);
-                acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[q].z, b0.z, acc[0], 0, 0, 0);
-                acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[q].z, b1.z, acc[1], 0, 0, 0);
-                acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[q].w, b0.w, acc[0], 0, 0, 0);
-                acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[q].w, b1.w, acc[1], 0, 0, 0);
+#pragma unroll
+                for (int t = 0; t < RT; t++) {
+                    acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[t][q].x, b0.x, acc[t][0], 0, 0, 0);
+                    acc[t][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[t][q].x, b1.x, acc[t][1], 0, 0, 0);
+                    acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[t][q].y, b0.y, acc[t][0], 0, 0, 0);
+                    acc[t][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[t][q].y, b1.y, acc[t][1], 0, 0, 0);
+                    acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[t][q].z, b0.z, acc[t][0], 0, 0, 0);
+                    acc[t][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[t][q].z, b1.z, acc[t][1], 0, 0, 0);
+                    acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[t][q].w, b0.w, acc[t][0], 0, 0, 0);
+                    acc[t][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[t][q].w, b1.w, acc[t][1], 0, 0, 0);
+                }
             }
 #pragma unroll
-            for (int q = 0; q < 4; q++) sa[q] = sb[q];
+            for (int t = 0; t < RT; t++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) sa[t][q] = sb[t][q];
         }
     }
     // raw sums: lane holds column ni*32 + lane%32, rows 8*(r/4) + 4*(lane/32) + r%4
     float* yp = ypart + (int64_t)split * (gridDim.x * (int64_t)GM) * EW;
 #pragma unroll
-    for (int ni = 0; ni < 2; ni++) {
-        const int n = ni * 32 + (lane & 31);
+    for (int t = 0; t < RT; t++)
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
-            const int64_t row = r0 + wv * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            yp[row * EW + n] = acc[ni][r];
+        for (int ni = 0; ni < 2; ni++) {
+            const int n = ni * 32 + (lane & 31);
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int64_t row = r0 + wv * 32 * RT + t * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                yp[row * EW + n] = acc[t][ni][r];
+            }
         }
-    }
 }
 
 constexpr int EROWS = 256;   // rows per workgroup of the epilogue
 
-__global__ __launch_bounds__(256) void k_gemm_epi(const float* __restrict__ ypart, int64_t Npad, int64_t Nnm,
-                                                  const float* __restrict__ in, float* __restrict__ out,
-                                                  SigmaArgs a, double* __restrict__ partial) {
+__global__ __launch_bounds__(256) void k_gemm_epi(const float* __restrict__ ypart, int nsplit, int64_t Npad,
+                                                  int64_t Nnm, const float* __restrict__ in,
+                                                  float* __restrict__ out, SigmaArgs a,
+                                                  double* __restrict__ partial) {
     __shared__ double sm[256];
     const int tid = threadIdx.x;
     const int n = tid & 63;
@@ -215,7 +236,7 @@ __global__ __launch_bounds__(256) void k_gemm_epi(const float* __restrict__ ypar
         const int64_t o = row * EW + n;
         float y = ypart[o];
 #pragma unroll
-        for (int sp = 1; sp < GSPLIT; sp++) y += ypart[(int64_t)sp * Npad * EW + o];
+        for (int sp = 1; sp < nsplit; sp++) y += ypart[(int64_t)sp * Npad * EW + o];
         float r;
         if (a.mode == 0) {
             r = div_rn(y, a.mqc);
@@ -566,18 +587,32 @@ hipError_t launch_grm_gemv(hipStream_t s, const float* S, int64_t Npad, int64_t 
     return hipGetLastError();
 }
 
+bool gemm_config_valid(GemmConfig g) {
+    return (g.rt == 1 || g.rt == 2) && (g.nsplit == 1 || g.nsplit == 2 || g.nsplit == 4 || g.nsplit == 8);
+}
+
 hipError_t launch_grm_gemm(hipStream_t s, const float* S, int64_t Npad, int64_t Nnm, const float* in,
-                           float* out, SigmaArgs a, double* partial, int64_t* nparts, float* ypart) {
-    hipLaunchKernelGGL(k_gemm_part, dim3((unsigned)(Npad / GM), GSPLIT), dim3(256), 0, s, S, Npad, in, ypart);
+                           float* out, SigmaArgs a, double* partial, int64_t* nparts, float* ypart, GemmConfig g) {
+    // Npad is a multiple of 256 = 128 * RT for both RT; a slice holds at least
+    // one superchunk when Npad / GKS >= nsplit
+    if (!gemm_config_valid(g) || Npad % 256 != 0) return hipErrorInvalidValue;
+    const int nsplit = (int)std::min<int64_t>(g.nsplit, Npad / GKS);
+    if (g.rt == 2)
+        hipLaunchKernelGGL(k_gemm_part<2>, dim3((unsigned)(Npad / 256), nsplit), dim3(256), 0, s, S, Npad, in,
+                           ypart);
+    else
+        hipLaunchKernelGGL(k_gemm_part<1>, dim3((unsigned)(Npad / 128), nsplit), dim3(256), 0, s, S, Npad, in,
+                           ypart);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int64_t nb = Npad / EROWS;
-    hipLaunchKernelGGL(k_gemm_epi, dim3((unsigned)nb), dim3(256), 0, s, ypart, Npad, Nnm, in, out, a, partial);
+    hipLaunchKernelGGL(k_gemm_epi, dim3((unsigned)nb), dim3(256), 0, s, ypart, nsplit, Npad, Nnm, in, out, a,
+                       partial);
     if (nparts) *nparts = nb;
     return hipGetLastError();
 }
 
-int64_t gemm_ypart_floats(int64_t Npad) { return (int64_t)GSPLIT * Npad * EW; }
+int64_t gemm_ypart_floats(int64_t Npad) { return (int64_t)GSPLIT_MAX * Npad * EW; }
 
 hipError_t launch_sigma_setup(hipStream_t s, const float* w, const uint8_t* mask_in, const float* diag,
                               int64_t Nnm, int64_t Npad, float tau0, float tau1, float mqc, int diag_one,

@@ -96,7 +96,7 @@ void free_engine(Engine& e) {
 
 void free_geno(saige_ctx* c) {
     dfree(c->d_stats); dfree(c->d_packed); dfree(c->d_packed_vr); dfree(c->d_lut); dfree(c->d_lut_vr);
-    dfree(c->d_diag); dfree(c->d_grm); dfree(c->d_vec);
+    dfree(c->d_diag); dfree(c->d_grm); dfree(c->d_vec); dfree(c->d_grm_loco); dfree(c->d_diag_loco);
     dfree(c->d_mv_in); dfree(c->d_mv_blk); dfree(c->d_mv_out); dfree(c->d_mv_ypart);
     free_engine(c->eng);
     free_engine(c->eng1);
@@ -105,6 +105,10 @@ void free_geno(saige_ctx* c) {
     c->vr_index.clear();
     c->loaded = false;
     c->grm_ready = false;
+    c->loco_ready = false;
+    c->loco_start = c->loco_end = -1;
+    c->loco_chrom = -1;
+    c->loco_m = 0;
     c->N = c->M = c->Nnm = c->Npad = c->Mqc = c->Mvr = c->bpm_nm = 0;
 }
 
@@ -188,6 +192,29 @@ int engine_reserve(saige_ctx* c, Engine& e, int kpad, int nsys) {
     return SAIGE_OK;
 }
 
+// The GRM a product uses: the full one (S/Mqc, diag) or the LOCO one
+// (S_loco/M_loco, diag_loco; getDiagOfSigma_LOCO ignores the diag-as-one
+// option, FG.cpp:2368-2392).
+struct GrmView {
+    const float* S;
+    const float* diag;
+    int64_t m;
+    int diag_one;
+};
+
+GrmView grm_view(const saige_ctx* c, int loco) {
+    if (loco) return {c->d_grm_loco, c->d_diag_loco, c->loco_m, 0};
+    return {c->d_grm, c->d_diag, c->Mqc, c->diag_one};
+}
+
+int check_loco(const saige_ctx* c) {
+    if (!c->loco_ready) {
+        set_error("no LOCO range: call saige_set_start_end_index first");
+        return SAIGE_E_STATE;
+    }
+    return SAIGE_OK;
+}
+
 int engine_set_system(saige_ctx* c, Engine& e, int s, const float* w_host, const float* tau,
                       const uint8_t* mask_host) {
     if (s < 0 || s >= e.nsys) {
@@ -205,8 +232,9 @@ int engine_set_system(saige_ctx* c, Engine& e, int s, const float* w_host, const
     SAIGE_HIP_TRY(hipMemcpyAsync(d_w, w_host, sizeof(float) * N, hipMemcpyHostToDevice, st));
     if (mask_host) SAIGE_HIP_TRY(hipMemcpyAsync(d_m, mask_host, N, hipMemcpyHostToDevice, st));
     SAIGE_HIP_TRY(hipMemcpyAsync(e.tau + 2 * s, tau, sizeof(float) * 2, hipMemcpyHostToDevice, st));
-    SAIGE_HIP_TRY(launch_sigma_setup(st, d_w, mask_host ? d_m : nullptr, c->d_diag, N, Np, tau[0], tau[1],
-                                     (float)c->Mqc, c->diag_one, e.winv + s * Np, e.minv + s * Np,
+    const GrmView g = grm_view(c, e.loco);
+    SAIGE_HIP_TRY(launch_sigma_setup(st, d_w, mask_host ? d_m : nullptr, g.diag, N, Np, tau[0], tau[1],
+                                     (float)g.m, g.diag_one, e.winv + s * Np, e.minv + s * Np,
                                      e.mask + s * Np, nullptr));
     SAIGE_HIP_TRY(hipStreamSynchronize(st));
     return SAIGE_OK;
@@ -293,8 +321,9 @@ int engine_step(saige_ctx* c, Engine& e, int n_iter, uint8_t* done_out, int32_t*
     a.tau = e.tau;
     a.sys = e.slot_sys;
     a.live = e.live;
+    const GrmView g = grm_view(c, e.loco);
     a.inv_m = 0.0f;
-    a.mqc = (float)c->Mqc;
+    a.mqc = (float)g.m;
     a.mode = 1;
     a.kind = e.kind;
     a.xout = e.X;
@@ -307,9 +336,9 @@ int engine_step(saige_ctx* c, Engine& e, int n_iter, uint8_t* done_out, int32_t*
         int64_t np = 0;
         SAIGE_HIP_TRY(hipEventRecord(c->ev0, st));
         if (e.kpad == 1)
-            SAIGE_HIP_TRY(launch_grm_gemv(st, c->d_grm, Np, N, e.P, e.AP, a, e.partial, &np));
+            SAIGE_HIP_TRY(launch_grm_gemv(st, g.S, Np, N, e.P, e.AP, a, e.partial, &np));
         else
-            SAIGE_HIP_TRY(launch_grm_gemm(st, c->d_grm, Np, N, e.P, e.AP, a, e.partial, &np, e.ypart));
+            SAIGE_HIP_TRY(launch_grm_gemm(st, g.S, Np, N, e.P, e.AP, a, e.partial, &np, e.ypart, c->gemm_cfg));
         SAIGE_HIP_TRY(hipEventRecord(c->ev1, st));
         SAIGE_HIP_TRY(launch_alpha(st, e.kpad, e.partial, np, e.rz, e.done, e.live, e.kind, e.alpha));
         int64_t np2 = 0;
@@ -786,6 +815,7 @@ int saige_build_grm(saige_ctx* c, int precision) {
         }
     }
     c->grm_ready = false;
+    c->loco_ready = false;   // S_loco derives from S
     if (precision == SAIGE_GRM_BF16) {
         // 2-bit operands straight into the MFMA SYRK: no Z panel
         const int64_t nchunk = (c->Mqc + 63) / 64;
@@ -898,24 +928,23 @@ int saige_get_grm_block(saige_ctx* c, int64_t r0, int64_t c0, int64_t nr, int64_
     return SAIGE_OK;
 }
 
-int saige_grm_matvec(saige_ctx* c, const float* B, int64_t k, float* Y) {
-    CHECK_CTX(c);
-    CHECK_GRM(c);
+static int grm_matvec(saige_ctx* c, int loco, const float* B, int64_t k, float* Y) {
     if (k <= 0 || !B || !Y) {
         set_error("invalid matvec arguments");
         return SAIGE_E_ARG;
     }
     hipStream_t st = c->stream;
     const int64_t Np = c->Npad, N = c->Nnm;
+    const GrmView g = grm_view(c, loco);
     SigmaArgs a{};
-    a.mqc = (float)c->Mqc;
+    a.mqc = (float)g.m;
     a.mode = 0;
     if (k == 1) {
         int rc = engine_reserve(c, c->eng1, 1, 1);
         if (rc) return rc;
         Engine& e = c->eng1;
         SAIGE_HIP_TRY(hipMemcpyAsync(e.stage, B, 4 * N, hipMemcpyHostToDevice, st));
-        SAIGE_HIP_TRY(launch_grm_gemv(st, c->d_grm, Np, N, e.stage, e.AP, a, nullptr, nullptr));
+        SAIGE_HIP_TRY(launch_grm_gemv(st, g.S, Np, N, e.stage, e.AP, a, nullptr, nullptr));
         SAIGE_HIP_TRY(hipMemcpyAsync(Y, e.AP, 4 * N, hipMemcpyDeviceToHost, st));
         SAIGE_HIP_TRY(hipStreamSynchronize(st));
         return SAIGE_OK;
@@ -935,7 +964,7 @@ int saige_grm_matvec(saige_ctx* c, const float* B, int64_t k, float* Y) {
         const int kb = (int)std::min<int64_t>(EW, k - k0);
         e = hipMemcpyAsync(d_in, B + k0 * N, 4 * N * kb, hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = launch_scatter_cols(st, d_in, kb, N, Np, EW, d_blk);
-        if (e == hipSuccess) e = launch_grm_gemm(st, c->d_grm, Np, N, d_blk, d_out, a, nullptr, nullptr, c->d_mv_ypart);
+        if (e == hipSuccess) e = launch_grm_gemm(st, g.S, Np, N, d_blk, d_out, a, nullptr, nullptr, c->d_mv_ypart, c->gemm_cfg);
         if (e == hipSuccess) e = launch_gather_cols(st, d_out, kb, N, EW, d_in);
         if (e == hipSuccess) e = hipMemcpyAsync(Y + k0 * N, d_in, 4 * N * kb, hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -947,13 +976,17 @@ int saige_grm_matvec(saige_ctx* c, const float* B, int64_t k, float* Y) {
     return SAIGE_OK;
 }
 
+int saige_grm_matvec(saige_ctx* c, const float* B, int64_t k, float* Y) {
+    CHECK_CTX(c);
+    CHECK_GRM(c);
+    return grm_matvec(c, 0, B, k, Y);
+}
+
 int saige_crossprod_mat_and_kin(saige_ctx* c, const float* b, float* out) {
     return saige_grm_matvec(c, b, 1, out);
 }
 
-int saige_get_diag_of_sigma(saige_ctx* c, const float* w, const float* tau, float* out) {
-    CHECK_CTX(c);
-    CHECK_LOADED(c);
+static int get_diag_of_sigma(saige_ctx* c, int loco, const float* w, const float* tau, float* out) {
     if (!w || !tau || !out) {
         set_error("invalid arguments");
         return SAIGE_E_ARG;
@@ -967,17 +1000,22 @@ int saige_get_diag_of_sigma(saige_ctx* c, const float* w, const float* tau, floa
     uint8_t* d_mask = nullptr;
     SAIGE_HIP_TRY(dalloc(&d_mask, Np));
     SAIGE_HIP_TRY(hipMemcpyAsync(d_w, w, 4 * N, hipMemcpyHostToDevice, st));
-    SAIGE_HIP_TRY(launch_sigma_setup(st, d_w, nullptr, c->d_diag, N, Np, tau[0], tau[1], (float)c->Mqc,
-                                     c->diag_one, d_winv, d_minv, d_mask, d_d));
+    const GrmView g = grm_view(c, loco);
+    SAIGE_HIP_TRY(launch_sigma_setup(st, d_w, nullptr, g.diag, N, Np, tau[0], tau[1], (float)g.m, g.diag_one,
+                                     d_winv, d_minv, d_mask, d_d));
     SAIGE_HIP_TRY(hipMemcpyAsync(out, d_d, 4 * N, hipMemcpyDeviceToHost, st));
     SAIGE_HIP_TRY(hipStreamSynchronize(st));
     dfree(d_mask);
     return SAIGE_OK;
 }
 
-int saige_get_crossprod(saige_ctx* c, const float* b, const float* w, const float* tau, float* out) {
+int saige_get_diag_of_sigma(saige_ctx* c, const float* w, const float* tau, float* out) {
     CHECK_CTX(c);
-    CHECK_GRM(c);
+    CHECK_LOADED(c);
+    return get_diag_of_sigma(c, 0, w, tau, out);
+}
+
+static int get_crossprod(saige_ctx* c, int loco, const float* b, const float* w, const float* tau, float* out) {
     if (!b || !w || !tau || !out) {
         set_error("invalid arguments");
         return SAIGE_E_ARG;
@@ -985,6 +1023,7 @@ int saige_get_crossprod(saige_ctx* c, const float* b, const float* w, const floa
     int rc = engine_reserve(c, c->eng1, 1, 1);
     if (rc) return rc;
     Engine& e = c->eng1;
+    e.loco = loco;
     rc = engine_set_system(c, e, 0, w, tau, nullptr);
     if (rc) return rc;
     hipStream_t st = c->stream;
@@ -1000,22 +1039,27 @@ int saige_get_crossprod(saige_ctx* c, const float* b, const float* w, const floa
     a.tau = e.tau;
     a.sys = e.slot_sys;
     a.live = e.live;
+    const GrmView g = grm_view(c, loco);
     a.inv_m = 0.f;
-    a.mqc = (float)c->Mqc;
+    a.mqc = (float)g.m;
     a.mode = 1;
     a.kind = nullptr;
     a.xout = nullptr;
-    SAIGE_HIP_TRY(launch_grm_gemv(st, c->d_grm, Np, N, e.stage, e.AP, a, e.partial, nullptr));
+    SAIGE_HIP_TRY(launch_grm_gemv(st, g.S, Np, N, e.stage, e.AP, a, e.partial, nullptr));
     SAIGE_HIP_TRY(hipMemcpyAsync(out, e.AP, 4 * N, hipMemcpyDeviceToHost, st));
     SAIGE_HIP_TRY(hipStreamSynchronize(st));
     e.h_live[0] = 0;
     return engine_sync_host_arrays(c, e, nullptr, nullptr);
 }
 
-int saige_pcg1(saige_ctx* c, const float* w, const float* tau, const float* b, int maxiter, float tol, float* x,
-               int32_t* iters) {
+int saige_get_crossprod(saige_ctx* c, const float* b, const float* w, const float* tau, float* out) {
     CHECK_CTX(c);
     CHECK_GRM(c);
+    return get_crossprod(c, 0, b, w, tau, out);
+}
+
+static int pcg1(saige_ctx* c, int loco, const float* w, const float* tau, const float* b, int maxiter, float tol, float* x,
+         int32_t* iters) {
     if (!w || !tau || !b || !x) {
         set_error("invalid arguments");
         return SAIGE_E_ARG;
@@ -1023,6 +1067,7 @@ int saige_pcg1(saige_ctx* c, const float* w, const float* tau, const float* b, i
     int rc = engine_reserve(c, c->eng1, 1, 1);
     if (rc) return rc;
     Engine& e = c->eng1;
+    e.loco = loco;
     if ((rc = engine_set_system(c, e, 0, w, tau, nullptr))) return rc;
     if ((rc = engine_add(c, e, 0, 0, b, maxiter, tol))) return rc;
     uint8_t done = 0;
@@ -1039,6 +1084,13 @@ int saige_pcg1(saige_ctx* c, const float* w, const float* tau, const float* b, i
     return SAIGE_OK;
 }
 
+int saige_pcg1(saige_ctx* c, const float* w, const float* tau, const float* b, int maxiter, float tol, float* x,
+               int32_t* iters) {
+    CHECK_CTX(c);
+    CHECK_GRM(c);
+    return pcg1(c, 0, w, tau, b, maxiter, tol, x, iters);
+}
+
 int saige_pcg_block(saige_ctx* c, const float* w, const float* tau, const float* B, int64_t k, int maxiter,
                     float tol, float* X, int32_t* iters) {
     CHECK_CTX(c);
@@ -1050,6 +1102,13 @@ int saige_pcg_block(saige_ctx* c, const float* w, const float* tau, const float*
     int rc = engine_reserve(c, c->eng, EW, std::max(1, c->eng.nsys));
     if (rc) return rc;
     Engine& e = c->eng;
+    // getSigma_X solves with the full GRM whatever the engine was pointed at
+    struct LocoGuard {
+        Engine& e;
+        int saved;
+        ~LocoGuard() { e.loco = saved; }
+    } guard{e, e.loco};
+    e.loco = 0;
     for (int k0 = 0; k0 < e.kpad; k0++)
         if (e.h_live[k0]) {
             set_error("engine has live slots; saige_pcg_block needs an idle engine");
@@ -1082,6 +1141,161 @@ int saige_pcg_block(saige_ctx* c, const float* w, const float* tau, const float*
         return SAIGE_E_NOTCONV;
     }
     return SAIGE_OK;
+}
+
+// ---------------------------------------------------------------- LOCO
+
+int saige_set_start_end_index(saige_ctx* c, int64_t start, int64_t end, int chrom) {
+    CHECK_CTX(c);
+    CHECK_GRM(c);
+    if (start < 0 || end < start || end >= c->Mqc) {
+        set_error("LOCO range [%lld, %lld] outside the %lld GRM markers", (long long)start, (long long)end,
+                  (long long)c->Mqc);
+        return SAIGE_E_ARG;
+    }
+    const int64_t cnt = end - start + 1;
+    if (cnt >= c->Mqc) {
+        set_error("LOCO range covers every GRM marker; the leave-out GRM would be empty");
+        return SAIGE_E_ARG;
+    }
+    if (c->eng.loco && c->eng.X) {
+        for (int k = 0; k < c->eng.kpad; k++)
+            if (c->eng.h_live[k]) {
+                set_error("the block engine has live LOCO slots; take them before changing the range");
+                return SAIGE_E_STATE;
+            }
+    }
+    hipStream_t st = c->stream;
+    const int64_t Np = c->Npad, N = c->Nnm;
+    if (!c->d_grm_loco) {
+        if (dalloc(&c->d_grm_loco, Np * Np) || dalloc(&c->d_diag_loco, Np)) {
+            dfree(c->d_grm_loco);
+            dfree(c->d_diag_loco);
+            set_error("cannot allocate the %lld x %lld FP32 LOCO GRM (%.1f GB)", (long long)Np, (long long)Np,
+                      4.0 * Np * Np / 1e9);
+            return SAIGE_E_NOMEM;
+        }
+    }
+    c->loco_ready = false;
+    const std::vector<int2> order = syrk_tile_order(Np / syrk_tile(), 8);
+    int2* d_tiles = nullptr;
+    float* d_rdiag = nullptr;
+    uint8_t* d_tp = nullptr;
+    uint64_t* d_l64 = nullptr;
+    void* d_panel = nullptr;
+    auto release = [&]() {
+        dfree(d_tiles); dfree(d_rdiag); dfree(d_tp); dfree(d_l64);
+        if (d_panel) (void)hipFree(d_panel);
+    };
+    if (dalloc(&d_tiles, (int64_t)order.size()) || dalloc(&d_rdiag, Np)) {
+        release();
+        set_error("device allocation failed (LOCO)");
+        return SAIGE_E_NOMEM;
+    }
+    hipError_t e = hipMemcpyAsync(d_tiles, order.data(), sizeof(int2) * order.size(), hipMemcpyHostToDevice, st);
+    // S_loco = S - Z_r Z_r^T with the kernel and operand precision S was built with
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->d_grm_loco, c->d_grm, sizeof(float) * Np * Np, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess && c->grm_precision == SAIGE_GRM_BF16) {
+        const int64_t nchunk = (c->Mqc + 63) / 64;
+        if (dalloc(&d_tp, (Np / syrk_tile()) * nchunk * 2048) || dalloc(&d_l64, nchunk * 64)) {
+            release();
+            set_error("cannot allocate the tiled genotypes for the LOCO SYRK");
+            return SAIGE_E_NOMEM;
+        }
+        e = launch_pack_tiles(st, c->d_packed, c->d_lut, c->Mqc, N, Np, c->bpm_nm, d_tp, d_l64);
+        if (e == hipSuccess) e = launch_lut64_range(st, c->d_lut, c->Mqc, start, end + 1, d_l64);
+        if (e == hipSuccess)
+            e = launch_syrk_pk(st, d_tp, d_l64, c->Mqc, Np, d_tiles, c->d_grm_loco, -1, start, end + 1);
+    } else if (e == hipSuccess) {
+        const bool bf16 = c->grm_precision == SAIGE_GRM_BF16_PANEL;
+        const int64_t esz = bf16 ? 2 : 4;
+        const int64_t kc = bf16 ? syrk_kchunk_bf16() : syrk_kchunk_f32();
+        size_t free_b = 0, total_b = 0;
+        e = hipMemGetInfo(&free_b, &total_b);
+        int64_t pw = std::min<int64_t>((int64_t)32 << 30, (int64_t)(free_b / 2)) / (Np * esz);
+        pw = std::min(std::max<int64_t>(kc, pw / 64 * 64), round_up(cnt, 64));
+        if (const char* env = std::getenv("SAIGE_GRM_PANEL_MARKERS"))
+            pw = std::min(pw, std::max<int64_t>(64, std::atoll(env) / 64 * 64));
+        if (e == hipSuccess) e = hipMalloc(&d_panel, (size_t)(Np * pw * esz));
+        for (int64_t j0 = start; j0 <= end && e == hipSuccess; j0 += pw) {
+            const int64_t np = std::min(pw, end + 1 - j0);
+            const int64_t ldz = round_up(np, 64);
+            if (bf16) {
+                e = launch_std_panel_bf16(st, c->d_packed, c->d_lut, j0, np, N, Np, c->bpm_nm, ldz,
+                                          static_cast<__hip_bfloat16*>(d_panel));
+                if (e == hipSuccess)
+                    e = launch_syrk_bf16(st, static_cast<__hip_bfloat16*>(d_panel), Np, ldz, ldz, d_tiles,
+                                         c->d_grm_loco, -1);
+            } else {
+                e = launch_std_panel_f32(st, c->d_packed, c->d_lut, j0, np, N, Np, c->bpm_nm, ldz,
+                                         static_cast<float*>(d_panel));
+                if (e == hipSuccess)
+                    e = launch_syrk_f32(st, static_cast<float*>(d_panel), Np, ldz, ldz, d_tiles, c->d_grm_loco,
+                                        -1);
+            }
+        }
+    }
+    // diag_loco = diag - sum over the range of z^2, the range summed from zero in
+    // marker order (set_Diagof_StdGeno_LOCO, FG.cpp:4947-4955)
+    if (e == hipSuccess) e = launch_fill_zero(st, d_rdiag, 4 * Np);
+    if (e == hipSuccess)
+        e = launch_diag_std(st, c->d_packed + start * c->bpm_nm, c->d_lut + start, cnt, N, c->bpm_nm, d_rdiag);
+    if (e == hipSuccess) e = launch_sub(st, c->d_diag, d_rdiag, Np, c->d_diag_loco);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    release();
+    if (e != hipSuccess) {
+        set_error("LOCO GRM build failed: %s", hipGetErrorString(e));
+        return SAIGE_E_DEVICE;
+    }
+    c->loco_start = start;
+    c->loco_end = end;
+    c->loco_chrom = chrom;
+    c->loco_m = c->Mqc - cnt;
+    c->loco_ready = true;
+    return SAIGE_OK;
+}
+
+int saige_loco_range(const saige_ctx* c, int64_t* start, int64_t* end, int* chrom, int64_t* m_loco) {
+    if (!c) {
+        set_error("null context");
+        return SAIGE_E_ARG;
+    }
+    const bool r = c->loco_ready;
+    if (start) *start = r ? c->loco_start : -1;
+    if (end) *end = r ? c->loco_end : -1;
+    if (chrom) *chrom = r ? c->loco_chrom : -1;
+    if (m_loco) *m_loco = r ? c->loco_m : -1;
+    return SAIGE_OK;
+}
+
+int saige_crossprod_mat_and_kin_loco(saige_ctx* c, const float* b, float* out) {
+    CHECK_CTX(c);
+    CHECK_GRM(c);
+    if (int rc = check_loco(c)) return rc;
+    return grm_matvec(c, 1, b, 1, out);
+}
+
+int saige_get_diag_of_sigma_loco(saige_ctx* c, const float* w, const float* tau, float* out) {
+    CHECK_CTX(c);
+    CHECK_GRM(c);
+    if (int rc = check_loco(c)) return rc;
+    return get_diag_of_sigma(c, 1, w, tau, out);
+}
+
+int saige_get_crossprod_loco(saige_ctx* c, const float* b, const float* w, const float* tau, float* out) {
+    CHECK_CTX(c);
+    CHECK_GRM(c);
+    if (int rc = check_loco(c)) return rc;
+    return get_crossprod(c, 1, b, w, tau, out);
+}
+
+int saige_pcg1_loco(saige_ctx* c, const float* w, const float* tau, const float* b, int maxiter, float tol,
+                    float* x, int32_t* iters) {
+    CHECK_CTX(c);
+    CHECK_GRM(c);
+    if (int rc = check_loco(c)) return rc;
+    return pcg1(c, 1, w, tau, b, maxiter, tol, x, iters);
 }
 
 int saige_engine_reserve(saige_ctx* c, int width, int n_systems) {
@@ -1161,6 +1375,120 @@ int saige_engine_take(saige_ctx* c, int slot, float* x) {
         return SAIGE_E_STATE;
     }
     return engine_take(c, c->eng, slot, x);
+}
+
+int saige_engine_use_loco(saige_ctx* c, int on) {
+    CHECK_CTX(c);
+    CHECK_GRM(c);
+    if (on && !c->loco_ready) return check_loco(c);
+    if (c->eng.X)
+        for (int k = 0; k < c->eng.kpad; k++)
+            if (c->eng.h_live[k]) {
+                set_error("engine has live slots; switch the GRM with an idle engine");
+                return SAIGE_E_STATE;
+            }
+    c->eng.loco = on ? 1 : 0;
+    return SAIGE_OK;
+}
+
+// ------------------------------------------------------------ autotuner
+
+namespace {
+// Block-GEMM shapes tried by saige_autotune, in the order of its table.
+constexpr GemmConfig kGemmCandidates[] = {{1, 1}, {1, 2}, {1, 4}, {1, 8}, {2, 1}, {2, 2}, {2, 4}, {2, 8}};
+constexpr int kNumGemmCandidates = (int)(sizeof(kGemmCandidates) / sizeof(kGemmCandidates[0]));
+
+int engine_idle(const saige_ctx* c, const char* what) {
+    if (c->eng.X)
+        for (int k = 0; k < c->eng.kpad; k++)
+            if (c->eng.h_live[k]) {
+                set_error("engine has live slots; %s needs an idle engine", what);
+                return SAIGE_E_STATE;
+            }
+    return SAIGE_OK;
+}
+
+int ensure_mv_scratch(saige_ctx* c) {
+    if (c->d_mv_in) return SAIGE_OK;
+    const int64_t Np = c->Npad, N = c->Nnm;
+    if (dalloc(&c->d_mv_in, N * EW) || dalloc(&c->d_mv_blk, Np * EW) || dalloc(&c->d_mv_out, Np * EW) ||
+        dalloc(&c->d_mv_ypart, gemm_ypart_floats(Np))) {
+        dfree(c->d_mv_in); dfree(c->d_mv_blk); dfree(c->d_mv_out); dfree(c->d_mv_ypart);
+        set_error("device allocation failed (block product scratch)");
+        return SAIGE_E_NOMEM;
+    }
+    return SAIGE_OK;
+}
+}  // namespace
+
+int saige_autotune(saige_ctx* c, int reps, double* ms_table) {
+    CHECK_CTX(c);
+    CHECK_GRM(c);
+    if (int rc = engine_idle(c, "saige_autotune")) return rc;
+    if (int rc = ensure_mv_scratch(c)) return rc;
+    reps = std::max(1, std::min(reps, 100));
+    hipStream_t st = c->stream;
+    const int64_t Np = c->Npad, N = c->Nnm;
+    SAIGE_HIP_TRY(launch_fill_zero(st, c->d_mv_blk, sizeof(float) * Np * EW));
+    SigmaArgs a{};
+    a.mqc = (float)c->Mqc;
+    a.mode = 0;
+    GemmConfig best = c->gemm_cfg;
+    double best_ms = 0.0;
+    for (int i = 0; i < kNumGemmCandidates; i++) {
+        const GemmConfig g = kGemmCandidates[i];
+        SAIGE_HIP_TRY(launch_grm_gemm(st, c->d_grm, Np, N, c->d_mv_blk, c->d_mv_out, a, nullptr, nullptr,
+                                      c->d_mv_ypart, g));   // warm-up
+        SAIGE_HIP_TRY(hipEventRecord(c->ev0, st));
+        for (int r = 0; r < reps; r++)
+            SAIGE_HIP_TRY(launch_grm_gemm(st, c->d_grm, Np, N, c->d_mv_blk, c->d_mv_out, a, nullptr, nullptr,
+                                          c->d_mv_ypart, g));
+        SAIGE_HIP_TRY(hipEventRecord(c->ev1, st));
+        SAIGE_HIP_TRY(hipEventSynchronize(c->ev1));
+        float ms = 0.f;
+        SAIGE_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        const double per = (double)ms / reps;
+        if (ms_table) ms_table[i] = per;
+        if (i == 0 || per < best_ms) {
+            best = g;
+            best_ms = per;
+        }
+    }
+    c->gemm_cfg = best;
+    c->gemm_tuned_ms = best_ms;
+    return SAIGE_OK;
+}
+
+int saige_autotune_candidates(int* rt, int* nsplit) {
+    for (int i = 0; i < kNumGemmCandidates; i++) {
+        if (rt) rt[i] = kGemmCandidates[i].rt;
+        if (nsplit) nsplit[i] = kGemmCandidates[i].nsplit;
+    }
+    return kNumGemmCandidates;
+}
+
+int saige_set_gemm_config(saige_ctx* c, int rt, int nsplit) {
+    CHECK_CTX(c);
+    const GemmConfig g{rt, nsplit};
+    if (!gemm_config_valid(g)) {
+        set_error("invalid block-GEMM shape rt=%d nsplit=%d (rt 1|2, nsplit 1|2|4|8)", rt, nsplit);
+        return SAIGE_E_ARG;
+    }
+    if (int rc = engine_idle(c, "changing the block-GEMM shape")) return rc;
+    c->gemm_cfg = g;
+    c->gemm_tuned_ms = 0.0;
+    return SAIGE_OK;
+}
+
+int saige_get_gemm_config(const saige_ctx* c, int* rt, int* nsplit, double* tuned_ms) {
+    if (!c) {
+        set_error("null context");
+        return SAIGE_E_ARG;
+    }
+    if (rt) *rt = c->gemm_cfg.rt;
+    if (nsplit) *nsplit = c->gemm_cfg.nsplit;
+    if (tuned_ms) *tuned_ms = c->gemm_tuned_ms;
+    return SAIGE_OK;
 }
 
 int saige_engine_stats(const saige_ctx* c, double* secs, int64_t* launches) {

@@ -44,9 +44,18 @@ constexpr int EW = 64;
 // Row padding of all N-length device arrays and of the GRM.
 constexpr int64_t NPAD_ALIGN = 256;
 
+// Block-GEMM shape: rt = 32-row tiles per wave (1 or 2), nsplit = column
+// slices of S (1, 2, 4 or 8).  The slice count fixes the summation order, so
+// results are reproducible for a fixed configuration.
+struct GemmConfig {
+    int rt = 1;
+    int nsplit = 4;
+};
+
 struct Engine {
     int width = 0;        // live slot capacity (<= EW)
     int nsys = 0;
+    int loco = 0;         // 1: products with the LOCO GRM S_loco, divisor M_loco
     int kpad = 0;         // 1 (GEMV) or EW (MFMA GEMM)
     // [Npad][kpad] row-major column blocks
     float* X = nullptr; float* R = nullptr; float* Z = nullptr; float* P = nullptr; float* AP = nullptr;
@@ -111,7 +120,16 @@ struct saige_ctx {
     float* d_diag = nullptr;                     // [Npad] sum z^2
     // GRM S = Z Z^T, [Npad][Npad] FP32
     float* d_grm = nullptr;
+    // LOCO (setStartEndIndex, FG.cpp:3067): S minus one range of GRM markers
+    float* d_grm_loco = nullptr;                 // [Npad][Npad]
+    float* d_diag_loco = nullptr;                // [Npad]
+    bool loco_ready = false;
+    int64_t loco_start = -1, loco_end = -1;      // inclusive range in the QC'd marker list
+    int64_t loco_m = 0;                          // Mqc - (end - start + 1)
+    int loco_chrom = -1;
     bool grm_ready = false;
+    saige::GemmConfig gemm_cfg;                  // block-GEMM shape (saige_autotune)
+    double gemm_tuned_ms = 0.0;                  // its measured time per launch, 0 if untuned
     int grm_precision = -1;
     double grm_seconds = 0.0;
     // scratch vectors
@@ -185,8 +203,13 @@ std::vector<int2> syrk_tile_order(int64_t nt, int64_t g);
 // nchunk = ceil(Mqc/64)), lut64 = nchunk*64 entries; both from launch_pack_tiles
 hipError_t launch_pack_tiles(hipStream_t s, const uint8_t* packed, const float4* lut, int64_t Mqc, int64_t Nnm,
                              int64_t Npad, int64_t bpm_nm, uint8_t* tp, uint64_t* lut64);
+// markers [j0, j1) only (j1 < 0: all); lut64 must be zero outside the range.
+// accumulate: 0 overwrite, 1 add, -1 subtract.
 hipError_t launch_syrk_pk(hipStream_t s, const uint8_t* tp, const uint64_t* lut64, int64_t Mqc, int64_t Npad,
-                          const int2* tiles, float* S, int accumulate);
+                          const int2* tiles, float* S, int accumulate, int64_t j0 = 0, int64_t j1 = -1);
+hipError_t launch_lut64_range(hipStream_t s, const float4* lut, int64_t Mqc, int64_t j0, int64_t j1,
+                              uint64_t* lut64);
+hipError_t launch_sub(hipStream_t s, const float* a, const float* b, int64_t n, float* out);
 int syrk_tile();
 int syrk_kchunk_f32();
 int syrk_kchunk_bf16();
@@ -208,9 +231,11 @@ struct SigmaArgs {
 // mode == 1 (partial: [nparts][kpad][2], slot 0).  Returns nparts used.
 hipError_t launch_grm_gemv(hipStream_t s, const float* S, int64_t Npad, int64_t Nnm, const float* in,
                            float* out, SigmaArgs a, double* partial, int64_t* nparts);
+bool gemm_config_valid(GemmConfig g);
 // ypart: gemm_ypart_floats(Npad) floats of scratch for the split-K slices
 hipError_t launch_grm_gemm(hipStream_t s, const float* S, int64_t Npad, int64_t Nnm, const float* in,
-                           float* out, SigmaArgs a, double* partial, int64_t* nparts, float* ypart);
+                           float* out, SigmaArgs a, double* partial, int64_t* nparts, float* ypart,
+                           GemmConfig g);
 int64_t gemm_ypart_floats(int64_t Npad);
 int64_t gemv_max_parts(int64_t Npad);
 int64_t gemm_max_parts(int64_t Npad);

@@ -68,6 +68,13 @@ SIGNATURES = {
     "saige_get_diag_of_sigma": (_int, [_c_p, _c_p, _c_p, _c_p]),
     "saige_get_crossprod": (_int, [_c_p, _c_p, _c_p, _c_p, _c_p]),
     "saige_pcg1": (_int, [_c_p, _c_p, _c_p, _c_p, _int, _f32, _c_p, _c_p]),
+    "saige_set_start_end_index": (_int, [_c_p, _i64, _i64, _int]),
+    "saige_loco_range": (_int, [_c_p, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_int),
+                                ctypes.POINTER(_i64)]),
+    "saige_crossprod_mat_and_kin_loco": (_int, [_c_p, _c_p, _c_p]),
+    "saige_get_diag_of_sigma_loco": (_int, [_c_p, _c_p, _c_p, _c_p]),
+    "saige_get_crossprod_loco": (_int, [_c_p, _c_p, _c_p, _c_p, _c_p]),
+    "saige_pcg1_loco": (_int, [_c_p, _c_p, _c_p, _c_p, _int, _f32, _c_p, _c_p]),
     "saige_pcg_block": (_int, [_c_p, _c_p, _c_p, _c_p, _i64, _int, _f32, _c_p, _c_p]),
     "saige_engine_reserve": (_int, [_c_p, _int, _int]),
     "saige_engine_set_system": (_int, [_c_p, _int, _c_p, _c_p, _c_p]),
@@ -77,6 +84,11 @@ SIGNATURES = {
     "saige_engine_take_batch": (_int, [_c_p, _int, _c_p, _c_p]),
     "saige_engine_step": (_int, [_c_p, _int, _c_p, _c_p]),
     "saige_engine_take": (_int, [_c_p, _int, _c_p]),
+    "saige_engine_use_loco": (_int, [_c_p, _int]),
+    "saige_autotune": (_int, [_c_p, _int, _c_p]),
+    "saige_autotune_candidates": (_int, [_c_p, _c_p]),
+    "saige_set_gemm_config": (_int, [_c_p, _int, _int]),
+    "saige_get_gemm_config": (_int, [_c_p, ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.POINTER(_f64)]),
     "saige_engine_stats": (_int, [_c_p, ctypes.POINTER(_f64), ctypes.POINTER(_i64)]),
     "saige_device_memory": (_int, [_c_p, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "saige_engine_system_bytes": (_i64, [_c_p]),

@@ -225,6 +225,73 @@ class Session:
         check(rc, allow_notconv=True)   # the reference warns and returns x (FG.cpp:2794-2797)
         return (x, int(it.value)) if return_iters else x
 
+    # ------------------------------------------------------------ LOCO
+    def setStartEndIndex(self, startIndex, endIndex, chromIndex):   # FG.cpp:3067-3082, 4934-4958
+        """Leave GRM markers [startIndex, endIndex] (QC'd-list indices,
+        inclusive) out of K: builds S_loco and diag_loco on the device."""
+        check(self._L.saige_set_start_end_index(self.ctx, int(startIndex), int(endIndex), int(chromIndex)))
+
+    def loco_range(self):
+        s, e, m = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
+        ch = ctypes.c_int(0)
+        check(self._L.saige_loco_range(self.ctx, ctypes.byref(s), ctypes.byref(e), ctypes.byref(ch),
+                                       ctypes.byref(m)))
+        return int(s.value), int(e.value), int(ch.value), int(m.value)
+
+    def getCrossprodMatAndKin_LOCO(self, bVec):                     # FG.cpp:1989-2000
+        b = f32(bVec)
+        out = np.zeros_like(b)
+        check(self._L.saige_crossprod_mat_and_kin_loco(self.ctx, ptr(b), ptr(out)))
+        return out
+
+    def getDiagOfSigma_LOCO(self, wVec, tauVec):                    # FG.cpp:2368-2392
+        w, tau = f32(wVec), f32(tauVec)
+        out = np.zeros_like(w)
+        check(self._L.saige_get_diag_of_sigma_loco(self.ctx, ptr(w), ptr(tau), ptr(out)))
+        return out
+
+    def getCrossprod_LOCO(self, bVec, wVec, tauVec):                # FG.cpp:2430-2460
+        b, w, tau = f32(bVec), f32(wVec), f32(tauVec)
+        out = np.zeros_like(b)
+        check(self._L.saige_get_crossprod_loco(self.ctx, ptr(b), ptr(w), ptr(tau), ptr(out)))
+        return out
+
+    def getPCG1ofSigmaAndVector_LOCO(self, wVec, tauVec, bVec, maxiterPCG=500, tolPCG=1e-5,
+                                     return_iters=False):           # FG.cpp:2943-3030
+        w, tau, b = f32(wVec), f32(tauVec), f32(bVec)
+        x = np.zeros_like(b)
+        it = ctypes.c_int32(0)
+        rc = self._L.saige_pcg1_loco(self.ctx, ptr(w), ptr(tau), ptr(b), int(maxiterPCG), float(tolPCG),
+                                     ptr(x), ctypes.byref(it))
+        check(rc, allow_notconv=True)
+        return (x, int(it.value)) if return_iters else x
+
+    # ------------------------------------------------------- autotuner
+    def autotune(self, reps=3):
+        """Time every block-GEMM shape on the resident GRM and keep the
+        fastest.  Returns {"rt", "nsplit", "ms", "table": [(rt, nsplit, ms)]}."""
+        n = self._L.saige_autotune_candidates(None, None)
+        rt = np.zeros(n, np.int32)
+        ns = np.zeros(n, np.int32)
+        self._L.saige_autotune_candidates(ptr(rt), ptr(ns))
+        ms = np.zeros(n, np.float64)
+        check(self._L.saige_autotune(self.ctx, int(reps), ptr(ms)))
+        cfg = self.gemm_config()
+        cfg["table"] = [(int(a), int(b), float(t)) for a, b, t in zip(rt, ns, ms)]
+        return cfg
+
+    def set_gemm_config(self, rt, nsplit):
+        check(self._L.saige_set_gemm_config(self.ctx, int(rt), int(nsplit)))
+
+    def gemm_config(self):
+        rt, ns, ms = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_double(0.0)
+        check(self._L.saige_get_gemm_config(self.ctx, ctypes.byref(rt), ctypes.byref(ns), ctypes.byref(ms)))
+        return {"rt": int(rt.value), "nsplit": int(ns.value), "ms": float(ms.value)}
+
+    def engine_use_loco(self, on):
+        """Point the block engine at the LOCO GRM (on) or the full one."""
+        check(self._L.saige_engine_use_loco(self.ctx, 1 if on else 0))
+
     def pcg_block(self, wVec, tauVec, B, maxiterPCG=500, tolPCG=1e-5):
         """Block PCG: k right-hand sides, same Sigma, one GEMM per iteration."""
         w, tau = f32(wVec), f32(tauVec)

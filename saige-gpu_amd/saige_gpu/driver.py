@@ -409,6 +409,49 @@ def fit_co(trait, y, X, probes, opt, **kw):
     raise ValueError(f"unknown trait type {trait!r}")
 
 
+# ------------------------------------------------ leave one chromosome out
+
+def update_chr_start_end_index(chr_vec):
+    """updateChrStartEndIndexVec (Util.R:29-68): for chromosomes 1..22 the
+    first and last 0-based position of that chromosome in chr_vec (None where
+    absent).  Raises if the chromosomes are not in order; LOCO is off when
+    fewer than two autosomes are present.  Returns (loco, starts, ends)."""
+    chr_vec = [str(c).strip() for c in chr_vec]
+    starts, ends = [], []
+    for i in range(1, 23):
+        pos = [k for k, c in enumerate(chr_vec) if c == str(i)]
+        if pos:
+            starts.append(min(pos))
+            ends.append(max(pos))
+            prev_s, prev_e = starts[i - 2] if i > 1 else None, ends[i - 2] if i > 1 else None
+            if prev_s is not None and (starts[-1] <= prev_s or ends[-1] <= prev_e):
+                raise ValueError("ERROR! chromosomes need to be ordered from 1 to 22 in the bim file.")
+        else:
+            starts.append(None)
+            ends.append(None)
+    present = sum(s is not None for s in starts)
+    return present > 1, starts, ends
+
+
+def loco_co(fit, y, X, trait, opt, offset=None):
+    """One chromosome of fitNULLGLMM's LOCO loop (FG.R:1240-1272 binary,
+    FG.R:1486-1512 quantitative): Get_Coef_LOCO (FG.R:42-73) from the full
+    fit's theta, coefficients and linear predictors, whose solves the
+    executor runs on Sigma_loco, then ScoreTest_NULL_Model with mu2 =
+    mu(1-mu) (binary) or 1/tau[0] (quantitative)."""
+    y = np.asarray(y, np.float64)
+    X = np.asarray(X, np.float64)
+    offset = np.zeros(len(y)) if offset is None else np.asarray(offset, np.float64)
+    fam = Binomial if trait == "binary" else Gaussian
+    tau = np.asarray(fit["theta"], np.float64)
+    rc = yield from get_coef_co(y, X, tau, fam, fit["coefficients"], fit["linear_predictors"], offset, opt)
+    mu = rc["mu"]
+    mu2 = mu * (1 - mu) if trait == "binary" else np.full(len(y), 1.0 / tau[0])
+    return {"isLOCO": True, "coefficients": rc["alpha"], "linear_predictors": rc["eta"],
+            "fitted_values": mu, "Y": rc["Y"], "residuals": y - mu, "cov": rc["cov"],
+            "obj_noK": score_test_null(mu, mu2, y, X)}
+
+
 # ---------------------------------------------------------- variance ratio
 
 def score_test_null(mu, mu2, y, X):
@@ -419,7 +462,7 @@ def score_test_null(mu, mu2, y, X):
     XVX_inv = np.linalg.inv(XVX)
     XXVX_inv = X @ XVX_inv
     return {"XV": XV, "XVX": XVX, "XXVX_inv": XXVX_inv, "XVX_inv": XVX_inv,
-            "S_a": (X * (y - mu)[:, None]).sum(0), "V": V}
+            "S_a": (X * (y - mu)[:, None]).sum(0), "XVX_inv_XV": XXVX_inv * V[:, None], "V": V}
 
 
 def variance_ratio_co(fit, y, X, trait, genos, categories, opt, num_markers=30, ratio_cv_cutoff=0.001):
@@ -490,14 +533,18 @@ def write_variance_ratio(path, table):
 
 class PlainExecutor:
     """Runs a fit coroutine the way the reference does: every right-hand side
-    is its own single-RHS PCG (saige_pcg1), every K b its own GEMV."""
+    is its own single-RHS PCG (saige_pcg1), every K b its own GEMV.  With
+    loco=True they are the _LOCO variants (getPCG1ofSigmaAndVector_LOCO,
+    getCrossprodMatAndKin_LOCO) on the range set by setStartEndIndex."""
 
-    def __init__(self, sess, opt):
+    def __init__(self, sess, opt, loco=False):
         self.sess = sess
         self.opt = opt
         self.n_solves = 0
         self.pcg_iters = 0
         self.n_products = 0
+        self._pcg = sess.getPCG1ofSigmaAndVector_LOCO if loco else sess.getPCG1ofSigmaAndVector
+        self._kin = sess.getCrossprodMatAndKin_LOCO if loco else sess.getCrossprodMatAndKin
 
     def run(self, gen):
         value = None
@@ -511,16 +558,15 @@ class PlainExecutor:
                 tau32 = np.asarray(tau, np.float32)
                 out = np.zeros(B.shape, np.float64)
                 for j in range(B.shape[1]):
-                    x, it = self.sess.getPCG1ofSigmaAndVector(w, tau32, B[:, j], self.opt.maxiterPCG,
-                                                              self.opt.tolPCG, return_iters=True)
+                    x, it = self._pcg(w, tau32, B[:, j], self.opt.maxiterPCG, self.opt.tolPCG,
+                                      return_iters=True)
                     out[:, j] = x
                     self.n_solves += 1
                     self.pcg_iters += it
                 value = out
             else:
                 _, B = req
-                value = np.column_stack([self.sess.getCrossprodMatAndKin(B[:, j])
-                                         for j in range(B.shape[1])]).astype(np.float64)
+                value = np.column_stack([self._kin(B[:, j]) for j in range(B.shape[1])]).astype(np.float64)
                 self.n_products += B.shape[1]
 
 
@@ -767,6 +813,51 @@ def variance_ratio(sess, fit, y, X, trait, categories, opt=None, vr_pool=True, n
     if h.error is not None:
         raise h.error
     return h.result
+
+
+def loco(sess, jobs, starts, ends, opt=None, batched=True, width=64, n_systems=16, timing=None):
+    """fitNULLGLMM's LOCO loop over chromosomes 1..22 for many fitted nulls.
+    jobs: dicts with fit (a fit result), trait, y, X and optionally subset.
+    starts/ends: update_chr_start_end_index's ranges, as indices into the
+    GRM's QC'd marker list.  Per chromosome: setStartEndIndex builds the
+    leave-out GRM on the device once, then every job's Get_Coef_LOCO runs,
+    batched through the scheduler on the LOCO GRM or one solve at a time.
+    Returns, per job, a list of 22 results ({"isLOCO": False} where the
+    chromosome is absent).  timing: optional Counter, gets the wall seconds
+    of the range builds ("set_range") and of the fits ("coef")."""
+    opt = opt or Options()
+    timing = collections.Counter() if timing is None else timing
+    out = [[] for _ in jobs]
+    for j in range(22):
+        if starts[j] is None or ends[j] is None:
+            for r in out:
+                r.append({"isLOCO": False})
+            continue
+        t = time.perf_counter()
+        sess.setStartEndIndex(starts[j], ends[j], j)
+        timing["set_range"] += time.perf_counter() - t
+        t = time.perf_counter()
+        if batched:
+            sch = BatchScheduler(sess, opt, width=width, n_systems=n_systems)
+            sess.engine_use_loco(True)
+            try:
+                hs = [sch.submit(loco_co(jb["fit"], jb["y"], jb["X"], jb["trait"], opt), jb.get("subset"))
+                      for jb in jobs]
+                sch.run()
+            finally:
+                sess.engine_use_loco(False)
+            for r, h in zip(out, hs):
+                if h.error is not None:
+                    raise h.error
+                r.append(h.result)
+        else:
+            if any(jb.get("subset") is not None for jb in jobs):
+                raise ValueError("sample subsets need the batched path")
+            ex = PlainExecutor(sess, opt, loco=True)
+            for r, jb in zip(out, jobs):
+                r.append(ex.run(loco_co(jb["fit"], jb["y"], jb["X"], jb["trait"], opt)))
+        timing["coef"] += time.perf_counter() - t
+    return out
 
 
 def fit_batched(sess, jobs, opt=None, width=64, n_systems=16, probe_seed=200):

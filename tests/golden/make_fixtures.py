@@ -61,6 +61,13 @@ def main(ref_root):
     subset(os.path.join(inp, pre + "_22chr.acount"), os.path.join(OUT, "acount_subset.tsv"))
     subset(os.path.join(inp, pre + ".vmiss"), os.path.join(OUT, "vmiss_subset.tsv"))
 
+    # null models saved by the reference's fitNULLGLMM (gzip'd R data files,
+    # read by the tests with a data-only parser) and its variance-ratio file
+    out = os.path.join(ref_root, "src", "SAIGE", "extdata", "output")
+    for name in ("example_binary_fullGRM.rda", "example_quantitative_fullGRM.rda",
+                 "example_binary_fullGRM.varianceRatio.txt"):
+        shutil.copyfile(os.path.join(out, name), os.path.join(OUT, name))
+
 
 if __name__ == "__main__":
     main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
