@@ -173,7 +173,7 @@ def c4_leg(args):
     grm_tflops = N * (N + 1) * Mqc / grm_s / 1e12
     log(f"C4: Mqc={Mqc}, VR markers {len(vr_index)}, GRM {args.grm} {grm_s:.2f}s {grm_tflops:.0f} TFLOP/s")
     tune = sess.autotune(reps=3)
-    log(f"C4 autotune: rt{tune['rt']}/s{tune['nsplit']} {tune['ms']:.2f} ms")
+    log(f"C4 autotune: rt{tune['rt']}/pd{tune['pd']}/s{tune['nsplit']} {tune['ms']:.2f} ms")
     opt = D.Options(nrun=30)
     g0, n0 = sess.engine_stats()
     t0 = time.perf_counter()
@@ -249,8 +249,8 @@ def main():
     grm_tflops = N * (N + 1) * Mqc / grm_s / 1e12
     log(f"GRM {args.grm}: {grm_s:.3f}s device time, {grm_tflops:.1f} TFLOP/s")
     tune = sess.autotune(reps=3)
-    log("autotune block GEMM: " + ", ".join(f"rt{a}/s{b} {t:.2f}ms" for a, b, t in tune["table"]) +
-        f" -> rt{tune['rt']}/s{tune['nsplit']}")
+    log("autotune block GEMM: " + ", ".join(f"rt{a}/pd{b}/s{c} {t:.2f}ms" for a, b, c, t in tune["table"]) +
+        f" -> rt{tune['rt']}/pd{tune['pd']}/s{tune['nsplit']}")
 
     opt = D.Options(nrun=args.nrun)
     jobs = [{"trait": "binary", "y": y, "X": X, "tag": f"prev{p}"} for y, p in zip(phenos, prevalences)]

@@ -245,21 +245,22 @@ int saige_engine_use_loco(saige_ctx* ctx, int on);
 /* ------------------------------------------------------------ autotuner
  * No reference counterpart (the reference's GRM product is one cuBLAS
  * SGEMV per right-hand side, GPU.cu:211-252).  The block-PCG GEMM
- * Y = S P (64 columns) has two shape parameters: rt, the 32-row tiles per
- * wave (1 or 2), and nsplit, the column slices of S summed in a fixed order
- * (1, 2, 4 or 8).  saige_autotune times every candidate on the resident GRM
- * (reps launches each after one warm-up) and keeps the fastest; ms_table
- * (may be NULL) receives the per-launch milliseconds in the order of
- * saige_autotune_candidates.  The shape changes the summation order, so call
- * it once before solving; results are reproducible for a fixed shape.
+ * Y = S P (64 columns) has three shape parameters: rt, the 32-row tiles per
+ * wave (1 or 2); pd, the 32-column chunks of S requested ahead of use (1 or
+ * 3); nsplit, the column slices of S summed in a fixed order (1, 2, 4 or 8).
+ * saige_autotune times every candidate on the resident GRM (reps launches
+ * each after one warm-up) and keeps the fastest; ms_table (may be NULL)
+ * receives the per-launch milliseconds in the order of
+ * saige_autotune_candidates.  nsplit changes the summation order, so call it
+ * once before solving; results are reproducible for a fixed shape.
  * SAIGE_E_STATE if the engine has live slots. */
 int saige_autotune(saige_ctx* ctx, int reps, double* ms_table);
-/* The candidate shapes (rt[i], nsplit[i]); returns their count (8). */
-int saige_autotune_candidates(int* rt, int* nsplit);
+/* The candidate shapes (rt[i], pd[i], nsplit[i]); returns their count (16). */
+int saige_autotune_candidates(int* rt, int* pd, int* nsplit);
 /* Set the block-GEMM shape explicitly (engine idle). */
-int saige_set_gemm_config(saige_ctx* ctx, int rt, int nsplit);
+int saige_set_gemm_config(saige_ctx* ctx, int rt, int pd, int nsplit);
 /* Current shape, and its tuned per-launch time in ms (0 if not tuned). */
-int saige_get_gemm_config(const saige_ctx* ctx, int* rt, int* nsplit, double* tuned_ms);
+int saige_get_gemm_config(const saige_ctx* ctx, int* rt, int* pd, int* nsplit, double* tuned_ms);
 
 /* Free and total device memory (hipMemGetInfo), for admission control. */
 int saige_device_memory(saige_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);

@@ -112,16 +112,20 @@ constexpr int GKS = 128;                // columns of S per P superchunk
 constexpr int GKP = GKS + 4;            // LDS pitch of sP (floats)
 constexpr int GSPLIT_MAX = 8;           // most column slices
 
-template <int RT>
+template <int RT, int PD>
 __global__ __launch_bounds__(256) void k_gemm_part(const float* __restrict__ S, int64_t ld,
                                                    const float* __restrict__ in, float* __restrict__ ypart) {
     constexpr int GM = 128 * RT;        // rows per workgroup
+    constexpr int CPS = GKS / 32;       // 32-column chunks per superchunk
+    constexpr int RING = PD + 1;        // S register buffers; RING divides CPS, so a
+    static_assert(CPS % RING == 0, "");  // superchunk always starts at ring slot 0
     __shared__ __attribute__((aligned(16))) float sP[EW * GKP];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int64_t r0 = (int64_t)blockIdx.x * GM;
     const int split = blockIdx.y, nsplit = gridDim.y;
     const int64_t nsc = ld / GKS;                           // superchunks in a row
     const int64_t sc0 = nsc * split / nsplit, sc1 = nsc * (split + 1) / nsplit;
+    const int64_t kend = sc1 * GKS;                         // end column of this slice
     const int hl = lane >> 5;                               // k half of this lane
     // wave wv owns rows r0 + wv*32*RT + t*32 + [0, 32) for tiles t < RT
     const float* srow = S + (r0 + wv * 32 * RT + (lane & 31)) * ld + hl * 16;
@@ -136,15 +140,23 @@ __global__ __launch_bounds__(256) void k_gemm_part(const float* __restrict__ S, 
 #pragma unroll
             for (int r = 0; r < 16; r++) acc[t][ni][r] = 0.0f;
 
-    float4 sa[RT][4], sb[RT][4];
+    // ring[slot][t][q]: the chunk starting at column kc lives in slot
+    // (kc / 32) % RING; chunk kc + 32*PD is requested while chunk kc is used
+    float4 ring[RING][RT][4];
     float pr[32];
     if (sc0 < sc1) {
         const int64_t k0 = sc0 * GKS;
 #pragma unroll
-        for (int t = 0; t < RT; t++)
+        for (int d = 0; d < PD; d++) {
+            if (k0 + 32 * d < kend) {
 #pragma unroll
-            for (int q = 0; q < 4; q++)
-                sa[t][q] = *reinterpret_cast<const float4*>(srow + (int64_t)t * 32 * ld + k0 + 4 * q);
+                for (int t = 0; t < RT; t++)
+#pragma unroll
+                    for (int q = 0; q < 4; q++)
+                        ring[d][t][q] =
+                            *reinterpret_cast<const float4*>(srow + (int64_t)t * 32 * ld + k0 + 32 * d + 4 * q);
+            }
+        }
 #pragma unroll
         for (int q = 0; q < 32; q++) pr[q] = in[(k0 + pq * 32 + q) * EW + pn];
     }
@@ -162,15 +174,16 @@ __global__ __launch_bounds__(256) void k_gemm_part(const float* __restrict__ S, 
             for (int q = 0; q < 32; q++) pr[q] = in[(kn + pq * 32 + q) * EW + pn];
         }
 #pragma unroll
-        for (int c = 0; c < GKS / 32; c++) {
-            // prefetch the next 32-column chunk of this wave's S rows
-            const int64_t kc_next = (c + 1 < GKS / 32) ? k0 + (c + 1) * 32 : ((sc + 1 < sc1) ? k0 + GKS : -1);
-            if (kc_next >= 0) {
+        for (int c = 0; c < CPS; c++) {
+            // request chunk c + PD of this wave's S rows (it may lie in the next superchunk)
+            const int64_t kf = k0 + 32 * (c + PD);
+            if (kf < kend) {
 #pragma unroll
                 for (int t = 0; t < RT; t++)
 #pragma unroll
                     for (int q = 0; q < 4; q++)
-                        sb[t][q] = *reinterpret_cast<const float4*>(srow + (int64_t)t * 32 * ld + kc_next + 4 * q);
+                        ring[(c + PD) % RING][t][q] =
+                            *reinterpret_cast<const float4*>(srow + (int64_t)t * 32 * ld + kf + 4 * q);
             }
             const float* pb = sP + c * 32 + hl * 16;
 #pragma unroll
@@ -179,20 +192,17 @@ __global__ __launch_bounds__(256) void k_gemm_part(const float* __restrict__ S, 
                 const float4 b1 = *reinterpret_cast<const float4*>(pb + (32 + (lane & 31)) * GKP + 4 * q);
 #pragma unroll
                 for (int t = 0; t < RT; t++) {
-                    acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[t][q].x, b0.x, acc[t][0], 0, 0, 0);
-                    acc[t][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[t][q].x, b1.x, acc[t][1], 0, 0, 0);
-                    acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[t][q].y, b0.y, acc[t][0], 0, 0, 0);
-                    acc[t][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[t][q].y, b1.y, acc[t][1], 0, 0, 0);
-                    acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[t][q].z, b0.z, acc[t][0], 0, 0, 0);
-                    acc[t][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[t][q].z, b1.z, acc[t][1], 0, 0, 0);
-                    acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[t][q].w, b0.w, acc[t][0], 0, 0, 0);
-                    acc[t][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[t][q].w, b1.w, acc[t][1], 0, 0, 0);
+                    const float4 a = ring[c % RING][t][q];
+                    acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b0.x, acc[t][0], 0, 0, 0);
+                    acc[t][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b1.x, acc[t][1], 0, 0, 0);
+                    acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b0.y, acc[t][0], 0, 0, 0);
+                    acc[t][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b1.y, acc[t][1], 0, 0, 0);
+                    acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, b0.z, acc[t][0], 0, 0, 0);
+                    acc[t][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, b1.z, acc[t][1], 0, 0, 0);
+                    acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.w, b0.w, acc[t][0], 0, 0, 0);
+                    acc[t][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.w, b1.w, acc[t][1], 0, 0, 0);
                 }
             }
-#pragma unroll
-            for (int t = 0; t < RT; t++)
-#pragma unroll
-                for (int q = 0; q < 4; q++) sa[t][q] = sb[t][q];
         }
     }
     // raw sums: lane holds column ni*32 + lane%32, rows 8*(r/4) + 4*(lane/32) + r%4
@@ -588,7 +598,8 @@ hipError_t launch_grm_gemv(hipStream_t s, const float* S, int64_t Npad, int64_t 
 }
 
 bool gemm_config_valid(GemmConfig g) {
-    return (g.rt == 1 || g.rt == 2) && (g.nsplit == 1 || g.nsplit == 2 || g.nsplit == 4 || g.nsplit == 8);
+    return (g.rt == 1 || g.rt == 2) && (g.pd == 1 || g.pd == 3) &&
+           (g.nsplit == 1 || g.nsplit == 2 || g.nsplit == 4 || g.nsplit == 8);
 }
 
 hipError_t launch_grm_gemm(hipStream_t s, const float* S, int64_t Npad, int64_t Nnm, const float* in,
@@ -597,12 +608,15 @@ hipError_t launch_grm_gemm(hipStream_t s, const float* S, int64_t Npad, int64_t 
     // one superchunk when Npad / GKS >= nsplit
     if (!gemm_config_valid(g) || Npad % 256 != 0) return hipErrorInvalidValue;
     const int nsplit = (int)std::min<int64_t>(g.nsplit, Npad / GKS);
-    if (g.rt == 2)
-        hipLaunchKernelGGL(k_gemm_part<2>, dim3((unsigned)(Npad / 256), nsplit), dim3(256), 0, s, S, Npad, in,
-                           ypart);
+    const dim3 grid((unsigned)(Npad / (128 * g.rt)), nsplit);
+    if (g.rt == 2 && g.pd == 3)
+        hipLaunchKernelGGL((k_gemm_part<2, 3>), grid, dim3(256), 0, s, S, Npad, in, ypart);
+    else if (g.rt == 2)
+        hipLaunchKernelGGL((k_gemm_part<2, 1>), grid, dim3(256), 0, s, S, Npad, in, ypart);
+    else if (g.pd == 3)
+        hipLaunchKernelGGL((k_gemm_part<1, 3>), grid, dim3(256), 0, s, S, Npad, in, ypart);
     else
-        hipLaunchKernelGGL(k_gemm_part<1>, dim3((unsigned)(Npad / 128), nsplit), dim3(256), 0, s, S, Npad, in,
-                           ypart);
+        hipLaunchKernelGGL((k_gemm_part<1, 1>), grid, dim3(256), 0, s, S, Npad, in, ypart);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int64_t nb = Npad / EROWS;

@@ -1395,7 +1395,10 @@ int saige_engine_use_loco(saige_ctx* c, int on) {
 
 namespace {
 // Block-GEMM shapes tried by saige_autotune, in the order of its table.
-constexpr GemmConfig kGemmCandidates[] = {{1, 1}, {1, 2}, {1, 4}, {1, 8}, {2, 1}, {2, 2}, {2, 4}, {2, 8}};
+// {rt, nsplit, pd}
+constexpr GemmConfig kGemmCandidates[] = {
+    {1, 1, 1}, {1, 2, 1}, {1, 4, 1}, {1, 8, 1}, {2, 1, 1}, {2, 2, 1}, {2, 4, 1}, {2, 8, 1},
+    {1, 1, 3}, {1, 2, 3}, {1, 4, 3}, {1, 8, 3}, {2, 1, 3}, {2, 2, 3}, {2, 4, 3}, {2, 8, 3}};
 constexpr int kNumGemmCandidates = (int)(sizeof(kGemmCandidates) / sizeof(kGemmCandidates[0]));
 
 int engine_idle(const saige_ctx* c, const char* what) {
@@ -1459,19 +1462,21 @@ int saige_autotune(saige_ctx* c, int reps, double* ms_table) {
     return SAIGE_OK;
 }
 
-int saige_autotune_candidates(int* rt, int* nsplit) {
+int saige_autotune_candidates(int* rt, int* pd, int* nsplit) {
     for (int i = 0; i < kNumGemmCandidates; i++) {
         if (rt) rt[i] = kGemmCandidates[i].rt;
+        if (pd) pd[i] = kGemmCandidates[i].pd;
         if (nsplit) nsplit[i] = kGemmCandidates[i].nsplit;
     }
     return kNumGemmCandidates;
 }
 
-int saige_set_gemm_config(saige_ctx* c, int rt, int nsplit) {
+int saige_set_gemm_config(saige_ctx* c, int rt, int pd, int nsplit) {
     CHECK_CTX(c);
-    const GemmConfig g{rt, nsplit};
+    const GemmConfig g{rt, nsplit, pd};
     if (!gemm_config_valid(g)) {
-        set_error("invalid block-GEMM shape rt=%d nsplit=%d (rt 1|2, nsplit 1|2|4|8)", rt, nsplit);
+        set_error("invalid block-GEMM shape rt=%d pd=%d nsplit=%d (rt 1|2, pd 1|3, nsplit 1|2|4|8)", rt, pd,
+                  nsplit);
         return SAIGE_E_ARG;
     }
     if (int rc = engine_idle(c, "changing the block-GEMM shape")) return rc;
@@ -1480,12 +1485,13 @@ int saige_set_gemm_config(saige_ctx* c, int rt, int nsplit) {
     return SAIGE_OK;
 }
 
-int saige_get_gemm_config(const saige_ctx* c, int* rt, int* nsplit, double* tuned_ms) {
+int saige_get_gemm_config(const saige_ctx* c, int* rt, int* pd, int* nsplit, double* tuned_ms) {
     if (!c) {
         set_error("null context");
         return SAIGE_E_ARG;
     }
     if (rt) *rt = c->gemm_cfg.rt;
+    if (pd) *pd = c->gemm_cfg.pd;
     if (nsplit) *nsplit = c->gemm_cfg.nsplit;
     if (tuned_ms) *tuned_ms = c->gemm_tuned_ms;
     return SAIGE_OK;

@@ -44,12 +44,14 @@ constexpr int EW = 64;
 // Row padding of all N-length device arrays and of the GRM.
 constexpr int64_t NPAD_ALIGN = 256;
 
-// Block-GEMM shape: rt = 32-row tiles per wave (1 or 2), nsplit = column
-// slices of S (1, 2, 4 or 8).  The slice count fixes the summation order, so
-// results are reproducible for a fixed configuration.
+// Block-GEMM shape: rt = 32-row tiles per wave (1 or 2), pd = 32-column
+// chunks of S requested ahead (1 or 3), nsplit = column slices of S (1, 2, 4
+// or 8).  The slice count fixes the summation order, so results are
+// reproducible for a fixed configuration.
 struct GemmConfig {
     int rt = 1;
     int nsplit = 4;
+    int pd = 1;
 };
 
 struct Engine {

@@ -86,9 +86,10 @@ SIGNATURES = {
     "saige_engine_take": (_int, [_c_p, _int, _c_p]),
     "saige_engine_use_loco": (_int, [_c_p, _int]),
     "saige_autotune": (_int, [_c_p, _int, _c_p]),
-    "saige_autotune_candidates": (_int, [_c_p, _c_p]),
-    "saige_set_gemm_config": (_int, [_c_p, _int, _int]),
-    "saige_get_gemm_config": (_int, [_c_p, ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.POINTER(_f64)]),
+    "saige_autotune_candidates": (_int, [_c_p, _c_p, _c_p]),
+    "saige_set_gemm_config": (_int, [_c_p, _int, _int, _int]),
+    "saige_get_gemm_config": (_int, [_c_p, ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.POINTER(_int),
+                                      ctypes.POINTER(_f64)]),
     "saige_engine_stats": (_int, [_c_p, ctypes.POINTER(_f64), ctypes.POINTER(_i64)]),
     "saige_device_memory": (_int, [_c_p, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "saige_engine_system_bytes": (_i64, [_c_p]),

@@ -269,24 +269,27 @@ class Session:
     # ------------------------------------------------------- autotuner
     def autotune(self, reps=3):
         """Time every block-GEMM shape on the resident GRM and keep the
-        fastest.  Returns {"rt", "nsplit", "ms", "table": [(rt, nsplit, ms)]}."""
-        n = self._L.saige_autotune_candidates(None, None)
+        fastest.  Returns {"rt", "pd", "nsplit", "ms", "table": [(rt, pd,
+        nsplit, ms)]}."""
+        n = self._L.saige_autotune_candidates(None, None, None)
         rt = np.zeros(n, np.int32)
+        pd = np.zeros(n, np.int32)
         ns = np.zeros(n, np.int32)
-        self._L.saige_autotune_candidates(ptr(rt), ptr(ns))
+        self._L.saige_autotune_candidates(ptr(rt), ptr(pd), ptr(ns))
         ms = np.zeros(n, np.float64)
         check(self._L.saige_autotune(self.ctx, int(reps), ptr(ms)))
         cfg = self.gemm_config()
-        cfg["table"] = [(int(a), int(b), float(t)) for a, b, t in zip(rt, ns, ms)]
+        cfg["table"] = [(int(a), int(b), int(c), float(t)) for a, b, c, t in zip(rt, pd, ns, ms)]
         return cfg
 
-    def set_gemm_config(self, rt, nsplit):
-        check(self._L.saige_set_gemm_config(self.ctx, int(rt), int(nsplit)))
+    def set_gemm_config(self, rt, pd, nsplit):
+        check(self._L.saige_set_gemm_config(self.ctx, int(rt), int(pd), int(nsplit)))
 
     def gemm_config(self):
-        rt, ns, ms = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_double(0.0)
-        check(self._L.saige_get_gemm_config(self.ctx, ctypes.byref(rt), ctypes.byref(ns), ctypes.byref(ms)))
-        return {"rt": int(rt.value), "nsplit": int(ns.value), "ms": float(ms.value)}
+        rt, pd, ns, ms = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0), ctypes.c_double(0.0)
+        check(self._L.saige_get_gemm_config(self.ctx, ctypes.byref(rt), ctypes.byref(pd), ctypes.byref(ns),
+                                            ctypes.byref(ms)))
+        return {"rt": int(rt.value), "pd": int(pd.value), "nsplit": int(ns.value), "ms": float(ms.value)}
 
     def engine_use_loco(self, on):
         """Point the block engine at the LOCO GRM (on) or the full one."""

@@ -266,17 +266,18 @@ def test_engine_product_slots(session, grm1000):
 
 
 def test_autotune_and_every_gemm_shape(session, grm1000):
-    """saige_autotune times the eight block-GEMM shapes and keeps the fastest;
-    every shape gives K B within the FP32 product tolerance."""
+    """saige_autotune times the sixteen block-GEMM shapes and keeps the
+    fastest; every shape gives K B within the FP32 product tolerance, and a
+    fixed shape repeats bitwise."""
     og, S = grm1000
     print("case: autotune / GEMM shapes", flush=True)
     session.build_grm("fp32")
     res = session.autotune(reps=2)
     table = res["table"]
-    assert len(table) == 8 and all(ms > 0 for _, _, ms in table)
-    best = min(table, key=lambda t: t[2])
-    assert (res["rt"], res["nsplit"]) == best[:2] and res["ms"] == best[2]
-    print("  " + ", ".join(f"rt{a}/s{b}: {t:.3f} ms" for a, b, t in table))
+    assert len(table) == 16 and all(ms > 0 for *_, ms in table)
+    best = min(table, key=lambda t: t[3])
+    assert (res["rt"], res["pd"], res["nsplit"]) == best[:3] and res["ms"] == best[3]
+    print("  " + ", ".join(f"rt{a}/pd{b}/s{c}: {t:.3f} ms" for a, b, c, t in table))
     K64 = _k64(og, S)
     rng = np.random.default_rng(13)
     B = rng.standard_normal((1000, 70)).astype(np.float32)
@@ -284,16 +285,22 @@ def test_autotune_and_every_gemm_shape(session, grm1000):
     w = rng.uniform(0.05, 0.25, 1000).astype(np.float32)
     tau = np.array([1.0, 0.5], np.float32)
     X64 = np.linalg.solve(_sigma64(og, S, w, tau), B[:, :3].astype(np.float64))
-    for rt, ns, _ in table:
-        session.set_gemm_config(rt, ns)
-        assert session.gemm_config() == {"rt": rt, "nsplit": ns, "ms": 0.0}
+    by_split = {}
+    for rt, pd, ns, _ in table:
+        session.set_gemm_config(rt, pd, ns)
+        assert session.gemm_config() == {"rt": rt, "pd": pd, "nsplit": ns, "ms": 0.0}
         KB = session.grm_matvec(B)
-        assert np.max(np.abs(KB - RB)) <= 2e-5 * np.max(np.abs(RB)), (rt, ns)
+        assert np.max(np.abs(KB - RB)) <= 2e-5 * np.max(np.abs(RB)), (rt, pd, ns)
         X, _ = session.pcg_block(w, tau, B[:, :3], 500, 1e-5)
-        assert np.max(np.abs(X - X64)) <= 2e-3 * np.max(np.abs(X64)), (rt, ns)
-        # a fixed shape is reproducible
-        assert np.array_equal(KB, session.grm_matvec(B))
+        assert np.max(np.abs(X - X64)) <= 2e-3 * np.max(np.abs(X64)), (rt, pd, ns)
+        assert np.array_equal(KB, session.grm_matvec(B))          # a fixed shape repeats bitwise
+        # rt and pd change the schedule, not the arithmetic: same bits for one nsplit
+        if ns in by_split:
+            assert np.array_equal(KB, by_split[ns]), (rt, pd, ns)
+        by_split[ns] = KB
     from saige_gpu import SaigeError
     with pytest.raises(SaigeError, match="shape"):
-        session.set_gemm_config(3, 4)
-    session.set_gemm_config(1, 4)
+        session.set_gemm_config(3, 1, 4)
+    with pytest.raises(SaigeError, match="shape"):
+        session.set_gemm_config(1, 2, 4)
+    session.set_gemm_config(1, 1, 4)
