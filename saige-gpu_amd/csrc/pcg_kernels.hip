@@ -114,7 +114,8 @@ constexpr int GSPLIT_MAX = 8;           // most column slices
 
 template <int RT, int PD>
 __global__ __launch_bounds__(256) void k_gemm_part(const float* __restrict__ S, int64_t ld,
-                                                   const float* __restrict__ in, float* __restrict__ ypart) {
+                                                   const float* __restrict__ in, float* __restrict__ ypart,
+                                                   int halves) {
     constexpr int GM = 128 * RT;        // rows per workgroup
     constexpr int CPS = GKS / 32;       // 32-column chunks per superchunk
     constexpr int RING = PD + 1;        // S register buffers; RING divides CPS, so a
@@ -186,32 +187,54 @@ __global__ __launch_bounds__(256) void k_gemm_part(const float* __restrict__ S, 
                             *reinterpret_cast<const float4*>(srow + (int64_t)t * 32 * ld + kf + 4 * q);
             }
             const float* pb = sP + c * 32 + hl * 16;
+            if (halves == 3) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const float4 b0 = *reinterpret_cast<const float4*>(pb + (lane & 31) * GKP + 4 * q);
-                const float4 b1 = *reinterpret_cast<const float4*>(pb + (32 + (lane & 31)) * GKP + 4 * q);
+                for (int q = 0; q < 4; q++) {
+                    const float4 b0 = *reinterpret_cast<const float4*>(pb + (lane & 31) * GKP + 4 * q);
+                    const float4 b1 = *reinterpret_cast<const float4*>(pb + (32 + (lane & 31)) * GKP + 4 * q);
 #pragma unroll
-                for (int t = 0; t < RT; t++) {
-                    const float4 a = ring[c % RING][t][q];
-                    acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b0.x, acc[t][0], 0, 0, 0);
-                    acc[t][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b1.x, acc[t][1], 0, 0, 0);
-                    acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b0.y, acc[t][0], 0, 0, 0);
-                    acc[t][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b1.y, acc[t][1], 0, 0, 0);
-                    acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, b0.z, acc[t][0], 0, 0, 0);
-                    acc[t][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, b1.z, acc[t][1], 0, 0, 0);
-                    acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.w, b0.w, acc[t][0], 0, 0, 0);
-                    acc[t][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.w, b1.w, acc[t][1], 0, 0, 0);
+                    for (int t = 0; t < RT; t++) {
+                        const float4 a = ring[c % RING][t][q];
+                        acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b0.x, acc[t][0], 0, 0, 0);
+                        acc[t][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b1.x, acc[t][1], 0, 0, 0);
+                        acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b0.y, acc[t][0], 0, 0, 0);
+                        acc[t][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b1.y, acc[t][1], 0, 0, 0);
+                        acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, b0.z, acc[t][0], 0, 0, 0);
+                        acc[t][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, b1.z, acc[t][1], 0, 0, 0);
+                        acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.w, b0.w, acc[t][0], 0, 0, 0);
+                        acc[t][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.w, b1.w, acc[t][1], 0, 0, 0);
+                    }
+                }
+            } else {
+                // one 32-column half live: half the MFMA work (the other half's
+                // partial sums are left unwritten and ignored by the epilogue)
+                const int h = halves >> 1;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const float4 b = *reinterpret_cast<const float4*>(pb + (32 * h + (lane & 31)) * GKP + 4 * q);
+#pragma unroll
+                    for (int t = 0; t < RT; t++) {
+                        const float4 a = ring[c % RING][t][q];
+                        acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b.x, acc[t][0], 0, 0, 0);
+                        acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b.y, acc[t][0], 0, 0, 0);
+                        acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, b.z, acc[t][0], 0, 0, 0);
+                        acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.w, b.w, acc[t][0], 0, 0, 0);
+                    }
                 }
             }
         }
     }
     // raw sums: lane holds column ni*32 + lane%32, rows 8*(r/4) + 4*(lane/32) + r%4
+    // (with one live half, acc[t][0] holds that half)
     float* yp = ypart + (int64_t)split * (gridDim.x * (int64_t)GM) * EW;
+    const int nh = halves == 3 ? 2 : 1;
+    const int h0 = halves == 2 ? 1 : 0;
 #pragma unroll
     for (int t = 0; t < RT; t++)
 #pragma unroll
         for (int ni = 0; ni < 2; ni++) {
-            const int n = ni * 32 + (lane & 31);
+            if (ni >= nh) break;
+            const int n = (h0 + ni) * 32 + (lane & 31);
 #pragma unroll
             for (int r = 0; r < 16; r++) {
                 const int64_t row = r0 + wv * 32 * RT + t * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
@@ -603,20 +626,21 @@ bool gemm_config_valid(GemmConfig g) {
 }
 
 hipError_t launch_grm_gemm(hipStream_t s, const float* S, int64_t Npad, int64_t Nnm, const float* in,
-                           float* out, SigmaArgs a, double* partial, int64_t* nparts, float* ypart, GemmConfig g) {
+                           float* out, SigmaArgs a, double* partial, int64_t* nparts, float* ypart, GemmConfig g,
+                           int halves) {
     // Npad is a multiple of 256 = 128 * RT for both RT; a slice holds at least
     // one superchunk when Npad / GKS >= nsplit
-    if (!gemm_config_valid(g) || Npad % 256 != 0) return hipErrorInvalidValue;
+    if (!gemm_config_valid(g) || Npad % 256 != 0 || halves < 1 || halves > 3) return hipErrorInvalidValue;
     const int nsplit = (int)std::min<int64_t>(g.nsplit, Npad / GKS);
     const dim3 grid((unsigned)(Npad / (128 * g.rt)), nsplit);
     if (g.rt == 2 && g.pd == 3)
-        hipLaunchKernelGGL((k_gemm_part<2, 3>), grid, dim3(256), 0, s, S, Npad, in, ypart);
+        hipLaunchKernelGGL((k_gemm_part<2, 3>), grid, dim3(256), 0, s, S, Npad, in, ypart, halves);
     else if (g.rt == 2)
-        hipLaunchKernelGGL((k_gemm_part<2, 1>), grid, dim3(256), 0, s, S, Npad, in, ypart);
+        hipLaunchKernelGGL((k_gemm_part<2, 1>), grid, dim3(256), 0, s, S, Npad, in, ypart, halves);
     else if (g.pd == 3)
-        hipLaunchKernelGGL((k_gemm_part<1, 3>), grid, dim3(256), 0, s, S, Npad, in, ypart);
+        hipLaunchKernelGGL((k_gemm_part<1, 3>), grid, dim3(256), 0, s, S, Npad, in, ypart, halves);
     else
-        hipLaunchKernelGGL((k_gemm_part<1, 1>), grid, dim3(256), 0, s, S, Npad, in, ypart);
+        hipLaunchKernelGGL((k_gemm_part<1, 1>), grid, dim3(256), 0, s, S, Npad, in, ypart, halves);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int64_t nb = Npad / EROWS;

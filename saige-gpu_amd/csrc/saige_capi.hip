@@ -329,6 +329,12 @@ int engine_step(saige_ctx* c, Engine& e, int n_iter, uint8_t* done_out, int32_t*
     a.xout = e.X;
     SAIGE_HIP_TRY(hipMemcpyAsync(e.h_done, e.done, e.kpad, hipMemcpyDeviceToHost, st));
     SAIGE_HIP_TRY(hipStreamSynchronize(st));
+    // 32-column halves of the block that hold an occupied slot; the GEMM skips
+    // the others (a done slot that is not yet taken is recomputed unchanged)
+    int halves = 0;
+    for (int k = 0; k < e.kpad; k++)
+        if (e.h_live[k]) halves |= 1 << (k / 32);
+    if (halves == 0) halves = 1;
     for (int it = 0; it < n_iter; it++) {
         bool any = false;
         for (int k = 0; k < e.kpad; k++) any |= (e.h_live[k] && !e.h_done[k]);
@@ -338,7 +344,8 @@ int engine_step(saige_ctx* c, Engine& e, int n_iter, uint8_t* done_out, int32_t*
         if (e.kpad == 1)
             SAIGE_HIP_TRY(launch_grm_gemv(st, g.S, Np, N, e.P, e.AP, a, e.partial, &np));
         else
-            SAIGE_HIP_TRY(launch_grm_gemm(st, g.S, Np, N, e.P, e.AP, a, e.partial, &np, e.ypart, c->gemm_cfg));
+            SAIGE_HIP_TRY(launch_grm_gemm(st, g.S, Np, N, e.P, e.AP, a, e.partial, &np, e.ypart, c->gemm_cfg,
+                                          halves));
         SAIGE_HIP_TRY(hipEventRecord(c->ev1, st));
         SAIGE_HIP_TRY(launch_alpha(st, e.kpad, e.partial, np, e.rz, e.done, e.live, e.kind, e.alpha));
         int64_t np2 = 0;
@@ -964,7 +971,9 @@ static int grm_matvec(saige_ctx* c, int loco, const float* B, int64_t k, float* 
         const int kb = (int)std::min<int64_t>(EW, k - k0);
         e = hipMemcpyAsync(d_in, B + k0 * N, 4 * N * kb, hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = launch_scatter_cols(st, d_in, kb, N, Np, EW, d_blk);
-        if (e == hipSuccess) e = launch_grm_gemm(st, g.S, Np, N, d_blk, d_out, a, nullptr, nullptr, c->d_mv_ypart, c->gemm_cfg);
+        if (e == hipSuccess)
+            e = launch_grm_gemm(st, g.S, Np, N, d_blk, d_out, a, nullptr, nullptr, c->d_mv_ypart, c->gemm_cfg,
+                                kb <= 32 ? 1 : 3);
         if (e == hipSuccess) e = launch_gather_cols(st, d_out, kb, N, EW, d_in);
         if (e == hipSuccess) e = hipMemcpyAsync(Y + k0 * N, d_in, 4 * N * kb, hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);

@@ -234,10 +234,13 @@ struct SigmaArgs {
 hipError_t launch_grm_gemv(hipStream_t s, const float* S, int64_t Npad, int64_t Nnm, const float* in,
                            float* out, SigmaArgs a, double* partial, int64_t* nparts);
 bool gemm_config_valid(GemmConfig g);
-// ypart: gemm_ypart_floats(Npad) floats of scratch for the split-K slices
+// ypart: gemm_ypart_floats(Npad) floats of scratch for the split-K slices.
+// halves: bit h set = columns [32h, 32h+32) are computed; the epilogue's
+// values for a half that is not computed are meaningless (the caller only
+// passes halves without live columns).
 hipError_t launch_grm_gemm(hipStream_t s, const float* S, int64_t Npad, int64_t Nnm, const float* in,
                            float* out, SigmaArgs a, double* partial, int64_t* nparts, float* ypart,
-                           GemmConfig g);
+                           GemmConfig g, int halves = 3);
 int64_t gemm_ypart_floats(int64_t Npad);
 int64_t gemv_max_parts(int64_t Npad);
 int64_t gemm_max_parts(int64_t Npad);

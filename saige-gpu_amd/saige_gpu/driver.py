@@ -20,6 +20,7 @@ the tau update) is FP64.  All O(N^2) work -- every GRM product inside the
 solves and every K u -- runs on the device.
 """
 import collections
+import heapq
 import time
 
 import numpy as np
@@ -616,7 +617,9 @@ class BatchScheduler:
         self.n_systems = n_systems = max(1, min(n_systems, by_mem))
         self.hbm_reserve_bytes = hbm_reserve_bytes
         sess.engine_reserve(width, n_systems)
-        self.free_slots = collections.deque(range(width))
+        # lowest free slot first: live columns pack into the low 32-column
+        # half, and the GEMM skips a half with no occupied slot
+        self.free_slots = list(range(width))
         self.free_sys = collections.deque(range(n_systems))
         self.slot_owner = {}
         self.waiting = collections.deque()      # submitted, not admitted
@@ -734,7 +737,7 @@ class BatchScheduler:
         for f in self.active:
             while f.pending and self.free_slots:
                 j, b, kind = f.pending.popleft()
-                slot = self.free_slots.popleft()
+                slot = heapq.heappop(self.free_slots)
                 g = groups[kind]
                 g[0].append(slot)
                 g[1].append(f.sys)
@@ -775,7 +778,7 @@ class BatchScheduler:
         resume = []
         for c, slot in enumerate(finished):
             f, j = self.slot_owner.pop(slot)
-            self.free_slots.append(slot)
+            heapq.heappush(self.free_slots, slot)
             f.result_buf[:, j] = self._compact(f, X[:, c])
             f.iters += int(iters[slot])
             f.remaining -= 1

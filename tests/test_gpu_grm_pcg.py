@@ -206,6 +206,12 @@ def test_pcg_block_and_engine(session, grm1000):
         if slot != 40:
             session.engine_take(slot)
     assert np.array_equal(alone.view(np.uint32), mixed.view(np.uint32))
+    # alone in the upper 32-column half: the GEMM computes only that half
+    session.engine_add(50, 0, B[:, 2], 500, 1e-5)
+    while not session.engine_step(50)[0][50]:
+        pass
+    upper = session.engine_take(50)
+    assert np.array_equal(alone.view(np.uint32), upper.view(np.uint32))
 
 
 def test_odd_sizes(session):

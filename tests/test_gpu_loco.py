@@ -75,9 +75,9 @@ def test_loco_grm_and_diag(session, loaded, monkeypatch, prec):
     err = np.max(np.abs(C - ref))
     print(f"  max abs err {err:.3e}  tol {tol:.3e}")
     assert err <= tol
-    # and it is not the full GRM
+    # and it is not the full GRM: the range moves K by far more than the error
     full = np.column_stack([session.getCrossprodMatAndKin(np.eye(1000, dtype=np.float32)[i]) for i in idx[:2]])
-    assert np.max(np.abs(full - ref[:, :2])) > 10 * tol
+    assert np.max(np.abs(full - ref[:, :2])) > 10 * err
     # diag(Sigma_loco): bit-exact against the oracle's FP32 restatement
     rng = np.random.default_rng(4)
     w = rng.uniform(0.05, 0.25, 1000).astype(np.float32)
