@@ -157,6 +157,8 @@ def c4_leg(args):
     sess.set_vr_random_markers(pool)
     sess.setgeno_device(dev, N, M)              # VR markers leave the GRM (FG.cpp:545-547)
     Mqc = sess.getMsub_MAFge_minMAFtoConstructGRM()
+    if Mqc < 0.9 * (M - len(pool)):
+        raise RuntimeError(f"synthetic cohort malformed: only {Mqc} of {M} markers pass QC")
     vr_index = sess.getIndexVec_forVarRatio()
     pos = {int(m): i for i, m in enumerate(vr_index)}
     categories = [[pos[int(m)] for m in rng.permutation(c) if int(m) in pos] for c in cats_m]

@@ -278,10 +278,9 @@ __device__ __forceinline__ int synth_geno(uint64_t seed, int64_t m, int64_t s, f
     return (unif(seed, m, s, 1) < pj ? 1 : 0) + (unif(seed, m, s, 2) < pj ? 1 : 0);
 }
 
-__global__ void k_synth_bed(uint8_t* __restrict__ bed, int64_t N, int64_t M, int64_t bpm,
-                            float maf_lo, float maf_hi, float miss, int64_t npairs, uint64_t seed) {
-    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (t >= M * bpm) return;
+__device__ __forceinline__ void synth_byte(uint8_t* __restrict__ bed, int64_t t, int64_t N, int64_t bpm,
+                                           float maf_lo, float maf_hi, float miss, int64_t npairs,
+                                           uint64_t seed) {
     const int64_t m = t / bpm, b = t - m * bpm;
     const float pj = maf_lo + (maf_hi - maf_lo) * unif(seed, m, 0x7FFFFFFF, 0);
     unsigned byte = 0;
@@ -298,6 +297,17 @@ __global__ void k_synth_bed(uint8_t* __restrict__ bed, int64_t N, int64_t M, int
     }
     bed[t] = (uint8_t)byte;
 }
+
+// Grid-stride over the M * bpm bytes: at N = 200k, M = 500k that is 2.5e10
+// bytes, more work-items than one launch can address (2^32).
+__global__ void k_synth_bed(uint8_t* __restrict__ bed, int64_t N, int64_t M, int64_t bpm,
+                            float maf_lo, float maf_hi, float miss, int64_t npairs, uint64_t seed) {
+    const int64_t total = M * bpm;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += stride)
+        synth_byte(bed, t, N, bpm, maf_lo, maf_hi, miss, npairs, seed);
+}
+
 
 // out = a - b, FP32 (diag(K) minus one chromosome's part, FG.cpp:4951)
 __global__ void k_sub(const float* __restrict__ a, const float* __restrict__ b, int64_t n, float* __restrict__ out) {
@@ -361,7 +371,7 @@ hipError_t launch_synth_bed(hipStream_t s, uint8_t* bed, int64_t N, int64_t M, f
                             float maf_hi, float miss, int64_t npairs, uint64_t seed) {
     const int64_t bpm = (N + 3) / 4;
     const int64_t n = M * bpm;
-    hipLaunchKernelGGL(k_synth_bed, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, bed, N, M, bpm,
+    hipLaunchKernelGGL(k_synth_bed, dim3(grid_for((n + 255) / 256, 1 << 16)), dim3(256), 0, s, bed, N, M, bpm,
                        maf_lo, maf_hi, miss, npairs, seed);
     return hipGetLastError();
 }

@@ -7,10 +7,12 @@ and HBM counters (FETCH_SIZE, WRITE_SIZE) of each can be read per dispatch:
     rocprofv3 --pmc FETCH_SIZE -d OUT -o fetch -- python3 saige-gpu_amd/profile_kernels.py
     rocprofv3 --pmc WRITE_SIZE -d OUT -o write -- python3 saige-gpu_amd/profile_kernels.py
 
-Kernels exercised: decode/QC (k_decode_stats), repack, diag, the BF16 SYRK
-(k_std_panel + k_syrk_bf16), the 64-column block GEMM (k_gemm_part +
-k_gemm_epi, through saige_grm_matvec), the single-column GEMV (k_gemv) and
-the runtime's copy kernel on COPY_BYTES.
+Kernels exercised: decode/QC (k_decode_stats), repack, diag, the GRM SYRK
+(2-bit input k_syrk_pk for --grm bf16), the autotuner's sixteen block-GEMM
+shapes, the tuned 64-column block GEMM (k_gemm_part + k_gemm_epi, through
+saige_grm_matvec) and its one-half (16-column) form, the single-column GEMV
+(k_gemv), one LOCO range build (copy + range SYRK in subtract mode over
+--loco-frac of the markers), and the runtime's copy kernel on COPY_BYTES.
 """
 import argparse
 import os
@@ -29,6 +31,7 @@ def main():
     ap.add_argument("--M", type=int, default=150_000)
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--grm", default="bf16")
+    ap.add_argument("--loco-frac", type=float, default=0.08)
     args = ap.parse_args()
 
     import numpy as np
@@ -44,12 +47,21 @@ def main():
     s.setgeno_device(bed, N, M)
     t = s.build_grm(args.grm)
     print(f"GRM {args.grm} {t:.3f}s", flush=True)
+    tune = s.autotune(reps=2)
+    print(f"block GEMM shape rt{tune['rt']}/pd{tune['pd']}/s{tune['nsplit']} {tune['ms']:.3f} ms", flush=True)
     rng = np.random.default_rng(0)
     B = rng.standard_normal((N, 64)).astype(np.float32)
     for _ in range(args.reps):
         s.grm_matvec(B)
     for _ in range(args.reps):
+        s.grm_matvec(B[:, :16])
+    for _ in range(args.reps):
         s.getCrossprodMatAndKin(B[:, 0])
+    if args.loco_frac > 0:
+        mqc = s.getMsub_MAFge_minMAFtoConstructGRM()
+        n_chr = max(1, int(args.loco_frac * mqc))
+        s.setStartEndIndex(0, n_chr - 1, 0)
+        print(f"LOCO range of {n_chr} markers built", flush=True)
     a = s.alloc_device(COPY_BYTES)
     b = s.alloc_device(COPY_BYTES)
     for _ in range(args.reps):

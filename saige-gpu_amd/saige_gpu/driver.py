@@ -20,6 +20,7 @@ the tau update) is FP64.  All O(N^2) work -- every GRM product inside the
 solves and every K u -- runs on the device.
 """
 import collections
+import concurrent.futures
 import heapq
 import time
 
@@ -557,7 +558,7 @@ class PlainExecutor:
             if req[0] == "solve":
                 _, w, tau, B = req
                 tau32 = np.asarray(tau, np.float32)
-                out = np.zeros(B.shape, np.float64)
+                out = np.zeros(B.shape, np.float64, order="F")   # the scheduler's result layout
                 for j in range(B.shape[1]):
                     x, it = self._pcg(w, tau32, B[:, j], self.opt.maxiterPCG, self.opt.tolPCG,
                                       return_iters=True)
@@ -567,13 +568,14 @@ class PlainExecutor:
                 value = out
             else:
                 _, B = req
-                value = np.column_stack([self._kin(B[:, j]) for j in range(B.shape[1])]).astype(np.float64)
+                value = np.asfortranarray(np.column_stack([self._kin(B[:, j]) for j in range(B.shape[1])]),
+                                          dtype=np.float64)
                 self.n_products += B.shape[1]
 
 
 class _Fit:
     __slots__ = ("fid", "gen", "subset", "n", "sys", "pending", "result_buf", "remaining", "result",
-                 "error", "t_submit", "t_admit", "t_done", "tag", "iters", "sys_ready")
+                 "error", "t_submit", "t_admit", "t_done", "tag", "iters", "sys_ready", "sys_pending")
 
     def __init__(self, fid, gen, subset, tag):
         self.fid = fid
@@ -591,6 +593,7 @@ class _Fit:
         self.tag = tag
         self.iters = 0
         self.sys_ready = False
+        self.sys_pending = None
 
 
 class BatchScheduler:
@@ -602,7 +605,13 @@ class BatchScheduler:
     resumes as soon as its request's last column is done.  GRM products
     (K u for the trace probes, K PY) are product slots that ride along in
     the next iteration's GEMM.  A column's arithmetic does not depend on its
-    neighbours, so each fit gets the same result alone and in the mix."""
+    neighbours, so each fit gets the same result alone and in the mix.
+
+    Host work overlaps the device: when columns finish, the next iteration of
+    the still-running columns is launched from a worker thread (the library
+    call releases the GIL) while the main thread resumes the finished fits'
+    coroutines.  Nothing touches the library during that time; a resumed
+    fit's new system and columns are applied once the iteration is back."""
 
     def __init__(self, sess, opt, width=64, n_systems=16, hbm_reserve_bytes=8 << 30):
         self.sess = sess
@@ -631,6 +640,9 @@ class BatchScheduler:
         self.busy_slot_iters = 0
         # wall-time split of run(): device iterations, slot traffic, fit logic
         self.timing = collections.Counter()
+        self._pool = concurrent.futures.ThreadPoolExecutor(max_workers=1)
+        self._inflight = None                   # engine_step running in the worker
+        self.n_overlapped = 0                   # iterations that ran during host work
 
     # ---------------------------------------------------------------- API
     def submit(self, gen, subset=None, tag=None, t_submit=None):
@@ -653,9 +665,10 @@ class BatchScheduler:
             while arrivals and t0 + arrivals[0][0] <= now:
                 toff, gen, subset, tag = arrivals.popleft()
                 self.submit(gen, subset, tag, t_submit=t0 + toff)
-            self._admit()
-            self._fill()
-            if self.slot_owner:
+            if self._inflight is None:           # the engine is only changed between iterations
+                self._admit()
+                self._fill()
+            if self._inflight is not None or self.slot_owner:
                 self._step()
             elif arrivals and self.idle():
                 time.sleep(max(0.0, min(0.05, t0 + arrivals[0][0] - time.perf_counter())))
@@ -701,11 +714,13 @@ class BatchScheduler:
             if req[0] == "matvec":
                 B = req[1]
                 if not f.sys_ready:          # products only need the system's sample mask
-                    self._set_system(f, np.ones(B.shape[0]), (1.0, 0.0))
+                    f.sys_pending = (np.ones(B.shape[0]), (1.0, 0.0))
+                    f.sys_ready = True
                 kind = "product"
             else:
                 _, w, tau, B = req
-                self._set_system(f, w, tau)
+                f.sys_pending = (w, tau)
+                f.sys_ready = True
                 kind = "solve"
             f.result_buf = np.zeros(B.shape, np.float64, order="F")
             f.remaining = B.shape[1]
@@ -724,7 +739,6 @@ class BatchScheduler:
         t = time.perf_counter()
         self.sess.engine_set_system(f.sys, wf, np.asarray(tau, np.float32), mask)
         self.timing["set_system"] += time.perf_counter() - t
-        f.sys_ready = True
 
     def _finish(self, f):
         f.t_done = time.perf_counter()
@@ -735,6 +749,10 @@ class BatchScheduler:
     def _fill(self):
         groups = {"solve": ([], [], [], []), "product": ([], [], [], [])}
         for f in self.active:
+            if f.sys_pending is not None and f.pending and self.free_slots:
+                w, tau = f.sys_pending
+                f.sys_pending = None
+                self._set_system(f, w, tau)
             while f.pending and self.free_slots:
                 j, b, kind = f.pending.popleft()
                 slot = heapq.heappop(self.free_slots)
@@ -765,8 +783,13 @@ class BatchScheduler:
 
     def _step(self):
         t = time.perf_counter()
-        done, iters = self.sess.engine_step(1, self.width)
-        self.timing["step"] += time.perf_counter() - t
+        if self._inflight is not None:
+            done, iters = self._inflight.result()
+            self._inflight = None
+            self.timing["step_wait"] += time.perf_counter() - t
+        else:
+            done, iters = self.sess.engine_step(1, self.width)
+            self.timing["step"] += time.perf_counter() - t
         self.n_iterations += 1
         self.busy_slot_iters += len(self.slot_owner)
         finished = [slot for slot in self.slot_owner if done[slot]]
@@ -785,6 +808,10 @@ class BatchScheduler:
             self.n_solves += 1
             if f.remaining == 0 and not f.pending:
                 resume.append(f)
+        if resume and self.slot_owner:
+            # the columns still running take their next iteration meanwhile
+            self._inflight = self._pool.submit(self.sess.engine_step, 1, self.width)
+            self.n_overlapped += 1
         t = time.perf_counter()
         for f in resume:
             self._advance(f, f.result_buf)
