@@ -177,7 +177,7 @@ def c4_leg(args):
     tune = sess.autotune(reps=3)
     log(f"C4 autotune: rt{tune['rt']}/pd{tune['pd']}/s{tune['nsplit']} {tune['ms']:.2f} ms")
     opt = D.Options(nrun=30)
-    g0, n0 = sess.engine_stats()
+    g0, n0 = sess.engine_stats("gemm_full")
     t0 = time.perf_counter()
     handles, sch = D.fit_batched(sess, [{"trait": "binary", "y": y, "X": X}], opt)
     if handles[0].error is not None:
@@ -186,7 +186,7 @@ def c4_leg(args):
     t1 = time.perf_counter()
     vr = D.variance_ratio(sess, fit, y, X, "binary", categories, opt, vr_pool=True, num_markers=100)
     t2 = time.perf_counter()
-    g1, n1 = sess.engine_stats()
+    g1, n1 = sess.engine_stats("gemm_full")
     gemm_avg = (g1 - g0) / max(1, n1 - n0)
     out = {"config": "C4", "N": N, "M": M, "Mqc": Mqc, "grm_seconds": grm_s, "grm_tflops": grm_tflops,
            "grm_GB": 4.0 * (N + 255) // 256 * 256 * ((N + 255) // 256 * 256) / 1e9,
@@ -266,13 +266,17 @@ def main():
 
     for _ in range(args.warmup):
         step()
-    g0, n0 = sess.engine_stats()
+    g0, n0 = sess.engine_stats("gemm_full")
+    h0, m0 = sess.engine_stats("gemm_half")
     t0 = time.perf_counter()
     results = []
     for _ in range(args.steps):
         results.append(step())
     t1 = time.perf_counter()
-    g1, n1 = sess.engine_stats()
+    g1, n1 = sess.engine_stats("gemm_full")
+    h1, m1 = sess.engine_stats("gemm_half")
+    half_launches = m1 - m0
+    half_avg = (h1 - h0) / max(1, half_launches)
     step_s = (t1 - t0) / args.steps
     fits = len(jobs) * args.steps
     fits_per_hour = fits / (t1 - t0) * 3600.0
@@ -283,8 +287,9 @@ def main():
     gemm_tflops = gemm_flop / gemm_avg / 1e12
     lat = sorted(h.t_done - h.t_submit for h in handles)
     log("batched step split (s): " + ", ".join(f"{k} {v:.3f}" for k, v in sorted(sch.timing.items())))
-    log(f"batched: {step_s:.2f}s/step, {fits_per_hour:.1f} fits/h, {gemm_launches} GEMMs avg "
-        f"{gemm_avg * 1e3:.2f} ms ({gemm_tflops:.1f} TFLOP/s, {4.0 * N * N / gemm_avg / 1e9:.0f} GB/s), "
+    log(f"batched: {step_s:.2f}s/step, {fits_per_hour:.1f} fits/h, {gemm_launches} 64-column GEMMs avg "
+        f"{gemm_avg * 1e3:.2f} ms ({gemm_tflops:.1f} TFLOP/s), {half_launches} 32-column GEMMs avg "
+        f"{half_avg * 1e3:.2f} ms ({4.0 * N * N / half_avg / 1e9:.0f} GB/s of S), "
         f"theta {[list(np.round(h.result['theta'], 4)) for h in handles]}")
 
     serving = None
@@ -325,11 +330,11 @@ def main():
 
     plain = None
     if not args.no_plain:
-        g2, n2 = sess.engine_stats()
+        g2, n2 = sess.engine_stats("gemv")
         tp = time.perf_counter()
         pr = D.fit_plain(sess, "binary", phenos[2], X, opt)
         plain_s = time.perf_counter() - tp
-        g3, n3 = sess.engine_stats()
+        g3, n3 = sess.engine_stats("gemv")
         gemv_avg = (g3 - g2) / max(1, n3 - n2)
         plain = {"seconds": plain_s, "fits_per_hour": 3600.0 / plain_s, "pcg_iters": pr["pcg_iters"],
                  "grm_products": pr["pcg_iters"] + pr["n_products"],
@@ -382,7 +387,8 @@ def main():
                      "achieved": gemm_tflops, "peak": FP32_MFMA_PEAK, "unit": "TFLOP/s",
                      "frac": gemm_tflops / FP32_MFMA_PEAK, "traffic": None,
                      "algorithmic": f"2*N^2*64 = {gemm_flop:.3e} flop per launch; 4*N^2 = {4.0 * N * N:.3e} B",
-                     "avg_launch_ms": gemm_avg * 1e3, "launches": gemm_launches},
+                     "avg_launch_ms": gemm_avg * 1e3, "launches": gemm_launches,
+                     "half_width_launches": half_launches, "half_width_avg_ms": half_avg * 1e3},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

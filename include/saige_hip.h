@@ -266,8 +266,12 @@ int saige_get_gemm_config(const saige_ctx* ctx, int* rt, int* pd, int* nsplit, d
 int saige_device_memory(saige_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);
 /* Device bytes one engine system (w, tau, sample mask) occupies. */
 int64_t saige_engine_system_bytes(const saige_ctx* ctx);
-/* Seconds of device time spent in engine GEMM/GEMV launches, and their count. */
-int saige_engine_stats(const saige_ctx* ctx, double* gemm_seconds, int64_t* gemm_launches);
+/* Seconds of device time spent in the engines' GRM product launches of one
+ * kind, and their count (HIP events around each launch). */
+#define SAIGE_LAUNCH_GEMV 0       /* width-1 engine (saige_pcg1): k_gemv          */
+#define SAIGE_LAUNCH_GEMM_HALF 1  /* block GEMM over one 32-column half            */
+#define SAIGE_LAUNCH_GEMM_FULL 2  /* block GEMM over all 64 columns                */
+int saige_engine_stats(const saige_ctx* ctx, int kind, double* gemm_seconds, int64_t* gemm_launches);
 
 /* ------------------------------------------------------- synthetic data */
 

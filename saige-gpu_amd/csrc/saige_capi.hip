@@ -358,8 +358,9 @@ int engine_step(saige_ctx* c, Engine& e, int n_iter, uint8_t* done_out, int32_t*
         SAIGE_HIP_TRY(hipStreamSynchronize(st));
         float ms = 0.f;
         SAIGE_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        e.gemm_seconds += ms * 1e-3;
-        e.gemm_launches++;
+        const int kind = e.kpad == 1 ? 0 : (halves == 3 ? 2 : 1);
+        e.gemm_seconds[kind] += ms * 1e-3;
+        e.gemm_launches[kind]++;
     }
     SAIGE_HIP_TRY(hipMemcpyAsync(e.h_iter, e.iter, 4 * e.kpad, hipMemcpyDeviceToHost, st));
     SAIGE_HIP_TRY(hipMemcpyAsync(e.h_done, e.done, e.kpad, hipMemcpyDeviceToHost, st));
@@ -1506,13 +1507,17 @@ int saige_get_gemm_config(const saige_ctx* c, int* rt, int* pd, int* nsplit, dou
     return SAIGE_OK;
 }
 
-int saige_engine_stats(const saige_ctx* c, double* secs, int64_t* launches) {
+int saige_engine_stats(const saige_ctx* c, int kind, double* secs, int64_t* launches) {
     if (!c) {
         set_error("null context");
         return SAIGE_E_ARG;
     }
-    if (secs) *secs = c->eng.gemm_seconds + c->eng1.gemm_seconds;
-    if (launches) *launches = c->eng.gemm_launches + c->eng1.gemm_launches;
+    if (kind < 0 || kind > 2) {
+        set_error("unknown launch kind %d", kind);
+        return SAIGE_E_ARG;
+    }
+    if (secs) *secs = c->eng.gemm_seconds[kind] + c->eng1.gemm_seconds[kind];
+    if (launches) *launches = c->eng.gemm_launches[kind] + c->eng1.gemm_launches[kind];
     return SAIGE_OK;
 }
 

@@ -91,8 +91,10 @@ struct Engine {
     int32_t* h_iter = nullptr;
     std::vector<int> h_slot_sys;
     std::vector<uint8_t> h_live;
-    double gemm_seconds = 0.0;
-    int64_t gemm_launches = 0;
+    // device time of the GRM product launches by kind: [0] GEMV (width 1),
+    // [1] GEMM over one 32-column half, [2] GEMM over both halves
+    double gemm_seconds[3] = {0.0, 0.0, 0.0};
+    int64_t gemm_launches[3] = {0, 0, 0};
 };
 
 }  // namespace saige

@@ -364,11 +364,21 @@ class Session:
         check(self._L.saige_engine_take(self.ctx, int(slot), ptr(x)))
         return x
 
-    def engine_stats(self):
-        s = ctypes.c_double(0.0)
-        n = ctypes.c_int64(0)
-        check(self._L.saige_engine_stats(self.ctx, ctypes.byref(s), ctypes.byref(n)))
-        return float(s.value), int(n.value)
+    LAUNCH_KINDS = ("gemv", "gemm_half", "gemm_full")
+
+    def engine_stats(self, kind=None):
+        """Device seconds and launch count of the GRM product launches: for one
+        kind ("gemv", "gemm_half", "gemm_full") or summed over all."""
+        kinds = self.LAUNCH_KINDS if kind is None else (kind,)
+        tot_s, tot_n = 0.0, 0
+        for k in kinds:
+            s = ctypes.c_double(0.0)
+            n = ctypes.c_int64(0)
+            check(self._L.saige_engine_stats(self.ctx, self.LAUNCH_KINDS.index(k), ctypes.byref(s),
+                                             ctypes.byref(n)))
+            tot_s += float(s.value)
+            tot_n += int(n.value)
+        return tot_s, tot_n
 
     def device_memory(self):
         f = ctypes.c_int64(0)

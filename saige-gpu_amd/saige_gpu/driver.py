@@ -21,6 +21,7 @@ solves and every K u -- runs on the device.
 """
 import collections
 import concurrent.futures
+import ctypes
 import heapq
 import time
 
@@ -77,7 +78,12 @@ def glm_start(y, X, family, offset=None, maxit=25, epsilon=1e-8):
         mu = y.copy()
         eta = mu.copy()
 
+    binary01 = family.name == "binomial" and bool(np.all((y == 0) | (y == 1)))
+
     def deviance(mu):
+        if binary01:
+            # y in {0, 1}: the unit deviance is -2 log(mu) or -2 log(1 - mu)
+            return -2.0 * float(np.sum(np.log(np.where(y > 0, mu, 1.0 - mu))))
         if family.name == "binomial":
             with np.errstate(divide="ignore", invalid="ignore"):
                 a = np.where(y > 0, y * np.log(y / mu), 0.0)
@@ -113,10 +119,23 @@ class RademacherProbes:
     every AI iteration reuses the same probes; here probe i is a fixed
     function of (seed, i) with the same property."""
 
+    _shared = {}
+
     def __init__(self, n, seed=200):
         self.n = n
         self.seed = seed
         self._cache = {}
+
+    @classmethod
+    def shared(cls, n, seed=200):
+        """One instance per (n, seed) for the process: the probes are a fixed
+        function of (seed, i), so fits of the same size share the vectors
+        (and their cached blocks) instead of regenerating them."""
+        key = (int(n), int(seed))
+        p = cls._shared.get(key)
+        if p is None:
+            p = cls._shared[key] = cls(n, seed)
+        return p
 
     def __call__(self, i):
         v = self._cache.get(i)
@@ -533,6 +552,29 @@ def write_variance_ratio(path, table):
 
 # --------------------------------------------------------------- executors
 
+_ALLOCATOR_TUNED = False
+
+
+def tune_host_allocator():
+    """Serve the per-request N x k host blocks from the heap instead of fresh
+    mmaps: glibc maps every allocation above its mmap threshold anew and
+    returns it on free, so each block pays its page faults again (tens of ms
+    per 100k x 16 block).  Raising the mmap and trim thresholds keeps freed
+    blocks for reuse.  Process-wide and idempotent; returns whether glibc
+    accepted both settings."""
+    global _ALLOCATOR_TUNED
+    if _ALLOCATOR_TUNED:
+        return True
+    try:
+        libc = ctypes.CDLL("libc.so.6")
+    except OSError:
+        return False
+    M_TRIM_THRESHOLD, M_MMAP_THRESHOLD = -1, -3
+    ok = bool(libc.mallopt(M_MMAP_THRESHOLD, 1 << 30)) or bool(libc.mallopt(M_MMAP_THRESHOLD, 32 << 20))
+    ok = bool(libc.mallopt(M_TRIM_THRESHOLD, (1 << 31) - 1)) and ok
+    _ALLOCATOR_TUNED = ok
+    return ok
+
 class PlainExecutor:
     """Runs a fit coroutine the way the reference does: every right-hand side
     is its own single-RHS PCG (saige_pcg1), every K b its own GEMV.  With
@@ -613,7 +655,9 @@ class BatchScheduler:
     coroutines.  Nothing touches the library during that time; a resumed
     fit's new system and columns are applied once the iteration is back."""
 
-    def __init__(self, sess, opt, width=64, n_systems=16, hbm_reserve_bytes=8 << 30):
+    def __init__(self, sess, opt, width=64, n_systems=16, hbm_reserve_bytes=8 << 30, tune_malloc=True):
+        if tune_malloc:
+            tune_host_allocator()
         self.sess = sess
         self.opt = opt
         self.width = width
@@ -799,10 +843,13 @@ class BatchScheduler:
         X = self.sess.engine_take_batch(finished)
         self.timing["take"] += time.perf_counter() - t
         resume = []
+        by_fit = {}
         for c, slot in enumerate(finished):
             f, j = self.slot_owner.pop(slot)
             heapq.heappush(self.free_slots, slot)
-            f.result_buf[:, j] = self._compact(f, X[:, c])
+            js, cs = by_fit.setdefault(f.fid, (f, [], []))[1:]
+            js.append(j)
+            cs.append(c)
             f.iters += int(iters[slot])
             f.remaining -= 1
             self.n_solves += 1
@@ -813,6 +860,11 @@ class BatchScheduler:
             self._inflight = self._pool.submit(self.sess.engine_step, 1, self.width)
             self.n_overlapped += 1
         t = time.perf_counter()
+        for f, js, cs in by_fit.values():
+            for j, c in zip(js, cs):            # contiguous column copies (F-order blocks)
+                f.result_buf[:, j] = X[:, c] if f.subset is None else X[f.subset, c]
+        self.timing["copy"] += time.perf_counter() - t
+        t = time.perf_counter()
         for f in resume:
             self._advance(f, f.result_buf)
         self.timing["host"] += time.perf_counter() - t
@@ -821,7 +873,7 @@ class BatchScheduler:
 def fit_plain(sess, trait, y, X, opt=None, probe_seed=200, **kw):
     """One null fit through the plain single-RHS path."""
     opt = opt or Options()
-    probes = RademacherProbes(len(y), probe_seed)
+    probes = RademacherProbes.shared(len(y), probe_seed)
     ex = PlainExecutor(sess, opt)
     res = ex.run(fit_co(trait, y, X, probes, opt, **kw))
     res["n_solves"] = ex.n_solves
@@ -898,7 +950,7 @@ def fit_batched(sess, jobs, opt=None, width=64, n_systems=16, probe_seed=200):
     handles = []
     for j in jobs:
         n = len(j["y"])
-        probes = RademacherProbes(n, probe_seed)
+        probes = RademacherProbes.shared(n, probe_seed)
         gen = fit_co(j["trait"], j["y"], j["X"], probes, opt)
         handles.append(sch.submit(gen, j.get("subset"), j.get("tag")))
     sch.run()
