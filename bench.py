@@ -28,6 +28,10 @@ sys.path[:0] = [ROOT, os.path.join(ROOT, "saige-gpu_amd")]
 
 HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK = 157.3
+# HBM bytes per 64-column block-GEMM launch (k_gemm_part + k_gemm_epi, FETCH_SIZE
+# and WRITE_SIZE scaled by a 1 GiB copy), measured with rocprofv3 on random
+# operands; keyed by (N, rt, pd, nsplit).  Source: profiles/kernels_N100k_M150k_bf16.md
+PROFILED_GEMM_TRAFFIC = {(100_000, 1, 3, 8): (42.10 + 0.205 + 0.108 + 0.026) * 1e9}
 BF16_MFMA_PEAK = 2500.0
 
 
@@ -385,7 +389,9 @@ def main():
         "loco": loco,
         "roofline": {"bound": "mfma", "kernel": "k_gemm (block-PCG GRM x 64 RHS, FP32 MFMA)",
                      "achieved": gemm_tflops, "peak": FP32_MFMA_PEAK, "unit": "TFLOP/s",
-                     "frac": gemm_tflops / FP32_MFMA_PEAK, "traffic": None,
+                     "frac": gemm_tflops / FP32_MFMA_PEAK,
+                     "traffic": PROFILED_GEMM_TRAFFIC.get((N, tune["rt"], tune["pd"], tune["nsplit"])),
+                     "traffic_source": "profiles/kernels_N100k_M150k_bf16.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
                      "algorithmic": f"2*N^2*64 = {gemm_flop:.3e} flop per launch; 4*N^2 = {4.0 * N * N:.3e} B",
                      "avg_launch_ms": gemm_avg * 1e3, "launches": gemm_launches,
                      "half_width_launches": half_launches, "half_width_avg_ms": half_avg * 1e3},

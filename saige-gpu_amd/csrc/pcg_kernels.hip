@@ -604,6 +604,20 @@ __global__ void k_fill_zero(uint32_t* p, int64_t n) {
     if (t < n) p[t] = 0u;
 }
 
+// Deterministic values in [-1, 1) (a 32-bit integer hash of the index), so
+// that timing runs see operands with realistic bit activity.
+__global__ void k_fill_hash(float* p, int64_t n, uint32_t seed) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    uint32_t h = (uint32_t)t * 0x9E3779B1u ^ seed;
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    p[t] = (float)(int32_t)h * (1.0f / 2147483648.0f);
+}
+
 inline unsigned nblk(int64_t n, int64_t b) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace
@@ -761,6 +775,12 @@ hipError_t launch_fill_zero(hipStream_t s, void* p, int64_t bytes) {
     const int64_t n = bytes / 4;
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_zero, dim3(nblk(n, 256)), dim3(256), 0, s, (uint32_t*)p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_hash(hipStream_t s, float* p, int64_t n, uint32_t seed) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_hash, dim3(nblk(n, 256)), dim3(256), 0, s, p, n, seed);
     return hipGetLastError();
 }
 

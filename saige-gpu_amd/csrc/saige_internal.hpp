@@ -287,5 +287,6 @@ hipError_t launch_scatter_cols(hipStream_t s, const float* src, int64_t k, int64
 hipError_t launch_gather_cols(hipStream_t s, const float* src, int64_t k, int64_t Nnm, int kpad,
                               float* dst);
 hipError_t launch_fill_zero(hipStream_t s, void* p, int64_t bytes);
+hipError_t launch_fill_hash(hipStream_t s, float* p, int64_t n, uint32_t seed);
 
 }  // namespace saige
