@@ -242,6 +242,121 @@ __global__ __launch_bounds__(256) void k_syrk_pk(const uint8_t* __restrict__ tp,
     store_tile(S, ldS, bi, bj, wr, wc, lane, acc, accumulate);
 }
 
+// 256x256 tiles for the 2-bit input: 512 threads (8 waves as 4 x 2), each
+// wave a 64 x 128 block of the tile (2 x 4 MFMA 32x32 blocks, 128
+// accumulator registers).  Per 64-marker chunk a thread decodes the same 64
+// values as in k_syrk_pk, but each decoded value feeds twice the MFMA work,
+// which halves the VALU decode per MFMA (the bound of the 128 tile).  The
+// per-element accumulation order (chunk by chunk, 16 markers per MFMA) is
+// that of k_syrk_pk, so both give bitwise equal S.
+constexpr int BT2 = 256;
+
+__device__ __forceinline__ void store_tile256(float* S, int64_t ldS, int64_t bi, int64_t bj, int wr, int wc,
+                                              int lane, const f32x16 (&acc)[2][4], int accumulate) {
+    const bool mirror = bi != bj;
+#pragma unroll
+    for (int mi = 0; mi < 2; mi++) {
+#pragma unroll
+        for (int ni = 0; ni < 4; ni++) {
+            const int64_t col = bj * BT2 + wc * 128 + ni * 32 + (lane & 31);
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const int64_t row0 = bi * BT2 + wr * 64 + mi * 32 + 8 * g + 4 * (lane >> 5);
+                float v[4];
+#pragma unroll
+                for (int e = 0; e < 4; e++) v[e] = acc[mi][ni][4 * g + e];
+                if (accumulate > 0) {
+#pragma unroll
+                    for (int e = 0; e < 4; e++) v[e] = S[(row0 + e) * ldS + col] + v[e];
+                } else if (accumulate < 0) {
+#pragma unroll
+                    for (int e = 0; e < 4; e++) v[e] = S[(row0 + e) * ldS + col] - v[e];
+                }
+#pragma unroll
+                for (int e = 0; e < 4; e++) S[(row0 + e) * ldS + col] = v[e];
+                if (mirror) {
+                    float4* dst = reinterpret_cast<float4*>(S + col * ldS + row0);
+                    *dst = make_float4(v[0], v[1], v[2], v[3]);
+                }
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(512) void k_syrk_pk256(const uint8_t* __restrict__ tp, int64_t nchunk, int64_t c0,
+                                                    int64_t c1, const uint64_t* __restrict__ lut64,
+                                                    const int2* __restrict__ tiles, int64_t nblk,
+                                                    float* __restrict__ S, int64_t ldS, int accumulate) {
+    __shared__ __attribute__((aligned(16))) __hip_bfloat16 sA[BT2 * P16];
+    __shared__ __attribute__((aligned(16))) __hip_bfloat16 sB[BT2 * P16];
+    const int2 t = tiles[xcd_remap(blockIdx.x, nblk)];
+    const int64_t bi = t.x, bj = t.y;
+    const bool diag = bi == bj;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wr = wv >> 1, wc = wv & 1;
+    // thread -> marker octet o, byte row b of the 256-sample panel (two
+    // 128-sample blocks of the tiled layout)
+    const int o = tid & 7, b = tid >> 3;
+    const int64_t pa = 2 * bi + (b >> 5), pb = 2 * bj + (b >> 5);
+    const uint64_t* gA = reinterpret_cast<const uint64_t*>(tp + (pa * nchunk * 32 + (b & 31)) * 64 + 8 * o);
+    const uint64_t* gB = reinterpret_cast<const uint64_t*>(tp + (pb * nchunk * 32 + (b & 31)) * 64 + 8 * o);
+    uint64_t ra = gA[c0 * 256], rb = gB[c0 * 256];
+    const uint64_t* gl = lut64 + 8 * o;
+    uint64_t la[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) la[e] = gl[c0 * 64 + e];
+
+    f32x16 acc[2][4];
+#pragma unroll
+    for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+        for (int ni = 0; ni < 4; ni++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[mi][ni][r] = 0.0f;
+
+    for (int64_t kc = c0; kc < c1; kc++) {
+        uint8_t ba[8], bb[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            ba[e] = (uint8_t)(ra >> (8 * e));
+            bb[e] = (uint8_t)(rb >> (8 * e));
+        }
+        uint64_t lc[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) lc[e] = la[e];
+        const int64_t kn = (kc + 1 < c1) ? kc + 1 : kc;
+        ra = gA[kn * 256];
+        rb = gB[kn * 256];
+#pragma unroll
+        for (int e = 0; e < 8; e++) la[e] = gl[kn * 64 + e];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            *reinterpret_cast<u32x4*>(sA + (4 * b + j) * P16 + 8 * o) = decode_row(lc, ba, j);
+            if (!diag) *reinterpret_cast<u32x4*>(sB + (4 * b + j) * P16 + 8 * o) = decode_row(lc, bb, j);
+        }
+        __syncthreads();
+        const __hip_bfloat16* pB = diag ? sA : sB;
+#pragma unroll
+        for (int ks = 0; ks < BK16 / 16; ks++) {
+            const int kk = ks * 16 + (lane >> 5) * 8;
+            bf16x8 fa[2], fb[4];
+#pragma unroll
+            for (int mi = 0; mi < 2; mi++)
+                fa[mi] = *reinterpret_cast<const bf16x8*>(sA + (wr * 64 + mi * 32 + (lane & 31)) * P16 + kk);
+#pragma unroll
+            for (int ni = 0; ni < 4; ni++)
+                fb[ni] = *reinterpret_cast<const bf16x8*>(pB + (wc * 128 + ni * 32 + (lane & 31)) * P16 + kk);
+#pragma unroll
+            for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+                for (int ni = 0; ni < 4; ni++)
+                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[mi], fb[ni], acc[mi][ni], 0, 0, 0);
+        }
+    }
+    store_tile256(S, ldS, bi, bj, wr, wc, lane, acc, accumulate);
+}
+
 // Re-tile the packed genotypes for k_syrk_pk: block (p, c) byte (b, m) =
 // byte 32p + b of QC'd marker 64c + m, with padding codes 0b01.
 __global__ __launch_bounds__(256) void k_pack_tiles(const uint8_t* __restrict__ packed, int64_t Mqc, int64_t Nnm,
@@ -385,16 +500,29 @@ hipError_t launch_lut64_range(hipStream_t s, const float4* lut, int64_t Mqc, int
 }
 
 hipError_t launch_syrk_pk(hipStream_t s, const uint8_t* tp, const uint64_t* lut64, int64_t Mqc, int64_t Npad,
-                          const int2* tiles, float* S, int accumulate, int64_t j0, int64_t j1) {
-    const int64_t nt = Npad / BT;
+                          const int2* tiles, float* S, int accumulate, int64_t j0, int64_t j1, int tile) {
+    if ((tile != BT && tile != BT2) || Npad % tile != 0) return hipErrorInvalidValue;
+    const int64_t nt = Npad / tile;
     const int64_t nblk = nt * (nt + 1) / 2;
     const int64_t nchunk = (Mqc + 63) / 64;
     if (j1 < 0) j1 = Mqc;
     const int64_t c0 = j0 / 64, c1 = (j1 + 63) / 64;
     if (c0 >= c1) return hipSuccess;
-    hipLaunchKernelGGL(k_syrk_pk, dim3((unsigned)nblk), dim3(256), 0, s, tp, nchunk, c0, c1, lut64, tiles, nblk, S,
-                       Npad, accumulate);
+    if (tile == BT2)
+        hipLaunchKernelGGL(k_syrk_pk256, dim3((unsigned)nblk), dim3(512), 0, s, tp, nchunk, c0, c1, lut64, tiles,
+                           nblk, S, Npad, accumulate);
+    else
+        hipLaunchKernelGGL(k_syrk_pk, dim3((unsigned)nblk), dim3(256), 0, s, tp, nchunk, c0, c1, lut64, tiles,
+                           nblk, S, Npad, accumulate);
     return hipGetLastError();
+}
+
+// Tile edge of the 2-bit-input SYRK: 256 unless SAIGE_SYRK_TILE=128 (the
+// tests compare the two, which are bitwise equal).
+int syrk_pk_tile() {
+    if (const char* env = std::getenv("SAIGE_SYRK_TILE"))
+        if (std::atoi(env) == BT) return BT;
+    return BT2;
 }
 
 int syrk_tile() { return BT; }

@@ -830,7 +830,8 @@ int saige_build_grm(saige_ctx* c, int precision) {
         uint8_t* d_tp = nullptr;
         uint64_t* d_l64 = nullptr;
         int2* d_tiles = nullptr;
-        const std::vector<int2> order = syrk_tile_order(Np / syrk_tile(), 8);
+        const int tile = syrk_pk_tile();
+        const std::vector<int2> order = syrk_tile_order(Np / tile, tile == 256 ? 4 : 8);
         auto release = [&]() { dfree(d_tp); dfree(d_l64); dfree(d_tiles); };
         if (dalloc(&d_tp, (Np / syrk_tile()) * nchunk * 2048) || dalloc(&d_l64, nchunk * 64) ||
             dalloc(&d_tiles, (int64_t)order.size())) {
@@ -841,7 +842,7 @@ int saige_build_grm(saige_ctx* c, int precision) {
         hipError_t e = hipMemcpyAsync(d_tiles, order.data(), sizeof(int2) * order.size(), hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = hipEventRecord(c->ev0, st);
         if (e == hipSuccess) e = launch_pack_tiles(st, c->d_packed, c->d_lut, c->Mqc, c->Nnm, Np, c->bpm_nm, d_tp, d_l64);
-        if (e == hipSuccess) e = launch_syrk_pk(st, d_tp, d_l64, c->Mqc, Np, d_tiles, c->d_grm, 0);
+        if (e == hipSuccess) e = launch_syrk_pk(st, d_tp, d_l64, c->Mqc, Np, d_tiles, c->d_grm, 0, 0, -1, tile);
         if (e == hipSuccess) e = hipEventRecord(c->ev1, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         release();
@@ -1187,7 +1188,9 @@ int saige_set_start_end_index(saige_ctx* c, int64_t start, int64_t end, int chro
         }
     }
     c->loco_ready = false;
-    const std::vector<int2> order = syrk_tile_order(Np / syrk_tile(), 8);
+    // the range SYRK uses the kernel (and tile) that built S
+    const int tile = c->grm_precision == SAIGE_GRM_BF16 ? syrk_pk_tile() : syrk_tile();
+    const std::vector<int2> order = syrk_tile_order(Np / tile, tile == 256 ? 4 : 8);
     int2* d_tiles = nullptr;
     float* d_rdiag = nullptr;
     uint8_t* d_tp = nullptr;
@@ -1216,7 +1219,7 @@ int saige_set_start_end_index(saige_ctx* c, int64_t start, int64_t end, int chro
         e = launch_pack_tiles(st, c->d_packed, c->d_lut, c->Mqc, N, Np, c->bpm_nm, d_tp, d_l64);
         if (e == hipSuccess) e = launch_lut64_range(st, c->d_lut, c->Mqc, start, end + 1, d_l64);
         if (e == hipSuccess)
-            e = launch_syrk_pk(st, d_tp, d_l64, c->Mqc, Np, d_tiles, c->d_grm_loco, -1, start, end + 1);
+            e = launch_syrk_pk(st, d_tp, d_l64, c->Mqc, Np, d_tiles, c->d_grm_loco, -1, start, end + 1, tile);
     } else if (e == hipSuccess) {
         const bool bf16 = c->grm_precision == SAIGE_GRM_BF16_PANEL;
         const int64_t esz = bf16 ? 2 : 4;

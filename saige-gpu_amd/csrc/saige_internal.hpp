@@ -209,8 +209,10 @@ hipError_t launch_pack_tiles(hipStream_t s, const uint8_t* packed, const float4*
                              int64_t Npad, int64_t bpm_nm, uint8_t* tp, uint64_t* lut64);
 // markers [j0, j1) only (j1 < 0: all); lut64 must be zero outside the range.
 // accumulate: 0 overwrite, 1 add, -1 subtract.
+// tile: 128 or 256 (tiles from syrk_tile_order(Npad / tile, ...)).
 hipError_t launch_syrk_pk(hipStream_t s, const uint8_t* tp, const uint64_t* lut64, int64_t Mqc, int64_t Npad,
-                          const int2* tiles, float* S, int accumulate, int64_t j0 = 0, int64_t j1 = -1);
+                          const int2* tiles, float* S, int accumulate, int64_t j0, int64_t j1, int tile);
+int syrk_pk_tile();
 hipError_t launch_lut64_range(hipStream_t s, const float4* lut, int64_t Mqc, int64_t j0, int64_t j1,
                               uint64_t* lut64);
 hipError_t launch_sub(hipStream_t s, const float* a, const float* b, int64_t n, float* out);

@@ -655,10 +655,14 @@ class BatchScheduler:
     coroutines.  Nothing touches the library during that time; a resumed
     fit's new system and columns are applied once the iteration is back."""
 
-    def __init__(self, sess, opt, width=64, n_systems=16, hbm_reserve_bytes=8 << 30, tune_malloc=True):
+    def __init__(self, sess, opt, width=64, n_systems=16, hbm_reserve_bytes=8 << 30, tune_malloc=True,
+                 host_threads=4):
         if tune_malloc:
             tune_host_allocator()
         self.sess = sess
+        self.host_threads = max(1, int(host_threads))
+        self._host_pool = concurrent.futures.ThreadPoolExecutor(max_workers=self.host_threads) \
+            if self.host_threads > 1 else None
         self.opt = opt
         self.width = width
         self.N = sess.getNnomissingOut()
@@ -724,13 +728,16 @@ class BatchScheduler:
             free, _ = self.sess.device_memory()
             if free < self.hbm_reserve_bytes:
                 return
+        admitted = []
         while self.waiting and self.free_sys:
             f = self.waiting.popleft()
             f.sys = self.free_sys.popleft()
             f.t_admit = time.perf_counter()
             self.active.append(f)
+            admitted.append((f, None))
+        if admitted:
             t = time.perf_counter()
-            self._advance(f, None)
+            self._advance(admitted)
             self.timing["host"] += time.perf_counter() - t
 
     def _expand(self, f, v):
@@ -743,34 +750,55 @@ class BatchScheduler:
     def _compact(self, f, v):
         return v if f.subset is None else v[f.subset]
 
-    def _advance(self, f, value):
-        while True:
-            try:
-                req = f.gen.send(value)
-            except StopIteration as e:
-                f.result = e.value
-                self._finish(f)
-                return
-            except Exception as e:   # a failing fit does not stop the others
-                f.error = e
-                self._finish(f)
-                return
-            if req[0] == "matvec":
-                B = req[1]
-                if not f.sys_ready:          # products only need the system's sample mask
-                    f.sys_pending = (np.ones(B.shape[0]), (1.0, 0.0))
-                    f.sys_ready = True
-                kind = "product"
-            else:
-                _, w, tau, B = req
-                f.sys_pending = (w, tau)
-                f.sys_ready = True
-                kind = "solve"
-            f.result_buf = np.zeros(B.shape, np.float64, order="F")
-            f.remaining = B.shape[1]
-            for j in range(B.shape[1]):
-                f.pending.append((j, B[:, j], kind))
+    @staticmethod
+    def _send(f, value):
+        """Resume a fit's coroutine: ("req", request) | ("done", result) |
+        ("error", exception).  Touches nothing but the fit's own generator,
+        so several fits may be resumed on worker threads at once."""
+        try:
+            return ("req", f.gen.send(value))
+        except StopIteration as e:
+            return ("done", e.value)
+        except Exception as e:   # a failing fit does not stop the others
+            return ("error", e)
+
+    def _apply(self, f, out):
+        kind, v = out
+        if kind == "done":
+            f.result = v
+            self._finish(f)
             return
+        if kind == "error":
+            f.error = v
+            self._finish(f)
+            return
+        req = v
+        if req[0] == "matvec":
+            B = req[1]
+            if not f.sys_ready:          # products only need the system's sample mask
+                f.sys_pending = (np.ones(B.shape[0]), (1.0, 0.0))
+                f.sys_ready = True
+            kind = "product"
+        else:
+            _, w, tau, B = req
+            f.sys_pending = (w, tau)
+            f.sys_ready = True
+            kind = "solve"
+        f.result_buf = np.zeros(B.shape, np.float64, order="F")
+        f.remaining = B.shape[1]
+        for j in range(B.shape[1]):
+            f.pending.append((j, B[:, j], kind))
+
+    def _advance(self, items):
+        """Resume fits [(fit, value), ...]: their host work (IRLS, projections,
+        trace dots: numpy, which releases the GIL) runs on a small thread pool,
+        then their requests are queued in order."""
+        if len(items) > 1 and self.host_threads > 1:
+            outs = list(self._host_pool.map(lambda fv: self._send(*fv), items))
+        else:
+            outs = [self._send(f, v) for f, v in items]
+        for (f, _), out in zip(items, outs):
+            self._apply(f, out)
 
     def _set_system(self, f, w, tau):
         mask = None
@@ -865,8 +893,7 @@ class BatchScheduler:
                 f.result_buf[:, j] = X[:, c] if f.subset is None else X[f.subset, c]
         self.timing["copy"] += time.perf_counter() - t
         t = time.perf_counter()
-        for f in resume:
-            self._advance(f, f.result_buf)
+        self._advance([(f, f.result_buf) for f in resume])
         self.timing["host"] += time.perf_counter() - t
 
 

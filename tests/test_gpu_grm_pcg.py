@@ -78,6 +78,24 @@ def test_bf16_two_bit_input_equals_panels(session, grm1000, monkeypatch):
     assert np.array_equal(Kp.view(np.uint32), Kb.view(np.uint32))
 
 
+def test_two_bit_syrk_tiles_agree(session, grm1000, monkeypatch):
+    """The 256x256-tile and 128x128-tile 2-bit SYRKs accumulate every element
+    in the same order: bitwise equal GRMs, and LOCO ranges built by each."""
+    print("case: SYRK 2-bit 128 vs 256 tiles", flush=True)
+    monkeypatch.delenv("SAIGE_GRM_PANEL_MARKERS", raising=False)
+    out = {}
+    for tile in ("128", "256"):
+        monkeypatch.setenv("SAIGE_SYRK_TILE", tile)
+        session.build_grm("bf16")
+        K = session.grm_block(0, 0, 1000, 1000)
+        session.setStartEndIndex(1000, 2999, 0)
+        e = np.zeros(1000, np.float32)
+        e[17] = 1
+        out[tile] = (K, session.getCrossprodMatAndKin_LOCO(e))
+    assert np.array_equal(out["128"][0].view(np.uint32), out["256"][0].view(np.uint32))
+    assert np.array_equal(out["128"][1].view(np.uint32), out["256"][1].view(np.uint32))
+
+
 def test_grm_products(session, grm1000):
     og, S = grm1000
     print("case: GRM products", flush=True)
