@@ -106,6 +106,7 @@ void free_geno(saige_ctx* c) {
     c->loaded = false;
     c->grm_ready = false;
     c->loco_ready = false;
+    c->loco_inplace = false;
     c->loco_start = c->loco_end = -1;
     c->loco_chrom = -1;
     c->loco_m = 0;
@@ -203,7 +204,7 @@ struct GrmView {
 };
 
 GrmView grm_view(const saige_ctx* c, int loco) {
-    if (loco) return {c->d_grm_loco, c->d_diag_loco, c->loco_m, 0};
+    if (loco) return {c->loco_inplace ? c->d_grm : c->d_grm_loco, c->d_diag_loco, c->loco_m, 0};
     return {c->d_grm, c->d_diag, c->Mqc, c->diag_one};
 }
 
@@ -552,6 +553,10 @@ int read_bed_body(const char* bed_file, int64_t N, int64_t M, std::vector<uint8_
 
 extern "C" {
 
+// Adds an in-place LOCO range back to S (no-op otherwise); every entry point
+// that uses the full GRM calls it first.
+static int restore_full_grm(saige_ctx* c);
+
 const char* saige_last_error(void) { return g_err.c_str(); }
 
 int saige_create(int device, saige_ctx** out) {
@@ -824,6 +829,7 @@ int saige_build_grm(saige_ctx* c, int precision) {
     }
     c->grm_ready = false;
     c->loco_ready = false;   // S_loco derives from S
+    c->loco_inplace = false;
     if (precision == SAIGE_GRM_BF16) {
         // 2-bit operands straight into the MFMA SYRK: no Z panel
         const int64_t nchunk = (c->Mqc + 63) / 64;
@@ -926,6 +932,7 @@ double saige_last_grm_seconds(const saige_ctx* c) { return c ? c->grm_seconds : 
 int saige_get_grm_block(saige_ctx* c, int64_t r0, int64_t c0, int64_t nr, int64_t nc, float* out) {
     CHECK_CTX(c);
     CHECK_GRM(c);
+    if (int rc = restore_full_grm(c)) return rc;
     if (r0 < 0 || c0 < 0 || nr < 0 || nc < 0 || r0 + nr > c->Nnm || c0 + nc > c->Nnm) {
         set_error("GRM block out of range");
         return SAIGE_E_ARG;
@@ -990,6 +997,7 @@ static int grm_matvec(saige_ctx* c, int loco, const float* B, int64_t k, float* 
 int saige_grm_matvec(saige_ctx* c, const float* B, int64_t k, float* Y) {
     CHECK_CTX(c);
     CHECK_GRM(c);
+    if (int rc = restore_full_grm(c)) return rc;
     return grm_matvec(c, 0, B, k, Y);
 }
 
@@ -1066,6 +1074,7 @@ static int get_crossprod(saige_ctx* c, int loco, const float* b, const float* w,
 int saige_get_crossprod(saige_ctx* c, const float* b, const float* w, const float* tau, float* out) {
     CHECK_CTX(c);
     CHECK_GRM(c);
+    if (int rc = restore_full_grm(c)) return rc;
     return get_crossprod(c, 0, b, w, tau, out);
 }
 
@@ -1099,6 +1108,7 @@ int saige_pcg1(saige_ctx* c, const float* w, const float* tau, const float* b, i
                int32_t* iters) {
     CHECK_CTX(c);
     CHECK_GRM(c);
+    if (int rc = restore_full_grm(c)) return rc;
     return pcg1(c, 0, w, tau, b, maxiter, tol, x, iters);
 }
 
@@ -1106,6 +1116,7 @@ int saige_pcg_block(saige_ctx* c, const float* w, const float* tau, const float*
                     float tol, float* X, int32_t* iters) {
     CHECK_CTX(c);
     CHECK_GRM(c);
+    if (int rc = restore_full_grm(c)) return rc;
     if (!w || !tau || !B || !X || k <= 0) {
         set_error("invalid arguments");
         return SAIGE_E_ARG;
@@ -1156,70 +1167,36 @@ int saige_pcg_block(saige_ctx* c, const float* w, const float* tau, const float*
 
 // ---------------------------------------------------------------- LOCO
 
-int saige_set_start_end_index(saige_ctx* c, int64_t start, int64_t end, int chrom) {
-    CHECK_CTX(c);
-    CHECK_GRM(c);
-    if (start < 0 || end < start || end >= c->Mqc) {
-        set_error("LOCO range [%lld, %lld] outside the %lld GRM markers", (long long)start, (long long)end,
-                  (long long)c->Mqc);
-        return SAIGE_E_ARG;
-    }
-    const int64_t cnt = end - start + 1;
-    if (cnt >= c->Mqc) {
-        set_error("LOCO range covers every GRM marker; the leave-out GRM would be empty");
-        return SAIGE_E_ARG;
-    }
-    if (c->eng.loco && c->eng.X) {
-        for (int k = 0; k < c->eng.kpad; k++)
-            if (c->eng.h_live[k]) {
-                set_error("the block engine has live LOCO slots; take them before changing the range");
-                return SAIGE_E_STATE;
-            }
-    }
+namespace {
+// S (+/-)= Z_r Z_r^T over GRM markers [start, end], with the kernel and
+// operand precision the resident S was built with (accumulate: 1 add, -1
+// subtract).  Asynchronous on the context's stream.
+hipError_t range_syrk(saige_ctx* c, int64_t start, int64_t end, float* S, int accumulate) {
     hipStream_t st = c->stream;
     const int64_t Np = c->Npad, N = c->Nnm;
-    if (!c->d_grm_loco) {
-        if (dalloc(&c->d_grm_loco, Np * Np) || dalloc(&c->d_diag_loco, Np)) {
-            dfree(c->d_grm_loco);
-            dfree(c->d_diag_loco);
-            set_error("cannot allocate the %lld x %lld FP32 LOCO GRM (%.1f GB)", (long long)Np, (long long)Np,
-                      4.0 * Np * Np / 1e9);
-            return SAIGE_E_NOMEM;
-        }
-    }
-    c->loco_ready = false;
-    // the range SYRK uses the kernel (and tile) that built S
+    const int64_t cnt = end - start + 1;
     const int tile = c->grm_precision == SAIGE_GRM_BF16 ? syrk_pk_tile() : syrk_tile();
     const std::vector<int2> order = syrk_tile_order(Np / tile, tile == 256 ? 4 : 8);
     int2* d_tiles = nullptr;
-    float* d_rdiag = nullptr;
     uint8_t* d_tp = nullptr;
     uint64_t* d_l64 = nullptr;
     void* d_panel = nullptr;
     auto release = [&]() {
-        dfree(d_tiles); dfree(d_rdiag); dfree(d_tp); dfree(d_l64);
+        (void)hipStreamSynchronize(st);
+        dfree(d_tiles); dfree(d_tp); dfree(d_l64);
         if (d_panel) (void)hipFree(d_panel);
     };
-    if (dalloc(&d_tiles, (int64_t)order.size()) || dalloc(&d_rdiag, Np)) {
-        release();
-        set_error("device allocation failed (LOCO)");
-        return SAIGE_E_NOMEM;
-    }
-    hipError_t e = hipMemcpyAsync(d_tiles, order.data(), sizeof(int2) * order.size(), hipMemcpyHostToDevice, st);
-    // S_loco = S - Z_r Z_r^T with the kernel and operand precision S was built with
+    hipError_t e = dalloc(&d_tiles, (int64_t)order.size());
     if (e == hipSuccess)
-        e = hipMemcpyAsync(c->d_grm_loco, c->d_grm, sizeof(float) * Np * Np, hipMemcpyDeviceToDevice, st);
+        e = hipMemcpyAsync(d_tiles, order.data(), sizeof(int2) * order.size(), hipMemcpyHostToDevice, st);
     if (e == hipSuccess && c->grm_precision == SAIGE_GRM_BF16) {
         const int64_t nchunk = (c->Mqc + 63) / 64;
-        if (dalloc(&d_tp, (Np / syrk_tile()) * nchunk * 2048) || dalloc(&d_l64, nchunk * 64)) {
-            release();
-            set_error("cannot allocate the tiled genotypes for the LOCO SYRK");
-            return SAIGE_E_NOMEM;
-        }
-        e = launch_pack_tiles(st, c->d_packed, c->d_lut, c->Mqc, N, Np, c->bpm_nm, d_tp, d_l64);
+        e = dalloc(&d_tp, (Np / syrk_tile()) * nchunk * 2048);
+        if (e == hipSuccess) e = dalloc(&d_l64, nchunk * 64);
+        if (e == hipSuccess) e = launch_pack_tiles(st, c->d_packed, c->d_lut, c->Mqc, N, Np, c->bpm_nm, d_tp, d_l64);
         if (e == hipSuccess) e = launch_lut64_range(st, c->d_lut, c->Mqc, start, end + 1, d_l64);
         if (e == hipSuccess)
-            e = launch_syrk_pk(st, d_tp, d_l64, c->Mqc, Np, d_tiles, c->d_grm_loco, -1, start, end + 1, tile);
+            e = launch_syrk_pk(st, d_tp, d_l64, c->Mqc, Np, d_tiles, S, accumulate, start, end + 1, tile);
     } else if (e == hipSuccess) {
         const bool bf16 = c->grm_precision == SAIGE_GRM_BF16_PANEL;
         const int64_t esz = bf16 ? 2 : 4;
@@ -1238,16 +1215,100 @@ int saige_set_start_end_index(saige_ctx* c, int64_t start, int64_t end, int chro
                 e = launch_std_panel_bf16(st, c->d_packed, c->d_lut, j0, np, N, Np, c->bpm_nm, ldz,
                                           static_cast<__hip_bfloat16*>(d_panel));
                 if (e == hipSuccess)
-                    e = launch_syrk_bf16(st, static_cast<__hip_bfloat16*>(d_panel), Np, ldz, ldz, d_tiles,
-                                         c->d_grm_loco, -1);
+                    e = launch_syrk_bf16(st, static_cast<__hip_bfloat16*>(d_panel), Np, ldz, ldz, d_tiles, S,
+                                         accumulate);
             } else {
                 e = launch_std_panel_f32(st, c->d_packed, c->d_lut, j0, np, N, Np, c->bpm_nm, ldz,
                                          static_cast<float*>(d_panel));
                 if (e == hipSuccess)
-                    e = launch_syrk_f32(st, static_cast<float*>(d_panel), Np, ldz, ldz, d_tiles, c->d_grm_loco,
-                                        -1);
+                    e = launch_syrk_f32(st, static_cast<float*>(d_panel), Np, ldz, ldz, d_tiles, S, accumulate);
             }
         }
+    }
+    release();
+    return e;
+}
+}  // namespace
+
+static int restore_full_grm(saige_ctx* c) {
+    if (!c->loco_inplace) return SAIGE_OK;
+    if (c->eng.X && c->eng.loco)
+        for (int k = 0; k < c->eng.kpad; k++)
+            if (c->eng.h_live[k]) {
+                set_error("live LOCO slots use the in-place LOCO GRM; take them before using the full GRM");
+                return SAIGE_E_STATE;
+            }
+    hipError_t e = range_syrk(c, c->loco_start, c->loco_end, c->d_grm, 1);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        set_error("restoring the GRM after an in-place LOCO range failed: %s", hipGetErrorString(e));
+        return SAIGE_E_DEVICE;
+    }
+    c->loco_inplace = false;
+    c->loco_ready = false;
+    return SAIGE_OK;
+}
+
+int saige_set_start_end_index(saige_ctx* c, int64_t start, int64_t end, int chrom) {
+    CHECK_CTX(c);
+    CHECK_GRM(c);
+    if (start < 0 || end < start || end >= c->Mqc) {
+        set_error("LOCO range [%lld, %lld] outside the %lld GRM markers", (long long)start, (long long)end,
+                  (long long)c->Mqc);
+        return SAIGE_E_ARG;
+    }
+    const int64_t cnt = end - start + 1;
+    if (cnt >= c->Mqc) {
+        set_error("LOCO range covers every GRM marker; the leave-out GRM would be empty");
+        return SAIGE_E_ARG;
+    }
+    if (c->eng.X) {
+        for (int k = 0; k < c->eng.kpad; k++)
+            if (c->eng.h_live[k]) {
+                set_error("the block engine has live slots; take them before changing the LOCO range");
+                return SAIGE_E_STATE;
+            }
+    }
+    hipStream_t st = c->stream;
+    const int64_t Np = c->Npad, N = c->Nnm;
+    // A second Npad^2 buffer when HBM allows (S stays intact); otherwise the
+    // range is subtracted from S itself and added back before the next
+    // full-GRM use (SAIGE_LOCO_INPLACE=1 forces this).
+    const char* env = std::getenv("SAIGE_LOCO_INPLACE");
+    const bool force_inplace = env && std::atoi(env) != 0;
+    if (!c->d_diag_loco && dalloc(&c->d_diag_loco, Np)) {
+        set_error("device allocation failed (LOCO diagonal)");
+        return SAIGE_E_NOMEM;
+    }
+    bool two_buffers = !force_inplace && c->d_grm_loco != nullptr;
+    if (!force_inplace && !two_buffers) {
+        size_t free_b = 0, total_b = 0;
+        SAIGE_HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+        if ((int64_t)free_b >= 4 * Np * Np + ((int64_t)4 << 30) && dalloc(&c->d_grm_loco, Np * Np) == hipSuccess)
+            two_buffers = true;
+    }
+    c->loco_ready = false;
+    float* d_rdiag = nullptr;
+    if (dalloc(&d_rdiag, Np)) {
+        set_error("device allocation failed (LOCO)");
+        return SAIGE_E_NOMEM;
+    }
+    hipError_t e = hipSuccess;
+    if (two_buffers) {
+        if (c->loco_inplace) {   // a range left in S by an earlier in-place call goes back first
+            if (int rc = restore_full_grm(c)) {
+                dfree(d_rdiag);
+                return rc;
+            }
+        }
+        // S_loco = S - Z_r Z_r^T with the kernel and operand precision S was built with
+        e = hipMemcpyAsync(c->d_grm_loco, c->d_grm, sizeof(float) * Np * Np, hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess) e = range_syrk(c, start, end, c->d_grm_loco, -1);
+    } else {
+        if (c->loco_inplace) e = range_syrk(c, c->loco_start, c->loco_end, c->d_grm, 1);
+        c->loco_inplace = false;
+        if (e == hipSuccess) e = range_syrk(c, start, end, c->d_grm, -1);
+        if (e == hipSuccess) c->loco_inplace = true;
     }
     // diag_loco = diag - sum over the range of z^2, the range summed from zero in
     // marker order (set_Diagof_StdGeno_LOCO, FG.cpp:4947-4955)
@@ -1256,7 +1317,7 @@ int saige_set_start_end_index(saige_ctx* c, int64_t start, int64_t end, int chro
         e = launch_diag_std(st, c->d_packed + start * c->bpm_nm, c->d_lut + start, cnt, N, c->bpm_nm, d_rdiag);
     if (e == hipSuccess) e = launch_sub(st, c->d_diag, d_rdiag, Np, c->d_diag_loco);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    release();
+    dfree(d_rdiag);
     if (e != hipSuccess) {
         set_error("LOCO GRM build failed: %s", hipGetErrorString(e));
         return SAIGE_E_DEVICE;
@@ -1328,6 +1389,7 @@ int saige_engine_set_system(saige_ctx* c, int s, const float* w, const float* ta
         set_error("call saige_engine_reserve first");
         return SAIGE_E_STATE;
     }
+    if (int rc = c->eng.loco ? check_loco(c) : restore_full_grm(c)) return rc;
     return engine_set_system(c, c->eng, s, w, tau, mask);
 }
 
@@ -1378,6 +1440,7 @@ int saige_engine_step(saige_ctx* c, int n_iter, uint8_t* done, int32_t* iters) {
         set_error("engine not reserved");
         return SAIGE_E_STATE;
     }
+    if (int rc = c->eng.loco ? check_loco(c) : restore_full_grm(c)) return rc;
     return engine_step(c, c->eng, n_iter, done, iters);
 }
 

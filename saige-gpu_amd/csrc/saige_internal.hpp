@@ -131,6 +131,7 @@ struct saige_ctx {
     int64_t loco_start = -1, loco_end = -1;      // inclusive range in the QC'd marker list
     int64_t loco_m = 0;                          // Mqc - (end - start + 1)
     int loco_chrom = -1;
+    bool loco_inplace = false;                   // the range is subtracted from d_grm itself
     bool grm_ready = false;
     saige::GemmConfig gemm_cfg;                  // block-GEMM shape (saige_autotune)
     double gemm_tuned_ms = 0.0;                  // its measured time per launch, 0 if untuned

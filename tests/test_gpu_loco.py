@@ -208,3 +208,45 @@ def test_loco_loop_vs_oracle(session, loaded, trait):
             assert np.allclose(got["coefficients"], ref["coefficients"], rtol=2e-3, atol=1e-6)
             for k in ("linear_predictors", "fitted_values"):
                 assert np.max(np.abs(got[k] - ref[k])) <= 2e-3 * np.max(np.abs(ref[k])), (j, k)
+
+
+def test_loco_in_place(session, loaded, monkeypatch):
+    """In-place mode (no second N^2 buffer): the range is subtracted from S
+    itself, a new range first adds the old one back, and the next full-GRM
+    use restores S to within a few FP32 ulps."""
+    og, L = loaded
+    print("case: LOCO in place", flush=True)
+    monkeypatch.setenv("SAIGE_LOCO_INPLACE", "1")
+    session.build_grm("fp32")
+    K0 = session.grm_block(0, 0, 1000, 1000)
+    M = og["Mqc"]
+    idx = [0, 17, 500, 999]
+    for rng_ in [(4000, 6499), RANGE]:
+        session.setStartEndIndex(*rng_, 0)
+        C = _columns(session, idx)
+        Lr = L if rng_ == RANGE else O.loco_grm(og, *rng_)
+        m_loco = M - (rng_[1] - rng_[0] + 1)
+        tol = 2 * 4.0 * np.sqrt(M) * 2.0 ** -24 * (M / m_loco) * np.max(np.abs(Lr.K))
+        assert np.max(np.abs(C - Lr.K[:, idx])) <= tol, rng_
+    # the batched engine runs on the in-place LOCO GRM too
+    rng = np.random.default_rng(9)
+    w = rng.uniform(0.05, 0.25, 1000).astype(np.float32)
+    tau = np.array([1.0, 0.4], np.float32)
+    b = rng.standard_normal(1000).astype(np.float32)
+    session.engine_reserve(64, 1)
+    session.engine_use_loco(True)
+    session.engine_set_system(0, w, tau)
+    session.engine_add(0, 0, b, 500, 1e-5)
+    # a full-GRM call would have to restore S under a live LOCO slot: refused
+    with pytest.raises(SaigeError, match="in-place"):
+        session.getCrossprodMatAndKin(b)
+    while not session.engine_step(20)[0][0]:
+        pass
+    x = session.engine_take(0)
+    session.engine_use_loco(False)
+    x64 = np.linalg.solve(L.sigma(w.astype(np.float64), tau.astype(np.float64)), b.astype(np.float64))
+    assert np.max(np.abs(x - x64)) <= 2e-3 * np.max(np.abs(x64))
+    # the next full-GRM use adds the range back
+    K1 = session.grm_block(0, 0, 1000, 1000)
+    assert session.loco_range() == (-1, -1, -1, -1)
+    assert np.max(np.abs(K1 - K0)) <= 8 * 2.0 ** -24 * np.max(np.abs(K0))
