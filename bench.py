@@ -256,7 +256,9 @@ def main():
     log(f"GRM {args.grm}: {grm_s:.3f}s device time, {grm_tflops:.1f} TFLOP/s")
     tune = sess.autotune(reps=3)
     log("autotune block GEMM: " + ", ".join(f"rt{a}/pd{b}/s{c} {t:.2f}ms" for a, b, c, t in tune["table"]) +
-        f" -> rt{tune['rt']}/pd{tune['pd']}/s{tune['nsplit']}")
+        f" -> rt{tune['rt']}/pd{tune['pd']}/s{tune['nsplit']}; one half: " +
+        ", ".join(f"rt{a}/pd{b} {t:.2f}ms" for a, b, _, t in tune["half_table"]) +
+        f" -> rt{tune['half']['rt']}/pd{tune['half']['pd']}")
 
     opt = D.Options(nrun=args.nrun)
     jobs = [{"trait": "binary", "y": y, "X": X, "tag": f"prev{p}"} for y, p in zip(phenos, prevalences)]

@@ -249,18 +249,25 @@ int saige_engine_use_loco(saige_ctx* ctx, int on);
  * wave (1 or 2); pd, the 32-column chunks of S requested ahead of use (1 or
  * 3); nsplit, the column slices of S summed in a fixed order (1, 2, 4 or 8).
  * saige_autotune times every candidate on the resident GRM (reps launches
- * each after one warm-up) and keeps the fastest; ms_table (may be NULL)
- * receives the per-launch milliseconds in the order of
- * saige_autotune_candidates.  nsplit changes the summation order, so call it
- * once before solving; results are reproducible for a fixed shape.
- * SAIGE_E_STATE if the engine has live slots. */
+ * each after one warm-up, on hashed operands) and keeps the fastest; then,
+ * for launches over one 32-column half of the block, it tunes rt and pd
+ * again at the chosen nsplit (nsplit fixes the summation order, so a
+ * column's bits must not depend on whether the other half is live).
+ * ms_table (may be NULL) receives 20 per-launch times in ms: the 16 full
+ * shapes in the order of saige_autotune_candidates, then the one-half
+ * shapes (rt, pd) = (1,1), (2,1), (1,3), (2,3).  Call it once before
+ * solving; results are reproducible for a fixed shape.  SAIGE_E_STATE if the
+ * engine has live slots. */
 int saige_autotune(saige_ctx* ctx, int reps, double* ms_table);
 /* The candidate shapes (rt[i], pd[i], nsplit[i]); returns their count (16). */
 int saige_autotune_candidates(int* rt, int* pd, int* nsplit);
-/* Set the block-GEMM shape explicitly (engine idle). */
+/* Set the block-GEMM shape explicitly, for full and one-half launches
+ * (engine idle). */
 int saige_set_gemm_config(saige_ctx* ctx, int rt, int pd, int nsplit);
 /* Current shape, and its tuned per-launch time in ms (0 if not tuned). */
 int saige_get_gemm_config(const saige_ctx* ctx, int* rt, int* pd, int* nsplit, double* tuned_ms);
+/* Shape of one-half launches (nsplit as the full shape). */
+int saige_get_gemm_config_half(const saige_ctx* ctx, int* rt, int* pd, double* tuned_ms);
 
 /* Free and total device memory (hipMemGetInfo), for admission control. */
 int saige_device_memory(saige_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);

@@ -345,8 +345,8 @@ int engine_step(saige_ctx* c, Engine& e, int n_iter, uint8_t* done_out, int32_t*
         if (e.kpad == 1)
             SAIGE_HIP_TRY(launch_grm_gemv(st, g.S, Np, N, e.P, e.AP, a, e.partial, &np));
         else
-            SAIGE_HIP_TRY(launch_grm_gemm(st, g.S, Np, N, e.P, e.AP, a, e.partial, &np, e.ypart, c->gemm_cfg,
-                                          halves));
+            SAIGE_HIP_TRY(launch_grm_gemm(st, g.S, Np, N, e.P, e.AP, a, e.partial, &np, e.ypart,
+                                          halves == 3 ? c->gemm_cfg : c->gemm_cfg_half, halves));
         SAIGE_HIP_TRY(hipEventRecord(c->ev1, st));
         SAIGE_HIP_TRY(launch_alpha(st, e.kpad, e.partial, np, e.rz, e.done, e.live, e.kind, e.alpha));
         int64_t np2 = 0;
@@ -981,8 +981,8 @@ static int grm_matvec(saige_ctx* c, int loco, const float* B, int64_t k, float* 
         e = hipMemcpyAsync(d_in, B + k0 * N, 4 * N * kb, hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = launch_scatter_cols(st, d_in, kb, N, Np, EW, d_blk);
         if (e == hipSuccess)
-            e = launch_grm_gemm(st, g.S, Np, N, d_blk, d_out, a, nullptr, nullptr, c->d_mv_ypart, c->gemm_cfg,
-                                kb <= 32 ? 1 : 3);
+            e = launch_grm_gemm(st, g.S, Np, N, d_blk, d_out, a, nullptr, nullptr, c->d_mv_ypart,
+                                kb <= 32 ? c->gemm_cfg_half : c->gemm_cfg, kb <= 32 ? 1 : 3);
         if (e == hipSuccess) e = launch_gather_cols(st, d_out, kb, N, EW, d_in);
         if (e == hipSuccess) e = hipMemcpyAsync(Y + k0 * N, d_in, 4 * N * kb, hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -1476,6 +1476,9 @@ constexpr GemmConfig kGemmCandidates[] = {
     {1, 1, 1}, {1, 2, 1}, {1, 4, 1}, {1, 8, 1}, {2, 1, 1}, {2, 2, 1}, {2, 4, 1}, {2, 8, 1},
     {1, 1, 3}, {1, 2, 3}, {1, 4, 3}, {1, 8, 3}, {2, 1, 3}, {2, 2, 3}, {2, 4, 3}, {2, 8, 3}};
 constexpr int kNumGemmCandidates = (int)(sizeof(kGemmCandidates) / sizeof(kGemmCandidates[0]));
+// {rt, -, pd} for one-half launches (nsplit taken from the full shape)
+constexpr GemmConfig kHalfCandidates[] = {{1, 0, 1}, {2, 0, 1}, {1, 0, 3}, {2, 0, 3}};
+constexpr int kNumHalfCandidates = (int)(sizeof(kHalfCandidates) / sizeof(kHalfCandidates[0]));
 
 int engine_idle(const saige_ctx* c, const char* what) {
     if (c->eng.X)
@@ -1514,29 +1517,49 @@ int saige_autotune(saige_ctx* c, int reps, double* ms_table) {
     SigmaArgs a{};
     a.mqc = (float)c->Mqc;
     a.mode = 0;
+    // time one shape: one warm-up launch, then reps launches between events
+    auto time_shape = [&](GemmConfig g, int halves, double* out_ms) -> hipError_t {
+        hipError_t e = launch_grm_gemm(st, c->d_grm, Np, N, c->d_mv_blk, c->d_mv_out, a, nullptr, nullptr,
+                                       c->d_mv_ypart, g, halves);
+        if (e == hipSuccess) e = hipEventRecord(c->ev0, st);
+        for (int r = 0; r < reps && e == hipSuccess; r++)
+            e = launch_grm_gemm(st, c->d_grm, Np, N, c->d_mv_blk, c->d_mv_out, a, nullptr, nullptr, c->d_mv_ypart,
+                                g, halves);
+        if (e == hipSuccess) e = hipEventRecord(c->ev1, st);
+        if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        *out_ms = (double)ms / reps;
+        return e;
+    };
     GemmConfig best = c->gemm_cfg;
     double best_ms = 0.0;
     for (int i = 0; i < kNumGemmCandidates; i++) {
-        const GemmConfig g = kGemmCandidates[i];
-        SAIGE_HIP_TRY(launch_grm_gemm(st, c->d_grm, Np, N, c->d_mv_blk, c->d_mv_out, a, nullptr, nullptr,
-                                      c->d_mv_ypart, g));   // warm-up
-        SAIGE_HIP_TRY(hipEventRecord(c->ev0, st));
-        for (int r = 0; r < reps; r++)
-            SAIGE_HIP_TRY(launch_grm_gemm(st, c->d_grm, Np, N, c->d_mv_blk, c->d_mv_out, a, nullptr, nullptr,
-                                          c->d_mv_ypart, g));
-        SAIGE_HIP_TRY(hipEventRecord(c->ev1, st));
-        SAIGE_HIP_TRY(hipEventSynchronize(c->ev1));
-        float ms = 0.f;
-        SAIGE_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        const double per = (double)ms / reps;
+        double per = 0.0;
+        SAIGE_HIP_TRY(time_shape(kGemmCandidates[i], 3, &per));
         if (ms_table) ms_table[i] = per;
         if (i == 0 || per < best_ms) {
-            best = g;
+            best = kGemmCandidates[i];
             best_ms = per;
+        }
+    }
+    // one-half launches: rt and pd only, the slice count stays the full shape's
+    GemmConfig best_h = best;
+    double best_h_ms = 0.0;
+    for (int i = 0; i < kNumHalfCandidates; i++) {
+        GemmConfig g{kHalfCandidates[i].rt, best.nsplit, kHalfCandidates[i].pd};
+        double per = 0.0;
+        SAIGE_HIP_TRY(time_shape(g, 1, &per));
+        if (ms_table) ms_table[kNumGemmCandidates + i] = per;
+        if (i == 0 || per < best_h_ms) {
+            best_h = g;
+            best_h_ms = per;
         }
     }
     c->gemm_cfg = best;
     c->gemm_tuned_ms = best_ms;
+    c->gemm_cfg_half = best_h;
+    c->gemm_tuned_ms_half = best_h_ms;
     return SAIGE_OK;
 }
 
@@ -1560,6 +1583,19 @@ int saige_set_gemm_config(saige_ctx* c, int rt, int pd, int nsplit) {
     if (int rc = engine_idle(c, "changing the block-GEMM shape")) return rc;
     c->gemm_cfg = g;
     c->gemm_tuned_ms = 0.0;
+    c->gemm_cfg_half = g;
+    c->gemm_tuned_ms_half = 0.0;
+    return SAIGE_OK;
+}
+
+int saige_get_gemm_config_half(const saige_ctx* c, int* rt, int* pd, double* tuned_ms) {
+    if (!c) {
+        set_error("null context");
+        return SAIGE_E_ARG;
+    }
+    if (rt) *rt = c->gemm_cfg_half.rt;
+    if (pd) *pd = c->gemm_cfg_half.pd;
+    if (tuned_ms) *tuned_ms = c->gemm_tuned_ms_half;
     return SAIGE_OK;
 }
 

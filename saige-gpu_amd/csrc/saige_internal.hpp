@@ -135,6 +135,10 @@ struct saige_ctx {
     bool grm_ready = false;
     saige::GemmConfig gemm_cfg;                  // block-GEMM shape (saige_autotune)
     double gemm_tuned_ms = 0.0;                  // its measured time per launch, 0 if untuned
+    // shape for launches over one 32-column half: same nsplit (the summation
+    // order, so a column's bits do not depend on the other half), own rt/pd
+    saige::GemmConfig gemm_cfg_half;
+    double gemm_tuned_ms_half = 0.0;
     int grm_precision = -1;
     double grm_seconds = 0.0;
     // scratch vectors

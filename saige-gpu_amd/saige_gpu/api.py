@@ -276,10 +276,12 @@ class Session:
         pd = np.zeros(n, np.int32)
         ns = np.zeros(n, np.int32)
         self._L.saige_autotune_candidates(ptr(rt), ptr(pd), ptr(ns))
-        ms = np.zeros(n, np.float64)
+        ms = np.zeros(n + 4, np.float64)
         check(self._L.saige_autotune(self.ctx, int(reps), ptr(ms)))
         cfg = self.gemm_config()
-        cfg["table"] = [(int(a), int(b), int(c), float(t)) for a, b, c, t in zip(rt, pd, ns, ms)]
+        cfg["table"] = [(int(a), int(b), int(c), float(t)) for a, b, c, t in zip(rt, pd, ns, ms[:n])]
+        cfg["half_table"] = [(a, b, cfg["nsplit"], float(t))
+                             for (a, b), t in zip([(1, 1), (2, 1), (1, 3), (2, 3)], ms[n:])]
         return cfg
 
     def set_gemm_config(self, rt, pd, nsplit):
@@ -289,7 +291,10 @@ class Session:
         rt, pd, ns, ms = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0), ctypes.c_double(0.0)
         check(self._L.saige_get_gemm_config(self.ctx, ctypes.byref(rt), ctypes.byref(pd), ctypes.byref(ns),
                                             ctypes.byref(ms)))
-        return {"rt": int(rt.value), "pd": int(pd.value), "nsplit": int(ns.value), "ms": float(ms.value)}
+        hr, hp, hm = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_double(0.0)
+        check(self._L.saige_get_gemm_config_half(self.ctx, ctypes.byref(hr), ctypes.byref(hp), ctypes.byref(hm)))
+        return {"rt": int(rt.value), "pd": int(pd.value), "nsplit": int(ns.value), "ms": float(ms.value),
+                "half": {"rt": int(hr.value), "pd": int(hp.value), "ms": float(hm.value)}}
 
     def engine_use_loco(self, on):
         """Point the block engine at the LOCO GRM (on) or the full one."""

@@ -656,7 +656,7 @@ class BatchScheduler:
     fit's new system and columns are applied once the iteration is back."""
 
     def __init__(self, sess, opt, width=64, n_systems=16, hbm_reserve_bytes=8 << 30, tune_malloc=True,
-                 host_threads=4):
+                 host_threads=1):
         if tune_malloc:
             tune_host_allocator()
         self.sess = sess
@@ -790,9 +790,11 @@ class BatchScheduler:
             f.pending.append((j, B[:, j], kind))
 
     def _advance(self, items):
-        """Resume fits [(fit, value), ...]: their host work (IRLS, projections,
-        trace dots: numpy, which releases the GIL) runs on a small thread pool,
-        then their requests are queued in order."""
+        """Resume fits [(fit, value), ...] and queue their requests in order.
+        With host_threads > 1 the coroutines run on a thread pool (numpy
+        releases the GIL); on the benchmark the per-fit numpy work is too
+        small for that to pay (thread hand-off costs more), so the default
+        is 1."""
         if len(items) > 1 and self.host_threads > 1:
             outs = list(self._host_pool.map(lambda fv: self._send(*fv), items))
         else:

@@ -301,7 +301,17 @@ def test_autotune_and_every_gemm_shape(session, grm1000):
     assert len(table) == 16 and all(ms > 0 for *_, ms in table)
     best = min(table, key=lambda t: t[3])
     assert (res["rt"], res["pd"], res["nsplit"]) == best[:3] and res["ms"] == best[3]
+    half = res["half_table"]
+    assert len(half) == 4 and all(ns == res["nsplit"] and ms > 0 for _, _, ns, ms in half)
+    hbest = min(half, key=lambda t: t[3])
+    assert (res["half"]["rt"], res["half"]["pd"]) == hbest[:2] and res["half"]["ms"] == hbest[3]
     print("  " + ", ".join(f"rt{a}/pd{b}/s{c}: {t:.3f} ms" for a, b, c, t in table))
+    # with the tuned shapes (full and one-half may differ in rt/pd), a column
+    # computed in a one-half launch equals the same column in a full launch
+    Bm = np.random.default_rng(14).standard_normal((1000, 70)).astype(np.float32)
+    narrow = session.grm_matvec(Bm[:, :5])           # 5 columns: one-half launch
+    wide = session.grm_matvec(Bm)                    # 70 columns: full launches
+    assert np.array_equal(narrow.view(np.uint32), wide[:, :5].view(np.uint32))
     K64 = _k64(og, S)
     rng = np.random.default_rng(13)
     B = rng.standard_normal((1000, 70)).astype(np.float32)
@@ -312,7 +322,8 @@ def test_autotune_and_every_gemm_shape(session, grm1000):
     by_split = {}
     for rt, pd, ns, _ in table:
         session.set_gemm_config(rt, pd, ns)
-        assert session.gemm_config() == {"rt": rt, "pd": pd, "nsplit": ns, "ms": 0.0}
+        assert session.gemm_config() == {"rt": rt, "pd": pd, "nsplit": ns, "ms": 0.0,
+                                         "half": {"rt": rt, "pd": pd, "ms": 0.0}}
         KB = session.grm_matvec(B)
         assert np.max(np.abs(KB - RB)) <= 2e-5 * np.max(np.abs(RB)), (rt, pd, ns)
         X, _ = session.pcg_block(w, tau, B[:, :3], 500, 1e-5)
