@@ -237,6 +237,40 @@ int saige_engine_take_batch(saige_ctx* ctx, int k, const int32_t* slots, float* 
 int saige_engine_step(saige_ctx* ctx, int n_iter, uint8_t* done_host, int32_t* iters_host);
 /* Copy the solution of a slot to host and free the slot. */
 int saige_engine_take(saige_ctx* ctx, int slot, float* x_host);
+/* ------------------------------------------ device-resident trace terms
+ * No reference counterparts: the reference downloads every solve and forms
+ * GetTrace's t_i = (K u_i)^T P u_i with Armadillo on the host
+ * (FG.cpp:3127-3145).  These keep the probe vectors, K u and Sigma^-1 u on
+ * the device and return only FP64 dot products. */
+/* engine_add_batch / engine_add_products with B already on this device
+ * (column-major, leading dimension Nnomissing). */
+int saige_engine_add_batch_device(saige_ctx* ctx, int k, const int32_t* slots, const int32_t* systems,
+                                  const float* B_dev, int maxiter, float tol);
+int saige_engine_add_products_device(saige_ctx* ctx, int k, const int32_t* slots, const int32_t* systems,
+                                     const float* B_dev);
+/* Take k finished slots: results to X_host (N x k, may be NULL) and/or to
+ * columns xcols[j] (NULL: j) of the device buffer X_dev (leading dimension
+ * ldx, may be NULL), and the FP64 dots of each result x_j with
+ * P1_dev[:, pcols[j]] and P2_dev[:, pcols[j]] (pairwise, either may be NULL;
+ * pcols NULL: j; leading dimension ldp) and with the m shared columns of
+ * Q_dev (leading dimension ldq) into dots_host[j * nd + d],
+ * nd = [P1] + [P2] + m (at most 16), in that order.  Sums run in a fixed
+ * order.  Block engine only. */
+int saige_engine_take_batch_ex(saige_ctx* ctx, int k, const int32_t* slots, float* X_host, float* X_dev,
+                               int64_t ldx, const int32_t* xcols, const float* P1_dev, const float* P2_dev,
+                               int64_t ldp, const int32_t* pcols, const float* Q_dev, int m, int64_t ldq,
+                               double* dots_host);
+/* FP64 dots of the k columns of A_dev (leading dimension lda >= N) with the
+ * m columns of Q_dev: out_host[j * m + r]. */
+int saige_dots_device(saige_ctx* ctx, const float* A_dev, int64_t lda, int k, const float* Q_dev, int64_t ldq,
+                      int m, double* out_host);
+/* Rademacher probes i0 .. i0+k-1 (entries +-1 from a counter-based hash of
+ * (seed, i, row); FG.cpp:3133-3137 draws them with R's rbinom) into dst_dev,
+ * column-major with leading dimension ld >= N. */
+int saige_probes_device(saige_ctx* ctx, float* dst_dev, int64_t ld, uint64_t seed, int64_t i0, int k);
+/* Host-to-device copy into a buffer from saige_alloc_device. */
+int saige_copy_host_to_device(saige_ctx* ctx, void* dst_dev, const void* src_host, int64_t bytes);
+
 /* Point the block engine at the LOCO GRM of saige_set_start_end_index
  * (on = 1) or back at the full GRM (on = 0); the Get_Coef_LOCO solves of many
  * fits (FG.R:1240-1244) then run batched.  Call with no live slots, before

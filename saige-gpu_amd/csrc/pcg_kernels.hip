@@ -581,6 +581,15 @@ __global__ void k_gather_col(const float* __restrict__ src, int slot, int kpad, 
     if (i < Nnm) dst[i] = src[i * kpad + slot];
 }
 
+// slot results to chosen columns of a column-major buffer: dst[:, cols[j]]
+__global__ void k_cols_gather_to(const float* __restrict__ src, const int32_t* __restrict__ slots,
+                                 const int32_t* __restrict__ cols, int kpad, int64_t Nnm, int64_t ld,
+                                 float* __restrict__ dst) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i < Nnm) dst[(int64_t)cols[j] * ld + i] = src[i * kpad + slots[j]];
+}
+
 // column-major [k][Nnm] (host layout) <-> [Npad][kpad] row-major blocks
 __global__ void k_scatter_cols(const float* __restrict__ src, int64_t k, int64_t Nnm, int64_t Npad,
                                int kpad, float* __restrict__ dst) {
@@ -616,6 +625,76 @@ __global__ void k_fill_hash(float* p, int64_t n, uint32_t seed) {
     h *= 0xC2B2AE35u;
     h ^= h >> 16;
     p[t] = (float)(int32_t)h * (1.0f / 2147483648.0f);
+}
+
+// ------------------------------------------------------ trace probes
+// Rademacher probe i, entry r: +1 or -1 from the top bit of a counter-based
+// 64-bit hash of (seed, i, r) (splitmix64 finaliser).  The same function is
+// restated in numpy on the host, so every executor sees identical probes.
+__device__ __forceinline__ uint64_t probe_mix(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ void k_rademacher(float* __restrict__ dst, int64_t ld, int64_t N, uint64_t seed, int64_t i0) {
+    const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t j = blockIdx.y;
+    if (r >= N) return;
+    const uint64_t h = probe_mix(seed ^ probe_mix((uint64_t)(i0 + j) * 0x9E3779B97F4A7C15ull +
+                                                  probe_mix((uint64_t)r * 0xC2B2AE3D27D4EB4Full + 0x5851F42Dull)));
+    dst[j * ld + r] = (h >> 63) ? -1.0f : 1.0f;
+}
+
+// FP64 dot products of k columns x_j with partner columns: P1[:, j], P2[:, j]
+// (pairwise, optional) and Q[:, r] for r < m (shared).  x_j is either a slot
+// of an engine block ([Npad][kpad] row-major, slots != nullptr) or column j
+// of a column-major buffer.  Workgroup (b, j) sums rows [256 b, 256 b + 256)
+// in a fixed tree order into partial[b][j][d]; k_col_dots_finish adds the
+// blocks in order, so the result does not depend on the launch.
+constexpr int DOT_ROWS = 256;
+constexpr int DOT_MAX = 16;   // 2 pairwise + at most 14 shared columns
+
+__global__ __launch_bounds__(256) void k_col_dots(const float* __restrict__ X, int64_t ldx, int kpad,
+                                                  const int32_t* __restrict__ slots, int k,
+                                                  const float* __restrict__ P1, const float* __restrict__ P2,
+                                                  int64_t ldp, const int32_t* __restrict__ pcols,
+                                                  const float* __restrict__ Q, int m, int64_t ldq,
+                                                  int64_t Nnm, double* __restrict__ partial) {
+    __shared__ double sm[256];
+    const int tid = threadIdx.x;
+    const int j = blockIdx.y;
+    const int64_t i = (int64_t)blockIdx.x * DOT_ROWS + tid;
+    const bool on = i < Nnm;
+    float x = 0.0f;
+    if (on) x = slots ? X[i * kpad + slots[j]] : X[(int64_t)j * ldx + i];
+    const int nd = (P1 ? 1 : 0) + (P2 ? 1 : 0) + m;
+    double* out = partial + ((int64_t)blockIdx.x * k + j) * nd;
+    int d = 0;
+    for (int q = -2; q < m; q++) {
+        const float* src = q == -2 ? P1 : (q == -1 ? P2 : Q + (int64_t)q * ldq);
+        if (q < 0 && !src) continue;
+        const int64_t pc = pcols ? pcols[j] : j;   // partner column of x_j
+        const float* col = q == -2 ? P1 + pc * ldp : (q == -1 ? P2 + pc * ldp : src);
+        sm[tid] = on ? (double)x * (double)col[i] : 0.0;
+        for (int s = 128; s >= 1; s >>= 1) {
+            __syncthreads();
+            if (tid < s) sm[tid] += sm[tid + s];
+        }
+        __syncthreads();
+        if (tid == 0) out[d] = sm[0];
+        d++;
+    }
+}
+
+__global__ void k_col_dots_finish(const double* __restrict__ partial, int64_t nb, int k, int nd,
+                                  double* __restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= k * nd) return;
+    const int j = t / nd, d = t - j * nd;
+    double s = 0.0;
+    for (int64_t b = 0; b < nb; b++) s += partial[(b * k + j) * nd + d];
+    out[t] = s;
 }
 
 inline unsigned nblk(int64_t n, int64_t b) { return (unsigned)((n + b - 1) / b); }
@@ -775,6 +854,36 @@ hipError_t launch_fill_zero(hipStream_t s, void* p, int64_t bytes) {
     const int64_t n = bytes / 4;
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_zero, dim3(nblk(n, 256)), dim3(256), 0, s, (uint32_t*)p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_rademacher(hipStream_t s, float* dst, int64_t ld, int64_t N, uint64_t seed, int64_t i0, int k) {
+    if (k <= 0 || N <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rademacher, dim3(nblk(N, 256), (unsigned)k), dim3(256), 0, s, dst, ld, N, seed, i0);
+    return hipGetLastError();
+}
+
+int col_dots_max() { return DOT_MAX; }
+int64_t col_dots_partial_doubles(int64_t Npad, int k) { return (Npad / DOT_ROWS + 1) * (int64_t)k * DOT_MAX; }
+
+hipError_t launch_cols_gather_to(hipStream_t s, const float* src, int k, const int32_t* slots, const int32_t* cols,
+                                 int kpad, int64_t Nnm, int64_t ld, float* dst) {
+    hipLaunchKernelGGL(k_cols_gather_to, dim3(nblk(Nnm, 256), (unsigned)k), dim3(256), 0, s, src, slots, cols, kpad,
+                       Nnm, ld, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_col_dots(hipStream_t s, const float* X, int64_t ldx, int kpad, const int32_t* slots, int k,
+                           const float* P1, const float* P2, int64_t ldp, const int32_t* pcols, const float* Q,
+                           int m, int64_t ldq, int64_t Nnm, double* partial, double* out) {
+    const int nd = (P1 ? 1 : 0) + (P2 ? 1 : 0) + m;
+    if (k <= 0 || nd <= 0 || nd > DOT_MAX) return hipErrorInvalidValue;
+    const int64_t nb = (Nnm + DOT_ROWS - 1) / DOT_ROWS;
+    hipLaunchKernelGGL(k_col_dots, dim3((unsigned)nb, (unsigned)k), dim3(256), 0, s, X, ldx, kpad, slots, k, P1, P2,
+                       ldp, pcols, Q, m, ldq, Nnm, partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_col_dots_finish, dim3(nblk((int64_t)k * nd, 64)), dim3(64), 0, s, partial, nb, k, nd, out);
     return hipGetLastError();
 }
 

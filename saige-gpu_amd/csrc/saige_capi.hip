@@ -98,6 +98,7 @@ void free_geno(saige_ctx* c) {
     dfree(c->d_stats); dfree(c->d_packed); dfree(c->d_packed_vr); dfree(c->d_lut); dfree(c->d_lut_vr);
     dfree(c->d_diag); dfree(c->d_grm); dfree(c->d_vec); dfree(c->d_grm_loco); dfree(c->d_diag_loco);
     dfree(c->d_mv_in); dfree(c->d_mv_blk); dfree(c->d_mv_out); dfree(c->d_mv_ypart);
+    dfree(c->d_dots_partial); dfree(c->d_dots_out); dfree(c->d_take_lists);
     free_engine(c->eng);
     free_engine(c->eng1);
     c->h_stats.clear();
@@ -255,7 +256,7 @@ int engine_sync_host_arrays(saige_ctx* c, Engine& e, const std::vector<int32_t>*
 // Start k solves at once: column j of B_host (N x k, column-major) in slot
 // slots[j] with system systems[j].  One upload, two launches, one sync.
 int engine_add_batch(saige_ctx* c, Engine& e, int k, const int32_t* slots, const int32_t* systems,
-                     const float* B_host, int maxiter, float tol, int product = 0) {
+                     const float* B_host, int maxiter, float tol, int product = 0, bool b_on_device = false) {
     if (product && e.kpad != EW) {
         set_error("GRM-product slots need the block engine");
         return SAIGE_E_STATE;
@@ -299,9 +300,14 @@ int engine_add_batch(saige_ctx* c, Engine& e, int k, const int32_t* slots, const
     SAIGE_HIP_TRY(hipMemcpyAsync(e.maxiter, e.h_maxiter.data(), 4 * e.kpad, hipMemcpyHostToDevice, st));
     SAIGE_HIP_TRY(hipMemcpyAsync(e.tol, e.h_tol.data(), 8 * e.kpad, hipMemcpyHostToDevice, st));
     SAIGE_HIP_TRY(hipMemcpyAsync(e.kind, e.h_kind.data(), e.kpad, hipMemcpyHostToDevice, st));
-    SAIGE_HIP_TRY(hipMemcpyAsync(e.stageB, B_host, sizeof(float) * N * k, hipMemcpyHostToDevice, st));
+    // right-hand sides: column-major N x k, uploaded, or read in place on the device
+    const float* B = B_host;
+    if (!b_on_device) {
+        SAIGE_HIP_TRY(hipMemcpyAsync(e.stageB, B_host, sizeof(float) * N * k, hipMemcpyHostToDevice, st));
+        B = e.stageB;
+    }
     int64_t np = 0;
-    SAIGE_HIP_TRY(launch_cols_init(st, e.stageB, k, e.d_list, e.d_list + e.kpad, e.kpad, N, Np, e.minv, e.mask, e.X,
+    SAIGE_HIP_TRY(launch_cols_init(st, B, k, e.d_list, e.d_list + e.kpad, e.kpad, N, Np, e.minv, e.mask, e.X,
                                    e.R, e.Z, e.P, e.partial, &np, product));
     SAIGE_HIP_TRY(launch_cols_init_finish(st, k, e.d_list, e.kpad, e.partial, np, e.rz, e.sumr2, e.maxiter, e.tol,
                                           e.iter, e.done, product));
@@ -391,6 +397,94 @@ int engine_take_batch(saige_ctx* c, Engine& e, int k, const int32_t* slots, floa
         std::copy(slots, slots + k, lists.begin());
         SAIGE_HIP_TRY(hipMemcpyAsync(e.d_list, lists.data(), 8 * e.kpad, hipMemcpyHostToDevice, st));
         SAIGE_HIP_TRY(launch_cols_gather(st, e.X, k, e.d_list, e.kpad, c->Nnm, e.stageB));
+        SAIGE_HIP_TRY(hipMemcpyAsync(X_host, e.stageB, sizeof(float) * c->Nnm * k, hipMemcpyDeviceToHost, st));
+    }
+    for (int j = 0; j < k; j++) e.h_live[slots[j]] = 0;
+    SAIGE_HIP_TRY(hipMemcpyAsync(e.live, e.h_live.data(), e.kpad, hipMemcpyHostToDevice, st));
+    SAIGE_HIP_TRY(hipStreamSynchronize(st));
+    return SAIGE_OK;
+}
+
+int ensure_dots_scratch(saige_ctx* c) {
+    if (c->d_dots_partial) return SAIGE_OK;
+    if (dalloc(&c->d_dots_partial, col_dots_partial_doubles(c->Npad, EW)) ||
+        dalloc(&c->d_dots_out, (int64_t)EW * col_dots_max()) || dalloc(&c->d_take_lists, 3 * EW)) {
+        dfree(c->d_dots_partial);
+        dfree(c->d_dots_out);
+        dfree(c->d_take_lists);
+        set_error("device allocation failed (dot scratch)");
+        return SAIGE_E_NOMEM;
+    }
+    return SAIGE_OK;
+}
+
+// FP64 dots of k columns (engine slots or a column-major device buffer) with
+// pairwise and shared partner columns; the k x nd results go to dots_host.
+int column_dots(saige_ctx* c, const float* X, int64_t ldx, int kpad, const int32_t* d_slots, int k,
+                const float* P1, const float* P2, int64_t ldp, const int32_t* d_pcols, const float* Q, int m,
+                int64_t ldq, double* dots_host) {
+    const int nd = (P1 ? 1 : 0) + (P2 ? 1 : 0) + m;
+    if (nd == 0) return SAIGE_OK;
+    if (m < 0 || nd > col_dots_max() || k < 1 || k > EW || !dots_host) {
+        set_error("invalid dot request: %d columns x %d partners", k, nd);
+        return SAIGE_E_ARG;
+    }
+    if ((P1 || P2) && ldp < c->Nnm) {
+        set_error("pair buffer leading dimension %lld < N", (long long)ldp);
+        return SAIGE_E_ARG;
+    }
+    if (m > 0 && (!Q || ldq < c->Nnm)) {
+        set_error("shared partner columns missing or leading dimension < N");
+        return SAIGE_E_ARG;
+    }
+    if (int rc = ensure_dots_scratch(c)) return rc;
+    hipStream_t st = c->stream;
+    SAIGE_HIP_TRY(launch_col_dots(st, X, ldx, kpad, d_slots, k, P1, P2, ldp, d_pcols, Q, m, ldq, c->Nnm,
+                                  c->d_dots_partial, c->d_dots_out));
+    SAIGE_HIP_TRY(hipMemcpyAsync(dots_host, c->d_dots_out, sizeof(double) * k * nd, hipMemcpyDeviceToHost, st));
+    SAIGE_HIP_TRY(hipStreamSynchronize(st));
+    return SAIGE_OK;
+}
+
+// engine_take_batch with device-side options: the results may go to host
+// (X_host), to a column-major device buffer (X_dev, ld ldx) or nowhere, and
+// FP64 dots with partner columns are returned before the slots are freed.
+int engine_take_batch_ex(saige_ctx* c, Engine& e, int k, const int32_t* slots, float* X_host, float* X_dev,
+                         int64_t ldx, const int32_t* xcols, const float* P1, const float* P2, int64_t ldp,
+                         const int32_t* pcols, const float* Q, int m, int64_t ldq, double* dots_host) {
+    if (k < 1 || k > e.kpad || e.kpad != EW || !slots) {
+        set_error("invalid take of %d columns (block engine only)", k);
+        return SAIGE_E_ARG;
+    }
+    for (int j = 0; j < k; j++) {
+        if (slots[j] < 0 || slots[j] >= e.kpad || !e.h_live[slots[j]]) {
+            set_error("slot %d is not a live slot", slots[j]);
+            return SAIGE_E_ARG;
+        }
+        if ((xcols && xcols[j] < 0) || (pcols && pcols[j] < 0)) {
+            set_error("negative column index");
+            return SAIGE_E_ARG;
+        }
+    }
+    if (X_dev && ldx < c->Nnm) {
+        set_error("destination leading dimension %lld < N", (long long)ldx);
+        return SAIGE_E_ARG;
+    }
+    if (int rc = ensure_dots_scratch(c)) return rc;
+    hipStream_t st = c->stream;
+    std::vector<int32_t> lists(3 * EW, 0);
+    for (int j = 0; j < k; j++) {
+        lists[j] = slots[j];
+        lists[EW + j] = xcols ? xcols[j] : j;
+        lists[2 * EW + j] = pcols ? pcols[j] : j;
+    }
+    int32_t* d_slots = c->d_take_lists;
+    SAIGE_HIP_TRY(hipMemcpyAsync(d_slots, lists.data(), sizeof(int32_t) * 3 * EW, hipMemcpyHostToDevice, st));
+    if (int rc = column_dots(c, e.X, 0, e.kpad, d_slots, k, P1, P2, ldp, d_slots + 2 * EW, Q, m, ldq, dots_host))
+        return rc;
+    if (X_dev) SAIGE_HIP_TRY(launch_cols_gather_to(st, e.X, k, d_slots, d_slots + EW, e.kpad, c->Nnm, ldx, X_dev));
+    if (X_host) {
+        SAIGE_HIP_TRY(launch_cols_gather(st, e.X, k, d_slots, e.kpad, c->Nnm, e.stageB));
         SAIGE_HIP_TRY(hipMemcpyAsync(X_host, e.stageB, sizeof(float) * c->Nnm * k, hipMemcpyDeviceToHost, st));
     }
     for (int j = 0; j < k; j++) e.h_live[slots[j]] = 0;
@@ -1431,6 +1525,74 @@ int saige_engine_take_batch(saige_ctx* c, int k, const int32_t* slots, float* X)
         return SAIGE_E_STATE;
     }
     return engine_take_batch(c, c->eng, k, slots, X);
+}
+
+int saige_engine_add_batch_device(saige_ctx* c, int k, const int32_t* slots, const int32_t* systems,
+                                  const float* B_dev, int maxiter, float tol) {
+    CHECK_CTX(c);
+    CHECK_GRM(c);
+    if (!c->eng.X) {
+        set_error("engine not reserved");
+        return SAIGE_E_STATE;
+    }
+    return engine_add_batch(c, c->eng, k, slots, systems, B_dev, maxiter, tol, 0, true);
+}
+
+int saige_engine_add_products_device(saige_ctx* c, int k, const int32_t* slots, const int32_t* systems,
+                                     const float* B_dev) {
+    CHECK_CTX(c);
+    CHECK_GRM(c);
+    if (!c->eng.X) {
+        set_error("engine not reserved");
+        return SAIGE_E_STATE;
+    }
+    return engine_add_batch(c, c->eng, k, slots, systems, B_dev, 1, 0.0f, 1, true);
+}
+
+int saige_engine_take_batch_ex(saige_ctx* c, int k, const int32_t* slots, float* X_host, float* X_dev,
+                               int64_t ldx, const int32_t* xcols, const float* P1_dev, const float* P2_dev,
+                               int64_t ldp, const int32_t* pcols, const float* Q_dev, int m, int64_t ldq,
+                               double* dots_host) {
+    CHECK_CTX(c);
+    if (!c->eng.X) {
+        set_error("engine not reserved");
+        return SAIGE_E_STATE;
+    }
+    return engine_take_batch_ex(c, c->eng, k, slots, X_host, X_dev, ldx, xcols, P1_dev, P2_dev, ldp, pcols, Q_dev, m,
+                                ldq, dots_host);
+}
+
+int saige_dots_device(saige_ctx* c, const float* A_dev, int64_t lda, int k, const float* Q_dev, int64_t ldq, int m,
+                      double* out_host) {
+    CHECK_CTX(c);
+    CHECK_LOADED(c);
+    if (!A_dev || lda < c->Nnm || m < 1) {
+        set_error("invalid dot arguments");
+        return SAIGE_E_ARG;
+    }
+    return column_dots(c, A_dev, lda, 0, nullptr, k, nullptr, nullptr, 0, nullptr, Q_dev, m, ldq, out_host);
+}
+
+int saige_probes_device(saige_ctx* c, float* dst_dev, int64_t ld, uint64_t seed, int64_t i0, int k) {
+    CHECK_CTX(c);
+    CHECK_LOADED(c);
+    if (!dst_dev || ld < c->Nnm || k < 1 || i0 < 0) {
+        set_error("invalid probe request");
+        return SAIGE_E_ARG;
+    }
+    SAIGE_HIP_TRY(launch_rademacher(c->stream, dst_dev, ld, c->Nnm, seed, i0, k));
+    SAIGE_HIP_TRY(hipStreamSynchronize(c->stream));
+    return SAIGE_OK;
+}
+
+int saige_copy_host_to_device(saige_ctx* c, void* dst_dev, const void* src_host, int64_t bytes) {
+    CHECK_CTX(c);
+    if (!dst_dev || !src_host || bytes < 0) {
+        set_error("invalid copy");
+        return SAIGE_E_ARG;
+    }
+    SAIGE_HIP_TRY(hipMemcpy(dst_dev, src_host, (size_t)bytes, hipMemcpyHostToDevice));
+    return SAIGE_OK;
 }
 
 int saige_engine_step(saige_ctx* c, int n_iter, uint8_t* done, int32_t* iters) {

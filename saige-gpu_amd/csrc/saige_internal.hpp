@@ -139,6 +139,10 @@ struct saige_ctx {
     // order, so a column's bits do not depend on the other half), own rt/pd
     saige::GemmConfig gemm_cfg_half;
     double gemm_tuned_ms_half = 0.0;
+    // FP64 column-dot scratch (saige_engine_take_batch_ex, saige_dots_device)
+    double* d_dots_partial = nullptr;
+    double* d_dots_out = nullptr;
+    int32_t* d_take_lists = nullptr;             // [3][EW]: slots, destination columns, partner columns
     int grm_precision = -1;
     double grm_seconds = 0.0;
     // scratch vectors
@@ -295,5 +299,16 @@ hipError_t launch_gather_cols(hipStream_t s, const float* src, int64_t k, int64_
                               float* dst);
 hipError_t launch_fill_zero(hipStream_t s, void* p, int64_t bytes);
 hipError_t launch_fill_hash(hipStream_t s, float* p, int64_t n, uint32_t seed);
+// Rademacher probes i0 .. i0+k-1 into dst (column-major, ld >= N)
+hipError_t launch_rademacher(hipStream_t s, float* dst, int64_t ld, int64_t N, uint64_t seed, int64_t i0, int k);
+// FP64 column dots (see k_col_dots); out[j * nd + d], nd = [P1] + [P2] + m
+int col_dots_max();
+int64_t col_dots_partial_doubles(int64_t Npad, int k);
+// pcols: partner column of each x_j in P1/P2 (nullptr: j)
+hipError_t launch_col_dots(hipStream_t s, const float* X, int64_t ldx, int kpad, const int32_t* slots, int k,
+                           const float* P1, const float* P2, int64_t ldp, const int32_t* pcols, const float* Q,
+                           int m, int64_t ldq, int64_t Nnm, double* partial, double* out);
+hipError_t launch_cols_gather_to(hipStream_t s, const float* src, int k, const int32_t* slots, const int32_t* cols,
+                                 int kpad, int64_t Nnm, int64_t ld, float* dst);
 
 }  // namespace saige

@@ -85,6 +85,13 @@ SIGNATURES = {
     "saige_engine_step": (_int, [_c_p, _int, _c_p, _c_p]),
     "saige_engine_take": (_int, [_c_p, _int, _c_p]),
     "saige_engine_use_loco": (_int, [_c_p, _int]),
+    "saige_engine_add_batch_device": (_int, [_c_p, _int, _c_p, _c_p, _c_p, _int, _f32]),
+    "saige_engine_add_products_device": (_int, [_c_p, _int, _c_p, _c_p, _c_p]),
+    "saige_engine_take_batch_ex": (_int, [_c_p, _int, _c_p, _c_p, _c_p, _i64, _c_p, _c_p, _c_p, _i64, _c_p,
+                                          _c_p, _int, _i64, _c_p]),
+    "saige_dots_device": (_int, [_c_p, _c_p, _i64, _int, _c_p, _i64, _int, _c_p]),
+    "saige_probes_device": (_int, [_c_p, _c_p, _i64, ctypes.c_uint64, _i64, _int]),
+    "saige_copy_host_to_device": (_int, [_c_p, _c_p, _c_p, _i64]),
     "saige_autotune": (_int, [_c_p, _int, _c_p]),
     "saige_autotune_candidates": (_int, [_c_p, _c_p, _c_p]),
     "saige_set_gemm_config": (_int, [_c_p, _int, _int, _int]),

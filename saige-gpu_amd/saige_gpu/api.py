@@ -364,6 +364,63 @@ class Session:
         check(self._L.saige_engine_step(self.ctx, int(n_iter), ptr(done), ptr(iters)))
         return done, iters
 
+    # ------------------------------------------ device-resident columns
+    # Device buffers are column-major with leading dimension N (Nnomissing);
+    # (buf, col) addresses column col of buffer buf.
+    def _col_ptr(self, buf, col=0):
+        if buf is None:
+            return None
+        return ctypes.c_void_p(buf.value + 4 * int(col) * self.getNnomissingOut())
+
+    def engine_add_batch_device(self, slots, systems, buf, col0, maxiter, tol):
+        """Start solves whose right-hand sides are columns col0.. of buf."""
+        sl = np.ascontiguousarray(slots, dtype=np.int32)
+        sy = np.ascontiguousarray(systems, dtype=np.int32)
+        check(self._L.saige_engine_add_batch_device(self.ctx, len(sl), ptr(sl), ptr(sy), self._col_ptr(buf, col0),
+                                                    int(maxiter), float(tol)))
+
+    def engine_add_products_device(self, slots, systems, buf, col0):
+        sl = np.ascontiguousarray(slots, dtype=np.int32)
+        sy = np.ascontiguousarray(systems, dtype=np.int32)
+        check(self._L.saige_engine_add_products_device(self.ctx, len(sl), ptr(sl), ptr(sy),
+                                                       self._col_ptr(buf, col0)))
+
+    def engine_take_batch_ex(self, slots, download=False, X_dev=None, xcols=None, P1=None, P2=None, pcols=None,
+                             Q=None, m=0):
+        """Take finished slots; optionally download them (N x k), store them in
+        columns xcols of device buffer X_dev, and return FP64 dots with
+        P1[:, pcols[j]], P2[:, pcols[j]] and the m columns of Q (k x nd)."""
+        sl = np.ascontiguousarray(slots, dtype=np.int32)
+        k = len(sl)
+        N = self.getNnomissingOut()
+        X = np.zeros((N, k), np.float32, order="F") if download else None
+        xc = None if xcols is None else np.ascontiguousarray(xcols, dtype=np.int32)
+        pc = None if pcols is None else np.ascontiguousarray(pcols, dtype=np.int32)
+        nd = (P1 is not None) + (P2 is not None) + int(m)
+        dots = np.zeros((k, max(nd, 1)), np.float64)
+        check(self._L.saige_engine_take_batch_ex(self.ctx, k, ptr(sl), ptr(X), X_dev, N, ptr(xc), P1, P2, N,
+                                                 ptr(pc), Q, int(m), N, ptr(dots) if nd else None))
+        return X, dots[:, :nd]
+
+    def dots_device(self, A, k, Q, m):
+        """FP64 dots of the first k columns of device buffer A with the m
+        columns of Q: k x m."""
+        out = np.zeros((k, m), np.float64)
+        N = self.getNnomissingOut()
+        check(self._L.saige_dots_device(self.ctx, A, N, int(k), Q, N, int(m), ptr(out)))
+        return out
+
+    def probes_device(self, buf, col0, seed, i0, k):
+        """Rademacher probes i0 .. i0+k-1 into columns col0.. of buf."""
+        check(self._L.saige_probes_device(self.ctx, self._col_ptr(buf, col0), self.getNnomissingOut(), int(seed),
+                                          int(i0), int(k)))
+
+    def copy_host_to_device(self, buf, arr, col0=0):
+        """Copy a host FP32 array (N or N x k, column-major) to columns col0.."""
+        a = np.asarray(arr, np.float32)
+        a = np.asfortranarray(a.reshape(a.shape[0], -1))
+        check(self._L.saige_copy_host_to_device(self.ctx, self._col_ptr(buf, col0), ptr(a), a.nbytes))
+
     def engine_take(self, slot):
         x = np.zeros(self.getNnomissingOut(), np.float32)
         check(self._L.saige_engine_take(self.ctx, int(slot), ptr(x)))

@@ -111,13 +111,35 @@ def glm_start(y, X, family, offset=None, maxit=25, epsilon=1e-8):
 # ------------------------------------------------------------------ probes
 
 
+_M1, _M2 = np.uint64(0xBF58476D1CE4E5B9), np.uint64(0x94D049BB133111EB)
+_C1, _C2, _C3 = np.uint64(0x9E3779B97F4A7C15), np.uint64(0xC2B2AE3D27D4EB4F), np.uint64(0x5851F42D)
+
+
+def _mix64(z):
+    z = (z ^ (z >> np.uint64(30))) * _M1
+    z = (z ^ (z >> np.uint64(27))) * _M2
+    return z ^ (z >> np.uint64(31))
+
+
+def probe_vector(seed, i, n):
+    """Probe i: entry r is -1 if the top bit of
+    mix(seed ^ mix(i*C1 + mix(r*C2 + C3))) is set, else +1 (64-bit wrapping
+    arithmetic; the same hash the device's saige_probes_device evaluates)."""
+    r = np.arange(n, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        inner = _mix64(r * _C2 + _C3)
+        h = _mix64(np.uint64(seed) ^ _mix64(np.uint64(i) * _C1 + inner))
+    return np.where((h >> np.uint64(63)) != 0, -1.0, 1.0)
+
+
 class RademacherProbes:
     """Deterministic Rademacher probe vectors u_i in {-1, +1}^n.
 
     The reference draws u = 2*rbinom(n, 1, 0.5) - 1 from R's RNG after
     comm.set.seed(200) at every GetTrace call (FG.cpp:3114, 3133-3137), so
     every AI iteration reuses the same probes; here probe i is a fixed
-    function of (seed, i) with the same property."""
+    function of (seed, i) with the same property (probe_vector), which the
+    device can also generate in place."""
 
     _shared = {}
 
@@ -140,11 +162,7 @@ class RademacherProbes:
     def __call__(self, i):
         v = self._cache.get(i)
         if v is None:
-            # one random byte per 8 entries (Philox keyed by (seed, i))
-            rng = np.random.Generator(np.random.Philox(key=[self.seed, i]))
-            bits = np.unpackbits(np.frombuffer(rng.bytes((self.n + 7) // 8), np.uint8))[: self.n]
-            v = bits.astype(np.float64) * 2.0 - 1.0
-            self._cache[i] = v
+            v = self._cache[i] = probe_vector(self.seed, i, self.n)
         return v
 
     def block(self, start, end):
@@ -247,8 +265,13 @@ def ai_score_co(Y, X, w, tau, Sigma_iY, Sigma_iX, cov, probes, opt, quantitative
     """getAIScore (FG.cpp:3279-3295) / getAIScore_q (FG.cpp:3479-3531) with
     GetTrace / GetTrace_q (FG.cpp:3113-3160, 3409-3472).
 
-    The probe solves, their K u products and the PAPY (and PA0PY) solves are
-    independent, so each goes out as one request of nrun+1 (+1) columns."""
+    Each round of probes is one "trace" request (see trace_round_host): the
+    executor returns the FP64 scalars the estimator needs, and with PY also
+    K PY and the PAPY (and PA0PY) solves.  With P u = Sigma^-1 u -
+    Sigma^-1 X cov (Sigma^-1 X)^T u, the probe terms are
+      (K u)^T P u = d - h^T cov g,   u^T P u = e - g^T cov g,
+    d = (K u)^T Sigma^-1 u, e = u^T Sigma^-1 u, h = (Sigma^-1 X)^T K u,
+    g = (Sigma^-1 X)^T u."""
     if quantitative:
         cov = _inv_sympd(X.T @ Sigma_iX)   # getAIScore_q recomputes cov (FG.cpp:3487-3494)
     PY = Sigma_iY - Sigma_iX @ (cov @ (Sigma_iX.T @ Y))
@@ -257,24 +280,13 @@ def ai_score_co(Y, X, w, tau, Sigma_iY, Sigma_iX, cov, probes, opt, quantitative
     APY = None
     extra = None
     while True:
-        U = probes.block(start, end) if hasattr(probes, "block") else \
-            np.column_stack([probes(i) for i in range(start, end)])
+        r = yield ("trace", w, tau, probes, start, end, Sigma_iX, quantitative, PY if APY is None else None)
         if APY is None:
-            KB = yield ("matvec", cols(PY, U))
-            APY = KB[:, 0]
-            AU = KB[:, 1:]
-            rhs = [U, APY] + ([PY] if quantitative else [])
-            sol = yield ("solve", w, tau, cols(*rhs))
-            k = U.shape[1]
-            Sigma_iU = sol[:, :k]
-            extra = sol[:, k:]
-        else:
-            AU = yield ("matvec", U)
-            Sigma_iU = yield ("solve", w, tau, U)
-        PU = _proj(U, Sigma_iU, Sigma_iX, cov)
-        t1.extend(np.einsum("ij,ij->j", AU, PU).tolist())
+            APY, extra = r["APY"], r["extra"]
+        hcg = np.einsum("jp,pq,jq->j", r["h"], cov, r["g"])
+        t1.extend((r["d"] - hcg).tolist())
         if quantitative:
-            t0.extend(np.einsum("ij,ij->j", U, PU).tolist())
+            t0.extend((r["e"] - np.einsum("jp,pq,jq->j", r["g"], cov, r["g"])).tolist())
         cv = cal_cv(t1)
         cv0 = cal_cv(t0) if quantitative else 0.0
         if cv > opt.traceCVcutoff or (quantitative and cv0 > opt.traceCVcutoff):
@@ -294,6 +306,50 @@ def ai_score_co(Y, X, w, tau, Sigma_iY, Sigma_iX, cov, probes, opt, quantitative
         out["Trace"] = np.array([np.mean(t0), np.mean(t1)])
         out["AI"] = np.array([[A0PY @ PA0PY, A0PY @ PAPY], [A0PY @ PAPY, APY @ PAPY]])
     return out
+
+
+def probe_block(probes, start, end):
+    if hasattr(probes, "block"):
+        return probes.block(start, end)
+    return np.column_stack([probes(i) for i in range(start, end)])
+
+
+def trace_round_co(req):
+    """A "trace" request as ordinary products and solves: K [PY, U], then
+    Sigma^-1 [U, APY, (PY)], and the scalars d, e, h, g in FP64 on the host.
+    Executors without device-resident trace rounds run it in place of the
+    request."""
+    _, w, tau, probes, start, end, SX, want_t0, PY = req
+    U = probe_block(probes, start, end)
+    k = U.shape[1]
+    out = {}
+    if PY is not None:
+        KB = yield ("matvec", cols(PY, U))
+        APY, AU = KB[:, 0], KB[:, 1:]
+        sol = yield ("solve", w, tau, cols(*([U, APY] + ([PY] if want_t0 else []))))
+        SU = sol[:, :k]
+        out["APY"], out["extra"] = APY, sol[:, k:]
+    else:
+        AU = yield ("matvec", U)
+        SU = yield ("solve", w, tau, U)
+    out["d"] = np.array([np.dot(AU[:, j], SU[:, j]) for j in range(k)])
+    out["h"] = AU.T @ SX
+    out["g"] = U.T @ SX
+    if want_t0:
+        out["e"] = np.array([np.dot(U[:, j], SU[:, j]) for j in range(k)])
+    return out
+
+
+def trace_round_host(req, matvec, solve):
+    """Run trace_round_co with a product and a solve function."""
+    gen = trace_round_co(req)
+    value = None
+    while True:
+        try:
+            r = gen.send(value)
+        except StopIteration as e:
+            return e.value
+        value = matvec(r[1]) if r[0] == "matvec" else solve(*r[1:])
 
 
 def fitglmmai_rpcg_co(Y, X, w, tau, Sigma_iY, Sigma_iX, cov, probes, opt):
@@ -590,6 +646,21 @@ class PlainExecutor:
         self._pcg = sess.getPCG1ofSigmaAndVector_LOCO if loco else sess.getPCG1ofSigmaAndVector
         self._kin = sess.getCrossprodMatAndKin_LOCO if loco else sess.getCrossprodMatAndKin
 
+    def _solve(self, w, tau, B):
+        tau32 = np.asarray(tau, np.float32)
+        out = np.zeros(B.shape, np.float64, order="F")   # the scheduler's result layout
+        for j in range(B.shape[1]):
+            x, it = self._pcg(w, tau32, B[:, j], self.opt.maxiterPCG, self.opt.tolPCG, return_iters=True)
+            out[:, j] = x
+            self.n_solves += 1
+            self.pcg_iters += it
+        return out
+
+    def _matvec(self, B):
+        self.n_products += B.shape[1]
+        return np.asfortranarray(np.column_stack([self._kin(B[:, j]) for j in range(B.shape[1])]),
+                                 dtype=np.float64)
+
     def run(self, gen):
         value = None
         while True:
@@ -598,26 +669,17 @@ class PlainExecutor:
             except StopIteration as e:
                 return e.value
             if req[0] == "solve":
-                _, w, tau, B = req
-                tau32 = np.asarray(tau, np.float32)
-                out = np.zeros(B.shape, np.float64, order="F")   # the scheduler's result layout
-                for j in range(B.shape[1]):
-                    x, it = self._pcg(w, tau32, B[:, j], self.opt.maxiterPCG, self.opt.tolPCG,
-                                      return_iters=True)
-                    out[:, j] = x
-                    self.n_solves += 1
-                    self.pcg_iters += it
-                value = out
+                value = self._solve(*req[1:])
+            elif req[0] == "matvec":
+                value = self._matvec(req[1])
             else:
-                _, B = req
-                value = np.asfortranarray(np.column_stack([self._kin(B[:, j]) for j in range(B.shape[1])]),
-                                          dtype=np.float64)
-                self.n_products += B.shape[1]
+                value = trace_round_host(req, self._matvec, self._solve)
 
 
 class _Fit:
     __slots__ = ("fid", "gen", "subset", "n", "sys", "pending", "result_buf", "remaining", "result",
-                 "error", "t_submit", "t_admit", "t_done", "tag", "iters", "sys_ready", "sys_pending")
+                 "error", "t_submit", "t_admit", "t_done", "tag", "iters", "sys_ready", "sys_pending",
+                 "trace", "sub")
 
     def __init__(self, fid, gen, subset, tag):
         self.fid = fid
@@ -636,6 +698,8 @@ class _Fit:
         self.iters = 0
         self.sys_ready = False
         self.sys_pending = None
+        self.trace = None
+        self.sub = None          # a trace round run as products and solves
 
 
 class BatchScheduler:
@@ -653,7 +717,17 @@ class BatchScheduler:
     the still-running columns is launched from a worker thread (the library
     call releases the GIL) while the main thread resumes the finished fits'
     coroutines.  Nothing touches the library during that time; a resumed
-    fit's new system and columns are applied once the iteration is back."""
+    fit's new system and columns are applied once the iteration is back.
+
+    Trace rounds stay on the device: the probes are generated (or uploaded)
+    into a per-system device buffer, K u goes from its product slot to a
+    second buffer with h = (Sigma^-1 X)^T K u, and the Sigma^-1 u solves are
+    never downloaded: their slots return d = (K u)^T Sigma^-1 u (and
+    e = u^T Sigma^-1 u) as FP64 dots.  Only PY, APY and the PAPY/PA0PY
+    solves travel as vectors."""
+
+    TRACE_MAX_PROBES = 64      # probe columns per round held on the device
+    TRACE_MAX_COVARIATES = 14  # shared dot partners per take (16 - 2 pairwise)
 
     def __init__(self, sess, opt, width=64, n_systems=16, hbm_reserve_bytes=8 << 30, tune_malloc=True,
                  host_threads=1):
@@ -691,6 +765,8 @@ class BatchScheduler:
         self._pool = concurrent.futures.ThreadPoolExecutor(max_workers=1)
         self._inflight = None                   # engine_step running in the worker
         self.n_overlapped = 0                   # iterations that ran during host work
+        self._devbufs = {}                      # system -> device buffers of its trace rounds
+        self.device_trace = hasattr(sess, "engine_take_batch_ex")
 
     # ---------------------------------------------------------------- API
     def submit(self, gen, subset=None, tag=None, t_submit=None):
@@ -702,6 +778,13 @@ class BatchScheduler:
 
     def idle(self):
         return not self.waiting and not self.active
+
+    def close(self):
+        """Free the device buffers of the trace rounds."""
+        for bufs in self._devbufs.values():
+            for b in bufs.values():
+                self.sess.free_device(b)
+        self._devbufs = {}
 
     def run(self, arrivals=None):
         """Run until every submitted fit finished.  arrivals: optional list of
@@ -750,13 +833,26 @@ class BatchScheduler:
     def _compact(self, f, v):
         return v if f.subset is None else v[f.subset]
 
-    @staticmethod
-    def _send(f, value):
+    def _send(self, f, value):
         """Resume a fit's coroutine: ("req", request) | ("done", result) |
-        ("error", exception).  Touches nothing but the fit's own generator,
-        so several fits may be resumed on worker threads at once."""
+        ("error", exception).  Touches nothing but the fit's own generators,
+        so several fits may be resumed on worker threads at once.  Without
+        device-resident trace rounds a "trace" request runs as
+        trace_round_co's products and solves."""
         try:
-            return ("req", f.gen.send(value))
+            while True:
+                if f.sub is not None:
+                    try:
+                        req = f.sub.send(value)
+                    except StopIteration as e:
+                        f.sub, value = None, e.value
+                        continue
+                else:
+                    req = f.gen.send(value)
+                if req[0] == "trace" and not self.device_trace:
+                    f.sub, value = trace_round_co(req), None
+                    continue
+                return ("req", req)
         except StopIteration as e:
             return ("done", e.value)
         except Exception as e:   # a failing fit does not stop the others
@@ -773,6 +869,9 @@ class BatchScheduler:
             self._finish(f)
             return
         req = v
+        if req[0] == "trace":
+            self._trace_start(f, req)
+            return
         if req[0] == "matvec":
             B = req[1]
             if not f.sys_ready:          # products only need the system's sample mask
@@ -787,7 +886,7 @@ class BatchScheduler:
         f.result_buf = np.zeros(B.shape, np.float64, order="F")
         f.remaining = B.shape[1]
         for j in range(B.shape[1]):
-            f.pending.append((j, B[:, j], kind))
+            f.pending.append((j, B[:, j], kind, ("host", j)))
 
     def _advance(self, items):
         """Resume fits [(fit, value), ...] and queue their requests in order.
@@ -801,6 +900,91 @@ class BatchScheduler:
             outs = [self._send(f, v) for f, v in items]
         for (f, _), out in zip(items, outs):
             self._apply(f, out)
+
+    # ------------------------------------------------------- trace rounds
+    def _buffers(self, sys):
+        bufs = self._devbufs.get(sys)
+        if bufs is None:
+            n = self.N * 4
+            bufs = self._devbufs[sys] = {
+                "U": self.sess.alloc_device(n * self.TRACE_MAX_PROBES),
+                "AU": self.sess.alloc_device(n * self.TRACE_MAX_PROBES),
+                "SX": self.sess.alloc_device(n * self.TRACE_MAX_COVARIATES),
+            }
+        return bufs
+
+    def _trace_start(self, f, req):
+        """Phase 1 of a trace round: probes into the device buffer, products
+        K [PY, U]; K u is kept on the device."""
+        _, w, tau, probes, start, end, SX, want_t0, PY = req
+        k, p = end - start, SX.shape[1]
+        if k > self.TRACE_MAX_PROBES or p > self.TRACE_MAX_COVARIATES:
+            raise ValueError(f"trace round of {k} probes x {p} covariates exceeds the device buffers")
+        # device work (buffers, probes, uploads) waits for _fill: this may run
+        # while an iteration is in flight, when nothing may touch the library
+        f.trace = {"phase": 1, "w": w, "tau": tau, "k": k, "p": p, "want_t0": want_t0, "PY": PY,
+                   "probes": probes, "start": start, "end": end, "SX": SX, "setup": True,
+                   "h": np.zeros((k, p)), "d": np.zeros(k), "e": np.zeros(k), "bufs": None}
+        if not f.sys_ready:              # products only need the system's sample mask
+            f.sys_pending = (np.ones(len(PY) if PY is not None else self._n_of(f)), (1.0, 0.0))
+            f.sys_ready = True
+        n_host = 1 if PY is not None else 0
+        f.result_buf = np.zeros((self._n_of(f), n_host), np.float64, order="F")
+        f.remaining = n_host + k
+        if PY is not None:
+            f.pending.append((0, PY, "product", ("host", 0)))
+        for j in range(k):
+            f.pending.append((j, ("dev", "U", j), "product", ("stash", j)))
+
+    def _trace_setup(self, f):
+        """Device side of phase 1 (called from _fill, between iterations):
+        probes into the system's U buffer (generated in place, or uploaded for
+        a sample subset or other probe sources), Sigma^-1 X into SX, and
+        g = (Sigma^-1 X)^T u."""
+        tr = f.trace
+        tr["setup"] = False
+        bufs = tr["bufs"] = self._buffers(f.sys)
+        k, p, probes = tr["k"], tr["p"], tr["probes"]
+        sub = f.subset if f.subset is not None else slice(None)
+        if f.subset is None and isinstance(probes, RademacherProbes) and probes.n == self.N:
+            self.sess.probes_device(bufs["U"], 0, probes.seed, tr["start"], k)
+        else:
+            Uf = np.zeros((self.N, k), np.float32, order="F")
+            Uf[sub] = probe_block(probes, tr["start"], tr["end"])
+            self.sess.copy_host_to_device(bufs["U"], Uf)
+        SXf = np.zeros((self.N, p), np.float32, order="F")
+        SXf[sub] = tr["SX"]
+        self.sess.copy_host_to_device(bufs["SX"], SXf)
+        tr["g"] = self.sess.dots_device(bufs["U"], k, bufs["SX"], p)
+
+    def _n_of(self, f):
+        return self.N if f.subset is None else len(f.subset)
+
+    def _trace_phase2(self, f):
+        """Phase 2: solves Sigma^-1 [U, APY, (PY)]; the U solves return only dots."""
+        tr = f.trace
+        tr["phase"] = 2
+        if tr["PY"] is not None:
+            tr["APY"] = f.result_buf[:, 0].copy()
+        f.sys_pending = (tr["w"], tr["tau"])
+        f.sys_ready = True
+        extra = [] if tr["PY"] is None else [tr["APY"]] + ([tr["PY"]] if tr["want_t0"] else [])
+        f.result_buf = np.zeros((self._n_of(f), len(extra)), np.float64, order="F")
+        f.remaining = tr["k"] + len(extra)
+        for j in range(tr["k"]):
+            f.pending.append((j, ("dev", "U", j), "solve", ("pair", j)))
+        for j, v in enumerate(extra):
+            f.pending.append((j, v, "solve", ("host", j)))
+
+    def _trace_result(self, f):
+        tr = f.trace
+        f.trace = None
+        out = {"d": tr["d"], "h": tr["h"], "g": tr["g"]}
+        if tr["want_t0"]:
+            out["e"] = tr["e"]
+        if tr["PY"] is not None:
+            out["APY"], out["extra"] = tr["APY"], f.result_buf
+        return out
 
     def _set_system(self, f, w, tau):
         mask = None
@@ -821,26 +1005,43 @@ class BatchScheduler:
         self.finished.append(f)
 
     def _fill(self):
-        groups = {"solve": ([], [], [], []), "product": ([], [], [], [])}
+        host = {"solve": ([], [], [], []), "product": ([], [], [], [])}
+        dev = []                          # runs of consecutive device columns: [kind, buf, col0, slots, systems, owners]
         for f in self.active:
             if f.sys_pending is not None and f.pending and self.free_slots:
                 w, tau = f.sys_pending
                 f.sys_pending = None
                 self._set_system(f, w, tau)
+            if f.trace is not None and f.trace["setup"] and f.pending and self.free_slots:
+                t = time.perf_counter()
+                self._trace_setup(f)
+                self.timing["trace_setup"] += time.perf_counter() - t
             while f.pending and self.free_slots:
-                j, b, kind = f.pending.popleft()
+                j, src, kind, action = f.pending.popleft()
                 slot = heapq.heappop(self.free_slots)
-                g = groups[kind]
-                g[0].append(slot)
-                g[1].append(f.sys)
-                g[2].append((f, b))
-                g[3].append((f, j))
+                if isinstance(src, tuple):               # ("dev", buffer name, column)
+                    _, name, col = src
+                    buf = f.trace["bufs"][name]
+                    run = dev[-1] if dev else None
+                    if (run is None or run[0] != kind or run[1] is not buf or
+                            run[2] + len(run[3]) != col or run[4][0] != f.sys):
+                        run = [kind, buf, col, [], [], []]
+                        dev.append(run)
+                    run[3].append(slot)
+                    run[4].append(f.sys)
+                    run[5].append((f, j, action))
+                else:
+                    g = host[kind]
+                    g[0].append(slot)
+                    g[1].append(f.sys)
+                    g[2].append((f, src))
+                    g[3].append((f, j, action))
             if not self.free_slots:
                 break
-        for kind, (slots, systems, cols, owners) in groups.items():
+        t = time.perf_counter()
+        for kind, (slots, systems, cols, owners) in host.items():
             if not slots:
                 continue
-            t = time.perf_counter()
             B = np.zeros((self.N, len(cols)), np.float32, order="F")   # column-major, as the ABI takes it
             for c, (f, b) in enumerate(cols):
                 if f.subset is None:
@@ -853,7 +1054,14 @@ class BatchScheduler:
                 self.sess.engine_add_products(slots, systems, B)
             for slot, own in zip(slots, owners):
                 self.slot_owner[slot] = own
-            self.timing["add"] += time.perf_counter() - t
+        for kind, buf, col0, slots, systems, owners in dev:
+            if kind == "solve":
+                self.sess.engine_add_batch_device(slots, systems, buf, col0, self.opt.maxiterPCG, self.opt.tolPCG)
+            else:
+                self.sess.engine_add_products_device(slots, systems, buf, col0)
+            for slot, own in zip(slots, owners):
+                self.slot_owner[slot] = own
+        self.timing["add"] += time.perf_counter() - t
 
     def _step(self):
         t = time.perf_counter()
@@ -869,33 +1077,56 @@ class BatchScheduler:
         finished = [slot for slot in self.slot_owner if done[slot]]
         if not finished:
             return
-        t = time.perf_counter()
-        X = self.sess.engine_take_batch(finished)
-        self.timing["take"] += time.perf_counter() - t
-        resume = []
-        by_fit = {}
-        for c, slot in enumerate(finished):
-            f, j = self.slot_owner.pop(slot)
+        host, dev = [], {}
+        for slot in finished:
+            f, j, action = self.slot_owner.pop(slot)
             heapq.heappush(self.free_slots, slot)
-            js, cs = by_fit.setdefault(f.fid, (f, [], []))[1:]
-            js.append(j)
-            cs.append(c)
+            if action[0] == "host":
+                host.append((slot, f, j))
+            else:
+                dev.setdefault((f.fid, action[0]), (f, action[0], []))[2].append((slot, action[1]))
             f.iters += int(iters[slot])
             f.remaining -= 1
             self.n_solves += 1
+        t = time.perf_counter()
+        X = self.sess.engine_take_batch([s for s, _, _ in host]) if host else None
+        for f, what, items in dev.values():          # trace columns: results stay on the device
+            tr = f.trace
+            slots = [s for s, _ in items]
+            colsj = [c for _, c in items]
+            if what == "stash":                       # K u -> AU buffer, h = (Sigma^-1 X)^T K u
+                _, dots = self.sess.engine_take_batch_ex(slots, X_dev=tr["bufs"]["AU"], xcols=colsj,
+                                                         Q=tr["bufs"]["SX"], m=tr["p"])
+                tr["h"][colsj] = dots
+            else:                                     # Sigma^-1 u: d (and e) only
+                _, dots = self.sess.engine_take_batch_ex(
+                    slots, P1=tr["bufs"]["AU"], P2=tr["bufs"]["U"] if tr["want_t0"] else None, pcols=colsj)
+                tr["d"][colsj] = dots[:, 0]
+                if tr["want_t0"]:
+                    tr["e"][colsj] = dots[:, 1]
+        self.timing["take"] += time.perf_counter() - t
+        touched = {f.fid: f for _, f, _ in host}
+        touched.update({f.fid: f for f, _, _ in dev.values()})
+        resume = []
+        for f in touched.values():
             if f.remaining == 0 and not f.pending:
+                if f.trace is not None and f.trace["phase"] == 1:
+                    continue                          # phase 2 is queued below, not a resume
                 resume.append(f)
-        if resume and self.slot_owner:
+        phase2 = [f for f in touched.values()
+                  if f.remaining == 0 and not f.pending and f.trace is not None and f.trace["phase"] == 1]
+        if (resume or phase2) and self.slot_owner:
             # the columns still running take their next iteration meanwhile
             self._inflight = self._pool.submit(self.sess.engine_step, 1, self.width)
             self.n_overlapped += 1
         t = time.perf_counter()
-        for f, js, cs in by_fit.values():
-            for j, c in zip(js, cs):            # contiguous column copies (F-order blocks)
-                f.result_buf[:, j] = X[:, c] if f.subset is None else X[f.subset, c]
+        for c, (slot, f, j) in enumerate(host):       # contiguous column copies (F-order blocks)
+            f.result_buf[:, j] = X[:, c] if f.subset is None else X[f.subset, c]
         self.timing["copy"] += time.perf_counter() - t
+        for f in phase2:
+            self._trace_phase2(f)
         t = time.perf_counter()
-        self._advance([(f, f.result_buf) for f in resume])
+        self._advance([(f, self._trace_result(f) if f.trace is not None else f.result_buf) for f in resume])
         self.timing["host"] += time.perf_counter() - t
 
 
@@ -982,5 +1213,8 @@ def fit_batched(sess, jobs, opt=None, width=64, n_systems=16, probe_seed=200):
         probes = RademacherProbes.shared(n, probe_seed)
         gen = fit_co(j["trait"], j["y"], j["X"], probes, opt)
         handles.append(sch.submit(gen, j.get("subset"), j.get("tag")))
-    sch.run()
+    try:
+        sch.run()
+    finally:
+        sch.close()
     return handles, sch

@@ -26,11 +26,15 @@ class DenseExecutor:
             except StopIteration as e:
                 return e.value
             if req[0] == "solve":
-                _, w, tau, B = req
-                value, _ = self.grm.pcg(np.asarray(w), np.asarray(tau, np.float64), B, self.opt.maxiterPCG,
-                                        self.opt.tolPCG)
-            else:
+                value = self._solve(*req[1:])
+            elif req[0] == "matvec":
                 value = self.grm.K @ req[1]
+            else:
+                value = D.trace_round_host(req, lambda B: self.grm.K @ B, self._solve)
+
+    def _solve(self, w, tau, B):
+        X, _ = self.grm.pcg(np.asarray(w), np.asarray(tau, np.float64), B, self.opt.maxiterPCG, self.opt.tolPCG)
+        return X
 
 
 @pytest.fixture(scope="module")

@@ -117,6 +117,65 @@ class FakeEngine:
         return X
 
 
+class _DevBuf:
+    """A stand-in device buffer: a column-major FP32 array with an address."""
+    _next = 1
+
+    def __init__(self, nbytes, N):
+        self.value = _DevBuf._next
+        _DevBuf._next += 1
+        self.a = np.zeros((N, max(1, nbytes // (4 * N))), np.float32, order="F")
+
+
+class FakeDeviceEngine(FakeEngine):
+    """FakeEngine plus the device-resident trace calls of saige_gpu.api.Session
+    (buffers, probes generated from the probe hash, FP64 column dots)."""
+
+    def alloc_device(self, nbytes):
+        self.calls["alloc"] += 1
+        return _DevBuf(nbytes, self.N)
+
+    def free_device(self, buf):
+        self.calls["free"] += 1
+
+    def copy_host_to_device(self, buf, arr, col0=0):
+        a = np.asarray(arr, np.float32).reshape(self.N, -1)
+        buf.a[:, col0:col0 + a.shape[1]] = a
+
+    def probes_device(self, buf, col0, seed, i0, k):
+        for j in range(k):
+            buf.a[:, col0 + j] = D.probe_vector(seed, i0 + j, self.N)
+
+    def dots_device(self, A, k, Q, m):
+        a, q = A.a.astype(np.float64), Q.a.astype(np.float64)
+        return np.array([[np.dot(a[:, j], q[:, r]) for r in range(m)] for j in range(k)])
+
+    def engine_add_batch_device(self, slots, systems, buf, col0, maxiter, tol):
+        self._add(slots, systems, buf.a[:, col0:col0 + len(slots)], "solve", maxiter, tol)
+
+    def engine_add_products_device(self, slots, systems, buf, col0):
+        self._add(slots, systems, buf.a[:, col0:col0 + len(slots)], "product")
+
+    def engine_take_batch_ex(self, slots, download=False, X_dev=None, xcols=None, P1=None, P2=None, pcols=None,
+                             Q=None, m=0):
+        self.calls["take_ex"] += 1
+        X = self.engine_take_batch(slots)
+        xf = X.astype(np.float64)
+        dots = []
+        for j in range(len(slots)):
+            row = []
+            pc = pcols[j] if pcols is not None else j
+            for P in (P1, P2):
+                if P is not None:
+                    row.append(np.dot(xf[:, j], P.a[:, pc].astype(np.float64)))
+            for r in range(m):
+                row.append(np.dot(xf[:, j], Q.a[:, r].astype(np.float64)))
+            dots.append(row)
+            if X_dev is not None:
+                X_dev.a[:, xcols[j] if xcols is not None else j] = X[:, j]
+        return (X if download else None), np.array(dots).reshape(len(slots), -1)
+
+
 class OneColumnExecutor:
     """Runs a coroutine with every column alone on the engine (reference)."""
 
@@ -155,12 +214,21 @@ class OneColumnExecutor:
                 return e.value
             if req[0] == "solve":
                 _, w, tau, B = req
-                value = np.column_stack([self._col(w, tau, B[:, j], "solve") for j in range(B.shape[1])])
+                value = self._solve(w, tau, B)
+            elif req[0] == "matvec":
+                value = self._matvec(req[1], w)
             else:
-                B = req[1]
-                ww = w if w is not None else np.ones(B.shape[0])
-                value = np.column_stack([self._col(ww, (1.0, 0.0), B[:, j], "product") for j in range(B.shape[1])])
-            value = np.asfortranarray(value, dtype=np.float64)   # the scheduler's result layout
+                w, tau = req[1], req[2]
+                value = D.trace_round_host(req, lambda B: self._matvec(B, w), self._solve)
+
+    def _solve(self, w, tau, B):
+        v = np.column_stack([self._col(w, tau, B[:, j], "solve") for j in range(B.shape[1])])
+        return np.asfortranarray(v, dtype=np.float64)   # the scheduler's result layout
+
+    def _matvec(self, B, w):
+        ww = w if w is not None else np.ones(B.shape[0])
+        v = np.column_stack([self._col(ww, (1.0, 0.0), B[:, j], "product") for j in range(B.shape[1])])
+        return np.asfortranarray(v, dtype=np.float64)
 
 
 @pytest.fixture(scope="module")
@@ -188,10 +256,13 @@ def cohort():
 
 
 def test_scheduler_equals_one_column_at_a_time(cohort):
+    """Without device-resident trace rounds (products and solves only) the
+    scheduler's results are bitwise those of one column at a time."""
     K, kdiag, jobs = cohort
     opt = D.Options(nrun=6, traceCVcutoff=1.0, maxiter=6, tolPCG=1e-8, maxiterPCG=300)
     eng = FakeEngine(K, kdiag)
     handles, sch = D.fit_batched(eng, jobs, opt, width=64, n_systems=3)
+    assert not sch.device_trace
     assert all(h.error is None for h in handles)
     # columns of several fits shared iterations, and host work overlapped an iteration
     assert sch.n_iterations < sum(h.iters for h in handles)
@@ -204,6 +275,39 @@ def test_scheduler_equals_one_column_at_a_time(cohort):
         assert np.array_equal(h.result["coefficients"], ref["coefficients"]), j["tag"]
         assert np.array_equal(h.result["linear_predictors"], ref["linear_predictors"]), j["tag"]
         assert h.result["iterations"] == ref["iterations"]
+
+
+def test_device_trace_rounds_match(cohort):
+    """With device-resident trace rounds (probes generated in place, K u
+    stashed, the Sigma^-1 u solves returning only FP64 dots) every fit matches
+    the one-column run: the same iterations, tau to FP64 rounding (the
+    device keeps Sigma^-1 X in FP32 for the h and g dots)."""
+    K, kdiag, jobs = cohort
+    opt = D.Options(nrun=6, traceCVcutoff=1.0, maxiter=6, tolPCG=1e-8, maxiterPCG=300)
+    eng = FakeDeviceEngine(K, kdiag)
+    handles, sch = D.fit_batched(eng, jobs, opt, width=64, n_systems=3)
+    assert sch.device_trace
+    assert all(h.error is None for h in handles), [h.error for h in handles]
+    assert eng.calls["take_ex"] > 0 and eng.calls["alloc"] == eng.calls["free"] > 0
+    for j, h in zip(jobs, handles):
+        probes = D.RademacherProbes(len(j["y"]), 200)
+        ref = OneColumnExecutor(FakeEngine(K, kdiag), opt, j.get("subset")).run(
+            D.fit_co(j["trait"], j["y"], j["X"], probes, opt))
+        assert h.result["iterations"] == ref["iterations"], j["tag"]
+        assert np.allclose(h.result["theta"], ref["theta"], rtol=1e-6, atol=1e-9), j["tag"]
+        assert np.allclose(h.result["coefficients"], ref["coefficients"], rtol=1e-6, atol=1e-9), j["tag"]
+
+
+def test_probe_hash_is_a_fair_sign():
+    """probe_vector: entries +-1, half of each over many rows, independent
+    across probes and seeds, fixed for a (seed, i)."""
+    a = D.probe_vector(200, 0, 200_000)
+    b = D.probe_vector(200, 1, 200_000)
+    c = D.probe_vector(201, 0, 200_000)
+    assert set(np.unique(a)) == {-1.0, 1.0}
+    assert abs(a.mean()) < 0.01 and abs((a * b).mean()) < 0.01 and abs((a * c).mean()) < 0.01
+    assert np.array_equal(a, D.probe_vector(200, 0, 200_000))
+    assert np.array_equal(a[:1000], D.probe_vector(200, 0, 1000))
 
 
 def test_scheduler_packs_low_slots_and_caps_systems(cohort):
