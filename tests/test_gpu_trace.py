@@ -56,7 +56,9 @@ def test_take_with_dots(session, grm1000):
     k = 6
     U = session.alloc_device(4 * N * k)
     AU = session.alloc_device(4 * N * 8)
+    Up = session.alloc_device(4 * N * 8)
     SX = session.alloc_device(4 * N * 2)
+    pc = [7, 0, 1, 2, 3, 4]                      # stash / partner column of each slot
     try:
         session.probes_device(U, 0, 200, 0, k)
         u = np.column_stack([D.probe_vector(200, j, N) for j in range(k)]).astype(np.float32)
@@ -67,17 +69,20 @@ def test_take_with_dots(session, grm1000):
         slots = [3, 9, 10, 11, 40, 41]
         session.engine_add_products_device(slots, [0] * k, U, 0)
         session.engine_step(1)
-        X, h = session.engine_take_batch_ex(slots, download=True, X_dev=AU, xcols=[7, 0, 1, 2, 3, 4], Q=SX, m=2)
+        X, h = session.engine_take_batch_ex(slots, download=True, X_dev=AU, xcols=pc, Q=SX, m=2)
         au = K @ u.astype(np.float64)
         assert np.max(np.abs(X - au)) <= 2e-5 * np.max(np.abs(au))
         assert np.allclose(h, X.astype(np.float64).T @ sx.astype(np.float64), rtol=1e-12, atol=1e-9)
         stash = session.copy_device_to_host(AU, 4 * N * 8).view(np.float32).reshape(8, N).T
-        assert np.array_equal(stash[:, [7, 0, 1, 2, 3, 4]], X)
+        assert np.array_equal(stash[:, pc], X)
+        up = np.zeros((N, 8), np.float32)
+        up[:, pc] = u
+        session.copy_host_to_device(Up, up)
         # solves from the device probes, returning d = (K u)^T x and e = u^T x only
         session.engine_add_batch_device(slots, [0] * k, U, 0, 500, 1e-5)
         while not all(session.engine_step(1)[0][slots]):
             pass
-        got, de = session.engine_take_batch_ex(slots, download=True, P1=AU, P2=U, pcols=[7, 0, 1, 2, 3, 4])
+        got, de = session.engine_take_batch_ex(slots, download=True, P1=AU, P2=Up, pcols=pc)
         xf = got.astype(np.float64)
         assert np.allclose(de[:, 0], np.einsum("ij,ij->j", X.astype(np.float64), xf), rtol=1e-12)
         assert np.allclose(de[:, 1], np.einsum("ij,ij->j", u.astype(np.float64), xf), rtol=1e-12)
@@ -85,5 +90,5 @@ def test_take_with_dots(session, grm1000):
         x64 = np.linalg.solve(Sig, u.astype(np.float64))
         assert np.max(np.abs(xf - x64)) <= 2e-3 * np.max(np.abs(x64))
     finally:
-        for b in (U, AU, SX):
+        for b in (U, AU, Up, SX):
             session.free_device(b)
