@@ -31,7 +31,7 @@ FP32_MFMA_PEAK = 157.3
 # HBM bytes per 64-column block-GEMM launch (k_gemm_part + k_gemm_epi, FETCH_SIZE
 # and WRITE_SIZE scaled by a 1 GiB copy), measured with rocprofv3 on random
 # operands; keyed by (N, rt, pd, nsplit).  Source: profiles/kernels_N100k_M150k_bf16.md
-PROFILED_GEMM_TRAFFIC = {(100_000, 1, 3, 8): (42.10 + 0.205 + 0.108 + 0.026) * 1e9}
+PROFILED_GEMM_TRAFFIC = {("fp32", 100_000, 1, 3, 8): (42.10 + 0.205 + 0.108 + 0.026) * 1e9}
 BF16_MFMA_PEAK = 2500.0
 
 
@@ -178,6 +178,7 @@ def c4_leg(args):
     grm_s = sess.build_grm(args.grm)
     grm_tflops = N * (N + 1) * Mqc / grm_s / 1e12
     log(f"C4: Mqc={Mqc}, VR markers {len(vr_index)}, GRM {args.grm} {grm_s:.2f}s {grm_tflops:.0f} TFLOP/s")
+    sess.set_gemm_mode(args.gemm_mode)
     tune = sess.autotune(reps=3)
     log(f"C4 autotune: rt{tune['rt']}/pd{tune['pd']}/s{tune['nsplit']} {tune['ms']:.2f} ms")
     opt = D.Options(nrun=30)
@@ -197,7 +198,7 @@ def c4_leg(args):
            "fit_seconds": t1 - t0, "theta": list(map(float, fit["theta"])), "vr_seconds": t2 - t1,
            "variance_ratios": [row[0] for row in vr["table"]], "vr_markers": [len(r) for r in vr["ratios"]],
            "gemm_avg_ms": gemm_avg * 1e3, "gemm_tflops": 2.0 * N * N * 64 / gemm_avg / 1e12,
-           "gemm_autotune": tune}
+           "gemm_hbm_GBs": 4.0 * N * N / gemm_avg / 1e9, "gemm_mode": args.gemm_mode, "gemm_autotune": tune}
     log(f"C4: fit {t1 - t0:.1f}s theta {fit['theta']}, VR {t2 - t1:.1f}s ratios {out['variance_ratios']}")
     print(json.dumps(out))
 
@@ -216,6 +217,9 @@ def main():
     ap.add_argument("--cpu-sample-markers", type=int, default=4000)
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--config", default="c2", choices=["c2", "c4"])
+    ap.add_argument("--gemm-mode", default="bf16x3", choices=["bf16x3", "fp32"],
+                    help="block-GEMM arithmetic: FP32 operands split into three BF16 pieces on the BF16 "
+                         "MFMA (FP32-accurate), or FP32 MFMA")
     ap.add_argument("--loco-chromosomes", type=int, default=22,
                     help="leave-one-chromosome-out fits for this many autosomes (0: skip)")
     args = ap.parse_args()
@@ -254,8 +258,9 @@ def main():
     grm_s = sess.build_grm(args.grm)
     grm_tflops = N * (N + 1) * Mqc / grm_s / 1e12
     log(f"GRM {args.grm}: {grm_s:.3f}s device time, {grm_tflops:.1f} TFLOP/s")
+    sess.set_gemm_mode(args.gemm_mode)
     tune = sess.autotune(reps=3)
-    log("autotune block GEMM: " + ", ".join(f"rt{a}/pd{b}/s{c} {t:.2f}ms" for a, b, c, t in tune["table"]) +
+    log(f"autotune block GEMM ({args.gemm_mode}): " + ", ".join(f"rt{a}/pd{b}/s{c} {t:.2f}ms" for a, b, c, t in tune["table"]) +
         f" -> rt{tune['rt']}/pd{tune['pd']}/s{tune['nsplit']}; one half: " +
         ", ".join(f"rt{a}/pd{b} {t:.2f}ms" for a, b, _, t in tune["half_table"]) +
         f" -> rt{tune['half']['rt']}/pd{tune['half']['pd']}")
@@ -364,6 +369,25 @@ def main():
         log(f"cpu baseline: {t_mv:.2f}s per K.v with {threads} threads -> {cpu_fph:.4f} fits/h (extrapolated)")
 
     sess.free_device(dev)
+    traffic = PROFILED_GEMM_TRAFFIC.get((args.gemm_mode, N, tune["rt"], tune["pd"], tune["nsplit"]))
+    common = {"traffic": traffic,
+              "traffic_source": "profiles/kernels_N100k_M150k_bf16.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
+              "algorithmic": f"2*N^2*64 = {gemm_flop:.3e} flop per launch; 4*N^2 = {4.0 * N * N:.3e} B of S",
+              "avg_launch_ms": gemm_avg * 1e3, "launches": gemm_launches,
+              "half_width_launches": half_launches, "half_width_avg_ms": half_avg * 1e3}
+    if args.gemm_mode == "fp32":
+        roofline = {"bound": "mfma", "kernel": "k_gemm_part + k_gemm_epi (block-PCG GRM x 64 RHS, FP32 MFMA)",
+                    "achieved": gemm_tflops, "peak": FP32_MFMA_PEAK, "unit": "TFLOP/s",
+                    "frac": gemm_tflops / FP32_MFMA_PEAK, **common}
+    else:
+        # six BF16 piece products per FP32 product: 192 flop per byte of S, under
+        # the BF16 ridge (2500 TFLOP/s / 8 TB/s = 312), so the bound is HBM
+        gbs = 4.0 * N * N / gemm_avg / 1e9
+        roofline = {"bound": "hbm", "kernel": "k_split_p + k_gemm_x3 + k_gemm_epi (block-PCG GRM x 64 RHS, "
+                                              "FP32 as three BF16 pieces on the BF16 MFMA)",
+                    "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                    "bf16_mfma_tflops": 6.0 * gemm_flop / gemm_avg / 1e12, "fp32_equiv_tflops": gemm_tflops,
+                    **common}
     out = {
         "metric": "null-GLMM fits/hour at N=100k batched vs plain; GRM TFLOP/s; p99 fit latency",
         "value": fits_per_hour,
@@ -379,7 +403,8 @@ def main():
         "data": "synthetic (device-generated PLINK .bed, simulated phenotypes)",
         "config": {"workload": f"batched binary null fits, N={N}, M={M}, 4 phenotypes x (nrun {args.nrun} + 1) "
                                f"= 64-RHS block PCG, {args.grm.upper()} GRM (FP32 storage)",
-                   "N": N, "M": M, "Mqc": Mqc, "phenotypes": len(jobs), "nrun": args.nrun, "grm": args.grm},
+                   "N": N, "M": M, "Mqc": Mqc, "phenotypes": len(jobs), "nrun": args.nrun, "grm": args.grm,
+                   "gemm_mode": args.gemm_mode},
         "grm_tflops": grm_tflops,
         "grm_seconds": grm_s,
         "gemm_autotune": tune,
@@ -389,14 +414,7 @@ def main():
         "p99_fit_latency_s": serving["p99_latency_s"] if serving else float(np.percentile(lat, 99)),
         "serving": serving,
         "loco": loco,
-        "roofline": {"bound": "mfma", "kernel": "k_gemm (block-PCG GRM x 64 RHS, FP32 MFMA)",
-                     "achieved": gemm_tflops, "peak": FP32_MFMA_PEAK, "unit": "TFLOP/s",
-                     "frac": gemm_tflops / FP32_MFMA_PEAK,
-                     "traffic": PROFILED_GEMM_TRAFFIC.get((N, tune["rt"], tune["pd"], tune["nsplit"])),
-                     "traffic_source": "profiles/kernels_N100k_M150k_bf16.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
-                     "algorithmic": f"2*N^2*64 = {gemm_flop:.3e} flop per launch; 4*N^2 = {4.0 * N * N:.3e} B",
-                     "avg_launch_ms": gemm_avg * 1e3, "launches": gemm_launches,
-                     "half_width_launches": half_launches, "half_width_avg_ms": half_avg * 1e3},
+        "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

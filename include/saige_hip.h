@@ -302,6 +302,19 @@ int saige_set_gemm_config(saige_ctx* ctx, int rt, int pd, int nsplit);
 int saige_get_gemm_config(const saige_ctx* ctx, int* rt, int* pd, int* nsplit, double* tuned_ms);
 /* Shape of one-half launches (nsplit as the full shape). */
 int saige_get_gemm_config_half(const saige_ctx* ctx, int* rt, int* pd, double* tuned_ms);
+/* Arithmetic of the block GEMM (no reference counterpart; the reference's
+ * product is an FP32 SGEMV).  SAIGE_GEMM_FP32: FP32 MFMA 32x32x2.
+ * SAIGE_GEMM_BF16X3: every FP32 operand split exactly into three BF16
+ * pieces (8 significand bits each), the six piece products of weight
+ * >= 2^-16 summed on the BF16 MFMA with FP32 accumulation; the dropped
+ * products are below 2^-23 of each |s p| term, the size of one FP32
+ * rounding.  Both give FP32-accurate products; their bits differ.  The
+ * shape parameters mean the same in both; changing the mode clears the
+ * tuned times (re-run saige_autotune).  Engine idle. */
+#define SAIGE_GEMM_FP32 0
+#define SAIGE_GEMM_BF16X3 1
+int saige_set_gemm_mode(saige_ctx* ctx, int mode);
+int saige_get_gemm_mode(const saige_ctx* ctx, int* mode);
 
 /* Free and total device memory (hipMemGetInfo), for admission control. */
 int saige_device_memory(saige_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);

@@ -243,6 +243,199 @@ __global__ __launch_bounds__(256) void k_gemm_part(const float* __restrict__ S, 
         }
 }
 
+// ------------------------------------------- GEMM, three-piece BF16 split
+// The same product Y = S P on the BF16 MFMA pipe (32x32x16) at FP32
+// accuracy.  Every FP32 operand x is split exactly into three BF16 pieces,
+// x = hi + mid + lo: hi keeps the top 8 significand bits (truncation), the
+// remainder x - hi is exact in FP32 and is split the same way, and the last
+// remainder has at most 8 significant bits, so lo holds it exactly.  Of the
+// nine piece products the six with weight >= 2^-16 are summed,
+//   hi*hi + hi*mid + mid*hi + hi*lo + lo*hi + mid*mid,
+// each exact in FP32 and accumulated by the MFMA in FP32; the three dropped
+// terms (mid*lo, lo*mid, lo*lo) are below 2^-23 of |s p| together.  Six
+// 32x32x16 BF16 MFMAs take 96 cycles per 32x32x16 block against 512 for
+// eight 32x32x2 FP32 MFMAs, so the product is bound by the HBM stream of S
+// rather than by the FP32 matrix pipe.
+//
+// k_split_p: P [Npad][EW] FP32 -> pb[3][Npad/8][EW] x 8 BF16 (piece, group of
+// 8 consecutive rows g, column n): one 16-byte B operand per (g, n).
+// k_gemm_x3<RT, PD, W>: rows and slices as k_gemm_part; S is split in
+// registers (lane l holds row l%32 and columns 16*(l/32) + [0, 16) of each
+// 32-column chunk: MFMA step s uses its 8 columns 16*(l/32) + 8s + [0, 8), so
+// the B operand of step s is group 4c + 2*(l/32) + s).  Each chunk's B pieces
+// (12 KB) are loaded once per workgroup (thread -> group 4c + wave, column
+// lane, three pieces) into a register ring at the same distance PD as the S
+// chunk, stored to a double-buffered LDS tile and read back by all four
+// waves.  The chunk loop is straight-line (prefetches past the slice end are
+// clamped to its last chunk and never used), so the compiler's wait counts
+// stay exact and loads remain PD chunks in flight; the barrier is an
+// LDS-only one (lgkmcnt, s_barrier), since __syncthreads' release fence
+// would also drain every outstanding global load.  W is the waves-per-SIMD
+// bound each shape is compiled for.  Raw sums go to Ypart exactly as
+// k_gemm_part writes them.
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;
+
+__device__ __forceinline__ void split3_pair(float x0, float x1, uint32_t& h, uint32_t& m, uint32_t& l) {
+    const uint32_t b0 = __float_as_uint(x0), b1 = __float_as_uint(x1);
+    const float r0 = x0 - __uint_as_float(b0 & 0xffff0000u);
+    const float r1 = x1 - __uint_as_float(b1 & 0xffff0000u);
+    const uint32_t c0 = __float_as_uint(r0), c1 = __float_as_uint(r1);
+    const float q0 = r0 - __uint_as_float(c0 & 0xffff0000u);
+    const float q1 = r1 - __uint_as_float(c1 & 0xffff0000u);
+    h = (b1 & 0xffff0000u) | (b0 >> 16);
+    m = (c1 & 0xffff0000u) | (c0 >> 16);
+    l = (__float_as_uint(q1) & 0xffff0000u) | (__float_as_uint(q0) >> 16);
+}
+
+__device__ __forceinline__ void split3_8(float4 a, float4 b, u32x4& h, u32x4& m, u32x4& l) {
+    uint32_t h0, h1, h2, h3, m0, m1, m2, m3, l0, l1, l2, l3;
+    split3_pair(a.x, a.y, h0, m0, l0);
+    split3_pair(a.z, a.w, h1, m1, l1);
+    split3_pair(b.x, b.y, h2, m2, l2);
+    split3_pair(b.z, b.w, h3, m3, l3);
+    h = u32x4{h0, h1, h2, h3};
+    m = u32x4{m0, m1, m2, m3};
+    l = u32x4{l0, l1, l2, l3};
+}
+
+__device__ __forceinline__ f32x16 mfma_bf(u32x4 a, u32x4 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b),
+                                                   c, 0, 0, 0);
+}
+
+// the six terms, smallest first
+__device__ __forceinline__ f32x16 mfma_x3(u32x4 ah, u32x4 am, u32x4 al, u32x4 bh, u32x4 bm, u32x4 bl,
+                                          f32x16 c) {
+    c = mfma_bf(al, bh, c);
+    c = mfma_bf(ah, bl, c);
+    c = mfma_bf(am, bm, c);
+    c = mfma_bf(am, bh, c);
+    c = mfma_bf(ah, bm, c);
+    return mfma_bf(ah, bh, c);
+}
+
+__global__ __launch_bounds__(256) void k_split_p(const float* __restrict__ in, int64_t G,
+                                                 u32x4* __restrict__ pb) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= G * EW) return;
+    const int64_t g = t / EW;
+    const int n = (int)(t % EW);
+    float x[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) x[e] = in[(8 * g + e) * EW + n];
+    u32x4 h, m, l;
+    split3_8(make_float4(x[0], x[1], x[2], x[3]), make_float4(x[4], x[5], x[6], x[7]), h, m, l);
+    pb[g * EW + n] = h;
+    pb[(G + g) * EW + n] = m;
+    pb[(2 * G + g) * EW + n] = l;
+}
+
+// wait for this wave's LDS traffic, then the workgroup barrier; outstanding
+// global loads stay in flight
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+template <int RT, int PD, int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) void k_gemm_x3(
+    const float* __restrict__ S, int64_t ld, const u32x4* __restrict__ pb, float* __restrict__ ypart, int halves) {
+    constexpr int GM = 128 * RT;
+    constexpr int RING = PD + 1;
+    __shared__ u32x4 sB[2][12 * 64];                        // [parity][(g*3 + piece)*64 + n]
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, hl = lane >> 5, ln = lane & 31;
+    const int64_t r0 = (int64_t)blockIdx.x * GM;
+    const int split = blockIdx.y, nsplit = gridDim.y;
+    const int64_t nch = ld / 32, G = ld / 8;
+    const int64_t c0 = nch * split / nsplit, c1 = nch * (split + 1) / nsplit;   // >= 4 chunks
+    const float* srow = S + (r0 + wv * 32 * RT + ln) * ld + hl * 16;
+    const u32x4* bsrc = pb + wv * EW + lane;
+
+    f32x16 acc[RT][2];
+#pragma unroll
+    for (int t = 0; t < RT; t++)
+#pragma unroll
+        for (int ni = 0; ni < 2; ni++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[t][ni][r] = 0.0f;
+    float4 ring[RING][RT][4];                               // S chunks in flight
+    u32x4 bring[RING][3];                                   // B pieces in flight
+    auto issue = [&](float4 (&rs)[RT][4], u32x4 (&br)[3], int64_t ch) {
+#pragma unroll
+        for (int p = 0; p < 3; p++) br[p] = bsrc[(p * G + 4 * ch) * EW];
+#pragma unroll
+        for (int t = 0; t < RT; t++)
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                rs[t][q] = *reinterpret_cast<const float4*>(srow + (int64_t)t * 32 * ld + ch * 32 + 4 * q);
+    };
+    auto stage = [&](u32x4* sb, const u32x4 (&br)[3]) {
+#pragma unroll
+        for (int p = 0; p < 3; p++) sb[(wv * 3 + p) * 64 + lane] = br[p];
+    };
+    const int h1 = halves >> 1;   // the live half when only one is live
+    auto compute = [&](const u32x4* sb, const float4 (&rs)[RT][4]) {
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            u32x4 ah[RT], am[RT], al[RT];
+#pragma unroll
+            for (int t = 0; t < RT; t++) split3_8(rs[t][2 * s], rs[t][2 * s + 1], ah[t], am[t], al[t]);
+            if (halves == 3) {
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const u32x4* bb = sb + (2 * hl + s) * 3 * 64 + 32 * h + ln;
+                    const u32x4 bh = bb[0], bm = bb[64], bl = bb[128];
+#pragma unroll
+                    for (int t = 0; t < RT; t++) acc[t][h] = mfma_x3(ah[t], am[t], al[t], bh, bm, bl, acc[t][h]);
+                }
+            } else {
+                const u32x4* bb = sb + (2 * hl + s) * 3 * 64 + 32 * h1 + ln;
+                const u32x4 bh = bb[0], bm = bb[64], bl = bb[128];
+#pragma unroll
+                for (int t = 0; t < RT; t++) acc[t][0] = mfma_x3(ah[t], am[t], al[t], bh, bm, bl, acc[t][0]);
+            }
+        }
+    };
+#pragma unroll
+    for (int d = 0; d < PD; d++) issue(ring[d], bring[d], c0 + d);
+    int64_t cb = c0;
+    for (; cb + RING <= c1; cb += RING) {
+#pragma unroll
+        for (int u = 0; u < RING; u++) {
+            const int64_t c = cb + u;
+            u32x4* sb = sB[(c - c0) & 1];
+            stage(sb, bring[u]);
+            lds_barrier();
+            issue(ring[(u + PD) % RING], bring[(u + PD) % RING], c + PD < c1 ? c + PD : c1 - 1);
+            compute(sb, ring[u]);
+        }
+    }
+    // the last (c1 - cb) < RING chunks are already in slots 0, 1, ...
+#pragma unroll
+    for (int u = 0; u < RING - 1; u++) {
+        const int64_t c = cb + u;
+        if (c < c1) {
+            u32x4* sb = sB[(c - c0) & 1];
+            stage(sb, bring[u]);
+            lds_barrier();
+            compute(sb, ring[u]);
+        }
+    }
+    float* yp = ypart + (int64_t)split * (gridDim.x * (int64_t)GM) * EW;
+    const int nh = halves == 3 ? 2 : 1;
+    const int hh0 = halves == 2 ? 1 : 0;
+#pragma unroll
+    for (int t = 0; t < RT; t++)
+#pragma unroll
+        for (int ni = 0; ni < 2; ni++) {
+            if (ni >= nh) break;
+            const int n = (hh0 + ni) * 32 + ln;
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int64_t row = r0 + wv * 32 * RT + t * 32 + 8 * (r >> 2) + 4 * hl + (r & 3);
+                yp[row * EW + n] = acc[t][ni][r];
+            }
+        }
+}
+
 constexpr int EROWS = 256;   // rows per workgroup of the epilogue
 
 __global__ __launch_bounds__(256) void k_gemm_epi(const float* __restrict__ ypart, int nsplit, int64_t Npad,
@@ -715,7 +908,8 @@ hipError_t launch_grm_gemv(hipStream_t s, const float* S, int64_t Npad, int64_t 
 
 bool gemm_config_valid(GemmConfig g) {
     return (g.rt == 1 || g.rt == 2) && (g.pd == 1 || g.pd == 3) &&
-           (g.nsplit == 1 || g.nsplit == 2 || g.nsplit == 4 || g.nsplit == 8);
+           (g.nsplit == 1 || g.nsplit == 2 || g.nsplit == 4 || g.nsplit == 8) &&
+           (g.mode == GEMM_FP32 || g.mode == GEMM_BF16X3);
 }
 
 hipError_t launch_grm_gemm(hipStream_t s, const float* S, int64_t Npad, int64_t Nnm, const float* in,
@@ -726,7 +920,21 @@ hipError_t launch_grm_gemm(hipStream_t s, const float* S, int64_t Npad, int64_t 
     if (!gemm_config_valid(g) || Npad % 256 != 0 || halves < 1 || halves > 3) return hipErrorInvalidValue;
     const int nsplit = (int)std::min<int64_t>(g.nsplit, Npad / GKS);
     const dim3 grid((unsigned)(Npad / (128 * g.rt)), nsplit);
-    if (g.rt == 2 && g.pd == 3)
+    if (g.mode == GEMM_BF16X3) {
+        // split P into the three BF16 pieces behind the slices in ypart
+        const int64_t G = Npad / 8;
+        u32x4* pb = reinterpret_cast<u32x4*>(ypart + (int64_t)GSPLIT_MAX * Npad * EW);
+        hipLaunchKernelGGL(k_split_p, dim3((unsigned)(G * EW / 256)), dim3(256), 0, s, in, G, pb);
+        // waves per SIMD: the most each shape's registers allow without spills
+        if (g.rt == 2 && g.pd == 3)
+            hipLaunchKernelGGL((k_gemm_x3<2, 3, 1>), grid, dim3(256), 0, s, S, Npad, pb, ypart, halves);
+        else if (g.rt == 2)
+            hipLaunchKernelGGL((k_gemm_x3<2, 1, 2>), grid, dim3(256), 0, s, S, Npad, pb, ypart, halves);
+        else if (g.pd == 3)
+            hipLaunchKernelGGL((k_gemm_x3<1, 3, 2>), grid, dim3(256), 0, s, S, Npad, pb, ypart, halves);
+        else
+            hipLaunchKernelGGL((k_gemm_x3<1, 1, 3>), grid, dim3(256), 0, s, S, Npad, pb, ypart, halves);
+    } else if (g.rt == 2 && g.pd == 3)
         hipLaunchKernelGGL((k_gemm_part<2, 3>), grid, dim3(256), 0, s, S, Npad, in, ypart, halves);
     else if (g.rt == 2)
         hipLaunchKernelGGL((k_gemm_part<2, 1>), grid, dim3(256), 0, s, S, Npad, in, ypart, halves);
@@ -743,7 +951,9 @@ hipError_t launch_grm_gemm(hipStream_t s, const float* S, int64_t Npad, int64_t 
     return hipGetLastError();
 }
 
-int64_t gemm_ypart_floats(int64_t Npad) { return (int64_t)GSPLIT_MAX * Npad * EW; }
+// the split-K slices, then (three-piece mode) P's BF16 pieces: 3 x 2 bytes
+// per element, 1.5 floats, rounded up to 2
+int64_t gemm_ypart_floats(int64_t Npad) { return (int64_t)(GSPLIT_MAX + 2) * Npad * EW; }
 
 hipError_t launch_sigma_setup(hipStream_t s, const float* w, const uint8_t* mask_in, const float* diag,
                               int64_t Nnm, int64_t Npad, float tau0, float tau1, float mqc, int diag_one,

@@ -1694,14 +1694,18 @@ int saige_autotune(saige_ctx* c, int reps, double* ms_table) {
         *out_ms = (double)ms / reps;
         return e;
     };
+    // shapes are tuned within the context's arithmetic mode
+    const int mode = c->gemm_cfg.mode;
     GemmConfig best = c->gemm_cfg;
     double best_ms = 0.0;
     for (int i = 0; i < kNumGemmCandidates; i++) {
+        GemmConfig g = kGemmCandidates[i];
+        g.mode = mode;
         double per = 0.0;
-        SAIGE_HIP_TRY(time_shape(kGemmCandidates[i], 3, &per));
+        SAIGE_HIP_TRY(time_shape(g, 3, &per));
         if (ms_table) ms_table[i] = per;
         if (i == 0 || per < best_ms) {
-            best = kGemmCandidates[i];
+            best = g;
             best_ms = per;
         }
     }
@@ -1709,7 +1713,7 @@ int saige_autotune(saige_ctx* c, int reps, double* ms_table) {
     GemmConfig best_h = best;
     double best_h_ms = 0.0;
     for (int i = 0; i < kNumHalfCandidates; i++) {
-        GemmConfig g{kHalfCandidates[i].rt, best.nsplit, kHalfCandidates[i].pd};
+        GemmConfig g{kHalfCandidates[i].rt, best.nsplit, kHalfCandidates[i].pd, mode};
         double per = 0.0;
         SAIGE_HIP_TRY(time_shape(g, 1, &per));
         if (ms_table) ms_table[kNumGemmCandidates + i] = per;
@@ -1736,7 +1740,7 @@ int saige_autotune_candidates(int* rt, int* pd, int* nsplit) {
 
 int saige_set_gemm_config(saige_ctx* c, int rt, int pd, int nsplit) {
     CHECK_CTX(c);
-    const GemmConfig g{rt, nsplit, pd};
+    const GemmConfig g{rt, nsplit, pd, c->gemm_cfg.mode};
     if (!gemm_config_valid(g)) {
         set_error("invalid block-GEMM shape rt=%d pd=%d nsplit=%d (rt 1|2, pd 1|3, nsplit 1|2|4|8)", rt, pd,
                   nsplit);
@@ -1747,6 +1751,32 @@ int saige_set_gemm_config(saige_ctx* c, int rt, int pd, int nsplit) {
     c->gemm_tuned_ms = 0.0;
     c->gemm_cfg_half = g;
     c->gemm_tuned_ms_half = 0.0;
+    return SAIGE_OK;
+}
+
+int saige_set_gemm_mode(saige_ctx* c, int mode) {
+    CHECK_CTX(c);
+    if (mode != SAIGE_GEMM_FP32 && mode != SAIGE_GEMM_BF16X3) {
+        set_error("unknown block-GEMM mode %d (0 = FP32 MFMA, 1 = three-piece BF16 split)", mode);
+        return SAIGE_E_ARG;
+    }
+    if (int rc = engine_idle(c, "changing the block-GEMM mode")) return rc;
+    if (c->gemm_cfg.mode != mode) {
+        // timings of the other mode do not carry over
+        c->gemm_cfg.mode = mode;
+        c->gemm_cfg_half.mode = mode;
+        c->gemm_tuned_ms = 0.0;
+        c->gemm_tuned_ms_half = 0.0;
+    }
+    return SAIGE_OK;
+}
+
+int saige_get_gemm_mode(const saige_ctx* c, int* mode) {
+    if (!c) {
+        set_error("null context");
+        return SAIGE_E_ARG;
+    }
+    if (mode) *mode = c->gemm_cfg.mode;
     return SAIGE_OK;
 }
 

@@ -48,10 +48,15 @@ constexpr int64_t NPAD_ALIGN = 256;
 // chunks of S requested ahead (1 or 3), nsplit = column slices of S (1, 2, 4
 // or 8).  The slice count fixes the summation order, so results are
 // reproducible for a fixed configuration.
+// Block-GEMM arithmetic: FP32 MFMA (32x32x2), or FP32 operands split
+// exactly into three BF16 pieces on the BF16 MFMA (32x32x16)
+enum GemmMode { GEMM_FP32 = 0, GEMM_BF16X3 = 1 };
+
 struct GemmConfig {
     int rt = 1;
     int nsplit = 4;
     int pd = 1;
+    int mode = GEMM_FP32;
 };
 
 struct Engine {

@@ -287,6 +287,19 @@ class Session:
     def set_gemm_config(self, rt, pd, nsplit):
         check(self._L.saige_set_gemm_config(self.ctx, int(rt), int(pd), int(nsplit)))
 
+    GEMM_MODES = ("fp32", "bf16x3")
+
+    def set_gemm_mode(self, mode):
+        """Block-GEMM arithmetic: "fp32" (FP32 MFMA) or "bf16x3" (FP32
+        operands split exactly into three BF16 pieces, six piece products on
+        the BF16 MFMA, FP32 accumulation).  Clears tuned times."""
+        check(self._L.saige_set_gemm_mode(self.ctx, self.GEMM_MODES.index(mode)))
+
+    def gemm_mode(self):
+        m = ctypes.c_int(0)
+        check(self._L.saige_get_gemm_mode(self.ctx, ctypes.byref(m)))
+        return self.GEMM_MODES[m.value]
+
     def gemm_config(self):
         rt, pd, ns, ms = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0), ctypes.c_double(0.0)
         check(self._L.saige_get_gemm_config(self.ctx, ctypes.byref(rt), ctypes.byref(pd), ctypes.byref(ns),
@@ -294,6 +307,7 @@ class Session:
         hr, hp, hm = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_double(0.0)
         check(self._L.saige_get_gemm_config_half(self.ctx, ctypes.byref(hr), ctypes.byref(hp), ctypes.byref(hm)))
         return {"rt": int(rt.value), "pd": int(pd.value), "nsplit": int(ns.value), "ms": float(ms.value),
+                "mode": self.gemm_mode(),
                 "half": {"rt": int(hr.value), "pd": int(hp.value), "ms": float(hm.value)}}
 
     def engine_use_loco(self, on):

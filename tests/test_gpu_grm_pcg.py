@@ -322,7 +322,7 @@ def test_autotune_and_every_gemm_shape(session, grm1000):
     by_split = {}
     for rt, pd, ns, _ in table:
         session.set_gemm_config(rt, pd, ns)
-        assert session.gemm_config() == {"rt": rt, "pd": pd, "nsplit": ns, "ms": 0.0,
+        assert session.gemm_config() == {"rt": rt, "pd": pd, "nsplit": ns, "ms": 0.0, "mode": "fp32",
                                          "half": {"rt": rt, "pd": pd, "ms": 0.0}}
         KB = session.grm_matvec(B)
         assert np.max(np.abs(KB - RB)) <= 2e-5 * np.max(np.abs(RB)), (rt, pd, ns)
@@ -339,3 +339,83 @@ def test_autotune_and_every_gemm_shape(session, grm1000):
     with pytest.raises(SaigeError, match="shape"):
         session.set_gemm_config(1, 2, 4)
     session.set_gemm_config(1, 1, 4)
+
+
+def test_gemm_bf16x3_mode(session, grm1000):
+    """The three-piece BF16 split GEMM: exact on unit vectors (the split
+    loses nothing: s = hi + mid + lo), within 2^-22 of s*b for one nonzero
+    entry (only products below 2^-23 |s b| are dropped), K B within the FP32
+    product tolerance, the same bits for every rt/pd at one nsplit and for
+    one-half vs full launches, and PCG/engine solves as in FP32 mode."""
+    og, S = grm1000
+    print("case: GEMM bf16x3 mode", flush=True)
+    session.build_grm("fp32")
+    K64 = _k64(og, S)
+    rng = np.random.default_rng(21)
+    E = np.zeros((1000, 64), np.float32)
+    cols = rng.choice(1000, 64, replace=False)
+    E[cols, np.arange(64)] = 1.0
+    session.set_gemm_config(1, 1, 4)
+    ref_e = session.grm_matvec(E)                       # FP32 MFMA: s * 1 exactly
+    Bv = np.zeros((1000, 64), np.float32)
+    vals = (rng.uniform(0.5, 2.0, 64) / 3.0).astype(np.float32)
+    Bv[cols, np.arange(64)] = vals
+    ref_v = session.grm_matvec(Bv)                      # FP32 MFMA: fl(s * b) / Mqc
+    B = rng.standard_normal((1000, 70)).astype(np.float32)
+    RB = K64 @ B.astype(np.float64)
+    KB32 = session.grm_matvec(B)
+    w = rng.uniform(0.05, 0.25, 1000).astype(np.float32)
+    tau = np.array([1.0, 0.5], np.float32)
+    X64 = np.linalg.solve(_sigma64(og, S, w, tau), B[:, :3].astype(np.float64))
+    try:
+        session.set_gemm_mode("bf16x3")
+        assert session.gemm_config()["mode"] == "bf16x3"
+        got_e = session.grm_matvec(E)
+        assert np.array_equal(got_e.view(np.uint32), ref_e.view(np.uint32))
+        # one nonzero b per column: the six kept products give s * b to
+        # 2^-23 before the division by Mqc; a dropped 2^-8 or 2^-16 term fails
+        got_v = session.grm_matvec(Bv)
+        assert np.all(np.abs(got_v.astype(np.float64) - ref_v) <= 2.0 ** -21 * np.abs(ref_v.astype(np.float64)))
+        by_split = {}
+        for rt, pd, ns in [(1, 1, 4), (2, 1, 4), (1, 3, 4), (2, 3, 4), (1, 1, 1), (2, 3, 8)]:
+            session.set_gemm_config(rt, pd, ns)
+            assert session.gemm_config()["mode"] == "bf16x3"
+            KB = session.grm_matvec(B)
+            err, err32 = np.max(np.abs(KB - RB)), np.max(np.abs(KB32 - RB))
+            print(f"  rt{rt}/pd{pd}/s{ns}: max|KB-K64 B| {err:.3e} (FP32 MFMA {err32:.3e})", flush=True)
+            assert err <= 2e-5 * np.max(np.abs(RB)), (rt, pd, ns)
+            assert np.max(np.abs(KB - KB32)) <= 2e-5 * np.max(np.abs(RB))
+            assert np.array_equal(KB, session.grm_matvec(B))
+            if ns in by_split:
+                assert np.array_equal(KB, by_split[ns]), (rt, pd, ns)
+            by_split[ns] = KB
+            narrow = session.grm_matvec(B[:, :5])
+            assert np.array_equal(narrow.view(np.uint32), KB[:, :5].view(np.uint32))
+            X, _ = session.pcg_block(w, tau, B[:, :3], 500, 1e-5)
+            assert np.max(np.abs(X - X64)) <= 2e-3 * np.max(np.abs(X64)), (rt, pd, ns)
+        # engine: alone vs in a full mix, bitwise
+        session.set_gemm_config(1, 3, 4)
+        session.engine_reserve(64, 2)
+        session.engine_set_system(0, w, tau)
+        session.engine_add(3, 0, B[:, 2], 500, 1e-5)
+        while not session.engine_step(50)[0][3]:
+            pass
+        alone = session.engine_take(3)
+        assert np.max(np.abs(alone - X64[:, 2])) <= 2e-3 * np.max(np.abs(X64[:, 2]))
+        for slot in range(64):
+            session.engine_add(slot, 0, B[:, slot % 5] * (1 + slot) if slot != 40 else B[:, 2], 500, 1e-5)
+        for _ in range(2000):
+            done, _ = session.engine_step(1)
+            if done.all():
+                break
+        mixed = session.engine_take(40)
+        for slot in range(64):
+            if slot != 40:
+                session.engine_take(slot)
+        assert np.array_equal(alone.view(np.uint32), mixed.view(np.uint32))
+        from saige_gpu import SaigeError
+        with pytest.raises(ValueError):
+            session.set_gemm_mode("fp16")
+    finally:
+        session.set_gemm_mode("fp32")
+        session.set_gemm_config(1, 1, 4)
