@@ -309,8 +309,11 @@ int saige_get_gemm_config_half(const saige_ctx* ctx, int* rt, int* pd, double* t
  * >= 2^-16 summed on the BF16 MFMA with FP32 accumulation; the dropped
  * products are below 2^-23 of each |s p| term, the size of one FP32
  * rounding.  Both give FP32-accurate products; their bits differ.  The
- * shape parameters mean the same in both; changing the mode clears the
- * tuned times (re-run saige_autotune).  Engine idle. */
+ * shape parameters mean the same in both, except that in SAIGE_GEMM_BF16X3
+ * rt = 1 runs workgroups of twelve one-tile waves (384 rows share each
+ * staged chunk of P) whose registers hold at most two chunks ahead, so pd = 3
+ * means two there.  Changing the mode clears the tuned times (re-run
+ * saige_autotune).  Engine idle. */
 #define SAIGE_GEMM_FP32 0
 #define SAIGE_GEMM_BF16X3 1
 int saige_set_gemm_mode(saige_ctx* ctx, int mode);

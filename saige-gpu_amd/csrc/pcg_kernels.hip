@@ -436,6 +436,106 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
         }
 }
 
+// k_gemm_x3w<PD>: the same per-wave work as k_gemm_x3<1, PD, 3> (one 32-row
+// tile per wave, identical arithmetic and bits) in workgroups of twelve
+// waves (384 rows), so each chunk's B pieces (12 KB, one 16-byte piece per
+// thread: wave w moves group 4c + w/3, piece w%3) serve three times as many
+// rows of S: B travels from L2 at a quarter of S's bytes instead of three
+// quarters.  A workgroup needs three waves per SIMD, so registers allow at
+// most PD = 2.  Waves whose rows lie past Npad (the last workgroup) read row
+// Npad-32 in place of theirs, take their part in staging and barriers, and
+// store nothing.
+template <int PD>
+__global__ __launch_bounds__(768) void k_gemm_x3w(const float* __restrict__ S, int64_t ld,
+                                                 const u32x4* __restrict__ pb, float* __restrict__ ypart,
+                                                 int halves) {
+    constexpr int NWV = 12;
+    constexpr int GM = 32 * NWV;
+    constexpr int RING = PD + 1;
+    __shared__ u32x4 sB[2][12 * 64];                        // [parity][(g*3 + piece)*64 + n]
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, hl = lane >> 5, ln = lane & 31;
+    const int64_t r0 = (int64_t)blockIdx.x * GM;
+    const int split = blockIdx.y, nsplit = gridDim.y;
+    const int64_t nch = ld / 32, G = ld / 8;
+    const int64_t c0 = nch * split / nsplit, c1 = nch * (split + 1) / nsplit;   // >= 4 chunks
+    const int64_t rw = r0 + wv * 32;                        // the wave's first row
+    const bool valid = rw < ld;                             // ld = Npad, a multiple of 32
+    const float* srow = S + ((valid ? rw : ld - 32) + ln) * ld + hl * 16;
+    const int bg = wv / 3, bp = wv % 3;                     // staging: group 4c + bg, piece bp
+    const u32x4* bsrc = pb + (bp * G + bg) * EW + lane;
+    u32x4* bdst0 = &sB[0][(bg * 3 + bp) * 64 + lane];
+    u32x4* bdst1 = &sB[1][(bg * 3 + bp) * 64 + lane];
+
+    f32x16 acc[2];
+#pragma unroll
+    for (int ni = 0; ni < 2; ni++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) acc[ni][r] = 0.0f;
+    float4 ring[RING][4];
+    u32x4 bring[RING];
+    auto issue = [&](float4 (&rs)[4], u32x4& br, int64_t ch) {
+        br = bsrc[4 * ch * EW];
+#pragma unroll
+        for (int q = 0; q < 4; q++) rs[q] = *reinterpret_cast<const float4*>(srow + ch * 32 + 4 * q);
+    };
+    const int h1 = halves >> 1;
+    auto compute = [&](const u32x4* sb, const float4 (&rs)[4]) {
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            u32x4 ah, am, al;
+            split3_8(rs[2 * s], rs[2 * s + 1], ah, am, al);
+            if (halves == 3) {
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const u32x4* bb = sb + (2 * hl + s) * 3 * 64 + 32 * h + ln;
+                    acc[h] = mfma_x3(ah, am, al, bb[0], bb[64], bb[128], acc[h]);
+                }
+            } else {
+                const u32x4* bb = sb + (2 * hl + s) * 3 * 64 + 32 * h1 + ln;
+                acc[0] = mfma_x3(ah, am, al, bb[0], bb[64], bb[128], acc[0]);
+            }
+        }
+    };
+#pragma unroll
+    for (int d = 0; d < PD; d++) issue(ring[d], bring[d], c0 + d);
+    int64_t cb = c0;
+    for (; cb + RING <= c1; cb += RING) {
+#pragma unroll
+        for (int u = 0; u < RING; u++) {
+            const int64_t c = cb + u;
+            const int par = (int)((c - c0) & 1);
+            *(par ? bdst1 : bdst0) = bring[u];
+            lds_barrier();
+            issue(ring[(u + PD) % RING], bring[(u + PD) % RING], c + PD < c1 ? c + PD : c1 - 1);
+            compute(sB[par], ring[u]);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < RING - 1; u++) {
+        const int64_t c = cb + u;
+        if (c < c1) {
+            const int par = (int)((c - c0) & 1);
+            *(par ? bdst1 : bdst0) = bring[u];
+            lds_barrier();
+            compute(sB[par], ring[u]);
+        }
+    }
+    if (!valid) return;
+    float* yp = ypart + (int64_t)split * ld * EW;
+    const int nh = halves == 3 ? 2 : 1;
+    const int hh0 = halves == 2 ? 1 : 0;
+#pragma unroll
+    for (int ni = 0; ni < 2; ni++) {
+        if (ni >= nh) break;
+        const int n = (hh0 + ni) * 32 + ln;
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int64_t row = rw + 8 * (r >> 2) + 4 * hl + (r & 3);
+            yp[row * EW + n] = acc[ni][r];
+        }
+    }
+}
+
 constexpr int EROWS = 256;   // rows per workgroup of the epilogue
 
 __global__ __launch_bounds__(256) void k_gemm_epi(const float* __restrict__ ypart, int nsplit, int64_t Npad,
@@ -925,15 +1025,18 @@ hipError_t launch_grm_gemm(hipStream_t s, const float* S, int64_t Npad, int64_t 
         const int64_t G = Npad / 8;
         u32x4* pb = reinterpret_cast<u32x4*>(ypart + (int64_t)GSPLIT_MAX * Npad * EW);
         hipLaunchKernelGGL(k_split_p, dim3((unsigned)(G * EW / 256)), dim3(256), 0, s, in, G, pb);
-        // waves per SIMD: the most each shape's registers allow without spills
-        if (g.rt == 2 && g.pd == 3)
-            hipLaunchKernelGGL((k_gemm_x3<2, 3, 1>), grid, dim3(256), 0, s, S, Npad, pb, ypart, halves);
-        else if (g.rt == 2)
-            hipLaunchKernelGGL((k_gemm_x3<2, 1, 2>), grid, dim3(256), 0, s, S, Npad, pb, ypart, halves);
+        // rt 1: twelve-wave workgroups (pd 3 -> two chunks ahead, the register
+        // limit at three waves per SIMD); rt 2: four waves of two tiles, at the
+        // most waves per SIMD each prefetch depth allows without spills
+        const dim3 grid12((unsigned)((Npad + 383) / 384), nsplit);
+        if (g.rt == 1 && g.pd == 3)
+            hipLaunchKernelGGL((k_gemm_x3w<2>), grid12, dim3(768), 0, s, S, Npad, pb, ypart, halves);
+        else if (g.rt == 1)
+            hipLaunchKernelGGL((k_gemm_x3w<1>), grid12, dim3(768), 0, s, S, Npad, pb, ypart, halves);
         else if (g.pd == 3)
-            hipLaunchKernelGGL((k_gemm_x3<1, 3, 2>), grid, dim3(256), 0, s, S, Npad, pb, ypart, halves);
+            hipLaunchKernelGGL((k_gemm_x3<2, 3, 1>), grid, dim3(256), 0, s, S, Npad, pb, ypart, halves);
         else
-            hipLaunchKernelGGL((k_gemm_x3<1, 1, 3>), grid, dim3(256), 0, s, S, Npad, pb, ypart, halves);
+            hipLaunchKernelGGL((k_gemm_x3<2, 1, 2>), grid, dim3(256), 0, s, S, Npad, pb, ypart, halves);
     } else if (g.rt == 2 && g.pd == 3)
         hipLaunchKernelGGL((k_gemm_part<2, 3>), grid, dim3(256), 0, s, S, Npad, in, ypart, halves);
     else if (g.rt == 2)
