@@ -28,10 +28,12 @@ sys.path[:0] = [ROOT, os.path.join(ROOT, "saige-gpu_amd")]
 
 HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK = 157.3
-# HBM bytes per 64-column block-GEMM launch (k_gemm_part + k_gemm_epi, FETCH_SIZE
-# and WRITE_SIZE scaled by a 1 GiB copy), measured with rocprofv3 on random
-# operands; keyed by (N, rt, pd, nsplit).  Source: profiles/kernels_N100k_M150k_bf16.md
-PROFILED_GEMM_TRAFFIC = {("fp32", 100_000, 1, 3, 8): (42.10 + 0.205 + 0.108 + 0.026) * 1e9}
+# HBM bytes per 64-column block-GEMM launch (all its kernels, FETCH_SIZE and
+# WRITE_SIZE scaled by a 1 GiB copy), measured with rocprofv3; keyed by
+# (mode, N, rt, pd, nsplit).  Sources: PROFILE_FILES
+PROFILED_GEMM_TRAFFIC = {("fp32", 100_000, 1, 3, 8): (42.10 + 0.205 + 0.108 + 0.026) * 1e9,
+                         ("bf16x3", 100_000, 1, 3, 8): (0.064 + 41.579 + 0.152) * 1e9}
+PROFILE_FILES = {"fp32": "profiles/kernels_N100k_M150k_bf16.md", "bf16x3": "profiles/gemm_bf16x3_N100k_M150k.md"}
 BF16_MFMA_PEAK = 2500.0
 
 
@@ -371,7 +373,7 @@ def main():
     sess.free_device(dev)
     traffic = PROFILED_GEMM_TRAFFIC.get((args.gemm_mode, N, tune["rt"], tune["pd"], tune["nsplit"]))
     common = {"traffic": traffic,
-              "traffic_source": "profiles/kernels_N100k_M150k_bf16.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
+              "traffic_source": PROFILE_FILES[args.gemm_mode] + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
               "algorithmic": f"2*N^2*64 = {gemm_flop:.3e} flop per launch; 4*N^2 = {4.0 * N * N:.3e} B of S",
               "avg_launch_ms": gemm_avg * 1e3, "launches": gemm_launches,
               "half_width_launches": half_launches, "half_width_avg_ms": half_avg * 1e3}

@@ -444,8 +444,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
 // quarters.  A workgroup needs three waves per SIMD, so registers allow at
 // most PD = 2.  Waves whose rows lie past Npad (the last workgroup) read row
 // Npad-32 in place of theirs, take their part in staging and barriers, and
-// store nothing.
-template <int PD>
+// store nothing.  FULL: both halves live, fixed at compile time (faster for
+// full launches); otherwise `halves` is read at run time (faster for
+// one-half launches, measured).
+template <int PD, bool FULL>
 __global__ __launch_bounds__(768) void k_gemm_x3w(const float* __restrict__ S, int64_t ld,
                                                  const u32x4* __restrict__ pb, float* __restrict__ ypart,
                                                  int halves) {
@@ -484,7 +486,7 @@ __global__ __launch_bounds__(768) void k_gemm_x3w(const float* __restrict__ S, i
         for (int s = 0; s < 2; s++) {
             u32x4 ah, am, al;
             split3_8(rs[2 * s], rs[2 * s + 1], ah, am, al);
-            if (halves == 3) {
+            if (FULL || halves == 3) {
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
                     const u32x4* bb = sb + (2 * hl + s) * 3 * 64 + 32 * h + ln;
@@ -522,8 +524,8 @@ __global__ __launch_bounds__(768) void k_gemm_x3w(const float* __restrict__ S, i
     }
     if (!valid) return;
     float* yp = ypart + (int64_t)split * ld * EW;
-    const int nh = halves == 3 ? 2 : 1;
-    const int hh0 = halves == 2 ? 1 : 0;
+    const int nh = (FULL || halves == 3) ? 2 : 1;
+    const int hh0 = (!FULL && halves == 2) ? 1 : 0;
 #pragma unroll
     for (int ni = 0; ni < 2; ni++) {
         if (ni >= nh) break;
@@ -1029,10 +1031,15 @@ hipError_t launch_grm_gemm(hipStream_t s, const float* S, int64_t Npad, int64_t 
         // limit at three waves per SIMD); rt 2: four waves of two tiles, at the
         // most waves per SIMD each prefetch depth allows without spills
         const dim3 grid12((unsigned)((Npad + 383) / 384), nsplit);
-        if (g.rt == 1 && g.pd == 3)
-            hipLaunchKernelGGL((k_gemm_x3w<2>), grid12, dim3(768), 0, s, S, Npad, pb, ypart, halves);
+        if (g.rt == 1 && halves == 3) {
+            if (g.pd == 3)
+                hipLaunchKernelGGL((k_gemm_x3w<2, true>), grid12, dim3(768), 0, s, S, Npad, pb, ypart, halves);
+            else
+                hipLaunchKernelGGL((k_gemm_x3w<1, true>), grid12, dim3(768), 0, s, S, Npad, pb, ypart, halves);
+        } else if (g.rt == 1 && g.pd == 3)
+            hipLaunchKernelGGL((k_gemm_x3w<2, false>), grid12, dim3(768), 0, s, S, Npad, pb, ypart, halves);
         else if (g.rt == 1)
-            hipLaunchKernelGGL((k_gemm_x3w<1>), grid12, dim3(768), 0, s, S, Npad, pb, ypart, halves);
+            hipLaunchKernelGGL((k_gemm_x3w<1, false>), grid12, dim3(768), 0, s, S, Npad, pb, ypart, halves);
         else if (g.pd == 3)
             hipLaunchKernelGGL((k_gemm_x3<2, 3, 1>), grid, dim3(256), 0, s, S, Npad, pb, ypart, halves);
         else
