@@ -305,6 +305,33 @@ def main():
         f"{half_avg * 1e3:.2f} ms ({4.0 * N * N / half_avg / 1e9:.0f} GB/s of S), "
         f"theta {[list(np.round(h.result['theta'], 4)) for h in handles]}")
 
+    # the same round with the block GEMM on the FP32 MFMA, for comparison:
+    # tuned in that mode, one step timed, results checked against the step above
+    fp32_mode = None
+    if args.gemm_mode != "fp32":
+        sess.set_gemm_mode("fp32")
+        t32 = sess.autotune(reps=3)
+        g0, n0 = sess.engine_stats("gemm_full")
+        h0, m0 = sess.engine_stats("gemm_half")
+        t0 = time.perf_counter()
+        h32, _ = step()
+        s32 = time.perf_counter() - t0
+        g1, n1 = sess.engine_stats("gemm_full")
+        h1, m1 = sess.engine_stats("gemm_half")
+        dtheta = max(float(np.max(np.abs(np.asarray(a.result["theta"]) - np.asarray(b.result["theta"]))
+                                  / np.maximum(np.abs(np.asarray(b.result["theta"])), 1e-12)))
+                     for a, b in zip(handles, h32))
+        fp32_mode = {"fits_per_hour": len(jobs) / s32 * 3600.0, "seconds_per_step": s32,
+                     "gemm_tuned": {k: t32[k] for k in ("rt", "pd", "nsplit", "ms")},
+                     "gemm_full_avg_ms": (g1 - g0) / max(1, n1 - n0) * 1e3,
+                     "gemm_half_avg_ms": (h1 - h0) / max(1, m1 - m0) * 1e3,
+                     "max_rel_theta_diff_vs_split_mode": dtheta}
+        log(f"FP32-MFMA mode: {s32:.2f}s/step, {fp32_mode['fits_per_hour']:.0f} fits/h, GEMM full "
+            f"{fp32_mode['gemm_full_avg_ms']:.2f} ms, half {fp32_mode['gemm_half_avg_ms']:.2f} ms, "
+            f"max rel theta diff vs {args.gemm_mode} {dtheta:.2e}")
+        sess.set_gemm_mode(args.gemm_mode)
+        sess.autotune(reps=3)                      # full and one-half shapes of this mode again
+
     serving = None
     if args.serve_requests > 0:
         capacity = len(jobs) / step_s
@@ -385,7 +412,7 @@ def main():
         # six BF16 piece products per FP32 product: 192 flop per byte of S, under
         # the BF16 ridge (2500 TFLOP/s / 8 TB/s = 312), so the bound is HBM
         gbs = 4.0 * N * N / gemm_avg / 1e9
-        roofline = {"bound": "hbm", "kernel": "k_split_p + k_gemm_x3 + k_gemm_epi (block-PCG GRM x 64 RHS, "
+        roofline = {"bound": "hbm", "kernel": "k_split_p + k_gemm_x3w (rt 1) or k_gemm_x3 (rt 2) + k_gemm_epi (block-PCG GRM x 64 RHS, "
                                               "FP32 as three BF16 pieces on the BF16 MFMA)",
                     "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
                     "bf16_mfma_tflops": 6.0 * gemm_flop / gemm_avg / 1e12, "fp32_equiv_tflops": gemm_tflops,
@@ -410,6 +437,7 @@ def main():
         "grm_tflops": grm_tflops,
         "grm_seconds": grm_s,
         "gemm_autotune": tune,
+        "fp32_mode": fp32_mode,
         "plain": plain,
         "batched_vs_plain": (fits_per_hour / plain["fits_per_hour"]) if plain else None,
         "p50_fit_latency_s": serving["p50_latency_s"] if serving else float(np.percentile(lat, 50)),
