@@ -169,6 +169,29 @@ __device__ __forceinline__ u32x4 decode_row(const uint64_t (&lut)[8], const uint
     return r;
 }
 
+// The same row as decode_row, by byte permutes: lo[e] / hi[e] hold the low /
+// high bytes of marker e's four BF16 table entries (entry c in byte c), so
+// the 8-byte pair table {lo[2k+1], lo[2k]} is indexed by c_even and
+// 4 + c_odd.  (byte >> 2j) & 3 of all eight markers at once gives the
+// selectors; four permutes pick the low and high bytes of two marker pairs
+// and two more interleave them into packed BF16 pairs.
+__device__ __forceinline__ u32x4 decode_row_perm(const uint32_t (&lo)[8], const uint32_t (&hi)[8], uint64_t bytes,
+                                                 int j) {
+    const uint64_t d = (bytes >> (2 * j)) & 0x0303030303030303ull;
+    u32x4 r;
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        const uint32_t sel = (uint32_t)(d >> (32 * q)) + 0x04000400u;   // [c0, c1+4, c2, c3+4]
+        const uint32_t a01 = __builtin_amdgcn_perm(lo[4 * q + 1], lo[4 * q], sel);
+        const uint32_t b01 = __builtin_amdgcn_perm(hi[4 * q + 1], hi[4 * q], sel);
+        const uint32_t a23 = __builtin_amdgcn_perm(lo[4 * q + 3], lo[4 * q + 2], sel);
+        const uint32_t b23 = __builtin_amdgcn_perm(hi[4 * q + 3], hi[4 * q + 2], sel);
+        r[2 * q] = __builtin_amdgcn_perm(b01, a01, 0x05010400u);        // markers 4q, 4q+1
+        r[2 * q + 1] = __builtin_amdgcn_perm(b23, a23, 0x07030602u);    // markers 4q+2, 4q+3
+    }
+    return r;
+}
+
 __global__ __launch_bounds__(256) void k_syrk_pk(const uint8_t* __restrict__ tp, int64_t nchunk, int64_t c0,
                                                  int64_t c1, const uint64_t* __restrict__ lut64,
                                                  const int2* __restrict__ tiles, int64_t nblk,
@@ -283,15 +306,28 @@ __device__ __forceinline__ void store_tile256(float* S, int64_t ldS, int64_t bi,
     }
 }
 
+// wait for this wave's LDS traffic, then the workgroup barrier; outstanding
+// global loads (the next chunk's bytes and tables) stay in flight
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// Double-buffered: while the MFMAs run on chunk k from one LDS buffer, the
+// same waves decode chunk k+1 into the other, so the matrix pipe and the
+// VALU decode overlap (one workgroup per CU); one LDS-only barrier per chunk.
+// The two buffers are separate arrays and the loop is unrolled by two, so
+// the compiler sees that the decode's stores and the MFMA operand loads
+// never alias and can interleave them; the loop body is one basic block (the
+// last step decodes a clamped copy of the last chunk that nothing reads, and
+// diagonal tiles, both panels the same, decode their B copy too).
 __global__ __launch_bounds__(512) void k_syrk_pk256(const uint8_t* __restrict__ tp, int64_t nchunk, int64_t c0,
                                                     int64_t c1, const uint64_t* __restrict__ lut64,
                                                     const int2* __restrict__ tiles, int64_t nblk,
                                                     float* __restrict__ S, int64_t ldS, int accumulate) {
-    __shared__ __attribute__((aligned(16))) __hip_bfloat16 sA[BT2 * P16];
-    __shared__ __attribute__((aligned(16))) __hip_bfloat16 sB[BT2 * P16];
+    __shared__ __attribute__((aligned(16))) __hip_bfloat16 sA0[BT2 * P16];
+    __shared__ __attribute__((aligned(16))) __hip_bfloat16 sB0[BT2 * P16];
+    __shared__ __attribute__((aligned(16))) __hip_bfloat16 sA1[BT2 * P16];
+    __shared__ __attribute__((aligned(16))) __hip_bfloat16 sB1[BT2 * P16];
     const int2 t = tiles[xcd_remap(blockIdx.x, nblk)];
     const int64_t bi = t.x, bj = t.y;
-    const bool diag = bi == bj;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wr = wv >> 1, wc = wv & 1;
     // thread -> marker octet o, byte row b of the 256-sample panel (two
@@ -300,11 +336,7 @@ __global__ __launch_bounds__(512) void k_syrk_pk256(const uint8_t* __restrict__ 
     const int64_t pa = 2 * bi + (b >> 5), pb = 2 * bj + (b >> 5);
     const uint64_t* gA = reinterpret_cast<const uint64_t*>(tp + (pa * nchunk * 32 + (b & 31)) * 64 + 8 * o);
     const uint64_t* gB = reinterpret_cast<const uint64_t*>(tp + (pb * nchunk * 32 + (b & 31)) * 64 + 8 * o);
-    uint64_t ra = gA[c0 * 256], rb = gB[c0 * 256];
     const uint64_t* gl = lut64 + 8 * o;
-    uint64_t la[8];
-#pragma unroll
-    for (int e = 0; e < 8; e++) la[e] = gl[c0 * 64 + e];
 
     f32x16 acc[2][4];
 #pragma unroll
@@ -314,36 +346,38 @@ __global__ __launch_bounds__(512) void k_syrk_pk256(const uint8_t* __restrict__ 
 #pragma unroll
             for (int r = 0; r < 16; r++) acc[mi][ni][r] = 0.0f;
 
-    for (int64_t kc = c0; kc < c1; kc++) {
-        uint8_t ba[8], bb[8];
+    uint64_t ra, rb, la[8];
+    auto fetch = [&](int64_t kc) {
+        ra = gA[kc * 256];
+        rb = gB[kc * 256];
+#pragma unroll
+        for (int e = 0; e < 8; e++) la[e] = gl[kc * 64 + e];
+    };
+    // decode the chunk held in ra/rb/la into (dA, dB), fetching chunk kn
+    auto decode_next = [&](__hip_bfloat16* dA, __hip_bfloat16* dB, int64_t kn) {
+        const uint64_t ca = ra, cb = rb;
+        uint32_t lo[8], hi[8];
 #pragma unroll
         for (int e = 0; e < 8; e++) {
-            ba[e] = (uint8_t)(ra >> (8 * e));
-            bb[e] = (uint8_t)(rb >> (8 * e));
+            const uint32_t l = (uint32_t)la[e], h = (uint32_t)(la[e] >> 32);
+            lo[e] = __builtin_amdgcn_perm(h, l, 0x06040200u);
+            hi[e] = __builtin_amdgcn_perm(h, l, 0x07050301u);
         }
-        uint64_t lc[8];
-#pragma unroll
-        for (int e = 0; e < 8; e++) lc[e] = la[e];
-        const int64_t kn = (kc + 1 < c1) ? kc + 1 : kc;
-        ra = gA[kn * 256];
-        rb = gB[kn * 256];
-#pragma unroll
-        for (int e = 0; e < 8; e++) la[e] = gl[kn * 64 + e];
-        __syncthreads();
+        fetch(kn);
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            *reinterpret_cast<u32x4*>(sA + (4 * b + j) * P16 + 8 * o) = decode_row(lc, ba, j);
-            if (!diag) *reinterpret_cast<u32x4*>(sB + (4 * b + j) * P16 + 8 * o) = decode_row(lc, bb, j);
+            *reinterpret_cast<u32x4*>(dA + (4 * b + j) * P16 + 8 * o) = decode_row_perm(lo, hi, ca, j);
+            *reinterpret_cast<u32x4*>(dB + (4 * b + j) * P16 + 8 * o) = decode_row_perm(lo, hi, cb, j);
         }
-        __syncthreads();
-        const __hip_bfloat16* pB = diag ? sA : sB;
+    };
+    auto multiply = [&](const __hip_bfloat16* pA, const __hip_bfloat16* pB) {
 #pragma unroll
         for (int ks = 0; ks < BK16 / 16; ks++) {
             const int kk = ks * 16 + (lane >> 5) * 8;
             bf16x8 fa[2], fb[4];
 #pragma unroll
             for (int mi = 0; mi < 2; mi++)
-                fa[mi] = *reinterpret_cast<const bf16x8*>(sA + (wr * 64 + mi * 32 + (lane & 31)) * P16 + kk);
+                fa[mi] = *reinterpret_cast<const bf16x8*>(pA + (wr * 64 + mi * 32 + (lane & 31)) * P16 + kk);
 #pragma unroll
             for (int ni = 0; ni < 4; ni++)
                 fb[ni] = *reinterpret_cast<const bf16x8*>(pB + (wc * 128 + ni * 32 + (lane & 31)) * P16 + kk);
@@ -353,7 +387,29 @@ __global__ __launch_bounds__(512) void k_syrk_pk256(const uint8_t* __restrict__ 
                 for (int ni = 0; ni < 4; ni++)
                     acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[mi], fb[ni], acc[mi][ni], 0, 0, 0);
         }
+    };
+    // step: multiply chunk kc from (cA, cB) while decoding chunk kc+1 into (nA, nB)
+    auto step = [&](const __hip_bfloat16* cA, const __hip_bfloat16* cB, __hip_bfloat16* nA, __hip_bfloat16* nB,
+                    int64_t kc) {
+        decode_next(nA, nB, kc + 2 < c1 ? kc + 2 : c1 - 1);
+        multiply(cA, cB);
+        // one MFMA, then about five decode instructions, 32 times
+#pragma unroll
+        for (int i = 0; i < 32; i++) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);
+        }
+        lds_barrier();
+    };
+    fetch(c0);
+    decode_next(sA0, sB0, c0 + 1 < c1 ? c0 + 1 : c0);
+    lds_barrier();
+    int64_t kc = c0;
+    for (; kc + 2 <= c1; kc += 2) {
+        step(sA0, sB0, sA1, sB1, kc);
+        step(sA1, sB1, sA0, sB0, kc + 1);
     }
+    if (kc < c1) multiply(sA0, sB0);
     store_tile256(S, ldS, bi, bj, wr, wc, lane, acc, accumulate);
 }
 
