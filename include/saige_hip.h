@@ -350,6 +350,73 @@ int saige_copy_device(saige_ctx* ctx, void* dst_dev, const void* src_dev, int64_
 /* Copy rows of a device .bed body to host (for building phenotypes). */
 int saige_copy_device_to_host(saige_ctx* ctx, const void* src_dev, int64_t bytes, void* dst_host);
 
+/* ---------------------------------------------- null fits in C++ (host)
+ * The R drivers of step 1 in C++, so that an R or C++ caller gets whole
+ * null fits -- many of them batched on the block-PCG engine -- without
+ * another runtime: glm() start values (FG.R:1119, 1370), Get_Coef
+ * (FG.R:2-35), glmmkin.ai_PCG_Rcpp_Binary / _Quantitative (FG.R:79-304,
+ * 309-549) with getCoefficients, getAIScore(_q), GetTrace(_q) and
+ * fitglmmaiRPCG(_q) (FG.cpp:3113-3160, 3167-3199, 3279-3341, 3409-3662),
+ * Covariate_Transform (FG.R:1612-1647) and ScoreTest_NULL_Model
+ * (FG.R:579-591).  Host arithmetic is FP64; trace probes are the
+ * deterministic Rademacher vectors of saige_probes_device (keyed by
+ * probe_seed and the probe index) in place of R's rbinom draws.  Matrices
+ * are column-major. */
+#define SAIGE_TRAIT_BINARY 0
+#define SAIGE_TRAIT_QUANTITATIVE 1
+typedef struct saige_fit_options {
+    int maxiter;            /* AI-REML and Get_Coef iterations (20)          */
+    double tol;             /* tau tolerance (0.02)                           */
+    int nrun;               /* probes in the first trace round (30)          */
+    double tol_pcg;         /* PCG tolerance (1e-5)                           */
+    int maxiter_pcg;        /* PCG iterations (500)                           */
+    double trace_cv_cutoff; /* trace CV cutoff (0.0025)                       */
+    uint64_t probe_seed;    /* probe hash seed (200)                          */
+} saige_fit_options;
+typedef struct saige_fit_spec {
+    int trait;              /* SAIGE_TRAIT_BINARY or SAIGE_TRAIT_QUANTITATIVE */
+    int64_t n;              /* samples in this fit                            */
+    const int64_t* subset;  /* n ascending indices into the N phenotyped
+                               samples, or NULL for all of them (n == N)      */
+    const double* y;        /* n                                              */
+    const double* X;        /* n x p, covariate-transformed                   */
+    int p;
+    const double* offset;   /* n, or NULL                                     */
+} saige_fit_spec;
+typedef struct saige_fit_result {
+    double theta[2];
+    double* coefficients;       /* p (caller-allocated; required)     */
+    double* linear_predictors;  /* n, or NULL                         */
+    double* fitted_values;      /* n, or NULL                         */
+    double* working_y;          /* n: Y of the last Get_Coef, or NULL */
+    double* cov;                /* p x p, or NULL                     */
+    int converged;
+    int iterations;
+    int status;                 /* SAIGE_OK, or why this fit failed   */
+} saige_fit_result;
+/* The defaults of fitNULLGLMM (FG.R:1036-1063). */
+void saige_fit_options_default(saige_fit_options* opt);
+/* Fit n_fits null GLMMs on the resident GRM (engine idle).  Up to
+ * max_concurrent fits (<= 0: 16) are admitted at a time, each with its own
+ * engine system (weights, tau, sample mask); their solves and GRM products
+ * share the engine's 64 slots (continuous batching), and each fit's host
+ * work runs in its own thread while the engine steps.  Returns SAIGE_OK when
+ * the engine ran; results[i].status says whether fit i succeeded (e.g.
+ * SAIGE_E_STATE when its first variance component went to 0, FG.R:449-451). */
+int saige_fit_null_batch(saige_ctx* ctx, int n_fits, const saige_fit_spec* specs, const saige_fit_options* opt,
+                         int max_concurrent, saige_fit_result* results);
+/* Covariate_Transform (FG.R:1612-1647): X (n x p) -> sqrt(n) Q of its QR
+ * decomposition (X_out n x p_kept, R_out p_kept x p_kept or NULL), after
+ * dropping linearly dependent columns (kept: their indices, or NULL). */
+int saige_covariate_transform(const double* X, int64_t n, int p, double* X_out, double* R_out, int32_t* kept,
+                              int* p_kept);
+/* ScoreTest_NULL_Model (FG.R:579-591) for fitted mu and variance V; any
+ * output may be NULL: XV (p x n), XVX, XVX_inv (p x p), XXVX_inv and
+ * XVX_inv_XV (n x p), S_a (p). */
+int saige_score_test_null(const double* mu, const double* V, const double* y, const double* X, int64_t n, int p,
+                          double* XV, double* XVX, double* XVX_inv, double* XXVX_inv, double* XVX_inv_XV,
+                          double* S_a);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,662 @@
+// Null-GLMM fitting logic (host, FP64); see fit_core.hpp.
+#include "fit_core.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+namespace saige {
+namespace fit {
+
+namespace {
+
+// ------------------------------------------------------------ dense algebra
+
+// Solve A X = B (A p x p) by Gaussian elimination with partial pivoting, as
+// LAPACK's dgesv; throws if A is singular.
+Matrix solve_dense(Matrix A, Matrix B) {
+    const int64_t p = A.rows;
+    std::vector<int64_t> piv(p);
+    for (int64_t k = 0; k < p; k++) {
+        int64_t r = k;
+        for (int64_t i = k + 1; i < p; i++)
+            if (std::fabs(A(i, k)) > std::fabs(A(r, k))) r = i;
+        if (A(r, k) == 0.0) throw std::runtime_error("singular matrix");
+        if (r != k) {
+            for (int64_t j = 0; j < p; j++) std::swap(A(k, j), A(r, j));
+            for (int64_t j = 0; j < B.cols; j++) std::swap(B(k, j), B(r, j));
+        }
+        for (int64_t i = k + 1; i < p; i++) {
+            const double f = A(i, k) / A(k, k);
+            A(i, k) = 0.0;
+            for (int64_t j = k + 1; j < p; j++) A(i, j) -= f * A(k, j);
+            for (int64_t j = 0; j < B.cols; j++) B(i, j) -= f * B(k, j);
+        }
+    }
+    for (int64_t j = 0; j < B.cols; j++)
+        for (int64_t i = p - 1; i >= 0; i--) {
+            double s = B(i, j);
+            for (int64_t l = i + 1; l < p; l++) s -= A(i, l) * B(l, j);
+            B(i, j) = s / A(i, i);
+        }
+    return B;
+}
+
+Matrix identity(int64_t p) {
+    Matrix I(p, p);
+    for (int64_t i = 0; i < p; i++) I(i, i) = 1.0;
+    return I;
+}
+
+Matrix inverse(const Matrix& A) { return solve_dense(A, identity(A.rows)); }
+
+bool cholesky_ok(const Matrix& A) {
+    const int64_t p = A.rows;
+    Matrix L(p, p);
+    for (int64_t j = 0; j < p; j++) {
+        double d = A(j, j);
+        for (int64_t k = 0; k < j; k++) d -= L(j, k) * L(j, k);
+        if (!(d > 0.0)) return false;
+        L(j, j) = std::sqrt(d);
+        for (int64_t i = j + 1; i < p; i++) {
+            double s = A(i, j);
+            for (int64_t k = 0; k < j; k++) s -= L(i, k) * L(j, k);
+            L(i, j) = s / L(j, j);
+        }
+    }
+    return true;
+}
+
+// Moore-Penrose inverse of a symmetric matrix from its Jacobi eigen
+// decomposition; singular values below 1e-15 of the largest count as zero
+// (numpy.linalg.pinv's default cutoff).
+Matrix pinv_sym(const Matrix& A) {
+    const int64_t p = A.rows;
+    Matrix a = A, v = identity(p);
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0.0;
+        for (int64_t i = 0; i < p; i++)
+            for (int64_t j = i + 1; j < p; j++) off += a(i, j) * a(i, j);
+        if (off < 1e-30) break;
+        for (int64_t i = 0; i < p; i++)
+            for (int64_t j = i + 1; j < p; j++) {
+                if (a(i, j) == 0.0) continue;
+                const double th = (a(j, j) - a(i, i)) / (2.0 * a(i, j));
+                const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int64_t k = 0; k < p; k++) {
+                    const double aki = a(k, i), akj = a(k, j);
+                    a(k, i) = c * aki - s * akj;
+                    a(k, j) = s * aki + c * akj;
+                }
+                for (int64_t k = 0; k < p; k++) {
+                    const double aik = a(i, k), ajk = a(j, k);
+                    a(i, k) = c * aik - s * ajk;
+                    a(j, k) = s * aik + c * ajk;
+                }
+                for (int64_t k = 0; k < p; k++) {
+                    const double vki = v(k, i), vkj = v(k, j);
+                    v(k, i) = c * vki - s * vkj;
+                    v(k, j) = s * vki + c * vkj;
+                }
+            }
+    }
+    double smax = 0.0;
+    for (int64_t i = 0; i < p; i++) smax = std::max(smax, std::fabs(a(i, i)));
+    Matrix out(p, p);
+    for (int64_t k = 0; k < p; k++) {
+        const double l = a(k, k);
+        if (std::fabs(l) <= 1e-15 * smax) continue;
+        for (int64_t i = 0; i < p; i++)
+            for (int64_t j = 0; j < p; j++) out(i, j) += v(i, k) * v(j, k) / l;
+    }
+    return out;
+}
+
+// C = A^T B (A n x p, B n x q)
+Matrix tmul(const Matrix& A, const Matrix& B) {
+    Matrix C(A.cols, B.cols);
+    for (int64_t i = 0; i < A.cols; i++)
+        for (int64_t j = 0; j < B.cols; j++) {
+            const double* a = A.col(i);
+            const double* b = B.col(j);
+            double s = 0.0;
+            for (int64_t r = 0; r < A.rows; r++) s += a[r] * b[r];
+            C(i, j) = s;
+        }
+    return C;
+}
+
+// y = A x (A n x p)
+std::vector<double> mul(const Matrix& A, const std::vector<double>& x) {
+    std::vector<double> y((size_t)A.rows, 0.0);
+    for (int64_t j = 0; j < A.cols; j++) {
+        const double* a = A.col(j);
+        const double xj = x[(size_t)j];
+        for (int64_t r = 0; r < A.rows; r++) y[(size_t)r] += a[r] * xj;
+    }
+    return y;
+}
+
+// y = A^T x
+std::vector<double> tmul(const Matrix& A, const std::vector<double>& x) {
+    std::vector<double> y((size_t)A.cols, 0.0);
+    for (int64_t j = 0; j < A.cols; j++) {
+        const double* a = A.col(j);
+        double s = 0.0;
+        for (int64_t r = 0; r < A.rows; r++) s += a[r] * x[(size_t)r];
+        y[(size_t)j] = s;
+    }
+    return y;
+}
+
+double dot(const double* a, const double* b, int64_t n) {
+    double s = 0.0;
+    for (int64_t i = 0; i < n; i++) s += a[i] * b[i];
+    return s;
+}
+
+double dot(const std::vector<double>& a, const std::vector<double>& b) {
+    return dot(a.data(), b.data(), (int64_t)a.size());
+}
+
+// ----------------------------------------------------------------- families
+
+double logistic(double eta) { return 1.0 / (1.0 + std::exp(-eta)); }
+double logistic_mu_eta(double eta) {
+    const double e = std::exp(-std::fabs(eta));
+    return e / ((1.0 + e) * (1.0 + e));
+}
+
+struct Family {
+    bool binomial;
+    double linkinv(double eta) const { return binomial ? logistic(eta) : eta; }
+    double mu_eta(double eta) const { return binomial ? logistic_mu_eta(eta) : 1.0; }
+    double variance(double mu) const { return binomial ? mu * (1.0 - mu) : 1.0; }
+};
+
+// ------------------------------------------------------------- the solvers
+
+struct Coefficients {
+    std::vector<double> Sigma_iY, alpha, eta;
+    Matrix Sigma_iX, cov;
+};
+
+// getCoefficients (FG.cpp:3167-3199): Sigma^-1 [Y, X] in one request
+Coefficients get_coefficients(Solver& s, const std::vector<double>& Y, const Matrix& X,
+                              const std::vector<double>& w, const double tau[2]) {
+    const int64_t n = X.rows, p = X.cols;
+    Matrix B(n, p + 1);
+    std::copy(Y.begin(), Y.end(), B.col(0));
+    std::copy(X.a.begin(), X.a.end(), B.col(1));
+    Matrix sol;
+    s.solve(w, tau, B, sol);
+    Coefficients c;
+    c.Sigma_iY.assign(sol.col(0), sol.col(0) + n);
+    c.Sigma_iX = Matrix(n, p);
+    std::copy(sol.col(1), sol.col(1) + n * p, c.Sigma_iX.a.begin());
+    c.cov = inv_sympd(tmul(X, c.Sigma_iX));
+    c.alpha = mul(c.cov, tmul(c.Sigma_iX, Y));
+    const std::vector<double> sxa = mul(c.Sigma_iX, c.alpha);
+    c.eta.resize((size_t)n);
+    for (int64_t i = 0; i < n; i++) c.eta[(size_t)i] = Y[(size_t)i] - tau[0] * (c.Sigma_iY[(size_t)i] - sxa[(size_t)i]) / w[(size_t)i];
+    return c;
+}
+
+struct CoefState {
+    std::vector<double> Y, alpha, eta, W, mu, Sigma_iY;
+    Matrix cov, Sigma_iX;
+};
+
+// Get_Coef (FG.R:2-35)
+CoefState get_coef(Solver& s, const std::vector<double>& y, const Matrix& X, const double tau[2], const Family& fam,
+                   std::vector<double> alpha0, const std::vector<double>& eta0, const std::vector<double>& offset,
+                   const Options& opt) {
+    const double tol_coef = 0.1;
+    const int64_t n = X.rows;
+    std::vector<double> Y((size_t)n), W((size_t)n), mu((size_t)n), eta;
+    auto working = [&](const std::vector<double>& e) {
+        for (int64_t i = 0; i < n; i++) {
+            const double m = fam.linkinv(e[(size_t)i]), me = fam.mu_eta(e[(size_t)i]);
+            mu[(size_t)i] = m;
+            Y[(size_t)i] = e[(size_t)i] - offset[(size_t)i] + (y[(size_t)i] - m) / me;
+            const double r = me / std::sqrt(fam.variance(m));
+            W[(size_t)i] = r * r;
+        }
+    };
+    working(eta0);
+    Coefficients rc;
+    std::vector<double> alpha;
+    for (int it = 0; it < opt.maxiter; it++) {
+        rc = get_coefficients(s, Y, X, W, tau);
+        alpha = rc.alpha;
+        eta = rc.eta;
+        for (int64_t i = 0; i < n; i++) eta[(size_t)i] += offset[(size_t)i];
+        working(eta);
+        double m = 0.0;
+        for (size_t j = 0; j < alpha.size(); j++)
+            m = std::max(m, std::fabs(alpha[j] - alpha0[j]) / (std::fabs(alpha[j]) + std::fabs(alpha0[j]) + tol_coef));
+        if (m < tol_coef) break;
+        alpha0 = alpha;
+    }
+    CoefState st;
+    st.Y = Y;
+    st.alpha = alpha;
+    st.eta = eta;
+    st.W = W;
+    st.mu = mu;
+    st.cov = rc.cov;
+    st.Sigma_iY = rc.Sigma_iY;
+    st.Sigma_iX = rc.Sigma_iX;
+    return st;
+}
+
+// P v = Sigma^-1 v - Sigma^-1 X cov (Sigma^-1 X)^T v
+std::vector<double> proj(const std::vector<double>& v, const double* Sigma_iv, const Matrix& Sigma_iX,
+                         const Matrix& cov) {
+    const std::vector<double> t = mul(Sigma_iX, mul(cov, tmul(Sigma_iX, v)));
+    std::vector<double> out(t.size());
+    for (size_t i = 0; i < t.size(); i++) out[i] = Sigma_iv[i] - t[i];
+    return out;
+}
+
+// b^T cov c for p-vectors
+double quad(const std::vector<double>& b, const Matrix& cov, const std::vector<double>& c) {
+    double s = 0.0;
+    for (int64_t i = 0; i < cov.rows; i++)
+        for (int64_t j = 0; j < cov.cols; j++) s += b[(size_t)i] * cov(i, j) * c[(size_t)j];
+    return s;
+}
+
+struct AIScore {
+    double YPAPY = 0.0, YPA0PY = 0.0;
+    double trace[2] = {0.0, 0.0};   // binary: trace[1] only
+    double AI[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    int nprobe = 0;
+};
+
+// getAIScore (FG.cpp:3279-3295) / getAIScore_q (FG.cpp:3479-3531) with
+// GetTrace / GetTrace_q (FG.cpp:3113-3160, 3409-3472).  Each round of probes
+// U is one product K [PY, U] (K U after the first round) and one solve
+// Sigma^-1 [U, APY, (PY)]; the probe terms are t_i = (K u_i)^T P u_i and, for
+// the quantitative trait, u_i^T P u_i.  A round that leaves the coefficient
+// of variation above the cutoff adds ten probes (FG.cpp:3147-3154).
+AIScore ai_score(Solver& s, const std::vector<double>& Y, const Matrix& X, const std::vector<double>& w,
+                 const double tau[2], const std::vector<double>& Sigma_iY, const Matrix& Sigma_iX, Matrix cov,
+                 const Options& opt, bool quantitative) {
+    const int64_t n = X.rows, p = X.cols;
+    if (quantitative) cov = inv_sympd(tmul(X, Sigma_iX));   // getAIScore_q recomputes cov (FG.cpp:3487-3494)
+    std::vector<double> PY = proj(Y, Sigma_iY.data(), Sigma_iX, cov);
+    std::vector<double> t1, t0, APY, SAPY, SPY;
+    int64_t start = 0, end = opt.nrun;
+    for (;;) {
+        const int64_t k = end - start;
+        const bool first = APY.empty();
+        Matrix U(n, k);
+        for (int64_t j = 0; j < k; j++) probe_vector(opt.probe_seed, start + j, n, U.col(j));
+        Matrix KB;
+        if (first) {
+            Matrix B(n, k + 1);
+            std::copy(PY.begin(), PY.end(), B.col(0));
+            std::copy(U.a.begin(), U.a.end(), B.col(1));
+            s.product(B, KB);
+        } else {
+            s.product(U, KB);
+        }
+        const int64_t u0 = first ? 1 : 0;   // first column of K U in KB
+        if (first) APY.assign(KB.col(0), KB.col(0) + n);
+        Matrix B2(n, k + (first ? 1 + (quantitative ? 1 : 0) : 0));
+        std::copy(U.a.begin(), U.a.end(), B2.col(0));
+        if (first) {
+            std::copy(APY.begin(), APY.end(), B2.col(k));
+            if (quantitative) std::copy(PY.begin(), PY.end(), B2.col(k + 1));
+        }
+        Matrix sol;
+        s.solve(w, tau, B2, sol);
+        if (first) {
+            SAPY.assign(sol.col(k), sol.col(k) + n);
+            if (quantitative) SPY.assign(sol.col(k + 1), sol.col(k + 1) + n);
+        }
+        for (int64_t j = 0; j < k; j++) {
+            const double* au = KB.col(u0 + j);
+            const double* su = sol.col(j);
+            const double* u = U.col(j);
+            const double d = dot(au, su, n);
+            std::vector<double> h((size_t)p), g((size_t)p);
+            for (int64_t c = 0; c < p; c++) {
+                h[(size_t)c] = dot(au, Sigma_iX.col(c), n);
+                g[(size_t)c] = dot(u, Sigma_iX.col(c), n);
+            }
+            t1.push_back(d - quad(h, cov, g));
+            if (quantitative) t0.push_back(dot(u, su, n) - quad(g, cov, g));
+        }
+        const double cv = cal_cv(t1);
+        const double cv0 = quantitative ? cal_cv(t0) : 0.0;
+        if (cv > opt.trace_cv_cutoff || (quantitative && cv0 > opt.trace_cv_cutoff)) {
+            start = end;
+            end = end + 10;
+            continue;
+        }
+        break;
+    }
+    AIScore r;
+    r.nprobe = (int)t1.size();
+    r.YPAPY = dot(PY, APY);
+    const std::vector<double> PAPY = proj(APY, SAPY.data(), Sigma_iX, cov);
+    const double mean1 = std::accumulate(t1.begin(), t1.end(), 0.0) / (double)t1.size();
+    if (!quantitative) {
+        r.trace[1] = mean1;
+        r.AI[1][1] = dot(APY, PAPY);
+    } else {
+        const std::vector<double> PA0PY = proj(PY, SPY.data(), Sigma_iX, cov);
+        r.YPA0PY = dot(PY, PY);
+        r.trace[0] = std::accumulate(t0.begin(), t0.end(), 0.0) / (double)t0.size();
+        r.trace[1] = mean1;
+        r.AI[0][0] = dot(PY, PA0PY);
+        r.AI[0][1] = r.AI[1][0] = dot(PY, PAPY);
+        r.AI[1][1] = dot(APY, PAPY);
+    }
+    return r;
+}
+
+// fitglmmaiRPCG (FG.cpp:3302-3341)
+void fitglmmai_rpcg(Solver& s, const CoefState& rc, const Matrix& X, double tau[2], const Options& opt) {
+    const AIScore re = ai_score(s, rc.Y, X, rc.W, tau, rc.Sigma_iY, rc.Sigma_iX, rc.cov, opt, false);
+    const double Dtau = (re.YPAPY - re.trace[1]) / re.AI[1][1];
+    const double t0[2] = {tau[0], tau[1]};
+    tau[1] = t0[1] + Dtau;
+    for (int i = 0; i < 2; i++)
+        if (tau[i] < opt.tol) tau[i] = 0.0;
+    double step = 1.0;
+    while (tau[1] < 0.0) {
+        step *= 0.5;
+        tau[1] = t0[1] + step * Dtau;
+    }
+    for (int i = 0; i < 2; i++)
+        if (tau[i] < opt.tol) tau[i] = 0.0;
+}
+
+// fitglmmaiRPCG_q (FG.cpp:3610-3662): 2x2 AI system, GMMAT zeroing
+void fitglmmai_rpcg_q(Solver& s, const CoefState& rc, const Matrix& X, double tau[2], const Options& opt) {
+    const bool zero[2] = {tau[0] < opt.tol, tau[1] < opt.tol};
+    const AIScore re = ai_score(s, rc.Y, X, rc.W, tau, rc.Sigma_iY, rc.Sigma_iX, rc.cov, opt, true);
+    Matrix AI(2, 2), score(2, 1);
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) AI(i, j) = re.AI[i][j];
+    score(0, 0) = re.YPA0PY - re.trace[0];
+    score(1, 0) = re.YPAPY - re.trace[1];
+    const Matrix D = solve_dense(AI, score);
+    const double t0[2] = {tau[0], tau[1]};
+    auto apply = [&](double step) {
+        for (int i = 0; i < 2; i++) {
+            tau[i] = t0[i] + step * D(i, 0);
+            if (zero[i] && tau[i] < opt.tol) tau[i] = 0.0;
+        }
+    };
+    apply(1.0);
+    double step = 1.0;
+    while (tau[0] < 0.0 || tau[1] < 0.0) {
+        step *= 0.5;
+        apply(step);
+    }
+    for (int i = 0; i < 2; i++)
+        if (tau[i] < opt.tol) tau[i] = 0.0;
+}
+
+double rel_change(const double a[2], const double b[2], double tol) {
+    double m = 0.0;
+    for (int i = 0; i < 2; i++) m = std::max(m, std::fabs(a[i] - b[i]) / (std::fabs(a[i]) + std::fabs(b[i]) + tol));
+    return m;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ public
+
+void probe_vector(uint64_t seed, int64_t i, int64_t n, double* out) {
+    auto mix = [](uint64_t z) {
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    };
+    const uint64_t ic = (uint64_t)i * 0x9E3779B97F4A7C15ull;
+    for (int64_t r = 0; r < n; r++) {
+        const uint64_t inner = mix((uint64_t)r * 0xC2B2AE3D27D4EB4Full + 0x5851F42Dull);
+        const uint64_t h = mix(seed ^ mix(ic + inner));
+        out[r] = (h >> 63) ? -1.0 : 1.0;
+    }
+}
+
+double cal_cv(const std::vector<double>& x) {
+    const double n = (double)x.size();
+    const double mean = std::accumulate(x.begin(), x.end(), 0.0) / n;
+    double ss = 0.0;
+    for (double v : x) ss += (v - mean) * (v - mean);
+    const double sd = std::sqrt(ss / (n - 1.0));
+    return (sd / mean) / n;
+}
+
+Matrix inv_sympd(const Matrix& A0) {
+    Matrix A = A0;
+    for (int64_t i = 0; i < A.rows; i++)
+        for (int64_t j = 0; j < i; j++) A(i, j) = A(j, i);   // symmatu
+    if (cholesky_ok(A)) return inverse(A);
+    return pinv_sym(A);
+}
+
+GlmStart glm_start(Trait trait, const std::vector<double>& y, const Matrix& X, const std::vector<double>& offset,
+                   int maxit, double epsilon) {
+    const int64_t n = X.rows, p = X.cols;
+    const Family fam{trait == Trait::Binary};
+    std::vector<double> off = offset.empty() ? std::vector<double>((size_t)n, 0.0) : offset;
+    std::vector<double> mu((size_t)n), eta((size_t)n);
+    bool binary01 = fam.binomial;
+    for (int64_t i = 0; i < n; i++) {
+        if (fam.binomial) {
+            mu[(size_t)i] = (y[(size_t)i] + 0.5) / 2.0;
+            eta[(size_t)i] = std::log(mu[(size_t)i] / (1.0 - mu[(size_t)i]));
+            if (y[(size_t)i] != 0.0 && y[(size_t)i] != 1.0) binary01 = false;
+        } else {
+            mu[(size_t)i] = eta[(size_t)i] = y[(size_t)i];
+        }
+    }
+    auto deviance = [&](const std::vector<double>& m) {
+        double d = 0.0;
+        for (int64_t i = 0; i < n; i++) {
+            const double yi = y[(size_t)i], mi = m[(size_t)i];
+            if (binary01) {
+                d += std::log(yi > 0 ? mi : 1.0 - mi);
+            } else if (fam.binomial) {
+                const double a = yi > 0 ? yi * std::log(yi / mi) : 0.0;
+                const double b = yi < 1 ? (1 - yi) * std::log((1 - yi) / (1 - mi)) : 0.0;
+                d += a + b;
+            } else {
+                d += (yi - mi) * (yi - mi);
+            }
+        }
+        if (binary01) return -2.0 * d;
+        return fam.binomial ? 2.0 * d : d;
+    };
+    double dev_old = deviance(mu);
+    std::vector<double> beta((size_t)p, 0.0);
+    for (int it = 0; it < maxit; it++) {
+        Matrix Xw(n, p);
+        std::vector<double> z((size_t)n);
+        for (int64_t i = 0; i < n; i++) {
+            const double me = fam.mu_eta(eta[(size_t)i]);
+            z[(size_t)i] = (eta[(size_t)i] - off[(size_t)i]) + (y[(size_t)i] - mu[(size_t)i]) / me;
+            const double wt = me * me / fam.variance(mu[(size_t)i]);
+            for (int64_t j = 0; j < p; j++) Xw(i, j) = X(i, j) * wt;
+        }
+        const Matrix A = tmul(X, Xw);
+        Matrix b(p, 1);
+        const std::vector<double> rhs = tmul(Xw, z);
+        std::copy(rhs.begin(), rhs.end(), b.a.begin());
+        const Matrix sol = solve_dense(A, b);
+        beta.assign(sol.a.begin(), sol.a.end());
+        eta = mul(X, beta);
+        for (int64_t i = 0; i < n; i++) {
+            eta[(size_t)i] += off[(size_t)i];
+            mu[(size_t)i] = fam.linkinv(eta[(size_t)i]);
+        }
+        const double dev = deviance(mu);
+        if (std::fabs(dev - dev_old) / (std::fabs(dev) + 0.1) < epsilon) break;
+        dev_old = dev;
+    }
+    return {beta, eta, mu};
+}
+
+Result fit_null(Solver& s, Trait trait, const std::vector<double>& y, const Matrix& X,
+                const std::vector<double>& offset_in, const Options& opt) {
+    const int64_t n = X.rows;
+    if ((int64_t)y.size() != n) throw std::invalid_argument("y and X differ in length");
+    const std::vector<double> offset = offset_in.empty() ? std::vector<double>((size_t)n, 0.0) : offset_in;
+    const bool quant = trait == Trait::Quantitative;
+    const Family fam{!quant};
+    const GlmStart fit0 = glm_start(trait, y, X, offset);
+    std::vector<double> eta = fit0.eta, alpha = fit0.coef;
+    Result res;
+    // starting values (FG.R:148-157, 389-398)
+    double tau[2] = {1.0, quant ? 0.0 : 0.1};
+    double tau0[2] = {tau[0], tau[1]};
+    CoefState rc = get_coef(s, y, X, tau, fam, alpha, eta, offset, opt);
+    const AIScore re = ai_score(s, rc.Y, X, rc.W, tau, rc.Sigma_iY, rc.Sigma_iX, rc.cov, opt, quant);
+    tau[1] = std::max(0.0, tau0[1] + tau0[1] * tau0[1] * (re.YPAPY - re.trace[1]) / (double)n);   // FG.R:165, 405
+    if (quant) tau[0] = std::max(0.0, tau0[0] + tau0[0] * tau0[0] * (re.YPA0PY - re.trace[0]) / (double)n);
+    res.theta_history.insert(res.theta_history.end(), tau, tau + 2);
+    int it = 0;
+    for (it = 1; it <= opt.maxiter; it++) {
+        const std::vector<double> alpha0 = quant ? alpha : rc.alpha;
+        tau0[0] = tau[0];
+        tau0[1] = tau[1];
+        rc = get_coef(s, y, X, tau, fam, alpha0, eta, offset, opt);
+        if (quant)
+            fitglmmai_rpcg_q(s, rc, X, tau, opt);
+        else
+            fitglmmai_rpcg(s, rc, X, tau, opt);
+        alpha = rc.alpha;
+        eta = rc.eta;
+        res.theta_history.insert(res.theta_history.end(), tau, tau + 2);
+        if (quant && tau[0] <= 0)
+            throw std::runtime_error("ERROR! The first variance component parameter estimate is 0");   // FG.R:449-451
+        if (tau[1] <= 0) break;                                                    // FG.R:200-206, 452-454
+        if (rel_change(tau, tau0, opt.tol) < opt.tol) break;                       // FG.R:206, 456
+        if (std::max(tau[0], tau[1]) > std::pow(opt.tol, -2)) {
+            it = opt.maxiter;
+            break;
+        }
+    }
+    if (it > opt.maxiter) it = opt.maxiter;   // the loop ran out
+    // final coefficients at the estimate (FG.R:218, 475)
+    rc = get_coef(s, y, X, tau, fam, quant ? alpha : rc.alpha, eta, offset, opt);
+    res.theta[0] = tau[0];
+    res.theta[1] = tau[1];
+    res.coefficients = rc.alpha;
+    res.linear_predictors = rc.eta;
+    res.fitted_values = rc.mu;
+    res.working_y = rc.Y;
+    res.cov = rc.cov;
+    res.converged = it < opt.maxiter;
+    res.iterations = it;
+    return res;
+}
+
+CovariateTransform covariate_transform(const Matrix& X0) {
+    auto qr = [](const Matrix& A, Matrix& Q, Matrix& R) {
+        const int64_t n = A.rows, p = A.cols;
+        Matrix a = A;
+        std::vector<double> taus((size_t)p, 0.0);
+        for (int64_t j = 0; j < p && j < n; j++) {
+            double xnorm = 0.0;
+            for (int64_t i = j + 1; i < n; i++) xnorm += a(i, j) * a(i, j);
+            xnorm = std::sqrt(xnorm);
+            const double alpha = a(j, j);
+            if (xnorm == 0.0) {
+                taus[(size_t)j] = 0.0;
+                continue;
+            }
+            const double beta = -std::copysign(std::hypot(alpha, xnorm), alpha);
+            taus[(size_t)j] = (beta - alpha) / beta;
+            const double scale = 1.0 / (alpha - beta);
+            for (int64_t i = j + 1; i < n; i++) a(i, j) *= scale;
+            a(j, j) = beta;
+            // apply H = I - tau v v^T (v = [1, a(j+1:, j)]) to the columns right of j
+            for (int64_t c = j + 1; c < p; c++) {
+                double s = a(j, c);
+                for (int64_t i = j + 1; i < n; i++) s += a(i, j) * a(i, c);
+                s *= taus[(size_t)j];
+                a(j, c) -= s;
+                for (int64_t i = j + 1; i < n; i++) a(i, c) -= s * a(i, j);
+            }
+        }
+        R = Matrix(p, p);
+        for (int64_t j = 0; j < p; j++)
+            for (int64_t i = 0; i <= j; i++) R(i, j) = a(i, j);
+        // Q = H_1 ... H_p applied to the first p columns of the identity
+        Q = Matrix(n, p);
+        for (int64_t j = 0; j < p; j++) Q(j, j) = 1.0;
+        for (int64_t j = p - 1; j >= 0; j--) {
+            const double t = taus[(size_t)j];
+            if (t == 0.0) continue;
+            for (int64_t c = 0; c < p; c++) {
+                double s = Q(j, c);
+                for (int64_t i = j + 1; i < n; i++) s += a(i, j) * Q(i, c);
+                s *= t;
+                Q(j, c) -= s;
+                for (int64_t i = j + 1; i < n; i++) Q(i, c) -= s * a(i, j);
+            }
+        }
+    };
+    CovariateTransform out;
+    Matrix Q, R;
+    qr(X0, Q, R);
+    double dmax = 1.0;
+    for (int64_t j = 0; j < R.rows; j++) dmax = std::max(dmax, std::fabs(R(j, j)));
+    std::vector<int> keep;
+    for (int64_t j = 0; j < R.rows; j++)
+        if (std::fabs(R(j, j)) > 1e-10 * dmax) keep.push_back((int)j);
+    if ((int64_t)keep.size() < X0.cols) {
+        Matrix Xk(X0.rows, (int64_t)keep.size());
+        for (size_t c = 0; c < keep.size(); c++) std::copy(X0.col(keep[c]), X0.col(keep[c]) + X0.rows, Xk.col((int64_t)c));
+        qr(Xk, Q, R);
+    }
+    const double sq = std::sqrt((double)X0.rows);
+    for (double& v : Q.a) v *= sq;
+    out.X = Q;
+    out.R = R;
+    out.kept = keep;
+    return out;
+}
+
+ScoreNull score_test_null(const std::vector<double>& mu, const std::vector<double>& V, const std::vector<double>& y,
+                          const Matrix& X) {
+    const int64_t n = X.rows, p = X.cols;
+    ScoreNull o;
+    o.XV = Matrix(p, n);
+    Matrix XVt(n, p);
+    for (int64_t j = 0; j < p; j++)
+        for (int64_t i = 0; i < n; i++) {
+            const double v = X(i, j) * V[(size_t)i];
+            o.XV(j, i) = v;
+            XVt(i, j) = v;
+        }
+    o.XVX = tmul(X, XVt);
+    o.XVX_inv = inverse(o.XVX);
+    o.XXVX_inv = Matrix(n, p);
+    for (int64_t j = 0; j < p; j++)
+        for (int64_t i = 0; i < n; i++) {
+            double s = 0.0;
+            for (int64_t k = 0; k < p; k++) s += X(i, k) * o.XVX_inv(k, j);
+            o.XXVX_inv(i, j) = s;
+        }
+    o.XVX_inv_XV = o.XXVX_inv;
+    for (int64_t j = 0; j < p; j++)
+        for (int64_t i = 0; i < n; i++) o.XVX_inv_XV(i, j) *= V[(size_t)i];
+    o.S_a.assign((size_t)p, 0.0);
+    for (int64_t j = 0; j < p; j++)
+        for (int64_t i = 0; i < n; i++) o.S_a[(size_t)j] += X(i, j) * (y[(size_t)i] - mu[(size_t)i]);
+    return o;
+}
+
+}  // namespace fit
+}  // namespace saige

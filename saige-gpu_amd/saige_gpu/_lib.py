@@ -32,8 +32,34 @@ _f32 = ctypes.c_float
 _f64 = ctypes.c_double
 _u64 = ctypes.c_uint64
 
+_pd = ctypes.POINTER(_f64)
+
+
+class FitOptions(ctypes.Structure):
+    """saige_fit_options"""
+    _fields_ = [("maxiter", _int), ("tol", _f64), ("nrun", _int), ("tol_pcg", _f64), ("maxiter_pcg", _int),
+                ("trace_cv_cutoff", _f64), ("probe_seed", _u64)]
+
+
+class FitSpec(ctypes.Structure):
+    """saige_fit_spec"""
+    _fields_ = [("trait", _int), ("n", _i64), ("subset", ctypes.POINTER(_i64)), ("y", _pd), ("X", _pd),
+                ("p", _int), ("offset", _pd)]
+
+
+class FitResult(ctypes.Structure):
+    """saige_fit_result"""
+    _fields_ = [("theta", _f64 * 2), ("coefficients", _pd), ("linear_predictors", _pd), ("fitted_values", _pd),
+                ("working_y", _pd), ("cov", _pd), ("converged", _int), ("iterations", _int), ("status", _int)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
+    "saige_fit_options_default": (None, [ctypes.POINTER(FitOptions)]),
+    "saige_fit_null_batch": (_int, [_c_p, _int, ctypes.POINTER(FitSpec), ctypes.POINTER(FitOptions), _int,
+                                    ctypes.POINTER(FitResult)]),
+    "saige_covariate_transform": (_int, [_pd, _i64, _int, _pd, _pd, ctypes.POINTER(_i32), ctypes.POINTER(_int)]),
+    "saige_score_test_null": (_int, [_pd, _pd, _pd, _pd, _i64, _int, _pd, _pd, _pd, _pd, _pd, _pd]),
     "saige_create": (_int, [_int, ctypes.POINTER(_c_p)]),
     "saige_destroy": (_int, [_c_p]),
     "saige_last_error": (ctypes.c_char_p, []),

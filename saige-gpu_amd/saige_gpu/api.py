@@ -442,6 +442,52 @@ class Session:
 
     LAUNCH_KINDS = ("gemv", "gemm_half", "gemm_full")
 
+    def fit_null_batch(self, jobs, opt=None, max_concurrent=16):
+        """Null fits through the library's C++ drivers (saige_fit_null_batch):
+        jobs are dicts with trait ("binary" / "quantitative"), y (n), X
+        (n x p, covariate-transformed), and optionally subset (ascending
+        indices into the N samples) and offset.  opt: driver.Options-like
+        (maxiter, tol, nrun, tolPCG, maxiterPCG, traceCVcutoff) or None for
+        the defaults.  Returns one dict per job (its "status" is 0 on
+        success)."""
+        o = _lib.FitOptions()
+        self._L.saige_fit_options_default(ctypes.byref(o))
+        if opt is not None:
+            o.maxiter, o.tol, o.nrun = int(opt.maxiter), float(opt.tol), int(opt.nrun)
+            o.tol_pcg, o.maxiter_pcg, o.trace_cv_cutoff = float(opt.tolPCG), int(opt.maxiterPCG), float(opt.traceCVcutoff)
+        n_fits = len(jobs)
+        specs = (_lib.FitSpec * n_fits)()
+        results = (_lib.FitResult * n_fits)()
+        keep, outs = [], []
+        pd = ctypes.POINTER(ctypes.c_double)
+        for i, j in enumerate(jobs):
+            y = np.ascontiguousarray(j["y"], np.float64)
+            X = np.asfortranarray(j["X"], np.float64)
+            n, p = X.shape
+            sub = None if j.get("subset") is None else np.ascontiguousarray(j["subset"], np.int64)
+            off = None if j.get("offset") is None else np.ascontiguousarray(j["offset"], np.float64)
+            keep += [y, X, sub, off]
+            specs[i].trait = 0 if j["trait"] == "binary" else 1
+            specs[i].n, specs[i].p = n, p
+            specs[i].subset = None if sub is None else sub.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+            specs[i].y, specs[i].X = y.ctypes.data_as(pd), X.ctypes.data_as(pd)
+            specs[i].offset = None if off is None else off.ctypes.data_as(pd)
+            o_ = {"coefficients": np.zeros(p), "linear_predictors": np.zeros(n), "fitted_values": np.zeros(n),
+                  "Y": np.zeros(n), "cov": np.zeros((p, p), order="F")}
+            outs.append(o_)
+            results[i].coefficients = o_["coefficients"].ctypes.data_as(pd)
+            results[i].linear_predictors = o_["linear_predictors"].ctypes.data_as(pd)
+            results[i].fitted_values = o_["fitted_values"].ctypes.data_as(pd)
+            results[i].working_y = o_["Y"].ctypes.data_as(pd)
+            results[i].cov = o_["cov"].ctypes.data_as(pd)
+        check(self._L.saige_fit_null_batch(self.ctx, n_fits, specs, ctypes.byref(o), int(max_concurrent), results))
+        for i, o_ in enumerate(outs):
+            r = results[i]
+            o_.update({"theta": np.array([r.theta[0], r.theta[1]]), "converged": bool(r.converged),
+                       "iterations": int(r.iterations), "status": int(r.status),
+                       "traitType": jobs[i]["trait"]})
+        return outs
+
     def engine_stats(self, kind=None):
         """Device seconds and launch count of the GRM product launches: for one
         kind ("gemv", "gemm_half", "gemm_full") or summed over all."""
@@ -486,3 +532,18 @@ class Session:
         src = ctypes.c_void_p(dev.value + int(offset))
         check(self._L.saige_copy_device_to_host(self.ctx, src, int(nbytes), ptr(out)))
         return out
+
+
+def covariate_transform_native(X):
+    """Covariate_Transform (FG.R:1612-1647) by the library's C++ code:
+    returns (X_new, R, kept_columns) as saige_gpu.plink.covariate_transform."""
+    L = _lib.load()
+    X = np.asfortranarray(X, np.float64)
+    n, p = X.shape
+    Xo, R = np.zeros(n * p), np.zeros(p * p)
+    kept, pk = np.zeros(p, np.int32), ctypes.c_int(0)
+    pd = ctypes.POINTER(ctypes.c_double)
+    check(L.saige_covariate_transform(X.ctypes.data_as(pd), n, p, Xo.ctypes.data_as(pd), R.ctypes.data_as(pd),
+                                      kept.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(pk)))
+    k = pk.value
+    return Xo[:n * k].reshape((n, k), order="F"), R[:k * k].reshape((k, k), order="F"), kept[:k].copy()

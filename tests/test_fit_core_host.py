@@ -1,0 +1,173 @@
+"""The C++ null-fit driver (saige-gpu_amd/csrc/fit_core.cpp: FG.R:2-35,
+79-304, 309-549, 579-591, 1612-1647 with FG.cpp:3113-3160, 3167-3199,
+3279-3341, 3409-3662) against the Python driver (saige_gpu.driver) on the
+CPU.  Both answer every solve with an exact dense FP64 solve of
+Sigma = tau0 diag(1/w) + tau1 K and every product with K b (the C++ side
+through the test harness tests/native/fit_dense.cpp, compiled here with g++),
+so the fits must agree to FP64 rounding: same iteration counts, same tau
+path, same coefficients.  The Python driver is itself pinned to the oracle's
+restatement of FG.R (test_host_driver.py)."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from fixtures import random_bed
+from oracle import oracle as O
+from saige_gpu import driver as D
+from saige_gpu import plink
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(ROOT, "saige-gpu_amd", "csrc")
+P = ctypes.POINTER
+
+
+@pytest.fixture(scope="module")
+def lib(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("fitdense") / "libfitdense.so")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-I", CSRC,
+                    os.path.join(HERE, "native", "fit_dense.cpp"), os.path.join(CSRC, "fit_core.cpp"),
+                    "-o", out], check=True)
+    L = ctypes.CDLL(out)
+    d, i64, u64 = P(ctypes.c_double), ctypes.c_int64, ctypes.c_uint64
+    L.fitnull_dense.argtypes = [ctypes.c_int, i64, ctypes.c_int, d, d, d, ctypes.c_int, ctypes.c_double,
+                                ctypes.c_int, ctypes.c_double, u64, d, d, d, d, d, d, P(ctypes.c_int),
+                                P(ctypes.c_int), d, P(ctypes.c_int)]
+    L.probes.argtypes = [u64, i64, i64, d]
+    L.calcv.argtypes = [d, i64]
+    L.calcv.restype = ctypes.c_double
+    L.covtrans.argtypes = [d, i64, ctypes.c_int, d, d, P(ctypes.c_int), P(ctypes.c_int)]
+    L.scorenull.argtypes = [d, d, d, d, i64, ctypes.c_int, d, d, d, d]
+    return L
+
+
+def ptr(a):
+    return a.ctypes.data_as(P(ctypes.c_double))
+
+
+class ExactExecutor:
+    """Answers the Python coroutines' requests with exact dense FP64 algebra."""
+
+    def __init__(self, K):
+        self.K = K
+
+    def solve(self, w, tau, B):
+        Sig = tau[0] * np.diag(1.0 / np.asarray(w)) + tau[1] * self.K
+        return np.linalg.solve(Sig, B)
+
+    def run(self, gen):
+        value = None
+        while True:
+            try:
+                req = gen.send(value)
+            except StopIteration as e:
+                return e.value
+            if req[0] == "solve":
+                value = self.solve(*req[1:])
+            elif req[0] == "matvec":
+                value = self.K @ req[1]
+            else:
+                value = D.trace_round_host(req, lambda B: self.K @ B, self.solve)
+
+
+@pytest.fixture(scope="module")
+def cohort():
+    N, M = 160, 500
+    body, G, miss = random_bed(N, M, 0.005, seed=31, maf_lo=0.05, maf_hi=0.5)
+    g = O.setgeno(body, N, M, min_maf=0.01)
+    K = O.grm_f64(g) / g["Mqc"]
+    rng = np.random.default_rng(7)
+    X = np.column_stack([np.ones(N), rng.normal(50, 10, N), rng.random(N) < 0.5])
+    Xt, _, _ = plink.covariate_transform(X)
+    gz = O.std_matrix(g).astype(np.float64)[:, :100] @ rng.normal(0, np.sqrt(0.5 / 100), 100)
+    yb = (rng.random(N) < 1 / (1 + np.exp(-(-0.8 + gz)))).astype(float)
+    yq = gz + rng.normal(0, np.sqrt(0.5), N)
+    return K, np.asfortranarray(Xt), yb, yq
+
+
+def run_cpp(lib, trait, K, y, X, opt):
+    n, p = X.shape
+    theta, coef = np.zeros(2), np.zeros(p)
+    eta, mu, Yw, cov = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros((p, p), order="F")
+    iters, conv, nh = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    hist = np.zeros(2 * (opt.maxiter + 2))
+    Kf = np.asfortranarray(K)
+    rc = lib.fitnull_dense(0 if trait == "binary" else 1, n, p, ptr(Kf), ptr(np.ascontiguousarray(y)), ptr(X),
+                           opt.maxiter, opt.tol, opt.nrun, opt.traceCVcutoff, 200, ptr(theta), ptr(coef), ptr(eta),
+                           ptr(mu), ptr(Yw), ptr(cov), ctypes.byref(iters), ctypes.byref(conv), ptr(hist),
+                           ctypes.byref(nh))
+    assert rc == 0
+    return {"theta": theta, "coefficients": coef, "linear_predictors": eta, "fitted_values": mu, "Y": Yw, "cov": cov,
+            "iterations": iters.value, "converged": bool(conv.value), "history": hist[:2 * nh.value].reshape(-1, 2)}
+
+
+@pytest.mark.parametrize("trait,cv_cutoff", [("binary", 1.0), ("quantitative", 1.0), ("binary", 0.0025),
+                                              ("quantitative", 0.0025)])
+def test_cpp_driver_matches_python_driver(lib, cohort, trait, cv_cutoff):
+    K, X, yb, yq = cohort
+    y = yb if trait == "binary" else yq
+    opt = D.Options(nrun=10, traceCVcutoff=cv_cutoff)
+    probes = D.RademacherProbes(len(y), 200)
+    ref = ExactExecutor(K).run(D.fit_co(trait, y, X, probes, opt))
+    got = run_cpp(lib, trait, K, y, X, opt)
+    assert got["iterations"] == ref["iterations"] and got["converged"] == ref["converged"]
+    assert np.allclose(got["history"], np.array(ref["history"]), rtol=1e-9, atol=1e-13)
+    assert np.allclose(got["theta"], ref["theta"], rtol=1e-9, atol=1e-13)
+    assert ref["theta"][1] > 0                      # a fit that moved off the start values
+    for k in ("coefficients", "linear_predictors", "fitted_values", "Y", "cov"):
+        assert np.allclose(got[k], ref[k], rtol=1e-8, atol=1e-11), k
+
+
+def test_cpp_probes_and_cv(lib):
+    for i in (0, 7, 123):
+        a = np.zeros(257)
+        lib.probes(200, i, 257, ptr(a))
+        assert np.array_equal(a, D.probe_vector(200, i, 257))
+    x = np.random.default_rng(2).normal(3.0, 0.5, 37)
+    assert np.isclose(lib.calcv(ptr(x), len(x)), D.cal_cv(x), rtol=1e-13)
+
+
+def test_cpp_covariate_transform_and_score_null(lib):
+    rng = np.random.default_rng(4)
+    N = 211
+    X = np.column_stack([np.ones(N), rng.normal(50, 10, N), rng.random(N) < 0.5])
+    for Xin in (X, np.column_stack([X, 2 * X[:, 1] - X[:, 0]])):       # the second has a collinear column
+        Xin = np.asfortranarray(Xin)
+        p = Xin.shape[1]
+        Xo, R = np.zeros(N * p), np.zeros(p * p)          # flat: the C++ side writes N x k, k x k
+        kept, pk = np.zeros(p, np.int32), ctypes.c_int(0)
+        lib.covtrans(ptr(Xin), N, p, ptr(Xo), ptr(R), kept.ctypes.data_as(P(ctypes.c_int)), ctypes.byref(pk))
+        Xt, Rt, keep = plink.covariate_transform(Xin)
+        k = pk.value
+        assert list(kept[:k]) == list(keep)
+        Xo = Xo[:N * k].reshape((N, k), order="F")
+        Ro = R[:k * k].reshape((k, k), order="F")
+        assert np.allclose(Xo, Xt, rtol=1e-10, atol=1e-10)
+        assert np.allclose(Ro, Rt, rtol=1e-10, atol=1e-10)
+    mu = rng.uniform(0.1, 0.9, N)
+    y = (rng.random(N) < mu).astype(float)
+    Xt = np.asfortranarray(Xt)
+    p = Xt.shape[1]
+    XVX, XVXi, S_a, XVXiXV = (np.zeros((p, p), order="F"), np.zeros((p, p), order="F"), np.zeros(p),
+                              np.zeros((N, p), order="F"))
+    V = mu * (1 - mu)
+    lib.scorenull(ptr(mu), ptr(V), ptr(y), ptr(Xt), N, p, ptr(XVX), ptr(XVXi), ptr(S_a), ptr(XVXiXV))
+    ref = D.score_test_null(mu, V, y, Xt)
+    for got, k in ((XVX, "XVX"), (XVXi, "XVX_inv"), (S_a, "S_a"), (XVXiXV, "XVX_inv_XV")):
+        assert np.allclose(got, ref[k], rtol=1e-10, atol=1e-12), k
+
+
+def test_library_covariate_transform_export():
+    """The same code exported by libsaigehip (saige_covariate_transform),
+    called without a device."""
+    from saige_gpu.api import covariate_transform_native
+    rng = np.random.default_rng(8)
+    N = 97
+    X = np.column_stack([np.ones(N), rng.normal(50, 10, N), rng.random(N) < 0.5, np.ones(N) * 3])
+    Xt, R, keep = covariate_transform_native(X)
+    Xr, Rr, keepr = plink.covariate_transform(X)
+    assert list(keep) == list(keepr) == [0, 1, 2]
+    assert np.allclose(Xt, Xr, rtol=1e-10, atol=1e-10) and np.allclose(R, Rr, rtol=1e-10, atol=1e-10)

@@ -96,6 +96,41 @@ def test_subset_fit_equals_restricted_problem(session, cohort):
     assert np.allclose(handles[0].result["theta"], ref["theta"], rtol=1e-3, atol=1e-6)
 
 
+def test_native_driver_batch(session, cohort):
+    """saige_fit_null_batch (the C++ drivers and scheduler) on a mix of
+    binary and quantitative fits plus a sample subset: same tau and
+    iteration count as the Python batch scheduler (both run the same
+    engine columns; host FP64 arithmetic differs in summation order only),
+    within the north-star bound of the FP64 oracle, and each fit's bits the
+    same alone and in the mix."""
+    c, grm, X, phenos = cohort
+    print("case: native C++ batched fits", flush=True)
+    opt = D.Options()
+    rng = np.random.default_rng(4)
+    sub = np.sort(rng.choice(c["N"], 1100, replace=False))
+    Xs, _, _ = plink.covariate_transform(c["X"][sub])
+    jobs = [{"trait": t, "y": y, "X": X, "tag": k} for k, (t, y) in phenos.items()]
+    jobs.append({"trait": "binary", "y": phenos["b20"][1][sub], "X": Xs, "subset": sub, "tag": "b20sub"})
+    nat = session.fit_null_batch(jobs, opt)
+    assert all(r["status"] == 0 for r in nat), [r["status"] for r in nat]
+    handles, _ = D.fit_batched(session, jobs, opt)
+    for r, h, j in zip(nat, handles, jobs):
+        ref = h.result
+        print(f"  {j['tag']}: native theta {r['theta']} python {ref['theta']} iters {r['iterations']}", flush=True)
+        assert r["iterations"] == ref["iterations"], j["tag"]
+        assert np.allclose(r["theta"], ref["theta"], rtol=1e-5, atol=1e-8), j["tag"]
+        assert np.allclose(r["coefficients"], ref["coefficients"], rtol=1e-5, atol=1e-7), j["tag"]
+        assert np.allclose(r["fitted_values"], ref["fitted_values"], rtol=1e-5, atol=1e-8), j["tag"]
+    for name in ("b20", "q40"):
+        i = list(phenos).index(name)
+        ref = _oracle_fit(grm, phenos[name][0], phenos[name][1], X, opt)
+        assert np.allclose(nat[i]["theta"], ref["theta"], rtol=1e-3, atol=1e-6), name
+    assert nat[0]["theta"][1] > 0
+    alone = session.fit_null_batch(jobs[:1], opt)
+    assert np.array_equal(alone[0]["theta"], nat[0]["theta"])
+    assert np.array_equal(alone[0]["coefficients"], nat[0]["coefficients"])
+
+
 def test_variance_ratio_vs_oracle(session):
     """Null fit + variance ratio (FG.R:2152-2423) with variance-ratio markers
     held out of the GRM (FG.cpp:545-547), against the FP64 oracle on the
