@@ -3,7 +3,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <numeric>
+#include <tuple>
 
 namespace saige {
 namespace fit {
@@ -268,6 +272,32 @@ double quad(const std::vector<double>& b, const Matrix& cov, const std::vector<d
     return s;
 }
 
+// Probe vectors are the same at every GetTrace call (FG.cpp:3114), so they
+// are kept process-wide per (seed, probe index, length), up to 1 GiB.
+const std::vector<double>& cached_probe(uint64_t seed, int64_t i, int64_t n) {
+    static std::mutex mu;
+    static std::map<std::tuple<uint64_t, int64_t, int64_t>, std::shared_ptr<std::vector<double>>> cache;
+    static int64_t bytes = 0;
+    thread_local std::vector<double> scratch;
+    const auto key = std::make_tuple(seed, i, n);
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) return *it->second;
+    }
+    auto v = std::make_shared<std::vector<double>>((size_t)n);
+    probe_vector(seed, i, n, v->data());
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return *it->second;
+    if (bytes + 8 * n > (int64_t(1) << 30)) {
+        scratch.swap(*v);
+        return scratch;
+    }
+    bytes += 8 * n;
+    return *cache.emplace(key, v).first->second;
+}
+
 struct AIScore {
     double YPAPY = 0.0, YPA0PY = 0.0;
     double trace[2] = {0.0, 0.0};   // binary: trace[1] only
@@ -293,42 +323,50 @@ AIScore ai_score(Solver& s, const std::vector<double>& Y, const Matrix& X, const
         const int64_t k = end - start;
         const bool first = APY.empty();
         Matrix U(n, k);
-        for (int64_t j = 0; j < k; j++) probe_vector(opt.probe_seed, start + j, n, U.col(j));
-        Matrix KB;
+        for (int64_t j = 0; j < k; j++) {
+            const std::vector<double>& pv = cached_probe(opt.probe_seed, start + j, n);
+            std::copy(pv.begin(), pv.end(), U.col(j));
+        }
+        // K [PY, U] (K U after the first round), then Sigma^-1 [U, K PY, (PY)]
+        Matrix KB, sol;
+        std::vector<int> src;
         if (first) {
             Matrix B(n, k + 1);
             std::copy(PY.begin(), PY.end(), B.col(0));
             std::copy(U.a.begin(), U.a.end(), B.col(1));
-            s.product(B, KB);
+            for (int64_t j = 0; j < k; j++) src.push_back((int)(1 + j));
+            src.push_back(-1);                      // K PY
+            if (quantitative) src.push_back(0);     // PY
+            s.product_then_solve(B, KB, w, tau, src, sol);
         } else {
-            s.product(U, KB);
+            for (int64_t j = 0; j < k; j++) src.push_back((int)j);
+            s.product_then_solve(U, KB, w, tau, src, sol);
         }
         const int64_t u0 = first ? 1 : 0;   // first column of K U in KB
         if (first) APY.assign(KB.col(0), KB.col(0) + n);
-        Matrix B2(n, k + (first ? 1 + (quantitative ? 1 : 0) : 0));
-        std::copy(U.a.begin(), U.a.end(), B2.col(0));
-        if (first) {
-            std::copy(APY.begin(), APY.end(), B2.col(k));
-            if (quantitative) std::copy(PY.begin(), PY.end(), B2.col(k + 1));
-        }
-        Matrix sol;
-        s.solve(w, tau, B2, sol);
         if (first) {
             SAPY.assign(sol.col(k), sol.col(k) + n);
             if (quantitative) SPY.assign(sol.col(k + 1), sol.col(k + 1) + n);
         }
         for (int64_t j = 0; j < k; j++) {
+            // d = (K u)^T Sigma^-1 u, e = u^T Sigma^-1 u, h = (Sigma^-1 X)^T K u,
+            // g = (Sigma^-1 X)^T u, in one pass over the samples
             const double* au = KB.col(u0 + j);
             const double* su = sol.col(j);
             const double* u = U.col(j);
-            const double d = dot(au, su, n);
-            std::vector<double> h((size_t)p), g((size_t)p);
-            for (int64_t c = 0; c < p; c++) {
-                h[(size_t)c] = dot(au, Sigma_iX.col(c), n);
-                g[(size_t)c] = dot(u, Sigma_iX.col(c), n);
+            double d = 0.0, e = 0.0;
+            std::vector<double> h((size_t)p, 0.0), g((size_t)p, 0.0);
+            for (int64_t r = 0; r < n; r++) {
+                d += au[r] * su[r];
+                e += u[r] * su[r];
+                for (int64_t c = 0; c < p; c++) {
+                    const double sx = Sigma_iX.a[(size_t)(c * n + r)];
+                    h[(size_t)c] += au[r] * sx;
+                    g[(size_t)c] += u[r] * sx;
+                }
             }
             t1.push_back(d - quad(h, cov, g));
-            if (quantitative) t0.push_back(dot(u, su, n) - quad(g, cov, g));
+            if (quantitative) t0.push_back(e - quad(g, cov, g));
         }
         const double cv = cal_cv(t1);
         const double cv0 = quantitative ? cal_cv(t0) : 0.0;
@@ -412,6 +450,21 @@ double rel_change(const double a[2], const double b[2], double tol) {
 }  // namespace
 
 // ------------------------------------------------------------------ public
+
+Matrix gather_columns(const Matrix& B, const Matrix& KB, const std::vector<int>& src) {
+    Matrix C(B.rows, (int64_t)src.size());
+    for (size_t c = 0; c < src.size(); c++) {
+        const double* from = src[c] >= 0 ? B.col(src[c]) : KB.col(-1 - src[c]);
+        std::copy(from, from + B.rows, C.col((int64_t)c));
+    }
+    return C;
+}
+
+void Solver::product_then_solve(const Matrix& B, Matrix& KB, const std::vector<double>& w, const double tau[2],
+                                const std::vector<int>& src, Matrix& X) {
+    product(B, KB);
+    solve(w, tau, gather_columns(B, KB, src), X);
+}
 
 void probe_vector(uint64_t seed, int64_t i, int64_t n, double* out) {
     auto mix = [](uint64_t z) {

@@ -52,7 +52,16 @@ public:
     virtual void solve(const std::vector<double>& w, const double tau[2], const Matrix& B, Matrix& X) = 0;
     // KB = K B
     virtual void product(const Matrix& B, Matrix& KB) = 0;
+    // KB = K B, then X = Sigma^-1 C where column c of C is B[:, src[c]] for
+    // src[c] >= 0 and KB[:, -1 - src[c]] otherwise (a trace round: K [PY, U]
+    // then Sigma^-1 [U, K PY, PY]).  The default runs the two in turn; the
+    // engine's solver starts the solves as soon as the products are done.
+    virtual void product_then_solve(const Matrix& B, Matrix& KB, const std::vector<double>& w, const double tau[2],
+                                    const std::vector<int>& src, Matrix& X);
 };
+
+// C from B, KB and src as product_then_solve defines it.
+Matrix gather_columns(const Matrix& B, const Matrix& KB, const std::vector<int>& src);
 
 enum class Trait { Binary = 0, Quantitative = 1 };
 

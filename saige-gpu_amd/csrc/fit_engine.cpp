@@ -40,6 +40,12 @@ struct Request {
     int64_t remaining = 0;        // columns not yet done
     bool done = false;
     std::string error;
+    // product_then_solve: when the products are done, the scheduler turns the
+    // request into the solve of C = gather_columns(B, KB, src) into chain_out
+    bool chain = false;
+    std::vector<int> src;
+    Matrix C;
+    Matrix* chain_out = nullptr;
 };
 
 struct FitTask : saige::fit::Solver {
@@ -53,10 +59,13 @@ struct FitTask : saige::fit::Solver {
     saige::fit::Result result;
     std::string error;
     bool finished = false;
+    int64_t posts = 0;            // requests posted so far
     std::condition_variable cv;   // the fit waits here for its request
 
     void solve(const std::vector<double>& w, const double tau[2], const Matrix& B, Matrix& X) override;
     void product(const Matrix& B, Matrix& KB) override;
+    void product_then_solve(const Matrix& B, Matrix& KB, const std::vector<double>& w, const double tau[2],
+                            const std::vector<int>& src, Matrix& X) override;
     void submit(Request& r);
 };
 
@@ -84,6 +93,7 @@ struct Scheduler {
             return;
         }
         inbox.emplace_back(f, r);
+        f->posts++;
         cv.notify_all();
         f->cv.wait(lk, [&] { return r->done; });
     }
@@ -115,6 +125,23 @@ void FitTask::product(const Matrix& B, Matrix& KB) {
     submit(r);
 }
 
+void FitTask::product_then_solve(const Matrix& B, Matrix& KB, const std::vector<double>& w, const double tau[2],
+                                 const std::vector<int>& src, Matrix& X) {
+    Request r;
+    r.solve = false;
+    r.B = &B;
+    r.out = &KB;
+    r.w = w;
+    r.tau[0] = tau[0];
+    r.tau[1] = tau[1];
+    r.chain = true;
+    r.src = src;
+    r.chain_out = &X;
+    KB = Matrix(B.rows, B.cols);
+    X = Matrix(B.rows, (int64_t)src.size());
+    submit(r);
+}
+
 int check(int rc) {
     if (rc != SAIGE_OK) throw std::runtime_error(saige_last_error());
     return rc;
@@ -140,6 +167,8 @@ void run_batch(Scheduler& sch, std::vector<std::unique_ptr<FitTask>>& fits, int 
     std::vector<float> colbuf;
     std::vector<uint8_t> mask((size_t)N);
     std::vector<float> wfull((size_t)N);
+    // fits whose request the last step completed, with their post count then
+    std::vector<std::pair<FitTask*, int64_t>> resumed;
 
     auto set_system = [&](FitTask* f, const double* w, const double tau[2]) {
         const saige_fit_spec* s = f->spec;
@@ -218,7 +247,7 @@ void run_batch(Scheduler& sch, std::vector<std::unique_ptr<FitTask>>& fits, int 
             Column c = pending.front();
             pending.pop_front();
             FitTask* f = c.fit;
-            if (c.req->solve) {
+            if (c.req->solve || c.req->chain) {
                 // a fit has one request in flight, so its system is free to change
                 const bool same = f->sys_set && f->set_tau[0] == c.req->tau[0] && f->set_tau[1] == c.req->tau[1] &&
                                   f->set_w == c.req->w;
@@ -264,7 +293,26 @@ void run_batch(Scheduler& sch, std::vector<std::unique_ptr<FitTask>>& fits, int 
                                             colbuf.data()));
         }
         if (live == 0) continue;
-        check(saige_engine_step(sch.ctx, 1, done.data(), iters.data()));
+        // the step runs on a helper thread while the fits that the previous
+        // step resumed compute their next request, so that its columns join
+        // the following iteration; nothing else calls the library meanwhile
+        int step_rc = SAIGE_OK;
+        std::string step_err;
+        std::thread stepper([&] {
+            step_rc = saige_engine_step(sch.ctx, 1, done.data(), iters.data());
+            if (step_rc != SAIGE_OK) step_err = saige_last_error();   // the message is per thread
+        });
+        if (!resumed.empty()) {
+            std::unique_lock<std::mutex> lk(sch.mu);
+            sch.cv.wait(lk, [&] {
+                for (auto& fp : resumed)
+                    if (!fp.first->finished && fp.first->posts <= fp.second) return false;
+                return true;
+            });
+            resumed.clear();
+        }
+        stepper.join();
+        if (step_rc != SAIGE_OK) throw std::runtime_error(step_err);
         std::vector<int32_t> fin;
         for (int s = 0; s < sch.width; s++)
             if (owner[(size_t)s].req && done[(size_t)s]) fin.push_back(s);
@@ -284,8 +332,21 @@ void run_batch(Scheduler& sch, std::vector<std::unique_ptr<FitTask>>& fits, int 
             double* o = c.req->out->col(c.j);
             for (int64_t i = 0; i < s->n; i++) o[i] = (double)x[sample_of(s, i)];
             if (--c.req->remaining == 0) {
-                c.req->done = true;
-                c.fit->cv.notify_all();
+                Request* r = c.req;
+                if (r->chain && !r->solve) {
+                    // the products are done: queue the solves now, the fit sleeps on
+                    r->C = saige::fit::gather_columns(*r->B, *r->out, r->src);
+                    r->solve = true;
+                    r->B = &r->C;
+                    r->out = r->chain_out;
+                    r->remaining = r->C.cols;
+                    for (int64_t j = 0; j < r->C.cols; j++) pending.push_back(Column{c.fit, r, j});
+                }
+                if (r->remaining == 0) {
+                    r->done = true;
+                    resumed.emplace_back(c.fit, c.fit->posts);
+                    c.fit->cv.notify_all();
+                }
             }
         }
     }
