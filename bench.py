@@ -208,7 +208,7 @@ def c4_leg(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--N", type=int, default=100_000)
     ap.add_argument("--M", type=int, default=150_000)
