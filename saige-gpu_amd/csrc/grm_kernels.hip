@@ -274,17 +274,21 @@ __global__ __launch_bounds__(256) void k_syrk_pk(const uint8_t* __restrict__ tp,
 // that of k_syrk_pk, so both give bitwise equal S.
 constexpr int BT2 = 256;
 
+// Store a 256x256 tile from waves of (32*MI) x 128 (wave (wr, wc) owns rows
+// wr*32*MI + [0, 32*MI) and columns wc*128 + [0, 128)), mirrored for
+// off-diagonal tiles; accumulate: +1 add to S, -1 subtract from S, 0 store.
+template <int MI>
 __device__ __forceinline__ void store_tile256(float* S, int64_t ldS, int64_t bi, int64_t bj, int wr, int wc,
-                                              int lane, const f32x16 (&acc)[2][4], int accumulate) {
+                                              int lane, const f32x16 (&acc)[MI][4], int accumulate) {
     const bool mirror = bi != bj;
 #pragma unroll
-    for (int mi = 0; mi < 2; mi++) {
+    for (int mi = 0; mi < MI; mi++) {
 #pragma unroll
         for (int ni = 0; ni < 4; ni++) {
             const int64_t col = bj * BT2 + wc * 128 + ni * 32 + (lane & 31);
 #pragma unroll
             for (int g = 0; g < 4; g++) {
-                const int64_t row0 = bi * BT2 + wr * 64 + mi * 32 + 8 * g + 4 * (lane >> 5);
+                const int64_t row0 = bi * BT2 + wr * 32 * MI + mi * 32 + 8 * g + 4 * (lane >> 5);
                 float v[4];
 #pragma unroll
                 for (int e = 0; e < 4; e++) v[e] = acc[mi][ni][4 * g + e];

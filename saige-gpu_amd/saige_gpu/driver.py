@@ -1207,11 +1207,23 @@ def fit_batched(sess, jobs, opt=None, width=64, n_systems=16, probe_seed=200):
     with keys trait, y, X and optionally subset (sample indices) and tag."""
     opt = opt or Options()
     sch = BatchScheduler(sess, opt, width=width, n_systems=n_systems)
+    # the glm() start values (host IRLS, no GRM) of all fits at once: numpy
+    # releases the GIL in the O(n) work, so a few threads overlap them
+    fams = {"binary": Binomial, "quantitative": Gaussian}
+
+    def start(j):
+        return glm_start(np.asarray(j["y"], np.float64), np.asarray(j["X"], np.float64), fams[j["trait"]])
+
+    if len(jobs) > 1:
+        with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as pool:
+            fit0s = list(pool.map(start, jobs))
+    else:
+        fit0s = [start(j) for j in jobs]
     handles = []
-    for j in jobs:
+    for j, fit0 in zip(jobs, fit0s):
         n = len(j["y"])
         probes = RademacherProbes.shared(n, probe_seed)
-        gen = fit_co(j["trait"], j["y"], j["X"], probes, opt)
+        gen = fit_co(j["trait"], j["y"], j["X"], probes, opt, fit0=fit0)
         handles.append(sch.submit(gen, j.get("subset"), j.get("tag")))
     try:
         sch.run()
