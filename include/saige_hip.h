@@ -312,10 +312,17 @@ int saige_get_gemm_config_half(const saige_ctx* ctx, int* rt, int* pd, double* t
  * shape parameters mean the same in both, except that in SAIGE_GEMM_BF16X3
  * rt = 1 runs workgroups of twelve one-tile waves (384 rows share each
  * staged chunk of P) whose registers hold at most two chunks ahead, so pd = 3
- * means two there.  Changing the mode clears the tuned times (re-run
- * saige_autotune).  Engine idle. */
+ * means two there.  SAIGE_GEMM_BF16X3_TRI: the same six piece products,
+ * read from the lower triangle of the GRM only (half the bytes): each 32x32
+ * block left of the diagonal gives both S_IJ P_J and S_IJ^T P_I.  rt selects
+ * the workgroup (1: twelve waves, 2: eight), pd 1 or 3 the prefetch depth
+ * (one or two chunks ahead); nsplit is unused.  It needs device scratch for
+ * its partial products, about 128 N^2 / (32 * waves) bytes (3.3 GB at
+ * N = 100k with twelve waves), allocated on first use.  Changing the mode
+ * clears the tuned times (re-run saige_autotune).  Engine idle. */
 #define SAIGE_GEMM_FP32 0
 #define SAIGE_GEMM_BF16X3 1
+#define SAIGE_GEMM_BF16X3_TRI 2
 int saige_set_gemm_mode(saige_ctx* ctx, int mode);
 int saige_get_gemm_mode(const saige_ctx* ctx, int* mode);
 

@@ -16,6 +16,9 @@
 // result does not depend on which other columns share the launch.
 #include "saige_internal.hpp"
 
+#include <algorithm>
+#include <type_traits>
+
 #pragma clang fp contract(off)
 
 namespace saige {
@@ -538,6 +541,275 @@ __global__ __launch_bounds__(768) void k_gemm_x3w(const float* __restrict__ S, i
     }
 }
 
+// ------------------------------------------------- lower-triangle product
+// k_gemm_x3t<NWV, PD, FULL>: the three-piece product read from the lower
+// triangle of S only (half the bytes).  S is symmetric, so a 32x32 block
+// S_IJ left of the diagonal serves two products:
+//   use 1: Y_I += S_IJ P_J    (the per-wave work of k_gemm_x3w)
+//   use 2: Y_J += S_IJ^T P_I
+// A workgroup of NWV waves owns a row set R of 32*NWV rows (wave w: rows
+// 32*(NWV*R + w) + [0, 32)) and walks one item: a range of 32-column chunks.
+// Chunks left of the row set's diagonal block (c < NWV*R) are used twice;
+// the item over the diagonal block (chunks NWV*R + [0, NWV)) runs use 1
+// alone over the whole square, its upper half included.
+// Use 2 needs S_IJ in the MFMA's B-operand layout (lane = column j, eight
+// rows i per lane), the transpose of how it is loaded (lane = row i).  The
+// MFMA transposes the pieces itself: T = A_0 E_0 + A_1 E_1 with 0/1
+// selection matrices E_s sums one BF16 value and zeros per element, exactly,
+// and the MFMA's result layout is column-per-lane, so T's high halves are
+// the pieces in the B-operand layout.  The wave's own rows of P (the A
+// operand P_I^T, three pieces) are split once per item into registers.
+// A wave's use-2 result (32 columns x the live RHS) covers its own 32 rows;
+// the NWV results are summed in LDS in wave order and stored as the partial
+// of (chunk, row set) in y2p.  With eight waves the LDS holds two such tiles,
+// and chunk c's sum is formed in chunk c+1's iteration, beside its MFMA work,
+// so one barrier per chunk suffices; twelve waves' tiles fit once, and the
+// sum follows a second barrier in the chunk's own iteration.  Use 1 accumulates in registers over the item
+// and is stored as the item's partial in y1p.  k_tri_reduce sums, for each
+// row, its row set's use-1 partials in slot order and then its chunk's use-2
+// partials in row-set order: a fixed order, so results are reproducible and
+// a column's bits do not depend on which other columns share the launch.
+// a use-2 tile in LDS: [column j][64 rhs], float4 slot s of row j stored at
+// slot s ^ (j % 16), so the column-per-lane writes and row-wise reads both
+// spread over all banks
+__device__ __forceinline__ int tri_sr_index(int j, int q) { return j * 64 + 4 * ((q >> 2) ^ (j & 15)) + (q & 3); }
+
+// index of the first use-2 partial of chunk c: chunk c' has NR - 1 - c'/NWV
+template <int NWV>
+__host__ __device__ __forceinline__ int64_t tri_y2_first(int64_t c, int64_t NR) {
+    const int64_t b = c / NWV;
+    return NWV * (b * (NR - 1) - b * (b - 1) / 2) + (c - b * NWV) * (NR - 1 - b);
+}
+
+// T = A_0 E_0 + A_1 E_1 (one exact value per element), repacked as the two
+// B operands of use 2: operand s2 holds T's accumulators 8*s2 + [0, 8)
+__device__ __forceinline__ void tri_transpose(const u32x4 (&a)[2], const u32x4 (&e)[2], u32x4 (&t)[2]) {
+    f32x16 z;
+#pragma unroll
+    for (int r = 0; r < 16; r++) z[r] = 0.0f;
+    f32x16 T = mfma_bf(a[0], e[0], z);
+    T = mfma_bf(a[1], e[1], T);
+#pragma unroll
+    for (int s2 = 0; s2 < 2; s2++)
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            t[s2][k] = __builtin_amdgcn_perm(__float_as_uint(T[8 * s2 + 2 * k + 1]),
+                                             __float_as_uint(T[8 * s2 + 2 * k]), 0x07060302u);
+}
+
+template <int NWV, int PD, bool FULL>
+__global__ __launch_bounds__(64 * NWV) void k_gemm_x3t(const float* __restrict__ S, int64_t ld,
+                                                       const u32x4* __restrict__ pb, const float* __restrict__ P,
+                                                       const int4* __restrict__ items, float* __restrict__ y1p,
+                                                       float* __restrict__ y2p, int halves) {
+    constexpr int GM = 32 * NWV;
+    constexpr int RING = PD + 1;
+    constexpr int NT = FULL ? 2 : 1;                 // 32-column RHS tiles computed
+    constexpr int NTH = 64 * NWV;
+    constexpr int STG = (768 + NTH - 1) / NTH;       // B pieces staged per thread and chunk
+    __shared__ u32x4 sB0[12 * 64], sB1[12 * 64];     // a chunk's P pieces, [(g*3 + piece)*64 + n]
+    constexpr int SRB = NWV == 8 ? 2 : 1;            // use-2 tile buffers
+    __shared__ float sR[SRB][NWV * 32 * 64];         // use-2 results, [wave][column j][rhs]
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, hl = lane >> 5, ln = lane & 31;
+    const int4 item = items[blockIdx.x];
+    const int64_t R = item.x, ca = item.y, cb = item.z, slot = item.w;
+    const int64_t nch = ld / 32, G = ld / 8, NR = (nch + NWV - 1) / NWV;
+    const int64_t rw = R * GM + wv * 32;             // the wave's first row
+    const bool valid = rw < ld;                      // ld = Npad, a multiple of 32
+    const float* srow = S + ((valid ? rw : ld - 32) + ln) * ld + hl * 16;
+    const int hsel = FULL ? 0 : (halves >> 1);       // the live tile of a one-half launch
+    // staging: element e of a chunk's 768 pieces (group e/192, piece (e/64)%3,
+    // column e%64); threads past 768 repeat their first element
+    const u32x4* bsrc[STG];
+    int bdi[STG];
+#pragma unroll
+    for (int s = 0; s < STG; s++) {
+        const int e = tid + s * NTH < 768 ? tid + s * NTH : tid;
+        bsrc[s] = pb + ((int64_t)((e / 64) % 3) * G + e / 192) * EW + (e % 64);
+        bdi[s] = e;
+    }
+    // E_s as B operand: lane (hl, ln) holds rows k = 8*hl + t of column ln,
+    // E_s[k][n] = 1 when n == 16*(k/8) + 8*s + k%8 (the column of S that step
+    // s gives lane-half k/8 as its k-th value)
+    u32x4 esel[2];
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const int t1 = ln - 16 * hl - 8 * s;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            esel[s][k] = (t1 == 2 * k ? 0x3F80u : 0u) | (t1 == 2 * k + 1 ? 0x3F800000u : 0u);
+    }
+    f32x16 acc1[NT];
+#pragma unroll
+    for (int h = 0; h < NT; h++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) acc1[h][r] = 0.0f;
+    float4 ring[RING][4];
+    u32x4 bring[RING][STG];
+    auto issue = [&](float4 (&rs)[4], u32x4 (&br)[STG], int64_t ch) {
+#pragma unroll
+        for (int s = 0; s < STG; s++) br[s] = bsrc[s][4 * ch * EW];
+#pragma unroll
+        for (int q = 0; q < 4; q++) rs[q] = *reinterpret_cast<const float4*>(srow + ch * 32 + 4 * q);
+    };
+    // the run over the item; U2: chunks left of the diagonal (use 1 and 2)
+    auto run = [&](auto u2tag) {
+        constexpr bool U2 = decltype(u2tag)::value;
+        // the A operand of use 2: P rows rw + 16*s2 + 8*(t/4) + 4*hl + t%4,
+        // t < 8, column ln of the tile, as three pieces (zero for rows past ld)
+        u32x4 pah[NT][2], pam[NT][2], pal[NT][2];
+        if (U2) {
+#pragma unroll
+            for (int h = 0; h < NT; h++) {
+                const int col = 32 * (FULL ? h : hsel) + ln;
+#pragma unroll
+                for (int s2 = 0; s2 < 2; s2++) {
+                    float v[8];
+#pragma unroll
+                    for (int t = 0; t < 8; t++)
+                        v[t] = valid ? P[(rw + 16 * s2 + 8 * (t >> 2) + 4 * hl + (t & 3)) * EW + col] : 0.0f;
+                    split3_8(make_float4(v[0], v[1], v[2], v[3]), make_float4(v[4], v[5], v[6], v[7]), pah[h][s2],
+                             pam[h][s2], pal[h][s2]);
+                }
+            }
+        }
+        // the sum over the waves of chunk cc's use-2 tiles: 4 values per lane,
+        // waves [0, 4*NT), stored as its partial for this row set
+        auto reduce = [&](int64_t cc, const float* sr) {
+            if (wv < 4 * NT) {
+                const int o = (wv * 64 + lane) * 4;
+                const int j = o / (32 * NT), ql = o % (32 * NT);
+                const float* src = sr + tri_sr_index(j, ql);
+                float4 sum = *reinterpret_cast<const float4*>(src);
+#pragma unroll
+                for (int w = 1; w < NWV; w++) {
+                    const float4 v = *reinterpret_cast<const float4*>(src + w * 32 * 64);
+                    sum.x += v.x;
+                    sum.y += v.y;
+                    sum.z += v.z;
+                    sum.w += v.w;
+                }
+                const int qg = FULL ? ql : 32 * hsel + ql;
+                const int64_t pslot = tri_y2_first<NWV>(cc, NR) + (R - cc / NWV - 1);
+                *reinterpret_cast<float4*>(y2p + pslot * 2048 + j * 64 + qg) = sum;
+            }
+        };
+        auto step = [&](int u, int64_t c, bool pre) {
+            const int par = (int)((c - ca) & 1);
+            u32x4* sbw = par ? sB1 : sB0;
+#pragma unroll
+            for (int s = 0; s < STG; s++) sbw[bdi[s]] = bring[u][s];
+            lds_barrier();
+            if (SRB == 2 && U2 && c > ca) reduce(c - 1, sR[(c - 1) & 1]);
+            if (pre) issue(ring[(u + PD) % RING], bring[(u + PD) % RING], c + PD < cb ? c + PD : cb - 1);
+            const u32x4* sb = par ? sB1 : sB0;
+            u32x4 ah[2], am[2], al[2];
+#pragma unroll
+            for (int s = 0; s < 2; s++) {
+                split3_8(ring[u][2 * s], ring[u][2 * s + 1], ah[s], am[s], al[s]);
+#pragma unroll
+                for (int h = 0; h < NT; h++) {
+                    const u32x4* bb = sb + (2 * hl + s) * 3 * 64 + 32 * (FULL ? h : hsel) + ln;
+                    acc1[h] = mfma_x3(ah[s], am[s], al[s], bb[0], bb[64], bb[128], acc1[h]);
+                }
+            }
+            if (U2) {
+                u32x4 th[2], tm[2], tl[2];
+                tri_transpose(ah, esel, th);
+                tri_transpose(am, esel, tm);
+                tri_transpose(al, esel, tl);
+#pragma unroll
+                for (int h = 0; h < NT; h++) {
+                    f32x16 c2;
+#pragma unroll
+                    for (int r = 0; r < 16; r++) c2[r] = 0.0f;
+#pragma unroll
+                    for (int s2 = 0; s2 < 2; s2++)
+                        c2 = mfma_x3(pah[h][s2], pam[h][s2], pal[h][s2], th[s2], tm[s2], tl[s2], c2);
+                    // c2[a]: rhs 8*(a/4) + 4*hl + a%4 of the tile, column ln.  With
+                    // two buffers, chunk c's tile goes to buffer c & 1, last read
+                    // for chunk c-2 before this chunk's barrier
+                    float* dst = sR[SRB == 2 ? (c & 1) : 0] + wv * 32 * 64;
+#pragma unroll
+                    for (int g = 0; g < 4; g++)
+                        *reinterpret_cast<float4*>(dst + tri_sr_index(ln, 32 * h + 8 * g + 4 * hl)) =
+                            make_float4(c2[4 * g], c2[4 * g + 1], c2[4 * g + 2], c2[4 * g + 3]);
+                }
+                if (SRB == 1) {
+                    lds_barrier();
+                    reduce(c, sR[0]);
+                }
+            }
+        };
+#pragma unroll
+        for (int d = 0; d < PD; d++) issue(ring[d], bring[d], ca + d < cb ? ca + d : cb - 1);
+        int64_t cbase = ca;
+        for (; cbase + RING <= cb; cbase += RING) {
+#pragma unroll
+            for (int u = 0; u < RING; u++) step(u, cbase + u, true);
+        }
+#pragma unroll
+        for (int u = 0; u < RING - 1; u++)
+            if (cbase + u < cb) step(u, cbase + u, false);
+        if (SRB == 2 && U2) {
+            lds_barrier();
+            reduce(cb - 1, sR[(cb - 1) & 1]);
+        }
+    };
+    if (ca >= R * NWV)
+        run(std::integral_constant<bool, false>());
+    else
+        run(std::integral_constant<bool, true>());
+    if (!valid) return;
+    float* yp = y1p + slot * GM * EW;
+#pragma unroll
+    for (int h = 0; h < NT; h++) {
+        const int n = 32 * (FULL ? h : hsel) + ln;
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int64_t row = wv * 32 + 8 * (r >> 2) + 4 * hl + (r & 3);
+            yp[row * EW + n] = acc1[h][r];
+        }
+    }
+}
+
+// k_tri_reduce: Y[row][q] = the row set's use-1 partials (rtab[R] = {first
+// slot, count}) in slot order, then the chunk's use-2 partials in row-set
+// order; float4 per thread, QW threads per row (16: all 64 columns, 8: the
+// half at column q0).
+template <int NWV, int QW>
+__global__ __launch_bounds__(256) void k_tri_reduce(const float* __restrict__ y1p, const float* __restrict__ y2p,
+                                                    const int2* __restrict__ rtab, int64_t ld, int q0,
+                                                    float* __restrict__ y) {
+    constexpr int GM = 32 * NWV;
+    const int64_t row = (int64_t)blockIdx.x * (256 / QW) + threadIdx.x / QW;
+    const int q = q0 + 4 * (threadIdx.x % QW);
+    const int64_t nch = ld / 32, NR = (nch + NWV - 1) / NWV;
+    const int64_t c = row / 32, R = c / NWV;
+    const int2 rt = rtab[R];
+    const int64_t rr = row - R * GM;
+    const float* p1 = y1p + ((int64_t)rt.x * GM + rr) * EW + q;
+    float4 s = *reinterpret_cast<const float4*>(p1);
+    for (int k = 1; k < rt.y; k++) {
+        const float4 v = *reinterpret_cast<const float4*>(p1 + (int64_t)k * GM * EW);
+        s.x += v.x;
+        s.y += v.y;
+        s.z += v.z;
+        s.w += v.w;
+    }
+    const float* p2 = y2p + tri_y2_first<NWV>(c, NR) * 2048 + (row % 32) * 64 + q;
+    const int64_t n2 = NR - 1 - R;
+#pragma unroll 8
+    for (int64_t k = 0; k < n2; k++) {
+        const float4 v = *reinterpret_cast<const float4*>(p2 + k * 2048);
+        s.x += v.x;
+        s.y += v.y;
+        s.z += v.z;
+        s.w += v.w;
+    }
+    *reinterpret_cast<float4*>(y + row * EW + q) = s;
+}
+
 constexpr int EROWS = 256;   // rows per workgroup of the epilogue
 
 __global__ __launch_bounds__(256) void k_gemm_epi(const float* __restrict__ ypart, int nsplit, int64_t Npad,
@@ -1011,15 +1283,104 @@ hipError_t launch_grm_gemv(hipStream_t s, const float* S, int64_t Npad, int64_t 
 bool gemm_config_valid(GemmConfig g) {
     return (g.rt == 1 || g.rt == 2) && (g.pd == 1 || g.pd == 3) &&
            (g.nsplit == 1 || g.nsplit == 2 || g.nsplit == 4 || g.nsplit == 8) &&
-           (g.mode == GEMM_FP32 || g.mode == GEMM_BF16X3);
+           (g.mode == GEMM_FP32 || g.mode == GEMM_BF16X3 || g.mode == GEMM_BF16X3_TRI);
 }
+
+void tri_plan_free(TriPlan& t) {
+    if (t.items) (void)hipFree(t.items);
+    if (t.rtab) (void)hipFree(t.rtab);
+    if (t.y1p) (void)hipFree(t.y1p);
+    if (t.y2p) (void)hipFree(t.y2p);
+    t = TriPlan{};
+}
+
+hipError_t tri_plan_build(TriPlan& t, int64_t Npad, int nwv, int ch) {
+    if (t.items && t.Npad == Npad && t.nwv == nwv && t.ch == ch) return hipSuccess;
+    tri_plan_free(t);
+    if ((nwv != 8 && nwv != 12) || ch < 1 || Npad <= 0 || Npad % 256 != 0) return hipErrorInvalidValue;
+    const int64_t nch = Npad / 32, nr = (nch + nwv - 1) / nwv;
+    std::vector<int4> items;
+    std::vector<int2> rtab((size_t)nr);
+    int64_t slot = 0;
+    for (int64_t R = 0; R < nr; R++) {
+        // chunks left of the diagonal block, cut into k near-equal items, then
+        // the diagonal block itself
+        const int64_t off = nwv * R, k = (off + ch - 1) / ch;
+        rtab[(size_t)R] = make_int2((int)slot, (int)(k + 1));
+        for (int64_t i = 0; i < k; i++)
+            items.push_back(make_int4((int)R, (int)(off * i / k), (int)(off * (i + 1) / k), (int)slot++));
+        items.push_back(make_int4((int)R, (int)off, (int)std::min(off + nwv, nch), (int)slot++));
+    }
+    // longest items first, so the short ones fill the tail of the launch
+    std::stable_sort(items.begin(), items.end(),
+                     [](const int4& a, const int4& b) { return a.z - a.y > b.z - b.y; });
+    const int64_t y2 = nwv == 8 ? tri_y2_first<8>(nch, nr) : tri_y2_first<12>(nch, nr);
+    t.Npad = Npad;
+    t.nwv = nwv;
+    t.ch = ch;
+    t.nitems = (int64_t)items.size();
+    t.nr = nr;
+    t.y1_slots = slot;
+    t.y2_slots = y2;
+    hipError_t e = hipMalloc(&t.items, sizeof(int4) * items.size());
+    if (e == hipSuccess) e = hipMalloc(&t.rtab, sizeof(int2) * rtab.size());
+    if (e == hipSuccess) e = hipMalloc(&t.y1p, sizeof(float) * (size_t)(slot * 32 * nwv * EW));
+    if (e == hipSuccess) e = hipMalloc(&t.y2p, sizeof(float) * (size_t)(std::max<int64_t>(y2, 1) * 32 * EW));
+    if (e == hipSuccess) e = hipMemcpy(t.items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(t.rtab, rtab.data(), sizeof(int2) * rtab.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) tri_plan_free(t);
+    return e;
+}
+
+namespace {
+template <int NWV>
+hipError_t launch_tri(hipStream_t s, const float* S, int64_t Npad, const float* in, float* ypart, int pd,
+                      int halves, const TriPlan& t) {
+    const int64_t G = Npad / 8;
+    u32x4* pb = reinterpret_cast<u32x4*>(ypart + (int64_t)GSPLIT_MAX * Npad * EW);
+    hipLaunchKernelGGL(k_split_p, dim3((unsigned)(G * EW / 256)), dim3(256), 0, s, in, G, pb);
+    const dim3 grid((unsigned)t.nitems), block(64 * NWV);
+    if (halves == 3) {
+        if (pd == 3)
+            hipLaunchKernelGGL((k_gemm_x3t<NWV, 2, true>), grid, block, 0, s, S, Npad, pb, in, t.items, t.y1p, t.y2p,
+                               halves);
+        else
+            hipLaunchKernelGGL((k_gemm_x3t<NWV, 1, true>), grid, block, 0, s, S, Npad, pb, in, t.items, t.y1p, t.y2p,
+                               halves);
+        hipLaunchKernelGGL((k_tri_reduce<NWV, 16>), dim3((unsigned)(Npad / 16)), dim3(256), 0, s, t.y1p, t.y2p,
+                           t.rtab, Npad, 0, ypart);
+    } else {
+        if (pd == 3)
+            hipLaunchKernelGGL((k_gemm_x3t<NWV, 2, false>), grid, block, 0, s, S, Npad, pb, in, t.items, t.y1p,
+                               t.y2p, halves);
+        else
+            hipLaunchKernelGGL((k_gemm_x3t<NWV, 1, false>), grid, block, 0, s, S, Npad, pb, in, t.items, t.y1p,
+                               t.y2p, halves);
+        hipLaunchKernelGGL((k_tri_reduce<NWV, 8>), dim3((unsigned)(Npad / 32)), dim3(256), 0, s, t.y1p, t.y2p,
+                           t.rtab, Npad, halves == 2 ? 32 : 0, ypart);
+    }
+    return hipGetLastError();
+}
+}  // namespace
 
 hipError_t launch_grm_gemm(hipStream_t s, const float* S, int64_t Npad, int64_t Nnm, const float* in,
                            float* out, SigmaArgs a, double* partial, int64_t* nparts, float* ypart, GemmConfig g,
-                           int halves) {
+                           int halves, const TriPlan* tri) {
     // Npad is a multiple of 256 = 128 * RT for both RT; a slice holds at least
     // one superchunk when Npad / GKS >= nsplit
     if (!gemm_config_valid(g) || Npad % 256 != 0 || halves < 1 || halves > 3) return hipErrorInvalidValue;
+    if (g.mode == GEMM_BF16X3_TRI) {
+        // the lower-triangle product sums its partials into slice 0
+        if (!tri || tri->Npad != Npad || !tri->items) return hipErrorInvalidValue;
+        hipError_t e = tri->nwv == 8 ? launch_tri<8>(s, S, Npad, in, ypart, g.pd, halves, *tri)
+                                     : launch_tri<12>(s, S, Npad, in, ypart, g.pd, halves, *tri);
+        if (e != hipSuccess) return e;
+        const int64_t nb = Npad / EROWS;
+        hipLaunchKernelGGL(k_gemm_epi, dim3((unsigned)nb), dim3(256), 0, s, ypart, 1, Npad, Nnm, in, out, a,
+                           partial);
+        if (nparts) *nparts = nb;
+        return hipGetLastError();
+    }
     const int nsplit = (int)std::min<int64_t>(g.nsplit, Npad / GKS);
     const dim3 grid((unsigned)(Npad / (128 * g.rt)), nsplit);
     if (g.mode == GEMM_BF16X3) {

@@ -99,6 +99,7 @@ void free_geno(saige_ctx* c) {
     dfree(c->d_diag); dfree(c->d_grm); dfree(c->d_vec); dfree(c->d_grm_loco); dfree(c->d_diag_loco);
     dfree(c->d_mv_in); dfree(c->d_mv_blk); dfree(c->d_mv_out); dfree(c->d_mv_ypart);
     dfree(c->d_dots_partial); dfree(c->d_dots_out); dfree(c->d_take_lists);
+    tri_plan_free(c->tri);
     free_engine(c->eng);
     free_engine(c->eng1);
     c->h_stats.clear();
@@ -319,9 +320,28 @@ int engine_add(saige_ctx* c, Engine& e, int slot, int s, const float* b_host, in
     return engine_add_batch(c, e, 1, &slot, &s, b_host, maxiter, tol);
 }
 
+// items per row set left of its diagonal block in the lower-triangle
+// product: at most TRI_CHUNKS chunks of 32 columns (3072 columns) per item
+constexpr int TRI_CHUNKS = 96;
+
+// the lower-triangle plan for the context's Npad and wave count (rt 1: 12
+// waves, rt 2: 8), when that mode is on
+int ensure_tri(saige_ctx* c) {
+    if (c->gemm_cfg.mode != GEMM_BF16X3_TRI) return SAIGE_OK;
+    const hipError_t e = tri_plan_build(c->tri, c->Npad, c->gemm_cfg.rt == 2 ? 8 : 12, TRI_CHUNKS);
+    if (e != hipSuccess) {
+        saige::set_error("lower-triangle product scratch for Npad=%lld: %s", (long long)c->Npad,
+                         hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? SAIGE_E_NOMEM : SAIGE_E_DEVICE;
+    }
+    return SAIGE_OK;
+}
+
 int engine_step(saige_ctx* c, Engine& e, int n_iter, uint8_t* done_out, int32_t* iters_out) {
     hipStream_t st = c->stream;
     const int64_t Np = c->Npad, N = c->Nnm;
+    if (e.kpad != 1)
+        if (int rc = ensure_tri(c)) return rc;
     SigmaArgs a;
     a.winv = e.winv;
     a.mask = e.mask;
@@ -352,7 +372,7 @@ int engine_step(saige_ctx* c, Engine& e, int n_iter, uint8_t* done_out, int32_t*
             SAIGE_HIP_TRY(launch_grm_gemv(st, g.S, Np, N, e.P, e.AP, a, e.partial, &np));
         else
             SAIGE_HIP_TRY(launch_grm_gemm(st, g.S, Np, N, e.P, e.AP, a, e.partial, &np, e.ypart,
-                                          halves == 3 ? c->gemm_cfg : c->gemm_cfg_half, halves));
+                                          halves == 3 ? c->gemm_cfg : c->gemm_cfg_half, halves, &c->tri));
         SAIGE_HIP_TRY(hipEventRecord(c->ev1, st));
         SAIGE_HIP_TRY(launch_alpha(st, e.kpad, e.partial, np, e.rz, e.done, e.live, e.kind, e.alpha));
         int64_t np2 = 0;
@@ -1059,6 +1079,7 @@ static int grm_matvec(saige_ctx* c, int loco, const float* B, int64_t k, float* 
         SAIGE_HIP_TRY(hipStreamSynchronize(st));
         return SAIGE_OK;
     }
+    if (int rc = ensure_tri(c)) return rc;
     // own scratch (not the engine's), so live engine slots are untouched
     if (!c->d_mv_in) {
         if (dalloc(&c->d_mv_in, N * EW) || dalloc(&c->d_mv_blk, Np * EW) || dalloc(&c->d_mv_out, Np * EW) ||
@@ -1076,7 +1097,7 @@ static int grm_matvec(saige_ctx* c, int loco, const float* B, int64_t k, float* 
         if (e == hipSuccess) e = launch_scatter_cols(st, d_in, kb, N, Np, EW, d_blk);
         if (e == hipSuccess)
             e = launch_grm_gemm(st, g.S, Np, N, d_blk, d_out, a, nullptr, nullptr, c->d_mv_ypart,
-                                kb <= 32 ? c->gemm_cfg_half : c->gemm_cfg, kb <= 32 ? 1 : 3);
+                                kb <= 32 ? c->gemm_cfg_half : c->gemm_cfg, kb <= 32 ? 1 : 3, &c->tri);
         if (e == hipSuccess) e = launch_gather_cols(st, d_out, kb, N, EW, d_in);
         if (e == hipSuccess) e = hipMemcpyAsync(Y + k0 * N, d_in, 4 * N * kb, hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -1682,11 +1703,11 @@ int saige_autotune(saige_ctx* c, int reps, double* ms_table) {
     // time one shape: one warm-up launch, then reps launches between events
     auto time_shape = [&](GemmConfig g, int halves, double* out_ms) -> hipError_t {
         hipError_t e = launch_grm_gemm(st, c->d_grm, Np, N, c->d_mv_blk, c->d_mv_out, a, nullptr, nullptr,
-                                       c->d_mv_ypart, g, halves);
+                                       c->d_mv_ypart, g, halves, &c->tri);
         if (e == hipSuccess) e = hipEventRecord(c->ev0, st);
         for (int r = 0; r < reps && e == hipSuccess; r++)
             e = launch_grm_gemm(st, c->d_grm, Np, N, c->d_mv_blk, c->d_mv_out, a, nullptr, nullptr, c->d_mv_ypart,
-                                g, halves);
+                                g, halves, &c->tri);
         if (e == hipSuccess) e = hipEventRecord(c->ev1, st);
         if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
         float ms = 0.f;
@@ -1694,30 +1715,42 @@ int saige_autotune(saige_ctx* c, int reps, double* ms_table) {
         *out_ms = (double)ms / reps;
         return e;
     };
-    // shapes are tuned within the context's arithmetic mode
+    // shapes are tuned within the context's arithmetic mode.  The
+    // lower-triangle mode has no column slices (only the nsplit 8 entries are
+    // timed; the others read -1) and its summation order depends on rt (the
+    // wave count), so one-half launches keep the full shape's rt there.
     const int mode = c->gemm_cfg.mode;
+    const bool tri = mode == GEMM_BF16X3_TRI;
     GemmConfig best = c->gemm_cfg;
-    double best_ms = 0.0;
+    double best_ms = -1.0;
     for (int i = 0; i < kNumGemmCandidates; i++) {
         GemmConfig g = kGemmCandidates[i];
         g.mode = mode;
+        if (ms_table) ms_table[i] = -1.0;
+        if (tri && g.nsplit != 8) continue;
+        c->gemm_cfg = g;
+        if (int rc = ensure_tri(c)) return rc;
         double per = 0.0;
         SAIGE_HIP_TRY(time_shape(g, 3, &per));
         if (ms_table) ms_table[i] = per;
-        if (i == 0 || per < best_ms) {
+        if (best_ms < 0.0 || per < best_ms) {
             best = g;
             best_ms = per;
         }
     }
+    c->gemm_cfg = best;
+    if (int rc = ensure_tri(c)) return rc;
     // one-half launches: rt and pd only, the slice count stays the full shape's
     GemmConfig best_h = best;
-    double best_h_ms = 0.0;
+    double best_h_ms = -1.0;
     for (int i = 0; i < kNumHalfCandidates; i++) {
         GemmConfig g{kHalfCandidates[i].rt, best.nsplit, kHalfCandidates[i].pd, mode};
+        if (ms_table) ms_table[kNumGemmCandidates + i] = -1.0;
+        if (tri && g.rt != best.rt) continue;
         double per = 0.0;
         SAIGE_HIP_TRY(time_shape(g, 1, &per));
         if (ms_table) ms_table[kNumGemmCandidates + i] = per;
-        if (i == 0 || per < best_h_ms) {
+        if (best_h_ms < 0.0 || per < best_h_ms) {
             best_h = g;
             best_h_ms = per;
         }
@@ -1756,8 +1789,9 @@ int saige_set_gemm_config(saige_ctx* c, int rt, int pd, int nsplit) {
 
 int saige_set_gemm_mode(saige_ctx* c, int mode) {
     CHECK_CTX(c);
-    if (mode != SAIGE_GEMM_FP32 && mode != SAIGE_GEMM_BF16X3) {
-        set_error("unknown block-GEMM mode %d (0 = FP32 MFMA, 1 = three-piece BF16 split)", mode);
+    if (mode != SAIGE_GEMM_FP32 && mode != SAIGE_GEMM_BF16X3 && mode != SAIGE_GEMM_BF16X3_TRI) {
+        set_error("unknown block-GEMM mode %d (0 = FP32 MFMA, 1 = three-piece BF16 split, 2 = the split over "
+                  "the lower triangle)", mode);
         return SAIGE_E_ARG;
     }
     if (int rc = engine_idle(c, "changing the block-GEMM mode")) return rc;

@@ -49,8 +49,27 @@ constexpr int64_t NPAD_ALIGN = 256;
 // or 8).  The slice count fixes the summation order, so results are
 // reproducible for a fixed configuration.
 // Block-GEMM arithmetic: FP32 MFMA (32x32x2), or FP32 operands split
-// exactly into three BF16 pieces on the BF16 MFMA (32x32x16)
-enum GemmMode { GEMM_FP32 = 0, GEMM_BF16X3 = 1 };
+// exactly into three BF16 pieces on the BF16 MFMA (32x32x16), over all of S
+// or over its lower triangle (each block used for S_IJ P_J and S_IJ^T P_I)
+enum GemmMode { GEMM_FP32 = 0, GEMM_BF16X3 = 1, GEMM_BF16X3_TRI = 2 };
+
+// Work list and partial-product scratch of the lower-triangle product for
+// one Npad.  Row set R = rows 32*nwv*R + [0, 32*nwv); item = (R, chunk range
+// [ca, cb) of 32 columns, use-1 slot); row set R's use-1 partials are the
+// slots rtab[R].x + [0, rtab[R].y).
+struct TriPlan {
+    int64_t Npad = 0;
+    int nwv = 0;                 // waves per workgroup (8 or 12)
+    int ch = 0;                  // most chunks per item left of the diagonal
+    int64_t nitems = 0, nr = 0, y1_slots = 0, y2_slots = 0;
+    int4* items = nullptr;       // device [nitems], longest first
+    int2* rtab = nullptr;        // device [nr]
+    float* y1p = nullptr;        // [y1_slots][32*nwv][EW]
+    float* y2p = nullptr;        // [y2_slots][32][EW]
+};
+// builds (or keeps, when Npad/nwv/ch match) the plan; frees on failure
+hipError_t tri_plan_build(TriPlan& t, int64_t Npad, int nwv, int ch);
+void tri_plan_free(TriPlan& t);
 
 struct GemmConfig {
     int rt = 1;
@@ -143,6 +162,7 @@ struct saige_ctx {
     // shape for launches over one 32-column half: same nsplit (the summation
     // order, so a column's bits do not depend on the other half), own rt/pd
     saige::GemmConfig gemm_cfg_half;
+    saige::TriPlan tri;                          // lower-triangle product (mode 2), built on first use
     double gemm_tuned_ms_half = 0.0;
     // FP64 column-dot scratch (saige_engine_take_batch_ex, saige_dots_device)
     double* d_dots_partial = nullptr;
@@ -256,9 +276,10 @@ bool gemm_config_valid(GemmConfig g);
 // halves: bit h set = columns [32h, 32h+32) are computed; the epilogue's
 // values for a half that is not computed are meaningless (the caller only
 // passes halves without live columns).
+// tri: the plan for Npad, required in mode GEMM_BF16X3_TRI (else unused).
 hipError_t launch_grm_gemm(hipStream_t s, const float* S, int64_t Npad, int64_t Nnm, const float* in,
                            float* out, SigmaArgs a, double* partial, int64_t* nparts, float* ypart,
-                           GemmConfig g, int halves = 3);
+                           GemmConfig g, int halves = 3, const TriPlan* tri = nullptr);
 int64_t gemm_ypart_floats(int64_t Npad);
 int64_t gemv_max_parts(int64_t Npad);
 int64_t gemm_max_parts(int64_t Npad);

@@ -279,20 +279,23 @@ class Session:
         ms = np.zeros(n + 4, np.float64)
         check(self._L.saige_autotune(self.ctx, int(reps), ptr(ms)))
         cfg = self.gemm_config()
-        cfg["table"] = [(int(a), int(b), int(c), float(t)) for a, b, c, t in zip(rt, pd, ns, ms[:n])]
+        # shapes a mode does not use read -1 and are left out
+        cfg["table"] = [(int(a), int(b), int(c), float(t)) for a, b, c, t in zip(rt, pd, ns, ms[:n]) if t >= 0]
         cfg["half_table"] = [(a, b, cfg["nsplit"], float(t))
-                             for (a, b), t in zip([(1, 1), (2, 1), (1, 3), (2, 3)], ms[n:])]
+                             for (a, b), t in zip([(1, 1), (2, 1), (1, 3), (2, 3)], ms[n:]) if t >= 0]
         return cfg
 
     def set_gemm_config(self, rt, pd, nsplit):
         check(self._L.saige_set_gemm_config(self.ctx, int(rt), int(pd), int(nsplit)))
 
-    GEMM_MODES = ("fp32", "bf16x3")
+    GEMM_MODES = ("fp32", "bf16x3", "bf16x3_tri")
 
     def set_gemm_mode(self, mode):
-        """Block-GEMM arithmetic: "fp32" (FP32 MFMA) or "bf16x3" (FP32
+        """Block-GEMM arithmetic: "fp32" (FP32 MFMA), "bf16x3" (FP32
         operands split exactly into three BF16 pieces, six piece products on
-        the BF16 MFMA, FP32 accumulation).  Clears tuned times."""
+        the BF16 MFMA, FP32 accumulation) or "bf16x3_tri" (the same products
+        read from the GRM's lower triangle only, each block used for S_IJ P_J
+        and S_IJ^T P_I).  Clears tuned times."""
         check(self._L.saige_set_gemm_mode(self.ctx, self.GEMM_MODES.index(mode)))
 
     def gemm_mode(self):

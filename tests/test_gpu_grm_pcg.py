@@ -289,6 +289,123 @@ def test_engine_product_slots(session, grm1000):
     assert np.max(np.abs(X - X64)) <= 2e-3 * np.max(np.abs(X64))
 
 
+def test_gemm_bf16x3_tri_mode(session, grm1000):
+    """The split GEMM over the lower triangle (each 32x32 block used for
+    S_IJ P_J and, transposed on the MFMA, S_IJ^T P_I): exact on unit vectors
+    (the transpose and every partial carry one exact value), within 2^-21 of
+    s*b for one nonzero entry, K B within the FP32 product tolerance, the
+    same bits for both prefetch depths at one wave count and for one-half vs
+    full launches, PCG solves as in FP32 mode, and an engine solve with the
+    same bits alone and in a full mix.  N = 1000 pads to 1024: the last
+    row set is partial (waves past the padded size read a clamped row and
+    store nothing)."""
+    og, S = grm1000
+    print("case: GEMM bf16x3 lower-triangle mode", flush=True)
+    session.build_grm("fp32")
+    K64 = _k64(og, S)
+    rng = np.random.default_rng(31)
+    E = np.zeros((1000, 64), np.float32)
+    cols = rng.choice(1000, 64, replace=False)
+    E[cols, np.arange(64)] = 1.0
+    session.set_gemm_config(1, 1, 4)
+    ref_e = session.grm_matvec(E)
+    Bv = np.zeros((1000, 64), np.float32)
+    vals = (rng.uniform(0.5, 2.0, 64) / 3.0).astype(np.float32)
+    Bv[cols, np.arange(64)] = vals
+    ref_v = session.grm_matvec(Bv)
+    B = rng.standard_normal((1000, 70)).astype(np.float32)
+    RB = K64 @ B.astype(np.float64)
+    KB32 = session.grm_matvec(B)
+    w = rng.uniform(0.05, 0.25, 1000).astype(np.float32)
+    tau = np.array([1.0, 0.5], np.float32)
+    X64 = np.linalg.solve(_sigma64(og, S, w, tau), B[:, :3].astype(np.float64))
+    try:
+        session.set_gemm_mode("bf16x3_tri")
+        by_rt = {}
+        for rt, pd in [(1, 1), (1, 3), (2, 1), (2, 3)]:
+            session.set_gemm_config(rt, pd, 8)
+            assert session.gemm_config()["mode"] == "bf16x3_tri"
+            print(f"  rt{rt}/pd{pd}", flush=True)
+            got_e = session.grm_matvec(E)
+            assert np.array_equal(got_e.view(np.uint32), ref_e.view(np.uint32)), (rt, pd)
+            got_v = session.grm_matvec(Bv)
+            assert np.all(np.abs(got_v.astype(np.float64) - ref_v) <= 2.0 ** -21 * np.abs(ref_v.astype(np.float64)))
+            KB = session.grm_matvec(B)
+            err, err32 = np.max(np.abs(KB - RB)), np.max(np.abs(KB32 - RB))
+            print(f"  rt{rt}/pd{pd}: max|KB-K64 B| {err:.3e} (FP32 MFMA {err32:.3e})", flush=True)
+            assert err <= 2e-5 * np.max(np.abs(RB)), (rt, pd)
+            assert np.array_equal(KB, session.grm_matvec(B))
+            if rt in by_rt:
+                assert np.array_equal(KB, by_rt[rt]), (rt, pd)
+            by_rt[rt] = KB
+            narrow = session.grm_matvec(B[:, :5])
+            assert np.array_equal(narrow.view(np.uint32), KB[:, :5].view(np.uint32))
+            X, _ = session.pcg_block(w, tau, B[:, :3], 500, 1e-5)
+            assert np.max(np.abs(X - X64)) <= 2e-3 * np.max(np.abs(X64)), (rt, pd)
+        session.set_gemm_config(1, 1, 8)
+        session.engine_reserve(64, 2)
+        session.engine_set_system(0, w, tau)
+        session.engine_add(3, 0, B[:, 2], 500, 1e-5)
+        while not session.engine_step(50)[0][3]:
+            pass
+        alone = session.engine_take(3)
+        assert np.max(np.abs(alone - X64[:, 2])) <= 2e-3 * np.max(np.abs(X64[:, 2]))
+        for slot in range(64):
+            session.engine_add(slot, 0, B[:, slot % 5] * (1 + slot) if slot != 40 else B[:, 2], 500, 1e-5)
+        for _ in range(2000):
+            done, _ = session.engine_step(1)
+            if done.all():
+                break
+        mixed = session.engine_take(40)
+        for slot in range(64):
+            if slot != 40:
+                session.engine_take(slot)
+        assert np.array_equal(alone.view(np.uint32), mixed.view(np.uint32))
+        # the upper half alone: a one-half launch of the other tile
+        session.engine_add(50, 0, B[:, 2], 500, 1e-5)
+        while not session.engine_step(50)[0][50]:
+            pass
+        upper = session.engine_take(50)
+        assert np.array_equal(alone.view(np.uint32), upper.view(np.uint32))
+    finally:
+        session.set_gemm_mode("fp32")
+        session.set_gemm_config(1, 1, 4)
+
+
+def test_gemm_tri_mode_many_items(session):
+    """The lower-triangle product where row sets span several items (N =
+    4700: 13 row sets of 384 rows, up to 144 chunks left of the diagonal
+    cut into items of at most 96), against the FP64 product of the GPU's own
+    FP32 GRM, for both wave counts, full and one-half launches."""
+    N, M = 4700, 300
+    print("case: GEMM lower-triangle mode, N=4700", flush=True)
+    body, G, mm = random_bed(N, M, 0.01, seed=47, maf_lo=0.05, maf_hi=0.5)
+    session.setminMAFforGRM(0.01)
+    session.setmaxMissingRateforGRM(0.15)
+    session.setgeno_bytes(body, N, M)
+    session.build_grm("fp32")
+    K = session.grm_block(0, 0, N, N).astype(np.float64)
+    rng = np.random.default_rng(5)
+    B = rng.standard_normal((N, 64)).astype(np.float32)
+    RB = K @ B.astype(np.float64)
+    try:
+        session.set_gemm_mode("bf16x3")
+        session.set_gemm_config(1, 3, 8)
+        KBx = session.grm_matvec(B)
+        session.set_gemm_mode("bf16x3_tri")
+        for rt in (1, 2):
+            session.set_gemm_config(rt, 3, 8)
+            KB = session.grm_matvec(B)
+            err = np.max(np.abs(KB - RB))
+            print(f"  rt{rt}: max|KB-K B| {err:.3e}, split mode {np.max(np.abs(KBx - RB)):.3e}", flush=True)
+            assert err <= 2e-5 * np.max(np.abs(RB)), rt
+            half = session.grm_matvec(B[:, :20])
+            assert np.array_equal(half.view(np.uint32), KB[:, :20].view(np.uint32)), rt
+    finally:
+        session.set_gemm_mode("fp32")
+        session.set_gemm_config(1, 1, 4)
+
+
 def test_autotune_and_every_gemm_shape(session, grm1000):
     """saige_autotune times the sixteen block-GEMM shapes and keeps the
     fastest; every shape gives K B within the FP32 product tolerance, and a
