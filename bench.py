@@ -33,7 +33,8 @@ FP32_MFMA_PEAK = 157.3
 # (mode, N, rt, pd, nsplit).  Sources: PROFILE_FILES
 PROFILED_GEMM_TRAFFIC = {("fp32", 100_000, 1, 3, 8): (42.10 + 0.205 + 0.108 + 0.026) * 1e9,
                          ("bf16x3", 100_000, 1, 3, 8): (0.064 + 41.579 + 0.152) * 1e9}
-PROFILE_FILES = {"fp32": "profiles/kernels_N100k_M150k_bf16.md", "bf16x3": "profiles/gemm_bf16x3_N100k_M150k.md"}
+PROFILE_FILES = {"fp32": "profiles/kernels_N100k_M150k_bf16.md", "bf16x3": "profiles/gemm_bf16x3_N100k_M150k.md",
+                 "bf16x3_tri": "profiles/gemm_bf16x3_tri_N100k_M150k.md"}
 BF16_MFMA_PEAK = 2500.0
 
 
@@ -219,9 +220,9 @@ def main():
     ap.add_argument("--cpu-sample-markers", type=int, default=4000)
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--config", default="c2", choices=["c2", "c4"])
-    ap.add_argument("--gemm-mode", default="bf16x3", choices=["bf16x3", "fp32"],
+    ap.add_argument("--gemm-mode", default="bf16x3_tri", choices=["bf16x3_tri", "bf16x3", "fp32"],
                     help="block-GEMM arithmetic: FP32 operands split into three BF16 pieces on the BF16 "
-                         "MFMA (FP32-accurate), or FP32 MFMA")
+                         "MFMA (FP32-accurate), the same over the GRM's lower triangle only, or FP32 MFMA")
     ap.add_argument("--loco-chromosomes", type=int, default=22,
                     help="leave-one-chromosome-out fits for this many autosomes (0: skip)")
     args = ap.parse_args()
@@ -428,6 +429,18 @@ def main():
         roofline = {"bound": "mfma", "kernel": "k_gemm_part + k_gemm_epi (block-PCG GRM x 64 RHS, FP32 MFMA)",
                     "achieved": gemm_tflops, "peak": FP32_MFMA_PEAK, "unit": "TFLOP/s",
                     "frac": gemm_tflops / FP32_MFMA_PEAK, **common}
+    elif args.gemm_mode == "bf16x3_tri":
+        # the lower triangle (2 N^2 bytes) carries all six piece products of
+        # both uses: 384 flop per byte, over the BF16 ridge (312), so the bound
+        # is the BF16 MFMA
+        bf16 = 6.0 * gemm_flop / gemm_avg / 1e12
+        common["algorithmic"] = (f"6 x 2*N^2*64 = {6 * gemm_flop:.3e} BF16 flop per launch; "
+                                 f"2*N^2 = {2.0 * N * N:.3e} B of S's lower triangle")
+        roofline = {"bound": "mfma", "kernel": "k_split_p + k_gemm_x3t + k_tri_reduce + k_gemm_epi (block-PCG GRM "
+                                               "x 64 RHS from the lower triangle, FP32 as three BF16 pieces)",
+                    "achieved": bf16, "peak": BF16_MFMA_PEAK, "unit": "TFLOP/s", "frac": bf16 / BF16_MFMA_PEAK,
+                    "hbm_GBs_of_triangle": 2.0 * N * N / gemm_avg / 1e9, "fp32_equiv_tflops": gemm_tflops,
+                    **common}
     else:
         # six BF16 piece products per FP32 product: 192 flop per byte of S, under
         # the BF16 ridge (2500 TFLOP/s / 8 TB/s = 312), so the bound is HBM

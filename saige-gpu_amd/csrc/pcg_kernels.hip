@@ -603,10 +603,12 @@ __global__ __launch_bounds__(64 * NWV) void k_gemm_x3t(const float* __restrict__
                                                        const int4* __restrict__ items, float* __restrict__ y1p,
                                                        float* __restrict__ y2p, int halves) {
     constexpr int GM = 32 * NWV;
-    constexpr int RING = PD + 1;
+    constexpr bool PIPE = NWV == 8;                  // software-pipelined chunk loop (register room)
+    constexpr int RING = PIPE ? 2 : PD + 1;          // raw chunks held
     constexpr int NT = FULL ? 2 : 1;                 // 32-column RHS tiles computed
     constexpr int NTH = 64 * NWV;
-    constexpr int STG = (768 + NTH - 1) / NTH;       // B pieces staged per thread and chunk
+    constexpr int NSTG = FULL ? 768 : 384;           // B pieces a chunk needs (one half: its 32 columns)
+    constexpr int STG = (NSTG + NTH - 1) / NTH;      // B pieces staged per thread and chunk
     __shared__ u32x4 sB0[12 * 64], sB1[12 * 64];     // a chunk's P pieces, [(g*3 + piece)*64 + n]
     constexpr int SRB = NWV == 8 ? 2 : 1;            // use-2 tile buffers
     __shared__ float sR[SRB][NWV * 32 * 64];         // use-2 results, [wave][column j][rhs]
@@ -619,12 +621,14 @@ __global__ __launch_bounds__(64 * NWV) void k_gemm_x3t(const float* __restrict__
     const float* srow = S + ((valid ? rw : ld - 32) + ln) * ld + hl * 16;
     const int hsel = FULL ? 0 : (halves >> 1);       // the live tile of a one-half launch
     // staging: element e of a chunk's 768 pieces (group e/192, piece (e/64)%3,
-    // column e%64); threads past 768 repeat their first element
+    // column e%64), of the live half's 384 in a one-half launch; threads past
+    // the count repeat an element of the first round
     const u32x4* bsrc[STG];
     int bdi[STG];
 #pragma unroll
     for (int s = 0; s < STG; s++) {
-        const int e = tid + s * NTH < 768 ? tid + s * NTH : tid;
+        const int k = tid + s * NTH < NSTG ? tid + s * NTH : tid % NSTG;
+        const int e = FULL ? k : (k / 32) * 64 + 32 * hsel + (k % 32);
         bsrc[s] = pb + ((int64_t)((e / 64) % 3) * G + e / 192) * EW + (e % 64);
         bdi[s] = e;
     }
@@ -694,19 +698,14 @@ __global__ __launch_bounds__(64 * NWV) void k_gemm_x3t(const float* __restrict__
                 *reinterpret_cast<float4*>(y2p + pslot * 2048 + j * 64 + qg) = sum;
             }
         };
-        auto step = [&](int u, int64_t c, bool pre) {
-            const int par = (int)((c - ca) & 1);
-            u32x4* sbw = par ? sB1 : sB0;
-#pragma unroll
-            for (int s = 0; s < STG; s++) sbw[bdi[s]] = bring[u][s];
-            lds_barrier();
-            if (SRB == 2 && U2 && c > ca) reduce(c - 1, sR[(c - 1) & 1]);
-            if (pre) issue(ring[(u + PD) % RING], bring[(u + PD) % RING], c + PD < cb ? c + PD : cb - 1);
+        // chunk c's MFMA work from its pieces (step s in [.][s]), with its B
+        // pieces in LDS buffer par: use 1, then (U2) the transposes, use 2 and
+        // the wave's use-2 tile to LDS
+        auto chunk_mfma = [&](int par, int64_t c, const u32x4 (&ah)[2], const u32x4 (&am)[2],
+                              const u32x4 (&al)[2]) {
             const u32x4* sb = par ? sB1 : sB0;
-            u32x4 ah[2], am[2], al[2];
 #pragma unroll
             for (int s = 0; s < 2; s++) {
-                split3_8(ring[u][2 * s], ring[u][2 * s + 1], ah[s], am[s], al[s]);
 #pragma unroll
                 for (int h = 0; h < NT; h++) {
                     const u32x4* bb = sb + (2 * hl + s) * 3 * 64 + 32 * (FULL ? h : hsel) + ln;
@@ -741,16 +740,62 @@ __global__ __launch_bounds__(64 * NWV) void k_gemm_x3t(const float* __restrict__
                 }
             }
         };
+        auto step = [&](int u, int64_t c, bool pre) {
+            const int par = (int)((c - ca) & 1);
+            u32x4* sbw = par ? sB1 : sB0;
 #pragma unroll
-        for (int d = 0; d < PD; d++) issue(ring[d], bring[d], ca + d < cb ? ca + d : cb - 1);
-        int64_t cbase = ca;
-        for (; cbase + RING <= cb; cbase += RING) {
+            for (int s = 0; s < STG; s++) sbw[bdi[s]] = bring[u][s];
+            lds_barrier();
+            if (SRB == 2 && U2 && c > ca) reduce(c - 1, sR[(c - 1) & 1]);
+            if (pre) issue(ring[(u + PD) % RING], bring[(u + PD) % RING], c + PD < cb ? c + PD : cb - 1);
+            u32x4 ah[2], am[2], al[2];
 #pragma unroll
-            for (int u = 0; u < RING; u++) step(u, cbase + u, true);
+            for (int s = 0; s < 2; s++) split3_8(ring[u][2 * s], ring[u][2 * s + 1], ah[s], am[s], al[s]);
+            chunk_mfma(par, c, ah, am, al);
+        };
+        if (PIPE) {
+            // software-pipelined: chunk c's pieces were split during chunk c-1,
+            // so this chunk's MFMAs run beside the next chunk's split.  Raw
+            // chunks c+1 and c+2 are resident or in flight (slot (c-ca) & 1),
+            // and so are their B pieces; chunk ids past the item's end are
+            // clamped to its last chunk, whose data is then split or staged
+            // unused.  (A deeper ring does not fit the registers.)
+            u32x4 pcs[2][3][2];                       // [set][piece][step]
+            auto split_into = [&](const float4 (&rs)[4], u32x4 (&p)[3][2]) {
+#pragma unroll
+                for (int st = 0; st < 2; st++) split3_8(rs[2 * st], rs[2 * st + 1], p[0][st], p[1][st], p[2][st]);
+            };
+            auto pstep = [&](int u, int64_t c) {
+                u32x4* sbw = u ? sB1 : sB0;
+#pragma unroll
+                for (int st = 0; st < STG; st++) sbw[bdi[st]] = bring[u][st];
+                lds_barrier();
+                if (SRB == 2 && U2 && c > ca) reduce(c - 1, sR[(c - 1) & 1]);
+                split_into(ring[u ^ 1], pcs[u ^ 1]);
+                issue(ring[u], bring[u], c + 2 < cb ? c + 2 : cb - 1);
+                chunk_mfma(u, c, pcs[u][0], pcs[u][1], pcs[u][2]);
+            };
+            issue(ring[0], bring[0], ca);
+            issue(ring[1], bring[1], ca + 1 < cb ? ca + 1 : cb - 1);
+            split_into(ring[0], pcs[0]);
+            int64_t c = ca;
+            for (; c + 2 <= cb; c += 2) {
+                pstep(0, c);
+                pstep(1, c + 1);
+            }
+            if (c < cb) pstep(0, c);
+        } else {
+#pragma unroll
+            for (int d = 0; d < PD; d++) issue(ring[d], bring[d], ca + d < cb ? ca + d : cb - 1);
+            int64_t cbase = ca;
+            for (; cbase + RING <= cb; cbase += RING) {
+#pragma unroll
+                for (int u = 0; u < RING; u++) step(u, cbase + u, true);
+            }
+#pragma unroll
+            for (int u = 0; u < RING - 1; u++)
+                if (cbase + u < cb) step(u, cbase + u, false);
         }
-#pragma unroll
-        for (int u = 0; u < RING - 1; u++)
-            if (cbase + u < cb) step(u, cbase + u, false);
         if (SRB == 2 && U2) {
             lds_barrier();
             reduce(cb - 1, sR[(cb - 1) & 1]);

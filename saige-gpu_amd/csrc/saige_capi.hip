@@ -1723,15 +1723,18 @@ int saige_autotune(saige_ctx* c, int reps, double* ms_table) {
     const bool tri = mode == GEMM_BF16X3_TRI;
     GemmConfig best = c->gemm_cfg;
     double best_ms = -1.0;
+    double tfull[kNumGemmCandidates];
     for (int i = 0; i < kNumGemmCandidates; i++) {
         GemmConfig g = kGemmCandidates[i];
         g.mode = mode;
+        tfull[i] = -1.0;
         if (ms_table) ms_table[i] = -1.0;
         if (tri && g.nsplit != 8) continue;
         c->gemm_cfg = g;
         if (int rc = ensure_tri(c)) return rc;
         double per = 0.0;
         SAIGE_HIP_TRY(time_shape(g, 3, &per));
+        tfull[i] = per;
         if (ms_table) ms_table[i] = per;
         if (best_ms < 0.0 || per < best_ms) {
             best = g;
@@ -1743,10 +1746,50 @@ int saige_autotune(saige_ctx* c, int reps, double* ms_table) {
     // one-half launches: rt and pd only, the slice count stays the full shape's
     GemmConfig best_h = best;
     double best_h_ms = -1.0;
-    for (int i = 0; i < kNumHalfCandidates; i++) {
+    if (tri) {
+        // rt fixes the summation order, so full and one-half launches share
+        // it: per rt the fastest pd of each, and the rt of the least
+        // full + 2 x half time (a fit round runs about twice as many
+        // one-half launches as full ones)
+        double score = -1.0;
+        for (int rt = 1; rt <= 2; rt++) {
+            GemmConfig bf{}, bh{};
+            double mf = -1.0, mh = -1.0;
+            for (int i = 0; i < kNumGemmCandidates; i++)
+                if (kGemmCandidates[i].rt == rt && tfull[i] >= 0.0) {
+                    const double t = tfull[i];
+                    if (mf < 0.0 || t < mf) {
+                        bf = kGemmCandidates[i];
+                        mf = t;
+                    }
+                }
+            c->gemm_cfg = GemmConfig{rt, 8, 1, mode};
+            if (int rc = ensure_tri(c)) return rc;
+            for (int i = 0; i < kNumHalfCandidates; i++) {
+                if (kHalfCandidates[i].rt != rt) continue;
+                const GemmConfig g{rt, 8, kHalfCandidates[i].pd, mode};
+                double per = 0.0;
+                SAIGE_HIP_TRY(time_shape(g, 1, &per));
+                if (ms_table) ms_table[kNumGemmCandidates + i] = per;
+                if (mh < 0.0 || per < mh) {
+                    bh = g;
+                    mh = per;
+                }
+            }
+            if (mf >= 0.0 && (score < 0.0 || mf + 2.0 * mh < score)) {
+                score = mf + 2.0 * mh;
+                best = GemmConfig{rt, 8, bf.pd, mode};
+                best_ms = mf;
+                best_h = bh;
+                best_h_ms = mh;
+            }
+        }
+        c->gemm_cfg = best;
+        if (int rc = ensure_tri(c)) return rc;
+    }
+    for (int i = 0; i < kNumHalfCandidates && !tri; i++) {
         GemmConfig g{kHalfCandidates[i].rt, best.nsplit, kHalfCandidates[i].pd, mode};
         if (ms_table) ms_table[kNumGemmCandidates + i] = -1.0;
-        if (tri && g.rt != best.rt) continue;
         double per = 0.0;
         SAIGE_HIP_TRY(time_shape(g, 1, &per));
         if (ms_table) ms_table[kNumGemmCandidates + i] = per;

@@ -9,8 +9,9 @@ and HBM counters (FETCH_SIZE, WRITE_SIZE) of each can be read per dispatch:
 
 Kernels exercised: decode/QC (k_decode_stats), repack, diag, the GRM SYRK
 (2-bit input k_syrk_pk for --grm bf16), the autotuner's sixteen block-GEMM
-shapes, the tuned 64-column block GEMM (k_split_p + k_gemm_x3 + k_gemm_epi
-in the default --gemm-mode bf16x3, k_gemm_part + k_gemm_epi in fp32, through
+shapes, the tuned 64-column block GEMM (k_split_p + k_gemm_x3t + k_tri_reduce
++ k_gemm_epi in the default --gemm-mode bf16x3_tri, k_split_p + k_gemm_x3w/x3
++ k_gemm_epi in bf16x3, k_gemm_part + k_gemm_epi in fp32, through
 saige_grm_matvec) and its one-half (16-column) form, the single-column GEMV
 (k_gemv), one LOCO range build (copy + range SYRK in subtract mode over
 --loco-frac of the markers), and the runtime's copy kernel on COPY_BYTES.
@@ -33,7 +34,7 @@ def main():
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--grm", default="bf16")
     ap.add_argument("--loco-frac", type=float, default=0.08)
-    ap.add_argument("--gemm-mode", default="bf16x3", choices=["bf16x3", "fp32"])
+    ap.add_argument("--gemm-mode", default="bf16x3_tri", choices=["bf16x3_tri", "bf16x3", "fp32"])
     args = ap.parse_args()
 
     import numpy as np
