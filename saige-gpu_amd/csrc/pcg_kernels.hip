@@ -564,11 +564,16 @@ __global__ __launch_bounds__(768) void k_gemm_x3w(const float* __restrict__ S, i
 // of (chunk, row set) in y2p.  With eight waves the LDS holds two such tiles,
 // and chunk c's sum is formed in chunk c+1's iteration, beside its MFMA work,
 // so one barrier per chunk suffices; twelve waves' tiles fit once, and the
-// sum follows a second barrier in the chunk's own iteration.  Use 1 accumulates in registers over the item
-// and is stored as the item's partial in y1p.  k_tri_reduce sums, for each
-// row, its row set's use-1 partials in slot order and then its chunk's use-2
-// partials in row-set order: a fixed order, so results are reproducible and
-// a column's bits do not depend on which other columns share the launch.
+// sum follows a second barrier in the chunk's own iteration.  Use 1
+// accumulates in registers over the item and is stored as the item's partial
+// in y1p.  k_tri_reduce sums, for each row, its row set's use-1 partials in
+// slot order and then its chunk's use-2 partials in row-set order: a fixed
+// order, so results are reproducible and a column's bits do not depend on
+// which other columns share the launch.
+// Eight waves have registers for a software-pipelined loop: chunk c's pieces
+// are split during chunk c-1, beside that chunk's MFMAs.  Twelve waves split
+// each chunk at its own turn, PD chunks of raw data ahead.
+
 // a use-2 tile in LDS: [column j][64 rhs], float4 slot s of row j stored at
 // slot s ^ (j % 16), so the column-per-lane writes and row-wise reads both
 // spread over all banks
