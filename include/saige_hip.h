@@ -423,6 +423,14 @@ int saige_covariate_transform(const double* X, int64_t n, int p, double* X_out, 
 int saige_score_test_null(const double* mu, const double* V, const double* y, const double* X, int64_t n, int p,
                           double* XV, double* XVX, double* XVX_inv, double* XXVX_inv, double* XVX_inv_XV,
                           double* S_a);
+/* The start values fitNULLGLMM takes from glm() (FG.R:1119, 1370): R's
+ * glm.fit IRLS with glm.control's defaults (25 iterations, epsilon 1e-8,
+ * relative deviance change) for trait SAIGE_TRAIT_BINARY (logit) or
+ * SAIGE_TRAIT_QUANTITATIVE (identity) on X (n x p column-major) with an
+ * optional offset (n, or NULL for 0).  Outputs coef (p), eta and mu (n).
+ * Host only, no context: callers may run several at once on their threads. */
+int saige_glm_start(int trait, int64_t n, const double* y, const double* X, int p, const double* offset,
+                    double* coef, double* eta, double* mu);
 
 #ifdef __cplusplus
 }

@@ -492,4 +492,25 @@ int saige_score_test_null(const double* mu, const double* V, const double* y, co
     return SAIGE_OK;
 }
 
+int saige_glm_start(int trait, int64_t n, const double* y, const double* X, int p, const double* offset,
+                    double* coef, double* eta, double* mu) {
+    if (!y || !X || n <= 0 || p <= 0 || !coef || !eta || !mu ||
+        (trait != SAIGE_TRAIT_BINARY && trait != SAIGE_TRAIT_QUANTITATIVE))
+        return SAIGE_E_ARG;
+    try {
+        saige::fit::Matrix Xm(n, p);
+        std::memcpy(Xm.a.data(), X, sizeof(double) * (size_t)(n * p));
+        const std::vector<double> off = offset ? std::vector<double>(offset, offset + n) : std::vector<double>();
+        const auto g = saige::fit::glm_start(trait == SAIGE_TRAIT_BINARY ? saige::fit::Trait::Binary
+                                                                         : saige::fit::Trait::Quantitative,
+                                             std::vector<double>(y, y + n), Xm, off);
+        std::memcpy(coef, g.coef.data(), sizeof(double) * (size_t)p);
+        std::memcpy(eta, g.eta.data(), sizeof(double) * (size_t)n);
+        std::memcpy(mu, g.mu.data(), sizeof(double) * (size_t)n);
+    } catch (const std::exception&) {
+        return SAIGE_E_ARG;
+    }
+    return SAIGE_OK;
+}
+
 }  // extern "C"

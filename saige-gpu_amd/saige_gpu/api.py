@@ -537,6 +537,25 @@ class Session:
         return out
 
 
+def glm_start_native(trait, y, X, offset=None):
+    """glm() start values (FG.R:1119, 1370) by the library's C++ IRLS:
+    returns {"coef", "eta", "mu", "offset"} as saige_gpu.driver.glm_start.
+    The library call holds no Python lock, so several fits' start values
+    can run at once on threads."""
+    L = _lib.load()
+    y = np.ascontiguousarray(y, np.float64)
+    X = np.asfortranarray(X, np.float64)
+    n, p = X.shape
+    off = None if offset is None else np.ascontiguousarray(offset, np.float64)
+    coef, eta, mu = np.zeros(p), np.zeros(n), np.zeros(n)
+    pd = ctypes.POINTER(ctypes.c_double)
+    t = {"binary": 0, "quantitative": 1}[trait]
+    check(L.saige_glm_start(t, n, y.ctypes.data_as(pd), X.ctypes.data_as(pd), p,
+                            None if off is None else off.ctypes.data_as(pd), coef.ctypes.data_as(pd),
+                            eta.ctypes.data_as(pd), mu.ctypes.data_as(pd)))
+    return {"coef": coef, "eta": eta, "mu": mu, "offset": np.zeros(n) if off is None else off}
+
+
 def covariate_transform_native(X):
     """Covariate_Transform (FG.R:1612-1647) by the library's C++ code:
     returns (X_new, R, kept_columns) as saige_gpu.plink.covariate_transform."""

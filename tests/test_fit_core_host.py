@@ -171,3 +171,27 @@ def test_library_covariate_transform_export():
     Xr, Rr, keepr = plink.covariate_transform(X)
     assert list(keep) == list(keepr) == [0, 1, 2]
     assert np.allclose(Xt, Xr, rtol=1e-10, atol=1e-10) and np.allclose(R, Rr, rtol=1e-10, atol=1e-10)
+
+
+def test_library_glm_start_export():
+    """saige_glm_start (the C++ glm.fit restatement the batched path takes
+    its start values from), called without a device, against the Python
+    glm_start for both families, with and without an offset.  The start
+    values also satisfy glm's score equations X^T (y - mu) = 0 (logit, no
+    offset), which a wrong weight or working response would break."""
+    from saige_gpu.api import glm_start_native
+    rng = np.random.default_rng(9)
+    N = 3000
+    X, _, _ = plink.covariate_transform(np.column_stack([np.ones(N), rng.normal(50, 10, N), rng.random(N) < 0.5]))
+    lin = X @ np.array([-0.03, 0.4, -0.2])
+    yb = (rng.random(N) < 1 / (1 + np.exp(-lin))).astype(np.float64)
+    yq = lin + rng.standard_normal(N)
+    off = 0.1 * rng.standard_normal(N)
+    for trait, fam, y in (("binary", D.Binomial, yb), ("quantitative", D.Gaussian, yq)):
+        for o in (None, off):
+            got = glm_start_native(trait, y, X, o)
+            ref = D.glm_start(y, X, fam, o)
+            for k in ("coef", "eta", "mu"):
+                assert np.allclose(got[k], ref[k], rtol=1e-10, atol=1e-12), (trait, o is None, k)
+    g = glm_start_native("binary", yb, X)
+    assert np.max(np.abs(X.T @ (yb - g["mu"]))) < 1e-6 * N
