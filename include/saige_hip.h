@@ -325,6 +325,18 @@ int saige_get_gemm_config_half(const saige_ctx* ctx, int* rt, int* pd, double* t
 #define SAIGE_GEMM_BF16X3_TRI 2
 int saige_set_gemm_mode(saige_ctx* ctx, int mode);
 int saige_get_gemm_mode(const saige_ctx* ctx, int* mode);
+/* Single-column GRM products (the PCG of saige_pcg1, saige_get_crossprod,
+ * saige_crossprod_mat_and_kin, one-column saige_grm_matvec; no reference
+ * counterpart, the reference's is an FP32 SGEMV).  SAIGE_GEMV_TRI (the
+ * default) reads only the GRM's lower triangle: each element serves
+ * y_i += s_ij x_j and y_j += s_ij x_i, partial sums are added in a fixed
+ * order, and it allocates about 2 N^2 / 256 bytes of scratch on first use
+ * (80 MB at N = 100k).  SAIGE_GEMV_FULL reads all of S (k_gemv).  Both are
+ * FP32 with different summation orders. */
+#define SAIGE_GEMV_FULL 0
+#define SAIGE_GEMV_TRI 1
+int saige_set_gemv_mode(saige_ctx* ctx, int mode);
+int saige_get_gemv_mode(const saige_ctx* ctx, int* mode);
 
 /* Free and total device memory (hipMemGetInfo), for admission control. */
 int saige_device_memory(saige_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);

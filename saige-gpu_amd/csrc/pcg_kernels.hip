@@ -33,6 +33,10 @@ __device__ __forceinline__ float wave_sum_f(float v) {
     return v;
 }
 
+// wait for this wave's LDS traffic, then the workgroup barrier; outstanding
+// global loads stay in flight
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // Sigma p for one element (getCrossprod, FG.cpp:2412): the reference forms
 // 1/w, then b % (1/w), then tau0 * that, plus tau1 * (K b); tau1 == 0
 // returns the first term alone (FG.cpp:2401-2404).
@@ -90,6 +94,122 @@ __global__ __launch_bounds__(256) void k_gemv(const float* __restrict__ S, int64
         __syncthreads();
         if (threadIdx.x == 0)
             partial[(int64_t)blockIdx.x * 2] = (s_dot[0] + s_dot[1]) + (s_dot[2] + s_dot[3]);
+    }
+}
+
+// ------------------------------------------------ lower-triangle GEMV
+// k_gemv_tri: y = S x read from the lower triangle of S only (half the
+// bytes).  A workgroup of eight waves owns a row set R of 256 rows (wave w:
+// the strip 256R + 32w + [0, 32)) and walks one item: a range of 256-column
+// chunks left of the row set's diagonal block, each element serving
+//   use 1: y_i += S_ij x_j    and    use 2: y_j += S_ij x_i,
+// or the diagonal 256 x 256 block itself (use 1 only, upper half included).
+// Lane l owns columns 4l + [0, 4) of a chunk.  Use 1 accumulates per lane and
+// row over the item and is summed across the wave's lanes at its end (one
+// partial per item and row); use 2 accumulates per lane over the strip's 32
+// rows, the eight strips' sums are added in LDS in wave order, and the sum
+// is stored as the partial of (chunk, row set).  k_gemv_tri_epi sums each
+// row's partials in a fixed order and applies k_gemv's epilogue.
+constexpr int TV_ROWS = 256;           // rows per row set (eight strips of 32)
+constexpr int TV_COLS = 256;           // columns per chunk (64 lanes x float4)
+
+// index of the first use-2 partial of chunk c: chunk c' has NR - 1 - c'
+__host__ __device__ __forceinline__ int64_t triv_y2_first(int64_t c, int64_t NR) {
+    return c * (NR - 1) - c * (c - 1) / 2;
+}
+
+__global__ __launch_bounds__(512) void k_gemv_tri(const float* __restrict__ S, int64_t ld,
+                                                  const float* __restrict__ x, const int4* __restrict__ items,
+                                                  float* __restrict__ y1p, float* __restrict__ y2p) {
+    __shared__ float4 sz[2][8][64];    // use-2 strip sums, two chunks
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int4 item = items[blockIdx.x];
+    const int64_t R = item.x, ca = item.y, cb = item.z, slot = item.w;
+    const int64_t NR = ld / TV_ROWS;
+    const int64_t r0 = R * TV_ROWS + 32 * wv;   // the wave's first row (ld is a multiple of 256)
+    const bool diag = ca >= R;
+    const float* Sb = S + r0 * ld + 4 * lane;
+    float xr[32];
+#pragma unroll
+    for (int r = 0; r < 32; r++) xr[r] = x[r0 + r];
+    float acc[32];
+#pragma unroll
+    for (int r = 0; r < 32; r++) acc[r] = 0.0f;
+    for (int64_t c = ca; c < cb; c++) {
+        const float4 xj = *reinterpret_cast<const float4*>(x + TV_COLS * c + 4 * lane);
+        float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int r = 0; r < 32; r++) {
+            const float4 v = *reinterpret_cast<const float4*>(Sb + r * ld + TV_COLS * c);
+            acc[r] += v.x * xj.x + v.y * xj.y + v.z * xj.z + v.w * xj.w;
+            z.x += v.x * xr[r];
+            z.y += v.y * xr[r];
+            z.z += v.z * xr[r];
+            z.w += v.w * xr[r];
+        }
+        if (!diag) {
+            // buffer c & 1 was last read for chunk c-2, before chunk c-1's barrier
+            sz[c & 1][wv][lane] = z;
+            lds_barrier();
+            if (wv == 0) {
+                float4 sum = sz[c & 1][0][lane];
+#pragma unroll
+                for (int w = 1; w < 8; w++) {
+                    const float4 v = sz[c & 1][w][lane];
+                    sum.x += v.x;
+                    sum.y += v.y;
+                    sum.z += v.z;
+                    sum.w += v.w;
+                }
+                const int64_t ps = triv_y2_first(c, NR) + (R - c - 1);
+                *reinterpret_cast<float4*>(y2p + ps * TV_COLS + 4 * lane) = sum;
+            }
+        }
+    }
+    float* yp = y1p + slot * TV_ROWS + 32 * wv;
+#pragma unroll
+    for (int r = 0; r < 32; r++) {
+        const float t = wave_sum_f(acc[r]);
+        if (lane == r) yp[r] = t;
+    }
+}
+
+// Y[row] = the row set's use-1 partials in slot order, then the row's chunk's
+// use-2 partials in row-set order; then k_gemv's epilogue and the block's
+// FP64 dot partial (mode 1), rows = threads
+__global__ __launch_bounds__(256) void k_gemv_tri_epi(const float* __restrict__ y1p, const float* __restrict__ y2p,
+                                                      const int2* __restrict__ rtab, int64_t ld, int64_t Nnm,
+                                                      const float* __restrict__ in, float* __restrict__ out,
+                                                      SigmaArgs a, double* __restrict__ partial) {
+    __shared__ double sm[256];
+    const int tid = threadIdx.x;
+    const int64_t R = blockIdx.x, row = R * TV_ROWS + tid, NR = ld / TV_ROWS;
+    const int2 rt = rtab[R];
+    float y = y1p[(int64_t)rt.x * TV_ROWS + tid];
+    for (int k = 1; k < rt.y; k++) y += y1p[((int64_t)rt.x + k) * TV_ROWS + tid];
+    const float* p2 = y2p + triv_y2_first(R, NR) * TV_COLS + tid;
+    const int64_t n2 = NR - 1 - R;
+#pragma unroll 8
+    for (int64_t k = 0; k < n2; k++) y += p2[k * TV_COLS];
+    double d = 0.0;
+    float o;
+    if (a.mode == 0) {
+        o = div_rn(y, a.mqc);
+    } else {
+        const int s = a.sys[0];
+        const bool on = a.live[0] && row < Nnm && a.mask[(int64_t)s * ld + row];
+        const float p = in[row];
+        o = on ? sigma_elem(p, y, a.winv[(int64_t)s * ld + row], a.tau[2 * s], a.tau[2 * s + 1], a.mqc) : 0.0f;
+        d = (double)p * (double)o;
+    }
+    out[row] = o;
+    if (a.mode == 1) {
+        sm[tid] = d;
+        for (int st = 128; st >= 1; st >>= 1) {
+            __syncthreads();
+            if (tid < st) sm[tid] += sm[tid + st];
+        }
+        if (tid == 0) partial[R * 2] = sm[0];
     }
 }
 
@@ -333,10 +453,6 @@ __global__ __launch_bounds__(256) void k_split_p(const float* __restrict__ in, i
     pb[(G + g) * EW + n] = m;
     pb[(2 * G + g) * EW + n] = l;
 }
-
-// wait for this wave's LDS traffic, then the workgroup barrier; outstanding
-// global loads stay in flight
-__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
 template <int RT, int PD, int W>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) void k_gemm_x3(
@@ -1323,7 +1439,17 @@ int64_t gemm_max_parts(int64_t Npad) { return Npad / EROWS; }
 int64_t vec_max_parts(int64_t Npad, int) { return Npad / VROWS; }
 
 hipError_t launch_grm_gemv(hipStream_t s, const float* S, int64_t Npad, int64_t Nnm, const float* in,
-                           float* out, SigmaArgs a, double* partial, int64_t* nparts) {
+                           float* out, SigmaArgs a, double* partial, int64_t* nparts, const GemvPlan* tri) {
+    if (tri) {
+        if (tri->Npad != Npad || !tri->items || Npad % TV_ROWS != 0) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_gemv_tri, dim3((unsigned)tri->nitems), dim3(512), 0, s, S, Npad, in, tri->items,
+                           tri->y1p, tri->y2p);
+        const int64_t nb = Npad / TV_ROWS;
+        hipLaunchKernelGGL(k_gemv_tri_epi, dim3((unsigned)nb), dim3(256), 0, s, tri->y1p, tri->y2p, tri->rtab, Npad,
+                           Nnm, in, out, a, partial);
+        if (nparts) *nparts = nb;
+        return hipGetLastError();
+    }
     const int64_t nb = Npad / GEMV_ROWS;
     hipLaunchKernelGGL(k_gemv, dim3((unsigned)nb), dim3(256), 0, s, S, Npad, Nnm, in, out, a, partial);
     if (nparts) *nparts = nb;
@@ -1334,6 +1460,50 @@ bool gemm_config_valid(GemmConfig g) {
     return (g.rt == 1 || g.rt == 2) && (g.pd == 1 || g.pd == 3) &&
            (g.nsplit == 1 || g.nsplit == 2 || g.nsplit == 4 || g.nsplit == 8) &&
            (g.mode == GEMM_FP32 || g.mode == GEMM_BF16X3 || g.mode == GEMM_BF16X3_TRI);
+}
+
+void gemv_plan_free(GemvPlan& t) {
+    if (t.items) (void)hipFree(t.items);
+    if (t.rtab) (void)hipFree(t.rtab);
+    if (t.y1p) (void)hipFree(t.y1p);
+    if (t.y2p) (void)hipFree(t.y2p);
+    t = GemvPlan{};
+}
+
+hipError_t gemv_plan_build(GemvPlan& t, int64_t Npad, int ch) {
+    if (t.items && t.Npad == Npad && t.ch == ch) return hipSuccess;
+    gemv_plan_free(t);
+    if (ch < 1 || Npad <= 0 || Npad % TV_ROWS != 0) return hipErrorInvalidValue;
+    const int64_t nr = Npad / TV_ROWS;     // row sets, and 256-column chunks
+    std::vector<int4> items;
+    std::vector<int2> rtab((size_t)nr);
+    int64_t slot = 0;
+    for (int64_t R = 0; R < nr; R++) {
+        // the R chunks left of the diagonal block cut into k near-equal items,
+        // then the diagonal block
+        const int64_t k = (R + ch - 1) / ch;
+        rtab[(size_t)R] = make_int2((int)slot, (int)(k + 1));
+        for (int64_t i = 0; i < k; i++)
+            items.push_back(make_int4((int)R, (int)(R * i / k), (int)(R * (i + 1) / k), (int)slot++));
+        items.push_back(make_int4((int)R, (int)R, (int)(R + 1), (int)slot++));
+    }
+    std::stable_sort(items.begin(), items.end(),
+                     [](const int4& a, const int4& b) { return a.z - a.y > b.z - b.y; });
+    const int64_t y2 = triv_y2_first(nr, nr);
+    t.Npad = Npad;
+    t.ch = ch;
+    t.nitems = (int64_t)items.size();
+    t.nr = nr;
+    t.y1_slots = slot;
+    t.y2_slots = y2;
+    hipError_t e = hipMalloc(&t.items, sizeof(int4) * items.size());
+    if (e == hipSuccess) e = hipMalloc(&t.rtab, sizeof(int2) * rtab.size());
+    if (e == hipSuccess) e = hipMalloc(&t.y1p, sizeof(float) * (size_t)(slot * TV_ROWS));
+    if (e == hipSuccess) e = hipMalloc(&t.y2p, sizeof(float) * (size_t)(std::max<int64_t>(y2, 1) * TV_COLS));
+    if (e == hipSuccess) e = hipMemcpy(t.items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(t.rtab, rtab.data(), sizeof(int2) * rtab.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) gemv_plan_free(t);
+    return e;
 }
 
 void tri_plan_free(TriPlan& t) {

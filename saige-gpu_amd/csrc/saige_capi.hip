@@ -100,6 +100,7 @@ void free_geno(saige_ctx* c) {
     dfree(c->d_mv_in); dfree(c->d_mv_blk); dfree(c->d_mv_out); dfree(c->d_mv_ypart);
     dfree(c->d_dots_partial); dfree(c->d_dots_out); dfree(c->d_take_lists);
     tri_plan_free(c->tri);
+    gemv_plan_free(c->triv);
     free_engine(c->eng);
     free_engine(c->eng1);
     c->h_stats.clear();
@@ -337,11 +338,31 @@ int ensure_tri(saige_ctx* c) {
     return SAIGE_OK;
 }
 
+// items of the lower-triangle GEMV: at most TRIV_CHUNKS chunks of 256
+// columns left of a row set's diagonal block
+constexpr int TRIV_CHUNKS = 24;
+
+// the lower-triangle GEMV's plan when that mode is on (else nullptr); rc != 0
+// on an allocation failure
+const GemvPlan* gemv_plan(saige_ctx* c, int* rc) {
+    *rc = SAIGE_OK;
+    if (c->gemv_mode != SAIGE_GEMV_TRI) return nullptr;
+    const hipError_t e = gemv_plan_build(c->triv, c->Npad, TRIV_CHUNKS);
+    if (e != hipSuccess) {
+        saige::set_error("lower-triangle GEMV scratch for Npad=%lld: %s", (long long)c->Npad, hipGetErrorString(e));
+        *rc = e == hipErrorOutOfMemory ? SAIGE_E_NOMEM : SAIGE_E_DEVICE;
+    }
+    return &c->triv;
+}
+
 int engine_step(saige_ctx* c, Engine& e, int n_iter, uint8_t* done_out, int32_t* iters_out) {
     hipStream_t st = c->stream;
     const int64_t Np = c->Npad, N = c->Nnm;
     if (e.kpad != 1)
         if (int rc = ensure_tri(c)) return rc;
+    int prc = 0;
+    const GemvPlan* tv = e.kpad == 1 ? gemv_plan(c, &prc) : nullptr;
+    if (prc) return prc;
     SigmaArgs a;
     a.winv = e.winv;
     a.mask = e.mask;
@@ -369,7 +390,7 @@ int engine_step(saige_ctx* c, Engine& e, int n_iter, uint8_t* done_out, int32_t*
         int64_t np = 0;
         SAIGE_HIP_TRY(hipEventRecord(c->ev0, st));
         if (e.kpad == 1)
-            SAIGE_HIP_TRY(launch_grm_gemv(st, g.S, Np, N, e.P, e.AP, a, e.partial, &np));
+            SAIGE_HIP_TRY(launch_grm_gemv(st, g.S, Np, N, e.P, e.AP, a, e.partial, &np, tv));
         else
             SAIGE_HIP_TRY(launch_grm_gemm(st, g.S, Np, N, e.P, e.AP, a, e.partial, &np, e.ypart,
                                           halves == 3 ? c->gemm_cfg : c->gemm_cfg_half, halves, &c->tri));
@@ -1074,7 +1095,10 @@ static int grm_matvec(saige_ctx* c, int loco, const float* B, int64_t k, float* 
         if (rc) return rc;
         Engine& e = c->eng1;
         SAIGE_HIP_TRY(hipMemcpyAsync(e.stage, B, 4 * N, hipMemcpyHostToDevice, st));
-        SAIGE_HIP_TRY(launch_grm_gemv(st, g.S, Np, N, e.stage, e.AP, a, nullptr, nullptr));
+        int prc = 0;
+        const GemvPlan* tv = gemv_plan(c, &prc);
+        if (prc) return prc;
+        SAIGE_HIP_TRY(launch_grm_gemv(st, g.S, Np, N, e.stage, e.AP, a, nullptr, nullptr, tv));
         SAIGE_HIP_TRY(hipMemcpyAsync(Y, e.AP, 4 * N, hipMemcpyDeviceToHost, st));
         SAIGE_HIP_TRY(hipStreamSynchronize(st));
         return SAIGE_OK;
@@ -1179,7 +1203,10 @@ static int get_crossprod(saige_ctx* c, int loco, const float* b, const float* w,
     a.mode = 1;
     a.kind = nullptr;
     a.xout = nullptr;
-    SAIGE_HIP_TRY(launch_grm_gemv(st, g.S, Np, N, e.stage, e.AP, a, e.partial, nullptr));
+    int prc = 0;
+    const GemvPlan* tv = gemv_plan(c, &prc);
+    if (prc) return prc;
+    SAIGE_HIP_TRY(launch_grm_gemv(st, g.S, Np, N, e.stage, e.AP, a, e.partial, nullptr, tv));
     SAIGE_HIP_TRY(hipMemcpyAsync(out, e.AP, 4 * N, hipMemcpyDeviceToHost, st));
     SAIGE_HIP_TRY(hipStreamSynchronize(st));
     e.h_live[0] = 0;
@@ -1845,6 +1872,26 @@ int saige_set_gemm_mode(saige_ctx* c, int mode) {
         c->gemm_tuned_ms = 0.0;
         c->gemm_tuned_ms_half = 0.0;
     }
+    return SAIGE_OK;
+}
+
+int saige_set_gemv_mode(saige_ctx* c, int mode) {
+    CHECK_CTX(c);
+    if (mode != SAIGE_GEMV_FULL && mode != SAIGE_GEMV_TRI) {
+        set_error("unknown GEMV mode %d (0 = all of S, 1 = the lower triangle)", mode);
+        return SAIGE_E_ARG;
+    }
+    if (int rc = engine_idle(c, "changing the GEMV mode")) return rc;
+    c->gemv_mode = mode;
+    return SAIGE_OK;
+}
+
+int saige_get_gemv_mode(const saige_ctx* c, int* mode) {
+    if (!c) {
+        set_error("null context");
+        return SAIGE_E_ARG;
+    }
+    if (mode) *mode = c->gemv_mode;
     return SAIGE_OK;
 }
 

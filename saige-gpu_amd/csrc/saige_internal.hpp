@@ -71,6 +71,22 @@ struct TriPlan {
 hipError_t tri_plan_build(TriPlan& t, int64_t Npad, int nwv, int ch);
 void tri_plan_free(TriPlan& t);
 
+// The lower-triangle GEMV's work list and partials: row sets of 256 rows,
+// chunks of 256 columns; items as in TriPlan (at most ch chunks left of the
+// diagonal block, or the diagonal block); use-1 partials [slot][256], use-2
+// partials [chunk c][row set R > c][256].
+struct GemvPlan {
+    int64_t Npad = 0;
+    int ch = 0;
+    int64_t nitems = 0, nr = 0, y1_slots = 0, y2_slots = 0;
+    int4* items = nullptr;
+    int2* rtab = nullptr;
+    float* y1p = nullptr;
+    float* y2p = nullptr;
+};
+hipError_t gemv_plan_build(GemvPlan& t, int64_t Npad, int ch);
+void gemv_plan_free(GemvPlan& t);
+
 struct GemmConfig {
     int rt = 1;
     int nsplit = 4;
@@ -163,6 +179,8 @@ struct saige_ctx {
     // order, so a column's bits do not depend on the other half), own rt/pd
     saige::GemmConfig gemm_cfg_half;
     saige::TriPlan tri;                          // lower-triangle product (mode 2), built on first use
+    int gemv_mode = SAIGE_GEMV_TRI;              // single-column products: lower triangle or all of S
+    saige::GemvPlan triv;                        // the lower-triangle GEMV's plan, built on first use
     double gemm_tuned_ms_half = 0.0;
     // FP64 column-dot scratch (saige_engine_take_batch_ex, saige_dots_device)
     double* d_dots_partial = nullptr;
@@ -269,8 +287,11 @@ struct SigmaArgs {
 };
 // out[N][kpad] = epilogue(S * in), plus per-column partial p^T out dots when
 // mode == 1 (partial: [nparts][kpad][2], slot 0).  Returns nparts used.
+// tri: read only the lower triangle (k_gemv_tri + k_gemv_tri_epi) with this
+// plan for Npad; nullptr: k_gemv over all of S.
 hipError_t launch_grm_gemv(hipStream_t s, const float* S, int64_t Npad, int64_t Nnm, const float* in,
-                           float* out, SigmaArgs a, double* partial, int64_t* nparts);
+                           float* out, SigmaArgs a, double* partial, int64_t* nparts,
+                           const GemvPlan* tri = nullptr);
 bool gemm_config_valid(GemmConfig g);
 // ypart: gemm_ypart_floats(Npad) floats of scratch for the split-K slices.
 // halves: bit h set = columns [32h, 32h+32) are computed; the epilogue's

@@ -127,6 +127,8 @@ SIGNATURES = {
     "saige_get_gemm_config_half": (_int, [_c_p, ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.POINTER(_f64)]),
     "saige_set_gemm_mode": (_int, [_c_p, _int]),
     "saige_get_gemm_mode": (_int, [_c_p, ctypes.POINTER(_int)]),
+    "saige_set_gemv_mode": (_int, [_c_p, _int]),
+    "saige_get_gemv_mode": (_int, [_c_p, ctypes.POINTER(_int)]),
     "saige_engine_stats": (_int, [_c_p, _int, ctypes.POINTER(_f64), ctypes.POINTER(_i64)]),
     "saige_device_memory": (_int, [_c_p, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "saige_engine_system_bytes": (_i64, [_c_p]),

@@ -298,6 +298,19 @@ class Session:
         and S_IJ^T P_I).  Clears tuned times."""
         check(self._L.saige_set_gemm_mode(self.ctx, self.GEMM_MODES.index(mode)))
 
+    GEMV_MODES = ("full", "tri")
+
+    def set_gemv_mode(self, mode):
+        """Single-column GRM products: "tri" (the default; the GRM's lower
+        triangle only, each element used for s_ij x_j and s_ij x_i) or
+        "full" (all of S)."""
+        check(self._L.saige_set_gemv_mode(self.ctx, self.GEMV_MODES.index(mode)))
+
+    def gemv_mode(self):
+        m = ctypes.c_int(0)
+        check(self._L.saige_get_gemv_mode(self.ctx, ctypes.byref(m)))
+        return self.GEMV_MODES[m.value]
+
     def gemm_mode(self):
         m = ctypes.c_int(0)
         check(self._L.saige_get_gemm_mode(self.ctx, ctypes.byref(m)))

@@ -406,6 +406,64 @@ def test_gemm_tri_mode_many_items(session):
         session.set_gemm_config(1, 1, 4)
 
 
+def test_gemv_tri_mode(session, grm1000):
+    """Single-column products over the lower triangle (the default) against
+    all of S and the FP64 product: K x, getCrossprod and the single-RHS PCG
+    within FP32 tolerances, reproducible bits, and the mode switch."""
+    og, S = grm1000
+    print("case: GEMV lower-triangle mode", flush=True)
+    session.build_grm("fp32")
+    K64 = _k64(og, S)
+    rng = np.random.default_rng(41)
+    x = rng.standard_normal(1000).astype(np.float32)
+    w = rng.uniform(0.05, 0.25, 1000).astype(np.float32)
+    tau = np.array([1.0, 0.5], np.float32)
+    ref = K64 @ x.astype(np.float64)
+    assert session.gemv_mode() == "tri"
+    try:
+        out = {}
+        for mode in ("full", "tri"):
+            session.set_gemv_mode(mode)
+            kx = session.getCrossprodMatAndKin(x)
+            # the FP32 GRM's own error against FP64 bounds both modes
+            assert np.max(np.abs(kx - ref)) <= 2e-5 * np.max(np.abs(ref)), mode
+            assert np.array_equal(kx, session.getCrossprodMatAndKin(x))
+            cp = session.getCrossprod(x, w, tau)
+            cp64 = _sigma64(og, S, w, tau) @ x.astype(np.float64)
+            assert np.max(np.abs(cp - cp64)) <= 2e-5 * np.max(np.abs(cp64)), mode
+            sol = session.getPCG1ofSigmaAndVector(w, tau, x)
+            x64 = np.linalg.solve(_sigma64(og, S, w, tau), x.astype(np.float64))
+            assert np.max(np.abs(sol - x64)) <= 2e-3 * np.max(np.abs(x64)), mode
+            out[mode] = kx
+        print(f"  max|tri - full| {np.max(np.abs(out['tri'] - out['full'])):.3e}", flush=True)
+        with pytest.raises(Exception):
+            session.set_gemv_mode("half")
+    finally:
+        session.set_gemv_mode("tri")
+
+
+def test_gemv_tri_many_items(session):
+    """N = 7000: 28 row sets of 256 rows, the last ones with their chunks
+    left of the diagonal cut into two items, against the FP64 product of the
+    GPU's own FP32 GRM, elementwise within an FP32 summation bound
+    (8 sqrt(N) 2^-24 sum_j |K_ij x_j|)."""
+    N, M = 7000, 200
+    print("case: GEMV lower-triangle mode, N=7000", flush=True)
+    body, G, mm = random_bed(N, M, 0.01, seed=57, maf_lo=0.05, maf_hi=0.5)
+    session.setminMAFforGRM(0.01)
+    session.setmaxMissingRateforGRM(0.15)
+    session.setgeno_bytes(body, N, M)
+    session.build_grm("fp32")
+    K = session.grm_block(0, 0, N, N).astype(np.float64)
+    rng = np.random.default_rng(6)
+    for _ in range(2):
+        x = rng.standard_normal(N).astype(np.float32)
+        ref = K @ x.astype(np.float64)
+        bound = 8.0 * np.sqrt(N) * 2.0 ** -24 * (np.abs(K) @ np.abs(x.astype(np.float64)))
+        got = session.getCrossprodMatAndKin(x)
+        assert np.all(np.abs(got - ref) <= bound)
+
+
 def test_autotune_and_every_gemm_shape(session, grm1000):
     """saige_autotune times the sixteen block-GEMM shapes and keeps the
     fastest; every shape gives K B within the FP32 product tolerance, and a
