@@ -1207,13 +1207,12 @@ def fit_batched(sess, jobs, opt=None, width=64, n_systems=16, probe_seed=200):
     with keys trait, y, X and optionally subset (sample indices) and tag."""
     opt = opt or Options()
     sch = BatchScheduler(sess, opt, width=width, n_systems=n_systems)
-    # the glm() start values (host IRLS, no GRM) of all fits at once, by the
-    # library's C++ restatement of glm_start, which runs without Python's
-    # lock: the fits' start values overlap on threads
-    from .api import glm_start_native
+    # the glm() start values (host IRLS, no GRM) of all fits at once: numpy
+    # releases the GIL in the O(n) work, so a few threads overlap them
+    fams = {"binary": Binomial, "quantitative": Gaussian}
 
     def start(j):
-        return glm_start_native(j["trait"], np.asarray(j["y"], np.float64), np.asarray(j["X"], np.float64))
+        return glm_start(np.asarray(j["y"], np.float64), np.asarray(j["X"], np.float64), fams[j["trait"]])
 
     if len(jobs) > 1:
         with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as pool:
