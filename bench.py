@@ -10,8 +10,9 @@ and held in HBM as FP32.
 
 A step = one batched round of null fits of the four phenotypes: AI-REML
 with nrun = 15 Rademacher probes per phenotype, so each trace phase is a
-block PCG over 4 x (15 probes + 1) = 64 right-hand sides (FP32 vectors and
-FP32 MFMA GRM products, FP64 dot products).  `value` = fits completed per
+block PCG over 4 x (15 probes + 1) = 64 right-hand sides (FP32 vectors;
+GRM products at FP32 accuracy from three-piece BF16 splits on the MFMA,
+read from the GRM's lower triangle; FP64 dot products).  `value` = fits completed per
 hour of step time.  The plain path (one phenotype, one right-hand side at a
 time, GEMV per PCG iteration) is timed once beside it.
 
@@ -400,10 +401,13 @@ def main():
         plain_s = time.perf_counter() - tp
         g3, n3 = sess.engine_stats("gemv")
         gemv_avg = (g3 - g2) / max(1, n3 - n2)
+        # GB/s of S the product needs: its lower triangle (2 N^2 bytes) in the
+        # default mode, all of it (4 N^2) in the "full" one
+        gemv_bytes = (2.0 if sess.gemv_mode() == "tri" else 4.0) * N * N
         plain = {"seconds": plain_s, "fits_per_hour": 3600.0 / plain_s, "pcg_iters": pr["pcg_iters"],
                  "grm_products": pr["pcg_iters"] + pr["n_products"],
-                 "n_solves": pr["n_solves"], "gemv_ms": gemv_avg * 1e3,
-                 "gemv_GBps": 4.0 * N * N / gemv_avg / 1e9, "theta": list(map(float, pr["theta"]))}
+                 "n_solves": pr["n_solves"], "gemv_ms": gemv_avg * 1e3, "gemv_mode": sess.gemv_mode(),
+                 "gemv_GBps": gemv_bytes / gemv_avg / 1e9, "theta": list(map(float, pr["theta"]))}
         log(f"plain: {plain_s:.1f}s per fit, {plain['fits_per_hour']:.1f} fits/h, GEMV {gemv_avg * 1e3:.2f} ms "
             f"({plain['gemv_GBps']:.0f} GB/s), theta {pr['theta']}")
 
