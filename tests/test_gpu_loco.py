@@ -109,10 +109,22 @@ def test_loco_sigma_and_pcg(session, loaded):
     assert 0 < it < 500
 
 
-def test_loco_engine(session, loaded):
+@pytest.mark.parametrize("gemm", ["fp32", "bf16x3_tri"])
+def test_loco_engine(session, loaded, gemm):
+    """The block engine on the LOCO GRM, with the FP32-MFMA product and with
+    the split product over the lower triangle (which reads the LOCO GRM's
+    lower half and its diagonal blocks' upper halves)."""
     og, L = loaded
-    print("case: LOCO engine", flush=True)
+    print(f"case: LOCO engine, GEMM {gemm}", flush=True)
     session.build_grm("fp32")
+    session.set_gemm_mode(gemm)
+    try:
+        _loco_engine_body(session, og, L)
+    finally:
+        session.set_gemm_mode("fp32")
+
+
+def _loco_engine_body(session, og, L):
     rng = np.random.default_rng(6)
     w = rng.uniform(0.05, 0.25, 1000).astype(np.float32)
     tau = np.array([1.0, 0.5], np.float32)
@@ -210,12 +222,22 @@ def test_loco_loop_vs_oracle(session, loaded, trait):
                 assert np.max(np.abs(got[k] - ref[k])) <= 2e-3 * np.max(np.abs(ref[k])), (j, k)
 
 
-def test_loco_in_place(session, loaded, monkeypatch):
+@pytest.mark.parametrize("gemm", ["fp32", "bf16x3_tri"])
+def test_loco_in_place(session, loaded, monkeypatch, gemm):
     """In-place mode (no second N^2 buffer): the range is subtracted from S
     itself, a new range first adds the old one back, and the next full-GRM
-    use restores S to within a few FP32 ulps."""
+    use restores S to within a few FP32 ulps.  The engine's solve on the
+    in-place LOCO GRM runs with each block-GEMM mode."""
     og, L = loaded
-    print("case: LOCO in place", flush=True)
+    print(f"case: LOCO in place, GEMM {gemm}", flush=True)
+    session.set_gemm_mode(gemm)
+    try:
+        _loco_in_place_body(session, og, L, monkeypatch)
+    finally:
+        session.set_gemm_mode("fp32")
+
+
+def _loco_in_place_body(session, og, L, monkeypatch):
     monkeypatch.setenv("SAIGE_LOCO_INPLACE", "1")
     session.build_grm("fp32")
     K0 = session.grm_block(0, 0, 1000, 1000)
