@@ -81,6 +81,42 @@ def test_batched_matches_plain_and_is_mix_invariant(session, cohort):
     assert sch.n_iterations > 0 and sch.busy_slot_iters > sch.n_iterations
 
 
+@pytest.mark.parametrize("mode", ["bf16x3", "bf16x3_tri"])
+def test_batched_split_modes(session, cohort, mode):
+    """A batched round (four phenotypes and a 1100-sample subset) with the
+    block GEMM in a split mode: tau within 1e-4 of the same round on the
+    FP32 MFMA and within the north-star bound of the FP64 oracle, the
+    subset's columns masked as in FP32 mode, and a fit's bits the same alone
+    and in the mix."""
+    c, grm, X, phenos = cohort
+    print(f"case: batched fits, GEMM {mode}", flush=True)
+    opt = D.Options()
+    rng = np.random.default_rng(4)
+    sub = np.sort(rng.choice(c["N"], 1100, replace=False))
+    Xs, _, _ = plink.covariate_transform(c["X"][sub])
+    jobs = [{"trait": t, "y": y, "X": X, "tag": k} for k, (t, y) in phenos.items()]
+    jobs.append({"trait": "binary", "y": phenos["b20"][1][sub], "X": Xs, "subset": sub, "tag": "b20sub"})
+    ref, _ = D.fit_batched(session, jobs, opt)
+    session.set_gemm_mode(mode)
+    try:
+        session.autotune(reps=1)
+        got, _ = D.fit_batched(session, jobs, opt)
+        alone, _ = D.fit_batched(session, jobs[-1:], opt)
+    finally:
+        session.set_gemm_mode("fp32")
+        session.set_gemm_config(1, 1, 4)
+    for g, r in zip(got, ref):
+        assert g.error is None, (g.tag, g.error)
+        print(f"  {g.tag}: {mode} theta {g.result['theta']} fp32 {r.result['theta']}", flush=True)
+        assert np.allclose(g.result["theta"], r.result["theta"], rtol=1e-4, atol=1e-7), g.tag
+    for name in ("b20", "q40"):
+        i = list(phenos).index(name)
+        o = _oracle_fit(grm, phenos[name][0], phenos[name][1], X, opt)
+        assert np.allclose(got[i].result["theta"], o["theta"], rtol=1e-3, atol=1e-6), name
+    assert np.array_equal(alone[0].result["theta"], got[-1].result["theta"])
+    assert np.array_equal(alone[0].result["coefficients"], got[-1].result["coefficients"])
+
+
 def test_subset_fit_equals_restricted_problem(session, cohort):
     c, grm, X, phenos = cohort
     print("case: subset fit", flush=True)
