@@ -725,7 +725,7 @@ __global__ __launch_bounds__(64 * NWV) void k_gemm_x3t(const float* __restrict__
                                                        float* __restrict__ y2p, int halves) {
     constexpr int GM = 32 * NWV;
     constexpr bool PIPE = NWV == 8;                  // software-pipelined chunk loop (register room)
-    constexpr int RING = PIPE ? 2 : PD + 1;          // raw chunks held
+    constexpr int RING = PD + 1;                     // raw chunks held
     constexpr int NT = FULL ? 2 : 1;                 // 32-column RHS tiles computed
     constexpr int NTH = 64 * NWV;
     constexpr int NSTG = FULL ? 768 : 384;           // B pieces a chunk needs (one half: its 32 columns)
@@ -877,34 +877,51 @@ __global__ __launch_bounds__(64 * NWV) void k_gemm_x3t(const float* __restrict__
         if (PIPE) {
             // software-pipelined: chunk c's pieces were split during chunk c-1,
             // so this chunk's MFMAs run beside the next chunk's split.  Raw
-            // chunks c+1 and c+2 are resident or in flight (slot (c-ca) & 1),
-            // and so are their B pieces; chunk ids past the item's end are
-            // clamped to its last chunk, whose data is then split or staged
-            // unused.  (A deeper ring does not fit the registers.)
+            // chunks c+1 .. c+RING are resident or in flight (slot (c-ca) %
+            // RING), and so are their B pieces; chunk ids past the item's end
+            // are clamped to its last chunk, whose data is then split or
+            // staged unused.  The loop is unrolled by UNR, a multiple of RING
+            // and of 2, so slots and the pieces' set (the chunk's parity) are
+            // compile-time indices.  RING 3 fits the registers of one-half
+            // launches only; the autotuner times both.
+            constexpr int UNR = RING == 3 ? 6 : 2;
             u32x4 pcs[2][3][2];                       // [set][piece][step]
             auto split_into = [&](const float4 (&rs)[4], u32x4 (&p)[3][2]) {
 #pragma unroll
                 for (int st = 0; st < 2; st++) split3_8(rs[2 * st], rs[2 * st + 1], p[0][st], p[1][st], p[2][st]);
             };
-            auto pstep = [&](int u, int64_t c) {
-                u32x4* sbw = u ? sB1 : sB0;
+            // k: the chunk's offset from ca modulo UNR
+            auto pstep = [&](int k, int64_t c) {
+                const int u = k % RING, par = k & 1;
+                u32x4* sbw = par ? sB1 : sB0;
 #pragma unroll
                 for (int st = 0; st < STG; st++) sbw[bdi[st]] = bring[u][st];
                 lds_barrier();
                 if (SRB == 2 && U2 && c > ca) reduce(c - 1, sR[(c - 1) & 1]);
-                split_into(ring[u ^ 1], pcs[u ^ 1]);
-                issue(ring[u], bring[u], c + 2 < cb ? c + 2 : cb - 1);
-                chunk_mfma(u, c, pcs[u][0], pcs[u][1], pcs[u][2]);
+                split_into(ring[(k + 1) % RING], pcs[par ^ 1]);
+                issue(ring[u], bring[u], c + RING < cb ? c + RING : cb - 1);
+                chunk_mfma(par, c, pcs[par][0], pcs[par][1], pcs[par][2]);
             };
-            issue(ring[0], bring[0], ca);
-            issue(ring[1], bring[1], ca + 1 < cb ? ca + 1 : cb - 1);
+#pragma unroll
+            for (int d = 0; d < RING; d++) issue(ring[d], bring[d], ca + d < cb ? ca + d : cb - 1);
             split_into(ring[0], pcs[0]);
             int64_t c = ca;
-            for (; c + 2 <= cb; c += 2) {
-                pstep(0, c);
-                pstep(1, c + 1);
+            if (RING == 2) {
+                // written out: the generic form below allocates more registers here
+                for (; c + 2 <= cb; c += 2) {
+                    pstep(0, c);
+                    pstep(1, c + 1);
+                }
+                if (c < cb) pstep(0, c);
+            } else {
+                for (; c + UNR <= cb; c += UNR) {
+#pragma unroll
+                    for (int k = 0; k < UNR; k++) pstep(k, c + k);
+                }
+#pragma unroll
+                for (int k = 0; k < UNR - 1; k++)
+                    if (c + k < cb) pstep(k, c + k);
             }
-            if (c < cb) pstep(0, c);
         } else {
 #pragma unroll
             for (int d = 0; d < PD; d++) issue(ring[d], bring[d], ca + d < cb ? ca + d : cb - 1);
