@@ -191,6 +191,7 @@ void run_batch(Scheduler& sch, std::vector<std::unique_ptr<FitTask>>& fits, int 
             waiting.pop_front();
             f->sys = free_sys.front();
             free_sys.pop_front();
+            resumed.emplace_back(f, 0);               // its first request is awaited like a resumed fit's
             f->th = std::thread([f, &sch] {
                 try {
                     const saige_fit_spec* s = f->spec;
@@ -214,13 +215,24 @@ void run_batch(Scheduler& sch, std::vector<std::unique_ptr<FitTask>>& fits, int 
         // collect new requests; release finished fits' systems
         {
             std::unique_lock<std::mutex> lk(sch.mu);
-            if (sch.inbox.empty() && live == 0 && pending.empty()) {
+            if (live == 0 && pending.empty()) {
+                // nothing runs: wait until every fit the last step resumed (or
+                // that was just admitted) has posted its next request or
+                // finished, so that the fits' next phases start in the same
+                // iteration and share its launches (fits that drift apart run
+                // more of them); with none such, for any request or finished fit
                 sch.cv.wait(lk, [&] {
+                    if (!resumed.empty()) {
+                        for (auto& fp : resumed)
+                            if (!fp.first->finished && fp.first->posts <= fp.second) return false;
+                        return true;
+                    }
                     if (!sch.inbox.empty()) return true;
                     for (auto& f : fits)
                         if (f->finished && f->sys >= 0) return true;
                     return false;
                 });
+                resumed.clear();
             }
             while (!sch.inbox.empty()) {
                 auto [f, r] = sch.inbox.front();
