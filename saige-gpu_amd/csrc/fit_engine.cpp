@@ -9,7 +9,10 @@
 // No reference counterpart: the reference fits one phenotype per R process
 // through fitNULLGLMM (FG.R:1036-1298), one right-hand side at a time.
 #include <algorithm>
+#include <chrono>
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <memory>
@@ -169,6 +172,13 @@ void run_batch(Scheduler& sch, std::vector<std::unique_ptr<FitTask>>& fits, int 
     std::vector<float> wfull((size_t)N);
     // fits whose request the last step completed, with their post count then
     std::vector<std::pair<FitTask*, int64_t>> resumed;
+    // wall seconds of the loop's parts, printed to stderr at the end when the
+    // environment sets SAIGE_ENGINE_TIMING (diagnostics only)
+    const bool timing = std::getenv("SAIGE_ENGINE_TIMING") != nullptr;
+    double t_idle = 0, t_fill = 0, t_step = 0, t_wait_posts = 0, t_take = 0;
+    int64_t n_steps = 0;
+    using clk = std::chrono::steady_clock;
+    auto secs = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
 
     auto set_system = [&](FitTask* f, const double* w, const double tau[2]) {
         const saige_fit_spec* s = f->spec;
@@ -215,6 +225,7 @@ void run_batch(Scheduler& sch, std::vector<std::unique_ptr<FitTask>>& fits, int 
         // collect new requests; release finished fits' systems
         {
             std::unique_lock<std::mutex> lk(sch.mu);
+            const auto t0 = clk::now();
             if (live == 0 && pending.empty()) {
                 // nothing runs: wait until every fit the last step resumed (or
                 // that was just admitted) has posted its next request or
@@ -234,6 +245,7 @@ void run_batch(Scheduler& sch, std::vector<std::unique_ptr<FitTask>>& fits, int 
                 });
                 resumed.clear();
             }
+            t_idle += secs(t0);
             while (!sch.inbox.empty()) {
                 auto [f, r] = sch.inbox.front();
                 sch.inbox.pop_front();
@@ -252,6 +264,7 @@ void run_batch(Scheduler& sch, std::vector<std::unique_ptr<FitTask>>& fits, int 
                 }
         }
         // fill free slots: one batch of solves and one of products
+        const auto t_fill0 = clk::now();
         std::vector<int32_t> s_slots, s_sys, p_slots, p_sys;
         std::vector<Column> s_cols, p_cols;
         std::deque<Column> keep;
@@ -304,6 +317,7 @@ void run_batch(Scheduler& sch, std::vector<std::unique_ptr<FitTask>>& fits, int 
             check(saige_engine_add_products(sch.ctx, (int)p_slots.size(), p_slots.data(), p_sys.data(),
                                             colbuf.data()));
         }
+        t_fill += secs(t_fill0);
         if (live == 0) continue;
         // the step runs on a helper thread while the fits that the previous
         // step resumed compute their next request, so that its columns join
@@ -314,6 +328,7 @@ void run_batch(Scheduler& sch, std::vector<std::unique_ptr<FitTask>>& fits, int 
             step_rc = saige_engine_step(sch.ctx, 1, done.data(), iters.data());
             if (step_rc != SAIGE_OK) step_err = saige_last_error();   // the message is per thread
         });
+        const auto t_step0 = clk::now();
         if (!resumed.empty()) {
             std::unique_lock<std::mutex> lk(sch.mu);
             sch.cv.wait(lk, [&] {
@@ -323,8 +338,12 @@ void run_batch(Scheduler& sch, std::vector<std::unique_ptr<FitTask>>& fits, int 
             });
             resumed.clear();
         }
+        t_wait_posts += secs(t_step0);
         stepper.join();
+        t_step += secs(t_step0);
+        n_steps++;
         if (step_rc != SAIGE_OK) throw std::runtime_error(step_err);
+        const auto t_take0 = clk::now();
         std::vector<int32_t> fin;
         for (int s = 0; s < sch.width; s++)
             if (owner[(size_t)s].req && done[(size_t)s]) fin.push_back(s);
@@ -361,7 +380,13 @@ void run_batch(Scheduler& sch, std::vector<std::unique_ptr<FitTask>>& fits, int 
                 }
             }
         }
+        t_take += secs(t_take0);
     }
+    if (timing)
+        std::fprintf(stderr,
+                     "saige engine: %lld steps; idle (waiting for requests) %.4f s, fill %.4f s, step %.4f s "
+                     "(of which waiting for resumed fits %.4f s), take %.4f s\n",
+                     (long long)n_steps, t_idle, t_fill, t_step, t_wait_posts, t_take);
     } catch (const std::exception& e) {
         // the engine failed: fail every request in the inbox, waiting for
         // slots or in a slot, so that every admitted fit's thread ends; later
