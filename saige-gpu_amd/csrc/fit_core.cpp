@@ -166,17 +166,28 @@ double dot(const std::vector<double>& a, const std::vector<double>& b) {
 
 // ----------------------------------------------------------------- families
 
-double logistic(double eta) { return 1.0 / (1.0 + std::exp(-eta)); }
-double logistic_mu_eta(double eta) {
-    const double e = std::exp(-std::fabs(eta));
-    return e / ((1.0 + e) * (1.0 + e));
-}
-
+// binomial(logit) or gaussian(identity).  linkinv and mu_eta run over n
+// samples with the exponentials taken as arrays (ex: n values of scratch):
+// logit mu = 1 / (1 + exp(-eta)), mu_eta = e / (1 + e)^2 with e = exp(-|eta|)
 struct Family {
     bool binomial;
-    double linkinv(double eta) const { return binomial ? logistic(eta) : eta; }
-    double mu_eta(double eta) const { return binomial ? logistic_mu_eta(eta) : 1.0; }
     double variance(double mu) const { return binomial ? mu * (1.0 - mu) : 1.0; }
+    void linkinv(const double* eta, int64_t n, double* mu, double* ex) const {
+        if (!binomial) {
+            std::copy(eta, eta + n, mu);
+            return;
+        }
+        vmath::exp_neg(eta, ex, n);
+        for (int64_t i = 0; i < n; i++) mu[i] = 1.0 / (1.0 + ex[i]);
+    }
+    void mu_eta(const double* eta, int64_t n, double* me, double* ex) const {
+        if (!binomial) {
+            std::fill(me, me + n, 1.0);
+            return;
+        }
+        vmath::exp_neg_abs(eta, ex, n);
+        for (int64_t i = 0; i < n; i++) me[i] = ex[i] / ((1.0 + ex[i]) * (1.0 + ex[i]));
+    }
 };
 
 // ------------------------------------------------------------- the solvers
@@ -218,13 +229,14 @@ CoefState get_coef(Solver& s, const std::vector<double>& y, const Matrix& X, con
                    const Options& opt) {
     const double tol_coef = 0.1;
     const int64_t n = X.rows;
-    std::vector<double> Y((size_t)n), W((size_t)n), mu((size_t)n), eta;
+    std::vector<double> Y((size_t)n), W((size_t)n), mu((size_t)n), eta, me((size_t)n), ex((size_t)n);
     auto working = [&](const std::vector<double>& e) {
+        fam.linkinv(e.data(), n, mu.data(), ex.data());
+        fam.mu_eta(e.data(), n, me.data(), ex.data());
         for (int64_t i = 0; i < n; i++) {
-            const double m = fam.linkinv(e[(size_t)i]), me = fam.mu_eta(e[(size_t)i]);
-            mu[(size_t)i] = m;
-            Y[(size_t)i] = e[(size_t)i] - offset[(size_t)i] + (y[(size_t)i] - m) / me;
-            const double r = me / std::sqrt(fam.variance(m));
+            const double m = mu[(size_t)i];
+            Y[(size_t)i] = e[(size_t)i] - offset[(size_t)i] + (y[(size_t)i] - m) / me[(size_t)i];
+            const double r = me[(size_t)i] / std::sqrt(fam.variance(m));
             W[(size_t)i] = r * r;
         }
     };
@@ -497,29 +509,81 @@ Matrix inv_sympd(const Matrix& A0) {
     return pinv_sym(A);
 }
 
+namespace {
+
+// A = X^T W X, b = X^T W z (X's p columns xc, W = diag(wt)) in one pass; P > 0
+// is p fixed at compile time, so the sums stay in registers
+template <int P>
+void normal_equations(const double* const* xc, int64_t p, int64_t n, const double* wt, const double* z, Matrix& A,
+                      Matrix& b) {
+    if (P > 0) {
+        constexpr int Q = P > 0 ? P : 1;
+        double a[Q][Q] = {}, bb[Q] = {};
+        const double* x[Q];
+        for (int j = 0; j < Q; j++) x[j] = xc[j];
+        for (int64_t i = 0; i < n; i++) {
+            double xw[Q];
+            for (int j = 0; j < Q; j++) xw[j] = x[j][i] * wt[i];
+            for (int j = 0; j < Q; j++) {
+                bb[j] += xw[j] * z[i];
+                for (int k = 0; k <= j; k++) a[j][k] += xw[j] * x[k][i];
+            }
+        }
+        for (int j = 0; j < Q; j++) {
+            b.a[(size_t)j] = bb[j];
+            for (int k = 0; k <= j; k++) A(j, k) = A(k, j) = a[j][k];
+        }
+        return;
+    }
+    std::vector<double> xw((size_t)p);
+    for (int64_t i = 0; i < n; i++) {
+        for (int64_t j = 0; j < p; j++) xw[(size_t)j] = xc[j][i] * wt[i];
+        for (int64_t j = 0; j < p; j++) {
+            b.a[(size_t)j] += xw[(size_t)j] * z[i];
+            for (int64_t k = 0; k <= j; k++) A(j, k) += xw[(size_t)j] * xc[k][i];
+        }
+    }
+    for (int64_t j = 0; j < p; j++)
+        for (int64_t k = 0; k < j; k++) A(k, j) = A(j, k);
+}
+
+}  // namespace
+
 GlmStart glm_start(Trait trait, const std::vector<double>& y, const Matrix& X, const std::vector<double>& offset,
                    int maxit, double epsilon) {
     const int64_t n = X.rows, p = X.cols;
     const Family fam{trait == Trait::Binary};
     std::vector<double> off = offset.empty() ? std::vector<double>((size_t)n, 0.0) : offset;
-    std::vector<double> mu((size_t)n), eta((size_t)n);
+    std::vector<double> mu((size_t)n), eta((size_t)n), me((size_t)n), ex((size_t)n);
     bool binary01 = fam.binomial;
     for (int64_t i = 0; i < n; i++) {
         if (fam.binomial) {
             mu[(size_t)i] = (y[(size_t)i] + 0.5) / 2.0;
-            eta[(size_t)i] = std::log(mu[(size_t)i] / (1.0 - mu[(size_t)i]));
+            eta[(size_t)i] = mu[(size_t)i] / (1.0 - mu[(size_t)i]);
             if (y[(size_t)i] != 0.0 && y[(size_t)i] != 1.0) binary01 = false;
         } else {
             mu[(size_t)i] = eta[(size_t)i] = y[(size_t)i];
         }
     }
+    if (fam.binomial) vmath::log(eta.data(), eta.data(), n);
     auto deviance = [&](const std::vector<double>& m) {
         double d = 0.0;
+        if (binary01) {
+            // y in {0, 1}: the unit deviance is -2 log(mu) or -2 log(1 - mu)
+            // y m + (1 - y)(1 - m) is m or 1 - m exactly for y in {0, 1}
+            for (int64_t i = 0; i < n; i++)
+                ex[(size_t)i] = y[(size_t)i] * m[(size_t)i] + (1.0 - y[(size_t)i]) * (1.0 - m[(size_t)i]);
+            vmath::log(ex.data(), ex.data(), n);
+            double d4[4] = {0.0, 0.0, 0.0, 0.0};
+            int64_t i = 0;
+            for (; i + 4 <= n; i += 4)
+                for (int k = 0; k < 4; k++) d4[k] += ex[(size_t)(i + k)];
+            for (; i < n; i++) d4[0] += ex[(size_t)i];
+            return -2.0 * ((d4[0] + d4[1]) + (d4[2] + d4[3]));
+        }
         for (int64_t i = 0; i < n; i++) {
             const double yi = y[(size_t)i], mi = m[(size_t)i];
-            if (binary01) {
-                d += std::log(yi > 0 ? mi : 1.0 - mi);
-            } else if (fam.binomial) {
+            if (fam.binomial) {
                 const double a = yi > 0 ? yi * std::log(yi / mi) : 0.0;
                 const double b = yi < 1 ? (1 - yi) * std::log((1 - yi) / (1 - mi)) : 0.0;
                 d += a + b;
@@ -527,31 +591,39 @@ GlmStart glm_start(Trait trait, const std::vector<double>& y, const Matrix& X, c
                 d += (yi - mi) * (yi - mi);
             }
         }
-        if (binary01) return -2.0 * d;
         return fam.binomial ? 2.0 * d : d;
     };
     double dev_old = deviance(mu);
     std::vector<double> beta((size_t)p, 0.0);
+    std::vector<const double*> xc((size_t)p);
+    for (int64_t j = 0; j < p; j++) xc[(size_t)j] = X.col(j);
+    std::vector<double> z((size_t)n);
     for (int it = 0; it < maxit; it++) {
-        Matrix Xw(n, p);
-        std::vector<double> z((size_t)n);
+        // the working response z and weights (in me), then the weighted
+        // normal equations X^T W X beta = X^T W z
+        fam.mu_eta(eta.data(), n, me.data(), ex.data());
         for (int64_t i = 0; i < n; i++) {
-            const double me = fam.mu_eta(eta[(size_t)i]);
-            z[(size_t)i] = (eta[(size_t)i] - off[(size_t)i]) + (y[(size_t)i] - mu[(size_t)i]) / me;
-            const double wt = me * me / fam.variance(mu[(size_t)i]);
-            for (int64_t j = 0; j < p; j++) Xw(i, j) = X(i, j) * wt;
+            const double mei = me[(size_t)i];
+            z[(size_t)i] = (eta[(size_t)i] - off[(size_t)i]) + (y[(size_t)i] - mu[(size_t)i]) / mei;
+            me[(size_t)i] = mei * mei / fam.variance(mu[(size_t)i]);
         }
-        const Matrix A = tmul(X, Xw);
-        Matrix b(p, 1);
-        const std::vector<double> rhs = tmul(Xw, z);
-        std::copy(rhs.begin(), rhs.end(), b.a.begin());
+        Matrix A(p, p), b(p, 1);
+        switch (p) {
+            case 1: normal_equations<1>(xc.data(), p, n, me.data(), z.data(), A, b); break;
+            case 2: normal_equations<2>(xc.data(), p, n, me.data(), z.data(), A, b); break;
+            case 3: normal_equations<3>(xc.data(), p, n, me.data(), z.data(), A, b); break;
+            case 4: normal_equations<4>(xc.data(), p, n, me.data(), z.data(), A, b); break;
+            default: normal_equations<0>(xc.data(), p, n, me.data(), z.data(), A, b);
+        }
         const Matrix sol = solve_dense(A, b);
         beta.assign(sol.a.begin(), sol.a.end());
-        eta = mul(X, beta);
+        // the other pass: eta, mu
         for (int64_t i = 0; i < n; i++) {
-            eta[(size_t)i] += off[(size_t)i];
-            mu[(size_t)i] = fam.linkinv(eta[(size_t)i]);
+            double e = off[(size_t)i];
+            for (int64_t j = 0; j < p; j++) e += xc[(size_t)j][i] * beta[(size_t)j];
+            eta[(size_t)i] = e;
         }
+        fam.linkinv(eta.data(), n, mu.data(), ex.data());
         const double dev = deviance(mu);
         if (std::fabs(dev - dev_old) / (std::fabs(dev) + 0.1) < epsilon) break;
         dev_old = dev;

@@ -108,6 +108,16 @@ def glm_start(y, X, family, offset=None, maxit=25, epsilon=1e-8):
     return {"coef": beta, "eta": eta, "mu": mu, "offset": off}
 
 
+def glm_start_library(y, X, family, offset=None):
+    """glm_start by the library's C++ IRLS (saige_glm_start, the same
+    restatement in FP64 with array exp/log): it holds no Python lock, so the
+    start values of several fits run at once on threads.  The GPU paths
+    (fit_batched, fit_plain, the serving loop) take their start values from
+    it; the dense test executors keep glm_start."""
+    from .api import glm_start_native
+    return glm_start_native("binary" if family.name == "binomial" else "quantitative", y, X, offset)
+
+
 # ------------------------------------------------------------------ probes
 
 
@@ -389,15 +399,16 @@ def fitglmmai_rpcg_q_co(Y, X, w, tau, Sigma_iY, Sigma_iX, cov, probes, opt):
 
 # ------------------------------------------------------------ R drivers
 
-def fit_binary_co(y, X, probes, opt, offset=None, fit0=None, tau_init=(0.0, 0.0)):
-    """glmmkin.ai_PCG_Rcpp_Binary (FG.R:79-304) without LOCO."""
+def fit_binary_co(y, X, probes, opt, offset=None, fit0=None, tau_init=(0.0, 0.0), start=glm_start):
+    """glmmkin.ai_PCG_Rcpp_Binary (FG.R:79-304) without LOCO.  fit0: the
+    glm() start values, else start(y, X, family, offset) computes them."""
     y = np.asarray(y, np.float64)
     X = np.asarray(X, np.float64)
     n = len(y)
     offset = np.zeros(n) if offset is None else np.asarray(offset, np.float64)
     fam = Binomial
     if fit0 is None:
-        fit0 = glm_start(y, X, fam, offset)
+        fit0 = start(y, X, fam, offset)
     eta = fit0["eta"]
     alpha0 = fit0["coef"]
     eta0 = eta
@@ -431,15 +442,16 @@ def fit_binary_co(y, X, probes, opt, offset=None, fit0=None, tau_init=(0.0, 0.0)
             "history": history, "iterations": it, "traitType": "binary"}
 
 
-def fit_quantitative_co(y, X, probes, opt, offset=None, fit0=None):
-    """glmmkin.ai_PCG_Rcpp_Quantitative (FG.R:309-549) without LOCO."""
+def fit_quantitative_co(y, X, probes, opt, offset=None, fit0=None, start=glm_start):
+    """glmmkin.ai_PCG_Rcpp_Quantitative (FG.R:309-549) without LOCO; fit0
+    and start as in fit_binary_co."""
     y = np.asarray(y, np.float64)
     X = np.asarray(X, np.float64)
     n = len(y)
     offset = np.zeros(n) if offset is None else np.asarray(offset, np.float64)
     fam = Gaussian
     if fit0 is None:
-        fit0 = glm_start(y, X, fam, offset)
+        fit0 = start(y, X, fam, offset)
     eta = fit0["eta"]
     alpha = fit0["coef"]
     alpha0 = alpha
@@ -1135,6 +1147,7 @@ def fit_plain(sess, trait, y, X, opt=None, probe_seed=200, **kw):
     opt = opt or Options()
     probes = RademacherProbes.shared(len(y), probe_seed)
     ex = PlainExecutor(sess, opt)
+    kw.setdefault("start", glm_start_library)
     res = ex.run(fit_co(trait, y, X, probes, opt, **kw))
     res["n_solves"] = ex.n_solves
     res["pcg_iters"] = ex.pcg_iters
@@ -1202,23 +1215,24 @@ def loco(sess, jobs, starts, ends, opt=None, batched=True, width=64, n_systems=1
     return out
 
 
-def fit_batched(sess, jobs, opt=None, width=64, n_systems=16, probe_seed=200):
+def fit_batched(sess, jobs, opt=None, width=64, n_systems=16, probe_seed=200, start=glm_start_library):
     """Many null fits through the batching scheduler.  jobs: list of dicts
-    with keys trait, y, X and optionally subset (sample indices) and tag."""
+    with keys trait, y, X and optionally subset (sample indices) and tag.
+    start(y, X, family, offset) gives the glm() start values."""
     opt = opt or Options()
     sch = BatchScheduler(sess, opt, width=width, n_systems=n_systems)
-    # the glm() start values (host IRLS, no GRM) of all fits at once: numpy
-    # releases the GIL in the O(n) work, so a few threads overlap them
+    # the glm() start values (host IRLS, no GRM) of all fits at once, on
+    # threads: the library's IRLS holds no Python lock
     fams = {"binary": Binomial, "quantitative": Gaussian}
 
-    def start(j):
-        return glm_start(np.asarray(j["y"], np.float64), np.asarray(j["X"], np.float64), fams[j["trait"]])
+    def start_values(j):
+        return start(np.asarray(j["y"], np.float64), np.asarray(j["X"], np.float64), fams[j["trait"]])
 
     if len(jobs) > 1:
         with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as pool:
-            fit0s = list(pool.map(start, jobs))
+            fit0s = list(pool.map(start_values, jobs))
     else:
-        fit0s = [start(j) for j in jobs]
+        fit0s = [start_values(j) for j in jobs]
     handles = []
     for j, fit0 in zip(jobs, fit0s):
         n = len(j["y"])

@@ -27,10 +27,14 @@ P = ctypes.POINTER
 
 @pytest.fixture(scope="module")
 def lib(tmp_path_factory):
-    out = str(tmp_path_factory.mktemp("fitdense") / "libfitdense.so")
+    tmp = tmp_path_factory.mktemp("fitdense")
+    out, vm = str(tmp / "libfitdense.so"), str(tmp / "vmath.o")
+    # the array exp/log file is built as the library's Makefile builds it
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffast-math", "-fopenmp-simd", "-c",
+                    os.path.join(CSRC, "vmath.cpp"), "-o", vm], check=True)
     subprocess.run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-I", CSRC,
-                    os.path.join(HERE, "native", "fit_dense.cpp"), os.path.join(CSRC, "fit_core.cpp"),
-                    "-o", out], check=True)
+                    os.path.join(HERE, "native", "fit_dense.cpp"), os.path.join(CSRC, "fit_core.cpp"), vm,
+                    "-lmvec", "-lm", "-o", out], check=True)
     L = ctypes.CDLL(out)
     d, i64, u64 = P(ctypes.c_double), ctypes.c_int64, ctypes.c_uint64
     L.fitnull_dense.argtypes = [ctypes.c_int, i64, ctypes.c_int, d, d, d, ctypes.c_int, ctypes.c_double,
