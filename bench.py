@@ -94,7 +94,7 @@ def serve_leg(sess, N, Xraw, Zc, n_req, rate, seed, opt):
             g = synth.polygenic(Zc[sub], h2, seed + r)
             y = synth.quantitative_phenotype(Xraw[sub], g, h2, seed + r)
             trait = "quantitative"
-        gen = D.fit_co(trait, y, Xs, D.RademacherProbes(len(sub), 200), opt, start=D.glm_start_library)
+        gen = D.fit_co(trait, y, Xs, D.RademacherProbes(len(sub), 200), opt, lib=True)
         arrivals.append((t, gen, sub, (trait, len(sub))))
     sch = D.BatchScheduler(sess, opt, width=64, n_systems=16)
     t0 = time.perf_counter()

@@ -443,6 +443,19 @@ int saige_score_test_null(const double* mu, const double* V, const double* y, co
  * Host only, no context: callers may run several at once on their threads. */
 int saige_glm_start(int trait, int64_t n, const double* y, const double* X, int p, const double* offset,
                     double* coef, double* eta, double* mu);
+/* Get_Coef's working response and weights at eta (FG.R:9-12, 24-27), the
+ * host step between its getCoefficients calls: mu = linkinv(eta),
+ * Y = eta - offset + (y - mu) / mu_eta(eta), W = (mu_eta(eta) /
+ * sqrt(variance(mu)))^2, all n long; offset may be NULL (0).  Host only. */
+int saige_glm_working(int trait, int64_t n, const double* y, const double* eta, const double* offset, double* mu,
+                      double* Y, double* W);
+/* getCoefficients (FG.cpp:3167-3199) after its solves, for hosts that batch
+ * the solves themselves: sol = Sigma^-1 [Y, X] (n x (1+p), column-major),
+ * X (n x p, column-major), w and Y (n).  Outputs cov = inv_sympd(X^T Sigma^-1
+ * X) (p x p), alpha = cov (Sigma^-1 X)^T Y (p) and eta = Y - tau0
+ * (Sigma^-1 Y - Sigma^-1 X alpha) / w (n).  Host only. */
+int saige_coefficients(int64_t n, int p, const double* Y, const double* X, const double* w, double tau0,
+                       const double* sol, double* cov, double* alpha, double* eta);
 
 #ifdef __cplusplus
 }

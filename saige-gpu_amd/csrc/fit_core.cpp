@@ -154,10 +154,15 @@ std::vector<double> tmul(const Matrix& A, const std::vector<double>& x) {
     return y;
 }
 
+// four interleaved partial sums, added pairwise at the end: independent
+// add chains instead of one
 double dot(const double* a, const double* b, int64_t n) {
-    double s = 0.0;
-    for (int64_t i = 0; i < n; i++) s += a[i] * b[i];
-    return s;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    int64_t i = 0;
+    for (; i + 4 <= n; i += 4)
+        for (int k = 0; k < 4; k++) s[k] += a[i + k] * b[i + k];
+    for (; i < n; i++) s[0] += a[i] * b[i];
+    return (s[0] + s[1]) + (s[2] + s[3]);
 }
 
 double dot(const std::vector<double>& a, const std::vector<double>& b) {
@@ -210,11 +215,11 @@ Coefficients get_coefficients(Solver& s, const std::vector<double>& Y, const Mat
     c.Sigma_iY.assign(sol.col(0), sol.col(0) + n);
     c.Sigma_iX = Matrix(n, p);
     std::copy(sol.col(1), sol.col(1) + n * p, c.Sigma_iX.a.begin());
-    c.cov = inv_sympd(tmul(X, c.Sigma_iX));
-    c.alpha = mul(c.cov, tmul(c.Sigma_iX, Y));
-    const std::vector<double> sxa = mul(c.Sigma_iX, c.alpha);
+    c.cov = Matrix(p, p);
+    c.alpha.resize((size_t)p);
     c.eta.resize((size_t)n);
-    for (int64_t i = 0; i < n; i++) c.eta[(size_t)i] = Y[(size_t)i] - tau[0] * (c.Sigma_iY[(size_t)i] - sxa[(size_t)i]) / w[(size_t)i];
+    coefficients_from_solution(n, p, Y.data(), X.a.data(), w.data(), tau[0], sol.a.data(), c.cov.a.data(),
+                               c.alpha.data(), c.eta.data());
     return c;
 }
 
@@ -229,16 +234,10 @@ CoefState get_coef(Solver& s, const std::vector<double>& y, const Matrix& X, con
                    const Options& opt) {
     const double tol_coef = 0.1;
     const int64_t n = X.rows;
-    std::vector<double> Y((size_t)n), W((size_t)n), mu((size_t)n), eta, me((size_t)n), ex((size_t)n);
+    std::vector<double> Y((size_t)n), W((size_t)n), mu((size_t)n), eta;
+    const Trait trait = fam.binomial ? Trait::Binary : Trait::Quantitative;
     auto working = [&](const std::vector<double>& e) {
-        fam.linkinv(e.data(), n, mu.data(), ex.data());
-        fam.mu_eta(e.data(), n, me.data(), ex.data());
-        for (int64_t i = 0; i < n; i++) {
-            const double m = mu[(size_t)i];
-            Y[(size_t)i] = e[(size_t)i] - offset[(size_t)i] + (y[(size_t)i] - m) / me[(size_t)i];
-            const double r = me[(size_t)i] / std::sqrt(fam.variance(m));
-            W[(size_t)i] = r * r;
-        }
+        glm_working(trait, n, y.data(), e.data(), offset.data(), mu.data(), Y.data(), W.data());
     };
     working(eta0);
     Coefficients rc;
@@ -548,6 +547,84 @@ void normal_equations(const double* const* xc, int64_t p, int64_t n, const doubl
 }
 
 }  // namespace
+
+void glm_working(Trait trait, int64_t n, const double* y, const double* eta, const double* offset, double* mu,
+                 double* Y, double* W) {
+    const Family fam{trait == Trait::Binary};
+    // W is the exponentials' scratch and Y holds mu_eta until it is read
+    fam.linkinv(eta, n, mu, W);
+    fam.mu_eta(eta, n, Y, W);
+    for (int64_t i = 0; i < n; i++) {
+        const double m = mu[i], me = Y[i];
+        Y[i] = eta[i] - (offset ? offset[i] : 0.0) + (y[i] - m) / me;
+        const double r = me / std::sqrt(fam.variance(m));
+        W[i] = r * r;
+    }
+}
+
+namespace {
+
+// A = X^T S (p x p) and b = S^T Y (p) over n rows (X, S column-major with p
+// columns) in one pass; P > 0 is p fixed at compile time, so the sums stay in
+// registers, else one dot per entry
+template <int P>
+void cross_products(const double* X, const double* S, const double* Y, int64_t n, int64_t p, Matrix& A,
+                    std::vector<double>& b) {
+    if (P > 0) {
+        constexpr int Q = P > 0 ? P : 1;
+        double a[Q][Q] = {}, bb[Q] = {};
+        for (int64_t i = 0; i < n; i++) {
+            double x[Q], sv[Q];
+            for (int j = 0; j < Q; j++) {
+                x[j] = X[j * n + i];
+                sv[j] = S[j * n + i];
+            }
+            for (int j = 0; j < Q; j++) {
+                bb[j] += sv[j] * Y[i];
+                for (int k = 0; k < Q; k++) a[j][k] += x[j] * sv[k];
+            }
+        }
+        for (int j = 0; j < Q; j++) {
+            b[(size_t)j] = bb[j];
+            for (int k = 0; k < Q; k++) A(j, k) = a[j][k];
+        }
+        return;
+    }
+    for (int64_t j = 0; j < p; j++) {
+        b[(size_t)j] = dot(S + j * n, Y, n);
+        for (int64_t k = 0; k < p; k++) A(j, k) = dot(X + j * n, S + k * n, n);
+    }
+}
+
+}  // namespace
+
+void coefficients_from_solution(int64_t n, int64_t p, const double* Y, const double* X, const double* w, double tau0,
+                                const double* sol, double* cov, double* alpha, double* eta) {
+    const double* siY = sol;
+    const double* siX = sol + n;
+    // X^T Sigma^-1 X and (Sigma^-1 X)^T Y
+    Matrix A(p, p);
+    std::vector<double> sy((size_t)p);
+    switch (p) {
+        case 1: cross_products<1>(X, siX, Y, n, p, A, sy); break;
+        case 2: cross_products<2>(X, siX, Y, n, p, A, sy); break;
+        case 3: cross_products<3>(X, siX, Y, n, p, A, sy); break;
+        case 4: cross_products<4>(X, siX, Y, n, p, A, sy); break;
+        default: cross_products<0>(X, siX, Y, n, p, A, sy);
+    }
+    const Matrix C = inv_sympd(A);
+    for (int64_t j = 0; j < p; j++) {
+        double a = 0.0;
+        for (int64_t k = 0; k < p; k++) a += C(j, k) * sy[(size_t)k];
+        alpha[j] = a;
+    }
+    std::copy(C.a.begin(), C.a.end(), cov);
+    for (int64_t i = 0; i < n; i++) {
+        double sxa = 0.0;
+        for (int64_t k = 0; k < p; k++) sxa += siX[k * n + i] * alpha[k];
+        eta[i] = Y[i] - tau0 * (siY[i] - sxa) / w[i];
+    }
+}
 
 GlmStart glm_start(Trait trait, const std::vector<double>& y, const Matrix& X, const std::vector<double>& offset,
                    int maxit, double epsilon) {

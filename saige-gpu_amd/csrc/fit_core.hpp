@@ -97,6 +97,19 @@ struct GlmStart {
 GlmStart glm_start(Trait trait, const std::vector<double>& y, const Matrix& X, const std::vector<double>& offset,
                    int maxit = 25, double epsilon = 1e-8);
 
+// Get_Coef's working response and weights at eta (FG.R:9-12, 24-27):
+// mu = linkinv(eta), Y = eta - offset + (y - mu) / mu_eta(eta),
+// W = (mu_eta(eta) / sqrt(variance(mu)))^2.  offset may be null (0).
+void glm_working(Trait trait, int64_t n, const double* y, const double* eta, const double* offset, double* mu,
+                 double* Y, double* W);
+
+// getCoefficients (FG.cpp:3167-3199) after its solves, sol = Sigma^-1 [Y, X]
+// (n x (1+p), column-major): cov = inv_sympd(X^T Sigma^-1 X), alpha = cov
+// (Sigma^-1 X)^T Y, eta = Y - tau0 (Sigma^-1 Y - Sigma^-1 X alpha) / w.
+// X is n x p column-major; cov is p x p.
+void coefficients_from_solution(int64_t n, int64_t p, const double* Y, const double* X, const double* w, double tau0,
+                                const double* sol, double* cov, double* alpha, double* eta);
+
 // Covariate_Transform (FG.R:1612-1647): X -> Q sqrt(n) of its QR
 // decomposition, after dropping columns that are linear combinations of
 // earlier ones (|R_jj| <= 1e-10 max(1, max |R_ii|)).  Householder QR with

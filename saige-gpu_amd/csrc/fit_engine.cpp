@@ -550,4 +550,20 @@ int saige_glm_start(int trait, int64_t n, const double* y, const double* X, int 
     return SAIGE_OK;
 }
 
+int saige_glm_working(int trait, int64_t n, const double* y, const double* eta, const double* offset, double* mu,
+                      double* Y, double* W) {
+    if (!y || !eta || n <= 0 || !mu || !Y || !W || (trait != SAIGE_TRAIT_BINARY && trait != SAIGE_TRAIT_QUANTITATIVE))
+        return SAIGE_E_ARG;
+    saige::fit::glm_working(trait == SAIGE_TRAIT_BINARY ? saige::fit::Trait::Binary : saige::fit::Trait::Quantitative,
+                            n, y, eta, offset, mu, Y, W);
+    return SAIGE_OK;
+}
+
+int saige_coefficients(int64_t n, int p, const double* Y, const double* X, const double* w, double tau0,
+                       const double* sol, double* cov, double* alpha, double* eta) {
+    if (!Y || !X || !w || !sol || n <= 0 || p <= 0 || !cov || !alpha || !eta) return SAIGE_E_ARG;
+    saige::fit::coefficients_from_solution(n, p, Y, X, w, tau0, sol, cov, alpha, eta);
+    return SAIGE_OK;
+}
+
 }  // extern "C"

@@ -61,6 +61,8 @@ SIGNATURES = {
     "saige_covariate_transform": (_int, [_pd, _i64, _int, _pd, _pd, ctypes.POINTER(_i32), ctypes.POINTER(_int)]),
     "saige_score_test_null": (_int, [_pd, _pd, _pd, _pd, _i64, _int, _pd, _pd, _pd, _pd, _pd, _pd]),
     "saige_glm_start": (_int, [_int, _i64, _pd, _pd, _int, _pd, _pd, _pd, _pd]),
+    "saige_glm_working": (_int, [_int, _i64, _pd, _pd, _pd, _pd, _pd, _pd]),
+    "saige_coefficients": (_int, [_i64, _int, _pd, _pd, _pd, _f64, _pd, _pd, _pd, _pd]),
     "saige_create": (_int, [_int, ctypes.POINTER(_c_p)]),
     "saige_destroy": (_int, [_c_p]),
     "saige_last_error": (ctypes.c_char_p, []),

@@ -569,6 +569,42 @@ def glm_start_native(trait, y, X, offset=None):
     return {"coef": coef, "eta": eta, "mu": mu, "offset": np.zeros(n) if off is None else off}
 
 
+def glm_working_native(trait, y, eta, offset=None):
+    """Get_Coef's working response (FG.R:9-12, 24-27) by the library's C++:
+    returns (mu, Y, W) as saige_gpu.driver.get_coef_co computes them."""
+    L = _lib.load()
+    y = np.ascontiguousarray(y, np.float64)
+    eta = np.ascontiguousarray(eta, np.float64)
+    off = None if offset is None else np.ascontiguousarray(offset, np.float64)
+    n = len(y)
+    mu, Y, W = np.empty(n), np.empty(n), np.empty(n)
+    pd = ctypes.POINTER(ctypes.c_double)
+    t = {"binary": 0, "quantitative": 1}[trait]
+    check(L.saige_glm_working(t, n, y.ctypes.data_as(pd), eta.ctypes.data_as(pd),
+                              None if off is None else off.ctypes.data_as(pd), mu.ctypes.data_as(pd),
+                              Y.ctypes.data_as(pd), W.ctypes.data_as(pd)))
+    return mu, Y, W
+
+
+def coefficients_native(Y, X, w, tau0, sol):
+    """getCoefficients' algebra after its solves (FG.cpp:3167-3199) by the
+    library's C++: sol = Sigma^-1 [Y, X] (n x (1+p)); X must be column-major
+    FP64 (it is not copied).  Returns (cov, alpha, eta)."""
+    L = _lib.load()
+    Y = np.ascontiguousarray(Y, np.float64)
+    w = np.ascontiguousarray(w, np.float64)
+    sol = np.asfortranarray(sol, np.float64)
+    assert X.dtype == np.float64 and X.flags["F_CONTIGUOUS"]
+    n, p = X.shape
+    assert sol.shape == (n, p + 1) and len(Y) == n and len(w) == n
+    cov, alpha, eta = np.zeros((p, p), order="F"), np.zeros(p), np.empty(n)
+    pd = ctypes.POINTER(ctypes.c_double)
+    check(L.saige_coefficients(n, p, Y.ctypes.data_as(pd), X.ctypes.data_as(pd), w.ctypes.data_as(pd),
+                               float(tau0), sol.ctypes.data_as(pd), cov.ctypes.data_as(pd), alpha.ctypes.data_as(pd),
+                               eta.ctypes.data_as(pd)))
+    return cov, alpha, eta
+
+
 def covariate_transform_native(X):
     """Covariate_Transform (FG.R:1612-1647) by the library's C++ code:
     returns (X_new, R, kept_columns) as saige_gpu.plink.covariate_transform."""

@@ -111,11 +111,26 @@ def glm_start(y, X, family, offset=None, maxit=25, epsilon=1e-8):
 def glm_start_library(y, X, family, offset=None):
     """glm_start by the library's C++ IRLS (saige_glm_start, the same
     restatement in FP64 with array exp/log): it holds no Python lock, so the
-    start values of several fits run at once on threads.  The GPU paths
-    (fit_batched, fit_plain, the serving loop) take their start values from
-    it; the dense test executors keep glm_start."""
+    start values of several fits run at once on threads.  The drivers take
+    it with lib=True, as the GPU paths (fit_batched, fit_plain, the serving
+    loop) run them; the dense test executors keep glm_start."""
     from .api import glm_start_native
     return glm_start_native("binary" if family.name == "binomial" else "quantitative", y, X, offset)
+
+
+_POOLS = {}
+
+
+def _shared_pool(name, workers):
+    """A process-wide thread pool, made on first use and kept: starting a
+    pool's threads anew costs milliseconds in every round of fits.  The
+    "step" pool has one worker, so engine iterations handed to it run one at
+    a time."""
+    pool = _POOLS.get(name)
+    if pool is None:
+        pool = _POOLS[name] = concurrent.futures.ThreadPoolExecutor(max_workers=workers,
+                                                                    thread_name_prefix=f"saige-{name}")
+    return pool
 
 
 # ------------------------------------------------------------------ probes
@@ -233,9 +248,16 @@ def cal_cv(x):
     return (np.std(x, ddof=1) / np.mean(x)) / len(x)
 
 
-def get_coefficients_co(Y, X, w, tau, opt):
-    """getCoefficients (FG.cpp:3167-3199): Sigma^-1 [Y, X] in one request."""
+def get_coefficients_co(Y, X, w, tau, opt, lib=False):
+    """getCoefficients (FG.cpp:3167-3199): Sigma^-1 [Y, X] in one request.
+    lib: the algebra after the solve by the library's C++
+    (saige_coefficients; X column-major FP64), else numpy."""
     sol = yield ("solve", w, tau, cols(Y, X))
+    if lib:
+        from .api import coefficients_native
+        sol = np.asfortranarray(sol, np.float64)
+        cov, alpha, eta = coefficients_native(Y, X, w, tau[0], sol)
+        return {"Sigma_iY": sol[:, 0], "Sigma_iX": sol[:, 1:], "cov": cov, "alpha": alpha, "eta": eta}
     Sigma_iY = np.ascontiguousarray(sol[:, 0])
     Sigma_iX = np.ascontiguousarray(sol[:, 1:])
     cov = _inv_sympd(X.T @ Sigma_iX)
@@ -244,21 +266,29 @@ def get_coefficients_co(Y, X, w, tau, opt):
     return {"Sigma_iY": Sigma_iY, "Sigma_iX": Sigma_iX, "cov": cov, "alpha": alpha, "eta": eta}
 
 
-def get_coef_co(y, X, tau, fam, alpha0, eta0, offset, opt):
-    """Get_Coef (FG.R:2-35)."""
+def get_coef_co(y, X, tau, fam, alpha0, eta0, offset, opt, lib=False):
+    """Get_Coef (FG.R:2-35).  lib: the working response and the algebra
+    after each solve by the library's C++ (saige_glm_working,
+    saige_coefficients), else numpy."""
     tol_coef = 0.1
-    mu = fam.linkinv(eta0)
-    me = fam.mu_eta(eta0)
-    Y = eta0 - offset + (y - mu) / me
-    W = (me / np.sqrt(fam.variance(mu))) ** 2
+    if lib:
+        from .api import glm_working_native
+        trait = "binary" if fam.name == "binomial" else "quantitative"
+        X = np.asfortranarray(X, np.float64)
+
+        def working(e):
+            return glm_working_native(trait, y, e, offset)
+    else:
+        def working(e):
+            mu = fam.linkinv(e)
+            me = fam.mu_eta(e)
+            return mu, e - offset + (y - mu) / me, (me / np.sqrt(fam.variance(mu))) ** 2
+    mu, Y, W = working(eta0)
     for _ in range(opt.maxiter):
-        rc = yield from get_coefficients_co(Y, X, W, tau, opt)
+        rc = yield from get_coefficients_co(Y, X, W, tau, opt, lib)
         alpha = rc["alpha"]
         eta = rc["eta"] + offset
-        mu = fam.linkinv(eta)
-        me = fam.mu_eta(eta)
-        Y = eta - offset + (y - mu) / me
-        W = (me / np.sqrt(fam.variance(mu))) ** 2
+        mu, Y, W = working(eta)
         if np.max(np.abs(alpha - alpha0) / (np.abs(alpha) + np.abs(alpha0) + tol_coef)) < tol_coef:
             break
         alpha0 = alpha
@@ -399,22 +429,24 @@ def fitglmmai_rpcg_q_co(Y, X, w, tau, Sigma_iY, Sigma_iX, cov, probes, opt):
 
 # ------------------------------------------------------------ R drivers
 
-def fit_binary_co(y, X, probes, opt, offset=None, fit0=None, tau_init=(0.0, 0.0), start=glm_start):
+def fit_binary_co(y, X, probes, opt, offset=None, fit0=None, tau_init=(0.0, 0.0), lib=False):
     """glmmkin.ai_PCG_Rcpp_Binary (FG.R:79-304) without LOCO.  fit0: the
-    glm() start values, else start(y, X, family, offset) computes them."""
+    glm() start values, else computed here.  lib: the start values, Get_Coef's
+    working response and getCoefficients' algebra by the library's C++
+    (glm_start_library, get_coef_co), else by numpy."""
     y = np.asarray(y, np.float64)
     X = np.asarray(X, np.float64)
     n = len(y)
     offset = np.zeros(n) if offset is None else np.asarray(offset, np.float64)
     fam = Binomial
     if fit0 is None:
-        fit0 = start(y, X, fam, offset)
+        fit0 = (glm_start_library if lib else glm_start)(y, X, fam, offset)
     eta = fit0["eta"]
     alpha0 = fit0["coef"]
     eta0 = eta
     tau = np.array([1.0, 0.1 if tau_init[1] == 0 else float(tau_init[1])])   # FG.R:148-157
     tau0 = tau.copy()
-    rc = yield from get_coef_co(y, X, tau, fam, alpha0, eta0, offset, opt)
+    rc = yield from get_coef_co(y, X, tau, fam, alpha0, eta0, offset, opt, lib)
     re = yield from ai_score_co(rc["Y"], X, rc["W"], tau, rc["Sigma_iY"], rc["Sigma_iX"], rc["cov"],
                                 probes, opt)
     tau[1] = max(0.0, tau0[1] + tau0[1] ** 2 * (re["YPAPY"] - re["Trace"]) / n)   # FG.R:165
@@ -424,7 +456,7 @@ def fit_binary_co(y, X, probes, opt, offset=None, fit0=None, tau_init=(0.0, 0.0)
         alpha0 = rc["alpha"]
         tau0 = tau.copy()
         eta0 = eta
-        rc = yield from get_coef_co(y, X, tau, fam, alpha0, eta0, offset, opt)
+        rc = yield from get_coef_co(y, X, tau, fam, alpha0, eta0, offset, opt, lib)
         tau, _ = yield from fitglmmai_rpcg_co(rc["Y"], X, rc["W"], tau, rc["Sigma_iY"], rc["Sigma_iX"],
                                               rc["cov"], probes, opt)
         eta = rc["eta"]
@@ -436,29 +468,29 @@ def fit_binary_co(y, X, probes, opt, offset=None, fit0=None, tau_init=(0.0, 0.0)
         if np.max(tau) > opt.tol ** (-2):
             it = opt.maxiter
             break
-    rc = yield from get_coef_co(y, X, tau, fam, rc["alpha"], eta, offset, opt)      # FG.R:218
+    rc = yield from get_coef_co(y, X, tau, fam, rc["alpha"], eta, offset, opt, lib)  # FG.R:218
     return {"theta": tau, "coefficients": rc["alpha"], "linear_predictors": rc["eta"],
             "fitted_values": rc["mu"], "Y": rc["Y"], "cov": rc["cov"], "converged": it < opt.maxiter,
             "history": history, "iterations": it, "traitType": "binary"}
 
 
-def fit_quantitative_co(y, X, probes, opt, offset=None, fit0=None, start=glm_start):
+def fit_quantitative_co(y, X, probes, opt, offset=None, fit0=None, lib=False):
     """glmmkin.ai_PCG_Rcpp_Quantitative (FG.R:309-549) without LOCO; fit0
-    and start as in fit_binary_co."""
+    and lib as in fit_binary_co."""
     y = np.asarray(y, np.float64)
     X = np.asarray(X, np.float64)
     n = len(y)
     offset = np.zeros(n) if offset is None else np.asarray(offset, np.float64)
     fam = Gaussian
     if fit0 is None:
-        fit0 = start(y, X, fam, offset)
+        fit0 = (glm_start_library if lib else glm_start)(y, X, fam, offset)
     eta = fit0["eta"]
     alpha = fit0["coef"]
     alpha0 = alpha
     eta0 = eta
     tau = np.array([1.0, 0.0])                                   # FG.R:389-398
     tau0 = tau.copy()
-    rc = yield from get_coef_co(y, X, tau, fam, alpha0, eta0, offset, opt)
+    rc = yield from get_coef_co(y, X, tau, fam, alpha0, eta0, offset, opt, lib)
     re = yield from ai_score_co(rc["Y"], X, rc["W"], tau, rc["Sigma_iY"], rc["Sigma_iX"], rc["cov"],
                                 probes, opt, quantitative=True)
     tau[1] = max(0.0, tau0[1] + tau0[1] ** 2 * (re["YPAPY"] - re["Trace"][1]) / n)   # FG.R:405-406
@@ -469,7 +501,7 @@ def fit_quantitative_co(y, X, probes, opt, offset=None, fit0=None, start=glm_sta
         alpha0 = alpha
         tau0 = tau.copy()
         eta0 = eta
-        rc = yield from get_coef_co(y, X, tau, fam, alpha0, eta0, offset, opt)
+        rc = yield from get_coef_co(y, X, tau, fam, alpha0, eta0, offset, opt, lib)
         tau, _ = yield from fitglmmai_rpcg_q_co(rc["Y"], X, rc["W"], tau, rc["Sigma_iY"], rc["Sigma_iX"],
                                                 rc["cov"], probes, opt)
         alpha = rc["alpha"]
@@ -484,7 +516,7 @@ def fit_quantitative_co(y, X, probes, opt, offset=None, fit0=None, start=glm_sta
         if np.max(tau) > opt.tol ** (-2):
             it = opt.maxiter
             break
-    rc = yield from get_coef_co(y, X, tau, fam, alpha, eta, offset, opt)           # FG.R:475
+    rc = yield from get_coef_co(y, X, tau, fam, alpha, eta, offset, opt, lib)       # FG.R:475
     return {"theta": tau, "coefficients": rc["alpha"], "linear_predictors": rc["eta"],
             "fitted_values": rc["mu"], "Y": rc["Y"], "cov": rc["cov"], "converged": it < opt.maxiter,
             "history": history, "iterations": it, "traitType": "quantitative"}
@@ -747,7 +779,7 @@ class BatchScheduler:
             tune_host_allocator()
         self.sess = sess
         self.host_threads = max(1, int(host_threads))
-        self._host_pool = concurrent.futures.ThreadPoolExecutor(max_workers=self.host_threads) \
+        self._host_pool = _shared_pool(f"host{self.host_threads}", self.host_threads) \
             if self.host_threads > 1 else None
         self.opt = opt
         self.width = width
@@ -774,7 +806,7 @@ class BatchScheduler:
         self.busy_slot_iters = 0
         # wall-time split of run(): device iterations, slot traffic, fit logic
         self.timing = collections.Counter()
-        self._pool = concurrent.futures.ThreadPoolExecutor(max_workers=1)
+        self._pool = _shared_pool("step", 1)
         self._inflight = None                   # engine_step running in the worker
         self.n_overlapped = 0                   # iterations that ran during host work
         self._devbufs = {}                      # system -> device buffers of its trace rounds
@@ -1147,7 +1179,7 @@ def fit_plain(sess, trait, y, X, opt=None, probe_seed=200, **kw):
     opt = opt or Options()
     probes = RademacherProbes.shared(len(y), probe_seed)
     ex = PlainExecutor(sess, opt)
-    kw.setdefault("start", glm_start_library)
+    kw.setdefault("lib", True)
     res = ex.run(fit_co(trait, y, X, probes, opt, **kw))
     res["n_solves"] = ex.n_solves
     res["pcg_iters"] = ex.pcg_iters
@@ -1215,29 +1247,30 @@ def loco(sess, jobs, starts, ends, opt=None, batched=True, width=64, n_systems=1
     return out
 
 
-def fit_batched(sess, jobs, opt=None, width=64, n_systems=16, probe_seed=200, start=glm_start_library):
+def fit_batched(sess, jobs, opt=None, width=64, n_systems=16, probe_seed=200, lib=True):
     """Many null fits through the batching scheduler.  jobs: list of dicts
     with keys trait, y, X and optionally subset (sample indices) and tag.
-    start(y, X, family, offset) gives the glm() start values."""
+    lib as in fit_binary_co."""
     opt = opt or Options()
     sch = BatchScheduler(sess, opt, width=width, n_systems=n_systems)
     # the glm() start values (host IRLS, no GRM) of all fits at once, on
     # threads: the library's IRLS holds no Python lock
     fams = {"binary": Binomial, "quantitative": Gaussian}
 
+    start = glm_start_library if lib else glm_start
+
     def start_values(j):
         return start(np.asarray(j["y"], np.float64), np.asarray(j["X"], np.float64), fams[j["trait"]])
 
     if len(jobs) > 1:
-        with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as pool:
-            fit0s = list(pool.map(start_values, jobs))
+        fit0s = list(_shared_pool("start", 8).map(start_values, jobs))
     else:
         fit0s = [start_values(j) for j in jobs]
     handles = []
     for j, fit0 in zip(jobs, fit0s):
         n = len(j["y"])
         probes = RademacherProbes.shared(n, probe_seed)
-        gen = fit_co(j["trait"], j["y"], j["X"], probes, opt, fit0=fit0)
+        gen = fit_co(j["trait"], j["y"], j["X"], probes, opt, fit0=fit0, lib=lib)
         handles.append(sch.submit(gen, j.get("subset"), j.get("tag")))
     try:
         sch.run()

@@ -199,3 +199,47 @@ def test_library_glm_start_export():
                 assert np.allclose(got[k], ref[k], rtol=1e-10, atol=1e-12), (trait, o is None, k)
     g = glm_start_native("binary", yb, X)
     assert np.max(np.abs(X.T @ (yb - g["mu"]))) < 1e-6 * N
+
+
+def test_library_get_coef_helpers():
+    """saige_glm_working and saige_coefficients (Get_Coef's working response
+    and getCoefficients' algebra after the solve, which the GPU paths take
+    from the library) against the numpy forms of driver.get_coef_co, for
+    both families, with and without an offset.  The coefficients are also
+    pinned by a closed form: with Sigma = diag(1/w) (tau = (1, 0)), sol is
+    W [Y, X] and alpha is the weighted least-squares fit of Y on X."""
+    from saige_gpu.api import coefficients_native, glm_working_native
+    rng = np.random.default_rng(12)
+    N = 2500
+    X, _, _ = plink.covariate_transform(np.column_stack([np.ones(N), rng.normal(50, 10, N), rng.random(N) < 0.5]))
+    Xf = np.asfortranarray(X)
+    eta = X @ np.array([-0.05, 0.3, 0.1]) + 0.2 * rng.standard_normal(N)
+    yb = (rng.random(N) < 1 / (1 + np.exp(-eta))).astype(np.float64)
+    yq = eta + rng.standard_normal(N)
+    off = 0.1 * rng.standard_normal(N)
+    for trait, fam, y in (("binary", D.Binomial, yb), ("quantitative", D.Gaussian, yq)):
+        for o in (None, off):
+            oo = np.zeros(N) if o is None else o
+            mu, Y, W = glm_working_native(trait, y, eta, o)
+            mr = fam.linkinv(eta)
+            me = fam.mu_eta(eta)
+            assert np.allclose(mu, mr, rtol=1e-14, atol=0), (trait, o is None)
+            assert np.allclose(Y, eta - oo + (y - mr) / me, rtol=1e-13, atol=1e-14), (trait, o is None)
+            assert np.allclose(W, (me / np.sqrt(fam.variance(mr))) ** 2, rtol=1e-13, atol=0), (trait, o is None)
+    w = 0.2 + rng.random(N)
+    Y = rng.standard_normal(N)
+    sol = np.asfortranarray(np.column_stack([Y * w, X * w[:, None]]))
+    cov, alpha, e = coefficients_native(Y, Xf, w, 1.0, sol)
+    XtWX = X.T @ (X * w[:, None])
+    assert np.allclose(cov, np.linalg.inv(XtWX), rtol=1e-10, atol=1e-14)
+    assert np.allclose(alpha, np.linalg.solve(XtWX, X.T @ (w * Y)), rtol=1e-10, atol=1e-12)
+    assert np.allclose(e, X @ alpha, rtol=1e-9, atol=1e-11)      # Y - (WY - WX alpha) / w
+    # a general solution block against the numpy algebra of get_coefficients_co
+    sol = np.asfortranarray(rng.standard_normal((N, 4)))
+    cov, alpha, e = coefficients_native(Y, Xf, w, 0.7, sol)
+    siY, siX = sol[:, 0], sol[:, 1:]
+    cr = D._inv_sympd(X.T @ siX)
+    ar = cr @ (siX.T @ Y)
+    assert np.allclose(cov, cr, rtol=1e-9, atol=1e-13)
+    assert np.allclose(alpha, ar, rtol=1e-9, atol=1e-13)
+    assert np.allclose(e, Y - 0.7 * (siY - siX @ ar) / w, rtol=1e-9, atol=1e-12)

@@ -270,7 +270,7 @@ def test_scheduler_equals_one_column_at_a_time(cohort):
     for j, h in zip(jobs, handles):
         probes = D.RademacherProbes(len(j["y"]), 200)
         ref = OneColumnExecutor(FakeEngine(K, kdiag), opt, j.get("subset")).run(
-            D.fit_co(j["trait"], j["y"], j["X"], probes, opt, start=D.glm_start_library))
+            D.fit_co(j["trait"], j["y"], j["X"], probes, opt, lib=True))
         assert np.array_equal(h.result["theta"], ref["theta"]), j["tag"]
         assert np.array_equal(h.result["coefficients"], ref["coefficients"]), j["tag"]
         assert np.array_equal(h.result["linear_predictors"], ref["linear_predictors"]), j["tag"]
@@ -292,7 +292,7 @@ def test_device_trace_rounds_match(cohort):
     for j, h in zip(jobs, handles):
         probes = D.RademacherProbes(len(j["y"]), 200)
         ref = OneColumnExecutor(FakeEngine(K, kdiag), opt, j.get("subset")).run(
-            D.fit_co(j["trait"], j["y"], j["X"], probes, opt, start=D.glm_start_library))
+            D.fit_co(j["trait"], j["y"], j["X"], probes, opt, lib=True))
         assert h.result["iterations"] == ref["iterations"], j["tag"]
         assert np.allclose(h.result["theta"], ref["theta"], rtol=1e-6, atol=1e-9), j["tag"]
         assert np.allclose(h.result["coefficients"], ref["coefficients"], rtol=1e-6, atol=1e-9), j["tag"]
