@@ -510,17 +510,18 @@ Matrix inv_sympd(const Matrix& A0) {
 
 namespace {
 
-// A = X^T W X, b = X^T W z (X's p columns xc, W = diag(wt)) in one pass; P > 0
-// is p fixed at compile time, so the sums stay in registers
+// A += X^T W X, b += X^T W z over rows i0 + [0, len) (X's p columns xc; wt
+// and z hold those rows' values), lower triangle of A only; P > 0 is p fixed
+// at compile time, so the sums stay in registers
 template <int P>
-void normal_equations(const double* const* xc, int64_t p, int64_t n, const double* wt, const double* z, Matrix& A,
-                      Matrix& b) {
+void normal_equations(const double* const* xc, int64_t p, int64_t i0, int64_t len, const double* wt,
+                      const double* z, Matrix& A, Matrix& b) {
     if (P > 0) {
         constexpr int Q = P > 0 ? P : 1;
         double a[Q][Q] = {}, bb[Q] = {};
         const double* x[Q];
-        for (int j = 0; j < Q; j++) x[j] = xc[j];
-        for (int64_t i = 0; i < n; i++) {
+        for (int j = 0; j < Q; j++) x[j] = xc[j] + i0;
+        for (int64_t i = 0; i < len; i++) {
             double xw[Q];
             for (int j = 0; j < Q; j++) xw[j] = x[j][i] * wt[i];
             for (int j = 0; j < Q; j++) {
@@ -529,21 +530,19 @@ void normal_equations(const double* const* xc, int64_t p, int64_t n, const doubl
             }
         }
         for (int j = 0; j < Q; j++) {
-            b.a[(size_t)j] = bb[j];
-            for (int k = 0; k <= j; k++) A(j, k) = A(k, j) = a[j][k];
+            b.a[(size_t)j] += bb[j];
+            for (int k = 0; k <= j; k++) A(j, k) += a[j][k];
         }
         return;
     }
     std::vector<double> xw((size_t)p);
-    for (int64_t i = 0; i < n; i++) {
-        for (int64_t j = 0; j < p; j++) xw[(size_t)j] = xc[j][i] * wt[i];
+    for (int64_t i = 0; i < len; i++) {
+        for (int64_t j = 0; j < p; j++) xw[(size_t)j] = xc[j][i0 + i] * wt[i];
         for (int64_t j = 0; j < p; j++) {
             b.a[(size_t)j] += xw[(size_t)j] * z[i];
-            for (int64_t k = 0; k <= j; k++) A(j, k) += xw[(size_t)j] * xc[k][i];
+            for (int64_t k = 0; k <= j; k++) A(j, k) += xw[(size_t)j] * xc[k][i0 + i];
         }
     }
-    for (int64_t j = 0; j < p; j++)
-        for (int64_t k = 0; k < j; k++) A(k, j) = A(j, k);
 }
 
 }  // namespace
@@ -630,8 +629,8 @@ GlmStart glm_start(Trait trait, const std::vector<double>& y, const Matrix& X, c
                    int maxit, double epsilon) {
     const int64_t n = X.rows, p = X.cols;
     const Family fam{trait == Trait::Binary};
-    std::vector<double> off = offset.empty() ? std::vector<double>((size_t)n, 0.0) : offset;
-    std::vector<double> mu((size_t)n), eta((size_t)n), me((size_t)n), ex((size_t)n);
+    const double* off = offset.empty() ? nullptr : offset.data();
+    std::vector<double> mu((size_t)n), eta((size_t)n);
     bool binary01 = fam.binomial;
     for (int64_t i = 0; i < n; i++) {
         if (fam.binomial) {
@@ -643,23 +642,35 @@ GlmStart glm_start(Trait trait, const std::vector<double>& y, const Matrix& X, c
         }
     }
     if (fam.binomial) vmath::log(eta.data(), eta.data(), n);
-    auto deviance = [&](const std::vector<double>& m) {
-        double d = 0.0;
+    // the samples go in tiles of T, so each pass's temporaries stay in cache
+    constexpr int64_t T = 2048;
+    std::vector<double> tmp((size_t)(4 * T));
+    double* ex = tmp.data();
+    double* me = ex + T;
+    double* z = me + T;
+    double* wt = z + T;
+    auto deviance = [&](const double* m) {
         if (binary01) {
-            // y in {0, 1}: the unit deviance is -2 log(mu) or -2 log(1 - mu)
-            // y m + (1 - y)(1 - m) is m or 1 - m exactly for y in {0, 1}
-            for (int64_t i = 0; i < n; i++)
-                ex[(size_t)i] = y[(size_t)i] * m[(size_t)i] + (1.0 - y[(size_t)i]) * (1.0 - m[(size_t)i]);
-            vmath::log(ex.data(), ex.data(), n);
+            // y in {0, 1}: the unit deviance is -2 log(mu) or -2 log(1 - mu);
+            // y m + (1 - y)(1 - m) is m or 1 - m exactly
             double d4[4] = {0.0, 0.0, 0.0, 0.0};
-            int64_t i = 0;
-            for (; i + 4 <= n; i += 4)
-                for (int k = 0; k < 4; k++) d4[k] += ex[(size_t)(i + k)];
-            for (; i < n; i++) d4[0] += ex[(size_t)i];
+            for (int64_t i0 = 0; i0 < n; i0 += T) {
+                const int64_t len = std::min(T, n - i0);
+                for (int64_t k = 0; k < len; k++) {
+                    const double yi = y[(size_t)(i0 + k)], mi = m[i0 + k];
+                    ex[k] = yi * mi + (1.0 - yi) * (1.0 - mi);
+                }
+                vmath::log(ex, ex, len);
+                int64_t k = 0;
+                for (; k + 4 <= len; k += 4)
+                    for (int q = 0; q < 4; q++) d4[q] += ex[k + q];
+                for (; k < len; k++) d4[0] += ex[k];
+            }
             return -2.0 * ((d4[0] + d4[1]) + (d4[2] + d4[3]));
         }
+        double d = 0.0;
         for (int64_t i = 0; i < n; i++) {
-            const double yi = y[(size_t)i], mi = m[(size_t)i];
+            const double yi = y[(size_t)i], mi = m[i];
             if (fam.binomial) {
                 const double a = yi > 0 ? yi * std::log(yi / mi) : 0.0;
                 const double b = yi < 1 ? (1 - yi) * std::log((1 - yi) / (1 - mi)) : 0.0;
@@ -670,38 +681,46 @@ GlmStart glm_start(Trait trait, const std::vector<double>& y, const Matrix& X, c
         }
         return fam.binomial ? 2.0 * d : d;
     };
-    double dev_old = deviance(mu);
+    double dev_old = deviance(mu.data());
     std::vector<double> beta((size_t)p, 0.0);
     std::vector<const double*> xc((size_t)p);
     for (int64_t j = 0; j < p; j++) xc[(size_t)j] = X.col(j);
-    std::vector<double> z((size_t)n);
     for (int it = 0; it < maxit; it++) {
-        // the working response z and weights (in me), then the weighted
-        // normal equations X^T W X beta = X^T W z
-        fam.mu_eta(eta.data(), n, me.data(), ex.data());
-        for (int64_t i = 0; i < n; i++) {
-            const double mei = me[(size_t)i];
-            z[(size_t)i] = (eta[(size_t)i] - off[(size_t)i]) + (y[(size_t)i] - mu[(size_t)i]) / mei;
-            me[(size_t)i] = mei * mei / fam.variance(mu[(size_t)i]);
-        }
+        // per tile: the working response z and weights wt, then the tile's
+        // share of the weighted normal equations X^T W X beta = X^T W z
         Matrix A(p, p), b(p, 1);
-        switch (p) {
-            case 1: normal_equations<1>(xc.data(), p, n, me.data(), z.data(), A, b); break;
-            case 2: normal_equations<2>(xc.data(), p, n, me.data(), z.data(), A, b); break;
-            case 3: normal_equations<3>(xc.data(), p, n, me.data(), z.data(), A, b); break;
-            case 4: normal_equations<4>(xc.data(), p, n, me.data(), z.data(), A, b); break;
-            default: normal_equations<0>(xc.data(), p, n, me.data(), z.data(), A, b);
+        for (int64_t i0 = 0; i0 < n; i0 += T) {
+            const int64_t len = std::min(T, n - i0);
+            fam.mu_eta(eta.data() + i0, len, me, ex);
+            for (int64_t k = 0; k < len; k++) {
+                const int64_t i = i0 + k;
+                const double mi = mu[(size_t)i];
+                z[k] = (eta[(size_t)i] - (off ? off[i] : 0.0)) + (y[(size_t)i] - mi) / me[k];
+                wt[k] = me[k] * me[k] / fam.variance(mi);
+            }
+            switch (p) {
+                case 1: normal_equations<1>(xc.data(), p, i0, len, wt, z, A, b); break;
+                case 2: normal_equations<2>(xc.data(), p, i0, len, wt, z, A, b); break;
+                case 3: normal_equations<3>(xc.data(), p, i0, len, wt, z, A, b); break;
+                case 4: normal_equations<4>(xc.data(), p, i0, len, wt, z, A, b); break;
+                default: normal_equations<0>(xc.data(), p, i0, len, wt, z, A, b);
+            }
         }
+        for (int64_t j = 0; j < p; j++)
+            for (int64_t k = 0; k < j; k++) A(k, j) = A(j, k);
         const Matrix sol = solve_dense(A, b);
         beta.assign(sol.a.begin(), sol.a.end());
-        // the other pass: eta, mu
-        for (int64_t i = 0; i < n; i++) {
-            double e = off[(size_t)i];
-            for (int64_t j = 0; j < p; j++) e += xc[(size_t)j][i] * beta[(size_t)j];
-            eta[(size_t)i] = e;
+        // eta = X beta + offset and mu, per tile
+        for (int64_t i0 = 0; i0 < n; i0 += T) {
+            const int64_t len = std::min(T, n - i0);
+            for (int64_t i = i0; i < i0 + len; i++) {
+                double e = off ? off[i] : 0.0;
+                for (int64_t j = 0; j < p; j++) e += xc[(size_t)j][i] * beta[(size_t)j];
+                eta[(size_t)i] = e;
+            }
+            fam.linkinv(eta.data() + i0, len, mu.data() + i0, ex);
         }
-        fam.linkinv(eta.data(), n, mu.data(), ex.data());
-        const double dev = deviance(mu);
+        const double dev = deviance(mu.data());
         if (std::fabs(dev - dev_old) / (std::fabs(dev) + 0.1) < epsilon) break;
         dev_old = dev;
     }

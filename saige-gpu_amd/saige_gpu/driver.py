@@ -435,7 +435,7 @@ def fit_binary_co(y, X, probes, opt, offset=None, fit0=None, tau_init=(0.0, 0.0)
     working response and getCoefficients' algebra by the library's C++
     (glm_start_library, get_coef_co), else by numpy."""
     y = np.asarray(y, np.float64)
-    X = np.asarray(X, np.float64)
+    X = np.asfortranarray(X, np.float64) if lib else np.asarray(X, np.float64)
     n = len(y)
     offset = np.zeros(n) if offset is None else np.asarray(offset, np.float64)
     fam = Binomial
@@ -478,7 +478,7 @@ def fit_quantitative_co(y, X, probes, opt, offset=None, fit0=None, lib=False):
     """glmmkin.ai_PCG_Rcpp_Quantitative (FG.R:309-549) without LOCO; fit0
     and lib as in fit_binary_co."""
     y = np.asarray(y, np.float64)
-    X = np.asarray(X, np.float64)
+    X = np.asfortranarray(X, np.float64) if lib else np.asarray(X, np.float64)
     n = len(y)
     offset = np.zeros(n) if offset is None else np.asarray(offset, np.float64)
     fam = Gaussian
@@ -1259,18 +1259,29 @@ def fit_batched(sess, jobs, opt=None, width=64, n_systems=16, probe_seed=200, li
 
     start = glm_start_library if lib else glm_start
 
-    def start_values(j):
-        return start(np.asarray(j["y"], np.float64), np.asarray(j["X"], np.float64), fams[j["trait"]])
+    # the inputs converted once here (one column-major copy per distinct
+    # covariate matrix in lib mode), so the threads below copy nothing under
+    # the Python lock and the drivers' own conversions are no-ops
+    conv = {}
 
+    def covariates(X):
+        if id(X) not in conv:
+            conv[id(X)] = (X, np.asfortranarray(X, np.float64) if lib else np.asarray(X, np.float64))
+        return conv[id(X)][1]
+
+    args = [(np.ascontiguousarray(j["y"], np.float64), covariates(j["X"]), fams[j["trait"]]) for j in jobs]
     if len(jobs) > 1:
-        fit0s = list(_shared_pool("start", 8).map(start_values, jobs))
+        # the main thread takes the last fit's start values itself
+        futs = [_shared_pool("start", 8).submit(start, *a) for a in args[:-1]]
+        last = start(*args[-1])
+        fit0s = [f.result() for f in futs] + [last]
     else:
-        fit0s = [start_values(j) for j in jobs]
+        fit0s = [start(*args[0])]
     handles = []
-    for j, fit0 in zip(jobs, fit0s):
+    for j, a, fit0 in zip(jobs, args, fit0s):
         n = len(j["y"])
         probes = RademacherProbes.shared(n, probe_seed)
-        gen = fit_co(j["trait"], j["y"], j["X"], probes, opt, fit0=fit0, lib=lib)
+        gen = fit_co(j["trait"], a[0], a[1], probes, opt, fit0=fit0, lib=lib)
         handles.append(sch.submit(gen, j.get("subset"), j.get("tag")))
     try:
         sch.run()
