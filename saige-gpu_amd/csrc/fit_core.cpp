@@ -171,20 +171,12 @@ double dot(const std::vector<double>& a, const std::vector<double>& b) {
 
 // ----------------------------------------------------------------- families
 
-// binomial(logit) or gaussian(identity).  linkinv and mu_eta run over n
+// binomial(logit) or gaussian(identity).  mu_eta and link_both run over n
 // samples with the exponentials taken as arrays (ex: n values of scratch):
 // logit mu = 1 / (1 + exp(-eta)), mu_eta = e / (1 + e)^2 with e = exp(-|eta|)
 struct Family {
     bool binomial;
     double variance(double mu) const { return binomial ? mu * (1.0 - mu) : 1.0; }
-    void linkinv(const double* eta, int64_t n, double* mu, double* ex) const {
-        if (!binomial) {
-            std::copy(eta, eta + n, mu);
-            return;
-        }
-        vmath::exp_neg(eta, ex, n);
-        for (int64_t i = 0; i < n; i++) mu[i] = 1.0 / (1.0 + ex[i]);
-    }
     void mu_eta(const double* eta, int64_t n, double* me, double* ex) const {
         if (!binomial) {
             std::fill(me, me + n, 1.0);
@@ -192,6 +184,22 @@ struct Family {
         }
         vmath::exp_neg_abs(eta, ex, n);
         for (int64_t i = 0; i < n; i++) me[i] = ex[i] / ((1.0 + ex[i]) * (1.0 + ex[i]));
+    }
+    // both from one exponential e = exp(-|eta|): logit mu = 1 / (1 + e) for
+    // eta >= 0 and e / (1 + e) below (the same value, rounded once more or
+    // less), mu_eta = e / (1 + e)^2
+    void link_both(const double* eta, int64_t n, double* mu, double* me, double* ex) const {
+        if (!binomial) {
+            std::copy(eta, eta + n, mu);
+            std::fill(me, me + n, 1.0);
+            return;
+        }
+        vmath::exp_neg_abs(eta, ex, n);
+        for (int64_t i = 0; i < n; i++) {
+            const double e = ex[i], d = 1.0 + e;
+            mu[i] = (eta[i] >= 0.0 ? 1.0 : e) / d;
+            me[i] = e / (d * d);
+        }
     }
 };
 
@@ -551,8 +559,7 @@ void glm_working(Trait trait, int64_t n, const double* y, const double* eta, con
                  double* Y, double* W) {
     const Family fam{trait == Trait::Binary};
     // W is the exponentials' scratch and Y holds mu_eta until it is read
-    fam.linkinv(eta, n, mu, W);
-    fam.mu_eta(eta, n, Y, W);
+    fam.link_both(eta, n, mu, Y, W);
     for (int64_t i = 0; i < n; i++) {
         const double m = mu[i], me = Y[i];
         Y[i] = eta[i] - (offset ? offset[i] : 0.0) + (y[i] - m) / me;
@@ -642,13 +649,14 @@ GlmStart glm_start(Trait trait, const std::vector<double>& y, const Matrix& X, c
         }
     }
     if (fam.binomial) vmath::log(eta.data(), eta.data(), n);
-    // the samples go in tiles of T, so each pass's temporaries stay in cache
+    // the samples go in tiles of T, so each pass's temporaries stay in cache;
+    // me_all holds mu_eta at the current eta, made with mu from one exponential
     constexpr int64_t T = 2048;
-    std::vector<double> tmp((size_t)(4 * T));
+    std::vector<double> tmp((size_t)(3 * T)), me_all((size_t)n);
     double* ex = tmp.data();
-    double* me = ex + T;
-    double* z = me + T;
+    double* z = ex + T;
     double* wt = z + T;
+    for (int64_t i0 = 0; i0 < n; i0 += T) fam.mu_eta(eta.data() + i0, std::min(T, n - i0), me_all.data() + i0, ex);
     auto deviance = [&](const double* m) {
         if (binary01) {
             // y in {0, 1}: the unit deviance is -2 log(mu) or -2 log(1 - mu);
@@ -691,7 +699,7 @@ GlmStart glm_start(Trait trait, const std::vector<double>& y, const Matrix& X, c
         Matrix A(p, p), b(p, 1);
         for (int64_t i0 = 0; i0 < n; i0 += T) {
             const int64_t len = std::min(T, n - i0);
-            fam.mu_eta(eta.data() + i0, len, me, ex);
+            const double* me = me_all.data() + i0;
             for (int64_t k = 0; k < len; k++) {
                 const int64_t i = i0 + k;
                 const double mi = mu[(size_t)i];
@@ -718,7 +726,7 @@ GlmStart glm_start(Trait trait, const std::vector<double>& y, const Matrix& X, c
                 for (int64_t j = 0; j < p; j++) e += xc[(size_t)j][i] * beta[(size_t)j];
                 eta[(size_t)i] = e;
             }
-            fam.linkinv(eta.data() + i0, len, mu.data() + i0, ex);
+            fam.link_both(eta.data() + i0, len, mu.data() + i0, me_all.data() + i0, ex);
         }
         const double dev = deviance(mu.data());
         if (std::fabs(dev - dev_old) / (std::fabs(dev) + 0.1) < epsilon) break;

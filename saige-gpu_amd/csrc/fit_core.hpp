@@ -20,10 +20,9 @@
 
 namespace saige {
 
-// y[i] = exp(-x[i]), exp(-|x[i]|), log(x[i]) over arrays by the vector math
-// library (vmath.cpp)
+// y[i] = exp(-|x[i]|), log(x[i]) over arrays by the vector math library
+// (vmath.cpp)
 namespace vmath {
-void exp_neg(const double* x, double* y, int64_t n);
 void exp_neg_abs(const double* x, double* y, int64_t n);
 void log(const double* x, double* y, int64_t n);
 }  // namespace vmath

@@ -16,9 +16,6 @@ namespace vmath {
 namespace {
 
 #define SAIGE_VLOOPS(SUFFIX, ATTR)                                                                         \
-    ATTR void exp_neg_##SUFFIX(const double* __restrict x, double* __restrict y, int64_t n) {              \
-        _Pragma("omp simd") for (int64_t i = 0; i < n; i++) y[i] = std::exp(-x[i]);                       \
-    }                                                                                                      \
     ATTR void exp_neg_abs_##SUFFIX(const double* __restrict x, double* __restrict y, int64_t n) {          \
         _Pragma("omp simd") for (int64_t i = 0; i < n; i++) y[i] = std::exp(-std::fabs(x[i]));            \
     }                                                                                                      \
@@ -37,8 +34,6 @@ bool use_avx2() {
 }
 
 }  // namespace
-
-void exp_neg(const double* x, double* y, int64_t n) { use_avx2() ? exp_neg_avx2(x, y, n) : exp_neg_base(x, y, n); }
 
 void exp_neg_abs(const double* x, double* y, int64_t n) {
     use_avx2() ? exp_neg_abs_avx2(x, y, n) : exp_neg_abs_base(x, y, n);
