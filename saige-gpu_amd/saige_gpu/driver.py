@@ -927,7 +927,7 @@ class BatchScheduler:
             f.sys_pending = (w, tau)
             f.sys_ready = True
             kind = "solve"
-        f.result_buf = np.zeros(B.shape, np.float64, order="F")
+        f.result_buf = np.empty(B.shape, np.float64, order="F")    # every column is taken before resuming
         f.remaining = B.shape[1]
         for j in range(B.shape[1]):
             f.pending.append((j, B[:, j], kind, ("host", j)))
@@ -973,7 +973,7 @@ class BatchScheduler:
             f.sys_pending = (np.ones(len(PY) if PY is not None else self._n_of(f)), (1.0, 0.0))
             f.sys_ready = True
         n_host = 1 if PY is not None else 0
-        f.result_buf = np.zeros((self._n_of(f), n_host), np.float64, order="F")
+        f.result_buf = np.empty((self._n_of(f), n_host), np.float64, order="F")
         f.remaining = n_host + k
         if PY is not None:
             f.pending.append((0, PY, "product", ("host", 0)))
@@ -1013,7 +1013,7 @@ class BatchScheduler:
         f.sys_pending = (tr["w"], tr["tau"])
         f.sys_ready = True
         extra = [] if tr["PY"] is None else [tr["APY"]] + ([tr["PY"]] if tr["want_t0"] else [])
-        f.result_buf = np.zeros((self._n_of(f), len(extra)), np.float64, order="F")
+        f.result_buf = np.empty((self._n_of(f), len(extra)), np.float64, order="F")
         f.remaining = tr["k"] + len(extra)
         for j in range(tr["k"]):
             f.pending.append((j, ("dev", "U", j), "solve", ("pair", j)))
@@ -1086,7 +1086,9 @@ class BatchScheduler:
         for kind, (slots, systems, cols, owners) in host.items():
             if not slots:
                 continue
-            B = np.zeros((self.N, len(cols)), np.float32, order="F")   # column-major, as the ABI takes it
+            # column-major, as the ABI takes it; rows outside a subset stay 0
+            alloc = np.zeros if any(f.subset is not None for f, _ in cols) else np.empty
+            B = alloc((self.N, len(cols)), np.float32, order="F")
             for c, (f, b) in enumerate(cols):
                 if f.subset is None:
                     B[:, c] = b
