@@ -25,6 +25,12 @@ class Session:
 
     def close(self):
         if self.ctx:
+            # device buffers a batch scheduler parked here for the next one
+            park = self.__dict__.pop("_trace_buffers", None)
+            if park is not None:
+                for bufs in park[1].values():
+                    for b in bufs.values():
+                        self.free_device(b)
             self._L.saige_destroy(self.ctx)
             self.ctx = None
 

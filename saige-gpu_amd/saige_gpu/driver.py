@@ -118,6 +118,35 @@ def glm_start_library(y, X, family, offset=None):
     return glm_start_native("binary" if family.name == "binomial" else "quantitative", y, X, offset)
 
 
+def _free_trace_buffers(sess, bufs):
+    for per_sys in bufs.values():
+        for b in per_sys.values():
+            sess.free_device(b)
+
+
+def _take_trace_buffers(sess, N):
+    """The trace rounds' device buffers (system -> buffers) parked on sess by
+    the last scheduler, if they were made for N samples; the session holds
+    none while a scheduler has them."""
+    park = getattr(sess, "_trace_buffers", None)
+    sess._trace_buffers = None
+    if park is None:
+        return {}
+    n, bufs = park
+    if n == N:
+        return bufs
+    _free_trace_buffers(sess, bufs)
+    return {}
+
+
+def release_trace_buffers(sess):
+    """Free the trace rounds' device buffers parked on sess."""
+    park = getattr(sess, "_trace_buffers", None)
+    sess._trace_buffers = None
+    if park is not None:
+        _free_trace_buffers(sess, park[1])
+
+
 _POOLS = {}
 
 
@@ -809,7 +838,9 @@ class BatchScheduler:
         self._pool = _shared_pool("step", 1)
         self._inflight = None                   # engine_step running in the worker
         self.n_overlapped = 0                   # iterations that ran during host work
-        self._devbufs = {}                      # system -> device buffers of its trace rounds
+        # system -> device buffers of its trace rounds, kept on the session
+        # between schedulers (allocating and freeing them costs each round ms)
+        self._devbufs = _take_trace_buffers(sess, self.N)
         self.device_trace = hasattr(sess, "engine_take_batch_ex")
 
     # ---------------------------------------------------------------- API
@@ -824,10 +855,13 @@ class BatchScheduler:
         return not self.waiting and not self.active
 
     def close(self):
-        """Free the device buffers of the trace rounds."""
-        for bufs in self._devbufs.values():
-            for b in bufs.values():
-                self.sess.free_device(b)
+        """Park the trace rounds' device buffers on the session for the next
+        scheduler (release_trace_buffers frees them), or free them when
+        another scheduler parked its own meanwhile."""
+        if getattr(self.sess, "_trace_buffers", None) is None:
+            self.sess._trace_buffers = (self.N, self._devbufs)
+        else:
+            _free_trace_buffers(self.sess, self._devbufs)
         self._devbufs = {}
 
     def run(self, arrivals=None):

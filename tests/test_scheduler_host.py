@@ -288,7 +288,13 @@ def test_device_trace_rounds_match(cohort):
     handles, sch = D.fit_batched(eng, jobs, opt, width=64, n_systems=3)
     assert sch.device_trace
     assert all(h.error is None for h in handles), [h.error for h in handles]
-    assert eng.calls["take_ex"] > 0 and eng.calls["alloc"] == eng.calls["free"] > 0
+    assert eng.calls["take_ex"] > 0 and eng.calls["alloc"] > 0 and eng.calls["free"] == 0
+    # the buffers stay on the engine for the next scheduler, and free once
+    n_alloc = eng.calls["alloc"]
+    D.fit_batched(eng, jobs[:1], opt, width=64, n_systems=3)
+    assert eng.calls["alloc"] == n_alloc and eng.calls["free"] == 0
+    D.release_trace_buffers(eng)
+    assert eng.calls["alloc"] == eng.calls["free"]
     for j, h in zip(jobs, handles):
         probes = D.RademacherProbes(len(j["y"]), 200)
         ref = OneColumnExecutor(FakeEngine(K, kdiag), opt, j.get("subset")).run(
