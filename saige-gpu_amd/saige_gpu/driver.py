@@ -110,10 +110,10 @@ def glm_start(y, X, family, offset=None, maxit=25, epsilon=1e-8):
 
 def glm_start_library(y, X, family, offset=None):
     """glm_start by the library's C++ IRLS (saige_glm_start, the same
-    restatement in FP64 with array exp/log): it holds no Python lock, so the
-    start values of several fits run at once on threads.  The drivers take
-    it with lib=True, as the GPU paths (fit_batched, fit_plain, the serving
-    loop) run them; the dense test executors keep glm_start."""
+    restatement in FP64 with array exp/log, about 4 ms at n = 100k; the call
+    holds no Python lock).  The drivers take it with lib=True, as the GPU
+    paths (fit_batched, fit_plain, the serving loop) run them; the dense test
+    executors keep glm_start."""
     from .api import glm_start_native
     return glm_start_native("binary" if family.name == "binomial" else "quantitative", y, X, offset)
 
@@ -1289,15 +1289,14 @@ def fit_batched(sess, jobs, opt=None, width=64, n_systems=16, probe_seed=200, li
     lib as in fit_binary_co."""
     opt = opt or Options()
     sch = BatchScheduler(sess, opt, width=width, n_systems=n_systems)
-    # the glm() start values (host IRLS, no GRM) of all fits at once, on
-    # threads: the library's IRLS holds no Python lock
+    # the glm() start values (host IRLS, no GRM) of all fits
     fams = {"binary": Binomial, "quantitative": Gaussian}
 
     start = glm_start_library if lib else glm_start
 
     # the inputs converted once here (one column-major copy per distinct
-    # covariate matrix in lib mode), so the threads below copy nothing under
-    # the Python lock and the drivers' own conversions are no-ops
+    # covariate matrix in lib mode), so the drivers' own conversions are
+    # no-ops
     conv = {}
 
     def covariates(X):
@@ -1306,13 +1305,10 @@ def fit_batched(sess, jobs, opt=None, width=64, n_systems=16, probe_seed=200, li
         return conv[id(X)][1]
 
     args = [(np.ascontiguousarray(j["y"], np.float64), covariates(j["X"]), fams[j["trait"]]) for j in jobs]
-    if len(jobs) > 1:
-        # the main thread takes the last fit's start values itself
-        futs = [_shared_pool("start", 8).submit(start, *a) for a in args[:-1]]
-        last = start(*args[-1])
-        fit0s = [f.result() for f in futs] + [last]
-    else:
-        fit0s = [start(*args[0])]
+    # one after another on this thread: on the GPU box's host, four library
+    # starts on worker threads reached the first GPU step 3 ms later
+    # (21.8 ms against 18.7 ms, medians of four rounds)
+    fit0s = [start(*a) for a in args]
     handles = []
     for j, a, fit0 in zip(jobs, args, fit0s):
         n = len(j["y"])
