@@ -96,10 +96,10 @@ def serve_leg(sess, N, Xraw, Zc, n_req, rate, seed, opt):
             trait = "quantitative"
         gen = D.fit_co(trait, y, Xs, D.RademacherProbes(len(sub), 200), opt, lib=True)
         arrivals.append((t, gen, sub, (trait, len(sub))))
-    sch = D.BatchScheduler(sess, opt, width=64, n_systems=16)
-    t0 = time.perf_counter()
-    done = sch.run(arrivals)
-    wall = time.perf_counter() - t0
+    with D.BatchScheduler(sess, opt, width=64, n_systems=16) as sch:
+        t0 = time.perf_counter()
+        done = sch.run(arrivals)
+        wall = time.perf_counter() - t0
     ok = [f for f in done if f.error is None]
     lat = np.array([f.t_done - f.t_submit for f in ok])
     return {"requests": n_req, "completed": len(ok), "errors": len(done) - len(ok),

@@ -455,18 +455,26 @@ def logistic(eta):
     return 1.0 / (1.0 + np.exp(-eta))
 
 
+_EPS = np.finfo(np.float64).eps
+
+
 class Binomial:
-    """R binomial(link='logit') family functions used by FG.R."""
+    """R binomial(link='logit') family functions used by FG.R.  R's
+    logit_linkinv takes exp(eta) as DBL_EPSILON below eta = -30 and as
+    1/DBL_EPSILON above 30; logit_mu_eta is DBL_EPSILON beyond |eta| > 30."""
     name = "binomial"
 
     @staticmethod
     def linkinv(eta):
-        return logistic(eta)
+        eta = np.asarray(eta, np.float64)
+        t = np.where(eta < -30.0, _EPS, np.where(eta > 30.0, 1.0 / _EPS, np.exp(np.clip(eta, -30.0, 30.0))))
+        return t / (1.0 + t)
 
     @staticmethod
     def mu_eta(eta):
-        e = np.exp(-np.abs(eta))
-        return e / (1.0 + e) ** 2
+        eta = np.asarray(eta, np.float64)
+        t = np.exp(np.clip(eta, -30.0, 30.0))
+        return np.where(np.abs(eta) > 30.0, _EPS, t / ((1.0 + t) * (1.0 + t)))
 
     @staticmethod
     def variance(mu):

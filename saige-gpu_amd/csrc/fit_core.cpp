@@ -2,6 +2,7 @@
 #include "fit_core.hpp"
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <map>
 #include <memory>
@@ -173,7 +174,13 @@ double dot(const std::vector<double>& a, const std::vector<double>& b) {
 
 // binomial(logit) or gaussian(identity).  mu_eta and link_both run over n
 // samples with the exponentials taken as arrays (ex: n values of scratch):
-// logit mu = 1 / (1 + exp(-eta)), mu_eta = e / (1 + e)^2 with e = exp(-|eta|)
+// logit mu = 1 / (1 + exp(-eta)), mu_eta = e / (1 + e)^2 with e = exp(-|eta|).
+// Beyond |eta| > 30 R's binomial() clamps (its logit_linkinv and
+// logit_mu_eta): exp(eta) is taken as 1/DBL_EPSILON or DBL_EPSILON, so mu
+// stays DBL_EPSILON/(1 + DBL_EPSILON) away from 0 and 1, and mu_eta is
+// DBL_EPSILON; quasi-separated phenotypes reach it.
+constexpr double kLogitThresh = 30.0;
+
 struct Family {
     bool binomial;
     double variance(double mu) const { return binomial ? mu * (1.0 - mu) : 1.0; }
@@ -183,7 +190,8 @@ struct Family {
             return;
         }
         vmath::exp_neg_abs(eta, ex, n);
-        for (int64_t i = 0; i < n; i++) me[i] = ex[i] / ((1.0 + ex[i]) * (1.0 + ex[i]));
+        for (int64_t i = 0; i < n; i++)
+            me[i] = std::fabs(eta[i]) > kLogitThresh ? DBL_EPSILON : ex[i] / ((1.0 + ex[i]) * (1.0 + ex[i]));
     }
     // both from one exponential e = exp(-|eta|): logit mu = 1 / (1 + e) for
     // eta >= 0 and e / (1 + e) below (the same value, rounded once more or
@@ -196,6 +204,11 @@ struct Family {
         }
         vmath::exp_neg_abs(eta, ex, n);
         for (int64_t i = 0; i < n; i++) {
+            if (std::fabs(eta[i]) > kLogitThresh) {
+                mu[i] = (eta[i] > 0.0 ? 1.0 : DBL_EPSILON) / (1.0 + DBL_EPSILON);
+                me[i] = DBL_EPSILON;
+                continue;
+            }
             const double e = ex[i], d = 1.0 + e;
             mu[i] = (eta[i] >= 0.0 ? 1.0 : e) / d;
             me[i] = e / (d * d);

@@ -10,6 +10,7 @@
 // through fitNULLGLMM (FG.R:1036-1298), one right-hand side at a time.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
@@ -424,6 +425,40 @@ void saige_fit_options_default(saige_fit_options* o) {
     o->probe_seed = d.probe_seed;
 }
 
+}  // extern "C"
+
+namespace {
+
+// What R's glm() and binomial() refuse before any arithmetic: non-finite
+// responses, covariates or offsets, and binomial responses outside [0, 1]
+// ("y values must be 0 <= y <= 1").  The IRLS loops assume finite values
+// (their array exp/log are built for finite arguments).
+bool valid_glm_inputs(int trait, int64_t n, const double* y, const double* X, int p, const double* offset,
+                      const char* who) {
+    for (int64_t i = 0; i < n; i++) {
+        if (!std::isfinite(y[i]) || (offset && !std::isfinite(offset[i]))) {
+            saige::set_error("%s: y or offset of sample %lld is not finite", who, (long long)i);
+            return false;
+        }
+        if (trait == SAIGE_TRAIT_BINARY && (y[i] < 0.0 || y[i] > 1.0)) {
+            saige::set_error("%s: binomial y of sample %lld is %g, outside [0, 1]", who, (long long)i, y[i]);
+            return false;
+        }
+    }
+    if (X)
+        for (int64_t i = 0; i < n * (int64_t)p; i++)
+            if (!std::isfinite(X[i])) {
+                saige::set_error("%s: covariate %lld of sample %lld is not finite", who, (long long)(i / n),
+                                 (long long)(i % n));
+                return false;
+            }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
 int saige_fit_null_batch(saige_ctx* ctx, int n_fits, const saige_fit_spec* specs, const saige_fit_options* o,
                          int max_concurrent, saige_fit_result* results) {
     if (!ctx || n_fits < 0 || (n_fits > 0 && (!specs || !results))) return SAIGE_E_ARG;
@@ -433,6 +468,7 @@ int saige_fit_null_batch(saige_ctx* ctx, int n_fits, const saige_fit_spec* specs
         if (s.n <= 0 || s.p <= 0 || !s.y || !s.X || (!s.subset && s.n != N) || s.n > N ||
             (s.trait != SAIGE_TRAIT_BINARY && s.trait != SAIGE_TRAIT_QUANTITATIVE) || !results[i].coefficients)
             return SAIGE_E_ARG;
+        if (!valid_glm_inputs(s.trait, s.n, s.y, s.X, s.p, s.offset, "saige_fit_null_batch")) return SAIGE_E_ARG;
         if (s.subset)
             for (int64_t k = 0; k < s.n; k++)
                 if (s.subset[k] < 0 || s.subset[k] >= N || (k > 0 && s.subset[k] <= s.subset[k - 1]))
@@ -534,6 +570,7 @@ int saige_glm_start(int trait, int64_t n, const double* y, const double* X, int 
     if (!y || !X || n <= 0 || p <= 0 || !coef || !eta || !mu ||
         (trait != SAIGE_TRAIT_BINARY && trait != SAIGE_TRAIT_QUANTITATIVE))
         return SAIGE_E_ARG;
+    if (!valid_glm_inputs(trait, n, y, X, p, offset, "saige_glm_start")) return SAIGE_E_ARG;
     try {
         saige::fit::Matrix Xm(n, p);
         std::memcpy(Xm.a.data(), X, sizeof(double) * (size_t)(n * p));
@@ -554,6 +591,12 @@ int saige_glm_working(int trait, int64_t n, const double* y, const double* eta, 
                       double* Y, double* W) {
     if (!y || !eta || n <= 0 || !mu || !Y || !W || (trait != SAIGE_TRAIT_BINARY && trait != SAIGE_TRAIT_QUANTITATIVE))
         return SAIGE_E_ARG;
+    if (!valid_glm_inputs(trait, n, y, nullptr, 0, offset, "saige_glm_working")) return SAIGE_E_ARG;
+    for (int64_t i = 0; i < n; i++)
+        if (!std::isfinite(eta[i])) {
+            saige::set_error("saige_glm_working: eta of sample %lld is not finite", (long long)i);
+            return SAIGE_E_ARG;
+        }
     saige::fit::glm_working(trait == SAIGE_TRAIT_BINARY ? saige::fit::Trait::Binary : saige::fit::Trait::Quantitative,
                             n, y, eta, offset, mu, Y, W);
     return SAIGE_OK;

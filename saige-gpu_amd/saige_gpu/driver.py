@@ -30,18 +30,27 @@ import numpy as np
 # ----------------------------------------------------------------- families
 
 
+_EPS = np.finfo(np.float64).eps
+
+
 class Binomial:
-    """R's binomial(link = "logit") as used by FG.R."""
+    """R's binomial(link = "logit") as used by FG.R, with its clamps beyond
+    |eta| > 30 (R's logit_linkinv / logit_mu_eta take exp(eta) as
+    1/DBL_EPSILON or DBL_EPSILON there; mu_eta is DBL_EPSILON)."""
     name = "binomial"
 
     @staticmethod
     def linkinv(eta):
-        return 1.0 / (1.0 + np.exp(-eta))
+        eta = np.asarray(eta, np.float64)
+        with np.errstate(over="ignore"):
+            mu = 1.0 / (1.0 + np.exp(-eta))
+        return np.where(eta > 30.0, 1.0 / (1.0 + _EPS), np.where(eta < -30.0, _EPS / (1.0 + _EPS), mu))
 
     @staticmethod
     def mu_eta(eta):
+        eta = np.asarray(eta, np.float64)
         e = np.exp(-np.abs(eta))
-        return e / (1.0 + e) ** 2
+        return np.where(np.abs(eta) > 30.0, _EPS, e / (1.0 + e) ** 2)
 
     @staticmethod
     def variance(mu):
@@ -839,8 +848,10 @@ class BatchScheduler:
         self._inflight = None                   # engine_step running in the worker
         self.n_overlapped = 0                   # iterations that ran during host work
         # system -> device buffers of its trace rounds, kept on the session
-        # between schedulers (allocating and freeing them costs each round ms)
-        self._devbufs = _take_trace_buffers(sess, self.N)
+        # between schedulers (allocating and freeing them costs each round ms);
+        # taken from the session at the first trace round, so a scheduler that
+        # runs none (variance ratios, LOCO) leaves them parked there
+        self._devbufs = None
         self.device_trace = hasattr(sess, "engine_take_batch_ex")
 
     # ---------------------------------------------------------------- API
@@ -857,12 +868,21 @@ class BatchScheduler:
     def close(self):
         """Park the trace rounds' device buffers on the session for the next
         scheduler (release_trace_buffers frees them), or free them when
-        another scheduler parked its own meanwhile."""
+        another scheduler parked its own meanwhile.  Idempotent."""
+        if self._devbufs is None:
+            return
         if getattr(self.sess, "_trace_buffers", None) is None:
             self.sess._trace_buffers = (self.N, self._devbufs)
         else:
             _free_trace_buffers(self.sess, self._devbufs)
-        self._devbufs = {}
+        self._devbufs = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
 
     def run(self, arrivals=None):
         """Run until every submitted fit finished.  arrivals: optional list of
@@ -927,7 +947,7 @@ class BatchScheduler:
                         continue
                 else:
                     req = f.gen.send(value)
-                if req[0] == "trace" and not self.device_trace:
+                if req[0] == "trace" and not self._device_round(req):
                     f.sub, value = trace_round_co(req), None
                     continue
                 return ("req", req)
@@ -980,7 +1000,17 @@ class BatchScheduler:
             self._apply(f, out)
 
     # ------------------------------------------------------- trace rounds
+    def _device_round(self, req):
+        """Whether a trace round fits the device-resident path: at most
+        TRACE_MAX_PROBES probes and TRACE_MAX_COVARIATES covariates; larger
+        rounds run as trace_round_co's products and solves."""
+        _, _, _, _, start, end, SX = req[:7]
+        return self.device_trace and end - start <= self.TRACE_MAX_PROBES \
+            and SX.shape[1] <= self.TRACE_MAX_COVARIATES
+
     def _buffers(self, sys):
+        if self._devbufs is None:
+            self._devbufs = _take_trace_buffers(self.sess, self.N)
         bufs = self._devbufs.get(sys)
         if bufs is None:
             n = self.N * 4
@@ -996,8 +1026,7 @@ class BatchScheduler:
         K [PY, U]; K u is kept on the device."""
         _, w, tau, probes, start, end, SX, want_t0, PY = req
         k, p = end - start, SX.shape[1]
-        if k > self.TRACE_MAX_PROBES or p > self.TRACE_MAX_COVARIATES:
-            raise ValueError(f"trace round of {k} probes x {p} covariates exceeds the device buffers")
+        assert k <= self.TRACE_MAX_PROBES and p <= self.TRACE_MAX_COVARIATES   # _device_round
         # device work (buffers, probes, uploads) waits for _fill: this may run
         # while an iteration is in flight, when nothing may touch the library
         f.trace = {"phase": 1, "w": w, "tau": tau, "k": k, "p": p, "want_t0": want_t0, "PY": PY,
@@ -1230,9 +1259,10 @@ def variance_ratio(sess, fit, y, X, trait, categories, opt=None, vr_pool=True, n
     out of the GRM (Get_OneSNP_Geno_forVarRatio) rather than GRM markers."""
     opt = opt or Options()
     get = sess.Get_OneSNP_Geno_forVarRatio if vr_pool else sess.Get_OneSNP_Geno
-    sch = BatchScheduler(sess, opt, width=width, n_systems=1)
-    h = sch.submit(variance_ratio_co(fit, y, X, trait, get, categories, opt, num_markers, ratio_cv_cutoff))
-    sch.run()
+    with BatchScheduler(sess, opt, width=width, n_systems=1) as sch:
+        h = sch.submit(variance_ratio_co(fit, y, X, trait, get, categories, opt, num_markers,
+                                         ratio_cv_cutoff))
+        sch.run()
     if h.error is not None:
         raise h.error
     return h.result
@@ -1261,12 +1291,12 @@ def loco(sess, jobs, starts, ends, opt=None, batched=True, width=64, n_systems=1
         timing["set_range"] += time.perf_counter() - t
         t = time.perf_counter()
         if batched:
-            sch = BatchScheduler(sess, opt, width=width, n_systems=n_systems)
             sess.engine_use_loco(True)
             try:
-                hs = [sch.submit(loco_co(jb["fit"], jb["y"], jb["X"], jb["trait"], opt), jb.get("subset"))
-                      for jb in jobs]
-                sch.run()
+                with BatchScheduler(sess, opt, width=width, n_systems=n_systems) as sch:
+                    hs = [sch.submit(loco_co(jb["fit"], jb["y"], jb["X"], jb["trait"], opt),
+                                     jb.get("subset")) for jb in jobs]
+                    sch.run()
             finally:
                 sess.engine_use_loco(False)
             for r, h in zip(out, hs):

@@ -243,3 +243,50 @@ def test_library_get_coef_helpers():
     assert np.allclose(cov, cr, rtol=1e-9, atol=1e-13)
     assert np.allclose(alpha, ar, rtol=1e-9, atol=1e-13)
     assert np.allclose(e, Y - 0.7 * (siY - siX @ ar) / w, rtol=1e-9, atol=1e-12)
+
+
+def test_logit_clamps_follow_r_and_bad_inputs_are_refused():
+    """R's binomial() beyond |eta| > 30 (logit_linkinv / logit_mu_eta take
+    exp(eta) as DBL_EPSILON or 1/DBL_EPSILON, mu_eta = DBL_EPSILON): the C++
+    family, the Python driver and the oracle give these values, so the
+    working response of a quasi-separated phenotype stays finite; inputs R's
+    glm() refuses (binomial y outside [0, 1], non-finite values) are refused
+    with SAIGE_E_ARG."""
+    from saige_gpu.api import glm_start_native, glm_working_native
+    from saige_gpu._lib import SaigeError
+    eps = np.finfo(np.float64).eps
+    eta = np.array([-800.0, -31.0, -30.5, -30.0, -1.0, 0.0, 1.0, 30.0, 30.5, 31.0, 800.0])
+    y = np.array([0, 0, 1, 0, 1, 0, 1, 1, 0, 1, 1], np.float64)
+    big = np.abs(eta) > 30
+    # R's values beyond the threshold, typed from its definition
+    mu_r = np.where(eta > 30, (1 / eps) / (1 + 1 / eps), eps / (1 + eps))
+    for F in (D.Binomial, O.Binomial):
+        assert np.array_equal(F.mu_eta(eta)[big], np.full(big.sum(), eps))
+        assert np.array_equal(F.linkinv(eta)[big], mu_r[big])
+        # inside, the textbook logistic (to rounding)
+        assert np.allclose(F.linkinv(eta[~big]), 1 / (1 + np.exp(-eta[~big])), rtol=1e-15)
+    mu, Y, W = glm_working_native("binary", y, eta)
+    assert np.array_equal(mu[big], mu_r[big])
+    assert np.all(np.isfinite(Y)) and np.all(np.isfinite(W)) and np.all(W > 0)
+    assert np.allclose(Y, eta + (y - mu) / D.Binomial.mu_eta(eta), rtol=1e-12)
+    # a quasi-separated covariate: the start values stay finite
+    rng = np.random.default_rng(3)
+    n = 400
+    x = rng.standard_normal(n)
+    ys = (x > 0).astype(float)
+    ys[:3] = 1 - ys[:3]
+    X = np.column_stack([np.ones(n), x])
+    g = glm_start_native("binary", ys, X)
+    assert np.all(np.isfinite(g["coef"])) and np.all(np.isfinite(g["eta"]))
+    assert np.all((g["mu"] > 0) & (g["mu"] < 1))
+    ref = D.glm_start(ys, X, D.Binomial, np.zeros(n))
+    assert np.allclose(g["coef"], ref["coef"], rtol=1e-9)
+    for bad_y in (np.where(np.arange(n) == 5, 2.0, ys), np.where(np.arange(n) == 7, np.nan, ys)):
+        with pytest.raises(SaigeError):
+            glm_start_native("binary", bad_y, X)
+    Xbad = X.copy()
+    Xbad[9, 1] = np.inf
+    with pytest.raises(SaigeError):
+        glm_start_native("quantitative", ys, Xbad)
+    with pytest.raises(SaigeError):
+        glm_working_native("binary", y, np.where(eta == 0, np.nan, eta))

@@ -176,16 +176,33 @@ def test_tau_updates_closed_form():
                                1000, 1e-12, tol=0.02, cv_cutoff=1.0)
     step = (re["YPAPY"] - re["Trace"]) / re["AI"]
     cand = 0.2 + step
-    if cand >= 0.02:
-        assert np.isclose(new[1], cand)
-    assert new[0] == 1.0
+    # this weakly heritable working trait gives a large negative step: the
+    # reference zeroes tau1 (FG.cpp:3318-3322) before its halving loop, so
+    # the answer is exactly 0, not a halved positive value
+    assert cand < 0.0
+    assert new[1] == 0.0 and new[0] == 1.0
+    # the accepted branch: the same score at a larger AI gives tau1 + dtau
+    big = dict(re, AI=re["AI"] * 1e3)
+    import oracle.oracle as OO
+    saved = OO.get_ai_score
+    OO.get_ai_score = lambda *a, **k: big
+    try:
+        new2, _ = O.fitglmmai_rpcg(grm, Y, X, w, tau, rc["Sigma_iY"], rc["Sigma_iX"], rc["cov"], 30,
+                                   probes, 1000, 1e-12, tol=0.02, cv_cutoff=1.0)
+    finally:
+        OO.get_ai_score = saved
+    assert 0.2 + step / 1e3 >= 0.02
+    assert new2[1] == 0.2 + step / 1e3
     # 2x2 update solves AI dtau = score
     newq, req = O.fitglmmai_rpcg_q(grm, Y, X, np.ones(N), tau, rc["Sigma_iY"], rc["Sigma_iX"], rc["cov"],
                                    30, probes, 1000, 1e-12, tol=0.02, cv_cutoff=1.0)
     score = np.array([req["YPA0PY"] - req["Trace"][0], req["YPAPY"] - req["Trace"][1]])
     d = np.linalg.solve(req["AI"], score)
-    if np.all(tau + d >= 0.02):
-        assert np.allclose(newq, tau + d)
+    # the full step leaves both components negative; halving stops at the
+    # first step 2^-k with both >= 0, here k = 5 (FG.cpp:3645-3653)
+    assert np.all(tau + d < 0)
+    assert np.any(tau + d / 16 < 0) and np.all(tau + d / 32 >= 0.02)
+    assert np.allclose(newq, tau + d / 32, rtol=1e-12)
     assert np.allclose(req["AI"], req["AI"].T)
 
 

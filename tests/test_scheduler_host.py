@@ -335,3 +335,56 @@ def test_scheduler_packs_low_slots_and_caps_systems(cohort):
     assert seen[0] == list(range(len(seen[0])))
     # with at most two fits of (1 + p) or nrun + 1 columns, no slot above 31 is ever used
     assert max(max(s) for s in seen) < 32
+
+
+def test_trace_buffers_survive_schedulers_without_trace_rounds(cohort):
+    """A scheduler that runs no trace round (variance ratios, LOCO, a
+    caller's own solves) leaves the parked buffers on the session, closed or
+    not, so release_trace_buffers frees every buffer fit_batched made."""
+    K, kdiag, jobs = cohort
+    opt = D.Options(nrun=4, traceCVcutoff=1.0, maxiter=3, tolPCG=1e-6, maxiterPCG=200)
+    eng = FakeDeviceEngine(K, kdiag)
+    handles, _ = D.fit_batched(eng, jobs[:2], opt, width=64, n_systems=3)
+    assert all(h.error is None for h in handles) and eng.calls["alloc"] > 0
+    fit = handles[0].result
+
+    def solves():
+        w = fit["fitted_values"] * (1 - fit["fitted_values"])
+        x = yield ("solve", w, fit["theta"], np.ones((len(w), 2)))
+        return x
+
+    sch = D.BatchScheduler(eng, opt, width=64, n_systems=1)    # never closed
+    h = sch.submit(solves())
+    sch.run()
+    assert h.error is None
+    with D.BatchScheduler(eng, opt, width=64, n_systems=1) as sch2:
+        h2 = sch2.submit(solves())
+        sch2.run()
+    assert np.array_equal(h.result, h2.result)
+    D.release_trace_buffers(eng)
+    assert eng.calls["alloc"] == eng.calls["free"]
+
+
+def test_trace_round_beyond_device_buffers_runs_on_host_path(cohort):
+    """A fit with more covariates than the device trace buffers hold (16 > 14)
+    runs beside a small fit: its trace rounds take the products-and-solves
+    path, the small fit keeps the device path, and both match one column at
+    a time."""
+    K, kdiag, jobs = cohort
+    N = K.shape[0]
+    rng = np.random.default_rng(12)
+    X16, _, _ = plink.covariate_transform(np.column_stack([np.ones(N), rng.standard_normal((N, 15))]))
+    assert X16.shape[1] == 16 > D.BatchScheduler.TRACE_MAX_COVARIATES
+    big = {"trait": "binary", "y": jobs[0]["y"], "X": X16, "tag": "p16"}
+    opt = D.Options(nrun=6, traceCVcutoff=1.0, maxiter=4, tolPCG=1e-8, maxiterPCG=300)
+    eng = FakeDeviceEngine(K, kdiag)
+    handles, sch = D.fit_batched(eng, [jobs[0], big], opt, width=64, n_systems=3)
+    assert all(h.error is None for h in handles), [h.error for h in handles]
+    assert eng.calls["take_ex"] > 0
+    for j, h in zip([jobs[0], big], handles):
+        probes = D.RademacherProbes(N, 200)
+        ref = OneColumnExecutor(FakeEngine(K, kdiag), opt).run(
+            D.fit_co(j["trait"], j["y"], j["X"], probes, opt, lib=True))
+        assert h.result["iterations"] == ref["iterations"], j["tag"]
+        assert np.allclose(h.result["theta"], ref["theta"], rtol=1e-6, atol=1e-9), j["tag"]
+    D.release_trace_buffers(eng)
