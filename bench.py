@@ -70,43 +70,227 @@ def make_workload(sess, N, M, maf_lo, maf_hi, miss, seed, prevalences, ncausal=1
     return dev, X, phenos
 
 
+def _phenotype(trait, Xraw, g, param, rng):
+    """A phenotype on one request's samples: binary with logit-scale genetic
+    part g and prevalence param (intercept by bisection on up to 4096 of the
+    samples), or quantitative with h2 = param (g scaled to variance h2)."""
+    import numpy as np
+    lin = Xraw[:, 1:] @ np.array([0.02, 0.3]) + g
+    lin = lin - lin.mean()
+    if trait == "binary":
+        ref = lin[rng.choice(len(lin), min(4096, len(lin)), replace=False)]
+        lo, hi = -30.0, 30.0
+        for _ in range(60):
+            a = 0.5 * (lo + hi)
+            lo, hi = (a, hi) if np.mean(1.0 / (1.0 + np.exp(-(a + ref)))) < param else (lo, a)
+        return (rng.random(len(lin)) < 1.0 / (1.0 + np.exp(-(a + lin)))).astype(np.float64)
+    gs = g * np.sqrt(param) / max(g.std(), 1e-12)
+    return Xraw[:, 1:] @ np.array([0.02, 0.3]) + gs + rng.normal(0.0, np.sqrt(1.0 - param), len(g))
+
+
 def serve_leg(sess, N, Xraw, Zc, n_req, rate, seed, opt):
-    """Continuous-batching serving on the resident GRM: a Poisson stream of
-    null-fit requests, each on a random 20-100% sample subset, 70% binary
-    (prevalence U(0.01, 0.3)) and 30% quantitative (h2 U(0.1, 0.6))."""
+    """Continuous-batching serving through the library's request server
+    (saige_submit / saige_poll): a Poisson stream of null-fit requests on
+    the resident GRM, each on a random 20-100% sample subset, 70% binary
+    (prevalence U(0.01, 0.3)) and 30% quantitative (h2 U(0.1, 0.6)).  The
+    genetic parts come from a pool of 16 polygenic vectors over all samples;
+    a request's phenotype is drawn at its submission.  Latency = the
+    server's own clock from submission to completion."""
+    import numpy as np
+    from saige_gpu import plink
+
+    rng = np.random.default_rng(seed)
+    pool = [Zc @ rng.normal(0.0, 1.0, Zc.shape[1]) for _ in range(16)]
+    pool = [(g - g.mean()) / g.std() for g in pool]
+    t_arr = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    results, tickets = {}, []
+    with sess.server(opt, max_concurrent=16, hbm_reserve_bytes=2 << 30) as srv:
+        t0 = time.perf_counter()
+        for r in range(n_req):
+            sub = np.sort(rng.choice(N, int(rng.uniform(0.2, 1.0) * N), replace=False))
+            Xs, _, _ = plink.covariate_transform(Xraw[sub])
+            g = pool[r % 16][sub]
+            if rng.random() < 0.7:
+                job = {"trait": "binary", "X": Xs, "subset": sub,
+                       "y": _phenotype("binary", Xraw[sub], np.sqrt(0.3) * g, rng.uniform(0.01, 0.3), rng)}
+            else:
+                job = {"trait": "quantitative", "X": Xs, "subset": sub,
+                       "y": _phenotype("quantitative", Xraw[sub], g, rng.uniform(0.1, 0.6), rng)}
+            while time.perf_counter() - t0 < t_arr[r]:
+                for tk in tickets:
+                    if tk not in results:
+                        out = srv.poll(tk, 0)
+                        if out is not None:
+                            results[tk] = out
+                time.sleep(0.001)
+            tickets.append(srv.submit(job))
+            if (r + 1) % 200 == 0:
+                log(f"serving: {r + 1} of {n_req} requests submitted, {len(results)} done")
+        for tk in tickets:
+            if tk not in results:
+                results[tk] = srv.poll(tk, -1)
+        wall = time.perf_counter() - t0
+    res = [results[tk] for tk in tickets]
+    ok = [r for r in res if r["status"] == 0]
+    lat = np.array([r["t_done"] for r in ok])
+    return {"path": "saige_submit / saige_poll (C++ drivers and scheduler)", "requests": n_req,
+            "completed": len(ok), "errors": len(res) - len(ok), "offered_rate_per_s": rate, "wall_s": wall,
+            "fits_per_hour": len(ok) / wall * 3600.0, "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)), "max_latency_s": float(lat.max()),
+            "mean_queue_s": float(np.mean([r["t_admit"] for r in ok])), "max_concurrent_fits": 16}
+
+
+class DenseFP64:
+    """Exact FP64 answers for the fit coroutines (driver.fit_co) on a dense
+    GRM: Sigma^-1 B by a Cholesky solve of Sigma = tau0 diag(1/w) + tau1 K,
+    K B by a dense product; the trace rounds as their products and solves."""
+
+    def __init__(self, K):
+        self.K = K
+
+    def _solve(self, w, tau, B):
+        import numpy as np
+        from scipy.linalg import cho_factor, cho_solve
+        Sig = tau[1] * self.K + np.diag(tau[0] / np.asarray(w, np.float64))
+        return cho_solve(cho_factor(Sig), np.asarray(B, np.float64))
+
+    def run(self, gen):
+        from saige_gpu import driver as D
+        value = None
+        while True:
+            try:
+                req = gen.send(value)
+            except StopIteration as e:
+                return e.value
+            if req[0] == "solve":
+                value = self._solve(*req[1:])
+            elif req[0] == "matvec":
+                value = self.K @ req[1]
+            else:
+                value = D.trace_round_host(req, lambda B: self.K @ B, self._solve)
+
+
+def subblock_genotypes(sess, dev, N, M, n0, chunk=8192):
+    """A1 counts and missing mask of samples 0..n0-1 from the device .bed."""
+    import numpy as np
+    from saige_gpu import synth
+    bpm = (N + 3) // 4
+    G = np.empty((n0, M), np.int8)
+    miss = np.empty((n0, M), bool)
+    for m0 in range(0, M, chunk):
+        k = min(chunk, M - m0)
+        raw = sess.copy_device_to_host(dev, k * bpm, offset=m0 * bpm).reshape(k, bpm)[:, : n0 // 4]
+        G[:, m0:m0 + k], miss[:, m0:m0 + k] = synth.decode_bed_rows(raw, n0)
+    return G, miss
+
+
+def fp64_check_leg(sess, G, miss, Xraw, opt, seed):
+    """Error of the device path against FP64 on the principal subblock of
+    the first n0 samples: the GRM (FP64 from the samples' genotypes over all
+    GRM markers, imputed as FG.cpp:447-454, frequencies of the whole cohort),
+    a 64-column block solve on the subblock (engine with the sample mask)
+    against a dense FP64 solve, and tau of a binary and a quantitative null
+    fit on the subblock's samples (the batched device path on the resident
+    GRM) against the same fits with exact FP64 solves on the FP64 GRM."""
     import numpy as np
     from saige_gpu import driver as D
     from saige_gpu import plink, synth
 
+    n0 = G.shape[0]
+    N = sess.getNnomissingOut()
+    st, mask = sess.marker_stats(), sess.getQCdMarkerIndex().astype(bool)
+    cols = np.nonzero(mask)[0]
+    S = np.zeros((n0, n0))
+    for c0 in range(0, len(cols), 16384):
+        c = cols[c0:c0 + 16384]
+        f = st["freq"][c].astype(np.float64)
+        g = np.where(miss[:, c], st["fill"][c][None, :], G[:, c]).astype(np.float64)
+        Z = (g - 2.0 * f) / np.sqrt(2.0 * f * (1.0 - f))
+        S += Z @ Z.T
+    K64 = S / len(cols)
+    K = sess.grm_block(0, 0, n0, n0).astype(np.float64)
+    grm_err = float(np.max(np.abs(K - K64)) / np.max(np.abs(K64)))
+    # a 64-column block solve on the subblock
     rng = np.random.default_rng(seed)
-    arrivals = []
-    t = 0.0
-    for r in range(n_req):
-        t += rng.exponential(1.0 / rate)
-        sub = np.sort(rng.choice(N, int(rng.uniform(0.2, 1.0) * N), replace=False))
-        Xs, _, _ = plink.covariate_transform(Xraw[sub])
-        if rng.random() < 0.7:
-            g = synth.polygenic(Zc[sub], 0.3, seed + r)
-            y = synth.binary_phenotype(Xraw[sub], g, rng.uniform(0.01, 0.3), seed + r)
-            trait = "binary"
-        else:
-            h2 = rng.uniform(0.1, 0.6)
-            g = synth.polygenic(Zc[sub], h2, seed + r)
-            y = synth.quantitative_phenotype(Xraw[sub], g, h2, seed + r)
-            trait = "quantitative"
-        gen = D.fit_co(trait, y, Xs, D.RademacherProbes(len(sub), 200), opt, lib=True)
-        arrivals.append((t, gen, sub, (trait, len(sub))))
-    with D.BatchScheduler(sess, opt, width=64, n_systems=16) as sch:
-        t0 = time.perf_counter()
-        done = sch.run(arrivals)
-        wall = time.perf_counter() - t0
-    ok = [f for f in done if f.error is None]
-    lat = np.array([f.t_done - f.t_submit for f in ok])
-    return {"requests": n_req, "completed": len(ok), "errors": len(done) - len(ok),
-            "offered_rate_per_s": rate, "wall_s": wall, "fits_per_hour": len(ok) / wall * 3600.0,
-            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
-            "max_concurrent_fits": sch.n_systems,
-            "slot_occupancy": sch.busy_slot_iters / max(1, sch.n_iterations * sch.width)}
+    w = np.ones(N)
+    w[:n0] = rng.uniform(0.1, 0.25, n0)
+    tau = np.array([1.0, 0.3])
+    B = np.zeros((N, 64))
+    B[:n0] = rng.standard_normal((n0, 64))
+    msk = np.zeros(N, np.uint8)
+    msk[:n0] = 1
+    sess.engine_reserve(64, 1)
+    sess.engine_set_system(0, w, tau, msk)
+    sess.engine_add_batch(list(range(64)), [0] * 64, B, opt.maxiterPCG, opt.tolPCG)
+    while not np.all(sess.engine_step(1, 64)[0][:64]):
+        pass
+    X = sess.engine_take_batch(list(range(64)))[:n0].astype(np.float64)
+    exact = DenseFP64(K64)._solve(w[:n0], tau, B[:n0])
+    sol_err = np.max(np.abs(X - exact), axis=0) / np.max(np.abs(exact), axis=0)
+    # tau of two null fits on the subblock's samples
+    sub = np.arange(n0)
+    Xs, _, _ = plink.covariate_transform(Xraw[sub])
+    causal = np.sort(rng.choice(G.shape[1], 500, replace=False))
+    Zs = synth.standardized(G[:, causal], miss[:, causal])
+    jobs = [{"trait": "binary", "X": Xs, "subset": sub,
+             "y": synth.binary_phenotype(Xraw[sub], synth.polygenic(Zs, 2.0, seed), 0.2, seed)},
+            {"trait": "quantitative", "X": Xs, "subset": sub,
+             "y": synth.quantitative_phenotype(Xraw[sub], synth.polygenic(Zs, 0.6, seed + 1), 0.6, seed + 1)}]
+    hs, _ = D.fit_batched(sess, jobs, opt)
+    fits = {}
+    for h, j in zip(hs, jobs):
+        if h.error is not None:
+            raise h.error
+        ref = DenseFP64(K64).run(D.fit_co(j["trait"], j["y"], Xs, D.RademacherProbes(n0, 200), opt))
+        fits[j["trait"]] = {"theta": list(map(float, h.result["theta"])),
+                            "theta_fp64": list(map(float, ref["theta"])),
+                            "rel_error": [float(abs(a - b) / abs(b)) if b != 0 else float(abs(a - b))
+                                          for a, b in zip(h.result["theta"], ref["theta"])],
+                            "iterations": [int(h.result["iterations"]), int(ref["iterations"])]}
+    return {"samples": n0, "grm_max_rel_error": grm_err,
+            "block_solve_rel_error": {"max": float(sol_err.max()), "median": float(np.median(sol_err))},
+            "tau": fits,
+            "reference": "FP64 GRM of the subblock from its genotypes over all GRM markers; exact (Cholesky) "
+                         "FP64 solves; the same driver and probes"}
+
+
+class CpuReference:
+    """The reference's CPU path, one right-hand side at a time:
+    getPCG1ofSigmaAndVector in FP32 (FG.cpp:2593-2809) over
+    parallelCrossProdOpenMP (FG.cpp:1576-1598: every marker decoded on every
+    product) -- the oracle's restatement -- driving the fit coroutines."""
+
+    def __init__(self, O, g, diag, threads, opt):
+        self.O, self.g, self.diag, self.threads, self.opt = O, g, diag, threads, opt
+
+    def _solve(self, w, tau, B):
+        import numpy as np
+        B = np.asarray(B, np.float64).reshape(len(w), -1)
+        w32, t32 = np.asarray(w, np.float32), np.asarray(tau, np.float32)
+        return np.column_stack([self.O.pcg_f32(self.g, self.diag, w32, t32, B[:, c].astype(np.float32),
+                                               self.opt.maxiterPCG, self.opt.tolPCG, nthreads=self.threads)[0]
+                                for c in range(B.shape[1])]).astype(np.float64)
+
+    def _matvec(self, B):
+        import numpy as np
+        B = np.asarray(B, np.float64).reshape(self.g["Nnm"], -1)
+        return np.column_stack([self.O.crossprod(self.g, B[:, c], nthreads=self.threads)
+                                for c in range(B.shape[1])]).astype(np.float64)
+
+    def run(self, gen):
+        from saige_gpu import driver as D
+        value = None
+        while True:
+            try:
+                req = gen.send(value)
+            except StopIteration as e:
+                return e.value
+            if req[0] == "solve":
+                value = self._solve(*req[1:])
+            elif req[0] == "matvec":
+                value = self._matvec(req[1])
+            else:
+                value = D.trace_round_host(req, self._matvec, self._solve)
 
 
 def cpu_baseline(sess, dev, N, M, n_matvec_fit, threads, sample_markers, min_seconds=10.0):
@@ -130,6 +314,24 @@ def cpu_baseline(sess, dev, N, M, n_matvec_fit, threads, sample_markers, min_sec
             break
     per_marker = (time.perf_counter() - t0) / (reps * og["Mqc"])
     return per_marker, og["Mqc"], reps
+
+
+def cpu_c0_fit(threads, opt):
+    """C0 (N = 2,000, M = 10,000, binary, h2 0.3 on 1,000 causal markers,
+    prevalence 0.2, intercept + age + sex) fitted end to end on the CPU
+    path of CpuReference.  Returns (seconds, fit)."""
+    from oracle import oracle as O
+    from saige_gpu import driver as D
+    from saige_gpu import plink, synth
+    c = synth.small_cohort(2000, 10000, maf_lo=0.05, maf_hi=0.5, miss_rate=0.005, seed=synth.config_seed(0),
+                           ncausal=1000)
+    og = O.setgeno(c["bed"], 2000, 10000, min_maf=0.01, max_missing=0.15)
+    X, _, _ = plink.covariate_transform(c["X"])
+    y = synth.binary_phenotype(c["X"], synth.polygenic(c["Zc"], 0.3, c["seed"]), 0.2, c["seed"])
+    ex = CpuReference(O, og, O.diag_std(og), threads, opt)
+    t0 = time.perf_counter()
+    fit = ex.run(D.fit_co("binary", y, X, D.RademacherProbes(2000, 200), opt))
+    return time.perf_counter() - t0, fit
 
 
 def c4_leg(args):
@@ -217,12 +419,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--N", type=int, default=100_000)
     ap.add_argument("--M", type=int, default=150_000)
-    ap.add_argument("--nrun", type=int, default=15)
+    ap.add_argument("--nrun", type=int, default=15,
+                    help="probes per trace round: 15 makes the C2 block of 60 probes + 4 phenotypes")
     ap.add_argument("--grm", default="bf16", choices=["bf16", "fp32"])
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-nrun30", action="store_true")
+    ap.add_argument("--no-fp64-check", action="store_true")
     ap.add_argument("--cpu-sample-markers", type=int, default=4000)
-    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--config", default="c2", choices=["c2", "c4"])
     ap.add_argument("--gemm-mode", default="bf16x3_tri", choices=["bf16x3_tri", "bf16x3", "fp32"],
                     help="block-GEMM arithmetic: FP32 operands split into three BF16 pieces on the BF16 "
@@ -260,8 +465,13 @@ def main():
     sess.setmaxMissingRateforGRM(0.15)
     t = time.perf_counter()
     sess.setgeno_device(dev, N, M)
-    log(f"setgeno (decode/QC/repack/diag) {time.perf_counter() - t:.2f}s, Mqc={sess.getMsub_MAFge_minMAFtoConstructGRM()}")
+    setgeno_s = time.perf_counter() - t
     Mqc = sess.getMsub_MAFge_minMAFtoConstructGRM()
+    log(f"setgeno (decode/QC/repack/diag) {setgeno_s:.3f}s, Mqc={Mqc}")
+    n0 = 2048
+    G0 = miss0 = None
+    if not args.no_fp64_check:
+        G0, miss0 = subblock_genotypes(sess, dev, N, M, n0)
     grm_s = sess.build_grm(args.grm)
     grm_tflops = N * (N + 1) * Mqc / grm_s / 1e12
     log(f"GRM {args.grm}: {grm_s:.3f}s device time, {grm_tflops:.1f} TFLOP/s")
@@ -275,16 +485,22 @@ def main():
     opt = D.Options(nrun=args.nrun)
     jobs = [{"trait": "binary", "y": y, "X": X, "tag": f"prev{p}"} for y, p in zip(phenos, prevalences)]
 
-    def step():
-        handles, sch = D.fit_batched(sess, jobs, opt, width=64, n_systems=len(jobs))
+    def step(o=opt):
+        handles, sch = D.fit_batched(sess, jobs, o, width=64, n_systems=len(jobs))
         for h in handles:
             if h.error is not None:
                 raise h.error
         return handles, sch
 
+    def native_step(o=opt):
+        res = sess.fit_null_batch(jobs, o)
+        if any(r["status"] != 0 for r in res):
+            raise RuntimeError(f"native driver failed: {[r['status'] for r in res]}")
+        return res
+
     for _ in range(args.warmup):
         step()
-    g0, n0 = sess.engine_stats("gemm_full")
+    g0, n0_ = sess.engine_stats("gemm_full")
     h0, m0 = sess.engine_stats("gemm_half")
     t0 = time.perf_counter()
     results = []
@@ -299,16 +515,37 @@ def main():
     fits = len(jobs) * args.steps
     fits_per_hour = fits / (t1 - t0) * 3600.0
     handles, sch = results[-1]
-    gemm_launches = n1 - n0
+    gemm_launches = n1 - n0_
     gemm_avg = (g1 - g0) / max(1, gemm_launches)
     gemm_flop = 2.0 * N * N * 64
     gemm_tflops = gemm_flop / gemm_avg / 1e12
     lat = sorted(h.t_done - h.t_submit for h in handles)
     log("batched step split (s): " + ", ".join(f"{k} {v:.3f}" for k, v in sorted(sch.timing.items())))
-    log(f"batched: {step_s:.2f}s/step, {fits_per_hour:.1f} fits/h, {gemm_launches} 64-column GEMMs avg "
+    log(f"batched: {step_s:.3f}s/step, {fits_per_hour:.1f} fits/h, {gemm_launches} 64-column GEMMs avg "
         f"{gemm_avg * 1e3:.2f} ms ({gemm_tflops:.1f} TFLOP/s), {half_launches} 32-column GEMMs avg "
-        f"{half_avg * 1e3:.2f} ms ({4.0 * N * N / half_avg / 1e9:.0f} GB/s of S), "
-        f"theta {[list(np.round(h.result['theta'], 4)) for h in handles]}")
+        f"{half_avg * 1e3:.2f} ms, theta {[list(np.round(h.result['theta'], 4)) for h in handles]}")
+
+    # the same rounds through the library's C++ drivers and scheduler
+    # (saige_fit_null_batch), the path an R or C++ caller reaches: timed
+    # over the same number of steps
+    native_step()
+    st0 = [sess.engine_stats(k) for k in ("gemm_full", "gemm_half")]
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        nat = native_step()
+    nat_s = (time.perf_counter() - t0) / args.steps
+    st1 = [sess.engine_stats(k) for k in ("gemm_full", "gemm_half")]
+    native = {"fits_per_hour": len(jobs) / nat_s * 3600.0, "seconds_per_step": nat_s, "steps": args.steps,
+              "max_rel_theta_diff_vs_python_driver": max(
+                  float(np.max(np.abs(r["theta"] - h.result["theta"]) / np.maximum(np.abs(h.result["theta"]), 1e-12)))
+                  for r, h in zip(nat, handles)),
+              "gemm_full_launches_per_step": (st1[0][1] - st0[0][1]) / args.steps,
+              "gemm_half_launches_per_step": (st1[1][1] - st0[1][1]) / args.steps,
+              "gemm_device_seconds_per_step": ((st1[0][0] - st0[0][0]) + (st1[1][0] - st0[1][0])) / args.steps}
+    log(f"C++ drivers (saige_fit_null_batch): {nat_s:.3f}s/step, {native['fits_per_hour']:.0f} fits/h, "
+        f"{native['gemm_full_launches_per_step']:.0f} full + {native['gemm_half_launches_per_step']:.0f} half GEMMs "
+        f"({native['gemm_device_seconds_per_step']:.3f}s) per step, max rel theta diff vs the Python scheduler "
+        f"{native['max_rel_theta_diff_vs_python_driver']:.1e}")
 
     # the same round with the block GEMM on the FP32 MFMA, for comparison:
     # tuned in that mode, one step timed, results checked against the step above
@@ -316,7 +553,7 @@ def main():
     if args.gemm_mode != "fp32":
         sess.set_gemm_mode("fp32")
         t32 = sess.autotune(reps=3)
-        g0, n0 = sess.engine_stats("gemm_full")
+        g0, n0_ = sess.engine_stats("gemm_full")
         h0, m0 = sess.engine_stats("gemm_half")
         t0 = time.perf_counter()
         h32, _ = step()
@@ -328,7 +565,7 @@ def main():
                      for a, b in zip(handles, h32))
         fp32_mode = {"fits_per_hour": len(jobs) / s32 * 3600.0, "seconds_per_step": s32,
                      "gemm_tuned": {k: t32[k] for k in ("rt", "pd", "nsplit", "ms")},
-                     "gemm_full_avg_ms": (g1 - g0) / max(1, n1 - n0) * 1e3,
+                     "gemm_full_avg_ms": (g1 - g0) / max(1, n1 - n0_) * 1e3,
                      "gemm_half_avg_ms": (h1 - h0) / max(1, m1 - m0) * 1e3,
                      "max_rel_theta_diff_vs_split_mode": dtheta}
         log(f"FP32-MFMA mode: {s32:.2f}s/step, {fp32_mode['fits_per_hour']:.0f} fits/h, GEMM full "
@@ -337,34 +574,58 @@ def main():
         sess.set_gemm_mode(args.gemm_mode)
         sess.autotune(reps=3)                      # full and one-half shapes of this mode again
 
-    # the same round through the library's C++ drivers and scheduler
-    # (saige_fit_null_batch): no Python in the loop
-    st0 = [sess.engine_stats(k) for k in ("gemm_full", "gemm_half")]
-    t0 = time.perf_counter()
-    nat = sess.fit_null_batch(jobs, opt)
-    nat_s = time.perf_counter() - t0
-    st1 = [sess.engine_stats(k) for k in ("gemm_full", "gemm_half")]
-    if any(r["status"] != 0 for r in nat):
-        raise RuntimeError(f"native driver failed: {[r['status'] for r in nat]}")
-    native = {"fits_per_hour": len(jobs) / nat_s * 3600.0, "seconds_per_step": nat_s,
-              "max_rel_theta_diff_vs_python_driver": max(
-                  float(np.max(np.abs(r["theta"] - h.result["theta"]) / np.maximum(np.abs(h.result["theta"]), 1e-12)))
-                  for r, h in zip(nat, handles)),
-              "gemm_full_launches": st1[0][1] - st0[0][1], "gemm_half_launches": st1[1][1] - st0[1][1],
-              "gemm_device_seconds": (st1[0][0] - st0[0][0]) + (st1[1][0] - st0[1][0])}
-    log(f"C++ drivers (saige_fit_null_batch): {nat_s:.2f}s/step, {native['fits_per_hour']:.0f} fits/h, "
-        f"{native['gemm_full_launches']} full + {native['gemm_half_launches']} half GEMMs "
-        f"({native['gemm_device_seconds']:.3f}s), max rel theta diff vs the Python scheduler "
-        f"{native['max_rel_theta_diff_vs_python_driver']:.1e}")
+    plain = None
+    if not args.no_plain:
+        g2, n2 = sess.engine_stats("gemv")
+        tp = time.perf_counter()
+        pr = D.fit_plain(sess, "binary", phenos[2], X, opt)
+        plain_s = time.perf_counter() - tp
+        g3, n3 = sess.engine_stats("gemv")
+        gemv_avg = (g3 - g2) / max(1, n3 - n2)
+        # GB/s of S the product needs: its lower triangle (2 N^2 bytes) in the
+        # default mode, all of it (4 N^2) in the "full" one
+        gemv_bytes = (2.0 if sess.gemv_mode() == "tri" else 4.0) * N * N
+        plain = {"seconds": plain_s, "fits_per_hour": 3600.0 / plain_s, "pcg_iters": pr["pcg_iters"],
+                 "grm_products": pr["pcg_iters"] + pr["n_products"],
+                 "n_solves": pr["n_solves"], "gemv_ms": gemv_avg * 1e3, "gemv_mode": sess.gemv_mode(),
+                 "gemv_GBps": gemv_bytes / gemv_avg / 1e9, "theta": list(map(float, pr["theta"]))}
+        log(f"plain: {plain_s:.2f}s per fit, {plain['fits_per_hour']:.1f} fits/h, GEMV {gemv_avg * 1e3:.2f} ms "
+            f"({plain['gemv_GBps']:.0f} GB/s), theta {pr['theta']}")
+
+    # the reference's default of 30 probes per trace round (step1_fitNULLGLMM.R:67)
+    nrun30 = None
+    if not args.no_nrun30:
+        o30 = D.Options(nrun=30)
+        t0 = time.perf_counter()
+        step(o30)
+        b30 = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        native_step(o30)
+        n30 = time.perf_counter() - t0
+        tp = time.perf_counter()
+        D.fit_plain(sess, "binary", phenos[2], X, o30)
+        p30 = time.perf_counter() - tp
+        nrun30 = {"batched_fits_per_hour": len(jobs) / b30 * 3600.0, "native_fits_per_hour": len(jobs) / n30 * 3600.0,
+                  "plain_fits_per_hour": 3600.0 / p30, "batched_vs_plain": p30 * len(jobs) / b30,
+                  "native_vs_plain": p30 * len(jobs) / n30}
+        log(f"nrun 30: batched {nrun30['batched_fits_per_hour']:.0f}, C++ {nrun30['native_fits_per_hour']:.0f}, "
+            f"plain {nrun30['plain_fits_per_hour']:.0f} fits/h")
 
     serving = None
     if args.serve_requests > 0:
-        capacity = len(jobs) / step_s
-        serving = serve_leg(sess, N, kept["Xraw"], kept["Zc"], args.serve_requests, 1.2 * capacity,
-                            seed + 7, opt)
+        capacity = len(jobs) / nat_s
+        serving = serve_leg(sess, N, kept["Xraw"], kept["Zc"], args.serve_requests, 1.2 * capacity, seed + 7, opt)
         log(f"serving: {serving['completed']}/{serving['requests']} fits at offered {serving['offered_rate_per_s']:.2f}/s: "
             f"{serving['fits_per_hour']:.0f} fits/h, p50 {serving['p50_latency_s']:.2f}s, "
-            f"p99 {serving['p99_latency_s']:.2f}s, slot occupancy {serving['slot_occupancy']:.2f}")
+            f"p99 {serving['p99_latency_s']:.2f}s")
+
+    fp64 = None
+    if G0 is not None:
+        fp64 = fp64_check_leg(sess, G0, miss0, kept["Xraw"], D.Options(nrun=30), seed + 11)
+        log(f"FP64 subblock check ({n0} samples): GRM {fp64['grm_max_rel_error']:.2e}, block solve "
+            f"{fp64['block_solve_rel_error']['max']:.2e}, tau rel error " +
+            ", ".join(f"{k} {v['rel_error']}" for k, v in fp64["tau"].items()))
+        del G0, miss0
 
     loco = None
     if args.loco_chromosomes > 0:
@@ -393,70 +654,68 @@ def main():
         log(f"LOCO: {nc} chromosomes x {len(ljobs)} fits in {loco_s:.1f}s "
             f"({tm['set_range'] / nc:.3f}s range build + {tm['coef'] / nc:.3f}s Get_Coef_LOCO per chromosome)")
 
-    plain = None
-    if not args.no_plain:
-        g2, n2 = sess.engine_stats("gemv")
-        tp = time.perf_counter()
-        pr = D.fit_plain(sess, "binary", phenos[2], X, opt)
-        plain_s = time.perf_counter() - tp
-        g3, n3 = sess.engine_stats("gemv")
-        gemv_avg = (g3 - g2) / max(1, n3 - n2)
-        # GB/s of S the product needs: its lower triangle (2 N^2 bytes) in the
-        # default mode, all of it (4 N^2) in the "full" one
-        gemv_bytes = (2.0 if sess.gemv_mode() == "tri" else 4.0) * N * N
-        plain = {"seconds": plain_s, "fits_per_hour": 3600.0 / plain_s, "pcg_iters": pr["pcg_iters"],
-                 "grm_products": pr["pcg_iters"] + pr["n_products"],
-                 "n_solves": pr["n_solves"], "gemv_ms": gemv_avg * 1e3, "gemv_mode": sess.gemv_mode(),
-                 "gemv_GBps": gemv_bytes / gemv_avg / 1e9, "theta": list(map(float, pr["theta"]))}
-        log(f"plain: {plain_s:.1f}s per fit, {plain['fits_per_hour']:.1f} fits/h, GEMV {gemv_avg * 1e3:.2f} ms "
-            f"({plain['gemv_GBps']:.0f} GB/s), theta {pr['theta']}")
-
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        # the box's CPU share: the cpuset this process may run on
+        threads = len(os.sched_getaffinity(0))
         n_matvec = plain["grm_products"] if plain else 3000
-        per_marker, ms, reps = cpu_baseline(sess, dev, N, M, n_matvec, threads, args.cpu_sample_markers)
-        t_mv = per_marker * Mqc
-        cpu_fph = 3600.0 / (n_matvec * t_mv)
-        cpu = {"value": cpu_fph, "unit": "fits/hour", "cores": threads, "kind": "port",
+        legs = {}
+        for nt in (threads, 1):
+            per_marker, ms, reps = cpu_baseline(sess, dev, N, M, n_matvec, nt, args.cpu_sample_markers,
+                                                min_seconds=8.0)
+            t_mv = per_marker * Mqc
+            legs[nt] = (t_mv, 3600.0 / (n_matvec * t_mv), ms, reps)
+            log(f"cpu baseline: {t_mv:.2f}s per K.v with {nt} threads -> {legs[nt][1]:.4f} fits/h (extrapolated)")
+        c0 = {}
+        for nt in (threads, 1):
+            secs, fit = cpu_c0_fit(nt, D.Options(nrun=30))
+            c0[str(nt)] = {"threads": nt, "seconds": secs, "fits_per_hour": 3600.0 / secs,
+                           "theta": list(map(float, fit["theta"])), "iterations": int(fit["iterations"])}
+            log(f"cpu C0 end to end ({nt} threads): {secs:.1f}s, theta {fit['theta']}")
+        t_mv, fph, ms, reps = legs[threads]
+        cpu = {"value": fph, "unit": "fits/hour", "cores": threads, "kind": "port",
                "sample": (f"oracle parallelCrossProdOpenMP restatement (per-marker 2-bit decode + FP32 "
                           f"dot/axpy) over {ms} of the {Mqc} QC'd markers at N={N}, {reps} reps, "
-                          f"{threads} threads: {t_mv:.2f}s per K.v, x {n_matvec} GRM products of the plain "
-                          f"GPU fit (extrapolated)")}
-        log(f"cpu baseline: {t_mv:.2f}s per K.v with {threads} threads -> {cpu_fph:.4f} fits/h (extrapolated)")
+                          f"{threads} threads (the process's CPU set; os.cpu_count() {os.cpu_count()}): "
+                          f"{t_mv:.2f}s per K.v, x {n_matvec} GRM products of the plain GPU fit (extrapolated)"),
+               "one_thread": {"value": legs[1][1], "seconds_per_matvec": legs[1][0],
+                              "note": "OMP_NUM_THREADS=1 as the reference's Dockerfile ships it"},
+               "c0_end_to_end": c0}
 
     sess.free_device(dev)
     traffic = PROFILED_GEMM_TRAFFIC.get((args.gemm_mode, N, tune["rt"], tune["pd"], tune["nsplit"]))
     common = {"traffic": traffic,
               "traffic_source": PROFILE_FILES[args.gemm_mode] + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
-              "algorithmic": f"2*N^2*64 = {gemm_flop:.3e} flop per launch; 4*N^2 = {4.0 * N * N:.3e} B of S",
-              "avg_launch_ms": gemm_avg * 1e3, "launches": gemm_launches,
-              "half_width_launches": half_launches, "half_width_avg_ms": half_avg * 1e3}
+              "avg_launch_ms": gemm_avg * 1e3, "launches": gemm_launches}
+    # the algorithmic unit: 2 N^2 k flop of an FP32 product with k = 64
+    # right-hand sides, against the FP32 matrix peak; the split modes reach
+    # it through BF16 pieces, whose MFMA count is reported beside it
+    half = {"avg_ms": half_avg * 1e3, "launches": half_launches,
+            "achieved": 2.0 * N * N * 32 / half_avg / 1e12, "unit": "TFLOP/s", "peak": FP32_MFMA_PEAK,
+            "frac": 2.0 * N * N * 32 / half_avg / 1e12 / FP32_MFMA_PEAK,
+            "algorithmic": f"2*N^2*32 = {2.0 * N * N * 32:.3e} flop per launch"}
+    roofline = {"bound": "mfma", "achieved": gemm_tflops, "peak": FP32_MFMA_PEAK, "unit": "TFLOP/s",
+                "frac": gemm_tflops / FP32_MFMA_PEAK,
+                "algorithmic": f"2*N^2*64 = {gemm_flop:.3e} FP32 flop per launch (64 right-hand sides)",
+                "half_width": half, **common}
     if args.gemm_mode == "fp32":
-        roofline = {"bound": "mfma", "kernel": "k_gemm_part + k_gemm_epi (block-PCG GRM x 64 RHS, FP32 MFMA)",
-                    "achieved": gemm_tflops, "peak": FP32_MFMA_PEAK, "unit": "TFLOP/s",
-                    "frac": gemm_tflops / FP32_MFMA_PEAK, **common}
+        roofline["kernel"] = "k_gemm_part + k_gemm_epi (block-PCG GRM x 64 RHS, FP32 MFMA)"
+        roofline["bytes"] = f"4*N^2 = {4.0 * N * N:.3e} B of S per launch"
     elif args.gemm_mode == "bf16x3_tri":
-        # the lower triangle (2 N^2 bytes) carries all six piece products of
-        # both uses: 384 flop per byte, over the BF16 ridge (312), so the bound
-        # is the BF16 MFMA
-        bf16 = 6.0 * gemm_flop / gemm_avg / 1e12
-        common["algorithmic"] = (f"6 x 2*N^2*64 = {6 * gemm_flop:.3e} BF16 flop per launch; "
-                                 f"2*N^2 = {2.0 * N * N:.3e} B of S's lower triangle")
-        roofline = {"bound": "mfma", "kernel": "k_split_p + k_gemm_x3t + k_tri_reduce + k_gemm_epi (block-PCG GRM "
-                                               "x 64 RHS from the lower triangle, FP32 as three BF16 pieces)",
-                    "achieved": bf16, "peak": BF16_MFMA_PEAK, "unit": "TFLOP/s", "frac": bf16 / BF16_MFMA_PEAK,
-                    "hbm_GBs_of_triangle": 2.0 * N * N / gemm_avg / 1e9, "fp32_equiv_tflops": gemm_tflops,
-                    **common}
+        roofline["kernel"] = ("k_split_p + k_gemm_x3t + k_tri_reduce + k_gemm_epi (block-PCG GRM x 64 RHS from "
+                              "the lower triangle, FP32 operands as three BF16 pieces)")
+        roofline["bytes"] = f"2*N^2 = {2.0 * N * N:.3e} B of S's lower triangle per launch"
+        roofline["hbm_GBs_of_triangle"] = 2.0 * N * N / gemm_avg / 1e9
+        roofline["bf16_piece_products"] = {"tflops": 6.0 * gemm_flop / gemm_avg / 1e12, "peak": BF16_MFMA_PEAK,
+                                           "frac": 6.0 * gemm_flop / gemm_avg / 1e12 / BF16_MFMA_PEAK,
+                                           "note": "six BF16 piece products per FP32 product (the emulation's cost)"}
     else:
-        # six BF16 piece products per FP32 product: 192 flop per byte of S, under
-        # the BF16 ridge (2500 TFLOP/s / 8 TB/s = 312), so the bound is HBM
-        gbs = 4.0 * N * N / gemm_avg / 1e9
-        roofline = {"bound": "hbm", "kernel": "k_split_p + k_gemm_x3w (rt 1) or k_gemm_x3 (rt 2) + k_gemm_epi (block-PCG GRM x 64 RHS, "
-                                              "FP32 as three BF16 pieces on the BF16 MFMA)",
-                    "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-                    "bf16_mfma_tflops": 6.0 * gemm_flop / gemm_avg / 1e12, "fp32_equiv_tflops": gemm_tflops,
-                    **common}
+        roofline["kernel"] = ("k_split_p + k_gemm_x3w (rt 1) or k_gemm_x3 (rt 2) + k_gemm_epi (block-PCG GRM x 64 "
+                              "RHS, FP32 operands as three BF16 pieces on the BF16 MFMA)")
+        roofline["bytes"] = f"4*N^2 = {4.0 * N * N:.3e} B of S per launch"
+        roofline["hbm_GBs_of_S"] = 4.0 * N * N / gemm_avg / 1e9
+        roofline["bf16_piece_products"] = {"tflops": 6.0 * gemm_flop / gemm_avg / 1e12, "peak": BF16_MFMA_PEAK,
+                                           "frac": 6.0 * gemm_flop / gemm_avg / 1e12 / BF16_MFMA_PEAK}
     out = {
         "metric": "null-GLMM fits/hour at N=100k batched vs plain; GRM TFLOP/s; p99 fit latency",
         "value": fits_per_hour,
@@ -476,14 +735,18 @@ def main():
                    "gemm_mode": args.gemm_mode},
         "grm_tflops": grm_tflops,
         "grm_seconds": grm_s,
+        "setgeno_seconds": setgeno_s,
         "gemm_autotune": tune,
-        "fp32_mode": fp32_mode,
         "native_cpp_driver": native,
+        "fp32_mode": fp32_mode,
         "plain": plain,
         "batched_vs_plain": (fits_per_hour / plain["fits_per_hour"]) if plain else None,
+        "native_vs_plain": (native["fits_per_hour"] / plain["fits_per_hour"]) if plain else None,
+        "nrun30": nrun30,
         "p50_fit_latency_s": serving["p50_latency_s"] if serving else float(np.percentile(lat, 50)),
         "p99_fit_latency_s": serving["p99_latency_s"] if serving else float(np.percentile(lat, 99)),
         "serving": serving,
+        "fp64_subblock_check": fp64,
         "loco": loco,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -37,6 +37,7 @@ extern "C" {
 #define SAIGE_E_IO (-4)       /* file could not be opened or read        */
 #define SAIGE_E_DEVICE (-5)   /* HIP runtime error                       */
 #define SAIGE_E_NOTCONV (-6)  /* PCG hit maxiter (result still written)  */
+#define SAIGE_PENDING 1       /* saige_poll: the request has not finished */
 
 /* GRM arithmetic for saige_build_grm */
 #define SAIGE_GRM_FP32 0        /* FP32 Z panels, FP32 MFMA (v_mfma_f32_32x32x2_f32) */
@@ -424,6 +425,37 @@ void saige_fit_options_default(saige_fit_options* opt);
  * SAIGE_E_STATE when its first variance component went to 0, FG.R:449-451). */
 int saige_fit_null_batch(saige_ctx* ctx, int n_fits, const saige_fit_spec* specs, const saige_fit_options* opt,
                          int max_concurrent, saige_fit_result* results);
+/* ------------------------------------------------- serving null fits
+ * Replaces the reference's per-phenotype task farming, one step-1 R
+ * process per phenotype (mvp_gwphewas_study/data_analysis/01_gwas_gpu/
+ * scripts_for_run/submit_step1_with_nvme_gpu_v2.r:118-120), with one
+ * process that keeps the resident GRM busy with a stream of requests
+ * (continuous batching).  saige_submit copies a request and returns at once;
+ * the server's scheduler thread admits requests while an engine system
+ * (max_concurrent of them) and HBM (hbm_reserve_bytes free; 0 = no check)
+ * are free; an admitted request's solves and GRM products take engine
+ * slots as they free up and join the iterations already under way, so a
+ * request joins a mix that is mid-solve; saige_poll hands a finished
+ * request's result over.  A fit's arithmetic does not depend on the other
+ * columns, so its result is bit-identical alone and in any mix.  While a
+ * server runs, its context does no other engine work. */
+typedef struct saige_server saige_server;
+/* Start a server on the resident GRM with the fit options (NULL: the
+ * defaults). */
+int saige_server_start(saige_ctx* ctx, const saige_fit_options* opt, int max_concurrent, int64_t hbm_reserve_bytes,
+                       saige_server** out);
+/* Queue one null fit (the spec's arrays are copied); *ticket names it. */
+int saige_submit(saige_server* srv, const saige_fit_spec* spec, int64_t* ticket);
+/* The result of request `ticket` once it has finished: SAIGE_OK with
+ * *result filled as saige_fit_null_batch fills it (result->status says
+ * whether the fit succeeded; the ticket is then released), or SAIGE_PENDING
+ * when it has not finished within wait_ms milliseconds (0: do not wait;
+ * negative: wait until it has).  times (2, or NULL): seconds from
+ * submission to admission and to completion. */
+int saige_poll(saige_server* srv, int64_t ticket, int wait_ms, saige_fit_result* result, double* times);
+/* Finish every queued request, stop the scheduler thread and free the
+ * server (results not yet polled are dropped). */
+int saige_server_stop(saige_server* srv);
 /* Covariate_Transform (FG.R:1612-1647): X (n x p) -> sqrt(n) Q of its QR
  * decomposition (X_out n x p_kept, R_out p_kept x p_kept or NULL), after
  * dropping linearly dependent columns (kept: their indices, or NULL). */

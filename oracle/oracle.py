@@ -182,6 +182,14 @@ def grm_f64(g):
     return S
 
 
+def grm_f64_blas(g):
+    """grm_f64 by BLAS: the same FP32 standardised values (std_matrix)
+    promoted to FP64, S = Z Z^T summed by the BLAS in its own order (equal to
+    grm_f64 to FP64 rounding; seconds instead of minutes at N = 2000)."""
+    Z = std_matrix(g).astype(np.float64)
+    return Z @ Z.T
+
+
 def marker_subset(g, idx):
     """The genotype dict restricted to QC'd markers idx (in that order)."""
     idx = np.asarray(idx, np.int64)
@@ -233,9 +241,10 @@ def sigma_mul_f32(b, kb, w, tau):
     return out
 
 
-def pcg_f32(g, diag, w, tau, b, maxiter=500, tol=1e-5, Kdense=None, diag_one=False):
+def pcg_f32(g, diag, w, tau, b, maxiter=500, tol=1e-5, Kdense=None, diag_one=False, nthreads=1):
     """getPCG1ofSigmaAndVector (FG.cpp:2593-2809), FP32 throughout.  With
-    Kdense (FP32, already / Mqc) the GRM product is the dense one."""
+    Kdense (FP32, already / Mqc) the GRM product is the dense one; else the
+    per-marker product of parallelCrossProdOpenMP on nthreads threads."""
     N = len(b)
     x = np.zeros(N, np.float32)
     Kd = None if Kdense is None else np.ascontiguousarray(Kdense, np.float32)
@@ -245,7 +254,7 @@ def pcg_f32(g, diag, w, tau, b, maxiter=500, tol=1e-5, Kdense=None, diag_one=Fal
                        ctypes.c_int64(N), _p(np.ascontiguousarray(w, np.float32)),
                        ctypes.c_float(tau[0]), ctypes.c_float(tau[1]),
                        _p(np.ascontiguousarray(b, np.float32)), ctypes.c_int(maxiter),
-                       ctypes.c_float(tol), _p(x))
+                       ctypes.c_float(tol), _p(x), ctypes.c_int(nthreads))
     return x, it
 
 

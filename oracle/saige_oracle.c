@@ -351,12 +351,14 @@ static float sdot(const float* a, const float* b, int64_t n)
  * getPCG1ofSigmaAndVector (FG.cpp:2593-2809), Jacobi preconditioner,
  * absolute ||r||^2 stop, all FP32.  The GRM product is either the
  * per-marker form (packed != NULL) or a dense FP32 K (Kdense != NULL,
- * already divided by Mqc).  Returns the iteration count.
+ * already divided by Mqc).  The per-marker product runs on nthreads
+ * OpenMP threads, as parallelCrossProdOpenMP does (FG.cpp:1576-1598).
+ * Returns the iteration count.
  */
 int orc_pcg(const uint8_t* packed, int64_t Mqc, const float* freq, const float* invStd,
             const float* Kdense, const float* diagStd, int diag_one,
             int64_t N, const float* w, float tau0, float tau1, const float* bvec,
-            int maxiter, float tol, float* x)
+            int maxiter, float tol, float* x, int nthreads)
 {
     float* r = (float*)malloc(sizeof(float) * N);
     float* z = (float*)malloc(sizeof(float) * N);
@@ -381,7 +383,7 @@ int orc_pcg(const uint8_t* packed, int64_t Mqc, const float* freq, const float* 
         iter++;
         if (tau1 != 0.0f) {
             if (Kdense) dense_matvec(Kdense, N, p, kp);
-            else orc_crossprod(packed, Mqc, N, freq, invStd, p, kp, 1);
+            else orc_crossprod(packed, Mqc, N, freq, invStd, p, kp, nthreads > 0 ? nthreads : 1);
         }
         orc_sigma_mul(p, kp, w, N, tau0, tau1, Ap);
         float a = sdot(r, z, N) / sdot(p, Ap, N);

@@ -354,51 +354,21 @@ AIScore ai_score(Solver& s, const std::vector<double>& Y, const Matrix& X, const
     for (;;) {
         const int64_t k = end - start;
         const bool first = APY.empty();
-        Matrix U(n, k);
-        for (int64_t j = 0; j < k; j++) {
-            const std::vector<double>& pv = cached_probe(opt.probe_seed, start + j, n);
-            std::copy(pv.begin(), pv.end(), U.col(j));
-        }
-        // K [PY, U] (K U after the first round), then Sigma^-1 [U, K PY, (PY)]
-        Matrix KB, sol;
-        std::vector<int> src;
+        TraceTerms tt;
+        s.trace_round(w, tau, Sigma_iX, first ? &PY : nullptr, quantitative, start, end, opt.probe_seed, tt);
         if (first) {
-            Matrix B(n, k + 1);
-            std::copy(PY.begin(), PY.end(), B.col(0));
-            std::copy(U.a.begin(), U.a.end(), B.col(1));
-            for (int64_t j = 0; j < k; j++) src.push_back((int)(1 + j));
-            src.push_back(-1);                      // K PY
-            if (quantitative) src.push_back(0);     // PY
-            s.product_then_solve(B, KB, w, tau, src, sol);
-        } else {
-            for (int64_t j = 0; j < k; j++) src.push_back((int)j);
-            s.product_then_solve(U, KB, w, tau, src, sol);
+            APY = std::move(tt.APY);
+            SAPY = std::move(tt.SAPY);
+            if (quantitative) SPY = std::move(tt.SPY);
         }
-        const int64_t u0 = first ? 1 : 0;   // first column of K U in KB
-        if (first) APY.assign(KB.col(0), KB.col(0) + n);
-        if (first) {
-            SAPY.assign(sol.col(k), sol.col(k) + n);
-            if (quantitative) SPY.assign(sol.col(k + 1), sol.col(k + 1) + n);
-        }
+        std::vector<double> h((size_t)p), g((size_t)p);
         for (int64_t j = 0; j < k; j++) {
-            // d = (K u)^T Sigma^-1 u, e = u^T Sigma^-1 u, h = (Sigma^-1 X)^T K u,
-            // g = (Sigma^-1 X)^T u, in one pass over the samples
-            const double* au = KB.col(u0 + j);
-            const double* su = sol.col(j);
-            const double* u = U.col(j);
-            double d = 0.0, e = 0.0;
-            std::vector<double> h((size_t)p, 0.0), g((size_t)p, 0.0);
-            for (int64_t r = 0; r < n; r++) {
-                d += au[r] * su[r];
-                e += u[r] * su[r];
-                for (int64_t c = 0; c < p; c++) {
-                    const double sx = Sigma_iX.a[(size_t)(c * n + r)];
-                    h[(size_t)c] += au[r] * sx;
-                    g[(size_t)c] += u[r] * sx;
-                }
+            for (int64_t c = 0; c < p; c++) {
+                h[(size_t)c] = tt.h(j, c);
+                g[(size_t)c] = tt.g(j, c);
             }
-            t1.push_back(d - quad(h, cov, g));
-            if (quantitative) t0.push_back(e - quad(g, cov, g));
+            t1.push_back(tt.d[(size_t)j] - quad(h, cov, g));
+            if (quantitative) t0.push_back(tt.e[(size_t)j] - quad(g, cov, g));
         }
         const double cv = cal_cv(t1);
         const double cv0 = quantitative ? cal_cv(t0) : 0.0;
@@ -409,6 +379,7 @@ AIScore ai_score(Solver& s, const std::vector<double>& Y, const Matrix& X, const
         }
         break;
     }
+    (void)n;
     AIScore r;
     r.nprobe = (int)t1.size();
     r.YPAPY = dot(PY, APY);
@@ -496,6 +467,68 @@ void Solver::product_then_solve(const Matrix& B, Matrix& KB, const std::vector<d
                                 const std::vector<int>& src, Matrix& X) {
     product(B, KB);
     solve(w, tau, gather_columns(B, KB, src), X);
+}
+
+// The host trace round: K [PY, U] (K U after the first round), then
+// Sigma^-1 [U, K PY, (PY)], and the dots in one pass over the samples.
+void Solver::trace_round(const std::vector<double>& w, const double tau[2], const Matrix& Sigma_iX,
+                         const std::vector<double>* PY, bool quantitative, int64_t start, int64_t end, uint64_t seed,
+                         TraceTerms& out) {
+    const int64_t n = Sigma_iX.rows, p = Sigma_iX.cols, k = end - start;
+    const bool first = PY != nullptr;
+    Matrix U(n, k);
+    for (int64_t j = 0; j < k; j++) {
+        const std::vector<double>& pv = cached_probe(seed, start + j, n);
+        std::copy(pv.begin(), pv.end(), U.col(j));
+    }
+    Matrix KB, sol;
+    std::vector<int> src;
+    if (first) {
+        Matrix B(n, k + 1);
+        std::copy(PY->begin(), PY->end(), B.col(0));
+        std::copy(U.a.begin(), U.a.end(), B.col(1));
+        for (int64_t j = 0; j < k; j++) src.push_back((int)(1 + j));
+        src.push_back(-1);                      // K PY
+        if (quantitative) src.push_back(0);     // PY
+        product_then_solve(B, KB, w, tau, src, sol);
+    } else {
+        for (int64_t j = 0; j < k; j++) src.push_back((int)j);
+        product_then_solve(U, KB, w, tau, src, sol);
+    }
+    const int64_t u0 = first ? 1 : 0;   // first column of K U in KB
+    if (first) {
+        out.APY.assign(KB.col(0), KB.col(0) + n);
+        out.SAPY.assign(sol.col(k), sol.col(k) + n);
+        if (quantitative) out.SPY.assign(sol.col(k + 1), sol.col(k + 1) + n);
+    }
+    out.d.assign((size_t)k, 0.0);
+    out.e.assign((size_t)k, 0.0);
+    out.h = Matrix(k, p);
+    out.g = Matrix(k, p);
+    std::vector<double> h((size_t)p), g((size_t)p);
+    for (int64_t j = 0; j < k; j++) {
+        const double* au = KB.col(u0 + j);
+        const double* su = sol.col(j);
+        const double* u = U.col(j);
+        double d = 0.0, e = 0.0;
+        std::fill(h.begin(), h.end(), 0.0);
+        std::fill(g.begin(), g.end(), 0.0);
+        for (int64_t r = 0; r < n; r++) {
+            d += au[r] * su[r];
+            e += u[r] * su[r];
+            for (int64_t c = 0; c < p; c++) {
+                const double sx = Sigma_iX.a[(size_t)(c * n + r)];
+                h[(size_t)c] += au[r] * sx;
+                g[(size_t)c] += u[r] * sx;
+            }
+        }
+        out.d[(size_t)j] = d;
+        out.e[(size_t)j] = e;
+        for (int64_t c = 0; c < p; c++) {
+            out.h(j, c) = h[(size_t)c];
+            out.g(j, c) = g[(size_t)c];
+        }
+    }
 }
 
 void probe_vector(uint64_t seed, int64_t i, int64_t n, double* out) {

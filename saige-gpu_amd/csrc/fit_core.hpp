@@ -51,6 +51,18 @@ struct Matrix {
     const double* col(int64_t j) const { return a.data() + j * rows; }
 };
 
+// One round of GetTrace / GetTrace_q (FG.cpp:3113-3160, 3409-3472) over
+// probes start..end-1 (k of them), as FP64 scalars:
+//   d_j = (K u_j)^T Sigma^-1 u_j,  e_j = u_j^T Sigma^-1 u_j (quantitative),
+//   h_j = (Sigma^-1 X)^T K u_j,    g_j = (Sigma^-1 X)^T u_j,
+// so that (K u)^T P u = d - h^T cov g and u^T P u = e - g^T cov g.  With PY
+// (the first round) also K PY, Sigma^-1 K PY and (quantitative) Sigma^-1 PY.
+struct TraceTerms {
+    std::vector<double> d, e;          // k
+    Matrix h, g;                       // k x p
+    std::vector<double> APY, SAPY, SPY;
+};
+
 // What a fit needs from the GRM, over the fit's own samples (n of them).
 // Calls block until the result is ready and throw on failure.
 class Solver {
@@ -66,6 +78,13 @@ public:
     // engine's solver starts the solves as soon as the products are done.
     virtual void product_then_solve(const Matrix& B, Matrix& KB, const std::vector<double>& w, const double tau[2],
                                     const std::vector<int>& src, Matrix& X);
+    // One trace round (TraceTerms) with the probes probe_vector(seed, i, n),
+    // i = start..end-1.  The default forms the probes on the host and runs
+    // product_then_solve; the engine's solver keeps the probes, K u and
+    // Sigma^-1 u on the device and returns only the dots.
+    virtual void trace_round(const std::vector<double>& w, const double tau[2], const Matrix& Sigma_iX,
+                             const std::vector<double>* PY, bool quantitative, int64_t start, int64_t end,
+                             uint64_t seed, TraceTerms& out);
 };
 
 // C from B, KB and src as product_then_solve defines it.

@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -35,6 +36,23 @@ using saige::fit::Matrix;
 
 struct Scheduler;
 
+// A trace round run on the device (Solver::trace_round): the probes U, K U
+// and Sigma^-1 X stay in the fit's system buffers; phase 1 takes the
+// products K [PY, U] (K u into AU with h = (Sigma^-1 X)^T K u), phase 2 the
+// solves Sigma^-1 [U, K PY, (PY)] (the U solves only as the dots d and e).
+struct TraceState {
+    const Matrix* SX = nullptr;
+    const std::vector<double>* PY = nullptr;
+    bool quantitative = false;
+    int64_t start = 0, k = 0, p = 0;
+    uint64_t seed = 0;
+    saige::fit::TraceTerms* out = nullptr;
+    int phase = 1;
+    bool setup = false;           // probes, Sigma^-1 X and g are on the device
+    Matrix host1, res1;           // phase 1 host columns (PY) and their products
+    Matrix host2, res2;           // phase 2 host columns (K PY, PY) and their solves
+};
+
 struct Request {
     bool solve = true;            // else a GRM product
     const Matrix* B = nullptr;    // n x k over the fit's samples
@@ -50,6 +68,7 @@ struct Request {
     std::vector<int> src;
     Matrix C;
     Matrix* chain_out = nullptr;
+    TraceState* trace = nullptr;  // a device trace round
 };
 
 struct FitTask : saige::fit::Solver {
@@ -63,6 +82,13 @@ struct FitTask : saige::fit::Solver {
     saige::fit::Result result;
     std::string error;
     bool finished = false;
+    bool released = false;        // its system went back to the pool (the thread has ended)
+    // serving: the request's own copy of its inputs, and when it was
+    // submitted, admitted and finished (steady clock)
+    saige_fit_spec own_spec{};
+    std::vector<double> own_y, own_X, own_offset;
+    std::vector<int64_t> own_subset;
+    std::chrono::steady_clock::time_point t_submit, t_admit, t_done;
     int64_t posts = 0;            // requests posted so far
     std::condition_variable cv;   // the fit waits here for its request
 
@@ -70,13 +96,36 @@ struct FitTask : saige::fit::Solver {
     void product(const Matrix& B, Matrix& KB) override;
     void product_then_solve(const Matrix& B, Matrix& KB, const std::vector<double>& w, const double tau[2],
                             const std::vector<int>& src, Matrix& X) override;
+    void trace_round(const std::vector<double>& w, const double tau[2], const Matrix& Sigma_iX,
+                     const std::vector<double>* PY, bool quantitative, int64_t start, int64_t end, uint64_t seed,
+                     saige::fit::TraceTerms& out) override;
     void submit(Request& r);
+};
+
+// What becomes of a column when its slot is done.
+enum Action : int8_t {
+    kHost = 0,    // download into req->out column j
+    kStash = 1,   // trace phase 1: K u_j into AU column j, h_j dots
+    kPair = 2,    // trace phase 2: Sigma^-1 u_j as the dots d_j (and e_j)
 };
 
 struct Column {
     FitTask* fit;
     Request* req;
     int64_t j;
+    int dev = -1;                 // source: column `dev` of the system's U buffer (else req->B column j)
+    Action act = kHost;
+};
+
+// Device buffers of a system's trace rounds, column-major with leading
+// dimension N: probes U and K U (kTraceProbes columns each), Sigma^-1 X
+// (kTraceCovariates).
+constexpr int kTraceProbes = 64;
+constexpr int kTraceCovariates = 16;   // the shared dot partners of one take
+struct TraceBuffers {
+    float* U = nullptr;
+    float* AU = nullptr;
+    float* SX = nullptr;
 };
 
 struct Scheduler {
@@ -84,10 +133,15 @@ struct Scheduler {
     int64_t N;
     saige::fit::Options opt;
     int width = 64;
+    bool device_trace = true;             // trace rounds on the device (else the host round)
     std::mutex mu;
     std::condition_variable cv;           // the scheduler waits here for requests / finished fits
     std::deque<std::pair<FitTask*, Request*>> inbox;
     std::string fatal;                    // engine failure: every fit stops
+    std::deque<FitTask*> queue;           // submitted, not yet admitted
+    bool stopping = false;                // no more submissions: run() returns when all are done
+    int64_t hbm_reserve = 0;              // admit only while this much HBM is free
+    std::condition_variable done_cv;      // a fit finished (saige_poll waits here)
 
     void post(FitTask* f, Request* r) {
         std::unique_lock<std::mutex> lk(mu);
@@ -146,6 +200,43 @@ void FitTask::product_then_solve(const Matrix& B, Matrix& KB, const std::vector<
     submit(r);
 }
 
+void FitTask::trace_round(const std::vector<double>& w, const double tau[2], const Matrix& Sigma_iX,
+                          const std::vector<double>* PY, bool quantitative, int64_t start, int64_t end,
+                          uint64_t seed, saige::fit::TraceTerms& out) {
+    const int64_t k = end - start;
+    if (!sch->device_trace || k > kTraceProbes || Sigma_iX.cols > kTraceCovariates) {
+        saige::fit::Solver::trace_round(w, tau, Sigma_iX, PY, quantitative, start, end, seed, out);
+        return;
+    }
+    TraceState t;
+    t.SX = &Sigma_iX;
+    t.PY = PY;
+    t.quantitative = quantitative;
+    t.start = start;
+    t.k = k;
+    t.p = Sigma_iX.cols;
+    t.seed = seed;
+    t.out = &out;
+    out.d.assign((size_t)k, 0.0);
+    out.e.assign((size_t)k, 0.0);
+    out.h = Matrix(k, t.p);
+    out.g = Matrix(k, t.p);
+    if (PY) {
+        t.host1 = Matrix(Sigma_iX.rows, 1);
+        std::copy(PY->begin(), PY->end(), t.host1.col(0));
+        t.res1 = Matrix(Sigma_iX.rows, 1);
+    }
+    Request r;
+    r.solve = false;
+    r.B = &t.host1;
+    r.out = &t.res1;
+    r.w = w;
+    r.tau[0] = tau[0];
+    r.tau[1] = tau[1];
+    r.trace = &t;
+    submit(r);
+}
+
 int check(int rc) {
     if (rc != SAIGE_OK) throw std::runtime_error(saige_last_error());
     return rc;
@@ -154,29 +245,42 @@ int check(int rc) {
 // the fit's samples in the N phenotyped samples
 int64_t sample_of(const saige_fit_spec* s, int64_t i) { return s->subset ? s->subset[i] : i; }
 
-void run_batch(Scheduler& sch, std::vector<std::unique_ptr<FitTask>>& fits, int n_systems) {
+// The scheduler loop: admits the queued fits while an engine system and
+// HBM are free, steps the engine over their columns, and returns once
+// `stopping` is set and every submitted fit has finished.
+void run_scheduler(Scheduler& sch, int n_systems) {
     const int64_t N = sch.N;
     check(saige_engine_reserve(sch.ctx, sch.width, n_systems));
     std::deque<int> free_sys;
     for (int s = 0; s < n_systems; s++) free_sys.push_back(s);
     std::vector<int> free_slots;                      // lowest slot first (min-heap)
     for (int s = sch.width - 1; s >= 0; s--) free_slots.push_back(s);
-    std::deque<FitTask*> waiting;
-    for (auto& f : fits) waiting.push_back(f.get());
+    std::vector<FitTask*> active;                     // admitted, system not yet released
     std::deque<Column> pending;
     std::vector<Column> owner((size_t)sch.width, Column{nullptr, nullptr, 0});
     std::vector<uint8_t> done((size_t)sch.width);
     std::vector<int32_t> iters((size_t)sch.width);
-    size_t n_finished = 0, live = 0;
+    size_t live = 0;
     std::vector<float> colbuf;
     std::vector<uint8_t> mask((size_t)N);
     std::vector<float> wfull((size_t)N);
+    // the trace rounds' device buffers, per system, made on first use and
+    // freed when the batch ends
+    struct Buffers {
+        saige_ctx* ctx;
+        std::vector<TraceBuffers> b;
+        ~Buffers() {
+            for (auto& t : b)
+                for (float* p : {t.U, t.AU, t.SX})
+                    if (p) saige_free_device(ctx, p);
+        }
+    } tbufs{sch.ctx, std::vector<TraceBuffers>((size_t)n_systems)};
     // fits whose request the last step completed, with their post count then
     std::vector<std::pair<FitTask*, int64_t>> resumed;
     // wall seconds of the loop's parts, printed to stderr at the end when the
     // environment sets SAIGE_ENGINE_TIMING (diagnostics only)
     const bool timing = std::getenv("SAIGE_ENGINE_TIMING") != nullptr;
-    double t_idle = 0, t_fill = 0, t_step = 0, t_wait_posts = 0, t_take = 0;
+    double t_idle = 0, t_fill = 0, t_step = 0, t_wait_posts = 0, t_take = 0, t_setup = 0;
     int64_t n_steps = 0;
     using clk = std::chrono::steady_clock;
     auto secs = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
@@ -194,14 +298,83 @@ void run_batch(Scheduler& sch, std::vector<std::unique_ptr<FitTask>>& fits, int 
         check(saige_engine_set_system(sch.ctx, f->sys, wfull.data(), t, mask.data()));
     };
 
+    // a trace round's device side before its first column runs: the probes
+    // into U (generated in place, or uploaded for a sample subset), Sigma^-1 X
+    // into SX, and g = (Sigma^-1 X)^T u
+    auto trace_setup = [&](FitTask* f, TraceState& t) {
+        const auto t0 = clk::now();
+        TraceBuffers& b = tbufs.b[(size_t)f->sys];
+        if (!b.U) {
+            void* p = nullptr;
+            check(saige_alloc_device(sch.ctx, 4 * N * kTraceProbes, &p));
+            b.U = (float*)p;
+            check(saige_alloc_device(sch.ctx, 4 * N * kTraceProbes, &p));
+            b.AU = (float*)p;
+            check(saige_alloc_device(sch.ctx, 4 * N * kTraceCovariates, &p));
+            b.SX = (float*)p;
+        }
+        const saige_fit_spec* sp = f->spec;
+        const int64_t n = sp->n, k = t.k, p = t.p;
+        if (!sp->subset) {
+            check(saige_probes_device(sch.ctx, b.U, N, t.seed, t.start, (int)k));
+        } else {
+            colbuf.assign((size_t)(N * k), 0.0f);
+            std::vector<double> pv((size_t)n);
+            for (int64_t j = 0; j < k; j++) {
+                saige::fit::probe_vector(t.seed, t.start + j, n, pv.data());
+                float* d = colbuf.data() + j * N;
+                for (int64_t i = 0; i < n; i++) d[sp->subset[i]] = (float)pv[(size_t)i];
+            }
+            check(saige_copy_host_to_device(sch.ctx, b.U, colbuf.data(), 4 * N * k));
+        }
+        colbuf.assign((size_t)(N * p), 0.0f);
+        for (int64_t c = 0; c < p; c++) {
+            const double* x = t.SX->col(c);
+            float* d = colbuf.data() + c * N;
+            for (int64_t i = 0; i < n; i++) d[sample_of(sp, i)] = (float)x[i];
+        }
+        check(saige_copy_host_to_device(sch.ctx, b.SX, colbuf.data(), 4 * N * p));
+        std::vector<double> g((size_t)(k * p));
+        check(saige_dots_device(sch.ctx, b.U, N, (int)k, b.SX, N, (int)p, g.data()));
+        for (int64_t j = 0; j < k; j++)
+            for (int64_t c = 0; c < p; c++) t.out->g(j, c) = g[(size_t)(j * p + c)];
+        t.setup = true;
+        t_setup += secs(t0);
+    };
+
+    // admission by free HBM: a fit's trace buffers and host-visible state
+    // come out of what is free on the device
+    auto hbm_ok = [&] {
+        if (sch.hbm_reserve <= 0) return true;
+        int64_t free_b = 0, total_b = 0;
+        return saige_device_memory(sch.ctx, &free_b, &total_b) == SAIGE_OK && free_b >= sch.hbm_reserve;
+    };
+
     try {
-    while (n_finished < fits.size()) {
-        // admit fits while a system is free
-        while (!waiting.empty() && !free_sys.empty()) {
-            FitTask* f = waiting.front();
-            waiting.pop_front();
+    for (;;) {
+        // admit fits while a system (and HBM) is free
+        std::vector<FitTask*> admit;
+        {
+            std::lock_guard<std::mutex> lk(sch.mu);
+            if (sch.stopping && sch.queue.empty() && active.empty()) break;
+            while (!sch.queue.empty() && free_sys.size() > admit.size()) {
+                admit.push_back(sch.queue.front());
+                sch.queue.pop_front();
+            }
+        }
+        bool hbm_short = false;
+        if (!admit.empty() && !hbm_ok()) {
+            std::lock_guard<std::mutex> lk(sch.mu);   // back to the head of the queue, in order
+            for (auto it = admit.rbegin(); it != admit.rend(); ++it) sch.queue.push_front(*it);
+            admit.clear();
+            hbm_short = true;
+        }
+        for (FitTask* f : admit) {
             f->sys = free_sys.front();
             free_sys.pop_front();
+            f->sys_set = false;
+            f->t_admit = clk::now();
+            active.push_back(f);
             resumed.emplace_back(f, 0);               // its first request is awaited like a resumed fit's
             f->th = std::thread([f, &sch] {
                 try {
@@ -219,8 +392,10 @@ void run_batch(Scheduler& sch, std::vector<std::unique_ptr<FitTask>>& fits, int 
                     f->error = e.what();
                 }
                 std::lock_guard<std::mutex> lk(sch.mu);
+                f->t_done = clk::now();
                 f->finished = true;
                 sch.cv.notify_all();
+                sch.done_cv.notify_all();
             });
         }
         // collect new requests; release finished fits' systems
@@ -232,18 +407,25 @@ void run_batch(Scheduler& sch, std::vector<std::unique_ptr<FitTask>>& fits, int 
                 // that was just admitted) has posted its next request or
                 // finished, so that the fits' next phases start in the same
                 // iteration and share its launches (fits that drift apart run
-                // more of them); with none such, for any request or finished fit
-                sch.cv.wait(lk, [&] {
+                // more of them); with none such, for any request, finished fit
+                // or arrival that can be admitted (with HBM short, arrivals
+                // are looked at again every 10 ms)
+                auto ready = [&] {
                     if (!resumed.empty()) {
                         for (auto& fp : resumed)
                             if (!fp.first->finished && fp.first->posts <= fp.second) return false;
                         return true;
                     }
                     if (!sch.inbox.empty()) return true;
-                    for (auto& f : fits)
-                        if (f->finished && f->sys >= 0) return true;
-                    return false;
-                });
+                    for (FitTask* f : active)
+                        if (f->finished) return true;
+                    if (!hbm_short && !sch.queue.empty() && !free_sys.empty()) return true;   // an arrival
+                    return sch.stopping && sch.queue.empty() && active.empty();
+                };
+                if (hbm_short)
+                    sch.cv.wait_for(lk, std::chrono::milliseconds(10), ready);
+                else
+                    sch.cv.wait(lk, ready);
                 resumed.clear();
             }
             t_idle += secs(t0);
@@ -252,52 +434,85 @@ void run_batch(Scheduler& sch, std::vector<std::unique_ptr<FitTask>>& fits, int 
                 sch.inbox.pop_front();
                 r->remaining = r->B->cols;
                 for (int64_t j = 0; j < r->B->cols; j++) pending.push_back(Column{f, r, j});
-                if (r->B->cols == 0) {
+                if (r->trace) {                        // phase 1: K [PY, U]; K u stays on the device
+                    r->remaining += r->trace->k;
+                    for (int64_t j = 0; j < r->trace->k; j++) pending.push_back(Column{f, r, j, (int)j, kStash});
+                }
+                if (r->remaining == 0) {
                     r->done = true;
                     f->cv.notify_all();
                 }
             }
-            for (auto& f : fits)
-                if (f->finished && f->sys >= 0) {
-                    free_sys.push_back(f->sys);
-                    f->sys = -1;
-                    n_finished++;
+            for (size_t i = 0; i < active.size();) {
+                FitTask* f = active[i];
+                if (!f->finished) {
+                    i++;
+                    continue;
                 }
+                free_sys.push_back(f->sys);
+                f->sys = -1;
+                active.erase(active.begin() + (ptrdiff_t)i);
+                f->released = true;
+                sch.done_cv.notify_all();
+            }
         }
-        // fill free slots: one batch of solves and one of products
+        // fill free slots: one batch of host solves, one of host products, and
+        // runs of consecutive device columns
         const auto t_fill0 = clk::now();
         std::vector<int32_t> s_slots, s_sys, p_slots, p_sys;
         std::vector<Column> s_cols, p_cols;
-        std::deque<Column> keep;
+        struct DevRun {
+            bool solve;
+            FitTask* fit;
+            int col0;
+            std::vector<int32_t> slots, systems;
+        };
+        std::vector<DevRun> runs;
+        const Request* checked = nullptr;             // the request whose system was last checked
         while (!pending.empty() && !free_slots.empty()) {
             Column c = pending.front();
             pending.pop_front();
             FitTask* f = c.fit;
-            if (c.req->solve || c.req->chain) {
-                // a fit has one request in flight, so its system is free to change
-                const bool same = f->sys_set && f->set_tau[0] == c.req->tau[0] && f->set_tau[1] == c.req->tau[1] &&
-                                  f->set_w == c.req->w;
-                if (!same) {
-                    set_system(f, c.req->w.data(), c.req->tau);
-                    f->set_w = c.req->w;
-                    f->set_tau[0] = c.req->tau[0];
-                    f->set_tau[1] = c.req->tau[1];
+            if (c.req->trace && !c.req->trace->setup) trace_setup(f, *c.req->trace);
+            if (c.req != checked) {
+                if (c.req->solve || c.req->chain) {
+                    // a fit has one request in flight, so its system is free to change
+                    const bool same = f->sys_set && f->set_tau[0] == c.req->tau[0] &&
+                                      f->set_tau[1] == c.req->tau[1] && f->set_w == c.req->w;
+                    if (!same) {
+                        set_system(f, c.req->w.data(), c.req->tau);
+                        f->set_w = c.req->w;
+                        f->set_tau[0] = c.req->tau[0];
+                        f->set_tau[1] = c.req->tau[1];
+                        f->sys_set = true;
+                    }
+                } else if (!f->sys_set) {
+                    const double t[2] = {1.0, 0.0};
+                    set_system(f, nullptr, t);          // products only need the sample mask
                     f->sys_set = true;
+                    f->set_w.clear();
                 }
-            } else if (!f->sys_set) {
-                const double t[2] = {1.0, 0.0};
-                set_system(f, nullptr, t);              // products only need the sample mask
-                f->sys_set = true;
-                f->set_w.clear();
+                checked = c.req;
             }
             std::pop_heap(free_slots.begin(), free_slots.end(), std::greater<int>());
             const int slot = free_slots.back();
             free_slots.pop_back();
             owner[(size_t)slot] = c;
+            live++;
+            if (c.dev >= 0) {
+                DevRun* run = runs.empty() ? nullptr : &runs.back();
+                if (!run || run->solve != c.req->solve || run->fit != f ||
+                    run->col0 + (int)run->slots.size() != c.dev) {
+                    runs.push_back(DevRun{c.req->solve, f, c.dev, {}, {}});
+                    run = &runs.back();
+                }
+                run->slots.push_back(slot);
+                run->systems.push_back(f->sys);
+                continue;
+            }
             (c.req->solve ? s_slots : p_slots).push_back(slot);
             (c.req->solve ? s_sys : p_sys).push_back(f->sys);
             (c.req->solve ? s_cols : p_cols).push_back(c);
-            live++;
         }
         auto upload = [&](const std::vector<Column>& cs) {
             colbuf.assign((size_t)(N * (int64_t)cs.size()), 0.0f);
@@ -317,6 +532,15 @@ void run_batch(Scheduler& sch, std::vector<std::unique_ptr<FitTask>>& fits, int 
             upload(p_cols);
             check(saige_engine_add_products(sch.ctx, (int)p_slots.size(), p_slots.data(), p_sys.data(),
                                             colbuf.data()));
+        }
+        for (const DevRun& r : runs) {
+            const float* B = tbufs.b[(size_t)r.fit->sys].U + (int64_t)r.col0 * N;
+            if (r.solve)
+                check(saige_engine_add_batch_device(sch.ctx, (int)r.slots.size(), r.slots.data(), r.systems.data(), B,
+                                                    sch.opt.maxiter_pcg, (float)sch.opt.tol_pcg));
+            else
+                check(saige_engine_add_products_device(sch.ctx, (int)r.slots.size(), r.slots.data(),
+                                                       r.systems.data(), B));
         }
         t_fill += secs(t_fill0);
         if (live == 0) continue;
@@ -345,24 +569,72 @@ void run_batch(Scheduler& sch, std::vector<std::unique_ptr<FitTask>>& fits, int 
         n_steps++;
         if (step_rc != SAIGE_OK) throw std::runtime_error(step_err);
         const auto t_take0 = clk::now();
-        std::vector<int32_t> fin;
+        std::vector<int32_t> fin, host;
         for (int s = 0; s < sch.width; s++)
-            if (owner[(size_t)s].req && done[(size_t)s]) fin.push_back(s);
+            if (owner[(size_t)s].req && done[(size_t)s]) {
+                fin.push_back(s);
+                if (owner[(size_t)s].act == kHost) host.push_back(s);
+            }
         if (fin.empty()) continue;
-        colbuf.assign((size_t)(N * (int64_t)fin.size()), 0.0f);
-        check(saige_engine_take_batch(sch.ctx, (int)fin.size(), fin.data(), colbuf.data()));
+        // device columns of a trace round: K u to AU with the h dots, or the
+        // Sigma^-1 u dots d (and e); one take per fit and kind
+        {
+            std::vector<int32_t> sl, cl;
+            std::vector<double> dots;
+            std::vector<char> seen(fin.size(), 0);
+            for (size_t a = 0; a < fin.size(); a++) {
+                const Column& c0 = owner[(size_t)fin[a]];
+                if (c0.act == kHost || seen[a]) continue;
+                sl.clear();
+                cl.clear();
+                for (size_t b = a; b < fin.size(); b++) {
+                    const Column& c = owner[(size_t)fin[b]];
+                    if (!seen[b] && c.act == c0.act && c.fit == c0.fit) {
+                        seen[b] = 1;
+                        sl.push_back(fin[b]);
+                        cl.push_back(c.dev);
+                    }
+                }
+                TraceState& t = *c0.req->trace;
+                const TraceBuffers& tb = tbufs.b[(size_t)c0.fit->sys];
+                const int kk = (int)sl.size();
+                if (c0.act == kStash) {
+                    dots.resize((size_t)(kk * t.p));
+                    check(saige_engine_take_batch_ex(sch.ctx, kk, sl.data(), nullptr, tb.AU, N, cl.data(), nullptr,
+                                                     nullptr, N, nullptr, tb.SX, (int)t.p, N, dots.data()));
+                    for (int q = 0; q < kk; q++)
+                        for (int64_t c = 0; c < t.p; c++) t.out->h(cl[(size_t)q], c) = dots[(size_t)(q * t.p + c)];
+                } else {
+                    const int nd = t.quantitative ? 2 : 1;
+                    dots.resize((size_t)(kk * nd));
+                    check(saige_engine_take_batch_ex(sch.ctx, kk, sl.data(), nullptr, nullptr, N, nullptr, tb.AU,
+                                                     t.quantitative ? tb.U : nullptr, N, cl.data(), nullptr, 0, N,
+                                                     dots.data()));
+                    for (int q = 0; q < kk; q++) {
+                        t.out->d[(size_t)cl[(size_t)q]] = dots[(size_t)(q * nd)];
+                        if (t.quantitative) t.out->e[(size_t)cl[(size_t)q]] = dots[(size_t)(q * nd + 1)];
+                    }
+                }
+            }
+        }
+        if (!host.empty()) {
+            colbuf.assign((size_t)(N * (int64_t)host.size()), 0.0f);
+            check(saige_engine_take_batch(sch.ctx, (int)host.size(), host.data(), colbuf.data()));
+        }
         std::lock_guard<std::mutex> lk(sch.mu);
-        for (size_t k = 0; k < fin.size(); k++) {
-            const int slot = fin[k];
+        size_t hk = 0;
+        for (const int32_t slot : fin) {
             Column c = owner[(size_t)slot];
             owner[(size_t)slot] = Column{nullptr, nullptr, 0};
             free_slots.push_back(slot);
             std::push_heap(free_slots.begin(), free_slots.end(), std::greater<int>());
             live--;
-            const saige_fit_spec* s = c.fit->spec;
-            const float* x = colbuf.data() + (int64_t)k * N;
-            double* o = c.req->out->col(c.j);
-            for (int64_t i = 0; i < s->n; i++) o[i] = (double)x[sample_of(s, i)];
+            if (c.act == kHost) {
+                const saige_fit_spec* s = c.fit->spec;
+                const float* x = colbuf.data() + (int64_t)(hk++) * N;
+                double* o = c.req->out->col(c.j);
+                for (int64_t i = 0; i < s->n; i++) o[i] = (double)x[sample_of(s, i)];
+            }
             if (--c.req->remaining == 0) {
                 Request* r = c.req;
                 if (r->chain && !r->solve) {
@@ -373,6 +645,31 @@ void run_batch(Scheduler& sch, std::vector<std::unique_ptr<FitTask>>& fits, int 
                     r->out = r->chain_out;
                     r->remaining = r->C.cols;
                     for (int64_t j = 0; j < r->C.cols; j++) pending.push_back(Column{c.fit, r, j});
+                } else if (r->trace && r->trace->phase == 1) {
+                    // phase 2: Sigma^-1 [U, K PY, (PY)], the U solves as dots only
+                    TraceState& t = *r->trace;
+                    t.phase = 2;
+                    const int64_t n = c.fit->spec->n;
+                    if (t.PY) {
+                        t.host2 = Matrix(n, t.quantitative ? 2 : 1);
+                        std::copy(t.res1.col(0), t.res1.col(0) + n, t.host2.col(0));
+                        if (t.quantitative) std::copy(t.PY->begin(), t.PY->end(), t.host2.col(1));
+                        t.res2 = Matrix(n, t.host2.cols);
+                    }
+                    r->solve = true;
+                    r->B = &t.host2;
+                    r->out = &t.res2;
+                    r->remaining = t.k + t.host2.cols;
+                    for (int64_t j = 0; j < t.k; j++) pending.push_back(Column{c.fit, r, j, (int)j, kPair});
+                    for (int64_t j = 0; j < t.host2.cols; j++) pending.push_back(Column{c.fit, r, j});
+                } else if (r->trace) {
+                    TraceState& t = *r->trace;
+                    if (t.PY) {
+                        const int64_t n = c.fit->spec->n;
+                        t.out->APY.assign(t.res1.col(0), t.res1.col(0) + n);
+                        t.out->SAPY.assign(t.res2.col(0), t.res2.col(0) + n);
+                        if (t.quantitative) t.out->SPY.assign(t.res2.col(1), t.res2.col(1) + n);
+                    }
                 }
                 if (r->remaining == 0) {
                     r->done = true;
@@ -385,9 +682,9 @@ void run_batch(Scheduler& sch, std::vector<std::unique_ptr<FitTask>>& fits, int 
     }
     if (timing)
         std::fprintf(stderr,
-                     "saige engine: %lld steps; idle (waiting for requests) %.4f s, fill %.4f s, step %.4f s "
-                     "(of which waiting for resumed fits %.4f s), take %.4f s\n",
-                     (long long)n_steps, t_idle, t_fill, t_step, t_wait_posts, t_take);
+                     "saige engine: %lld steps; idle (waiting for requests) %.4f s, fill %.4f s (of which trace "
+                     "setup %.4f s), step %.4f s (of which waiting for resumed fits %.4f s), take %.4f s\n",
+                     (long long)n_steps, t_idle, t_fill, t_setup, t_step, t_wait_posts, t_take);
     } catch (const std::exception& e) {
         // the engine failed: fail every request in the inbox, waiting for
         // slots or in a slot, so that every admitted fit's thread ends; later
@@ -455,7 +752,58 @@ bool valid_glm_inputs(int trait, int64_t n, const double* y, const double* X, in
     return true;
 }
 
+// A finished fit's outputs into the caller's result (its buffers sized by
+// the spec); false when the fit failed (status says why).
+bool fill_result(const FitTask& f, const saige_fit_spec& s, int engine_rc, saige_fit_result& out) {
+    const bool ran = f.finished && f.error.empty();
+    out.status = ran ? SAIGE_OK : (engine_rc != SAIGE_OK ? SAIGE_E_DEVICE : SAIGE_E_STATE);
+    if (!ran) return false;
+    const auto& r = f.result;
+    out.theta[0] = r.theta[0];
+    out.theta[1] = r.theta[1];
+    out.converged = r.converged ? 1 : 0;
+    out.iterations = r.iterations;
+    std::memcpy(out.coefficients, r.coefficients.data(), sizeof(double) * (size_t)s.p);
+    if (out.linear_predictors)
+        std::memcpy(out.linear_predictors, r.linear_predictors.data(), sizeof(double) * (size_t)s.n);
+    if (out.fitted_values) std::memcpy(out.fitted_values, r.fitted_values.data(), sizeof(double) * (size_t)s.n);
+    if (out.working_y) std::memcpy(out.working_y, r.working_y.data(), sizeof(double) * (size_t)s.n);
+    if (out.cov) std::memcpy(out.cov, r.cov.a.data(), sizeof(double) * (size_t)(s.p * s.p));
+    return true;
+}
+
+// The checks saige_fit_null_batch and saige_submit make on one request.
+int check_spec(const saige_fit_spec& s, int64_t N, const char* who) {
+    if (s.n <= 0 || s.p <= 0 || !s.y || !s.X || (!s.subset && s.n != N) || s.n > N ||
+        (s.trait != SAIGE_TRAIT_BINARY && s.trait != SAIGE_TRAIT_QUANTITATIVE)) {
+        saige::set_error("%s: invalid fit specification", who);
+        return SAIGE_E_ARG;
+    }
+    if (s.subset)
+        for (int64_t k = 0; k < s.n; k++)
+            if (s.subset[k] < 0 || s.subset[k] >= N || (k > 0 && s.subset[k] <= s.subset[k - 1])) {
+                saige::set_error("%s: the sample subset must be ascending indices below %lld", who, (long long)N);
+                return SAIGE_E_ARG;
+            }
+    if (!valid_glm_inputs(s.trait, s.n, s.y, s.X, s.p, s.offset, who)) return SAIGE_E_ARG;
+    return SAIGE_OK;
+}
+
 }  // namespace
+
+// A request server on one context: the scheduler loop on a thread of its
+// own, fed by saige_submit; tasks stay until saige_poll hands their result
+// over.
+struct saige_server {
+    Scheduler sch;
+    int n_systems = 16;
+    std::thread th;
+    std::map<int64_t, std::unique_ptr<FitTask>> tasks;   // under sch.mu
+    int64_t next_ticket = 1;
+    bool ended = false;          // the scheduler thread has returned
+    int rc = SAIGE_OK;
+    std::string fatal;
+};
 
 extern "C" {
 
@@ -464,15 +812,8 @@ int saige_fit_null_batch(saige_ctx* ctx, int n_fits, const saige_fit_spec* specs
     if (!ctx || n_fits < 0 || (n_fits > 0 && (!specs || !results))) return SAIGE_E_ARG;
     const int64_t N = saige_get_nnomissing(ctx);
     for (int i = 0; i < n_fits; i++) {
-        const saige_fit_spec& s = specs[i];
-        if (s.n <= 0 || s.p <= 0 || !s.y || !s.X || (!s.subset && s.n != N) || s.n > N ||
-            (s.trait != SAIGE_TRAIT_BINARY && s.trait != SAIGE_TRAIT_QUANTITATIVE) || !results[i].coefficients)
-            return SAIGE_E_ARG;
-        if (!valid_glm_inputs(s.trait, s.n, s.y, s.X, s.p, s.offset, "saige_fit_null_batch")) return SAIGE_E_ARG;
-        if (s.subset)
-            for (int64_t k = 0; k < s.n; k++)
-                if (s.subset[k] < 0 || s.subset[k] >= N || (k > 0 && s.subset[k] <= s.subset[k - 1]))
-                    return SAIGE_E_ARG;
+        if (!results[i].coefficients) return SAIGE_E_ARG;
+        if (int rc = check_spec(specs[i], N, "saige_fit_null_batch")) return rc;
     }
     Scheduler sch;
     sch.ctx = ctx;
@@ -496,7 +837,15 @@ int saige_fit_null_batch(saige_ctx* ctx, int n_fits, const saige_fit_spec* specs
     int rc = SAIGE_OK;
     std::string fatal;
     try {
-        run_batch(sch, fits, n_systems);
+        {
+            std::lock_guard<std::mutex> lk(sch.mu);
+            for (auto& f : fits) {
+                f->t_submit = std::chrono::steady_clock::now();
+                sch.queue.push_back(f.get());
+            }
+            sch.stopping = true;
+        }
+        run_scheduler(sch, n_systems);
     } catch (const std::exception& e) {
         rc = SAIGE_E_DEVICE;
         fatal = e.what();
@@ -505,28 +854,131 @@ int saige_fit_null_batch(saige_ctx* ctx, int n_fits, const saige_fit_spec* specs
         if (f->th.joinable()) f->th.join();
     std::string first_error;
     for (int i = 0; i < n_fits; i++) {
-        saige_fit_result& out = results[i];
-        const saige_fit_spec& s = specs[i];
         const FitTask& f = *fits[(size_t)i];
-        const bool ran = f.finished && f.error.empty();
-        out.status = ran ? SAIGE_OK : (rc != SAIGE_OK ? SAIGE_E_DEVICE : SAIGE_E_STATE);
-        if (!ran) {
-            if (first_error.empty()) first_error = f.error.empty() ? fatal : f.error;
-            continue;
-        }
-        const auto& r = f.result;
-        out.theta[0] = r.theta[0];
-        out.theta[1] = r.theta[1];
-        out.converged = r.converged ? 1 : 0;
-        out.iterations = r.iterations;
-        std::memcpy(out.coefficients, r.coefficients.data(), sizeof(double) * (size_t)s.p);
-        if (out.linear_predictors)
-            std::memcpy(out.linear_predictors, r.linear_predictors.data(), sizeof(double) * (size_t)s.n);
-        if (out.fitted_values) std::memcpy(out.fitted_values, r.fitted_values.data(), sizeof(double) * (size_t)s.n);
-        if (out.working_y) std::memcpy(out.working_y, r.working_y.data(), sizeof(double) * (size_t)s.n);
-        if (out.cov) std::memcpy(out.cov, r.cov.a.data(), sizeof(double) * (size_t)(s.p * s.p));
+        if (!fill_result(f, specs[i], rc, results[i]) && first_error.empty())
+            first_error = f.error.empty() ? fatal : f.error;
     }
     if (!first_error.empty()) saige::set_error("%s", first_error.c_str());
+    return rc;
+}
+
+int saige_server_start(saige_ctx* ctx, const saige_fit_options* o, int max_concurrent, int64_t hbm_reserve_bytes,
+                       saige_server** out) {
+    if (!ctx || !out || hbm_reserve_bytes < 0) return SAIGE_E_ARG;
+    const int64_t N = saige_get_nnomissing(ctx);
+    if (N <= 0) {
+        saige::set_error("saige_server_start: no genotypes loaded");
+        return SAIGE_E_STATE;
+    }
+    auto srv = std::make_unique<saige_server>();
+    srv->sch.ctx = ctx;
+    srv->sch.N = N;
+    if (o) {
+        srv->sch.opt.maxiter = o->maxiter;
+        srv->sch.opt.tol = o->tol;
+        srv->sch.opt.nrun = o->nrun;
+        srv->sch.opt.tol_pcg = o->tol_pcg;
+        srv->sch.opt.maxiter_pcg = o->maxiter_pcg;
+        srv->sch.opt.trace_cv_cutoff = o->trace_cv_cutoff;
+        srv->sch.opt.probe_seed = o->probe_seed;
+    }
+    srv->sch.hbm_reserve = hbm_reserve_bytes;
+    srv->n_systems = std::max(1, max_concurrent > 0 ? max_concurrent : 16);
+    saige_server* sp = srv.get();
+    sp->th = std::thread([sp] {
+        int rc = SAIGE_OK;
+        std::string err;
+        try {
+            run_scheduler(sp->sch, sp->n_systems);
+        } catch (const std::exception& e) {
+            rc = SAIGE_E_DEVICE;
+            err = e.what();
+        }
+        std::lock_guard<std::mutex> lk(sp->sch.mu);
+        sp->rc = rc;
+        sp->fatal = err;
+        sp->ended = true;
+        sp->sch.done_cv.notify_all();
+    });
+    *out = srv.release();
+    return SAIGE_OK;
+}
+
+int saige_submit(saige_server* srv, const saige_fit_spec* spec, int64_t* ticket) {
+    if (!srv || !spec || !ticket) return SAIGE_E_ARG;
+    if (int rc = check_spec(*spec, srv->sch.N, "saige_submit")) return rc;
+    auto f = std::make_unique<FitTask>();
+    const saige_fit_spec& s = *spec;
+    f->own_y.assign(s.y, s.y + s.n);
+    f->own_X.assign(s.X, s.X + s.n * (int64_t)s.p);
+    if (s.offset) f->own_offset.assign(s.offset, s.offset + s.n);
+    if (s.subset) f->own_subset.assign(s.subset, s.subset + s.n);
+    f->own_spec = s;
+    f->own_spec.y = f->own_y.data();
+    f->own_spec.X = f->own_X.data();
+    f->own_spec.offset = s.offset ? f->own_offset.data() : nullptr;
+    f->own_spec.subset = s.subset ? f->own_subset.data() : nullptr;
+    f->spec = &f->own_spec;
+    f->sch = &srv->sch;
+    std::lock_guard<std::mutex> lk(srv->sch.mu);
+    if (srv->ended || srv->sch.stopping) {
+        saige::set_error("saige_submit: the server has stopped%s%s", srv->fatal.empty() ? "" : ": ",
+                         srv->fatal.c_str());
+        return SAIGE_E_STATE;
+    }
+    f->t_submit = std::chrono::steady_clock::now();
+    srv->sch.queue.push_back(f.get());
+    *ticket = srv->next_ticket++;
+    srv->tasks.emplace(*ticket, std::move(f));
+    srv->sch.cv.notify_all();
+    return SAIGE_OK;
+}
+
+int saige_poll(saige_server* srv, int64_t ticket, int wait_ms, saige_fit_result* result, double* times) {
+    if (!srv || !result || !result->coefficients) return SAIGE_E_ARG;
+    std::unique_lock<std::mutex> lk(srv->sch.mu);
+    auto it = srv->tasks.find(ticket);
+    if (it == srv->tasks.end()) {
+        saige::set_error("saige_poll: unknown ticket %lld", (long long)ticket);
+        return SAIGE_E_ARG;
+    }
+    FitTask* f = it->second.get();
+    auto ready = [&] { return f->released || (srv->ended && f->finished) || (srv->ended && !f->th.joinable()); };
+    if (wait_ms < 0)
+        srv->sch.done_cv.wait(lk, ready);
+    else if (wait_ms > 0)
+        srv->sch.done_cv.wait_for(lk, std::chrono::milliseconds(wait_ms), ready);
+    if (!ready()) return SAIGE_PENDING;
+    std::unique_ptr<FitTask> task = std::move(it->second);
+    srv->tasks.erase(it);
+    const int engine_rc = srv->rc;
+    const std::string fatal = srv->fatal;
+    lk.unlock();
+    if (task->th.joinable()) task->th.join();
+    if (times) {
+        using sec = std::chrono::duration<double>;
+        const bool admitted = task->t_admit.time_since_epoch().count() != 0;
+        times[0] = admitted ? sec(task->t_admit - task->t_submit).count() : -1.0;
+        times[1] = task->finished ? sec(task->t_done - task->t_submit).count() : -1.0;
+    }
+    if (!fill_result(*task, task->own_spec, engine_rc, *result))
+        saige::set_error("%s", task->error.empty() ? (fatal.empty() ? "not run" : fatal.c_str()) : task->error.c_str());
+    return SAIGE_OK;
+}
+
+int saige_server_stop(saige_server* srv) {
+    if (!srv) return SAIGE_E_ARG;
+    {
+        std::lock_guard<std::mutex> lk(srv->sch.mu);
+        srv->sch.stopping = true;
+        srv->sch.cv.notify_all();
+    }
+    if (srv->th.joinable()) srv->th.join();
+    for (auto& kv : srv->tasks)
+        if (kv.second->th.joinable()) kv.second->th.join();
+    const int rc = srv->rc;
+    if (rc != SAIGE_OK) saige::set_error("%s", srv->fatal.c_str());
+    delete srv;
     return rc;
 }
 

@@ -31,79 +31,135 @@ __device__ __forceinline__ unsigned internal_code(int g) {
     return g == 0 ? 3u : (g == 1 ? 2u : 0u);
 }
 
-// One workgroup per marker (grid-stride).  Counts over phenotyped samples,
-// then thread 0 evaluates the reference's FP32 formulas in order
-// (FG.cpp:424-547, 914-921).
+// Counts of one 32-bit word of PLINK codes (16 samples) under a code mask
+// (bit 2s set: sample s of the word is counted): with l, h the low and high
+// bit of a code, missing = l & ~h (01), and a counted non-missing code adds
+// 2 - l - h A1 alleles, where l = 1 only together with h = 1 (code 11).
+// miss += missing codes, nonm += counted non-missing codes, t += sum of
+// l + h over them; alleleCount = 2 nonm - t.
+__device__ __forceinline__ void count_word(uint32_t w, uint32_t msk, uint32_t& miss, uint32_t& nonm,
+                                           uint32_t& t) {
+    const uint32_t L = w & msk, H = (w >> 1) & msk;
+    const uint32_t Mi = L & ~H;
+    const uint32_t nmiss = (uint32_t)__popc(Mi);
+    miss += nmiss;
+    nonm += (uint32_t)__popc(msk) - nmiss;
+    t += (uint32_t)__popc(L & H) + (uint32_t)__popc(H);
+}
+
+// The reference's FP32 per-marker formulas in order (FG.cpp:424-547,
+// 914-921) from the integer counts over the phenotyped samples.
+__device__ __forceinline__ MarkerStat marker_stat(int alleleCount, int numMissing, const DecodeParams& p,
+                                                  int inr) {
+    const uint64_t nnm = (uint64_t)p.Nnm;
+    // FG.cpp:424 and 442
+    float altFreq = div_rn((float)alleleCount, (float)((nnm - (uint64_t)numMissing) * 2u));
+    float missingRate = div_rn((float)numMissing, (float)nnm);
+    // FG.cpp:447; NaN (all calls missing) maps to 0
+    float twice = __fmul_rn(2.0f, altFreq);
+    int fill = (twice != twice) ? 0 : (int)roundf(twice);
+    if (numMissing > 0) alleleCount = alleleCount + fill * numMissing;   // FG.cpp:454
+    altFreq = div_rn((float)alleleCount, (float)(nnm * 2u));          // FG.cpp:484
+    float one_m = __fsub_rn(1.0f, altFreq);
+    float maf = (one_m < altFreq) ? one_m : altFreq;                     // std::min
+    int twoN = (int)p.Nnm * 2;
+    int mac = (alleleCount < twoN - alleleCount) ? alleleCount : twoN - alleleCount;
+    int passQC = (maf >= p.min_maf && missingRate <= p.max_missing) ? 1 : 0;
+    int passVR = 0;
+    if (p.vr_mode == 1) {
+        if ((float)mac >= p.min_mac_vr && (float)mac < p.max_mac_vr) passVR = 1;
+        else if ((float)mac >= p.max_mac_vr) passVR = inr;
+    } else if (p.vr_mode == 2) {
+        if ((float)mac >= p.min_mac_vr) passVR = inr;
+    }
+    if (passVR) passQC = 0;
+    // FG.cpp:914-921
+    float Std = sqrt_rn(__fmul_rn(__fmul_rn(2.0f, altFreq), one_m));
+    float invStd = (Std == 0.0f) ? 0.0f : div_rn(1.0f, Std);
+    MarkerStat st;
+    st.allele_count = alleleCount;
+    st.num_missing = numMissing;
+    st.mac = mac;
+    st.fill = fill;
+    st.freq = altFreq;
+    st.missing_rate = missingRate;
+    st.inv_std = invStd;
+    st.pass_qc = (uint8_t)passQC;
+    st.pass_vr = (uint8_t)passVR;
+    st.pad[0] = st.pad[1] = 0;
+    return st;
+}
+
+// One wave per marker, four markers per workgroup, no workgroup barrier.
+// A row of bpm bytes is read as the 16-byte chunks of the aligned span that
+// covers it (a chunk always holds at least one byte of the row, so it never
+// leaves the row's page), two chunks per lane in flight.  Bytes outside the
+// row and the pad samples of its last byte are masked out: without a sample
+// indicator (every .fam sample phenotyped) from the byte positions; with
+// one (MASKED) from imask, 16 copies of the row's code mask (2 bits per
+// sample, 0b11 when indicatorGenoSamplesWithPheno is set), copy r shifted by
+// r bytes with stride ld, so that copy (row address mod 16) lines up with
+// the aligned chunks.  Lane 0 evaluates the per-marker formulas.
+template <bool MASKED>
 __global__ __launch_bounds__(256) void k_decode_stats(const uint8_t* __restrict__ bed,
-                                                      const uint8_t* __restrict__ ind,
-                                                      const uint8_t* __restrict__ randset,
-                                                      DecodeParams p,
+                                                      const uint8_t* __restrict__ imask, int64_t ld,
+                                                      const uint8_t* __restrict__ randset, DecodeParams p,
                                                       MarkerStat* __restrict__ stats) {
-    __shared__ int s_ac[4], s_miss[4];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    for (int64_t m = blockIdx.x; m < p.M; m += gridDim.x) {
-        const uint8_t* row = bed + m * p.bpm;
-        int ac = 0, miss = 0;
-        for (int64_t b = tid; b < p.bpm; b += 256) {
-            unsigned byte = row[b];
-            const int64_t s0 = b * 4;
+    const int lane = threadIdx.x & 63;
+    const int64_t m = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (m >= p.M) return;
+    const uint8_t* row = bed + m * p.bpm;
+    const int64_t r = (int64_t)((uintptr_t)row & 15u);
+    const uint8_t* A = row - r;
+    const int64_t span = r + p.bpm;                   // bytes from A to the row's end
+    const int64_t nchunk = (span + 15) / 16;
+    const uint8_t* mk = MASKED ? imask + r * ld : nullptr;
+    // codes of the last byte that are samples (the rest is padding)
+    const int last_codes = (int)(p.N - 4 * (p.bpm - 1));
+    const uint32_t last_mask = last_codes >= 4 ? 0xFFu : ((1u << (2 * last_codes)) - 1u);
+    uint32_t miss = 0, nonm = 0, t = 0;
+    auto chunk = [&](int64_t q, const uint4& v) {
+        const int64_t x0 = 16 * q;
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+        uint32_t mw[4];
+        if (MASKED) {
+            const uint4 mv = *reinterpret_cast<const uint4*>(mk + x0);
+            mw[0] = mv.x; mw[1] = mv.y; mw[2] = mv.z; mw[3] = mv.w;
+        } else if (x0 - r >= 0 && x0 - r + 16 <= p.bpm - 1) {
+            mw[0] = mw[1] = mw[2] = mw[3] = 0xFFFFFFFFu;    // interior: every byte a full row byte
+        } else {
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int64_t s = s0 + j;
-                if (s < p.N && ind[s]) {
-                    int g = code_geno((byte >> (2 * j)) & 3u);
-                    if (g == 3) miss++;
-                    else ac += g;
+            for (int d = 0; d < 4; d++) {
+                uint32_t mm = 0;
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    const int64_t pos = x0 + 4 * d + b - r;
+                    const uint32_t bm = (pos < 0 || pos >= p.bpm) ? 0u : (pos == p.bpm - 1 ? last_mask : 0xFFu);
+                    mm |= bm << (8 * b);
                 }
+                mw[d] = mm;
             }
         }
-        ac = wave_sum_i(ac);
-        miss = wave_sum_i(miss);
-        if (lane == 0) { s_ac[wv] = ac; s_miss[wv] = miss; }
-        __syncthreads();
-        if (tid == 0) {
-            int alleleCount = s_ac[0] + s_ac[1] + s_ac[2] + s_ac[3];
-            int numMissing = s_miss[0] + s_miss[1] + s_miss[2] + s_miss[3];
-            const uint64_t nnm = (uint64_t)p.Nnm;
-            // FG.cpp:424 and 442
-            float altFreq = div_rn((float)alleleCount, (float)((nnm - (uint64_t)numMissing) * 2u));
-            float missingRate = div_rn((float)numMissing, (float)nnm);
-            // FG.cpp:447; NaN (all calls missing) maps to 0
-            float twice = __fmul_rn(2.0f, altFreq);
-            int fill = (twice != twice) ? 0 : (int)roundf(twice);
-            if (numMissing > 0) alleleCount = alleleCount + fill * numMissing;   // FG.cpp:454
-            altFreq = div_rn((float)alleleCount, (float)(nnm * 2u));          // FG.cpp:484
-            float one_m = __fsub_rn(1.0f, altFreq);
-            float maf = (one_m < altFreq) ? one_m : altFreq;                     // std::min
-            int twoN = (int)p.Nnm * 2;
-            int mac = (alleleCount < twoN - alleleCount) ? alleleCount : twoN - alleleCount;
-            int passQC = (maf >= p.min_maf && missingRate <= p.max_missing) ? 1 : 0;
-            int passVR = 0;
-            int inr = randset ? (int)randset[m] : 0;
-            if (p.vr_mode == 1) {
-                if ((float)mac >= p.min_mac_vr && (float)mac < p.max_mac_vr) passVR = 1;
-                else if ((float)mac >= p.max_mac_vr) passVR = inr;
-            } else if (p.vr_mode == 2) {
-                if ((float)mac >= p.min_mac_vr) passVR = inr;
-            }
-            if (passVR) passQC = 0;
-            // FG.cpp:914-921
-            float Std = sqrt_rn(__fmul_rn(__fmul_rn(2.0f, altFreq), one_m));
-            float invStd = (Std == 0.0f) ? 0.0f : div_rn(1.0f, Std);
-            MarkerStat st;
-            st.allele_count = alleleCount;
-            st.num_missing = numMissing;
-            st.mac = mac;
-            st.fill = fill;
-            st.freq = altFreq;
-            st.missing_rate = missingRate;
-            st.inv_std = invStd;
-            st.pass_qc = (uint8_t)passQC;
-            st.pass_vr = (uint8_t)passVR;
-            st.pad[0] = st.pad[1] = 0;
-            stats[m] = st;
-        }
-        __syncthreads();
+#pragma unroll
+        for (int d = 0; d < 4; d++) count_word(w[d], mw[d] & 0x55555555u, miss, nonm, t);
+    };
+    int64_t q = lane;
+    for (; q + 64 < nchunk; q += 128) {
+        const uint4 v0 = *reinterpret_cast<const uint4*>(A + 16 * q);
+        const uint4 v1 = *reinterpret_cast<const uint4*>(A + 16 * (q + 64));
+        chunk(q, v0);
+        chunk(q + 64, v1);
+    }
+    if (q < nchunk) chunk(q, *reinterpret_cast<const uint4*>(A + 16 * q));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        miss += __shfl_xor(miss, o, 64);
+        nonm += __shfl_xor(nonm, o, 64);
+        t += __shfl_xor(t, o, 64);
+    }
+    if (lane == 0) {
+        const int alleleCount = (int)(2u * nonm - t);
+        stats[m] = marker_stat(alleleCount, (int)miss, p, randset ? (int)randset[m] : 0);
     }
 }
 
@@ -319,10 +375,13 @@ inline unsigned grid_for(int64_t n, int64_t cap) { return (unsigned)(n < cap ? (
 
 }  // namespace
 
-hipError_t launch_decode_stats(hipStream_t s, const uint8_t* bed, const uint8_t* indicator,
+hipError_t launch_decode_stats(hipStream_t s, const uint8_t* bed, const uint8_t* imask, int64_t ld,
                                const uint8_t* randset, DecodeParams p, MarkerStat* stats) {
-    hipLaunchKernelGGL(k_decode_stats, dim3(grid_for(p.M, 8192)), dim3(256), 0, s, bed, indicator,
-                       randset, p, stats);
+    const dim3 grid((unsigned)((p.M + 3) / 4));
+    if (imask)
+        hipLaunchKernelGGL(k_decode_stats<true>, grid, dim3(256), 0, s, bed, imask, ld, randset, p, stats);
+    else
+        hipLaunchKernelGGL(k_decode_stats<false>, grid, dim3(256), 0, s, bed, imask, ld, randset, p, stats);
     return hipGetLastError();
 }
 

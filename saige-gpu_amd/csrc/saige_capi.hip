@@ -574,17 +574,34 @@ int setgeno_device_impl(saige_ctx* c, const uint8_t* d_bed, int64_t N, int64_t M
     for (int32_t v : c->vr_rand)
         if (v >= 0 && v < M) rset[v] = 1;
 
+    // the decode's sample mask: none when every .fam sample is phenotyped,
+    // else the rows' 2-bit code mask in 16 copies, copy r shifted by r bytes,
+    // so that it lines up with the decode's 16-byte aligned loads of a row
+    // whose address is r mod 16
+    bool all_in = true;
+    for (int64_t i = 0; i < N && all_in; i++) all_in = ind[i] != 0;
+    const int64_t ldm = all_in ? 0 : round_up(bpm + 16, 16);
+    std::vector<uint8_t> imask;
+    if (!all_in) {
+        std::vector<uint8_t> mb(bpm, 0);
+        for (int64_t i = 0; i < N; i++)
+            if (ind[i]) mb[i / 4] |= (uint8_t)(3u << (2 * (i % 4)));
+        imask.assign(16 * ldm, 0);
+        for (int64_t r = 0; r < 16; r++) std::memcpy(imask.data() + r * ldm + r, mb.data(), bpm);
+    }
+
     uint8_t *d_ind = nullptr, *d_rset = nullptr;
     int32_t *d_sub = nullptr, *d_list = nullptr;
     auto cleanup = [&]() { dfree(d_ind); dfree(d_rset); dfree(d_sub); dfree(d_list); };
     int rc = SAIGE_OK;
     do {
-        if (dalloc(&d_ind, N) || dalloc(&d_rset, M) || dalloc(&d_sub, n_sub) || dalloc(&c->d_stats, M)) {
+        if (dalloc(&d_ind, all_in ? 16 : 16 * ldm) || dalloc(&d_rset, M) || dalloc(&d_sub, n_sub) ||
+            dalloc(&c->d_stats, M)) {
             set_error("device allocation failed (setgeno)");
             rc = SAIGE_E_NOMEM;
             break;
         }
-        if (hipMemcpyAsync(d_ind, ind.data(), N, hipMemcpyHostToDevice, st) ||
+        if ((!all_in && hipMemcpyAsync(d_ind, imask.data(), 16 * ldm, hipMemcpyHostToDevice, st)) ||
             hipMemcpyAsync(d_rset, rset.data(), M, hipMemcpyHostToDevice, st) ||
             hipMemcpyAsync(d_sub, subidx, 4 * n_sub, hipMemcpyHostToDevice, st)) {
             set_error("host to device copy failed (setgeno)");
@@ -601,7 +618,7 @@ int setgeno_device_impl(saige_ctx* c, const uint8_t* d_bed, int64_t N, int64_t M
         p.vr_mode = c->is_var_ratio ? (c->max_mac_vr != -1.0f ? 1 : 2) : 0;
         p.min_mac_vr = c->min_mac_vr;
         p.max_mac_vr = c->max_mac_vr;
-        hipError_t e = launch_decode_stats(st, d_bed, d_ind, d_rset, p, c->d_stats);
+        hipError_t e = launch_decode_stats(st, d_bed, all_in ? nullptr : d_ind, ldm, d_rset, p, c->d_stats);
         c->h_stats.resize(M);
         if (e == hipSuccess)
             e = hipMemcpyAsync(c->h_stats.data(), c->d_stats, sizeof(MarkerStat) * M, hipMemcpyDeviceToHost, st);

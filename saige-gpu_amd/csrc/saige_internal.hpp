@@ -223,7 +223,10 @@ __device__ __forceinline__ float div_rn(float a, float b) { return (float)((doub
 __device__ __forceinline__ float sqrt_rn(float a) { return (float)__builtin_sqrt((double)a); }
 
 // ---- genotype kernels (geno_kernels.hip)
-hipError_t launch_decode_stats(hipStream_t s, const uint8_t* bed, const uint8_t* indicator,
+// imask: NULL when every .fam sample is phenotyped, else 16 copies (stride
+// ld, a multiple of 16) of the rows' 2-bit code mask, copy r shifted by r
+// bytes (decode_mask_copies)
+hipError_t launch_decode_stats(hipStream_t s, const uint8_t* bed, const uint8_t* imask, int64_t ld,
                                const uint8_t* randset, DecodeParams p, MarkerStat* stats);
 hipError_t launch_repack(hipStream_t s, const uint8_t* bed, int64_t bpm, const int32_t* markers,
                          int64_t nmark, const MarkerStat* stats, const int32_t* subidx,

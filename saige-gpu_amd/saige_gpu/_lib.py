@@ -19,6 +19,7 @@ SAIGE_E_NOMEM = -3
 SAIGE_E_IO = -4
 SAIGE_E_DEVICE = -5
 SAIGE_E_NOTCONV = -6
+SAIGE_PENDING = 1
 
 GRM_FP32 = 0
 GRM_BF16 = 1
@@ -58,6 +59,10 @@ SIGNATURES = {
     "saige_fit_options_default": (None, [ctypes.POINTER(FitOptions)]),
     "saige_fit_null_batch": (_int, [_c_p, _int, ctypes.POINTER(FitSpec), ctypes.POINTER(FitOptions), _int,
                                     ctypes.POINTER(FitResult)]),
+    "saige_server_start": (_int, [_c_p, ctypes.POINTER(FitOptions), _int, _i64, ctypes.POINTER(_c_p)]),
+    "saige_submit": (_int, [_c_p, ctypes.POINTER(FitSpec), ctypes.POINTER(_i64)]),
+    "saige_poll": (_int, [_c_p, _i64, _int, ctypes.POINTER(FitResult), _pd]),
+    "saige_server_stop": (_int, [_c_p]),
     "saige_covariate_transform": (_int, [_pd, _i64, _int, _pd, _pd, ctypes.POINTER(_i32), ctypes.POINTER(_int)]),
     "saige_score_test_null": (_int, [_pd, _pd, _pd, _pd, _i64, _int, _pd, _pd, _pd, _pd, _pd, _pd]),
     "saige_glm_start": (_int, [_int, _i64, _pd, _pd, _int, _pd, _pd, _pd, _pd]),
@@ -171,8 +176,8 @@ def load():
     return _LIB
 
 
-def check(rc, allow_notconv=False):
-    if rc == SAIGE_OK:
+def check(rc, allow_notconv=False, allow_pending=False):
+    if rc == SAIGE_OK or (allow_pending and rc == SAIGE_PENDING):
         return rc
     msg = load().saige_last_error().decode("utf-8", "replace")
     if rc == SAIGE_E_NOTCONV:

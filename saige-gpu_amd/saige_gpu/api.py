@@ -472,43 +472,20 @@ class Session:
         (maxiter, tol, nrun, tolPCG, maxiterPCG, traceCVcutoff) or None for
         the defaults.  Returns one dict per job (its "status" is 0 on
         success)."""
-        o = _lib.FitOptions()
-        self._L.saige_fit_options_default(ctypes.byref(o))
-        if opt is not None:
-            o.maxiter, o.tol, o.nrun = int(opt.maxiter), float(opt.tol), int(opt.nrun)
-            o.tol_pcg, o.maxiter_pcg, o.trace_cv_cutoff = float(opt.tolPCG), int(opt.maxiterPCG), float(opt.traceCVcutoff)
+        o = _fit_options(self._L, opt)
         n_fits = len(jobs)
         specs = (_lib.FitSpec * n_fits)()
         results = (_lib.FitResult * n_fits)()
         keep, outs = [], []
-        pd = ctypes.POINTER(ctypes.c_double)
         for i, j in enumerate(jobs):
-            y = np.ascontiguousarray(j["y"], np.float64)
-            X = np.asfortranarray(j["X"], np.float64)
-            n, p = X.shape
-            sub = None if j.get("subset") is None else np.ascontiguousarray(j["subset"], np.int64)
-            off = None if j.get("offset") is None else np.ascontiguousarray(j["offset"], np.float64)
-            keep += [y, X, sub, off]
-            specs[i].trait = 0 if j["trait"] == "binary" else 1
-            specs[i].n, specs[i].p = n, p
-            specs[i].subset = None if sub is None else sub.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
-            specs[i].y, specs[i].X = y.ctypes.data_as(pd), X.ctypes.data_as(pd)
-            specs[i].offset = None if off is None else off.ctypes.data_as(pd)
-            o_ = {"coefficients": np.zeros(p), "linear_predictors": np.zeros(n), "fitted_values": np.zeros(n),
-                  "Y": np.zeros(n), "cov": np.zeros((p, p), order="F")}
-            outs.append(o_)
-            results[i].coefficients = o_["coefficients"].ctypes.data_as(pd)
-            results[i].linear_predictors = o_["linear_predictors"].ctypes.data_as(pd)
-            results[i].fitted_values = o_["fitted_values"].ctypes.data_as(pd)
-            results[i].working_y = o_["Y"].ctypes.data_as(pd)
-            results[i].cov = o_["cov"].ctypes.data_as(pd)
+            keep.append(_fit_spec(j, specs[i]))
+            outs.append(_fit_result(j, results[i]))
         check(self._L.saige_fit_null_batch(self.ctx, n_fits, specs, ctypes.byref(o), int(max_concurrent), results))
-        for i, o_ in enumerate(outs):
-            r = results[i]
-            o_.update({"theta": np.array([r.theta[0], r.theta[1]]), "converged": bool(r.converged),
-                       "iterations": int(r.iterations), "status": int(r.status),
-                       "traitType": jobs[i]["trait"]})
-        return outs
+        return [_result_dict(results[i], o_, jobs[i]["trait"]) for i, o_ in enumerate(outs)]
+
+    def server(self, opt=None, max_concurrent=16, hbm_reserve_bytes=1 << 30):
+        """A request server on this context (saige_server_start); see Server."""
+        return Server(self, opt, max_concurrent, hbm_reserve_bytes)
 
     def engine_stats(self, kind=None):
         """Device seconds and launch count of the GRM product launches: for one
@@ -554,6 +531,106 @@ class Session:
         src = ctypes.c_void_p(dev.value + int(offset))
         check(self._L.saige_copy_device_to_host(self.ctx, src, int(nbytes), ptr(out)))
         return out
+
+
+def _fit_options(L, opt):
+    o = _lib.FitOptions()
+    L.saige_fit_options_default(ctypes.byref(o))
+    if opt is not None:
+        o.maxiter, o.tol, o.nrun = int(opt.maxiter), float(opt.tol), int(opt.nrun)
+        o.tol_pcg, o.maxiter_pcg, o.trace_cv_cutoff = float(opt.tolPCG), int(opt.maxiterPCG), float(opt.traceCVcutoff)
+    return o
+
+
+def _fit_spec(job, spec):
+    """Fill a saige_fit_spec from a job dict; returns the arrays it points at."""
+    pd = ctypes.POINTER(ctypes.c_double)
+    y = np.ascontiguousarray(job["y"], np.float64)
+    X = np.asfortranarray(job["X"], np.float64)
+    n, p = X.shape
+    sub = None if job.get("subset") is None else np.ascontiguousarray(job["subset"], np.int64)
+    off = None if job.get("offset") is None else np.ascontiguousarray(job["offset"], np.float64)
+    spec.trait = 0 if job["trait"] == "binary" else 1
+    spec.n, spec.p = n, p
+    spec.subset = None if sub is None else sub.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    spec.y, spec.X = y.ctypes.data_as(pd), X.ctypes.data_as(pd)
+    spec.offset = None if off is None else off.ctypes.data_as(pd)
+    return (y, X, sub, off)
+
+
+def _fit_result(job, res):
+    """Output arrays for one fit, wired into a saige_fit_result."""
+    pd = ctypes.POINTER(ctypes.c_double)
+    n, p = np.shape(job["X"])
+    o = {"coefficients": np.zeros(p), "linear_predictors": np.zeros(n), "fitted_values": np.zeros(n),
+         "Y": np.zeros(n), "cov": np.zeros((p, p), order="F")}
+    res.coefficients = o["coefficients"].ctypes.data_as(pd)
+    res.linear_predictors = o["linear_predictors"].ctypes.data_as(pd)
+    res.fitted_values = o["fitted_values"].ctypes.data_as(pd)
+    res.working_y = o["Y"].ctypes.data_as(pd)
+    res.cov = o["cov"].ctypes.data_as(pd)
+    return o
+
+
+def _result_dict(r, o, trait):
+    o.update({"theta": np.array([r.theta[0], r.theta[1]]), "converged": bool(r.converged),
+              "iterations": int(r.iterations), "status": int(r.status), "traitType": trait})
+    return o
+
+
+class Server:
+    """Null-fit requests served on a session's resident GRM (saige_server_*,
+    include/saige_hip.h): submit() queues a job dict (as fit_null_batch
+    takes) and returns its ticket at once; poll() returns the result dict
+    once the request has finished (with "t_admit" and "t_done", seconds from
+    submission), or None.  The server's scheduler admits requests while an
+    engine system and HBM are free and batches their PCG iterations with
+    those already under way."""
+
+    def __init__(self, sess, opt=None, max_concurrent=16, hbm_reserve_bytes=1 << 30):
+        self._L = sess._L
+        self.sess = sess
+        o = _fit_options(self._L, opt)
+        h = ctypes.c_void_p()
+        check(self._L.saige_server_start(sess.ctx, ctypes.byref(o), int(max_concurrent), int(hbm_reserve_bytes),
+                                         ctypes.byref(h)))
+        self.h = h
+        self._jobs = {}
+
+    def submit(self, job):
+        spec = _lib.FitSpec()
+        keep = _fit_spec(job, spec)
+        t = ctypes.c_int64(0)
+        check(self._L.saige_submit(self.h, ctypes.byref(spec), ctypes.byref(t)))
+        del keep                                  # the server holds its own copy
+        self._jobs[t.value] = job
+        return t.value
+
+    def poll(self, ticket, wait_ms=0):
+        job = self._jobs[ticket]
+        res = _lib.FitResult()
+        o = _fit_result(job, res)
+        times = np.zeros(2)
+        rc = check(self._L.saige_poll(self.h, int(ticket), int(wait_ms), ctypes.byref(res),
+                                      times.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), allow_pending=True)
+        if rc == _lib.SAIGE_PENDING:
+            return None
+        del self._jobs[ticket]
+        out = _result_dict(res, o, job["trait"])
+        out["t_admit"], out["t_done"] = float(times[0]), float(times[1])
+        return out
+
+    def close(self):
+        if self.h is not None:
+            h, self.h = self.h, None
+            check(self._L.saige_server_stop(h))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
 
 
 def glm_start_native(trait, y, X, offset=None):

@@ -1315,8 +1315,9 @@ def loco(sess, jobs, starts, ends, opt=None, batched=True, width=64, n_systems=1
 
 def fit_batched(sess, jobs, opt=None, width=64, n_systems=16, probe_seed=200, lib=True):
     """Many null fits through the batching scheduler.  jobs: list of dicts
-    with keys trait, y, X and optionally subset (sample indices) and tag.
-    lib as in fit_binary_co."""
+    with keys trait, y, X and optionally subset (sample indices), offset
+    (the covariate offset of isCovariateOffset = TRUE, FG.R:1055-1065) and
+    tag.  lib as in fit_binary_co."""
     opt = opt or Options()
     sch = BatchScheduler(sess, opt, width=width, n_systems=n_systems)
     # the glm() start values (host IRLS, no GRM) of all fits
@@ -1334,7 +1335,12 @@ def fit_batched(sess, jobs, opt=None, width=64, n_systems=16, probe_seed=200, li
             conv[id(X)] = (X, np.asfortranarray(X, np.float64) if lib else np.asarray(X, np.float64))
         return conv[id(X)][1]
 
-    args = [(np.ascontiguousarray(j["y"], np.float64), covariates(j["X"]), fams[j["trait"]]) for j in jobs]
+    def offset(j):
+        o = j.get("offset")
+        return None if o is None else np.ascontiguousarray(o, np.float64)
+
+    args = [(np.ascontiguousarray(j["y"], np.float64), covariates(j["X"]), fams[j["trait"]], offset(j))
+            for j in jobs]
     # one after another on this thread: on the GPU box's host, four library
     # starts on worker threads reached the first GPU step 3 ms later
     # (21.8 ms against 18.7 ms, medians of four rounds)
@@ -1343,7 +1349,7 @@ def fit_batched(sess, jobs, opt=None, width=64, n_systems=16, probe_seed=200, li
     for j, a, fit0 in zip(jobs, args, fit0s):
         n = len(j["y"])
         probes = RademacherProbes.shared(n, probe_seed)
-        gen = fit_co(j["trait"], a[0], a[1], probes, opt, fit0=fit0, lib=lib)
+        gen = fit_co(j["trait"], a[0], a[1], probes, opt, offset=a[3], fit0=fit0, lib=lib)
         handles.append(sch.submit(gen, j.get("subset"), j.get("tag")))
     try:
         sch.run()

@@ -42,3 +42,18 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
         assert "not built" in str(e)
     else:
         raise AssertionError("load() must raise when the library is missing")
+
+
+def test_serving_entry_points_reject_null_handles():
+    """saige_server_start / saige_submit / saige_poll / saige_server_stop
+    refuse null handles with SAIGE_E_ARG before touching a device."""
+    from saige_gpu import _lib
+    L = _lib.load()
+    h = ctypes.c_void_p()
+    assert L.saige_server_start(None, None, 4, 0, ctypes.byref(h)) == _lib.SAIGE_E_ARG
+    spec = _lib.FitSpec()
+    t = ctypes.c_int64(0)
+    assert L.saige_submit(None, ctypes.byref(spec), ctypes.byref(t)) == _lib.SAIGE_E_ARG
+    res = _lib.FitResult()
+    assert L.saige_poll(None, 1, 0, ctypes.byref(res), None) == _lib.SAIGE_E_ARG
+    assert L.saige_server_stop(None) == _lib.SAIGE_E_ARG

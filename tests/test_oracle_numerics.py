@@ -46,6 +46,13 @@ def test_diag_sequential_fp32(small):
     assert np.allclose(O.diag_std(g), np.diag(S), rtol=1e-5)
 
 
+def test_grm_f64_blas_equals_grm_f64(small):
+    g = small
+    S, Sb = O.grm_f64(g), O.grm_f64_blas(g)
+    assert np.allclose(Sb, S, rtol=0, atol=1e-12 * np.max(np.abs(S)))
+    assert np.allclose(Sb, Sb.T)
+
+
 def test_crossprod_against_dense(small):
     g = small
     rng = np.random.default_rng(1)
