@@ -211,7 +211,8 @@ int saige_pcg1_loco(saige_ctx* ctx, const float* w_host, const float* tau, const
  * leaves between iterations (continuous batching).  The reference has no
  * counterpart: it solves one right-hand side at a time (FG.cpp:2593). */
 
-/* (Re)allocate for `width` slots and `n_systems` systems. */
+/* Allocate the engine for `n_systems` systems; it always holds 64 slots
+ * (width, 1..64, is checked and otherwise unused). */
 int saige_engine_reserve(saige_ctx* ctx, int width, int n_systems);
 /* Set system s: Sigma_s = tau0 diag(1/w) + tau1 K restricted to mask (N
  * flags; NULL = all samples).  Samples outside the mask are held at zero. */
@@ -234,7 +235,8 @@ int saige_engine_add_products(saige_ctx* ctx, int k, const int32_t* slots, const
  * discard) and free the slots. */
 int saige_engine_take_batch(saige_ctx* ctx, int k, const int32_t* slots, float* X_host);
 /* Run `n_iter` iterations on every live slot.  done[slot] = 1 when the slot
- * has stopped; iters[slot] = its iteration count (either may be NULL). */
+ * has stopped; iters[slot] = its iteration count (either may be NULL; each
+ * holds 64 entries). */
 int saige_engine_step(saige_ctx* ctx, int n_iter, uint8_t* done_host, int32_t* iters_host);
 /* Copy the solution of a slot to host and free the slot. */
 int saige_engine_take(saige_ctx* ctx, int slot, float* x_host);

@@ -452,6 +452,11 @@ void run_scheduler(Scheduler& sch, int n_systems) {
                 free_sys.push_back(f->sys);
                 f->sys = -1;
                 active.erase(active.begin() + (ptrdiff_t)i);
+                // a released fit may be freed at once by saige_poll: nothing
+                // here may point at it any more
+                resumed.erase(std::remove_if(resumed.begin(), resumed.end(),
+                                             [f](const std::pair<FitTask*, int64_t>& fp) { return fp.first == f; }),
+                              resumed.end());
                 f->released = true;
                 sch.done_cv.notify_all();
             }

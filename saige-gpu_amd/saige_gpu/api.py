@@ -394,11 +394,15 @@ class Session:
         check(self._L.saige_engine_take_batch(self.ctx, len(sl), ptr(sl), ptr(X)))
         return X
 
+    ENGINE_WIDTH = 64      # the block engine's slots (saige_engine_reserve always makes 64)
+
     def engine_step(self, n_iter, width=64):
-        done = np.zeros(width, np.uint8)
-        iters = np.zeros(width, np.int32)
+        """One or more engine iterations; returns the done flags and iteration
+        counts of the first `width` slots.  The library writes all 64."""
+        done = np.zeros(self.ENGINE_WIDTH, np.uint8)
+        iters = np.zeros(self.ENGINE_WIDTH, np.int32)
         check(self._L.saige_engine_step(self.ctx, int(n_iter), ptr(done), ptr(iters)))
-        return done, iters
+        return done[:width], iters[:width]
 
     # ------------------------------------------ device-resident columns
     # Device buffers are column-major with leading dimension N (Nnomissing);

@@ -111,18 +111,18 @@ def causal_std_device(sess, dev, N, M, n_causal, seed):
 
 
 def engine_solve(sess, w, tau, B, mask=None, maxiter=500, tol=1e-5):
-    """Sigma^-1 B through the slot engine on the samples of mask (K_SS of
-    the resident GRM; w and B are full-length, zero outside the mask): one
-    column at width 1 (the GEMV), more at width 64 (the block GEMM).
-    Returns (X, iterations)."""
+    """Sigma^-1 B through the block engine (64 slots, the session's block
+    GEMM) on the samples of mask (K_SS of the resident GRM; w and B are
+    full-length, zero outside the mask), k <= 64 columns.  Returns
+    (X, iterations)."""
     B = np.asarray(B, np.float64).reshape(len(w), -1)
     k = B.shape[1]
-    width = 1 if k == 1 else 64
-    sess.engine_reserve(width, 1)
+    assert k <= 64
+    sess.engine_reserve(64, 1)
     sess.engine_set_system(0, w, tau, mask)
     sess.engine_add_batch(list(range(k)), [0] * k, B, maxiter, tol)
     while True:
-        done, iters = sess.engine_step(1, width)
+        done, iters = sess.engine_step(1)
         if np.all(done[:k]):
             break
     return sess.engine_take_batch(list(range(k))).astype(np.float64), np.asarray(iters[:k])

@@ -9,9 +9,10 @@ so the checks are:
   4 sqrt(Mqc) 2^-24 max|K64|);
 * a full-N plain solve: ||Sigma x - b||^2 <= 1.5 tolPCG in FP64 on the
   GPU's own GRM;
-* the subblock's solve (the engine at width 1 with the subblock's sample
-  mask, i.e. the GEMV on K_SS) against a dense FP64 solve and against the
-  oracle's FP32 PCG (FG.cpp:2593-2809 restated) on the FP64 subblock GRM;
+* the subblock's solve (the block engine in FP32-MFMA mode with the
+  subblock's sample mask, i.e. K_SS of the resident GRM) against a dense
+  FP64 solve and against the oracle's FP32 PCG (FG.cpp:2593-2809 restated)
+  on the FP64 subblock GRM;
 * a quantitative (h2 = 0.4) null fit at full N through the plain path, the
   batched scheduler and the C++ drivers: the same iterations and tau within
   1e-3 of each other (GEMV in FP32 against the block GEMM's FP32-accurate
@@ -81,7 +82,8 @@ def test_c1_plain_pcg(c1):
     r2 = fp64_residual2(s, w, tau, x, b)[0]
     print(f"  C1 plain PCG at full N: ||Sigma x - b||^2 = {r2:.3e} (tol 1e-5)", flush=True)
     assert r2 <= 1.5e-5
-    # the subblock: the GEMV on K_SS through the sample mask
+    # the subblock: K_SS through the sample mask, FP32 MFMA products
+    assert s.gemm_mode() == "fp32"
     mask = np.zeros(N, np.uint8)
     mask[:N0] = 1
     bf = np.zeros(N)
