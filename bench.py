@@ -184,10 +184,22 @@ def subblock_genotypes(sess, dev, N, M, n0, chunk=8192):
     return G, miss
 
 
-def fp64_check_leg(sess, G, miss, Xraw, opt, seed):
+def _offdiag_error(K, K64):
+    import numpy as np
+    D = K - K64
+    np.fill_diagonal(D, 0.0)
+    R = K64.copy()
+    np.fill_diagonal(R, 0.0)
+    return float(np.linalg.norm(D) / np.linalg.norm(R))
+
+
+def fp64_check_leg(sess, G, miss, Xraw, opt, seed, K_single=None):
     """Error of the device path against FP64 on the principal subblock of
     the first n0 samples: the GRM (FP64 from the samples' genotypes over all
-    GRM markers, imputed as FG.cpp:447-454, frequencies of the whole cohort),
+    GRM markers, imputed as FG.cpp:447-454, frequencies of the whole cohort;
+    the resident GRM, and K_single -- the same subblock of a single-pass BF16
+    build -- when given; max error over max|K64| and the off-diagonal
+    relative Frobenius error, the relatedness signal tau1 rests on),
     a 64-column block solve on the subblock (engine with the sample mask)
     against a dense FP64 solve, and tau of a binary and a quantitative null
     fit on the subblock's samples (the batched device path on the resident
@@ -210,6 +222,12 @@ def fp64_check_leg(sess, G, miss, Xraw, opt, seed):
     K64 = S / len(cols)
     K = sess.grm_block(0, 0, n0, n0).astype(np.float64)
     grm_err = float(np.max(np.abs(K - K64)) / np.max(np.abs(K64)))
+    grm_off = _offdiag_error(K, K64)
+    single = None
+    if K_single is not None:
+        Ks = K_single.astype(np.float64)
+        single = {"grm_max_rel_error": float(np.max(np.abs(Ks - K64)) / np.max(np.abs(K64))),
+                  "grm_offdiag_rel_error": _offdiag_error(Ks, K64)}
     # a 64-column block solve on the subblock
     rng = np.random.default_rng(seed)
     w = np.ones(N)
@@ -247,7 +265,8 @@ def fp64_check_leg(sess, G, miss, Xraw, opt, seed):
                             "rel_error": [float(abs(a - b) / abs(b)) if b != 0 else float(abs(a - b))
                                           for a, b in zip(h.result["theta"], ref["theta"])],
                             "iterations": [int(h.result["iterations"]), int(ref["iterations"])]}
-    return {"samples": n0, "grm_max_rel_error": grm_err,
+    return {"samples": n0, "grm": sess.grm_precision(), "grm_max_rel_error": grm_err,
+            "grm_offdiag_rel_error": grm_off, "single_pass_bf16": single,
             "block_solve_rel_error": {"max": float(sol_err.max()), "median": float(np.median(sol_err))},
             "tau": fits,
             "reference": "FP64 GRM of the subblock from its genotypes over all GRM markers; exact (Cholesky) "
@@ -402,7 +421,8 @@ def c4_leg(args):
     t2 = time.perf_counter()
     g1, n1 = sess.engine_stats("gemm_full")
     gemm_avg = (g1 - g0) / max(1, n1 - n0)
-    out = {"config": "C4", "N": N, "M": M, "Mqc": Mqc, "grm_seconds": grm_s, "grm_tflops": grm_tflops,
+    out = {"config": "C4", "N": N, "M": M, "Mqc": Mqc, "grm": args.grm, "grm_seconds": grm_s,
+           "grm_tflops": grm_tflops if args.grm != "bf16_split" else None,
            "grm_GB": 4.0 * (N + 255) // 256 * 256 * ((N + 255) // 256 * 256) / 1e9,
            "fit_seconds": t1 - t0, "theta": list(map(float, fit["theta"])), "vr_seconds": t2 - t1,
            "variance_ratios": [row[0] for row in vr["table"]], "vr_markers": [len(r) for r in vr["ratios"]],
@@ -421,7 +441,8 @@ def main():
     ap.add_argument("--M", type=int, default=150_000)
     ap.add_argument("--nrun", type=int, default=15,
                     help="probes per trace round: 15 makes the C2 block of 60 probes + 4 phenotypes")
-    ap.add_argument("--grm", default="bf16", choices=["bf16", "fp32"])
+    ap.add_argument("--grm", default="bf16_split", choices=["bf16_split", "bf16", "fp32"],
+                    help="GRM the fits use (bf16_split: FP32-accurate from three BF16 MFMA passes)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-nrun30", action="store_true")
@@ -472,9 +493,21 @@ def main():
     G0 = miss0 = None
     if not args.no_fp64_check:
         G0, miss0 = subblock_genotypes(sess, dev, N, M, n0)
-    grm_s = sess.build_grm(args.grm)
+    # GRM TFLOP/s is the single-pass build (N(N+1)Mqc flop of the SYRK's
+    # lower triangle); the split build the fits use runs three such passes
+    grm_s = sess.build_grm("bf16" if args.grm == "bf16_split" else args.grm)
     grm_tflops = N * (N + 1) * Mqc / grm_s / 1e12
-    log(f"GRM {args.grm}: {grm_s:.3f}s device time, {grm_tflops:.1f} TFLOP/s")
+    log(f"GRM {sess.grm_precision()}: {grm_s:.3f}s device time, {grm_tflops:.1f} TFLOP/s")
+    K_single, split = None, None
+    if args.grm == "bf16_split":
+        if G0 is not None:
+            K_single = sess.grm_block(0, 0, n0, n0)
+        split_s = sess.build_grm("bf16_split")
+        split = {"seconds": split_s, "fp32_equivalent_tflops": N * (N + 1) * Mqc / split_s / 1e12,
+                 "bf16_piece_tflops": 3 * N * (N + 1) * Mqc / split_s / 1e12,
+                 "note": "each standardised value as two BF16 pieces; hi.hi + hi.lo + lo.hi on the MFMA"}
+        log(f"GRM bf16_split: {split_s:.3f}s device time ({split['bf16_piece_tflops']:.0f} TFLOP/s of BF16 "
+            f"piece products)")
     sess.set_gemm_mode(args.gemm_mode)
     tune = sess.autotune(reps=3)
     log(f"autotune block GEMM ({args.gemm_mode}): " + ", ".join(f"rt{a}/pd{b}/s{c} {t:.2f}ms" for a, b, c, t in tune["table"]) +
@@ -611,18 +644,30 @@ def main():
         log(f"nrun 30: batched {nrun30['batched_fits_per_hour']:.0f}, C++ {nrun30['native_fits_per_hour']:.0f}, "
             f"plain {nrun30['plain_fits_per_hour']:.0f} fits/h")
 
-    serving = None
+    serving = saturation = None
     if args.serve_requests > 0:
+        # saturation: a burst at four times the batch path's capacity gives the
+        # server's own throughput on this request mix; the latency leg then
+        # offers the C3 stream of requests at 0.8 of it, so p50/p99 measure
+        # service and modest queueing rather than a backlog that grows with
+        # the length of the run
         capacity = len(jobs) / nat_s
-        serving = serve_leg(sess, N, kept["Xraw"], kept["Zc"], args.serve_requests, 1.2 * capacity, seed + 7, opt)
+        n_sat = max(1, args.serve_requests * 3 // 10)
+        saturation = serve_leg(sess, N, kept["Xraw"], kept["Zc"], n_sat, 4.0 * capacity, seed + 6, opt)
+        log(f"serving saturation: {saturation['completed']}/{saturation['requests']} fits offered at "
+            f"{saturation['offered_rate_per_s']:.2f}/s: {saturation['fits_per_hour']:.0f} fits/h")
+        rate = 0.8 * saturation["fits_per_hour"] / 3600.0
+        serving = serve_leg(sess, N, kept["Xraw"], kept["Zc"], args.serve_requests, rate, seed + 7, opt)
+        serving["load_vs_saturation"] = 0.8
         log(f"serving: {serving['completed']}/{serving['requests']} fits at offered {serving['offered_rate_per_s']:.2f}/s: "
             f"{serving['fits_per_hour']:.0f} fits/h, p50 {serving['p50_latency_s']:.2f}s, "
             f"p99 {serving['p99_latency_s']:.2f}s")
 
     fp64 = None
     if G0 is not None:
-        fp64 = fp64_check_leg(sess, G0, miss0, kept["Xraw"], D.Options(nrun=30), seed + 11)
-        log(f"FP64 subblock check ({n0} samples): GRM {fp64['grm_max_rel_error']:.2e}, block solve "
+        fp64 = fp64_check_leg(sess, G0, miss0, kept["Xraw"], D.Options(nrun=30), seed + 11, K_single)
+        log(f"FP64 subblock check ({n0} samples): GRM {fp64['grm_max_rel_error']:.2e} (off-diagonal "
+            f"{fp64['grm_offdiag_rel_error']:.2e}), block solve "
             f"{fp64['block_solve_rel_error']['max']:.2e}, tau rel error " +
             ", ".join(f"{k} {v['rel_error']}" for k, v in fp64["tau"].items()))
         del G0, miss0
@@ -730,11 +775,13 @@ def main():
         "dtype": "f32",
         "data": "synthetic (device-generated PLINK .bed, simulated phenotypes)",
         "config": {"workload": f"batched binary null fits, N={N}, M={M}, 4 phenotypes x (nrun {args.nrun} + 1) "
-                               f"= 64-RHS block PCG, {args.grm.upper()} GRM (FP32 storage)",
+                               f"= 64-RHS block PCG, {args.grm} GRM (FP32 storage)",
                    "N": N, "M": M, "Mqc": Mqc, "phenotypes": len(jobs), "nrun": args.nrun, "grm": args.grm,
                    "gemm_mode": args.gemm_mode},
         "grm_tflops": grm_tflops,
         "grm_seconds": grm_s,
+        "grm_tflops_pass": "single BF16 pass" if args.grm != "fp32" else "FP32",
+        "grm_split": split,
         "setgeno_seconds": setgeno_s,
         "gemm_autotune": tune,
         "native_cpp_driver": native,
@@ -746,6 +793,7 @@ def main():
         "p50_fit_latency_s": serving["p50_latency_s"] if serving else float(np.percentile(lat, 50)),
         "p99_fit_latency_s": serving["p99_latency_s"] if serving else float(np.percentile(lat, 99)),
         "serving": serving,
+        "serving_saturation": saturation,
         "fp64_subblock_check": fp64,
         "loco": loco,
         "roofline": roofline,

@@ -44,6 +44,11 @@ extern "C" {
 #define SAIGE_GRM_BF16 1        /* 2-bit genotypes standardised to BF16 inside the SYRK,
                                    BF16 MFMA with FP32 accumulation                    */
 #define SAIGE_GRM_BF16_PANEL 2  /* same arithmetic from materialised BF16 Z panels     */
+#define SAIGE_GRM_BF16_SPLIT 3  /* 2-bit genotypes standardised to two BF16 pieces each,
+                                   z = hi + lo (lo the rounded remainder, z - hi - lo
+                                   within 2^-18 |z|); hi.hi + hi.lo + lo.hi on the BF16
+                                   MFMA with FP32 accumulation, three SYRK passes: an
+                                   FP32-accurate S at three times the BF16 build      */
 
 typedef struct saige_ctx saige_ctx;
 
@@ -143,6 +148,8 @@ int saige_get_grm_block(saige_ctx* ctx, int64_t r0, int64_t c0, int64_t nr, int6
                         float* out_host);
 /* Seconds of device time of the last saige_build_grm (HIP events). */
 double saige_last_grm_seconds(const saige_ctx* ctx);
+/* The SAIGE_GRM_* precision the resident GRM was built with, -1 if none. */
+int saige_grm_precision(const saige_ctx* ctx);
 
 /* getCrossprodMatAndKin / parallelCrossProd (FG.cpp:1953-1981, 1669-1708):
  * y = K b.  Replaces gpuSymMatMult::sym_sgemv (GPU.hpp:30, GPU.cu:211-252). */

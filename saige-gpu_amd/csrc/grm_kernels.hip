@@ -322,8 +322,12 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 // never alias and can interleave them; the loop body is one basic block (the
 // last step decodes a clamped copy of the last chunk that nothing reads, and
 // diagonal tiles, both panels the same, decode their B copy too).
+// TWO: the B panel is decoded through its own table lut64b (the split
+// GRM's cross terms hi x lo and lo x hi); otherwise both use lut64.
+template <bool TWO>
 __global__ __launch_bounds__(512) void k_syrk_pk256(const uint8_t* __restrict__ tp, int64_t nchunk, int64_t c0,
                                                     int64_t c1, const uint64_t* __restrict__ lut64,
+                                                    const uint64_t* __restrict__ lut64b,
                                                     const int2* __restrict__ tiles, int64_t nblk,
                                                     float* __restrict__ S, int64_t ldS, int accumulate) {
     __shared__ __attribute__((aligned(16))) __hip_bfloat16 sA0[BT2 * P16];
@@ -341,6 +345,7 @@ __global__ __launch_bounds__(512) void k_syrk_pk256(const uint8_t* __restrict__ 
     const uint64_t* gA = reinterpret_cast<const uint64_t*>(tp + (pa * nchunk * 32 + (b & 31)) * 64 + 8 * o);
     const uint64_t* gB = reinterpret_cast<const uint64_t*>(tp + (pb * nchunk * 32 + (b & 31)) * 64 + 8 * o);
     const uint64_t* gl = lut64 + 8 * o;
+    const uint64_t* glb = (TWO ? lut64b : lut64) + 8 * o;
 
     f32x16 acc[2][4];
 #pragma unroll
@@ -350,22 +355,44 @@ __global__ __launch_bounds__(512) void k_syrk_pk256(const uint8_t* __restrict__ 
 #pragma unroll
             for (int r = 0; r < 16; r++) acc[mi][ni][r] = 0.0f;
 
-    uint64_t ra, rb, la[8];
+    uint64_t ra, rb, la[8], lb[TWO ? 8 : 1];
     auto fetch = [&](int64_t kc) {
         ra = gA[kc * 256];
         rb = gB[kc * 256];
 #pragma unroll
         for (int e = 0; e < 8; e++) la[e] = gl[kc * 64 + e];
+        if (TWO) {
+#pragma unroll
+            for (int e = 0; e < 8; e++) lb[e] = glb[kc * 64 + e];
+        }
     };
-    // decode the chunk held in ra/rb/la into (dA, dB), fetching chunk kn
+    // a table's low and high bytes of its BF16 entries
+    auto split_bytes = [&](const uint64_t (&t)[8], uint32_t (&lo)[8], uint32_t (&hi)[8]) {
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const uint32_t l = (uint32_t)t[e], h = (uint32_t)(t[e] >> 32);
+            lo[e] = __builtin_amdgcn_perm(h, l, 0x06040200u);
+            hi[e] = __builtin_amdgcn_perm(h, l, 0x07050301u);
+        }
+    };
+    // decode the chunk held in ra/rb/la(/lb) into (dA, dB), fetching chunk kn
     auto decode_next = [&](__hip_bfloat16* dA, __hip_bfloat16* dB, int64_t kn) {
         const uint64_t ca = ra, cb = rb;
         uint32_t lo[8], hi[8];
+        split_bytes(la, lo, hi);
+        if (TWO) {
+            uint32_t lo2[8], hi2[8];
+            uint64_t tb[8];
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const uint32_t l = (uint32_t)la[e], h = (uint32_t)(la[e] >> 32);
-            lo[e] = __builtin_amdgcn_perm(h, l, 0x06040200u);
-            hi[e] = __builtin_amdgcn_perm(h, l, 0x07050301u);
+            for (int e = 0; e < 8; e++) tb[e] = lb[e];
+            split_bytes(tb, lo2, hi2);
+            fetch(kn);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                *reinterpret_cast<u32x4*>(dA + (4 * b + j) * P16 + 8 * o) = decode_row_perm(lo, hi, ca, j);
+                *reinterpret_cast<u32x4*>(dB + (4 * b + j) * P16 + 8 * o) = decode_row_perm(lo2, hi2, cb, j);
+            }
+            return;
         }
         fetch(kn);
 #pragma unroll
@@ -439,20 +466,41 @@ __global__ __launch_bounds__(256) void k_pack_tiles(const uint8_t* __restrict__ 
     }
 }
 
-// lut64[j] for j < nchunk*64 (zero for padding markers)
-__global__ void k_lut64(const float4* __restrict__ lut, int64_t j0, int64_t j1, int64_t n,
+// lut64[j] for j < nchunk*64 (zero for padding markers).  piece 0: each
+// value rounded to BF16 (hi); piece 1: the remainder x - hi, exact in FP32,
+// rounded to BF16 (lo), so that x - hi - lo is within 2^-18 |x|.
+__device__ __forceinline__ uint64_t bf16_piece(float x, int piece) {
+    const __hip_bfloat16 hi = __float2bfloat16(x);
+    if (piece == 0) return __bfloat16_as_ushort(hi);
+    return __bfloat16_as_ushort(__float2bfloat16(__fsub_rn(x, __bfloat162float(hi))));
+}
+
+__global__ void k_lut64(const float4* __restrict__ lut, int64_t j0, int64_t j1, int64_t n, int piece,
                         uint64_t* __restrict__ out) {
     const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (j >= n) return;
     uint64_t v = 0;
     if (j >= j0 && j < j1) {
         const float4 L = lut[j];
-        const uint64_t l0 = __bfloat16_as_ushort(__float2bfloat16(L.x));
-        const uint64_t l1 = __bfloat16_as_ushort(__float2bfloat16(L.y));
-        const uint64_t l2 = __bfloat16_as_ushort(__float2bfloat16(L.z));
+        const uint64_t l0 = bf16_piece(L.x, piece);
+        const uint64_t l1 = bf16_piece(L.y, piece);
+        const uint64_t l2 = bf16_piece(L.z, piece);
         v = l2 | (0ull << 16) | (l1 << 32) | (l0 << 48);   // codes 00, 01, 10, 11
     }
     out[j] = v;
+}
+
+// Copy each diagonal tile's lower half onto its upper half: S[c][r] =
+// S[r][c] for r > c inside tile t (rows and columns t*T + [0, T)).  The split
+// GRM's cross passes add hi.lo and lo.hi to the two mirror entries of a
+// diagonal tile in opposite orders; this keeps S exactly symmetric, with the
+// lower triangle's values, as off-diagonal tiles are mirrored.  Block (t, r):
+// row r of tile t; its threads write column r of the upper part.
+__global__ __launch_bounds__(256) void k_mirror_diag(float* __restrict__ S, int64_t ld, int T) {
+    const int64_t t = blockIdx.x, r = blockIdx.y;
+    const int64_t base = t * T;
+    for (int64_t c = threadIdx.x; c < r; c += blockDim.x)
+        S[(base + c) * ld + base + r] = S[(base + r) * ld + base + c];
 }
 
 // ------------------------------------------------------------------ FP32
@@ -547,33 +595,44 @@ hipError_t launch_pack_tiles(hipStream_t s, const uint8_t* packed, const float4*
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_lut64, dim3((unsigned)((nchunk * 64 + 255) / 256)), dim3(256), 0, s, lut, (int64_t)0, Mqc,
-                       nchunk * 64, lut64);
+                       nchunk * 64, 0, lut64);
     return hipGetLastError();
 }
 
-hipError_t launch_lut64_range(hipStream_t s, const float4* lut, int64_t Mqc, int64_t j0, int64_t j1,
+hipError_t launch_lut64_range(hipStream_t s, const float4* lut, int64_t Mqc, int64_t j0, int64_t j1, int piece,
                               uint64_t* lut64) {
     const int64_t nchunk = (Mqc + 63) / 64;
     hipLaunchKernelGGL(k_lut64, dim3((unsigned)((nchunk * 64 + 255) / 256)), dim3(256), 0, s, lut, j0, j1,
-                       nchunk * 64, lut64);
+                       nchunk * 64, piece, lut64);
     return hipGetLastError();
 }
 
-hipError_t launch_syrk_pk(hipStream_t s, const uint8_t* tp, const uint64_t* lut64, int64_t Mqc, int64_t Npad,
-                          const int2* tiles, float* S, int accumulate, int64_t j0, int64_t j1, int tile) {
+hipError_t launch_syrk_pk(hipStream_t s, const uint8_t* tp, const uint64_t* lut64, const uint64_t* lut64b,
+                          int64_t Mqc, int64_t Npad, const int2* tiles, float* S, int accumulate, int64_t j0,
+                          int64_t j1, int tile) {
     if ((tile != BT && tile != BT2) || Npad % tile != 0) return hipErrorInvalidValue;
+    if (lut64b && lut64b != lut64 && tile != BT2) return hipErrorInvalidValue;   // two tables: 256 tiles only
     const int64_t nt = Npad / tile;
     const int64_t nblk = nt * (nt + 1) / 2;
     const int64_t nchunk = (Mqc + 63) / 64;
     if (j1 < 0) j1 = Mqc;
     const int64_t c0 = j0 / 64, c1 = (j1 + 63) / 64;
     if (c0 >= c1) return hipSuccess;
-    if (tile == BT2)
-        hipLaunchKernelGGL(k_syrk_pk256, dim3((unsigned)nblk), dim3(512), 0, s, tp, nchunk, c0, c1, lut64, tiles,
-                           nblk, S, Npad, accumulate);
+    if (tile == BT2 && lut64b && lut64b != lut64)
+        hipLaunchKernelGGL(k_syrk_pk256<true>, dim3((unsigned)nblk), dim3(512), 0, s, tp, nchunk, c0, c1, lut64,
+                           lut64b, tiles, nblk, S, Npad, accumulate);
+    else if (tile == BT2)
+        hipLaunchKernelGGL(k_syrk_pk256<false>, dim3((unsigned)nblk), dim3(512), 0, s, tp, nchunk, c0, c1, lut64,
+                           lut64, tiles, nblk, S, Npad, accumulate);
     else
         hipLaunchKernelGGL(k_syrk_pk, dim3((unsigned)nblk), dim3(256), 0, s, tp, nchunk, c0, c1, lut64, tiles,
                            nblk, S, Npad, accumulate);
+    return hipGetLastError();
+}
+
+hipError_t launch_mirror_diag(hipStream_t s, float* S, int64_t Npad, int tile) {
+    if (tile <= 0 || Npad % tile != 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_mirror_diag, dim3((unsigned)(Npad / tile), (unsigned)tile), dim3(256), 0, s, S, Npad, tile);
     return hipGetLastError();
 }
 

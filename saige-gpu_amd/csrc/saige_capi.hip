@@ -958,10 +958,29 @@ int saige_get_diag_of_kin(saige_ctx* c, float* out) {
     return SAIGE_OK;
 }
 
+// The 2-bit-input SYRK over markers [j0, j1) into S: one pass with the BF16
+// table (hi), or, for the split GRM, hi x hi, hi x lo and lo x hi, the last
+// two accumulated onto the first (each pass rounds its tile sums into S).
+// accumulate: 0 store, +1 add, -1 subtract.
+static hipError_t syrk_pk_passes(hipStream_t st, const uint8_t* tp, const uint64_t* lhi, const uint64_t* llo,
+                                 int64_t Mqc, int64_t Np, const int2* tiles, float* S, int accumulate, int64_t j0,
+                                 int64_t j1, int tile) {
+    hipError_t e = launch_syrk_pk(st, tp, lhi, nullptr, Mqc, Np, tiles, S, accumulate, j0, j1, tile);
+    if (e != hipSuccess || !llo) return e;
+    const int acc2 = accumulate == 0 ? 1 : accumulate;
+    e = launch_syrk_pk(st, tp, lhi, llo, Mqc, Np, tiles, S, acc2, j0, j1, tile);
+    if (e == hipSuccess) e = launch_syrk_pk(st, tp, llo, lhi, Mqc, Np, tiles, S, acc2, j0, j1, tile);
+    // the cross passes reach a diagonal tile's mirror entries in opposite
+    // orders: keep S exactly symmetric
+    if (e == hipSuccess) e = launch_mirror_diag(st, S, Np, tile);
+    return e;
+}
+
 int saige_build_grm(saige_ctx* c, int precision) {
     CHECK_CTX(c);
     CHECK_LOADED(c);
-    if (precision != SAIGE_GRM_FP32 && precision != SAIGE_GRM_BF16 && precision != SAIGE_GRM_BF16_PANEL) {
+    if (precision != SAIGE_GRM_FP32 && precision != SAIGE_GRM_BF16 && precision != SAIGE_GRM_BF16_PANEL &&
+        precision != SAIGE_GRM_BF16_SPLIT) {
         set_error("unknown GRM precision %d", precision);
         return SAIGE_E_ARG;
     }
@@ -982,17 +1001,18 @@ int saige_build_grm(saige_ctx* c, int precision) {
     c->grm_ready = false;
     c->loco_ready = false;   // S_loco derives from S
     c->loco_inplace = false;
-    if (precision == SAIGE_GRM_BF16) {
+    if (precision == SAIGE_GRM_BF16 || precision == SAIGE_GRM_BF16_SPLIT) {
         // 2-bit operands straight into the MFMA SYRK: no Z panel
+        const bool split = precision == SAIGE_GRM_BF16_SPLIT;
         const int64_t nchunk = (c->Mqc + 63) / 64;
         uint8_t* d_tp = nullptr;
-        uint64_t* d_l64 = nullptr;
+        uint64_t *d_l64 = nullptr, *d_llo = nullptr;
         int2* d_tiles = nullptr;
-        const int tile = syrk_pk_tile();
+        const int tile = split ? 256 : syrk_pk_tile();   // two tables: the 256 tile only
         const std::vector<int2> order = syrk_tile_order(Np / tile, tile == 256 ? 4 : 8);
-        auto release = [&]() { dfree(d_tp); dfree(d_l64); dfree(d_tiles); };
+        auto release = [&]() { dfree(d_tp); dfree(d_l64); dfree(d_llo); dfree(d_tiles); };
         if (dalloc(&d_tp, (Np / syrk_tile()) * nchunk * 2048) || dalloc(&d_l64, nchunk * 64) ||
-            dalloc(&d_tiles, (int64_t)order.size())) {
+            (split && dalloc(&d_llo, nchunk * 64)) || dalloc(&d_tiles, (int64_t)order.size())) {
             release();
             set_error("cannot allocate the tiled genotypes for the SYRK");
             return SAIGE_E_NOMEM;
@@ -1000,7 +1020,9 @@ int saige_build_grm(saige_ctx* c, int precision) {
         hipError_t e = hipMemcpyAsync(d_tiles, order.data(), sizeof(int2) * order.size(), hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = hipEventRecord(c->ev0, st);
         if (e == hipSuccess) e = launch_pack_tiles(st, c->d_packed, c->d_lut, c->Mqc, c->Nnm, Np, c->bpm_nm, d_tp, d_l64);
-        if (e == hipSuccess) e = launch_syrk_pk(st, d_tp, d_l64, c->Mqc, Np, d_tiles, c->d_grm, 0, 0, -1, tile);
+        if (e == hipSuccess && split) e = launch_lut64_range(st, c->d_lut, c->Mqc, 0, c->Mqc, 1, d_llo);
+        if (e == hipSuccess)
+            e = syrk_pk_passes(st, d_tp, d_l64, d_llo, c->Mqc, Np, d_tiles, c->d_grm, 0, 0, -1, tile);
         if (e == hipSuccess) e = hipEventRecord(c->ev1, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         release();
@@ -1080,6 +1102,8 @@ int saige_build_grm(saige_ctx* c, int precision) {
 }
 
 double saige_last_grm_seconds(const saige_ctx* c) { return c ? c->grm_seconds : 0.0; }
+
+int saige_grm_precision(const saige_ctx* c) { return c && c->grm_ready ? c->grm_precision : -1; }
 
 int saige_get_grm_block(saige_ctx* c, int64_t r0, int64_t c0, int64_t nr, int64_t nc, float* out) {
     CHECK_CTX(c);
@@ -1334,28 +1358,32 @@ hipError_t range_syrk(saige_ctx* c, int64_t start, int64_t end, float* S, int ac
     hipStream_t st = c->stream;
     const int64_t Np = c->Npad, N = c->Nnm;
     const int64_t cnt = end - start + 1;
-    const int tile = c->grm_precision == SAIGE_GRM_BF16 ? syrk_pk_tile() : syrk_tile();
+    const bool split = c->grm_precision == SAIGE_GRM_BF16_SPLIT;
+    const bool pk = c->grm_precision == SAIGE_GRM_BF16 || split;
+    const int tile = split ? 256 : (pk ? syrk_pk_tile() : syrk_tile());
     const std::vector<int2> order = syrk_tile_order(Np / tile, tile == 256 ? 4 : 8);
     int2* d_tiles = nullptr;
     uint8_t* d_tp = nullptr;
-    uint64_t* d_l64 = nullptr;
+    uint64_t *d_l64 = nullptr, *d_llo = nullptr;
     void* d_panel = nullptr;
     auto release = [&]() {
         (void)hipStreamSynchronize(st);
-        dfree(d_tiles); dfree(d_tp); dfree(d_l64);
+        dfree(d_tiles); dfree(d_tp); dfree(d_l64); dfree(d_llo);
         if (d_panel) (void)hipFree(d_panel);
     };
     hipError_t e = dalloc(&d_tiles, (int64_t)order.size());
     if (e == hipSuccess)
         e = hipMemcpyAsync(d_tiles, order.data(), sizeof(int2) * order.size(), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess && c->grm_precision == SAIGE_GRM_BF16) {
+    if (e == hipSuccess && pk) {
         const int64_t nchunk = (c->Mqc + 63) / 64;
         e = dalloc(&d_tp, (Np / syrk_tile()) * nchunk * 2048);
         if (e == hipSuccess) e = dalloc(&d_l64, nchunk * 64);
+        if (e == hipSuccess && split) e = dalloc(&d_llo, nchunk * 64);
         if (e == hipSuccess) e = launch_pack_tiles(st, c->d_packed, c->d_lut, c->Mqc, N, Np, c->bpm_nm, d_tp, d_l64);
-        if (e == hipSuccess) e = launch_lut64_range(st, c->d_lut, c->Mqc, start, end + 1, d_l64);
+        if (e == hipSuccess) e = launch_lut64_range(st, c->d_lut, c->Mqc, start, end + 1, 0, d_l64);
+        if (e == hipSuccess && split) e = launch_lut64_range(st, c->d_lut, c->Mqc, start, end + 1, 1, d_llo);
         if (e == hipSuccess)
-            e = launch_syrk_pk(st, d_tp, d_l64, c->Mqc, Np, d_tiles, S, accumulate, start, end + 1, tile);
+            e = syrk_pk_passes(st, d_tp, d_l64, d_llo, c->Mqc, Np, d_tiles, S, accumulate, start, end + 1, tile);
     } else if (e == hipSuccess) {
         const bool bf16 = c->grm_precision == SAIGE_GRM_BF16_PANEL;
         const int64_t esz = bf16 ? 2 : 4;

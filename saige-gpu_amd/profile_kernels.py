@@ -8,7 +8,10 @@ and HBM counters (FETCH_SIZE, WRITE_SIZE) of each can be read per dispatch:
     rocprofv3 --pmc WRITE_SIZE -d OUT -o write -- python3 saige-gpu_amd/profile_kernels.py
 
 Kernels exercised: decode/QC (k_decode_stats), repack, diag, the GRM SYRK
-(2-bit input k_syrk_pk for --grm bf16), the autotuner's sixteen block-GEMM
+(2-bit input k_syrk_pk256 for --grm bf16; three passes of it, two with the
+remainder table, and k_mirror_diag for bf16_split; --grm takes a
+comma-separated list, built in turn, the last one resident), the
+autotuner's sixteen block-GEMM
 shapes, the tuned 64-column block GEMM (k_split_p + k_gemm_x3t + k_tri_reduce
 + k_gemm_epi in the default --gemm-mode bf16x3_tri, k_split_p + k_gemm_x3w/x3
 + k_gemm_epi in bf16x3, k_gemm_part + k_gemm_epi in fp32, through
@@ -48,8 +51,9 @@ def main():
     s.setminMAFforGRM(0.01)
     s.setmaxMissingRateforGRM(0.15)
     s.setgeno_device(bed, N, M)
-    t = s.build_grm(args.grm)
-    print(f"GRM {args.grm} {t:.3f}s", flush=True)
+    for prec in args.grm.split(","):
+        t = s.build_grm(prec)
+        print(f"GRM {prec} {t:.3f}s", flush=True)
     s.set_gemm_mode(args.gemm_mode)
     tune = s.autotune(reps=2)
     print(f"block GEMM shape rt{tune['rt']}/pd{tune['pd']}/s{tune['nsplit']} {tune['ms']:.3f} ms", flush=True)

@@ -24,6 +24,7 @@ SAIGE_PENDING = 1
 GRM_FP32 = 0
 GRM_BF16 = 1
 GRM_BF16_PANEL = 2
+GRM_BF16_SPLIT = 3
 
 _c_p = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -97,6 +98,7 @@ SIGNATURES = {
     "saige_build_grm": (_int, [_c_p, _int]),
     "saige_get_grm_block": (_int, [_c_p, _i64, _i64, _i64, _i64, _c_p]),
     "saige_last_grm_seconds": (_f64, [_c_p]),
+    "saige_grm_precision": (_int, [_c_p]),
     "saige_crossprod_mat_and_kin": (_int, [_c_p, _c_p, _c_p]),
     "saige_grm_matvec": (_int, [_c_p, _c_p, _i64, _c_p]),
     "saige_get_diag_of_sigma": (_int, [_c_p, _c_p, _c_p, _c_p]),

@@ -179,10 +179,17 @@ class Session:
     get_GRMdiagVec = get_DiagofKin                                  # FG.cpp:4913-4920
 
     # ------------------------------------------------------------ GRM
+    _GRM_PRECISIONS = {"fp32": _lib.GRM_FP32, "bf16": _lib.GRM_BF16, "bf16_panel": _lib.GRM_BF16_PANEL,
+                       "bf16_split": _lib.GRM_BF16_SPLIT}
+
     def build_grm(self, precision="fp32"):
-        prec = {"fp32": _lib.GRM_FP32, "bf16": _lib.GRM_BF16, "bf16_panel": _lib.GRM_BF16_PANEL}[precision]
-        check(self._L.saige_build_grm(self.ctx, prec))
+        check(self._L.saige_build_grm(self.ctx, self._GRM_PRECISIONS[precision]))
         return float(self._L.saige_last_grm_seconds(self.ctx))
+
+    def grm_precision(self):
+        """The precision name the resident GRM was built with (None if none)."""
+        p = int(self._L.saige_grm_precision(self.ctx))
+        return next((k for k, v in self._GRM_PRECISIONS.items() if v == p), None)
 
     def grm_block(self, r0, c0, nr, nc):
         out = np.zeros((nr, nc), np.float32)

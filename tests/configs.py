@@ -65,6 +65,17 @@ def grm_rel_error(K, K64):
     return float(np.max(np.abs(K.astype(np.float64) - K64)) / np.max(np.abs(K64)))
 
 
+def grm_offdiag_error(K, K64):
+    """||offdiag(K - K64)||_F / ||offdiag(K64)||_F: the relatedness signal
+    that tau1 is estimated from (about 1/sqrt(Mqc) for unrelated samples,
+    far below the diagonal's 1)."""
+    D = K.astype(np.float64) - K64
+    np.fill_diagonal(D, 0.0)
+    R = K64.copy()
+    np.fill_diagonal(R, 0.0)
+    return float(np.linalg.norm(D) / np.linalg.norm(R))
+
+
 def fp64_residual2(sess, w, tau, X, B, rows=2048):
     """||Sigma x_c - b_c||^2 per column in FP64, Sigma = tau0 diag(1/w) +
     tau1 K with K the GPU's own GRM (FP32, read back in row blocks); w and

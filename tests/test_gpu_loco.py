@@ -10,6 +10,7 @@ Tolerances (M = Mqc, M_loco = M - range size):
                               (S and the range product each carry the FP32
                               SYRK bound; both are divided by M_loco)
   K_loco columns, BF16      : 2 * 4e-3 (M / M_loco) max|K64|
+  K_loco columns, split BF16: as FP32 (three BF16 piece products)
   diag(Sigma_loco)          : bit-exact (same FP32 operations in the same order)
   Sigma_loco b              : 4e-5 max|ref|
   PCG (tol 1e-5)            : 2e-3 max|x64|, ||Sigma x - b||^2 <= 1.5 tol
@@ -55,7 +56,7 @@ def _columns(session, idx):
     return np.column_stack(cols)
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16", "bf16_panel"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "bf16_panel", "bf16_split"])
 def test_loco_grm_and_diag(session, loaded, monkeypatch, prec):
     og, L = loaded
     print(f"case: LOCO GRM {prec} range {RANGE}", flush=True)
@@ -70,7 +71,7 @@ def test_loco_grm_and_diag(session, loaded, monkeypatch, prec):
     idx = [0, 1, 17, 333, 500, 998, 999]
     C = _columns(session, idx)
     ref = L.K[:, idx]
-    base = 4.0 * np.sqrt(M) * 2.0 ** -24 if prec == "fp32" else 4e-3
+    base = 4.0 * np.sqrt(M) * 2.0 ** -24 if prec in ("fp32", "bf16_split") else 4e-3
     tol = 2 * base * (M / m_loco) * np.max(np.abs(L.K))
     err = np.max(np.abs(C - ref))
     print(f"  max abs err {err:.3e}  tol {tol:.3e}")
