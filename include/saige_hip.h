@@ -274,6 +274,15 @@ int saige_engine_take_batch_ex(saige_ctx* ctx, int k, const int32_t* slots, floa
  * m columns of Q_dev: out_host[j * m + r]. */
 int saige_dots_device(saige_ctx* ctx, const float* A_dev, int64_t lda, int k, const float* Q_dev, int64_t ldq,
                       int m, double* out_host);
+/* saige_dots_device without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  Its scratch is a fixed per-device array of 8.4 MB: at most
+ * 262,144 samples (SAIGE_E_ARG above), k <= 64 columns, 1 <= m <= 16
+ * partners.  Every call on one GPU shares that array, so calls that overlap
+ * on one GPU race; the results are bitwise those of saige_dots_device. */
+int saige_dots_device_noalloc(saige_ctx* ctx, const float* A_dev, int64_t lda, int k, const float* Q_dev,
+                              int64_t ldq, int m, double* out_host);
 /* Rademacher probes i0 .. i0+k-1 (entries +-1 from a counter-based hash of
  * (seed, i, row); FG.cpp:3133-3137 draws them with R's rbinom) into dst_dev,
  * column-major with leading dimension ld >= N. */

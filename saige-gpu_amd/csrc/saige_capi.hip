@@ -14,6 +14,8 @@
 
 #include "saige_internal.hpp"
 
+#include <mutex>
+
 using namespace saige;
 
 namespace saige {
@@ -711,6 +713,38 @@ static int restore_full_grm(saige_ctx* c);
 
 const char* saige_last_error(void) { return g_err.c_str(); }
 
+// ------------------------------------------------ allocation-free dots
+// The scratch of saige_dots_device_noalloc: FP64 partial sums of k_col_dots
+// (one per 256-row block, column and partner) and the finished dots, sized
+// for NOALLOC_MAX_N samples, 64 columns and 16 partners (8.4 MB).  Each
+// device holds its own copy for the life of the process; host code reaches
+// it only through the address hipGetSymbolAddress returns for that device
+// (noalloc_scratch_ptr), never through the array's name.
+constexpr int64_t NOALLOC_MAX_N = 262144;
+constexpr int64_t NOALLOC_OUT = (int64_t)EW * 16;
+constexpr int64_t NOALLOC_PARTIAL = (NOALLOC_MAX_N / 256) * NOALLOC_OUT;
+static __device__ __align__(256) double scratch[NOALLOC_PARTIAL + NOALLOC_OUT];
+
+static std::mutex g_scratch_mu;
+static std::vector<double*> g_scratch_ptr;   // per device, filled once with that device current
+
+// the device address of `scratch` on the current device (device `dev`)
+static int noalloc_scratch_ptr(int dev, double** out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize((size_t)dev + 1, nullptr);
+    if (!g_scratch_ptr[(size_t)dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+            return SAIGE_E_DEVICE;
+        }
+        g_scratch_ptr[(size_t)dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[(size_t)dev];
+    return SAIGE_OK;
+}
+
 int saige_create(int device, saige_ctx** out) {
     if (!out) {
         set_error("out is null");
@@ -734,6 +768,10 @@ int saige_create(int device, saige_ctx** out) {
         delete c;
         return SAIGE_E_DEVICE;
     }
+    // the allocation-free entry point's scratch address on this device,
+    // looked up once per device while it is current; if the lookup fails,
+    // only saige_dots_device_noalloc reports it
+    if (noalloc_scratch_ptr(device, &c->d_noalloc) != SAIGE_OK) c->d_noalloc = nullptr;
     *out = c;
     return SAIGE_OK;
 }
@@ -1664,6 +1702,35 @@ int saige_dots_device(saige_ctx* c, const float* A_dev, int64_t lda, int k, cons
         return SAIGE_E_ARG;
     }
     return column_dots(c, A_dev, lda, 0, nullptr, k, nullptr, nullptr, 0, nullptr, Q_dev, m, ldq, out_host);
+}
+
+int saige_dots_device_noalloc(saige_ctx* c, const float* A_dev, int64_t lda, int k, const float* Q_dev, int64_t ldq,
+                              int m, double* out_host) {
+    CHECK_CTX(c);
+    CHECK_LOADED(c);
+    if (!A_dev || lda < c->Nnm || !Q_dev || ldq < c->Nnm || m < 1 || m > 16 || k < 1 || k > EW || !out_host) {
+        set_error("invalid dot arguments: %d columns x %d partners (at most %d x 16)", k, m, EW);
+        return SAIGE_E_ARG;
+    }
+    if (c->Nnm > NOALLOC_MAX_N) {
+        set_error("saige_dots_device_noalloc: %lld samples exceed its fixed scratch (at most %lld)",
+                  (long long)c->Nnm, (long long)NOALLOC_MAX_N);
+        return SAIGE_E_ARG;
+    }
+    if (!c->d_noalloc) {
+        double* p = nullptr;                 // the lookup failed at saige_create: its error again
+        if (int rc = noalloc_scratch_ptr(c->device, &p)) return rc;
+        c->d_noalloc = p;
+    }
+    hipStream_t st = c->stream;
+    // every partial the finishing kernel reads is written first by k_col_dots:
+    // nothing to clear
+    SAIGE_HIP_TRY(launch_col_dots(st, A_dev, lda, 0, nullptr, k, nullptr, nullptr, 0, nullptr, Q_dev, m, ldq,
+                                  c->Nnm, c->d_noalloc, c->d_noalloc + NOALLOC_PARTIAL));
+    SAIGE_HIP_TRY(hipMemcpyAsync(out_host, c->d_noalloc + NOALLOC_PARTIAL, sizeof(double) * k * m,
+                                 hipMemcpyDeviceToHost, st));
+    SAIGE_HIP_TRY(hipStreamSynchronize(st));
+    return SAIGE_OK;
 }
 
 int saige_probes_device(saige_ctx* c, float* dst_dev, int64_t ld, uint64_t seed, int64_t i0, int k) {

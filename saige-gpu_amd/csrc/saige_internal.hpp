@@ -184,6 +184,7 @@ struct saige_ctx {
     double gemm_tuned_ms_half = 0.0;
     // FP64 column-dot scratch (saige_engine_take_batch_ex, saige_dots_device)
     double* d_dots_partial = nullptr;
+    double* d_noalloc = nullptr;            // this device's address of the allocation-free scratch
     double* d_dots_out = nullptr;
     int32_t* d_take_lists = nullptr;             // [3][EW]: slots, destination columns, partner columns
     int grm_precision = -1;

@@ -126,6 +126,7 @@ SIGNATURES = {
     "saige_engine_take_batch_ex": (_int, [_c_p, _int, _c_p, _c_p, _c_p, _i64, _c_p, _c_p, _c_p, _i64, _c_p,
                                           _c_p, _int, _i64, _c_p]),
     "saige_dots_device": (_int, [_c_p, _c_p, _i64, _int, _c_p, _i64, _int, _c_p]),
+    "saige_dots_device_noalloc": (_int, [_c_p, _c_p, _i64, _int, _c_p, _i64, _int, _c_p]),
     "saige_probes_device": (_int, [_c_p, _c_p, _i64, ctypes.c_uint64, _i64, _int]),
     "saige_copy_host_to_device": (_int, [_c_p, _c_p, _c_p, _i64]),
     "saige_autotune": (_int, [_c_p, _int, _c_p]),

@@ -457,6 +457,14 @@ class Session:
         check(self._L.saige_dots_device(self.ctx, A, N, int(k), Q, N, int(m), ptr(out)))
         return out
 
+    def dots_device_noalloc(self, A, k, Q, m):
+        """dots_device through saige_dots_device_noalloc (fixed device
+        scratch, no allocation; at most 262,144 samples)."""
+        out = np.zeros((k, m), np.float64)
+        N = self.getNnomissingOut()
+        check(self._L.saige_dots_device_noalloc(self.ctx, A, N, int(k), Q, N, int(m), ptr(out)))
+        return out
+
     def probes_device(self, buf, col0, seed, i0, k):
         """Rademacher probes i0 .. i0+k-1 into columns col0.. of buf."""
         check(self._L.saige_probes_device(self.ctx, self._col_ptr(buf, col0), self.getNnomissingOut(), int(seed),
