@@ -6,15 +6,18 @@ U(0.01, 0.5), 1% missing calls, synthetic PLINK .bed generated in HBM;
 intercept + age + sex covariates (QR-transformed as fitNULLGLMM does);
 four binary phenotypes with prevalences 0.05, 0.1, 0.2, 0.3.  The GRM is
 built once from BF16 standardised genotypes on MFMA with FP32 accumulation
-and held in HBM as FP32.
+(a single pass, timed for GRM TFLOP/s, then the split build the fits use:
+each value as two BF16 pieces, three passes) and held in HBM as FP32.
 
-A step = one batched round of null fits of the four phenotypes: AI-REML
-with nrun = 15 Rademacher probes per phenotype, so each trace phase is a
-block PCG over 4 x (15 probes + 1) = 64 right-hand sides (FP32 vectors;
-GRM products at FP32 accuracy from three-piece BF16 splits on the MFMA,
-read from the GRM's lower triangle; FP64 dot products).  `value` = fits completed per
-hour of step time.  The plain path (one phenotype, one right-hand side at a
-time, GEMV per PCG iteration) is timed once beside it.
+A step = one batched round of null fits of the four phenotypes through the
+library's C++ drivers (saige_fit_null_batch): AI-REML with nrun = 15
+Rademacher probes per phenotype, so each trace phase is a block PCG over
+4 x (15 probes + 1) = 64 right-hand sides (FP32 vectors; GRM products at
+FP32 accuracy from three-piece BF16 splits on the MFMA, read from the GRM's
+lower triangle; FP64 dot products).  `value` = fits completed per hour of
+step time.  The Python drivers over the same engine and the plain path (one
+phenotype, one right-hand side at a time, GEMV per PCG iteration) are timed
+beside it.
 
 Prints one JSON line.
 """
@@ -531,14 +534,15 @@ def main():
             raise RuntimeError(f"native driver failed: {[r['status'] for r in res]}")
         return res
 
+    # the timed steps: the library's C++ drivers and scheduler
+    # (saige_fit_null_batch), the path an R or C++ caller reaches
     for _ in range(args.warmup):
-        step()
+        native_step()
     g0, n0_ = sess.engine_stats("gemm_full")
     h0, m0 = sess.engine_stats("gemm_half")
     t0 = time.perf_counter()
-    results = []
     for _ in range(args.steps):
-        results.append(step())
+        nat = native_step()
     t1 = time.perf_counter()
     g1, n1 = sess.engine_stats("gemm_full")
     h1, m1 = sess.engine_stats("gemm_half")
@@ -547,38 +551,37 @@ def main():
     step_s = (t1 - t0) / args.steps
     fits = len(jobs) * args.steps
     fits_per_hour = fits / (t1 - t0) * 3600.0
-    handles, sch = results[-1]
     gemm_launches = n1 - n0_
     gemm_avg = (g1 - g0) / max(1, gemm_launches)
     gemm_flop = 2.0 * N * N * 64
     gemm_tflops = gemm_flop / gemm_avg / 1e12
-    lat = sorted(h.t_done - h.t_submit for h in handles)
-    log("batched step split (s): " + ", ".join(f"{k} {v:.3f}" for k, v in sorted(sch.timing.items())))
-    log(f"batched: {step_s:.3f}s/step, {fits_per_hour:.1f} fits/h, {gemm_launches} 64-column GEMMs avg "
-        f"{gemm_avg * 1e3:.2f} ms ({gemm_tflops:.1f} TFLOP/s), {half_launches} 32-column GEMMs avg "
-        f"{half_avg * 1e3:.2f} ms, theta {[list(np.round(h.result['theta'], 4)) for h in handles]}")
+    nat_s = step_s
+    log(f"C++ drivers (saige_fit_null_batch): {step_s:.3f}s/step, {fits_per_hour:.1f} fits/h, {gemm_launches} "
+        f"64-column GEMMs avg {gemm_avg * 1e3:.2f} ms ({gemm_tflops:.1f} TFLOP/s), {half_launches} 32-column GEMMs "
+        f"avg {half_avg * 1e3:.2f} ms ({(g1 - g0 + h1 - h0) / args.steps:.3f}s of GEMM per step), theta "
+        f"{[list(np.round(r['theta'], 4)) for r in nat]}")
 
-    # the same rounds through the library's C++ drivers and scheduler
-    # (saige_fit_null_batch), the path an R or C++ caller reaches: timed
-    # over the same number of steps
-    native_step()
+    # the same rounds through the Python drivers and scheduler (the
+    # library's test harness), timed over the same number of steps
+    step()
     st0 = [sess.engine_stats(k) for k in ("gemm_full", "gemm_half")]
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        nat = native_step()
-    nat_s = (time.perf_counter() - t0) / args.steps
+        handles, sch = step()
+    py_s = (time.perf_counter() - t0) / args.steps
     st1 = [sess.engine_stats(k) for k in ("gemm_full", "gemm_half")]
-    native = {"fits_per_hour": len(jobs) / nat_s * 3600.0, "seconds_per_step": nat_s, "steps": args.steps,
-              "max_rel_theta_diff_vs_python_driver": max(
-                  float(np.max(np.abs(r["theta"] - h.result["theta"]) / np.maximum(np.abs(h.result["theta"]), 1e-12)))
-                  for r, h in zip(nat, handles)),
-              "gemm_full_launches_per_step": (st1[0][1] - st0[0][1]) / args.steps,
-              "gemm_half_launches_per_step": (st1[1][1] - st0[1][1]) / args.steps,
-              "gemm_device_seconds_per_step": ((st1[0][0] - st0[0][0]) + (st1[1][0] - st0[1][0])) / args.steps}
-    log(f"C++ drivers (saige_fit_null_batch): {nat_s:.3f}s/step, {native['fits_per_hour']:.0f} fits/h, "
-        f"{native['gemm_full_launches_per_step']:.0f} full + {native['gemm_half_launches_per_step']:.0f} half GEMMs "
-        f"({native['gemm_device_seconds_per_step']:.3f}s) per step, max rel theta diff vs the Python scheduler "
-        f"{native['max_rel_theta_diff_vs_python_driver']:.1e}")
+    lat = sorted(h.t_done - h.t_submit for h in handles)
+    python_driver = {
+        "fits_per_hour": len(jobs) / py_s * 3600.0, "seconds_per_step": py_s, "steps": args.steps,
+        "max_rel_theta_diff_vs_cpp_driver": max(
+            float(np.max(np.abs(h.result["theta"] - r["theta"]) / np.maximum(np.abs(r["theta"]), 1e-12)))
+            for r, h in zip(nat, handles)),
+        "gemm_full_launches_per_step": (st1[0][1] - st0[0][1]) / args.steps,
+        "gemm_half_launches_per_step": (st1[1][1] - st0[1][1]) / args.steps,
+        "gemm_device_seconds_per_step": ((st1[0][0] - st0[0][0]) + (st1[1][0] - st0[1][0])) / args.steps}
+    log("Python scheduler step split (s): " + ", ".join(f"{k} {v:.3f}" for k, v in sorted(sch.timing.items())))
+    log(f"Python drivers: {py_s:.3f}s/step, {python_driver['fits_per_hour']:.0f} fits/h, max rel theta diff vs the "
+        f"C++ drivers {python_driver['max_rel_theta_diff_vs_cpp_driver']:.1e}")
 
     # the same round with the block GEMM on the FP32 MFMA, for comparison:
     # tuned in that mode, one step timed, results checked against the step above
@@ -589,13 +592,13 @@ def main():
         g0, n0_ = sess.engine_stats("gemm_full")
         h0, m0 = sess.engine_stats("gemm_half")
         t0 = time.perf_counter()
-        h32, _ = step()
+        r32 = native_step()
         s32 = time.perf_counter() - t0
         g1, n1 = sess.engine_stats("gemm_full")
         h1, m1 = sess.engine_stats("gemm_half")
-        dtheta = max(float(np.max(np.abs(np.asarray(a.result["theta"]) - np.asarray(b.result["theta"]))
-                                  / np.maximum(np.abs(np.asarray(b.result["theta"])), 1e-12)))
-                     for a, b in zip(handles, h32))
+        dtheta = max(float(np.max(np.abs(np.asarray(a["theta"]) - np.asarray(b["theta"]))
+                                  / np.maximum(np.abs(np.asarray(a["theta"])), 1e-12)))
+                     for a, b in zip(nat, r32))
         fp32_mode = {"fits_per_hour": len(jobs) / s32 * 3600.0, "seconds_per_step": s32,
                      "gemm_tuned": {k: t32[k] for k in ("rt", "pd", "nsplit", "ms")},
                      "gemm_full_avg_ms": (g1 - g0) / max(1, n1 - n0_) * 1e3,
@@ -630,18 +633,18 @@ def main():
     if not args.no_nrun30:
         o30 = D.Options(nrun=30)
         t0 = time.perf_counter()
-        step(o30)
-        b30 = time.perf_counter() - t0
-        t0 = time.perf_counter()
         native_step(o30)
         n30 = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        step(o30)
+        b30 = time.perf_counter() - t0
         tp = time.perf_counter()
         D.fit_plain(sess, "binary", phenos[2], X, o30)
         p30 = time.perf_counter() - tp
-        nrun30 = {"batched_fits_per_hour": len(jobs) / b30 * 3600.0, "native_fits_per_hour": len(jobs) / n30 * 3600.0,
-                  "plain_fits_per_hour": 3600.0 / p30, "batched_vs_plain": p30 * len(jobs) / b30,
-                  "native_vs_plain": p30 * len(jobs) / n30}
-        log(f"nrun 30: batched {nrun30['batched_fits_per_hour']:.0f}, C++ {nrun30['native_fits_per_hour']:.0f}, "
+        nrun30 = {"fits_per_hour": len(jobs) / n30 * 3600.0, "python_fits_per_hour": len(jobs) / b30 * 3600.0,
+                  "plain_fits_per_hour": 3600.0 / p30, "batched_vs_plain": p30 * len(jobs) / n30,
+                  "python_vs_plain": p30 * len(jobs) / b30}
+        log(f"nrun 30: C++ {nrun30['fits_per_hour']:.0f}, Python {nrun30['python_fits_per_hour']:.0f}, "
             f"plain {nrun30['plain_fits_per_hour']:.0f} fits/h")
 
     serving = saturation = None
@@ -774,8 +777,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (device-generated PLINK .bed, simulated phenotypes)",
-        "config": {"workload": f"batched binary null fits, N={N}, M={M}, 4 phenotypes x (nrun {args.nrun} + 1) "
-                               f"= 64-RHS block PCG, {args.grm} GRM (FP32 storage)",
+        "config": {"workload": f"batched binary null fits through the C++ drivers (saige_fit_null_batch), N={N}, "
+                               f"M={M}, 4 phenotypes x (nrun {args.nrun} + 1) = 64-RHS block PCG, {args.grm} GRM "
+                               f"(FP32 storage)",
                    "N": N, "M": M, "Mqc": Mqc, "phenotypes": len(jobs), "nrun": args.nrun, "grm": args.grm,
                    "gemm_mode": args.gemm_mode},
         "grm_tflops": grm_tflops,
@@ -784,11 +788,11 @@ def main():
         "grm_split": split,
         "setgeno_seconds": setgeno_s,
         "gemm_autotune": tune,
-        "native_cpp_driver": native,
+        "python_driver": python_driver,
         "fp32_mode": fp32_mode,
         "plain": plain,
         "batched_vs_plain": (fits_per_hour / plain["fits_per_hour"]) if plain else None,
-        "native_vs_plain": (native["fits_per_hour"] / plain["fits_per_hour"]) if plain else None,
+        "python_vs_plain": (python_driver["fits_per_hour"] / plain["fits_per_hour"]) if plain else None,
         "nrun30": nrun30,
         "p50_fit_latency_s": serving["p50_latency_s"] if serving else float(np.percentile(lat, 50)),
         "p99_fit_latency_s": serving["p99_latency_s"] if serving else float(np.percentile(lat, 99)),

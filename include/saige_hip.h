@@ -46,7 +46,7 @@ extern "C" {
 #define SAIGE_GRM_BF16_PANEL 2  /* same arithmetic from materialised BF16 Z panels     */
 #define SAIGE_GRM_BF16_SPLIT 3  /* 2-bit genotypes standardised to two BF16 pieces each,
                                    z = hi + lo (lo the rounded remainder, z - hi - lo
-                                   within 2^-18 |z|); hi.hi + hi.lo + lo.hi on the BF16
+                                   within 2^-16 |z|); hi.hi + hi.lo + lo.hi on the BF16
                                    MFMA with FP32 accumulation, three SYRK passes: an
                                    FP32-accurate S at three times the BF16 build      */
 

@@ -468,7 +468,8 @@ __global__ __launch_bounds__(256) void k_pack_tiles(const uint8_t* __restrict__ 
 
 // lut64[j] for j < nchunk*64 (zero for padding markers).  piece 0: each
 // value rounded to BF16 (hi); piece 1: the remainder x - hi, exact in FP32,
-// rounded to BF16 (lo), so that x - hi - lo is within 2^-18 |x|.
+// rounded to BF16 (lo), so that x - hi - lo is within 2^-16 |x| (BF16 keeps
+// 8 significant bits: each rounding is within 2^-8 relative).
 __device__ __forceinline__ uint64_t bf16_piece(float x, int piece) {
     const __hip_bfloat16 hi = __float2bfloat16(x);
     if (piece == 0) return __bfloat16_as_ushort(hi);

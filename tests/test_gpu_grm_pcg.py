@@ -5,7 +5,7 @@ operators and the PCG solvers, against the oracle (FG.cpp:1576-1708,
 Tolerances (stated here, derived in DESIGN.md):
   GRM FP32 SYRK   : max |K - K64| <= 4 sqrt(Mqc) 2^-24 max|K64|  (FP32 sum of Mqc terms)
   GRM BF16 SYRK   : max |K - K64| <= 4e-3 max|K64|               (BF16 z, 2^-9 per operand)
-  GRM split BF16  : as FP32 (z = hi + lo within 2^-18 |z|, lo.lo dropped)
+  GRM split BF16  : as FP32 (z = hi + lo within 2^-16 |z|, lo.lo dropped)
   K b (FP32)      : max |Kb - Kb64| <= 2e-5 * max|Kb64|
   PCG (tol 1e-5)  : max |x - x64| <= 2e-3 * max|x64|, ||Sigma x - b||^2 <= 1.5 tol
 """
