@@ -1531,6 +1531,47 @@ void tri_plan_free(TriPlan& t) {
     t = TriPlan{};
 }
 
+// The dispatch order of the triangle's items.  A launch's workgroups go to
+// the eight XCDs in turn (workgroup b to XCD b % 8), one per CU at a time,
+// 32 CUs per XCD, and each XCD has its own L2.  Items of different row sets
+// over the same chunk range read the same staged P pieces, so full-length
+// items are grouped by range into tiles of 32; a round of eight tiles takes
+// 256 consecutive positions, tile x at positions x, x + 8, ...: the 32
+// workgroups of one XCD then walk the same chunks together and share their
+// pieces in its L2.  The rest (partial tiles, shorter and diagonal items)
+// follows, longest first, to fill the tail.
+static std::vector<int4> tri_dispatch_order(const std::vector<int4>& items, int nwv, int ch) {
+    constexpr int XCD = 8, PER_XCD = 32;
+    std::vector<std::vector<int4>> groups;
+    std::vector<int4> rest;
+    for (const int4& it : items) {
+        const bool left = it.y < nwv * it.x;         // left of the row set's diagonal block
+        if (left && it.z - it.y == ch) {
+            const size_t g = (size_t)(it.y / ch);    // full items start at multiples of ch
+            if (groups.size() <= g) groups.resize(g + 1);
+            groups[g].push_back(it);
+        } else {
+            rest.push_back(it);
+        }
+    }
+    std::vector<std::vector<int4>> tiles;
+    for (const auto& v : groups) {
+        size_t i = 0;
+        for (; i + PER_XCD <= v.size(); i += PER_XCD) tiles.emplace_back(v.begin() + i, v.begin() + i + PER_XCD);
+        rest.insert(rest.end(), v.begin() + i, v.end());
+    }
+    const size_t rounds = tiles.size() / XCD;
+    for (size_t t = rounds * XCD; t < tiles.size(); t++) rest.insert(rest.end(), tiles[t].begin(), tiles[t].end());
+    std::vector<int4> out;
+    out.reserve(items.size());
+    for (size_t r = 0; r < rounds; r++)
+        for (int i = 0; i < PER_XCD; i++)
+            for (int x = 0; x < XCD; x++) out.push_back(tiles[r * XCD + x][(size_t)i]);
+    std::stable_sort(rest.begin(), rest.end(), [](const int4& a, const int4& b) { return a.z - a.y > b.z - b.y; });
+    out.insert(out.end(), rest.begin(), rest.end());
+    return out;
+}
+
 hipError_t tri_plan_build(TriPlan& t, int64_t Npad, int nwv, int ch) {
     if (t.items && t.Npad == Npad && t.nwv == nwv && t.ch == ch) return hipSuccess;
     tri_plan_free(t);
@@ -1540,17 +1581,15 @@ hipError_t tri_plan_build(TriPlan& t, int64_t Npad, int nwv, int ch) {
     std::vector<int2> rtab((size_t)nr);
     int64_t slot = 0;
     for (int64_t R = 0; R < nr; R++) {
-        // chunks left of the diagonal block, cut into k near-equal items, then
-        // the diagonal block itself
-        const int64_t off = nwv * R, k = (off + ch - 1) / ch;
-        rtab[(size_t)R] = make_int2((int)slot, (int)(k + 1));
-        for (int64_t i = 0; i < k; i++)
-            items.push_back(make_int4((int)R, (int)(off * i / k), (int)(off * (i + 1) / k), (int)slot++));
+        // chunks left of the diagonal block, cut at multiples of ch (the last
+        // piece shorter), then the diagonal block itself; slots in chunk order
+        const int64_t off = nwv * R, first = slot;
+        for (int64_t a = 0; a < off; a += ch)
+            items.push_back(make_int4((int)R, (int)a, (int)std::min<int64_t>(a + ch, off), (int)slot++));
         items.push_back(make_int4((int)R, (int)off, (int)std::min(off + nwv, nch), (int)slot++));
+        rtab[(size_t)R] = make_int2((int)first, (int)(slot - first));
     }
-    // longest items first, so the short ones fill the tail of the launch
-    std::stable_sort(items.begin(), items.end(),
-                     [](const int4& a, const int4& b) { return a.z - a.y > b.z - b.y; });
+    items = tri_dispatch_order(items, nwv, ch);
     const int64_t y2 = nwv == 8 ? tri_y2_first<8>(nch, nr) : tri_y2_first<12>(nch, nr);
     t.Npad = Npad;
     t.nwv = nwv;
