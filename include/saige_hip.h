@@ -469,7 +469,9 @@ int saige_submit(saige_server* srv, const saige_fit_spec* spec, int64_t* ticket)
  * whether the fit succeeded; the ticket is then released), or SAIGE_PENDING
  * when it has not finished within wait_ms milliseconds (0: do not wait;
  * negative: wait until it has).  times (2, or NULL): seconds from
- * submission to admission and to completion. */
+ * submission to admission and to completion.  Polls of one ticket from
+ * several threads are safe: one collects the result, the others get
+ * SAIGE_E_ARG. */
 int saige_poll(saige_server* srv, int64_t ticket, int wait_ms, saige_fit_result* result, double* times);
 /* Finish every queued request, stop the scheduler thread and free the
  * server (results not yet polled are dropped). */

@@ -942,18 +942,28 @@ int saige_submit(saige_server* srv, const saige_fit_spec* spec, int64_t* ticket)
 int saige_poll(saige_server* srv, int64_t ticket, int wait_ms, saige_fit_result* result, double* times) {
     if (!srv || !result || !result->coefficients) return SAIGE_E_ARG;
     std::unique_lock<std::mutex> lk(srv->sch.mu);
-    auto it = srv->tasks.find(ticket);
-    if (it == srv->tasks.end()) {
+    if (srv->tasks.find(ticket) == srv->tasks.end()) {
         saige::set_error("saige_poll: unknown ticket %lld", (long long)ticket);
         return SAIGE_E_ARG;
     }
-    FitTask* f = it->second.get();
-    auto ready = [&] { return f->released || (srv->ended && f->finished) || (srv->ended && !f->th.joinable()); };
+    // the task is looked up afresh after every wake: another thread's poll
+    // of the same ticket may have collected (and freed) it meanwhile
+    auto ready = [&] {
+        const auto jt = srv->tasks.find(ticket);
+        if (jt == srv->tasks.end()) return true;
+        const FitTask* g = jt->second.get();
+        return g->released || (srv->ended && (g->finished || !g->th.joinable()));
+    };
     if (wait_ms < 0)
         srv->sch.done_cv.wait(lk, ready);
     else if (wait_ms > 0)
         srv->sch.done_cv.wait_for(lk, std::chrono::milliseconds(wait_ms), ready);
     if (!ready()) return SAIGE_PENDING;
+    auto it = srv->tasks.find(ticket);
+    if (it == srv->tasks.end()) {
+        saige::set_error("saige_poll: ticket %lld was collected by another poll", (long long)ticket);
+        return SAIGE_E_ARG;
+    }
     std::unique_ptr<FitTask> task = std::move(it->second);
     srv->tasks.erase(it);
     const int engine_rc = srv->rc;

@@ -167,6 +167,44 @@ def test_native_driver_batch(session, cohort):
     assert np.array_equal(alone[0]["coefficients"], nat[0]["coefficients"])
 
 
+def test_server_ticket_polled_from_two_threads(session, cohort):
+    """saige_poll of one ticket from two threads at once: one collects the
+    result (the same fit as saige_fit_null_batch gives), the other is told
+    the ticket is gone, and the server stops cleanly."""
+    import ctypes
+    import threading
+    from saige_gpu import _lib, api
+    c, grm, X, phenos = cohort
+    print("case: server, one ticket polled from two threads", flush=True)
+    opt = D.Options()
+    job = {"trait": "binary", "y": phenos["b20"][1], "X": X}
+    ref = session.fit_null_batch([job], opt)[0]
+    L = session._L
+    with session.server(opt, max_concurrent=4, hbm_reserve_bytes=0) as srv:
+        ticket = srv.submit(job)
+        rcs, outs = [None, None], [None, None]
+
+        def poll(i):
+            res = _lib.FitResult()
+            keep = api._fit_result(job, res)
+            rcs[i] = L.saige_poll(srv.h, ctypes.c_int64(ticket), -1, ctypes.byref(res), None)
+            outs[i] = (res, keep)
+
+        th = [threading.Thread(target=poll, args=(i,)) for i in range(2)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join(timeout=120)
+        assert not any(t.is_alive() for t in th)
+        print(f"  poll return codes {rcs}", flush=True)
+        assert sorted(rcs) == [_lib.SAIGE_E_ARG, _lib.SAIGE_OK]
+        i = rcs.index(_lib.SAIGE_OK)
+        got = api._result_dict(outs[i][0], outs[i][1], "binary")
+        assert np.array_equal(got["theta"], ref["theta"])
+        assert np.array_equal(got["coefficients"], ref["coefficients"])
+        srv._jobs.pop(ticket, None)
+
+
 def test_variance_ratio_vs_oracle(session):
     """Null fit + variance ratio (FG.R:2152-2423) with variance-ratio markers
     held out of the GRM (FG.cpp:545-547), against the FP64 oracle on the
