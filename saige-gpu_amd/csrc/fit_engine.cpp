@@ -91,6 +91,7 @@ struct FitTask : saige::fit::Solver {
     std::chrono::steady_clock::time_point t_submit, t_admit, t_done;
     int64_t posts = 0;            // requests posted so far
     std::condition_variable cv;   // the fit waits here for its request
+    std::chrono::steady_clock::time_point t_resumed{};   // its thread last woke (diagnostics)
 
     void solve(const std::vector<double>& w, const double tau[2], const Matrix& B, Matrix& X) override;
     void product(const Matrix& B, Matrix& KB) override;
@@ -142,9 +143,14 @@ struct Scheduler {
     bool stopping = false;                // no more submissions: run() returns when all are done
     int64_t hbm_reserve = 0;              // admit only while this much HBM is free
     std::condition_variable done_cv;      // a fit finished (saige_poll waits here)
+    double t_fit_host = 0;                // fits' own host seconds between requests (diagnostics)
 
     void post(FitTask* f, Request* r) {
+        using clk = std::chrono::steady_clock;
+        const auto now = clk::now();
         std::unique_lock<std::mutex> lk(mu);
+        if (f->t_resumed.time_since_epoch().count() != 0)
+            t_fit_host += std::chrono::duration<double>(now - f->t_resumed).count();
         if (!fatal.empty()) {
             r->error = fatal;
             r->done = true;
@@ -154,6 +160,7 @@ struct Scheduler {
         f->posts++;
         cv.notify_all();
         f->cv.wait(lk, [&] { return r->done; });
+        f->t_resumed = clk::now();
     }
 };
 
@@ -281,11 +288,13 @@ void run_scheduler(Scheduler& sch, int n_systems) {
     // environment sets SAIGE_ENGINE_TIMING (diagnostics only)
     const bool timing = std::getenv("SAIGE_ENGINE_TIMING") != nullptr;
     double t_idle = 0, t_fill = 0, t_step = 0, t_wait_posts = 0, t_take = 0, t_setup = 0;
+    double t_sys = 0, t_upload = 0, t_devadd = 0, t_take_dots = 0, t_take_host = 0;
     int64_t n_steps = 0;
     using clk = std::chrono::steady_clock;
     auto secs = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
 
     auto set_system = [&](FitTask* f, const double* w, const double tau[2]) {
+        const auto t0 = clk::now();
         const saige_fit_spec* s = f->spec;
         std::fill(mask.begin(), mask.end(), (uint8_t)(s->subset ? 0 : 1));
         std::fill(wfull.begin(), wfull.end(), 1.0f);
@@ -296,6 +305,7 @@ void run_scheduler(Scheduler& sch, int n_systems) {
         }
         const float t[2] = {(float)tau[0], (float)tau[1]};
         check(saige_engine_set_system(sch.ctx, f->sys, wfull.data(), t, mask.data()));
+        t_sys += secs(t0);
     };
 
     // a trace round's device side before its first column runs: the probes
@@ -377,6 +387,7 @@ void run_scheduler(Scheduler& sch, int n_systems) {
             active.push_back(f);
             resumed.emplace_back(f, 0);               // its first request is awaited like a resumed fit's
             f->th = std::thread([f, &sch] {
+                f->t_resumed = std::chrono::steady_clock::now();
                 try {
                     const saige_fit_spec* s = f->spec;
                     Matrix X(s->n, s->p);
@@ -528,6 +539,7 @@ void run_scheduler(Scheduler& sch, int n_systems) {
                 for (int64_t i = 0; i < s->n; i++) d[sample_of(s, i)] = (float)b[i];
             }
         };
+        const auto t_up0 = clk::now();
         if (!s_cols.empty()) {
             upload(s_cols);
             check(saige_engine_add_batch(sch.ctx, (int)s_slots.size(), s_slots.data(), s_sys.data(), colbuf.data(),
@@ -538,6 +550,8 @@ void run_scheduler(Scheduler& sch, int n_systems) {
             check(saige_engine_add_products(sch.ctx, (int)p_slots.size(), p_slots.data(), p_sys.data(),
                                             colbuf.data()));
         }
+        t_upload += secs(t_up0);
+        const auto t_dev0 = clk::now();
         for (const DevRun& r : runs) {
             const float* B = tbufs.b[(size_t)r.fit->sys].U + (int64_t)r.col0 * N;
             if (r.solve)
@@ -547,6 +561,7 @@ void run_scheduler(Scheduler& sch, int n_systems) {
                 check(saige_engine_add_products_device(sch.ctx, (int)r.slots.size(), r.slots.data(),
                                                        r.systems.data(), B));
         }
+        t_devadd += secs(t_dev0);
         t_fill += secs(t_fill0);
         if (live == 0) continue;
         // the step runs on a helper thread while the fits that the previous
@@ -583,6 +598,7 @@ void run_scheduler(Scheduler& sch, int n_systems) {
         if (fin.empty()) continue;
         // device columns of a trace round: K u to AU with the h dots, or the
         // Sigma^-1 u dots d (and e); one take per fit and kind
+        const auto t_td0 = clk::now();
         {
             std::vector<int32_t> sl, cl;
             std::vector<double> dots;
@@ -622,10 +638,13 @@ void run_scheduler(Scheduler& sch, int n_systems) {
                 }
             }
         }
+        t_take_dots += secs(t_td0);
+        const auto t_th0 = clk::now();
         if (!host.empty()) {
             colbuf.assign((size_t)(N * (int64_t)host.size()), 0.0f);
             check(saige_engine_take_batch(sch.ctx, (int)host.size(), host.data(), colbuf.data()));
         }
+        t_take_host += secs(t_th0);
         std::lock_guard<std::mutex> lk(sch.mu);
         size_t hk = 0;
         for (const int32_t slot : fin) {
@@ -688,8 +707,11 @@ void run_scheduler(Scheduler& sch, int n_systems) {
     if (timing)
         std::fprintf(stderr,
                      "saige engine: %lld steps; idle (waiting for requests) %.4f s, fill %.4f s (of which trace "
-                     "setup %.4f s), step %.4f s (of which waiting for resumed fits %.4f s), take %.4f s\n",
-                     (long long)n_steps, t_idle, t_fill, t_setup, t_step, t_wait_posts, t_take);
+                     "setup %.4f s, systems %.4f s, host-column uploads %.4f s, device-column adds %.4f s), step "
+                     "%.4f s (of which waiting for resumed fits %.4f s), take %.4f s (of which dot takes %.4f s, "
+                     "host-column downloads %.4f s); fits' host work between requests %.4f s in all\n",
+                     (long long)n_steps, t_idle, t_fill, t_setup, t_sys, t_upload, t_devadd, t_step, t_wait_posts,
+                     t_take, t_take_dots, t_take_host, sch.t_fit_host);
     } catch (const std::exception& e) {
         // the engine failed: fail every request in the inbox, waiting for
         // slots or in a slot, so that every admitted fit's thread ends; later
