@@ -143,7 +143,9 @@ int saige_get_diag_of_kin(saige_ctx* ctx, float* out_host);
  * (FG.cpp:1893-1950, GPU.cu:30-138; GPU.hpp:29), which upload the FP32 Z
  * instead.  K = S / Mqc. */
 int saige_build_grm(saige_ctx* ctx, int precision);
-/* Copy K[r0:r0+nr, c0:c0+nc] (= S / Mqc, FP32) to host, row-major. */
+/* The three calls below inspect the resident GRM; the reference keeps no
+ * GRM (it multiplies by Z twice per product), so they have no counterpart.
+ * Copy K[r0:r0+nr, c0:c0+nc] (= S / Mqc, FP32) to host, row-major. */
 int saige_get_grm_block(saige_ctx* ctx, int64_t r0, int64_t c0, int64_t nr, int64_t nc,
                         float* out_host);
 /* Seconds of device time of the last saige_build_grm (HIP events). */
