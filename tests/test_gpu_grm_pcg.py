@@ -254,6 +254,48 @@ def test_odd_sizes(session):
     assert np.max(np.abs(x - x64)) <= 2e-3 * np.max(np.abs(x64))
 
 
+@pytest.mark.parametrize("N,M", [(5, 70), (257, 129), (333, 250)])
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "bf16_split"])
+def test_grm_tiny_and_odd_sizes(session, monkeypatch, N, M, prec):
+    """Cohorts far below one 256-sample tile and just past it, marker counts
+    off the 64-marker chunk: every GRM precision against FP64 (the split
+    within its worst case, 3 2^-16 per product, plus the FP32 sum bound; one
+    BF16 pass within 2^-7 per product), exactly symmetric; then a 64-column
+    block solve in the default lower-triangle mode against a dense solve
+    (at N=257 the second row set runs a short item left of its diagonal)."""
+    print(f"case: tiny/odd GRM N={N} M={M} {prec}", flush=True)
+    monkeypatch.delenv("SAIGE_GRM_PANEL_MARKERS", raising=False)
+    body, G, mm = random_bed(N, M, 0.02, seed=N + M, maf_lo=0.05, maf_hi=0.5)
+    session.setminMAFforGRM(0.01)
+    session.setmaxMissingRateforGRM(0.15)
+    session.setgeno_bytes(body, N, M)
+    og = O.setgeno(body, N, M, min_maf=0.01, max_missing=0.15)
+    session.build_grm(prec)
+    K = session.grm_block(0, 0, N, N)
+    K64 = O.grm_f64(og) / og["Mqc"]
+    tol = {"fp32": fp32_tol(og["Mqc"]), "bf16": 2.0 ** -7 + fp32_tol(og["Mqc"]),
+           "bf16_split": 3 * 2.0 ** -16 + fp32_tol(og["Mqc"])}[prec]
+    err = np.max(np.abs(K - K64)) / np.max(np.abs(K64))
+    print(f"  max|K - K64| / max|K64| = {err:.2e} (tol {tol:.2e})", flush=True)
+    assert err <= tol
+    assert np.array_equal(K, K.T)
+    mode = session.gemm_mode()
+    try:
+        session.set_gemm_mode("bf16x3_tri")
+        rng = np.random.default_rng(N)
+        w = rng.uniform(0.1, 0.25, N)
+        tau = np.array([1.0, 0.5])
+        B = rng.standard_normal((N, 64))
+        X, it = session.pcg_block(w, tau, B, 500, 1e-6)
+        Kd = K.astype(np.float64)
+        exact = np.linalg.solve(tau[0] * np.diag(1.0 / w.astype(np.float32).astype(np.float64)) + tau[1] * Kd, B)
+        rel = np.max(np.abs(X - exact)) / np.max(np.abs(exact))
+        print(f"  block solve rel error {rel:.2e}, iterations {it.min()}-{it.max()}", flush=True)
+        assert rel < 2e-3
+    finally:
+        session.set_gemm_mode(mode)
+
+
 def test_engine_product_slots(session, grm1000):
     """GRM products ride along in the engine's GEMM next to live solves."""
     og, S = grm1000
