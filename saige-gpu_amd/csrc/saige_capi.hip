@@ -733,8 +733,16 @@ static int noalloc_scratch_ptr(int dev, double** out) {
     std::lock_guard<std::mutex> lk(g_scratch_mu);
     if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize((size_t)dev + 1, nullptr);
     if (!g_scratch_ptr[(size_t)dev]) {
+        // the symbol's address on device `dev`: looked up with it current
+        // (the calling thread's device is put back afterwards)
+        int prev = -1;
+        if (hipGetDevice(&prev) != hipSuccess || hipSetDevice(dev) != hipSuccess) {
+            set_error("cannot make device %d current for the scratch lookup", dev);
+            return SAIGE_E_DEVICE;
+        }
         void* p = nullptr;
         const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (prev != dev) (void)hipSetDevice(prev);
         if (e != hipSuccess || !p) {
             set_error("hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
             return SAIGE_E_DEVICE;
