@@ -7,7 +7,7 @@ intercept + age + sex covariates (QR-transformed as fitNULLGLMM does);
 four binary phenotypes with prevalences 0.05, 0.1, 0.2, 0.3.  The GRM is
 built once from BF16 standardised genotypes on MFMA with FP32 accumulation
 (a single pass, timed for GRM TFLOP/s, then the split build the fits use:
-each value as two BF16 pieces, three passes) and held in HBM as FP32.
+each value as two BF16 pieces, three products) and held in HBM as FP32.
 
 A step = one batched round of null fits of the four phenotypes through the
 library's C++ drivers (saige_fit_null_batch): AI-REML with nrun = 15
@@ -497,7 +497,7 @@ def main():
     if not args.no_fp64_check:
         G0, miss0 = subblock_genotypes(sess, dev, N, M, n0)
     # GRM TFLOP/s is the single-pass build (N(N+1)Mqc flop of the SYRK's
-    # lower triangle); the split build the fits use runs three such passes
+    # lower triangle); the split build the fits use does three times the MFMA work
     grm_s = sess.build_grm("bf16" if args.grm == "bf16_split" else args.grm)
     grm_tflops = N * (N + 1) * Mqc / grm_s / 1e12
     log(f"GRM {sess.grm_precision()}: {grm_s:.3f}s device time, {grm_tflops:.1f} TFLOP/s")

@@ -47,8 +47,8 @@ extern "C" {
 #define SAIGE_GRM_BF16_SPLIT 3  /* 2-bit genotypes standardised to two BF16 pieces each,
                                    z = hi + lo (lo the rounded remainder, z - hi - lo
                                    within 2^-16 |z|); hi.hi + hi.lo + lo.hi on the BF16
-                                   MFMA with FP32 accumulation, three SYRK passes: an
-                                   FP32-accurate S at three times the BF16 build      */
+                                   MFMA with FP32 accumulation in one SYRK pass: an
+                                   FP32-accurate S at about three times the BF16 build */
 
 typedef struct saige_ctx saige_ctx;
 

@@ -322,12 +322,8 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 // never alias and can interleave them; the loop body is one basic block (the
 // last step decodes a clamped copy of the last chunk that nothing reads, and
 // diagonal tiles, both panels the same, decode their B copy too).
-// TWO: the B panel is decoded through its own table lut64b (the split
-// GRM's cross terms hi x lo and lo x hi); otherwise both use lut64.
-template <bool TWO>
 __global__ __launch_bounds__(512) void k_syrk_pk256(const uint8_t* __restrict__ tp, int64_t nchunk, int64_t c0,
                                                     int64_t c1, const uint64_t* __restrict__ lut64,
-                                                    const uint64_t* __restrict__ lut64b,
                                                     const int2* __restrict__ tiles, int64_t nblk,
                                                     float* __restrict__ S, int64_t ldS, int accumulate) {
     __shared__ __attribute__((aligned(16))) __hip_bfloat16 sA0[BT2 * P16];
@@ -345,7 +341,6 @@ __global__ __launch_bounds__(512) void k_syrk_pk256(const uint8_t* __restrict__ 
     const uint64_t* gA = reinterpret_cast<const uint64_t*>(tp + (pa * nchunk * 32 + (b & 31)) * 64 + 8 * o);
     const uint64_t* gB = reinterpret_cast<const uint64_t*>(tp + (pb * nchunk * 32 + (b & 31)) * 64 + 8 * o);
     const uint64_t* gl = lut64 + 8 * o;
-    const uint64_t* glb = (TWO ? lut64b : lut64) + 8 * o;
 
     f32x16 acc[2][4];
 #pragma unroll
@@ -355,16 +350,12 @@ __global__ __launch_bounds__(512) void k_syrk_pk256(const uint8_t* __restrict__ 
 #pragma unroll
             for (int r = 0; r < 16; r++) acc[mi][ni][r] = 0.0f;
 
-    uint64_t ra, rb, la[8], lb[TWO ? 8 : 1];
+    uint64_t ra, rb, la[8];
     auto fetch = [&](int64_t kc) {
         ra = gA[kc * 256];
         rb = gB[kc * 256];
 #pragma unroll
         for (int e = 0; e < 8; e++) la[e] = gl[kc * 64 + e];
-        if (TWO) {
-#pragma unroll
-            for (int e = 0; e < 8; e++) lb[e] = glb[kc * 64 + e];
-        }
     };
     // a table's low and high bytes of its BF16 entries
     auto split_bytes = [&](const uint64_t (&t)[8], uint32_t (&lo)[8], uint32_t (&hi)[8]) {
@@ -375,25 +366,11 @@ __global__ __launch_bounds__(512) void k_syrk_pk256(const uint8_t* __restrict__ 
             hi[e] = __builtin_amdgcn_perm(h, l, 0x07050301u);
         }
     };
-    // decode the chunk held in ra/rb/la(/lb) into (dA, dB), fetching chunk kn
+    // decode the chunk held in ra/rb/la into (dA, dB), fetching chunk kn
     auto decode_next = [&](__hip_bfloat16* dA, __hip_bfloat16* dB, int64_t kn) {
         const uint64_t ca = ra, cb = rb;
         uint32_t lo[8], hi[8];
         split_bytes(la, lo, hi);
-        if (TWO) {
-            uint32_t lo2[8], hi2[8];
-            uint64_t tb[8];
-#pragma unroll
-            for (int e = 0; e < 8; e++) tb[e] = lb[e];
-            split_bytes(tb, lo2, hi2);
-            fetch(kn);
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                *reinterpret_cast<u32x4*>(dA + (4 * b + j) * P16 + 8 * o) = decode_row_perm(lo, hi, ca, j);
-                *reinterpret_cast<u32x4*>(dB + (4 * b + j) * P16 + 8 * o) = decode_row_perm(lo2, hi2, cb, j);
-            }
-            return;
-        }
         fetch(kn);
 #pragma unroll
         for (int j = 0; j < 4; j++) {
@@ -441,6 +418,129 @@ __global__ __launch_bounds__(512) void k_syrk_pk256(const uint8_t* __restrict__ 
         step(sA1, sB1, sA0, sB0, kc + 1);
     }
     if (kc < c1) multiply(sA0, sB0);
+    store_tile256(S, ldS, bi, bj, wr, wc, lane, acc, accumulate);
+}
+
+// The split GRM in one pass: per 32-marker chunk each thread decodes its
+// panel's bytes through both tables, hi and lo, and every wave adds
+// hi.hi, hi.lo and lo.hi of its 64 x 128 block in that order (three MFMAs
+// per operand pair into one accumulator).  Against three separate passes
+// this reads the genotypes and the tables once, keeps a single set of
+// accumulators and stores S once.  LDS: two buffers of the four decoded
+// operands (A hi, A lo, B hi, B lo; 256 samples x 32 markers each, pitch 40
+// for conflict-free 16-byte reads) = 160 KiB.  Threads 0-255 decode the row
+// panel, 256-511 the column panel: thread (o, b) holds marker octet o of the
+// chunk and byte row b (four samples).
+constexpr int BKS = 32;            // markers per chunk
+constexpr int PS = BKS + 8;        // LDS row pitch (bf16 elements)
+
+__global__ __launch_bounds__(512) void k_syrk_pk256_split(const uint8_t* __restrict__ tp, int64_t nchunk,
+                                                          int64_t k0, int64_t k1,
+                                                          const uint64_t* __restrict__ lhi,
+                                                          const uint64_t* __restrict__ llo,
+                                                          const int2* __restrict__ tiles, int64_t nblk,
+                                                          float* __restrict__ S, int64_t ldS, int accumulate) {
+    __shared__ __attribute__((aligned(16))) __hip_bfloat16 s0[4 * BT2 * PS];   // [A hi | A lo | B hi | B lo]
+    __shared__ __attribute__((aligned(16))) __hip_bfloat16 s1[4 * BT2 * PS];
+    const int2 t = tiles[xcd_remap(blockIdx.x, nblk)];
+    const int64_t bi = t.x, bj = t.y;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wr = wv >> 1, wc = wv & 1;
+    const int r = tid >> 8, o = tid & 3, b = (tid & 255) >> 2;   // panel (0 rows, 1 columns), octet, byte row
+    const int64_t pn = 2 * (r ? bj : bi) + (b >> 5);
+    // uint64 units: 64-marker chunk c is 256 words on from chunk c-1, its
+    // second 32 markers 4 words into a byte row
+    const uint64_t* g = reinterpret_cast<const uint64_t*>(tp + (pn * nchunk * 32 + (b & 31)) * 64 + 8 * o);
+    const uint64_t* gh = lhi + 8 * o;
+    const uint64_t* gl = llo + 8 * o;
+    const int dst = (2 * r) * BT2 * PS + (4 * b) * PS + 8 * o;   // this thread's hi rows; lo rows BT2*PS on
+
+    f32x16 acc[2][4];
+#pragma unroll
+    for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+        for (int ni = 0; ni < 4; ni++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) acc[mi][ni][q] = 0.0f;
+
+    uint64_t raw, th[8], tl[8];
+    auto fetch = [&](int64_t kc) {
+        raw = g[(kc >> 1) * 256 + (kc & 1) * 4];
+#pragma unroll
+        for (int e = 0; e < 8; e++) th[e] = gh[kc * 32 + e];
+#pragma unroll
+        for (int e = 0; e < 8; e++) tl[e] = gl[kc * 32 + e];
+    };
+    auto split_bytes = [&](const uint64_t (&tb)[8], uint32_t (&lo)[8], uint32_t (&hi)[8]) {
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const uint32_t l = (uint32_t)tb[e], h = (uint32_t)(tb[e] >> 32);
+            lo[e] = __builtin_amdgcn_perm(h, l, 0x06040200u);
+            hi[e] = __builtin_amdgcn_perm(h, l, 0x07050301u);
+        }
+    };
+    // decode the chunk held in raw/th/tl into buffer d, fetching chunk kn
+    auto decode_next = [&](__hip_bfloat16* d, int64_t kn) {
+        const uint64_t c = raw;
+        uint32_t hlo[8], hhi[8], llo8[8], lhi8[8];
+        split_bytes(th, hlo, hhi);
+        split_bytes(tl, llo8, lhi8);
+        fetch(kn);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            *reinterpret_cast<u32x4*>(d + dst + j * PS) = decode_row_perm(hlo, hhi, c, j);
+            *reinterpret_cast<u32x4*>(d + dst + BT2 * PS + j * PS) = decode_row_perm(llo8, lhi8, c, j);
+        }
+    };
+    auto multiply = [&](const __hip_bfloat16* d) {
+        const __hip_bfloat16* Ah = d;
+        const __hip_bfloat16* Al = d + BT2 * PS;
+        const __hip_bfloat16* Bh = d + 2 * BT2 * PS;
+        const __hip_bfloat16* Bl = d + 3 * BT2 * PS;
+#pragma unroll
+        for (int ks = 0; ks < BKS / 16; ks++) {
+            const int kk = ks * 16 + (lane >> 5) * 8;
+            bf16x8 fah[2], fal[2];
+#pragma unroll
+            for (int mi = 0; mi < 2; mi++) {
+                const int row = (wr * 64 + mi * 32 + (lane & 31)) * PS + kk;
+                fah[mi] = *reinterpret_cast<const bf16x8*>(Ah + row);
+                fal[mi] = *reinterpret_cast<const bf16x8*>(Al + row);
+            }
+#pragma unroll
+            for (int ni = 0; ni < 4; ni++) {
+                const int col = (wc * 128 + ni * 32 + (lane & 31)) * PS + kk;
+                const bf16x8 fbh = *reinterpret_cast<const bf16x8*>(Bh + col);
+                const bf16x8 fbl = *reinterpret_cast<const bf16x8*>(Bl + col);
+#pragma unroll
+                for (int mi = 0; mi < 2; mi++) {
+                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fah[mi], fbh, acc[mi][ni], 0, 0, 0);
+                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fah[mi], fbl, acc[mi][ni], 0, 0, 0);
+                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fal[mi], fbh, acc[mi][ni], 0, 0, 0);
+                }
+            }
+        }
+    };
+    // multiply chunk kc from cur while decoding chunk kc+1 into nxt
+    auto step = [&](const __hip_bfloat16* cur, __hip_bfloat16* nxt, int64_t kc) {
+        decode_next(nxt, kc + 2 < k1 ? kc + 2 : k1 - 1);
+        multiply(cur);
+#pragma unroll
+        for (int i = 0; i < 48; i++) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
+        }
+        lds_barrier();
+    };
+    fetch(k0);
+    decode_next(s0, k0 + 1 < k1 ? k0 + 1 : k0);
+    lds_barrier();
+    int64_t kc = k0;
+    for (; kc + 2 <= k1; kc += 2) {
+        step(s0, s1, kc);
+        step(s1, s0, kc + 1);
+    }
+    if (kc < k1) multiply(s0);
     store_tile256(S, ldS, bi, bj, wr, wc, lane, acc, accumulate);
 }
 
@@ -493,7 +593,7 @@ __global__ void k_lut64(const float4* __restrict__ lut, int64_t j0, int64_t j1, 
 
 // Copy each diagonal tile's lower half onto its upper half: S[c][r] =
 // S[r][c] for r > c inside tile t (rows and columns t*T + [0, T)).  The split
-// GRM's cross passes add hi.lo and lo.hi to the two mirror entries of a
+// GRM's cross terms add hi.lo and lo.hi to the two mirror entries of a
 // diagonal tile in opposite orders; this keeps S exactly symmetric, with the
 // lower triangle's values, as off-diagonal tiles are mirrored.  Block (t, r):
 // row r of tile t; its threads write column r of the upper part.
@@ -608,26 +708,36 @@ hipError_t launch_lut64_range(hipStream_t s, const float4* lut, int64_t Mqc, int
     return hipGetLastError();
 }
 
-hipError_t launch_syrk_pk(hipStream_t s, const uint8_t* tp, const uint64_t* lut64, const uint64_t* lut64b,
-                          int64_t Mqc, int64_t Npad, const int2* tiles, float* S, int accumulate, int64_t j0,
-                          int64_t j1, int tile) {
+hipError_t launch_syrk_pk(hipStream_t s, const uint8_t* tp, const uint64_t* lut64, int64_t Mqc, int64_t Npad,
+                          const int2* tiles, float* S, int accumulate, int64_t j0, int64_t j1, int tile) {
     if ((tile != BT && tile != BT2) || Npad % tile != 0) return hipErrorInvalidValue;
-    if (lut64b && lut64b != lut64 && tile != BT2) return hipErrorInvalidValue;   // two tables: 256 tiles only
     const int64_t nt = Npad / tile;
     const int64_t nblk = nt * (nt + 1) / 2;
     const int64_t nchunk = (Mqc + 63) / 64;
     if (j1 < 0) j1 = Mqc;
     const int64_t c0 = j0 / 64, c1 = (j1 + 63) / 64;
     if (c0 >= c1) return hipSuccess;
-    if (tile == BT2 && lut64b && lut64b != lut64)
-        hipLaunchKernelGGL(k_syrk_pk256<true>, dim3((unsigned)nblk), dim3(512), 0, s, tp, nchunk, c0, c1, lut64,
-                           lut64b, tiles, nblk, S, Npad, accumulate);
-    else if (tile == BT2)
-        hipLaunchKernelGGL(k_syrk_pk256<false>, dim3((unsigned)nblk), dim3(512), 0, s, tp, nchunk, c0, c1, lut64,
-                           lut64, tiles, nblk, S, Npad, accumulate);
+    if (tile == BT2)
+        hipLaunchKernelGGL(k_syrk_pk256, dim3((unsigned)nblk), dim3(512), 0, s, tp, nchunk, c0, c1, lut64, tiles,
+                           nblk, S, Npad, accumulate);
     else
         hipLaunchKernelGGL(k_syrk_pk, dim3((unsigned)nblk), dim3(256), 0, s, tp, nchunk, c0, c1, lut64, tiles,
                            nblk, S, Npad, accumulate);
+    return hipGetLastError();
+}
+
+hipError_t launch_syrk_pk_split(hipStream_t s, const uint8_t* tp, const uint64_t* lhi, const uint64_t* llo,
+                                int64_t Mqc, int64_t Npad, const int2* tiles, float* S, int accumulate, int64_t j0,
+                                int64_t j1) {
+    if (Npad % BT2 != 0 || !lhi || !llo) return hipErrorInvalidValue;
+    const int64_t nt = Npad / BT2;
+    const int64_t nblk = nt * (nt + 1) / 2;
+    const int64_t nchunk = (Mqc + 63) / 64;
+    if (j1 < 0) j1 = Mqc;
+    const int64_t k0 = j0 / BKS, k1 = (j1 + BKS - 1) / BKS;
+    if (k0 >= k1) return hipSuccess;
+    hipLaunchKernelGGL(k_syrk_pk256_split, dim3((unsigned)nblk), dim3(512), 0, s, tp, nchunk, k0, k1, lhi, llo,
+                       tiles, nblk, S, Npad, accumulate);
     return hipGetLastError();
 }
 

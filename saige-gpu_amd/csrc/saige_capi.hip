@@ -1005,18 +1005,14 @@ int saige_get_diag_of_kin(saige_ctx* c, float* out) {
 }
 
 // The 2-bit-input SYRK over markers [j0, j1) into S: one pass with the BF16
-// table (hi), or, for the split GRM, hi x hi, hi x lo and lo x hi, the last
-// two accumulated onto the first (each pass rounds its tile sums into S).
-// accumulate: 0 store, +1 add, -1 subtract.
+// table (hi), or, for the split GRM (llo given), hi x hi + hi x lo + lo x hi
+// in one pass over both tables.  accumulate: 0 store, +1 add, -1 subtract.
 static hipError_t syrk_pk_passes(hipStream_t st, const uint8_t* tp, const uint64_t* lhi, const uint64_t* llo,
                                  int64_t Mqc, int64_t Np, const int2* tiles, float* S, int accumulate, int64_t j0,
                                  int64_t j1, int tile) {
-    hipError_t e = launch_syrk_pk(st, tp, lhi, nullptr, Mqc, Np, tiles, S, accumulate, j0, j1, tile);
-    if (e != hipSuccess || !llo) return e;
-    const int acc2 = accumulate == 0 ? 1 : accumulate;
-    e = launch_syrk_pk(st, tp, lhi, llo, Mqc, Np, tiles, S, acc2, j0, j1, tile);
-    if (e == hipSuccess) e = launch_syrk_pk(st, tp, llo, lhi, Mqc, Np, tiles, S, acc2, j0, j1, tile);
-    // the cross passes reach a diagonal tile's mirror entries in opposite
+    if (!llo) return launch_syrk_pk(st, tp, lhi, Mqc, Np, tiles, S, accumulate, j0, j1, tile);
+    hipError_t e = launch_syrk_pk_split(st, tp, lhi, llo, Mqc, Np, tiles, S, accumulate, j0, j1);
+    // the cross terms reach a diagonal tile's mirror entries in opposite
     // orders: keep S exactly symmetric
     if (e == hipSuccess) e = launch_mirror_diag(st, S, Np, tile);
     return e;
@@ -1054,7 +1050,7 @@ int saige_build_grm(saige_ctx* c, int precision) {
         uint8_t* d_tp = nullptr;
         uint64_t *d_l64 = nullptr, *d_llo = nullptr;
         int2* d_tiles = nullptr;
-        const int tile = split ? 256 : syrk_pk_tile();   // two tables: the 256 tile only
+        const int tile = split ? 256 : syrk_pk_tile();   // the split kernel: 256 tiles only
         const std::vector<int2> order = syrk_tile_order(Np / tile, tile == 256 ? 4 : 8);
         auto release = [&]() { dfree(d_tp); dfree(d_l64); dfree(d_llo); dfree(d_tiles); };
         if (dalloc(&d_tp, (Np / syrk_tile()) * nchunk * 2048) || dalloc(&d_l64, nchunk * 64) ||

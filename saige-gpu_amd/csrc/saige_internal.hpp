@@ -266,12 +266,16 @@ hipError_t launch_pack_tiles(hipStream_t s, const uint8_t* packed, const float4*
 // markers [j0, j1) only (j1 < 0: all); lut64 must be zero outside the range.
 // accumulate: 0 overwrite, 1 add, -1 subtract.
 // tile: 128 or 256 (tiles from syrk_tile_order(Npad / tile, ...)).
-hipError_t launch_syrk_pk(hipStream_t s, const uint8_t* tp, const uint64_t* lut64, const uint64_t* lut64b,
-                          int64_t Mqc, int64_t Npad, const int2* tiles, float* S, int accumulate, int64_t j0,
-                          int64_t j1, int tile);
+hipError_t launch_syrk_pk(hipStream_t s, const uint8_t* tp, const uint64_t* lut64, int64_t Mqc, int64_t Npad,
+                          const int2* tiles, float* S, int accumulate, int64_t j0, int64_t j1, int tile);
 int syrk_pk_tile();
 // copy each diagonal tile's lower half onto its upper half (tile edge `tile`)
 hipError_t launch_mirror_diag(hipStream_t s, float* S, int64_t Npad, int tile);
+// the split GRM in one pass over markers [j0, j1): hi.hi + hi.lo + lo.hi per
+// 32-marker chunk from the two tables (256 tiles; accumulate as launch_syrk_pk)
+hipError_t launch_syrk_pk_split(hipStream_t s, const uint8_t* tp, const uint64_t* lhi, const uint64_t* llo,
+                                int64_t Mqc, int64_t Npad, const int2* tiles, float* S, int accumulate, int64_t j0,
+                                int64_t j1);
 // piece 0: the values rounded to BF16 (hi); 1: the remainders rounded to BF16 (lo)
 hipError_t launch_lut64_range(hipStream_t s, const float4* lut, int64_t Mqc, int64_t j0, int64_t j1, int piece,
                               uint64_t* lut64);

@@ -8,8 +8,8 @@ and HBM counters (FETCH_SIZE, WRITE_SIZE) of each can be read per dispatch:
     rocprofv3 --pmc WRITE_SIZE -d OUT -o write -- python3 saige-gpu_amd/profile_kernels.py
 
 Kernels exercised: decode/QC (k_decode_stats), repack, diag, the GRM SYRK
-(2-bit input k_syrk_pk256 for --grm bf16; three passes of it, two with the
-remainder table, and k_mirror_diag for bf16_split; --grm takes a
+(2-bit input k_syrk_pk256 for --grm bf16; k_syrk_pk256_split, hi.hi + hi.lo
++ lo.hi from two tables, and k_mirror_diag for bf16_split; --grm takes a
 comma-separated list, built in turn, the last one resident), the
 autotuner's sixteen block-GEMM
 shapes, the tuned 64-column block GEMM (k_split_p + k_gemm_x3t + k_tri_reduce
