@@ -37,11 +37,11 @@ FP32_MFMA_PEAK = 157.3
 # (mode, N, rt, pd, nsplit).  Sources: PROFILE_FILES
 PROFILED_GEMM_TRAFFIC = {("fp32", 100_000, 1, 3, 8): (42.10 + 0.205 + 0.108 + 0.026) * 1e9,
                          ("bf16x3", 100_000, 1, 3, 8): (0.064 + 41.579 + 0.152) * 1e9,
-                         # lower triangle, eight waves (pd has no effect there): all four kernels of a launch
-                         ("bf16x3_tri", 100_000, 2, 1, 8): (31.85 + 5.61) * 1e9,
-                         ("bf16x3_tri", 100_000, 2, 3, 8): (31.85 + 5.61) * 1e9}
+                         # lower triangle, eight waves: all four kernels of a launch (pd 3 spills)
+                         ("bf16x3_tri", 100_000, 2, 1, 8): (26.69 + 5.61) * 1e9,
+                         ("bf16x3_tri", 100_000, 2, 3, 8): (36.89 + 7.75) * 1e9}
 PROFILE_FILES = {"fp32": "profiles/kernels_N100k_M150k_bf16.md", "bf16x3": "profiles/gemm_bf16x3_N100k_M150k.md",
-                 "bf16x3_tri": "profiles/gemm_bf16x3_tri_N100k_M150k.md"}
+                 "bf16x3_tri": "profiles/kernels_N100k_M150k_split_grm.md"}
 BF16_MFMA_PEAK = 2500.0
 
 
