@@ -4,16 +4,106 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <condition_variable>
+#include <cstdlib>
+#include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <numeric>
+#include <thread>
 #include <tuple>
 
 namespace saige {
 namespace fit {
 
 namespace {
+
+// ------------------------------------------------------------- host pieces
+
+// The O(n) loops of a fit (the working response, the coefficients after a
+// solve, glm()'s IRLS) run as kPieces fixed pieces of the samples when n is
+// at least kPieceMin: the pieces go to a small pool of host threads and the
+// calling thread, and sums add the pieces' partial sums in piece order, so a
+// result depends on n only, not on which thread ran a piece or how many the
+// pool has (a fit's bits are the same alone and beside other fits).
+constexpr int kPieces = 4;
+
+// n from which a loop is cut: 32768, or SAIGE_HOST_PIECE_MIN (tests lower it
+// to run the pieces on small problems)
+int64_t piece_min() {
+    static const int64_t v = [] {
+        const char* e = std::getenv("SAIGE_HOST_PIECE_MIN");
+        const long long x = e ? std::atoll(e) : 0;
+        return x > 0 ? (int64_t)x : (int64_t)32768;
+    }();
+    return v;
+}
+
+int pieces_for(int64_t n) { return n >= piece_min() ? kPieces : 1; }
+int64_t piece_begin(int64_t n, int c, int C) { return n * c / C; }
+
+class HostPool {
+public:
+    static HostPool& get() {
+        static HostPool* pool = new HostPool();   // never destroyed: its threads outlive static destructors
+        return *pool;
+    }
+    // fn(c) for c = 0..C-1; piece 0 on the calling thread
+    void run(int C, const std::function<void(int)>& fn) {
+        if (C <= 1 || nthreads_ == 0) {
+            for (int c = 0; c < C; c++) fn(c);
+            return;
+        }
+        std::mutex m;
+        std::condition_variable cv;
+        int left = C - 1;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            for (int c = 1; c < C; c++)
+                queue_.push_back([&, c] {
+                    fn(c);
+                    std::lock_guard<std::mutex> l2(m);
+                    if (--left == 0) cv.notify_one();
+                });
+        }
+        cv_.notify_all();
+        fn(0);
+        std::unique_lock<std::mutex> lk(m);
+        cv.wait(lk, [&] { return left == 0; });
+    }
+
+private:
+    HostPool() {
+        const unsigned hw = std::thread::hardware_concurrency();
+        nthreads_ = hw > 4 ? std::min(12u, hw - 4) : 0u;
+        for (unsigned t = 0; t < nthreads_; t++) std::thread([this] { work(); }).detach();
+    }
+    void work() {
+        for (;;) {
+            std::function<void()> f;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return !queue_.empty(); });
+                f = std::move(queue_.front());
+                queue_.pop_front();
+            }
+            f();
+        }
+    }
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<std::function<void()>> queue_;
+    unsigned nthreads_ = 0;
+};
+
+// fn(i0, i1, c) over the pieces [i0, i1) of [0, n)
+template <class F>
+void for_pieces(int64_t n, F fn) {
+    const int C = pieces_for(n);
+    HostPool::get().run(C, [&](int c) { fn(piece_begin(n, c, C), piece_begin(n, c + 1, C), c); });
+}
 
 // ------------------------------------------------------------ dense algebra
 
@@ -604,14 +694,16 @@ void normal_equations(const double* const* xc, int64_t p, int64_t i0, int64_t le
 void glm_working(Trait trait, int64_t n, const double* y, const double* eta, const double* offset, double* mu,
                  double* Y, double* W) {
     const Family fam{trait == Trait::Binary};
-    // W is the exponentials' scratch and Y holds mu_eta until it is read
-    fam.link_both(eta, n, mu, Y, W);
-    for (int64_t i = 0; i < n; i++) {
-        const double m = mu[i], me = Y[i];
-        Y[i] = eta[i] - (offset ? offset[i] : 0.0) + (y[i] - m) / me;
-        const double r = me / std::sqrt(fam.variance(m));
-        W[i] = r * r;
-    }
+    for_pieces(n, [&](int64_t i0, int64_t i1, int) {
+        // W is the exponentials' scratch and Y holds mu_eta until it is read
+        fam.link_both(eta + i0, i1 - i0, mu + i0, Y + i0, W + i0);
+        for (int64_t i = i0; i < i1; i++) {
+            const double m = mu[i], me = Y[i];
+            Y[i] = eta[i] - (offset ? offset[i] : 0.0) + (y[i] - m) / me;
+            const double r = me / std::sqrt(fam.variance(m));
+            W[i] = r * r;
+        }
+    });
 }
 
 namespace {
@@ -620,12 +712,12 @@ namespace {
 // columns) in one pass; P > 0 is p fixed at compile time, so the sums stay in
 // registers, else one dot per entry
 template <int P>
-void cross_products(const double* X, const double* S, const double* Y, int64_t n, int64_t p, Matrix& A,
-                    std::vector<double>& b) {
+void cross_products(const double* X, const double* S, const double* Y, int64_t n, int64_t p, int64_t i0, int64_t i1,
+                    Matrix& A, std::vector<double>& b) {
     if (P > 0) {
         constexpr int Q = P > 0 ? P : 1;
         double a[Q][Q] = {}, bb[Q] = {};
-        for (int64_t i = 0; i < n; i++) {
+        for (int64_t i = i0; i < i1; i++) {
             double x[Q], sv[Q];
             for (int j = 0; j < Q; j++) {
                 x[j] = X[j * n + i];
@@ -643,8 +735,8 @@ void cross_products(const double* X, const double* S, const double* Y, int64_t n
         return;
     }
     for (int64_t j = 0; j < p; j++) {
-        b[(size_t)j] = dot(S + j * n, Y, n);
-        for (int64_t k = 0; k < p; k++) A(j, k) = dot(X + j * n, S + k * n, n);
+        b[(size_t)j] = dot(S + j * n + i0, Y + i0, i1 - i0);
+        for (int64_t k = 0; k < p; k++) A(j, k) = dot(X + j * n + i0, S + k * n + i0, i1 - i0);
     }
 }
 
@@ -654,15 +746,27 @@ void coefficients_from_solution(int64_t n, int64_t p, const double* Y, const dou
                                 const double* sol, double* cov, double* alpha, double* eta) {
     const double* siY = sol;
     const double* siX = sol + n;
-    // X^T Sigma^-1 X and (Sigma^-1 X)^T Y
-    Matrix A(p, p);
-    std::vector<double> sy((size_t)p);
-    switch (p) {
-        case 1: cross_products<1>(X, siX, Y, n, p, A, sy); break;
-        case 2: cross_products<2>(X, siX, Y, n, p, A, sy); break;
-        case 3: cross_products<3>(X, siX, Y, n, p, A, sy); break;
-        case 4: cross_products<4>(X, siX, Y, n, p, A, sy); break;
-        default: cross_products<0>(X, siX, Y, n, p, A, sy);
+    // X^T Sigma^-1 X and (Sigma^-1 X)^T Y, per piece of the samples, the
+    // pieces' sums added in order
+    const int P = pieces_for(n);
+    std::vector<Matrix> Ap((size_t)P, Matrix(p, p));
+    std::vector<std::vector<double>> syp((size_t)P, std::vector<double>((size_t)p));
+    for_pieces(n, [&](int64_t i0, int64_t i1, int c) {
+        Matrix& Ac = Ap[(size_t)c];
+        std::vector<double>& sc = syp[(size_t)c];
+        switch (p) {
+            case 1: cross_products<1>(X, siX, Y, n, p, i0, i1, Ac, sc); break;
+            case 2: cross_products<2>(X, siX, Y, n, p, i0, i1, Ac, sc); break;
+            case 3: cross_products<3>(X, siX, Y, n, p, i0, i1, Ac, sc); break;
+            case 4: cross_products<4>(X, siX, Y, n, p, i0, i1, Ac, sc); break;
+            default: cross_products<0>(X, siX, Y, n, p, i0, i1, Ac, sc);
+        }
+    });
+    Matrix A = Ap[0];
+    std::vector<double> sy = syp[0];
+    for (int c = 1; c < P; c++) {
+        for (size_t k = 0; k < A.a.size(); k++) A.a[k] += Ap[(size_t)c].a[k];
+        for (int64_t j = 0; j < p; j++) sy[(size_t)j] += syp[(size_t)c][(size_t)j];
     }
     const Matrix C = inv_sympd(A);
     for (int64_t j = 0; j < p; j++) {
@@ -671,11 +775,13 @@ void coefficients_from_solution(int64_t n, int64_t p, const double* Y, const dou
         alpha[j] = a;
     }
     std::copy(C.a.begin(), C.a.end(), cov);
-    for (int64_t i = 0; i < n; i++) {
-        double sxa = 0.0;
-        for (int64_t k = 0; k < p; k++) sxa += siX[k * n + i] * alpha[k];
-        eta[i] = Y[i] - tau0 * (siY[i] - sxa) / w[i];
-    }
+    for_pieces(n, [&](int64_t i0, int64_t i1, int) {
+        for (int64_t i = i0; i < i1; i++) {
+            double sxa = 0.0;
+            for (int64_t k = 0; k < p; k++) sxa += siX[k * n + i] * alpha[k];
+            eta[i] = Y[i] - tau0 * (siY[i] - sxa) / w[i];
+        }
+    });
 }
 
 GlmStart glm_start(Trait trait, const std::vector<double>& y, const Matrix& X, const std::vector<double>& offset,
@@ -695,44 +801,59 @@ GlmStart glm_start(Trait trait, const std::vector<double>& y, const Matrix& X, c
         }
     }
     if (fam.binomial) vmath::log(eta.data(), eta.data(), n);
-    // the samples go in tiles of T, so each pass's temporaries stay in cache;
-    // me_all holds mu_eta at the current eta, made with mu from one exponential
+    // the samples go in pieces (one per thread) and, within a piece, in tiles
+    // of T, so each pass's temporaries stay in cache; me_all holds mu_eta at
+    // the current eta, made with mu from one exponential
     constexpr int64_t T = 2048;
-    std::vector<double> tmp((size_t)(3 * T)), me_all((size_t)n);
-    double* ex = tmp.data();
-    double* z = ex + T;
-    double* wt = z + T;
-    for (int64_t i0 = 0; i0 < n; i0 += T) fam.mu_eta(eta.data() + i0, std::min(T, n - i0), me_all.data() + i0, ex);
+    const int P = pieces_for(n);
+    std::vector<std::vector<double>> tmp((size_t)P, std::vector<double>((size_t)(3 * T)));
+    std::vector<double> me_all((size_t)n);
+    for_pieces(n, [&](int64_t p0, int64_t p1, int c) {
+        double* ex = tmp[(size_t)c].data();
+        for (int64_t i0 = p0; i0 < p1; i0 += T)
+            fam.mu_eta(eta.data() + i0, std::min(T, p1 - i0), me_all.data() + i0, ex);
+    });
+    // per piece: four interleaved partial sums (binary y) or one, then the
+    // pieces in order
     auto deviance = [&](const double* m) {
-        if (binary01) {
-            // y in {0, 1}: the unit deviance is -2 log(mu) or -2 log(1 - mu);
-            // y m + (1 - y)(1 - m) is m or 1 - m exactly
-            double d4[4] = {0.0, 0.0, 0.0, 0.0};
-            for (int64_t i0 = 0; i0 < n; i0 += T) {
-                const int64_t len = std::min(T, n - i0);
-                for (int64_t k = 0; k < len; k++) {
-                    const double yi = y[(size_t)(i0 + k)], mi = m[i0 + k];
-                    ex[k] = yi * mi + (1.0 - yi) * (1.0 - mi);
+        std::vector<double> part((size_t)P, 0.0);
+        for_pieces(n, [&](int64_t p0, int64_t p1, int c) {
+            double* ex = tmp[(size_t)c].data();
+            if (binary01) {
+                // y in {0, 1}: the unit deviance is -2 log(mu) or -2 log(1 - mu);
+                // y m + (1 - y)(1 - m) is m or 1 - m exactly
+                double d4[4] = {0.0, 0.0, 0.0, 0.0};
+                for (int64_t i0 = p0; i0 < p1; i0 += T) {
+                    const int64_t len = std::min(T, p1 - i0);
+                    for (int64_t k = 0; k < len; k++) {
+                        const double yi = y[(size_t)(i0 + k)], mi = m[i0 + k];
+                        ex[k] = yi * mi + (1.0 - yi) * (1.0 - mi);
+                    }
+                    vmath::log(ex, ex, len);
+                    int64_t k = 0;
+                    for (; k + 4 <= len; k += 4)
+                        for (int q = 0; q < 4; q++) d4[q] += ex[k + q];
+                    for (; k < len; k++) d4[0] += ex[k];
                 }
-                vmath::log(ex, ex, len);
-                int64_t k = 0;
-                for (; k + 4 <= len; k += 4)
-                    for (int q = 0; q < 4; q++) d4[q] += ex[k + q];
-                for (; k < len; k++) d4[0] += ex[k];
+                part[(size_t)c] = (d4[0] + d4[1]) + (d4[2] + d4[3]);
+                return;
             }
-            return -2.0 * ((d4[0] + d4[1]) + (d4[2] + d4[3]));
-        }
+            double d = 0.0;
+            for (int64_t i = p0; i < p1; i++) {
+                const double yi = y[(size_t)i], mi = m[i];
+                if (fam.binomial) {
+                    const double a = yi > 0 ? yi * std::log(yi / mi) : 0.0;
+                    const double b = yi < 1 ? (1 - yi) * std::log((1 - yi) / (1 - mi)) : 0.0;
+                    d += a + b;
+                } else {
+                    d += (yi - mi) * (yi - mi);
+                }
+            }
+            part[(size_t)c] = d;
+        });
         double d = 0.0;
-        for (int64_t i = 0; i < n; i++) {
-            const double yi = y[(size_t)i], mi = m[i];
-            if (fam.binomial) {
-                const double a = yi > 0 ? yi * std::log(yi / mi) : 0.0;
-                const double b = yi < 1 ? (1 - yi) * std::log((1 - yi) / (1 - mi)) : 0.0;
-                d += a + b;
-            } else {
-                d += (yi - mi) * (yi - mi);
-            }
-        }
+        for (int c = 0; c < P; c++) d += part[(size_t)c];
+        if (binary01) return -2.0 * d;
         return fam.binomial ? 2.0 * d : d;
     };
     double dev_old = deviance(mu.data());
@@ -741,39 +862,54 @@ GlmStart glm_start(Trait trait, const std::vector<double>& y, const Matrix& X, c
     for (int64_t j = 0; j < p; j++) xc[(size_t)j] = X.col(j);
     for (int it = 0; it < maxit; it++) {
         // per tile: the working response z and weights wt, then the tile's
-        // share of the weighted normal equations X^T W X beta = X^T W z
-        Matrix A(p, p), b(p, 1);
-        for (int64_t i0 = 0; i0 < n; i0 += T) {
-            const int64_t len = std::min(T, n - i0);
-            const double* me = me_all.data() + i0;
-            for (int64_t k = 0; k < len; k++) {
-                const int64_t i = i0 + k;
-                const double mi = mu[(size_t)i];
-                z[k] = (eta[(size_t)i] - (off ? off[i] : 0.0)) + (y[(size_t)i] - mi) / me[k];
-                wt[k] = me[k] * me[k] / fam.variance(mi);
+        // share of the weighted normal equations X^T W X beta = X^T W z; the
+        // pieces' sums added in order
+        std::vector<Matrix> Ap((size_t)P, Matrix(p, p)), bp((size_t)P, Matrix(p, 1));
+        for_pieces(n, [&](int64_t p0, int64_t p1, int c) {
+            double* z = tmp[(size_t)c].data() + T;
+            double* wt = z + T;
+            Matrix& A = Ap[(size_t)c];
+            Matrix& b = bp[(size_t)c];
+            for (int64_t i0 = p0; i0 < p1; i0 += T) {
+                const int64_t len = std::min(T, p1 - i0);
+                const double* me = me_all.data() + i0;
+                for (int64_t k = 0; k < len; k++) {
+                    const int64_t i = i0 + k;
+                    const double mi = mu[(size_t)i];
+                    z[k] = (eta[(size_t)i] - (off ? off[i] : 0.0)) + (y[(size_t)i] - mi) / me[k];
+                    wt[k] = me[k] * me[k] / fam.variance(mi);
+                }
+                switch (p) {
+                    case 1: normal_equations<1>(xc.data(), p, i0, len, wt, z, A, b); break;
+                    case 2: normal_equations<2>(xc.data(), p, i0, len, wt, z, A, b); break;
+                    case 3: normal_equations<3>(xc.data(), p, i0, len, wt, z, A, b); break;
+                    case 4: normal_equations<4>(xc.data(), p, i0, len, wt, z, A, b); break;
+                    default: normal_equations<0>(xc.data(), p, i0, len, wt, z, A, b);
+                }
             }
-            switch (p) {
-                case 1: normal_equations<1>(xc.data(), p, i0, len, wt, z, A, b); break;
-                case 2: normal_equations<2>(xc.data(), p, i0, len, wt, z, A, b); break;
-                case 3: normal_equations<3>(xc.data(), p, i0, len, wt, z, A, b); break;
-                case 4: normal_equations<4>(xc.data(), p, i0, len, wt, z, A, b); break;
-                default: normal_equations<0>(xc.data(), p, i0, len, wt, z, A, b);
-            }
+        });
+        Matrix A = Ap[0], b = bp[0];
+        for (int c = 1; c < P; c++) {
+            for (size_t k = 0; k < A.a.size(); k++) A.a[k] += Ap[(size_t)c].a[k];
+            for (size_t k = 0; k < b.a.size(); k++) b.a[k] += bp[(size_t)c].a[k];
         }
         for (int64_t j = 0; j < p; j++)
             for (int64_t k = 0; k < j; k++) A(k, j) = A(j, k);
         const Matrix sol = solve_dense(A, b);
         beta.assign(sol.a.begin(), sol.a.end());
         // eta = X beta + offset and mu, per tile
-        for (int64_t i0 = 0; i0 < n; i0 += T) {
-            const int64_t len = std::min(T, n - i0);
-            for (int64_t i = i0; i < i0 + len; i++) {
-                double e = off ? off[i] : 0.0;
-                for (int64_t j = 0; j < p; j++) e += xc[(size_t)j][i] * beta[(size_t)j];
-                eta[(size_t)i] = e;
+        for_pieces(n, [&](int64_t p0, int64_t p1, int c) {
+            double* ex = tmp[(size_t)c].data();
+            for (int64_t i0 = p0; i0 < p1; i0 += T) {
+                const int64_t len = std::min(T, p1 - i0);
+                for (int64_t i = i0; i < i0 + len; i++) {
+                    double e = off ? off[i] : 0.0;
+                    for (int64_t j = 0; j < p; j++) e += xc[(size_t)j][i] * beta[(size_t)j];
+                    eta[(size_t)i] = e;
+                }
+                fam.link_both(eta.data() + i0, len, mu.data() + i0, me_all.data() + i0, ex);
             }
-            fam.link_both(eta.data() + i0, len, mu.data() + i0, me_all.data() + i0, ex);
-        }
+        });
         const double dev = deviance(mu.data());
         if (std::fabs(dev - dev_old) / (std::fabs(dev) + 0.1) < epsilon) break;
         dev_old = dev;

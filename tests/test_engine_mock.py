@@ -211,16 +211,20 @@ def test_admission_waits_for_free_memory(lib, cohort):
 @pytest.mark.parametrize("san", ["address,undefined", "thread"])
 def test_scheduler_under_host_sanitizers(tmp_path, san):
     """engine_mock_main.cpp (batch, alone vs mix, server with submitting and
-    polling threads, injected failure) built with g++ -fsanitize=... on the
-    host code only, run to completion with no report."""
+    polling threads, injected failure; the fits' loops in pieces on the host
+    thread pool) built with g++ -fsanitize=... on the host code only, run to
+    completion with no report."""
     exe = str(tmp_path / "mock_main")
     cmd = ["g++", "-O1", "-g", "-std=c++17", "-pthread", f"-fsanitize={san}", "-fno-omit-frame-pointer",
            "-I", CSRC, "-I", INC, "-o", exe, os.path.join(HERE, "native", "engine_mock_main.cpp"),
            os.path.join(HERE, "native", "engine_mock.cpp"), os.path.join(CSRC, "fit_core.cpp"),
            os.path.join(CSRC, "fit_engine.cpp"), os.path.join(CSRC, "vmath.cpp"), "-lmvec", "-lm"]
     subprocess.run(cmd, check=True)
+    # SAIGE_HOST_PIECE_MIN=64: the fits' O(n) loops run as pieces on the
+    # host thread pool even at this size, so the sanitizers watch it too
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=1:verify_asan_link_order=0",
-               UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1", TSAN_OPTIONS="halt_on_error=1")
+               UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1", TSAN_OPTIONS="halt_on_error=1",
+               SAIGE_HOST_PIECE_MIN="64")
     p = subprocess.run([exe], env=env, capture_output=True, text=True, timeout=600)
     assert p.returncode == 0, p.stdout[-3000:] + p.stderr[-6000:]
     assert "all scenarios passed" in p.stdout
