@@ -289,6 +289,12 @@ int saige_dots_device_noalloc(saige_ctx* ctx, const float* A_dev, int64_t lda, i
  * (seed, i, row); FG.cpp:3133-3137 draws them with R's rbinom) into dst_dev,
  * column-major with leading dimension ld >= N. */
 int saige_probes_device(saige_ctx* ctx, float* dst_dev, int64_t ld, uint64_t seed, int64_t i0, int k);
+/* The same probes for a fit on a sample subset: entry i of probe i0 + j
+ * (the hash of the position i in the subset, as for a fit of n samples) goes
+ * to row rows_dev[i] of column j, every other row of the k columns is zero.
+ * rows_dev: n ascending indices into the N samples, on this device. */
+int saige_probes_device_rows(saige_ctx* ctx, float* dst_dev, int64_t ld, uint64_t seed, int64_t i0, int k,
+                             const int64_t* rows_dev, int64_t n);
 /* Host-to-device copy into a buffer from saige_alloc_device. */
 int saige_copy_host_to_device(saige_ctx* ctx, void* dst_dev, const void* src_host, int64_t bytes);
 

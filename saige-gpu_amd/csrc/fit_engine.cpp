@@ -127,6 +127,8 @@ struct TraceBuffers {
     float* U = nullptr;
     float* AU = nullptr;
     float* SX = nullptr;
+    int64_t* rows = nullptr;        // the sample subset of rows_fit (N entries at most)
+    const FitTask* rows_fit = nullptr;
 };
 
 struct Scheduler {
@@ -277,9 +279,11 @@ void run_scheduler(Scheduler& sch, int n_systems) {
         saige_ctx* ctx;
         std::vector<TraceBuffers> b;
         ~Buffers() {
-            for (auto& t : b)
+            for (auto& t : b) {
                 for (float* p : {t.U, t.AU, t.SX})
                     if (p) saige_free_device(ctx, p);
+                if (t.rows) saige_free_device(ctx, t.rows);
+            }
         }
     } tbufs{sch.ctx, std::vector<TraceBuffers>((size_t)n_systems)};
     // fits whose request the last step completed, with their post count then
@@ -328,14 +332,18 @@ void run_scheduler(Scheduler& sch, int n_systems) {
         if (!sp->subset) {
             check(saige_probes_device(sch.ctx, b.U, N, t.seed, t.start, (int)k));
         } else {
-            colbuf.assign((size_t)(N * k), 0.0f);
-            std::vector<double> pv((size_t)n);
-            for (int64_t j = 0; j < k; j++) {
-                saige::fit::probe_vector(t.seed, t.start + j, n, pv.data());
-                float* d = colbuf.data() + j * N;
-                for (int64_t i = 0; i < n; i++) d[sp->subset[i]] = (float)pv[(size_t)i];
+            // the subset's rows go to the device once per fit on this system;
+            // its probes are then made there like the full cohort's
+            if (b.rows_fit != f) {
+                if (!b.rows) {
+                    void* p = nullptr;
+                    check(saige_alloc_device(sch.ctx, 8 * N, &p));
+                    b.rows = (int64_t*)p;
+                }
+                check(saige_copy_host_to_device(sch.ctx, b.rows, sp->subset, 8 * n));
+                b.rows_fit = f;
             }
-            check(saige_copy_host_to_device(sch.ctx, b.U, colbuf.data(), 4 * N * k));
+            check(saige_probes_device_rows(sch.ctx, b.U, N, t.seed, t.start, (int)k, b.rows, n));
         }
         colbuf.assign((size_t)(N * p), 0.0f);
         for (int64_t c = 0; c < p; c++) {
@@ -461,6 +469,7 @@ void run_scheduler(Scheduler& sch, int n_systems) {
                     continue;
                 }
                 free_sys.push_back(f->sys);
+                tbufs.b[(size_t)f->sys].rows_fit = nullptr;   // a later fit may reuse this address
                 f->sys = -1;
                 active.erase(active.begin() + (ptrdiff_t)i);
                 // a released fit may be freed at once by saige_poll: nothing

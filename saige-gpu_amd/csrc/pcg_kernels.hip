@@ -1396,6 +1396,21 @@ __global__ void k_rademacher(float* __restrict__ dst, int64_t ld, int64_t N, uin
     dst[j * ld + r] = (h >> 63) ? -1.0f : 1.0f;
 }
 
+// The probes of a fit on a sample subset: entry i of probe i0 + j (the hash
+// of the position i within the subset, as probe_vector(seed, i0 + j, n))
+// goes to row rows[i] of column j; rows outside [0, N) are skipped.
+__global__ void k_rademacher_rows(float* __restrict__ dst, int64_t ld, int64_t N, const int64_t* __restrict__ rows,
+                                  int64_t n, uint64_t seed, int64_t i0) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t j = blockIdx.y;
+    if (i >= n) return;
+    const int64_t r = rows[i];
+    if (r < 0 || r >= N) return;
+    const uint64_t h = probe_mix(seed ^ probe_mix((uint64_t)(i0 + j) * 0x9E3779B97F4A7C15ull +
+                                                  probe_mix((uint64_t)i * 0xC2B2AE3D27D4EB4Full + 0x5851F42Dull)));
+    dst[j * ld + r] = (h >> 63) ? -1.0f : 1.0f;
+}
+
 // FP64 dot products of k columns x_j with partner columns: P1[:, j], P2[:, j]
 // (pairwise, optional) and Q[:, r] for r < m (shared).  x_j is either a slot
 // of an engine block ([Npad][kpad] row-major, slots != nullptr) or column j
@@ -1811,6 +1826,14 @@ hipError_t launch_fill_zero(hipStream_t s, void* p, int64_t bytes) {
     const int64_t n = bytes / 4;
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_zero, dim3(nblk(n, 256)), dim3(256), 0, s, (uint32_t*)p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_rademacher_rows(hipStream_t s, float* dst, int64_t ld, int64_t N, const int64_t* rows, int64_t n,
+                                  uint64_t seed, int64_t i0, int k) {
+    if (k < 1 || n < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_rademacher_rows, dim3(nblk(n, 256), (unsigned)k), dim3(256), 0, s, dst, ld, N, rows, n,
+                       seed, i0);
     return hipGetLastError();
 }
 

@@ -1749,6 +1749,20 @@ int saige_probes_device(saige_ctx* c, float* dst_dev, int64_t ld, uint64_t seed,
     return SAIGE_OK;
 }
 
+int saige_probes_device_rows(saige_ctx* c, float* dst_dev, int64_t ld, uint64_t seed, int64_t i0, int k,
+                             const int64_t* rows_dev, int64_t n) {
+    CHECK_CTX(c);
+    CHECK_LOADED(c);
+    if (!dst_dev || ld < c->Nnm || k < 1 || i0 < 0 || !rows_dev || n < 1 || n > c->Nnm) {
+        set_error("invalid subset probe request");
+        return SAIGE_E_ARG;
+    }
+    SAIGE_HIP_TRY(launch_fill_zero(c->stream, dst_dev, 4 * ld * k));
+    SAIGE_HIP_TRY(launch_rademacher_rows(c->stream, dst_dev, ld, c->Nnm, rows_dev, n, seed, i0, k));
+    SAIGE_HIP_TRY(hipStreamSynchronize(c->stream));
+    return SAIGE_OK;
+}
+
 int saige_copy_host_to_device(saige_ctx* c, void* dst_dev, const void* src_host, int64_t bytes) {
     CHECK_CTX(c);
     if (!dst_dev || !src_host || bytes < 0) {

@@ -360,6 +360,9 @@ hipError_t launch_fill_zero(hipStream_t s, void* p, int64_t bytes);
 hipError_t launch_fill_hash(hipStream_t s, float* p, int64_t n, uint32_t seed);
 // Rademacher probes i0 .. i0+k-1 into dst (column-major, ld >= N)
 hipError_t launch_rademacher(hipStream_t s, float* dst, int64_t ld, int64_t N, uint64_t seed, int64_t i0, int k);
+// probes of a sample subset: entry i of probe i0 + j to row rows[i] of column j
+hipError_t launch_rademacher_rows(hipStream_t s, float* dst, int64_t ld, int64_t N, const int64_t* rows, int64_t n,
+                                  uint64_t seed, int64_t i0, int k);
 // FP64 column dots (see k_col_dots); out[j * nd + d], nd = [P1] + [P2] + m
 int col_dots_max();
 int64_t col_dots_partial_doubles(int64_t Npad, int k);

@@ -470,6 +470,13 @@ class Session:
         check(self._L.saige_probes_device(self.ctx, self._col_ptr(buf, col0), self.getNnomissingOut(), int(seed),
                                           int(i0), int(k)))
 
+    def probes_device_rows(self, buf, col0, seed, i0, k, rows_buf, n):
+        """Probes i0 .. i0+k-1 of a fit on n samples into columns col0.. of
+        buf, entry i at row rows_buf[i] (a device array of n int64), zero
+        elsewhere."""
+        check(self._L.saige_probes_device_rows(self.ctx, self._col_ptr(buf, col0), self.getNnomissingOut(),
+                                               int(seed), int(i0), int(k), rows_buf, int(n)))
+
     def copy_host_to_device(self, buf, arr, col0=0):
         """Copy a host FP32 array (N or N x k, column-major) to columns col0.."""
         a = np.asarray(arr, np.float32)
@@ -541,6 +548,11 @@ class Session:
     def synth_bed_device(self, dev, N, M, maf_lo, maf_hi, miss_rate, pair_frac, seed):
         check(self._L.saige_synth_bed_device(self.ctx, dev, int(N), int(M), float(maf_lo), float(maf_hi),
                                              float(miss_rate), float(pair_frac), int(seed)))
+
+    def copy_bytes_to_device(self, dst, arr):
+        """Copy a host array's bytes, as they are, to device memory dst."""
+        a = np.ascontiguousarray(arr)
+        check(self._L.saige_copy_host_to_device(self.ctx, dst, ptr(a), a.nbytes))
 
     def copy_device(self, dst, src, nbytes):
         check(self._L.saige_copy_device(self.ctx, dst, src, int(nbytes)))

@@ -852,6 +852,7 @@ class BatchScheduler:
         # taken from the session at the first trace round, so a scheduler that
         # runs none (variance ratios, LOCO) leaves them parked there
         self._devbufs = None
+        self._rows_owner = {}                   # system -> fit whose subset rows its buffer holds
         self.device_trace = hasattr(sess, "engine_take_batch_ex")
 
     # ---------------------------------------------------------------- API
@@ -1055,6 +1056,14 @@ class BatchScheduler:
         sub = f.subset if f.subset is not None else slice(None)
         if f.subset is None and isinstance(probes, RademacherProbes) and probes.n == self.N:
             self.sess.probes_device(bufs["U"], 0, probes.seed, tr["start"], k)
+        elif isinstance(probes, RademacherProbes) and probes.n == len(f.subset):
+            # the subset's rows go to the device once per fit on this system
+            if "rows" not in bufs:
+                bufs["rows"] = self.sess.alloc_device(8 * self.N)
+            if self._rows_owner.get(f.sys) != f.fid:
+                self.sess.copy_bytes_to_device(bufs["rows"], f.subset)
+                self._rows_owner[f.sys] = f.fid
+            self.sess.probes_device_rows(bufs["U"], 0, probes.seed, tr["start"], k, bufs["rows"], len(f.subset))
         else:
             Uf = np.zeros((self.N, k), np.float32, order="F")
             Uf[sub] = probe_block(probes, tr["start"], tr["end"])

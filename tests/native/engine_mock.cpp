@@ -14,6 +14,7 @@
 // the real engine, so a fit's result is the same alone and in any mix.
 // Hooks (extern "C", mock_*) set K, fail a chosen step, and set the free
 // memory the admission check sees.
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdarg>
@@ -355,6 +356,19 @@ int saige_dots_device(saige_ctx* c, const float* A, int64_t lda, int k, const fl
             for (int64_t i = 0; i < c->N; i++) s += (double)A[j * lda + i] * (double)Q[q * ldq + i];
             out[j * m + q] = s;
         }
+    return SAIGE_OK;
+}
+
+int saige_probes_device_rows(saige_ctx* c, float* dst, int64_t ld, uint64_t seed, int64_t i0, int k,
+                             const int64_t* rows, int64_t n) {
+    if (!c || !dst || k < 1 || !rows || n < 1 || n > c->N) return SAIGE_E_ARG;
+    std::vector<double> v((size_t)n);
+    for (int j = 0; j < k; j++) {
+        std::fill(dst + j * ld, dst + j * ld + c->N, 0.0f);
+        saige::fit::probe_vector(seed, i0 + j, n, v.data());
+        for (int64_t i = 0; i < n; i++)
+            if (rows[i] >= 0 && rows[i] < c->N) dst[j * ld + rows[i]] = (float)v[(size_t)i];
+    }
     return SAIGE_OK;
 }
 

@@ -44,6 +44,32 @@ def test_probes_and_dots(session, grm1000):
         session.free_device(Q)
 
 
+def test_subset_probes(session, grm1000):
+    """Probes of a fit on a sample subset, made on the device: entry i of
+    probe i0 + j is the hash at position i of the subset, at row rows[i];
+    every other row of the column is zero (whatever the buffer held)."""
+    print("case: device subset probes", flush=True)
+    N = 1000
+    rng = np.random.default_rng(8)
+    U = session.alloc_device(4 * N * 9)
+    R = session.alloc_device(8 * N)
+    try:
+        for n in (1000, 617, 1):
+            rows = np.sort(rng.choice(N, n, replace=False)).astype(np.int64)
+            session.copy_host_to_device(U, np.full((N, 9), 7.0, np.float32))
+            session.copy_bytes_to_device(R, rows)
+            session.probes_device_rows(U, 1, 200, 5, 8, R, n)
+            got = session.copy_device_to_host(U, 4 * N * 9).view(np.float32).reshape(9, N).T
+            assert np.all(got[:, 0] == 7.0)                      # column before col0 untouched
+            for j in range(8):
+                want = np.zeros(N, np.float32)
+                want[rows] = D.probe_vector(200, 5 + j, n)
+                assert np.array_equal(got[:, 1 + j], want), (n, j)
+    finally:
+        session.free_device(U)
+        session.free_device(R)
+
+
 def test_take_with_dots(session, grm1000):
     """Product slots stashed on the device with dots against shared columns,
     then solve slots returning pairwise dots, equal the host computation."""

@@ -146,6 +146,16 @@ class FakeDeviceEngine(FakeEngine):
         for j in range(k):
             buf.a[:, col0 + j] = D.probe_vector(seed, i0 + j, self.N)
 
+    def copy_bytes_to_device(self, dst, arr):
+        dst.rows = np.array(arr, np.int64)
+
+    def probes_device_rows(self, buf, col0, seed, i0, k, rows_buf, n):
+        self.calls["probes_rows"] += 1
+        rows = rows_buf.rows[:n]
+        for j in range(k):
+            buf.a[:, col0 + j] = 0
+            buf.a[rows, col0 + j] = D.probe_vector(seed, i0 + j, n)
+
     def dots_device(self, A, k, Q, m):
         a, q = A.a.astype(np.float64), Q.a.astype(np.float64)
         return np.array([[np.dot(a[:, j], q[:, r]) for r in range(m)] for j in range(k)])
@@ -289,6 +299,7 @@ def test_device_trace_rounds_match(cohort):
     assert sch.device_trace
     assert all(h.error is None for h in handles), [h.error for h in handles]
     assert eng.calls["take_ex"] > 0 and eng.calls["alloc"] > 0 and eng.calls["free"] == 0
+    assert eng.calls["probes_rows"] > 0                  # the subset fit's probes made in place
     # the buffers stay on the engine for the next scheduler, and free once
     n_alloc = eng.calls["alloc"]
     D.fit_batched(eng, jobs[:1], opt, width=64, n_systems=3)
