@@ -48,7 +48,11 @@ extern "C" {
                                    z = hi + lo (lo the rounded remainder, z - hi - lo
                                    within 2^-16 |z|); hi.hi + hi.lo + lo.hi on the BF16
                                    MFMA with FP32 accumulation in one SYRK pass: an
-                                   FP32-accurate S at about three times the BF16 build */
+                                   FP32-accurate S at about three times the BF16 build,
+                                   stored rounded to 16 significant bits (within 2^-16
+                                   of each value; LOCO ranges of it are stored the same
+                                   way), so the block GEMM splits it into two BF16
+                                   pieces instead of three with the same bits */
 
 typedef struct saige_ctx saige_ctx;
 

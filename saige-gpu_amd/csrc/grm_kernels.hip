@@ -274,10 +274,21 @@ __global__ __launch_bounds__(256) void k_syrk_pk(const uint8_t* __restrict__ tp,
 // that of k_syrk_pk, so both give bitwise equal S.
 constexpr int BT2 = 256;
 
+// x rounded to nearest (ties to even) at 16 significant bits: the FP32
+// value with the low 8 of its 23 fraction bits cleared.  Such a value is
+// exactly the sum of its two BF16 pieces by truncation (the upper half of
+// its bits, then the upper half of the remainder's), which the block GEMM
+// over the lower triangle relies on.  Finite x only.
+__device__ __forceinline__ float round16(float x) {
+    const uint32_t b = __float_as_uint(x);
+    return __uint_as_float((b + 0x7Fu + ((b >> 8) & 1u)) & 0xFFFFFF00u);
+}
+
 // Store a 256x256 tile from waves of (32*MI) x 128 (wave (wr, wc) owns rows
 // wr*32*MI + [0, 32*MI) and columns wc*128 + [0, 128)), mirrored for
 // off-diagonal tiles; accumulate: +1 add to S, -1 subtract from S, 0 store.
-template <int MI>
+// R16: the stored values (after the add or subtract) rounded by round16.
+template <int MI, bool R16>
 __device__ __forceinline__ void store_tile256(float* S, int64_t ldS, int64_t bi, int64_t bj, int wr, int wc,
                                               int lane, const f32x16 (&acc)[MI][4], int accumulate) {
     const bool mirror = bi != bj;
@@ -298,6 +309,10 @@ __device__ __forceinline__ void store_tile256(float* S, int64_t ldS, int64_t bi,
                 } else if (accumulate < 0) {
 #pragma unroll
                     for (int e = 0; e < 4; e++) v[e] = S[(row0 + e) * ldS + col] - v[e];
+                }
+                if (R16) {
+#pragma unroll
+                    for (int e = 0; e < 4; e++) v[e] = round16(v[e]);
                 }
 #pragma unroll
                 for (int e = 0; e < 4; e++) S[(row0 + e) * ldS + col] = v[e];
@@ -418,7 +433,7 @@ __global__ __launch_bounds__(512) void k_syrk_pk256(const uint8_t* __restrict__ 
         step(sA1, sB1, sA0, sB0, kc + 1);
     }
     if (kc < c1) multiply(sA0, sB0);
-    store_tile256(S, ldS, bi, bj, wr, wc, lane, acc, accumulate);
+    store_tile256<2, false>(S, ldS, bi, bj, wr, wc, lane, acc, accumulate);
 }
 
 // The split GRM in one pass: per 32-marker chunk each thread decodes its
@@ -430,7 +445,10 @@ __global__ __launch_bounds__(512) void k_syrk_pk256(const uint8_t* __restrict__ 
 // operands (A hi, A lo, B hi, B lo; 256 samples x 32 markers each, pitch 40
 // for conflict-free 16-byte reads) = 160 KiB.  Threads 0-255 decode the row
 // panel, 256-511 the column panel: thread (o, b) holds marker octet o of the
-// chunk and byte row b (four samples).
+// chunk and byte row b (four samples).  S is stored rounded to 16 significant
+// bits (round16, after the add or subtract of an accumulating pass), which
+// moves each value by at most 2^-16 of itself; the block GEMM then needs two
+// BF16 pieces of S instead of three.
 constexpr int BKS = 32;            // markers per chunk
 constexpr int PS = BKS + 8;        // LDS row pitch (bf16 elements)
 
@@ -541,7 +559,7 @@ __global__ __launch_bounds__(512) void k_syrk_pk256_split(const uint8_t* __restr
         step(s1, s0, kc + 1);
     }
     if (kc < k1) multiply(s0);
-    store_tile256(S, ldS, bi, bj, wr, wc, lane, acc, accumulate);
+    store_tile256<2, true>(S, ldS, bi, bj, wr, wc, lane, acc, accumulate);
 }
 
 // Re-tile the packed genotypes for k_syrk_pk: block (p, c) byte (b, m) =

@@ -422,6 +422,26 @@ __device__ __forceinline__ void split3_8(float4 a, float4 b, u32x4& h, u32x4& m,
     l = u32x4{l0, l1, l2, l3};
 }
 
+// The first two pieces of split3_pair: h + m is x exactly when x has at most
+// 16 significant bits (the split GRM's storage), whose third piece is zero.
+__device__ __forceinline__ void split2_pair(float x0, float x1, uint32_t& h, uint32_t& m) {
+    const uint32_t b0 = __float_as_uint(x0), b1 = __float_as_uint(x1);
+    const float r0 = x0 - __uint_as_float(b0 & 0xffff0000u);
+    const float r1 = x1 - __uint_as_float(b1 & 0xffff0000u);
+    h = (b1 & 0xffff0000u) | (b0 >> 16);
+    m = (__float_as_uint(r1) & 0xffff0000u) | (__float_as_uint(r0) >> 16);
+}
+
+__device__ __forceinline__ void split2_8(float4 a, float4 b, u32x4& h, u32x4& m) {
+    uint32_t h0, h1, h2, h3, m0, m1, m2, m3;
+    split2_pair(a.x, a.y, h0, m0);
+    split2_pair(a.z, a.w, h1, m1);
+    split2_pair(b.x, b.y, h2, m2);
+    split2_pair(b.z, b.w, h3, m3);
+    h = u32x4{h0, h1, h2, h3};
+    m = u32x4{m0, m1, m2, m3};
+}
+
 __device__ __forceinline__ f32x16 mfma_bf(u32x4 a, u32x4 b, f32x16 c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b),
                                                    c, 0, 0, 0);
@@ -432,6 +452,25 @@ __device__ __forceinline__ f32x16 mfma_x3(u32x4 ah, u32x4 am, u32x4 al, u32x4 bh
                                           f32x16 c) {
     c = mfma_bf(al, bh, c);
     c = mfma_bf(ah, bl, c);
+    c = mfma_bf(am, bm, c);
+    c = mfma_bf(am, bh, c);
+    c = mfma_bf(ah, bm, c);
+    return mfma_bf(ah, bh, c);
+}
+
+// mfma_x3 with a zero third A piece (al = 0: its product adds exact zeros to
+// the accumulator and is left out) ...
+__device__ __forceinline__ f32x16 mfma_x3_a2(u32x4 ah, u32x4 am, u32x4 bh, u32x4 bm, u32x4 bl, f32x16 c) {
+    c = mfma_bf(ah, bl, c);
+    c = mfma_bf(am, bm, c);
+    c = mfma_bf(am, bh, c);
+    c = mfma_bf(ah, bm, c);
+    return mfma_bf(ah, bh, c);
+}
+
+// ... and with a zero third B piece (bl = 0)
+__device__ __forceinline__ f32x16 mfma_x3_b2(u32x4 ah, u32x4 am, u32x4 al, u32x4 bh, u32x4 bm, f32x16 c) {
+    c = mfma_bf(al, bh, c);
     c = mfma_bf(am, bm, c);
     c = mfma_bf(am, bh, c);
     c = mfma_bf(ah, bm, c);
@@ -658,7 +697,7 @@ __global__ __launch_bounds__(768) void k_gemm_x3w(const float* __restrict__ S, i
 }
 
 // ------------------------------------------------- lower-triangle product
-// k_gemm_x3t<NWV, PD, FULL>: the three-piece product read from the lower
+// k_gemm_x3t<NWV, PD, FULL, SP>: the three-piece product read from the lower
 // triangle of S only (half the bytes).  S is symmetric, so a 32x32 block
 // S_IJ left of the diagonal serves two products:
 //   use 1: Y_I += S_IJ P_J    (the per-wave work of k_gemm_x3w)
@@ -689,6 +728,12 @@ __global__ __launch_bounds__(768) void k_gemm_x3w(const float* __restrict__ S, i
 // Eight waves have registers for a software-pipelined loop: chunk c's pieces
 // are split during chunk c-1, beside that chunk's MFMAs.  Twelve waves split
 // each chunk at its own turn, PD chunks of raw data ahead.
+// SP = 2: S holds values of at most 16 significant bits (the split GRM's
+// storage), so its third piece is zero: S is split into two pieces, and the
+// products and the transpose of the zero piece are left out (five piece
+// products per use instead of six, two transposes instead of three).  Adding
+// a zero product leaves an accumulator unchanged, so on such an S both forms
+// give the same bits.
 
 // a use-2 tile in LDS: [column j][64 rhs], float4 slot s of row j stored at
 // slot s ^ (j % 16), so the column-per-lane writes and row-wise reads both
@@ -718,7 +763,7 @@ __device__ __forceinline__ void tri_transpose(const u32x4 (&a)[2], const u32x4 (
                                              __float_as_uint(T[8 * s2 + 2 * k]), 0x07060302u);
 }
 
-template <int NWV, int PD, bool FULL>
+template <int NWV, int PD, bool FULL, int SP>
 __global__ __launch_bounds__(64 * NWV) void k_gemm_x3t(const float* __restrict__ S, int64_t ld,
                                                        const u32x4* __restrict__ pb, const float* __restrict__ P,
                                                        const int4* __restrict__ items, float* __restrict__ y1p,
@@ -830,22 +875,29 @@ __global__ __launch_bounds__(64 * NWV) void k_gemm_x3t(const float* __restrict__
 #pragma unroll
                 for (int h = 0; h < NT; h++) {
                     const u32x4* bb = sb + (2 * hl + s) * 3 * 64 + 32 * (FULL ? h : hsel) + ln;
-                    acc1[h] = mfma_x3(ah[s], am[s], al[s], bb[0], bb[64], bb[128], acc1[h]);
+                    if (SP == 3)
+                        acc1[h] = mfma_x3(ah[s], am[s], al[s], bb[0], bb[64], bb[128], acc1[h]);
+                    else
+                        acc1[h] = mfma_x3_a2(ah[s], am[s], bb[0], bb[64], bb[128], acc1[h]);
                 }
             }
             if (U2) {
                 u32x4 th[2], tm[2], tl[2];
                 tri_transpose(ah, esel, th);
                 tri_transpose(am, esel, tm);
-                tri_transpose(al, esel, tl);
+                if (SP == 3) tri_transpose(al, esel, tl);
 #pragma unroll
                 for (int h = 0; h < NT; h++) {
                     f32x16 c2;
 #pragma unroll
                     for (int r = 0; r < 16; r++) c2[r] = 0.0f;
 #pragma unroll
-                    for (int s2 = 0; s2 < 2; s2++)
-                        c2 = mfma_x3(pah[h][s2], pam[h][s2], pal[h][s2], th[s2], tm[s2], tl[s2], c2);
+                    for (int s2 = 0; s2 < 2; s2++) {
+                        if (SP == 3)
+                            c2 = mfma_x3(pah[h][s2], pam[h][s2], pal[h][s2], th[s2], tm[s2], tl[s2], c2);
+                        else
+                            c2 = mfma_x3_b2(pah[h][s2], pam[h][s2], pal[h][s2], th[s2], tm[s2], c2);
+                    }
                     // c2[a]: rhs 8*(a/4) + 4*hl + a%4 of the tile, column ln.  With
                     // two buffers, chunk c's tile goes to buffer c & 1, last read
                     // for chunk c-2 before this chunk's barrier
@@ -871,7 +923,12 @@ __global__ __launch_bounds__(64 * NWV) void k_gemm_x3t(const float* __restrict__
             if (pre) issue(ring[(u + PD) % RING], bring[(u + PD) % RING], c + PD < cb ? c + PD : cb - 1);
             u32x4 ah[2], am[2], al[2];
 #pragma unroll
-            for (int s = 0; s < 2; s++) split3_8(ring[u][2 * s], ring[u][2 * s + 1], ah[s], am[s], al[s]);
+            for (int s = 0; s < 2; s++) {
+                if (SP == 3)
+                    split3_8(ring[u][2 * s], ring[u][2 * s + 1], ah[s], am[s], al[s]);
+                else
+                    split2_8(ring[u][2 * s], ring[u][2 * s + 1], ah[s], am[s]);
+            }
             chunk_mfma(par, c, ah, am, al);
         };
         if (PIPE) {
@@ -885,10 +942,15 @@ __global__ __launch_bounds__(64 * NWV) void k_gemm_x3t(const float* __restrict__
             // compile-time indices.  RING 3 fits the registers of one-half
             // launches only; the autotuner times both.
             constexpr int UNR = RING == 3 ? 6 : 2;
-            u32x4 pcs[2][3][2];                       // [set][piece][step]
-            auto split_into = [&](const float4 (&rs)[4], u32x4 (&p)[3][2]) {
+            u32x4 pcs[2][SP][2];                      // [set][piece][step]
+            auto split_into = [&](const float4 (&rs)[4], u32x4 (&p)[SP][2]) {
 #pragma unroll
-                for (int st = 0; st < 2; st++) split3_8(rs[2 * st], rs[2 * st + 1], p[0][st], p[1][st], p[2][st]);
+                for (int st = 0; st < 2; st++) {
+                    if constexpr (SP == 3)
+                        split3_8(rs[2 * st], rs[2 * st + 1], p[0][st], p[1][st], p[2][st]);
+                    else
+                        split2_8(rs[2 * st], rs[2 * st + 1], p[0][st], p[1][st]);
+                }
             };
             // k: the chunk's offset from ca modulo UNR
             auto pstep = [&](int k, int64_t c) {
@@ -900,7 +962,7 @@ __global__ __launch_bounds__(64 * NWV) void k_gemm_x3t(const float* __restrict__
                 if (SRB == 2 && U2 && c > ca) reduce(c - 1, sR[(c - 1) & 1]);
                 split_into(ring[(k + 1) % RING], pcs[par ^ 1]);
                 issue(ring[u], bring[u], c + RING < cb ? c + RING : cb - 1);
-                chunk_mfma(par, c, pcs[par][0], pcs[par][1], pcs[par][2]);
+                chunk_mfma(par, c, pcs[par][0], pcs[par][1], pcs[par][SP - 1]);
             };
 #pragma unroll
             for (int d = 0; d < RING; d++) issue(ring[d], bring[d], ca + d < cb ? ca + d : cb - 1);
@@ -1624,7 +1686,7 @@ hipError_t tri_plan_build(TriPlan& t, int64_t Npad, int nwv, int ch) {
 }
 
 namespace {
-template <int NWV>
+template <int NWV, int SP>
 hipError_t launch_tri(hipStream_t s, const float* S, int64_t Npad, const float* in, float* ypart, int pd,
                       int halves, const TriPlan& t) {
     const int64_t G = Npad / 8;
@@ -1633,19 +1695,19 @@ hipError_t launch_tri(hipStream_t s, const float* S, int64_t Npad, const float* 
     const dim3 grid((unsigned)t.nitems), block(64 * NWV);
     if (halves == 3) {
         if (pd == 3)
-            hipLaunchKernelGGL((k_gemm_x3t<NWV, 2, true>), grid, block, 0, s, S, Npad, pb, in, t.items, t.y1p, t.y2p,
-                               halves);
+            hipLaunchKernelGGL((k_gemm_x3t<NWV, 2, true, SP>), grid, block, 0, s, S, Npad, pb, in, t.items, t.y1p,
+                               t.y2p, halves);
         else
-            hipLaunchKernelGGL((k_gemm_x3t<NWV, 1, true>), grid, block, 0, s, S, Npad, pb, in, t.items, t.y1p, t.y2p,
-                               halves);
+            hipLaunchKernelGGL((k_gemm_x3t<NWV, 1, true, SP>), grid, block, 0, s, S, Npad, pb, in, t.items, t.y1p,
+                               t.y2p, halves);
         hipLaunchKernelGGL((k_tri_reduce<NWV, 16>), dim3((unsigned)(Npad / 16)), dim3(256), 0, s, t.y1p, t.y2p,
                            t.rtab, Npad, 0, ypart);
     } else {
         if (pd == 3)
-            hipLaunchKernelGGL((k_gemm_x3t<NWV, 2, false>), grid, block, 0, s, S, Npad, pb, in, t.items, t.y1p,
+            hipLaunchKernelGGL((k_gemm_x3t<NWV, 2, false, SP>), grid, block, 0, s, S, Npad, pb, in, t.items, t.y1p,
                                t.y2p, halves);
         else
-            hipLaunchKernelGGL((k_gemm_x3t<NWV, 1, false>), grid, block, 0, s, S, Npad, pb, in, t.items, t.y1p,
+            hipLaunchKernelGGL((k_gemm_x3t<NWV, 1, false, SP>), grid, block, 0, s, S, Npad, pb, in, t.items, t.y1p,
                                t.y2p, halves);
         hipLaunchKernelGGL((k_tri_reduce<NWV, 8>), dim3((unsigned)(Npad / 32)), dim3(256), 0, s, t.y1p, t.y2p,
                            t.rtab, Npad, halves == 2 ? 32 : 0, ypart);
@@ -1656,15 +1718,20 @@ hipError_t launch_tri(hipStream_t s, const float* S, int64_t Npad, const float* 
 
 hipError_t launch_grm_gemm(hipStream_t s, const float* S, int64_t Npad, int64_t Nnm, const float* in,
                            float* out, SigmaArgs a, double* partial, int64_t* nparts, float* ypart, GemmConfig g,
-                           int halves, const TriPlan* tri) {
+                           int halves, const TriPlan* tri, bool s16) {
     // Npad is a multiple of 256 = 128 * RT for both RT; a slice holds at least
     // one superchunk when Npad / GKS >= nsplit
     if (!gemm_config_valid(g) || Npad % 256 != 0 || halves < 1 || halves > 3) return hipErrorInvalidValue;
     if (g.mode == GEMM_BF16X3_TRI) {
         // the lower-triangle product sums its partials into slice 0
         if (!tri || tri->Npad != Npad || !tri->items) return hipErrorInvalidValue;
-        hipError_t e = tri->nwv == 8 ? launch_tri<8>(s, S, Npad, in, ypart, g.pd, halves, *tri)
-                                     : launch_tri<12>(s, S, Npad, in, ypart, g.pd, halves, *tri);
+        hipError_t e;
+        if (s16)
+            e = tri->nwv == 8 ? launch_tri<8, 2>(s, S, Npad, in, ypart, g.pd, halves, *tri)
+                              : launch_tri<12, 2>(s, S, Npad, in, ypart, g.pd, halves, *tri);
+        else
+            e = tri->nwv == 8 ? launch_tri<8, 3>(s, S, Npad, in, ypart, g.pd, halves, *tri)
+                              : launch_tri<12, 3>(s, S, Npad, in, ypart, g.pd, halves, *tri);
         if (e != hipSuccess) return e;
         const int64_t nb = Npad / EROWS;
         hipLaunchKernelGGL(k_gemm_epi, dim3((unsigned)nb), dim3(256), 0, s, ypart, 1, Npad, Nnm, in, out, a,

@@ -200,17 +200,23 @@ int engine_reserve(saige_ctx* c, Engine& e, int kpad, int nsys) {
 
 // The GRM a product uses: the full one (S/Mqc, diag) or the LOCO one
 // (S_loco/M_loco, diag_loco; getDiagOfSigma_LOCO ignores the diag-as-one
-// option, FG.cpp:2368-2392).
+// option, FG.cpp:2368-2392).  s16: S's values have at most 16 significant
+// bits (the split GRM and its LOCO ranges are stored so), and the block GEMM
+// may split them into two BF16 pieces; SAIGE_GEMM_S_PIECES=3 keeps three
+// (the same bits, more work).
 struct GrmView {
     const float* S;
     const float* diag;
     int64_t m;
     int diag_one;
+    bool s16;
 };
 
 GrmView grm_view(const saige_ctx* c, int loco) {
-    if (loco) return {c->loco_inplace ? c->d_grm : c->d_grm_loco, c->d_diag_loco, c->loco_m, 0};
-    return {c->d_grm, c->d_diag, c->Mqc, c->diag_one};
+    const char* pieces = std::getenv("SAIGE_GEMM_S_PIECES");
+    const bool s16 = c->grm_precision == SAIGE_GRM_BF16_SPLIT && !(pieces && std::atoi(pieces) == 3);
+    if (loco) return {c->loco_inplace ? c->d_grm : c->d_grm_loco, c->d_diag_loco, c->loco_m, 0, s16};
+    return {c->d_grm, c->d_diag, c->Mqc, c->diag_one, s16};
 }
 
 int check_loco(const saige_ctx* c) {
@@ -395,7 +401,7 @@ int engine_step(saige_ctx* c, Engine& e, int n_iter, uint8_t* done_out, int32_t*
             SAIGE_HIP_TRY(launch_grm_gemv(st, g.S, Np, N, e.P, e.AP, a, e.partial, &np, tv));
         else
             SAIGE_HIP_TRY(launch_grm_gemm(st, g.S, Np, N, e.P, e.AP, a, e.partial, &np, e.ypart,
-                                          halves == 3 ? c->gemm_cfg : c->gemm_cfg_half, halves, &c->tri));
+                                          halves == 3 ? c->gemm_cfg : c->gemm_cfg_half, halves, &c->tri, g.s16));
         SAIGE_HIP_TRY(hipEventRecord(c->ev1, st));
         SAIGE_HIP_TRY(launch_alpha(st, e.kpad, e.partial, np, e.rz, e.done, e.live, e.kind, e.alpha));
         int64_t np2 = 0;
@@ -1204,7 +1210,7 @@ static int grm_matvec(saige_ctx* c, int loco, const float* B, int64_t k, float* 
         if (e == hipSuccess) e = launch_scatter_cols(st, d_in, kb, N, Np, EW, d_blk);
         if (e == hipSuccess)
             e = launch_grm_gemm(st, g.S, Np, N, d_blk, d_out, a, nullptr, nullptr, c->d_mv_ypart,
-                                kb <= 32 ? c->gemm_cfg_half : c->gemm_cfg, kb <= 32 ? 1 : 3, &c->tri);
+                                kb <= 32 ? c->gemm_cfg_half : c->gemm_cfg, kb <= 32 ? 1 : 3, &c->tri, g.s16);
         if (e == hipSuccess) e = launch_gather_cols(st, d_out, kb, N, EW, d_in);
         if (e == hipSuccess) e = hipMemcpyAsync(Y + k0 * N, d_in, 4 * N * kb, hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -1858,13 +1864,14 @@ int saige_autotune(saige_ctx* c, int reps, double* ms_table) {
     a.mqc = (float)c->Mqc;
     a.mode = 0;
     // time one shape: one warm-up launch, then reps launches between events
+    const bool s16 = grm_view(c, 0).s16;
     auto time_shape = [&](GemmConfig g, int halves, double* out_ms) -> hipError_t {
         hipError_t e = launch_grm_gemm(st, c->d_grm, Np, N, c->d_mv_blk, c->d_mv_out, a, nullptr, nullptr,
-                                       c->d_mv_ypart, g, halves, &c->tri);
+                                       c->d_mv_ypart, g, halves, &c->tri, s16);
         if (e == hipSuccess) e = hipEventRecord(c->ev0, st);
         for (int r = 0; r < reps && e == hipSuccess; r++)
             e = launch_grm_gemm(st, c->d_grm, Np, N, c->d_mv_blk, c->d_mv_out, a, nullptr, nullptr, c->d_mv_ypart,
-                                g, halves, &c->tri);
+                                g, halves, &c->tri, s16);
         if (e == hipSuccess) e = hipEventRecord(c->ev1, st);
         if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
         float ms = 0.f;

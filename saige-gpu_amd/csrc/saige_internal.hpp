@@ -310,9 +310,12 @@ bool gemm_config_valid(GemmConfig g);
 // values for a half that is not computed are meaningless (the caller only
 // passes halves without live columns).
 // tri: the plan for Npad, required in mode GEMM_BF16X3_TRI (else unused).
+// s16: every value of S has at most 16 significant bits (the split GRM's
+// storage); the lower-triangle mode then splits S into two BF16 pieces
+// instead of three, with the same bits.
 hipError_t launch_grm_gemm(hipStream_t s, const float* S, int64_t Npad, int64_t Nnm, const float* in,
                            float* out, SigmaArgs a, double* partial, int64_t* nparts, float* ypart,
-                           GemmConfig g, int halves = 3, const TriPlan* tri = nullptr);
+                           GemmConfig g, int halves = 3, const TriPlan* tri = nullptr, bool s16 = false);
 int64_t gemm_ypart_floats(int64_t Npad);
 int64_t gemv_max_parts(int64_t Npad);
 int64_t gemm_max_parts(int64_t Npad);

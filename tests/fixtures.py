@@ -85,3 +85,19 @@ def random_bed(N, M, miss_rate, seed, maf_lo=0.0, maf_hi=0.5):
     miss = rng.random((N, M)) < miss_rate
     from saige_gpu.plink import pack_bed
     return pack_bed(G, miss), G, miss
+
+
+def round16(S):
+    """FP32 values rounded to nearest (ties to even) at 16 significant bits:
+    the split GRM's storage."""
+    b = np.ascontiguousarray(S, np.float32).view(np.uint32)
+    return ((b + np.uint32(0x7F) + ((b >> np.uint32(8)) & np.uint32(1))) & np.uint32(0xFFFFFF00)).view(np.float32)
+
+
+def on_16bit_grid(K, m):
+    """True when K, as the GRM block accessor returns it (FP32 S / m), comes
+    from an S whose values all have at most 16 significant bits: K * m lies
+    within an FP32 ulp of S, which round16 then recovers exactly."""
+    K = np.asarray(K, np.float32)
+    S = round16((K.astype(np.float64) * m).astype(np.float32))
+    return bool(np.array_equal(S / np.float32(m), K))

@@ -5,14 +5,15 @@ operators and the PCG solvers, against the oracle (FG.cpp:1576-1708,
 Tolerances (stated here, derived in DESIGN.md):
   GRM FP32 SYRK   : max |K - K64| <= 4 sqrt(Mqc) 2^-24 max|K64|  (FP32 sum of Mqc terms)
   GRM BF16 SYRK   : max |K - K64| <= 4e-3 max|K64|               (BF16 z, 2^-9 per operand)
-  GRM split BF16  : as FP32 (z = hi + lo within 2^-16 |z|, lo.lo dropped)
+  GRM split BF16  : as FP32 (z = hi + lo within 2^-16 |z|, lo.lo dropped),
+                    plus 2^-16 for its storage at 16 significant bits
   K b (FP32)      : max |Kb - Kb64| <= 2e-5 * max|Kb64|
   PCG (tol 1e-5)  : max |x - x64| <= 2e-3 * max|x64|, ||Sigma x - b||^2 <= 1.5 tol
 """
 import numpy as np
 import pytest
 
-from fixtures import load_bed, random_bed
+from fixtures import load_bed, on_16bit_grid, random_bed
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -59,6 +60,9 @@ def test_grm_syrk(session, grm1000, monkeypatch, prec, tol, panel):
     K = session.grm_block(0, 0, 1000, 1000)
     K64 = _k64(og, S)
     tol = tol or fp32_tol(og["Mqc"])
+    if prec == "bf16_split":
+        tol += 2.0 ** -16                 # stored at 16 significant bits
+        assert on_16bit_grid(K, og["Mqc"])
     err = np.max(np.abs(K - K64))
     print(f"  max abs err {err:.3e}  max|K| {np.max(np.abs(K64)):.3f}  tol {tol:.2e}")
     assert err <= tol * np.max(np.abs(K64))
@@ -259,7 +263,8 @@ def test_odd_sizes(session):
 def test_grm_tiny_and_odd_sizes(session, monkeypatch, N, M, prec):
     """Cohorts far below one 256-sample tile and just past it, marker counts
     off the 64-marker chunk: every GRM precision against FP64 (the split
-    within its worst case, 3 2^-16 per product, plus the FP32 sum bound; one
+    within its worst case, 3 2^-16 per product, plus 2^-16 for its storage
+    at 16 significant bits and the FP32 sum bound; one
     BF16 pass within 2^-7 per product), exactly symmetric; then a 64-column
     block solve in the default lower-triangle mode against a dense solve
     (at N=257 the second row set runs a short item left of its diagonal)."""
@@ -274,7 +279,7 @@ def test_grm_tiny_and_odd_sizes(session, monkeypatch, N, M, prec):
     K = session.grm_block(0, 0, N, N)
     K64 = O.grm_f64(og) / og["Mqc"]
     tol = {"fp32": fp32_tol(og["Mqc"]), "bf16": 2.0 ** -7 + fp32_tol(og["Mqc"]),
-           "bf16_split": 3 * 2.0 ** -16 + fp32_tol(og["Mqc"])}[prec]
+           "bf16_split": 4 * 2.0 ** -16 + fp32_tol(og["Mqc"])}[prec]     # 3 per product, 1 storage
     err = np.max(np.abs(K - K64)) / np.max(np.abs(K64))
     print(f"  max|K - K64| / max|K64| = {err:.2e} (tol {tol:.2e})", flush=True)
     assert err <= tol
@@ -444,6 +449,63 @@ def test_gemm_tri_mode_many_items(session):
             assert err <= 2e-5 * np.max(np.abs(RB)), rt
             half = session.grm_matvec(B[:, :20])
             assert np.array_equal(half.view(np.uint32), KB[:, :20].view(np.uint32)), rt
+    finally:
+        session.set_gemm_mode("fp32")
+        session.set_gemm_config(1, 1, 4)
+
+
+def test_split_grm_two_piece_gemm(session, monkeypatch):
+    """The split GRM is stored at 16 significant bits, so the lower-triangle
+    GEMM splits S into two BF16 pieces and leaves out the products of the
+    third, which would be zero: K B (64 columns, a full launch; 20, one
+    half) has bitwise the bits of the three-piece form
+    (SAIGE_GEMM_S_PIECES=3), for both wave counts, at N = 4700 (row sets
+    over several items); and the same for engine products on a LOCO range
+    of the split GRM (subtracted in place, stored the same way)."""
+    N, M = 4700, 300
+    print("case: split GRM, two-piece GEMM, N=4700", flush=True)
+    body, G, mm = random_bed(N, M, 0.01, seed=48, maf_lo=0.05, maf_hi=0.5)
+    session.setminMAFforGRM(0.01)
+    session.setmaxMissingRateforGRM(0.15)
+    session.setgeno_bytes(body, N, M)
+    session.build_grm("bf16_split")
+    K = session.grm_block(0, 0, N, N)
+    assert on_16bit_grid(K, session.getMsub_MAFge_minMAFtoConstructGRM())
+    rng = np.random.default_rng(6)
+    B = rng.standard_normal((N, 64)).astype(np.float32)
+    RB = K.astype(np.float64) @ B.astype(np.float64)
+    try:
+        session.set_gemm_mode("bf16x3_tri")
+        for rt in (1, 2):
+            session.set_gemm_config(rt, 3, 8)
+            monkeypatch.delenv("SAIGE_GEMM_S_PIECES", raising=False)
+            two = session.grm_matvec(B), session.grm_matvec(B[:, :20])
+            monkeypatch.setenv("SAIGE_GEMM_S_PIECES", "3")
+            three = session.grm_matvec(B), session.grm_matvec(B[:, :20])
+            for a, b in zip(two, three):
+                assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), rt
+            assert np.max(np.abs(two[0] - RB)) <= 2e-5 * np.max(np.abs(RB)), rt
+        # a LOCO range, subtracted in place: products through the engine
+        monkeypatch.setenv("SAIGE_LOCO_INPLACE", "1")
+        session.setStartEndIndex(40, 139, 0)
+        session.engine_reserve(64, 1)
+        session.engine_use_loco(True)
+        session.engine_set_system(0, rng.uniform(0.05, 0.25, N).astype(np.float32),
+                                  np.array([1.0, 0.3], np.float32))
+        out = {}
+        for pieces in (None, "3"):
+            if pieces:
+                monkeypatch.setenv("SAIGE_GEMM_S_PIECES", pieces)
+            else:
+                monkeypatch.delenv("SAIGE_GEMM_S_PIECES", raising=False)
+            for k in (64, 20):
+                session.engine_add_products(list(range(k)), [0] * k, B[:, :k])
+                done, _ = session.engine_step(1)
+                assert all(done[:k])
+                out[pieces, k] = session.engine_take_batch(list(range(k)))
+        for k in (64, 20):
+            assert np.array_equal(out[None, k].view(np.uint32), out["3", k].view(np.uint32)), k
+        session.engine_use_loco(False)
     finally:
         session.set_gemm_mode("fp32")
         session.set_gemm_config(1, 1, 4)

@@ -6,7 +6,7 @@ or a transposed operand fails."""
 import numpy as np
 import pytest
 
-from fixtures import random_bed
+from fixtures import on_16bit_grid, random_bed, round16
 from oracle import oracle as O
 
 
@@ -224,3 +224,21 @@ def test_glm_start_score_equations():
     yq = X @ np.array([1.0, 2.0, -1.0]) + rng.standard_normal(N)
     fq = O.glm_fit(yq, X, O.Gaussian)
     assert np.allclose(fq["coef"], np.linalg.lstsq(X, yq, rcond=None)[0])
+
+
+def test_16bit_grid_helper():
+    """The GPU tests' check of the split GRM's storage: round16 is rounding
+    to nearest (ties to even) at 16 significant bits, against the same
+    rounding in FP64; values so rounded pass on_16bit_grid through the FP32
+    division by M that the GRM block accessor applies, and FP32 values in
+    general do not."""
+    rng = np.random.default_rng(0)
+    S = rng.normal(0, 30, 20000).astype(np.float32)
+    S[:4] = [1.0, -2.0, 1.0 + 2.0 ** -16, 1.0 + 3 * 2.0 ** -16]       # exact, and two ties
+    x = S.astype(np.float64)
+    q = 2.0 ** (np.floor(np.log2(np.abs(x))) - 15)
+    assert np.array_equal(round16(S).astype(np.float64), np.round(x / q) * q)
+    assert round16(S[2:3])[0] == 1.0 and round16(S[3:4])[0] == 1.0 + 2.0 ** -14
+    m = 12345
+    assert on_16bit_grid(round16(S) / np.float32(m), m)
+    assert not on_16bit_grid(S / np.float32(m), m)
