@@ -7,14 +7,16 @@ intercept + age + sex covariates (QR-transformed as fitNULLGLMM does);
 four binary phenotypes with prevalences 0.05, 0.1, 0.2, 0.3.  The GRM is
 built once from BF16 standardised genotypes on MFMA with FP32 accumulation
 (a single pass, timed for GRM TFLOP/s, then the split build the fits use:
-each value as two BF16 pieces, three products) and held in HBM as FP32.
+each value as two BF16 pieces, three products) and held in HBM as FP32
+(the split GRM's values rounded to 16 significant bits).
 
 A step = one batched round of null fits of the four phenotypes through the
 library's C++ drivers (saige_fit_null_batch): AI-REML with nrun = 15
 Rademacher probes per phenotype, so each trace phase is a block PCG over
 4 x (15 probes + 1) = 64 right-hand sides (FP32 vectors; GRM products at
-FP32 accuracy from three-piece BF16 splits on the MFMA, read from the GRM's
-lower triangle; FP64 dot products).  `value` = fits completed per hour of
+FP32 accuracy from BF16 pieces on the MFMA, three of each vector and two of
+the GRM's 16-bit values, read from the GRM's lower triangle; FP64 dot
+products).  `value` = fits completed per hour of
 step time.  The Python drivers over the same engine and the plain path (one
 phenotype, one right-hand side at a time, GEMV per PCG iteration) are timed
 beside it.
@@ -37,9 +39,10 @@ FP32_MFMA_PEAK = 157.3
 # (mode, N, rt, pd, nsplit).  Sources: PROFILE_FILES
 PROFILED_GEMM_TRAFFIC = {("fp32", 100_000, 1, 3, 8): (42.10 + 0.205 + 0.108 + 0.026) * 1e9,
                          ("bf16x3", 100_000, 1, 3, 8): (0.064 + 41.579 + 0.152) * 1e9,
-                         # lower triangle, eight waves: all four kernels of a launch (pd 3 spills)
-                         ("bf16x3_tri", 100_000, 2, 1, 8): (26.69 + 5.61) * 1e9,
-                         ("bf16x3_tri", 100_000, 2, 3, 8): (36.89 + 7.75) * 1e9}
+                         # lower triangle, eight waves, on the split GRM (two pieces of S):
+                         # all four kernels of a launch (pd 3 spills)
+                         ("bf16x3_tri", 100_000, 2, 1, 8): (26.65 + 5.54) * 1e9,
+                         ("bf16x3_tri", 100_000, 2, 3, 8): (31.15 + 5.92) * 1e9}
 PROFILE_FILES = {"fp32": "profiles/kernels_N100k_M150k_bf16.md", "bf16x3": "profiles/gemm_bf16x3_N100k_M150k.md",
                  "bf16x3_tri": "profiles/kernels_N100k_M150k_split_grm.md"}
 BF16_MFMA_PEAK = 2500.0
@@ -750,13 +753,18 @@ def main():
         roofline["kernel"] = "k_gemm_part + k_gemm_epi (block-PCG GRM x 64 RHS, FP32 MFMA)"
         roofline["bytes"] = f"4*N^2 = {4.0 * N * N:.3e} B of S per launch"
     elif args.gemm_mode == "bf16x3_tri":
+        # on the split GRM (stored at 16 significant bits) S has two pieces:
+        # five piece products per FP32 product instead of six
+        npc = 5 if sess.grm_precision() == "bf16_split" else 6
         roofline["kernel"] = ("k_split_p + k_gemm_x3t + k_tri_reduce + k_gemm_epi (block-PCG GRM x 64 RHS from "
-                              "the lower triangle, FP32 operands as three BF16 pieces)")
+                              "the lower triangle, P as three BF16 pieces, S as " +
+                              ("two (the split GRM's 16-bit values)" if npc == 5 else "three") + ")")
         roofline["bytes"] = f"2*N^2 = {2.0 * N * N:.3e} B of S's lower triangle per launch"
         roofline["hbm_GBs_of_triangle"] = 2.0 * N * N / gemm_avg / 1e9
-        roofline["bf16_piece_products"] = {"tflops": 6.0 * gemm_flop / gemm_avg / 1e12, "peak": BF16_MFMA_PEAK,
-                                           "frac": 6.0 * gemm_flop / gemm_avg / 1e12 / BF16_MFMA_PEAK,
-                                           "note": "six BF16 piece products per FP32 product (the emulation's cost)"}
+        roofline["bf16_piece_products"] = {"tflops": npc * gemm_flop / gemm_avg / 1e12, "peak": BF16_MFMA_PEAK,
+                                           "frac": npc * gemm_flop / gemm_avg / 1e12 / BF16_MFMA_PEAK,
+                                           "note": f"{npc} BF16 piece products per FP32 product (the emulation's "
+                                                   "cost)"}
     else:
         roofline["kernel"] = ("k_split_p + k_gemm_x3w (rt 1) or k_gemm_x3 (rt 2) + k_gemm_epi (block-PCG GRM x 64 "
                               "RHS, FP32 operands as three BF16 pieces on the BF16 MFMA)")
