@@ -843,10 +843,27 @@ __global__ __launch_bounds__(64 * NWV) void k_gemm_x3t(const float* __restrict__
                 }
             }
         }
-        // the sum over the waves of chunk cc's use-2 tiles: 4 values per lane,
-        // waves [0, 4*NT), stored as its partial for this row set
+        // the sum over the waves of chunk cc's use-2 tiles, stored as its
+        // partial for this row set: 4 values per lane in waves [0, 4*NT) of
+        // a full launch, 2 per lane in waves [0, 8) of a one-half launch
+        // (the same sums in the same wave order, spread over more waves)
         auto reduce = [&](int64_t cc, const float* sr) {
-            if (wv < 4 * NT) {
+            if (!FULL) {
+                if (wv < 8) {
+                    const int o = (wv * 64 + lane) * 2;
+                    const int j = o / 32, ql = o % 32;
+                    const float* src = sr + tri_sr_index(j, ql);
+                    float2 sum = *reinterpret_cast<const float2*>(src);
+#pragma unroll
+                    for (int w = 1; w < NWV; w++) {
+                        const float2 v = *reinterpret_cast<const float2*>(src + w * 32 * 64);
+                        sum.x += v.x;
+                        sum.y += v.y;
+                    }
+                    const int64_t pslot = tri_y2_first<NWV>(cc, NR) + (R - cc / NWV - 1);
+                    *reinterpret_cast<float2*>(y2p + pslot * 2048 + j * 64 + 32 * hsel + ql) = sum;
+                }
+            } else if (wv < 4 * NT) {
                 const int o = (wv * 64 + lane) * 4;
                 const int j = o / (32 * NT), ql = o % (32 * NT);
                 const float* src = sr + tri_sr_index(j, ql);
