@@ -394,6 +394,12 @@ int saige_synth_bed_device(saige_ctx* ctx, void* bed_body_dev, int64_t N, int64_
 /* Device allocation helper for the above (freed by saige_free_device). */
 int saige_alloc_device(saige_ctx* ctx, int64_t bytes, void** out);
 int saige_free_device(saige_ctx* ctx, void* p);
+/* Page-locked host memory (freed by saige_free_host): host columns staged in
+ * it move to and from the device at the link's rate.  No counterpart in the
+ * reference, whose vectors stay in host memory; the library's own
+ * scheduler stages its column transfers here. */
+int saige_alloc_host(saige_ctx* ctx, int64_t bytes, void** out);
+int saige_free_host(saige_ctx* ctx, void* p);
 /* Device-to-device copy on the context's stream (the known-size copy that
  * calibrates HBM traffic counters in profiles). */
 int saige_copy_device(saige_ctx* ctx, void* dst_dev, const void* src_dev, int64_t bytes);

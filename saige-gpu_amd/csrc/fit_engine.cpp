@@ -254,6 +254,77 @@ int check(int rc) {
 // the fit's samples in the N phenotyped samples
 int64_t sample_of(const saige_fit_spec* s, int64_t i) { return s->subset ? s->subset[i] : i; }
 
+// Page-locked host blocks from the library, kept for the process once made
+// (pinning tens of MB takes milliseconds, about what a fit round's transfers
+// gain from it): a scheduler takes the blocks it needs and gives them back
+// when it ends.
+class HostBlocks {
+public:
+    static HostBlocks& get() {
+        static HostBlocks* b = new HostBlocks();   // never destroyed, like the blocks
+        return *b;
+    }
+    // a block of at least `bytes`, or nullptr if none can be made
+    void* take(saige_ctx* ctx, int64_t bytes) {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            for (size_t i = 0; i < free_.size(); i++)
+                if (free_[i].first >= bytes) {
+                    void* p = free_[i].second;
+                    used_.emplace_back(free_[i]);
+                    free_.erase(free_.begin() + (ptrdiff_t)i);
+                    return p;
+                }
+        }
+        void* p = nullptr;
+        if (saige_alloc_host(ctx, bytes, &p) != SAIGE_OK) return nullptr;
+        std::lock_guard<std::mutex> lk(mu_);
+        used_.emplace_back(bytes, p);
+        return p;
+    }
+    void give(void* p) {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (size_t i = 0; i < used_.size(); i++)
+            if (used_[i].second == p) {
+                free_.push_back(used_[i]);
+                used_.erase(used_.begin() + (ptrdiff_t)i);
+                return;
+            }
+    }
+
+private:
+    std::mutex mu_;
+    std::vector<std::pair<int64_t, void*>> free_, used_;
+};
+
+// n values in a page-locked block (zeroed plain memory if none can be made):
+// host columns staged here move to and from the device at the link's rate
+// instead of through the runtime's bounce buffers
+template <class T>
+class HostArray {
+public:
+    HostArray(saige_ctx* ctx, int64_t n) {
+        const int64_t m = std::max<int64_t>(n, 1);
+        p_ = static_cast<T*>(HostBlocks::get().take(ctx, (int64_t)sizeof(T) * m));
+        pinned_ = p_ != nullptr;
+        if (!pinned_) p_ = new T[(size_t)m]();
+    }
+    ~HostArray() {
+        if (pinned_)
+            HostBlocks::get().give(p_);
+        else
+            delete[] p_;
+    }
+    HostArray(const HostArray&) = delete;
+    HostArray& operator=(const HostArray&) = delete;
+    T* data() { return p_; }
+    T& operator[](size_t i) { return p_[i]; }
+
+private:
+    T* p_ = nullptr;
+    bool pinned_ = false;
+};
+
 // The scheduler loop: admits the queued fits while an engine system and
 // HBM are free, steps the engine over their columns, and returns once
 // `stopping` is set and every submitted fit has finished.
@@ -270,9 +341,11 @@ void run_scheduler(Scheduler& sch, int n_systems) {
     std::vector<uint8_t> done((size_t)sch.width);
     std::vector<int32_t> iters((size_t)sch.width);
     size_t live = 0;
-    std::vector<float> colbuf;
-    std::vector<uint8_t> mask((size_t)N);
-    std::vector<float> wfull((size_t)N);
+    // staging for host columns (at most one per slot, or the trace round's
+    // covariate columns), a system's weights and its sample mask
+    HostArray<float> colbuf(sch.ctx, N * std::max<int64_t>(sch.width, kTraceCovariates));
+    HostArray<float> wfull(sch.ctx, N);
+    HostArray<uint8_t> mask(sch.ctx, N);
     // the trace rounds' device buffers, per system, made on first use and
     // freed when the batch ends
     struct Buffers {
@@ -300,8 +373,8 @@ void run_scheduler(Scheduler& sch, int n_systems) {
     auto set_system = [&](FitTask* f, const double* w, const double tau[2]) {
         const auto t0 = clk::now();
         const saige_fit_spec* s = f->spec;
-        std::fill(mask.begin(), mask.end(), (uint8_t)(s->subset ? 0 : 1));
-        std::fill(wfull.begin(), wfull.end(), 1.0f);
+        std::fill(mask.data(), mask.data() + N, (uint8_t)(s->subset ? 0 : 1));
+        std::fill(wfull.data(), wfull.data() + N, 1.0f);
         for (int64_t i = 0; i < s->n; i++) {
             const int64_t r = sample_of(s, i);
             mask[(size_t)r] = 1;
@@ -345,10 +418,10 @@ void run_scheduler(Scheduler& sch, int n_systems) {
             }
             check(saige_probes_device_rows(sch.ctx, b.U, N, t.seed, t.start, (int)k, b.rows, n));
         }
-        colbuf.assign((size_t)(N * p), 0.0f);
         for (int64_t c = 0; c < p; c++) {
             const double* x = t.SX->col(c);
             float* d = colbuf.data() + c * N;
+            if (sp->subset) std::fill(d, d + N, 0.0f);
             for (int64_t i = 0; i < n; i++) d[sample_of(sp, i)] = (float)x[i];
         }
         check(saige_copy_host_to_device(sch.ctx, b.SX, colbuf.data(), 4 * N * p));
@@ -540,11 +613,11 @@ void run_scheduler(Scheduler& sch, int n_systems) {
             (c.req->solve ? s_cols : p_cols).push_back(c);
         }
         auto upload = [&](const std::vector<Column>& cs) {
-            colbuf.assign((size_t)(N * (int64_t)cs.size()), 0.0f);
             for (size_t k = 0; k < cs.size(); k++) {
                 const saige_fit_spec* s = cs[k].fit->spec;
                 const double* b = cs[k].req->B->col(cs[k].j);
                 float* d = colbuf.data() + (int64_t)k * N;
+                if (s->subset) std::fill(d, d + N, 0.0f);
                 for (int64_t i = 0; i < s->n; i++) d[sample_of(s, i)] = (float)b[i];
             }
         };
@@ -649,10 +722,7 @@ void run_scheduler(Scheduler& sch, int n_systems) {
         }
         t_take_dots += secs(t_td0);
         const auto t_th0 = clk::now();
-        if (!host.empty()) {
-            colbuf.assign((size_t)(N * (int64_t)host.size()), 0.0f);
-            check(saige_engine_take_batch(sch.ctx, (int)host.size(), host.data(), colbuf.data()));
-        }
+        if (!host.empty()) check(saige_engine_take_batch(sch.ctx, (int)host.size(), host.data(), colbuf.data()));
         t_take_host += secs(t_th0);
         std::lock_guard<std::mutex> lk(sch.mu);
         size_t hk = 0;

@@ -2122,6 +2122,22 @@ int saige_free_device(saige_ctx* c, void* p) {
     return SAIGE_OK;
 }
 
+int saige_alloc_host(saige_ctx* c, int64_t bytes, void** out) {
+    CHECK_CTX(c);
+    if (!out || bytes <= 0) {
+        set_error("invalid allocation request");
+        return SAIGE_E_ARG;
+    }
+    SAIGE_HIP_TRY(hipHostMalloc(out, (size_t)bytes, hipHostMallocDefault));
+    return SAIGE_OK;
+}
+
+int saige_free_host(saige_ctx* c, void* p) {
+    CHECK_CTX(c);
+    if (p) SAIGE_HIP_TRY(hipHostFree(p));
+    return SAIGE_OK;
+}
+
 int saige_copy_device(saige_ctx* c, void* dst, const void* src, int64_t bytes) {
     CHECK_CTX(c);
     if (!src || !dst || bytes < 0) {

@@ -100,6 +100,8 @@ SIGNATURES = {
     "saige_last_grm_seconds": (_f64, [_c_p]),
     "saige_grm_precision": (_int, [_c_p]),
     "saige_probes_device_rows": (_int, [_c_p, _c_p, _i64, ctypes.c_uint64, _i64, _int, _c_p, _i64]),
+    "saige_alloc_host": (_int, [_c_p, _i64, ctypes.POINTER(_c_p)]),
+    "saige_free_host": (_int, [_c_p, _c_p]),
     "saige_crossprod_mat_and_kin": (_int, [_c_p, _c_p, _c_p]),
     "saige_grm_matvec": (_int, [_c_p, _c_p, _i64, _c_p]),
     "saige_get_diag_of_sigma": (_int, [_c_p, _c_p, _c_p, _c_p]),

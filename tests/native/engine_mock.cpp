@@ -185,6 +185,18 @@ int saige_free_device(saige_ctx* c, void* p) {
     }
     return SAIGE_OK;
 }
+// host blocks: plain memory, not counted among the device buffers (the
+// scheduler keeps its blocks for the process)
+int saige_alloc_host(saige_ctx* c, int64_t bytes, void** out) {
+    if (!c || !out || bytes <= 0) return SAIGE_E_ARG;
+    *out = std::calloc((size_t)bytes, 1);
+    return *out ? SAIGE_OK : SAIGE_E_NOMEM;
+}
+int saige_free_host(saige_ctx* c, void* p) {
+    if (!c) return SAIGE_E_ARG;
+    std::free(p);
+    return SAIGE_OK;
+}
 int saige_copy_host_to_device(saige_ctx* c, void* dst, const void* src, int64_t bytes) {
     if (!c || !dst || !src || bytes < 0) return SAIGE_E_ARG;
     std::memcpy(dst, src, (size_t)bytes);
