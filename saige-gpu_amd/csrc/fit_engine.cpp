@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "../../include/saige_hip.h"
+#include "engine_park.hpp"
 #include "fit_core.hpp"
 
 namespace saige {
@@ -346,19 +347,28 @@ void run_scheduler(Scheduler& sch, int n_systems) {
     HostArray<float> colbuf(sch.ctx, N * std::max<int64_t>(sch.width, kTraceCovariates));
     HostArray<float> wfull(sch.ctx, N);
     HostArray<uint8_t> mask(sch.ctx, N);
-    // the trace rounds' device buffers, per system, made on first use and
-    // freed when the batch ends
+    // the trace rounds' device buffers, per system, taken on first use from
+    // those the context keeps (or made), and handed back to it when the batch
+    // ends
+    const int64_t probe_bytes = 4 * N * kTraceProbes, cov_bytes = 4 * N * kTraceCovariates;
     struct Buffers {
         saige_ctx* ctx;
         std::vector<TraceBuffers> b;
+        int64_t probe_bytes, cov_bytes, rows_bytes;
         ~Buffers() {
             for (auto& t : b) {
-                for (float* p : {t.U, t.AU, t.SX})
-                    if (p) saige_free_device(ctx, p);
-                if (t.rows) saige_free_device(ctx, t.rows);
+                saige::ctx_park(ctx, t.U, probe_bytes);
+                saige::ctx_park(ctx, t.AU, probe_bytes);
+                saige::ctx_park(ctx, t.SX, cov_bytes);
+                saige::ctx_park(ctx, t.rows, rows_bytes);
             }
         }
-    } tbufs{sch.ctx, std::vector<TraceBuffers>((size_t)n_systems)};
+    } tbufs{sch.ctx, std::vector<TraceBuffers>((size_t)n_systems), probe_bytes, cov_bytes, 8 * N};
+    auto device_block = [&](int64_t bytes) {
+        void* p = saige::ctx_unpark(sch.ctx, bytes);
+        if (!p) check(saige_alloc_device(sch.ctx, bytes, &p));
+        return p;
+    };
     // fits whose request the last step completed, with their post count then
     std::vector<std::pair<FitTask*, int64_t>> resumed;
     // wall seconds of the loop's parts, printed to stderr at the end when the
@@ -392,13 +402,9 @@ void run_scheduler(Scheduler& sch, int n_systems) {
         const auto t0 = clk::now();
         TraceBuffers& b = tbufs.b[(size_t)f->sys];
         if (!b.U) {
-            void* p = nullptr;
-            check(saige_alloc_device(sch.ctx, 4 * N * kTraceProbes, &p));
-            b.U = (float*)p;
-            check(saige_alloc_device(sch.ctx, 4 * N * kTraceProbes, &p));
-            b.AU = (float*)p;
-            check(saige_alloc_device(sch.ctx, 4 * N * kTraceCovariates, &p));
-            b.SX = (float*)p;
+            b.U = (float*)device_block(probe_bytes);
+            b.AU = (float*)device_block(probe_bytes);
+            b.SX = (float*)device_block(cov_bytes);
         }
         const saige_fit_spec* sp = f->spec;
         const int64_t n = sp->n, k = t.k, p = t.p;
@@ -408,11 +414,7 @@ void run_scheduler(Scheduler& sch, int n_systems) {
             // the subset's rows go to the device once per fit on this system;
             // its probes are then made there like the full cohort's
             if (b.rows_fit != f) {
-                if (!b.rows) {
-                    void* p = nullptr;
-                    check(saige_alloc_device(sch.ctx, 8 * N, &p));
-                    b.rows = (int64_t*)p;
-                }
+                if (!b.rows) b.rows = (int64_t*)device_block(8 * N);
                 check(saige_copy_host_to_device(sch.ctx, b.rows, sp->subset, 8 * n));
                 b.rows_fit = f;
             }

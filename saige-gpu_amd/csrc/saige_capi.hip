@@ -12,6 +12,7 @@
 #include <fstream>
 #include <numeric>
 
+#include "engine_park.hpp"
 #include "saige_internal.hpp"
 
 #include <mutex>
@@ -96,7 +97,14 @@ void free_engine(Engine& e) {
     e.width = e.nsys = e.kpad = 0;
 }
 
+void free_parked(saige_ctx* c) {
+    std::lock_guard<std::mutex> lk(c->park_mu);
+    for (auto& b : c->parked) (void)hipFree(b.second);
+    c->parked.clear();
+}
+
 void free_geno(saige_ctx* c) {
+    free_parked(c);
     dfree(c->d_stats); dfree(c->d_packed); dfree(c->d_packed_vr); dfree(c->d_lut); dfree(c->d_lut_vr);
     dfree(c->d_diag); dfree(c->d_grm); dfree(c->d_vec); dfree(c->d_grm_loco); dfree(c->d_diag_loco);
     dfree(c->d_mv_in); dfree(c->d_mv_blk); dfree(c->d_mv_out); dfree(c->d_mv_ypart);
@@ -2121,6 +2129,27 @@ int saige_free_device(saige_ctx* c, void* p) {
     if (p) SAIGE_HIP_TRY(hipFree(p));
     return SAIGE_OK;
 }
+
+extern "C++" {
+namespace saige {
+void* ctx_unpark(saige_ctx* c, int64_t bytes) {
+    std::lock_guard<std::mutex> lk(c->park_mu);
+    for (size_t i = 0; i < c->parked.size(); i++)
+        if (c->parked[i].first == bytes) {
+            void* p = c->parked[i].second;
+            c->parked.erase(c->parked.begin() + (ptrdiff_t)i);
+            return p;
+        }
+    return nullptr;
+}
+
+void ctx_park(saige_ctx* c, void* p, int64_t bytes) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(c->park_mu);
+    c->parked.emplace_back(bytes, p);
+}
+}  // namespace saige
+}  // extern "C++"
 
 int saige_alloc_host(saige_ctx* c, int64_t bytes, void** out) {
     CHECK_CTX(c);

@@ -4,7 +4,9 @@
 #include <hip/hip_bf16.h>
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/saige_hip.h"
@@ -185,6 +187,9 @@ struct saige_ctx {
     // FP64 column-dot scratch (saige_engine_take_batch_ex, saige_dots_device)
     double* d_dots_partial = nullptr;
     double* d_noalloc = nullptr;            // this device's address of the allocation-free scratch
+    // device blocks parked by the scheduler between runs (engine_park.hpp)
+    std::mutex park_mu;
+    std::vector<std::pair<int64_t, void*>> parked;
     double* d_dots_out = nullptr;
     int32_t* d_take_lists = nullptr;             // [3][EW]: slots, destination columns, partner columns
     int grm_precision = -1;

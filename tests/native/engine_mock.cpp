@@ -25,6 +25,7 @@
 #include <string>
 #include <vector>
 
+#include "engine_park.hpp"
 #include "fit_core.hpp"
 #include "saige_hip.h"
 
@@ -62,7 +63,8 @@ struct saige_ctx {
     int64_t steps = 0;
     int64_t fail_at_step = -1;             // mock_fail_step: this step returns SAIGE_E_DEVICE
     std::atomic<int64_t> free_bytes{int64_t(1) << 40};
-    std::atomic<int64_t> live_allocs{0};
+    std::atomic<int64_t> live_allocs{0};  // device buffers neither freed nor parked
+    std::vector<std::pair<int64_t, void*>> parked;
 };
 
 namespace {
@@ -153,11 +155,15 @@ saige_ctx* mock_create(const double* K, int64_t N) {
     c->K.assign(K, K + N * N);
     return c;
 }
-void mock_destroy(saige_ctx* c) { delete c; }
+void mock_destroy(saige_ctx* c) {
+    for (auto& b : c->parked) std::free(b.second);
+    delete c;
+}
 void mock_fail_step(saige_ctx* c, int64_t step) { c->fail_at_step = step; }
 void mock_set_free_bytes(saige_ctx* c, int64_t b) { c->free_bytes = b; }
 int64_t mock_steps(const saige_ctx* c) { return c->steps; }
 int64_t mock_live_allocs(const saige_ctx* c) { return c->live_allocs; }
+int64_t mock_parked(const saige_ctx* c) { return (int64_t)c->parked.size(); }
 
 // ------------------------------------------------------- the C-ABI subset
 const char* saige_last_error(void) { return saige::g_err.c_str(); }
@@ -185,6 +191,27 @@ int saige_free_device(saige_ctx* c, void* p) {
     }
     return SAIGE_OK;
 }
+// parked blocks: kept on the context, not counted as live
+extern "C++" {
+namespace saige {
+void* ctx_unpark(saige_ctx* c, int64_t bytes) {
+    for (size_t i = 0; i < c->parked.size(); i++)
+        if (c->parked[i].first == bytes) {
+            void* p = c->parked[i].second;
+            c->parked.erase(c->parked.begin() + (ptrdiff_t)i);
+            c->live_allocs++;
+            return p;
+        }
+    return nullptr;
+}
+void ctx_park(saige_ctx* c, void* p, int64_t bytes) {
+    if (!p) return;
+    c->parked.emplace_back(bytes, p);
+    c->live_allocs--;
+}
+}  // namespace saige
+}  // extern "C++"
+
 // host blocks: plain memory, not counted among the device buffers (the
 // scheduler keeps its blocks for the process)
 int saige_alloc_host(saige_ctx* c, int64_t bytes, void** out) {

@@ -268,7 +268,7 @@ int main() {
             return 1;
         }
     if (mock_live_allocs(c) != 0) {
-        std::printf("%lld device buffers left allocated\n", (long long)mock_live_allocs(c));
+        std::printf("%lld device buffers neither freed nor back on the context\n", (long long)mock_live_allocs(c));
         return 1;
     }
     mock_destroy(c);

@@ -9,8 +9,9 @@ Checks: a fit's result is bitwise the same alone, in a batch and through the
 server with staggered arrivals; tau agrees with the FP64 oracle driver on
 the same K and probes; an injected engine failure ends every fit with an
 error and no hang; admission waits while the reported free memory is below
-the reserve; every trace buffer is freed.  tests/native/engine_mock_main.cpp
-runs the same scenarios from several threads under the host sanitizers."""
+the reserve; every trace buffer goes back to the context when a batch ends
+and the next batch takes it again.  tests/native/engine_mock_main.cpp runs
+the same scenarios from several threads under the host sanitizers."""
 import ctypes
 import os
 import subprocess
@@ -49,8 +50,8 @@ def lib(tmp_path_factory):
     L.mock_destroy.argtypes = [ctypes.c_void_p]
     L.mock_fail_step.argtypes = [ctypes.c_void_p, ctypes.c_int64]
     L.mock_set_free_bytes.argtypes = [ctypes.c_void_p, ctypes.c_int64]
-    L.mock_steps.restype = L.mock_live_allocs.restype = ctypes.c_int64
-    L.mock_steps.argtypes = L.mock_live_allocs.argtypes = [ctypes.c_void_p]
+    L.mock_steps.restype = L.mock_live_allocs.restype = L.mock_parked.restype = ctypes.c_int64
+    L.mock_steps.argtypes = L.mock_live_allocs.argtypes = L.mock_parked.argtypes = [ctypes.c_void_p]
     return L
 
 
@@ -105,11 +106,13 @@ def test_batch_alone_mixed_and_fp64_oracle(lib, cohort):
     try:
         rc, mixed = _batch(lib, ctx, jobs, OPT)
         assert rc == _lib.SAIGE_OK and all(r["status"] == 0 for r in mixed)
+        parked = lib.mock_parked(ctx)
+        assert parked > 0 and lib.mock_live_allocs(ctx) == 0     # trace buffers back on the context
         for j, r in zip(jobs, mixed):
             rc1, alone = _batch(lib, ctx, [j], OPT)
             assert rc1 == _lib.SAIGE_OK
             _same(alone[0], r)
-        assert lib.mock_live_allocs(ctx) == 0          # trace buffers all freed
+        assert lib.mock_live_allocs(ctx) == 0 and lib.mock_parked(ctx) == parked   # reused, none made
         # tau against the FP64 oracle driver on the same K and probes
         grm = O.DenseGRM(K, 1)
         kw = dict(nrun=OPT.nrun, tolPCG=OPT.tolPCG, maxiterPCG=OPT.maxiterPCG, cv_cutoff=OPT.traceCVcutoff,
