@@ -543,14 +543,18 @@ def main():
         native_step()
     g0, n0_ = sess.engine_stats("gemm_full")
     h0, m0 = sess.engine_stats("gemm_half")
+    q0_, k0 = sess.engine_stats("gemm_quarter")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         nat = native_step()
     t1 = time.perf_counter()
     g1, n1 = sess.engine_stats("gemm_full")
     h1, m1 = sess.engine_stats("gemm_half")
+    q1_, k1 = sess.engine_stats("gemm_quarter")
     half_launches = m1 - m0
     half_avg = (h1 - h0) / max(1, half_launches)
+    quarter_launches = k1 - k0
+    quarter_avg = (q1_ - q0_) / max(1, quarter_launches)
     step_s = (t1 - t0) / args.steps
     fits = len(jobs) * args.steps
     fits_per_hour = fits / (t1 - t0) * 3600.0
@@ -561,7 +565,8 @@ def main():
     nat_s = step_s
     log(f"C++ drivers (saige_fit_null_batch): {step_s:.3f}s/step, {fits_per_hour:.1f} fits/h, {gemm_launches} "
         f"64-column GEMMs avg {gemm_avg * 1e3:.2f} ms ({gemm_tflops:.1f} TFLOP/s), {half_launches} 32-column GEMMs "
-        f"avg {half_avg * 1e3:.2f} ms ({(g1 - g0 + h1 - h0) / args.steps:.3f}s of GEMM per step), theta "
+        f"avg {half_avg * 1e3:.2f} ms, {quarter_launches} 16-column GEMMs avg {quarter_avg * 1e3:.2f} ms "
+        f"({(g1 - g0 + h1 - h0 + q1_ - q0_) / args.steps:.3f}s of GEMM per step), theta "
         f"{[list(np.round(r['theta'], 4)) for r in nat]}")
 
     # the same rounds through the Python drivers and scheduler (the
@@ -741,14 +746,19 @@ def main():
     # the algorithmic unit: 2 N^2 k flop of an FP32 product with k = 64
     # right-hand sides, against the FP32 matrix peak; the split modes reach
     # it through BF16 pieces, whose MFMA count is reported beside it
-    half = {"avg_ms": half_avg * 1e3, "launches": half_launches,
-            "achieved": 2.0 * N * N * 32 / half_avg / 1e12, "unit": "TFLOP/s", "peak": FP32_MFMA_PEAK,
-            "frac": 2.0 * N * N * 32 / half_avg / 1e12 / FP32_MFMA_PEAK,
-            "algorithmic": f"2*N^2*32 = {2.0 * N * N * 32:.3e} flop per launch"}
+    def narrow(avg, launches, k):
+        # a launch over k of the 64 columns: 2 N^2 k flop
+        if launches == 0:
+            return {"launches": 0}
+        return {"avg_ms": avg * 1e3, "launches": launches, "achieved": 2.0 * N * N * k / avg / 1e12,
+                "unit": "TFLOP/s", "peak": FP32_MFMA_PEAK, "frac": 2.0 * N * N * k / avg / 1e12 / FP32_MFMA_PEAK,
+                "algorithmic": f"2*N^2*{k} = {2.0 * N * N * k:.3e} flop per launch"}
+
     roofline = {"bound": "mfma", "achieved": gemm_tflops, "peak": FP32_MFMA_PEAK, "unit": "TFLOP/s",
                 "frac": gemm_tflops / FP32_MFMA_PEAK,
                 "algorithmic": f"2*N^2*64 = {gemm_flop:.3e} FP32 flop per launch (64 right-hand sides)",
-                "half_width": half, **common}
+                "half_width": narrow(half_avg, half_launches, 32),
+                "quarter_width": narrow(quarter_avg, quarter_launches, 16), **common}
     if args.gemm_mode == "fp32":
         roofline["kernel"] = "k_gemm_part + k_gemm_epi (block-PCG GRM x 64 RHS, FP32 MFMA)"
         roofline["bytes"] = f"4*N^2 = {4.0 * N * N:.3e} B of S per launch"

@@ -378,6 +378,8 @@ int64_t saige_engine_system_bytes(const saige_ctx* ctx);
 #define SAIGE_LAUNCH_GEMV 0       /* width-1 engine (saige_pcg1): k_gemv          */
 #define SAIGE_LAUNCH_GEMM_HALF 1  /* block GEMM over one 32-column half            */
 #define SAIGE_LAUNCH_GEMM_FULL 2  /* block GEMM over all 64 columns                */
+#define SAIGE_LAUNCH_GEMM_QUARTER 3 /* lower-triangle block GEMM over one 16-column
+                                     quarter (its only live columns)              */
 int saige_engine_stats(const saige_ctx* ctx, int kind, double* gemm_seconds, int64_t* gemm_launches);
 
 /* ------------------------------------------------------- synthetic data */

@@ -458,24 +458,6 @@ __device__ __forceinline__ f32x16 mfma_x3(u32x4 ah, u32x4 am, u32x4 al, u32x4 bh
     return mfma_bf(ah, bh, c);
 }
 
-// mfma_x3 with a zero third A piece (al = 0: its product adds exact zeros to
-// the accumulator and is left out) ...
-__device__ __forceinline__ f32x16 mfma_x3_a2(u32x4 ah, u32x4 am, u32x4 bh, u32x4 bm, u32x4 bl, f32x16 c) {
-    c = mfma_bf(ah, bl, c);
-    c = mfma_bf(am, bm, c);
-    c = mfma_bf(am, bh, c);
-    c = mfma_bf(ah, bm, c);
-    return mfma_bf(ah, bh, c);
-}
-
-// ... and with a zero third B piece (bl = 0)
-__device__ __forceinline__ f32x16 mfma_x3_b2(u32x4 ah, u32x4 am, u32x4 al, u32x4 bh, u32x4 bm, f32x16 c) {
-    c = mfma_bf(al, bh, c);
-    c = mfma_bf(am, bm, c);
-    c = mfma_bf(am, bh, c);
-    c = mfma_bf(ah, bm, c);
-    return mfma_bf(ah, bh, c);
-}
 
 __global__ __launch_bounds__(256) void k_split_p(const float* __restrict__ in, int64_t G,
                                                  u32x4* __restrict__ pb) {
@@ -697,34 +679,47 @@ __global__ __launch_bounds__(768) void k_gemm_x3w(const float* __restrict__ S, i
 }
 
 // ------------------------------------------------- lower-triangle product
-// k_gemm_x3t<NWV, PD, FULL, SP>: the three-piece product read from the lower
-// triangle of S only (half the bytes).  S is symmetric, so a 32x32 block
-// S_IJ left of the diagonal serves two products:
-//   use 1: Y_I += S_IJ P_J    (the per-wave work of k_gemm_x3w)
+// k_gemm_t16<NWV, PD, NQ, SP>: the three-piece product read from the lower
+// triangle of S only (half the bytes), on the 16x16x32 BF16 MFMA, over NQ
+// tiles of 16 right-hand sides (columns 16*q0 + [0, 16*NQ)): NQ = 4 for 64
+// columns, 2 for 32, 1 for 16.  S is symmetric, so a 32x32 block S_IJ left
+// of the diagonal serves two products:
+//   use 1: Y_I += S_IJ P_J
 //   use 2: Y_J += S_IJ^T P_I
 // A workgroup of NWV waves owns a row set R of 32*NWV rows (wave w: rows
 // 32*(NWV*R + w) + [0, 32)) and walks one item: a range of 32-column chunks.
 // Chunks left of the row set's diagonal block (c < NWV*R) are used twice;
 // the item over the diagonal block (chunks NWV*R + [0, NWV)) runs use 1
-// alone over the whole square, its upper half included.
-// Use 2 needs S_IJ in the MFMA's B-operand layout (lane = column j, eight
-// rows i per lane), the transpose of how it is loaded (lane = row i).  The
-// MFMA transposes the pieces itself: T = A_0 E_0 + A_1 E_1 with 0/1
-// selection matrices E_s sums one BF16 value and zeros per element, exactly,
-// and the MFMA's result layout is column-per-lane, so T's high halves are
-// the pieces in the B-operand layout.  The wave's own rows of P (the A
-// operand P_I^T, three pieces) are split once per item into registers.
-// A wave's use-2 result (32 columns x the live RHS) covers its own 32 rows;
-// the NWV results are summed in LDS in wave order and stored as the partial
-// of (chunk, row set) in y2p.  With eight waves the LDS holds two such tiles,
-// and chunk c's sum is formed in chunk c+1's iteration, beside its MFMA work,
-// so one barrier per chunk suffices; twelve waves' tiles fit once, and the
-// sum follows a second barrier in the chunk's own iteration.  Use 1
-// accumulates in registers over the item and is stored as the item's partial
-// in y1p.  k_tri_reduce sums, for each row, its row set's use-1 partials in
-// slot order and then its chunk's use-2 partials in row-set order: a fixed
-// order, so results are reproducible and a column's bits do not depend on
-// which other columns share the launch.
+// alone over the whole square, its upper half included.  Per wave and
+// chunk:
+// * the lane (l%16, l/16) loads rows 16r + l%16 (row tiles r = 0, 1),
+//   columns 8*(l/16) + [0, 8): the A operand of use 1 for row tile r, K = the
+//   chunk's columns, split into BF16 pieces in registers;
+// * use 1: Y[16r + .][tile t] += S_r P_J, B = the chunk's P pieces, staged
+//   once per workgroup in LDS (lane: column l%16 of the tile, rows
+//   8*(l/16) + [0, 8) of the chunk);
+// * use 2 needs S in the B layout with K = rows.  T_r = S_r E_jg (E_jg the 0/1
+//   matrix picking columns 16*jg + [0, 16)) is exact, one value and zeros per
+//   element, and its result layout (lane: column l%16, rows 4*(l/16) +
+//   [0, 4) of the tile) gives lane group g rows 4g + [0, 4) of each row
+//   tile.  The B operand's eight K values of lane group g are rows 4g +
+//   [0, 4) of tile 0 and of tile 1 (T's high halves), and the A operand
+//   P_I^T (lane: rhs l%16, three pieces split once per item) is loaded in
+//   that row order;
+// * use 2: Y_J^T[tile t][16*jg + .] = P_I^T T_jg per j-group jg, stored to
+//   the wave's use-2 tile [column j][rhs] in LDS.
+// The NWV use-2 tiles of a chunk are summed in LDS in wave order and stored
+// as the partial of (chunk, row set) in y2p.  With eight waves the LDS holds
+// two such tiles, and chunk c's sum is formed in chunk c+1's iteration,
+// beside its MFMA work, so one barrier per chunk suffices; twelve waves'
+// tiles fit once, and the sum follows a second barrier in the chunk's own
+// iteration.  Use 1 accumulates in registers over the item and is stored as
+// the item's partial in y1p.  k_tri_reduce sums, for each row, its row set's
+// use-1 partials in slot order and then its chunk's use-2 partials in
+// row-set order: a fixed order, so results are reproducible, and a column's
+// arithmetic is the same whatever NQ, so a launch can be as narrow as its
+// live columns (a 16-column launch does a quarter of a full one's MFMA work)
+// and a column's bits still do not depend on its co-tenants.
 // Eight waves have registers for a software-pipelined loop: chunk c's pieces
 // are split during chunk c-1, beside that chunk's MFMAs.  Twelve waves split
 // each chunk at its own turn, PD chunks of raw data ahead.
@@ -747,112 +742,156 @@ __host__ __device__ __forceinline__ int64_t tri_y2_first(int64_t c, int64_t NR) 
     return NWV * (b * (NR - 1) - b * (b - 1) / 2) + (c - b * NWV) * (NR - 1 - b);
 }
 
-// T = A_0 E_0 + A_1 E_1 (one exact value per element), repacked as the two
-// B operands of use 2: operand s2 holds T's accumulators 8*s2 + [0, 8)
-__device__ __forceinline__ void tri_transpose(const u32x4 (&a)[2], const u32x4 (&e)[2], u32x4 (&t)[2]) {
-    f32x16 z;
-#pragma unroll
-    for (int r = 0; r < 16; r++) z[r] = 0.0f;
-    f32x16 T = mfma_bf(a[0], e[0], z);
-    T = mfma_bf(a[1], e[1], T);
-#pragma unroll
-    for (int s2 = 0; s2 < 2; s2++)
-#pragma unroll
-        for (int k = 0; k < 4; k++)
-            t[s2][k] = __builtin_amdgcn_perm(__float_as_uint(T[8 * s2 + 2 * k + 1]),
-                                             __float_as_uint(T[8 * s2 + 2 * k]), 0x07060302u);
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+
+__device__ __forceinline__ f32x4 mfma16(u32x4 a, u32x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b),
+                                                   c, 0, 0, 0);
 }
 
-template <int NWV, int PD, bool FULL, int SP>
-__global__ __launch_bounds__(64 * NWV) void k_gemm_x3t(const float* __restrict__ S, int64_t ld,
+// the piece products in mfma_x3's order (smallest first); _a2 / _b2: the
+// third A / B piece is zero (its products add exact zeros to the
+// accumulator) and they are left out
+__device__ __forceinline__ f32x4 mfma16_x3(u32x4 ah, u32x4 am, u32x4 al, u32x4 bh, u32x4 bm, u32x4 bl, f32x4 c) {
+    c = mfma16(al, bh, c);
+    c = mfma16(ah, bl, c);
+    c = mfma16(am, bm, c);
+    c = mfma16(am, bh, c);
+    c = mfma16(ah, bm, c);
+    return mfma16(ah, bh, c);
+}
+
+__device__ __forceinline__ f32x4 mfma16_a2(u32x4 ah, u32x4 am, u32x4 bh, u32x4 bm, u32x4 bl, f32x4 c) {
+    c = mfma16(ah, bl, c);
+    c = mfma16(am, bm, c);
+    c = mfma16(am, bh, c);
+    c = mfma16(ah, bm, c);
+    return mfma16(ah, bh, c);
+}
+
+__device__ __forceinline__ f32x4 mfma16_b2(u32x4 ah, u32x4 am, u32x4 al, u32x4 bh, u32x4 bm, f32x4 c) {
+    c = mfma16(al, bh, c);
+    c = mfma16(am, bm, c);
+    c = mfma16(am, bh, c);
+    c = mfma16(ah, bm, c);
+    return mfma16(ah, bh, c);
+}
+
+// one piece of both row tiles, transposed for use 2: operand jg holds
+// S[rows 4g + [0, 4) of tile 0, then of tile 1][column 16*jg + l%16]
+__device__ __forceinline__ void t16_transpose(const u32x4 (&a)[2], const u32x4 (&e)[2], u32x4 (&t)[2]) {
+    const f32x4 z = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int jg = 0; jg < 2; jg++) {
+        const f32x4 T0 = mfma16(a[0], e[jg], z);
+        const f32x4 T1 = mfma16(a[1], e[jg], z);
+        t[jg][0] = __builtin_amdgcn_perm(__float_as_uint(T0[1]), __float_as_uint(T0[0]), 0x07060302u);
+        t[jg][1] = __builtin_amdgcn_perm(__float_as_uint(T0[3]), __float_as_uint(T0[2]), 0x07060302u);
+        t[jg][2] = __builtin_amdgcn_perm(__float_as_uint(T1[1]), __float_as_uint(T1[0]), 0x07060302u);
+        t[jg][3] = __builtin_amdgcn_perm(__float_as_uint(T1[3]), __float_as_uint(T1[2]), 0x07060302u);
+    }
+}
+
+template <int NWV, int PD, int NQ, int SP>
+__global__ __launch_bounds__(64 * NWV) void k_gemm_t16(const float* __restrict__ S, int64_t ld,
                                                        const u32x4* __restrict__ pb, const float* __restrict__ P,
                                                        const int4* __restrict__ items, float* __restrict__ y1p,
-                                                       float* __restrict__ y2p, int halves) {
+                                                       float* __restrict__ y2p, int q0) {
     constexpr int GM = 32 * NWV;
     constexpr bool PIPE = NWV == 8;                  // software-pipelined chunk loop (register room)
     constexpr int RING = PD + 1;                     // raw chunks held
-    constexpr int NT = FULL ? 2 : 1;                 // 32-column RHS tiles computed
     constexpr int NTH = 64 * NWV;
-    constexpr int NSTG = FULL ? 768 : 384;           // B pieces a chunk needs (one half: its 32 columns)
-    constexpr int STG = (NSTG + NTH - 1) / NTH;      // B pieces staged per thread and chunk
+    constexpr int NSTG = 192 * NQ;                   // P pieces a chunk needs: 4 groups x 3 pieces x 16*NQ columns
+    constexpr int STG = (NSTG + NTH - 1) / NTH;      // P pieces staged per thread and chunk
     __shared__ u32x4 sB0[12 * 64], sB1[12 * 64];     // a chunk's P pieces, [(g*3 + piece)*64 + n]
     constexpr int SRB = NWV == 8 ? 2 : 1;            // use-2 tile buffers
-    __shared__ float sR[SRB][NWV * 32 * 64];         // use-2 results, [wave][column j][rhs]
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, hl = lane >> 5, ln = lane & 31;
+    __shared__ float sR[SRB][NWV * 32 * 64];         // use-2 results, [wave][column j][rhs - 16*q0]
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, g = lane >> 4, ln = lane & 15;
     const int4 item = items[blockIdx.x];
     const int64_t R = item.x, ca = item.y, cb = item.z, slot = item.w;
     const int64_t nch = ld / 32, G = ld / 8, NR = (nch + NWV - 1) / NWV;
     const int64_t rw = R * GM + wv * 32;             // the wave's first row
     const bool valid = rw < ld;                      // ld = Npad, a multiple of 32
-    const float* srow = S + ((valid ? rw : ld - 32) + ln) * ld + hl * 16;
-    const int hsel = FULL ? 0 : (halves >> 1);       // the live tile of a one-half launch
-    // staging: element e of a chunk's 768 pieces (group e/192, piece (e/64)%3,
-    // column e%64), of the live half's 384 in a one-half launch; threads past
-    // the count repeat an element of the first round
+    const float* srow = S + ((valid ? rw : ld - 32) + ln) * ld + 8 * g;
+    const int c0 = 16 * q0;                          // the launch's first column
+    // staging: element k of a chunk's NSTG pieces is column c0 + k % (16*NQ)
+    // of (group, piece) k / (16*NQ); threads past the count repeat an
+    // element of the first round
     const u32x4* bsrc[STG];
     int bdi[STG];
 #pragma unroll
     for (int s = 0; s < STG; s++) {
         const int k = tid + s * NTH < NSTG ? tid + s * NTH : tid % NSTG;
-        const int e = FULL ? k : (k / 32) * 64 + 32 * hsel + (k % 32);
-        bsrc[s] = pb + ((int64_t)((e / 64) % 3) * G + e / 192) * EW + (e % 64);
-        bdi[s] = e;
+        const int gp = k / (16 * NQ), n = c0 + k % (16 * NQ);
+        bsrc[s] = pb + ((int64_t)(gp % 3) * G + gp / 3) * EW + n;
+        bdi[s] = gp * 64 + n;
     }
-    // E_s as B operand: lane (hl, ln) holds rows k = 8*hl + t of column ln,
-    // E_s[k][n] = 1 when n == 16*(k/8) + 8*s + k%8 (the column of S that step
-    // s gives lane-half k/8 as its k-th value)
+    // E_jg as B operand: lane (ln, g) holds K = columns 8g + [0, 8) of column
+    // ln, E_jg[k][n] = 1 when k == 16*jg + n
     u32x4 esel[2];
 #pragma unroll
-    for (int s = 0; s < 2; s++) {
-        const int t1 = ln - 16 * hl - 8 * s;
+    for (int jg = 0; jg < 2; jg++) {
+        const int t1 = 16 * jg + ln - 8 * g;
 #pragma unroll
         for (int k = 0; k < 4; k++)
-            esel[s][k] = (t1 == 2 * k ? 0x3F80u : 0u) | (t1 == 2 * k + 1 ? 0x3F800000u : 0u);
+            esel[jg][k] = (t1 == 2 * k ? 0x3F80u : 0u) | (t1 == 2 * k + 1 ? 0x3F800000u : 0u);
     }
-    f32x16 acc1[NT];
+    f32x4 acc1[2][NQ];
 #pragma unroll
-    for (int h = 0; h < NT; h++)
+    for (int r = 0; r < 2; r++)
 #pragma unroll
-        for (int r = 0; r < 16; r++) acc1[h][r] = 0.0f;
-    float4 ring[RING][4];
+        for (int t = 0; t < NQ; t++) acc1[r][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    float4 ring[RING][2][2];                         // [slot][row tile][half of the lane's 8 columns]
     u32x4 bring[RING][STG];
-    auto issue = [&](float4 (&rs)[4], u32x4 (&br)[STG], int64_t ch) {
+    auto issue = [&](float4 (&rs)[2][2], u32x4 (&br)[STG], int64_t ch) {
 #pragma unroll
         for (int s = 0; s < STG; s++) br[s] = bsrc[s][4 * ch * EW];
 #pragma unroll
-        for (int q = 0; q < 4; q++) rs[q] = *reinterpret_cast<const float4*>(srow + ch * 32 + 4 * q);
+        for (int r = 0; r < 2; r++)
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+                rs[r][h] = *reinterpret_cast<const float4*>(srow + (int64_t)r * 16 * ld + ch * 32 + 4 * h);
     };
     // the run over the item; U2: chunks left of the diagonal (use 1 and 2)
     auto run = [&](auto u2tag) {
         constexpr bool U2 = decltype(u2tag)::value;
-        // the A operand of use 2: P rows rw + 16*s2 + 8*(t/4) + 4*hl + t%4,
-        // t < 8, column ln of the tile, as three pieces (zero for rows past ld)
-        u32x4 pah[NT][2], pam[NT][2], pal[NT][2];
+        // the A operand of use 2: P rows rw + 4g + tt (tt < 4) and
+        // rw + 16 + 4g + tt - 4 (tt >= 4), column c0 + 16t + ln, as three
+        // pieces (zero for rows past ld)
+        u32x4 pah[NQ], pam[NQ], pal[NQ];
         if (U2) {
 #pragma unroll
-            for (int h = 0; h < NT; h++) {
-                const int col = 32 * (FULL ? h : hsel) + ln;
+            for (int t = 0; t < NQ; t++) {
+                const int col = c0 + 16 * t + ln;
+                float v[8];
 #pragma unroll
-                for (int s2 = 0; s2 < 2; s2++) {
-                    float v[8];
-#pragma unroll
-                    for (int t = 0; t < 8; t++)
-                        v[t] = valid ? P[(rw + 16 * s2 + 8 * (t >> 2) + 4 * hl + (t & 3)) * EW + col] : 0.0f;
-                    split3_8(make_float4(v[0], v[1], v[2], v[3]), make_float4(v[4], v[5], v[6], v[7]), pah[h][s2],
-                             pam[h][s2], pal[h][s2]);
-                }
+                for (int tt = 0; tt < 8; tt++)
+                    v[tt] = valid ? P[(rw + (tt < 4 ? 4 * g + tt : 12 + 4 * g + tt)) * EW + col] : 0.0f;
+                split3_8(make_float4(v[0], v[1], v[2], v[3]), make_float4(v[4], v[5], v[6], v[7]), pah[t], pam[t],
+                         pal[t]);
             }
         }
-        // the sum over the waves of chunk cc's use-2 tiles, stored as its
-        // partial for this row set: 4 values per lane in waves [0, 4*NT) of
-        // a full launch, 2 per lane in waves [0, 8) of a one-half launch
-        // (the same sums in the same wave order, spread over more waves)
+        // the sum over the waves of chunk cc's use-2 tiles (32 columns x
+        // 16*NQ rhs, NQ values per lane of waves [0, 8)), stored as its
+        // partial for this row set
         auto reduce = [&](int64_t cc, const float* sr) {
-            if (!FULL) {
-                if (wv < 8) {
-                    const int o = (wv * 64 + lane) * 2;
-                    const int j = o / 32, ql = o % 32;
-                    const float* src = sr + tri_sr_index(j, ql);
+            if (wv < 8) {
+                const int o = (wv * 64 + lane) * NQ;
+                const int j = o / (16 * NQ), ql = o % (16 * NQ);
+                const float* src = sr + tri_sr_index(j, ql);
+                float* dst = y2p + (tri_y2_first<NWV>(cc, NR) + (R - cc / NWV - 1)) * 2048 + j * 64 + c0 + ql;
+                if constexpr (NQ == 4) {
+                    float4 sum = *reinterpret_cast<const float4*>(src);
+#pragma unroll
+                    for (int w = 1; w < NWV; w++) {
+                        const float4 v = *reinterpret_cast<const float4*>(src + w * 32 * 64);
+                        sum.x += v.x;
+                        sum.y += v.y;
+                        sum.z += v.z;
+                        sum.w += v.w;
+                    }
+                    *reinterpret_cast<float4*>(dst) = sum;
+                } else if constexpr (NQ == 2) {
                     float2 sum = *reinterpret_cast<const float2*>(src);
 #pragma unroll
                     for (int w = 1; w < NWV; w++) {
@@ -860,69 +899,53 @@ __global__ __launch_bounds__(64 * NWV) void k_gemm_x3t(const float* __restrict__
                         sum.x += v.x;
                         sum.y += v.y;
                     }
-                    const int64_t pslot = tri_y2_first<NWV>(cc, NR) + (R - cc / NWV - 1);
-                    *reinterpret_cast<float2*>(y2p + pslot * 2048 + j * 64 + 32 * hsel + ql) = sum;
-                }
-            } else if (wv < 4 * NT) {
-                const int o = (wv * 64 + lane) * 4;
-                const int j = o / (32 * NT), ql = o % (32 * NT);
-                const float* src = sr + tri_sr_index(j, ql);
-                float4 sum = *reinterpret_cast<const float4*>(src);
+                    *reinterpret_cast<float2*>(dst) = sum;
+                } else {
+                    float sum = src[0];
 #pragma unroll
-                for (int w = 1; w < NWV; w++) {
-                    const float4 v = *reinterpret_cast<const float4*>(src + w * 32 * 64);
-                    sum.x += v.x;
-                    sum.y += v.y;
-                    sum.z += v.z;
-                    sum.w += v.w;
+                    for (int w = 1; w < NWV; w++) sum += src[w * 32 * 64];
+                    *dst = sum;
                 }
-                const int qg = FULL ? ql : 32 * hsel + ql;
-                const int64_t pslot = tri_y2_first<NWV>(cc, NR) + (R - cc / NWV - 1);
-                *reinterpret_cast<float4*>(y2p + pslot * 2048 + j * 64 + qg) = sum;
             }
         };
-        // chunk c's MFMA work from its pieces (step s in [.][s]), with its B
+        // chunk c's MFMA work from its pieces ([.][r]: row tile r), with its P
         // pieces in LDS buffer par: use 1, then (U2) the transposes, use 2 and
         // the wave's use-2 tile to LDS
         auto chunk_mfma = [&](int par, int64_t c, const u32x4 (&ah)[2], const u32x4 (&am)[2],
                               const u32x4 (&al)[2]) {
-            const u32x4* sb = par ? sB1 : sB0;
+            const u32x4* sb = (par ? sB1 : sB0) + g * 3 * 64 + c0 + ln;
 #pragma unroll
-            for (int s = 0; s < 2; s++) {
+            for (int r = 0; r < 2; r++) {
 #pragma unroll
-                for (int h = 0; h < NT; h++) {
-                    const u32x4* bb = sb + (2 * hl + s) * 3 * 64 + 32 * (FULL ? h : hsel) + ln;
+                for (int t = 0; t < NQ; t++) {
+                    const u32x4* bb = sb + 16 * t;
                     if (SP == 3)
-                        acc1[h] = mfma_x3(ah[s], am[s], al[s], bb[0], bb[64], bb[128], acc1[h]);
+                        acc1[r][t] = mfma16_x3(ah[r], am[r], al[r], bb[0], bb[64], bb[128], acc1[r][t]);
                     else
-                        acc1[h] = mfma_x3_a2(ah[s], am[s], bb[0], bb[64], bb[128], acc1[h]);
+                        acc1[r][t] = mfma16_a2(ah[r], am[r], bb[0], bb[64], bb[128], acc1[r][t]);
                 }
             }
             if (U2) {
                 u32x4 th[2], tm[2], tl[2];
-                tri_transpose(ah, esel, th);
-                tri_transpose(am, esel, tm);
-                if (SP == 3) tri_transpose(al, esel, tl);
+                t16_transpose(ah, esel, th);
+                t16_transpose(am, esel, tm);
+                if (SP == 3) t16_transpose(al, esel, tl);
+                // with two buffers, chunk c's tile goes to buffer c & 1, last
+                // read for chunk c-2 before this chunk's barrier
+                float* dst = sR[SRB == 2 ? (c & 1) : 0] + wv * 32 * 64;
 #pragma unroll
-                for (int h = 0; h < NT; h++) {
-                    f32x16 c2;
+                for (int t = 0; t < NQ; t++) {
 #pragma unroll
-                    for (int r = 0; r < 16; r++) c2[r] = 0.0f;
-#pragma unroll
-                    for (int s2 = 0; s2 < 2; s2++) {
+                    for (int jg = 0; jg < 2; jg++) {
+                        f32x4 c2 = {0.0f, 0.0f, 0.0f, 0.0f};
                         if (SP == 3)
-                            c2 = mfma_x3(pah[h][s2], pam[h][s2], pal[h][s2], th[s2], tm[s2], tl[s2], c2);
+                            c2 = mfma16_x3(pah[t], pam[t], pal[t], th[jg], tm[jg], tl[jg], c2);
                         else
-                            c2 = mfma_x3_b2(pah[h][s2], pam[h][s2], pal[h][s2], th[s2], tm[s2], c2);
+                            c2 = mfma16_b2(pah[t], pam[t], pal[t], th[jg], tm[jg], c2);
+                        // c2[a]: rhs 16t + 4g + a, column 16*jg + ln
+                        *reinterpret_cast<float4*>(dst + tri_sr_index(16 * jg + ln, 16 * t + 4 * g)) =
+                            make_float4(c2[0], c2[1], c2[2], c2[3]);
                     }
-                    // c2[a]: rhs 8*(a/4) + 4*hl + a%4 of the tile, column ln.  With
-                    // two buffers, chunk c's tile goes to buffer c & 1, last read
-                    // for chunk c-2 before this chunk's barrier
-                    float* dst = sR[SRB == 2 ? (c & 1) : 0] + wv * 32 * 64;
-#pragma unroll
-                    for (int g = 0; g < 4; g++)
-                        *reinterpret_cast<float4*>(dst + tri_sr_index(ln, 32 * h + 8 * g + 4 * hl)) =
-                            make_float4(c2[4 * g], c2[4 * g + 1], c2[4 * g + 2], c2[4 * g + 3]);
                 }
                 if (SRB == 1) {
                     lds_barrier();
@@ -940,33 +963,32 @@ __global__ __launch_bounds__(64 * NWV) void k_gemm_x3t(const float* __restrict__
             if (pre) issue(ring[(u + PD) % RING], bring[(u + PD) % RING], c + PD < cb ? c + PD : cb - 1);
             u32x4 ah[2], am[2], al[2];
 #pragma unroll
-            for (int s = 0; s < 2; s++) {
+            for (int r = 0; r < 2; r++) {
                 if (SP == 3)
-                    split3_8(ring[u][2 * s], ring[u][2 * s + 1], ah[s], am[s], al[s]);
+                    split3_8(ring[u][r][0], ring[u][r][1], ah[r], am[r], al[r]);
                 else
-                    split2_8(ring[u][2 * s], ring[u][2 * s + 1], ah[s], am[s]);
+                    split2_8(ring[u][r][0], ring[u][r][1], ah[r], am[r]);
             }
             chunk_mfma(par, c, ah, am, al);
         };
         if (PIPE) {
-            // software-pipelined: chunk c's pieces were split during chunk c-1,
-            // so this chunk's MFMAs run beside the next chunk's split.  Raw
-            // chunks c+1 .. c+RING are resident or in flight (slot (c-ca) %
-            // RING), and so are their B pieces; chunk ids past the item's end
-            // are clamped to its last chunk, whose data is then split or
+            // software-pipelined: chunk c's pieces were split during chunk
+            // c-1, so this chunk's MFMAs run beside the next chunk's split.
+            // Raw chunks c+1 .. c+RING are resident or in flight (slot (c-ca)
+            // % RING), and so are their P pieces; chunk ids past the item's
+            // end are clamped to its last chunk, whose data is then split or
             // staged unused.  The loop is unrolled by UNR, a multiple of RING
             // and of 2, so slots and the pieces' set (the chunk's parity) are
-            // compile-time indices.  RING 3 fits the registers of one-half
-            // launches only; the autotuner times both.
+            // compile-time indices.
             constexpr int UNR = RING == 3 ? 6 : 2;
-            u32x4 pcs[2][SP][2];                      // [set][piece][step]
-            auto split_into = [&](const float4 (&rs)[4], u32x4 (&p)[SP][2]) {
+            u32x4 pcs[2][SP][2];                      // [set][piece][row tile]
+            auto split_into = [&](const float4 (&rs)[2][2], u32x4 (&p)[SP][2]) {
 #pragma unroll
-                for (int st = 0; st < 2; st++) {
+                for (int r = 0; r < 2; r++) {
                     if constexpr (SP == 3)
-                        split3_8(rs[2 * st], rs[2 * st + 1], p[0][st], p[1][st], p[2][st]);
+                        split3_8(rs[r][0], rs[r][1], p[0][r], p[1][r], p[2][r]);
                     else
-                        split2_8(rs[2 * st], rs[2 * st + 1], p[0][st], p[1][st]);
+                        split2_8(rs[r][0], rs[r][1], p[0][r], p[1][r]);
                 }
             };
             // k: the chunk's offset from ca modulo UNR
@@ -1025,14 +1047,11 @@ __global__ __launch_bounds__(64 * NWV) void k_gemm_x3t(const float* __restrict__
     if (!valid) return;
     float* yp = y1p + slot * GM * EW;
 #pragma unroll
-    for (int h = 0; h < NT; h++) {
-        const int n = 32 * (FULL ? h : hsel) + ln;
+    for (int r = 0; r < 2; r++)
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
-            const int64_t row = wv * 32 + 8 * (r >> 2) + 4 * hl + (r & 3);
-            yp[row * EW + n] = acc1[h][r];
-        }
-    }
+        for (int t = 0; t < NQ; t++)
+#pragma unroll
+            for (int a = 0; a < 4; a++) yp[(int64_t)(wv * 32 + 16 * r + 4 * g + a) * EW + c0 + 16 * t + ln] = acc1[r][t][a];
 }
 
 // k_tri_reduce: Y[row][q] = the row set's use-1 partials (rtab[R] = {first
@@ -1703,31 +1722,35 @@ hipError_t tri_plan_build(TriPlan& t, int64_t Npad, int nwv, int ch) {
 }
 
 namespace {
+template <int NWV, int PD, int NQ, int SP>
+void launch_t16(hipStream_t s, dim3 grid, const float* S, int64_t Npad, const u32x4* pb, const float* in,
+                const TriPlan& t, int q0, float* ypart) {
+    hipLaunchKernelGGL((k_gemm_t16<NWV, PD, NQ, SP>), grid, dim3(64 * NWV), 0, s, S, Npad, pb, in, t.items, t.y1p,
+                       t.y2p, q0);
+    hipLaunchKernelGGL((k_tri_reduce<NWV, 4 * NQ>), dim3((unsigned)(Npad * 4 * NQ / 256)), dim3(256), 0, s, t.y1p,
+                       t.y2p, t.rtab, Npad, 16 * q0, ypart);
+}
+
+// the lower-triangle product over the launch's 64, 32 (halves 1 or 2) or 16
+// (quarter >= 0: columns 16*quarter + [0, 16)) columns
 template <int NWV, int SP>
 hipError_t launch_tri(hipStream_t s, const float* S, int64_t Npad, const float* in, float* ypart, int pd,
-                      int halves, const TriPlan& t) {
+                      int halves, int quarter, const TriPlan& t) {
     const int64_t G = Npad / 8;
     u32x4* pb = reinterpret_cast<u32x4*>(ypart + (int64_t)GSPLIT_MAX * Npad * EW);
     hipLaunchKernelGGL(k_split_p, dim3((unsigned)(G * EW / 256)), dim3(256), 0, s, in, G, pb);
-    const dim3 grid((unsigned)t.nitems), block(64 * NWV);
-    if (halves == 3) {
-        if (pd == 3)
-            hipLaunchKernelGGL((k_gemm_x3t<NWV, 2, true, SP>), grid, block, 0, s, S, Npad, pb, in, t.items, t.y1p,
-                               t.y2p, halves);
-        else
-            hipLaunchKernelGGL((k_gemm_x3t<NWV, 1, true, SP>), grid, block, 0, s, S, Npad, pb, in, t.items, t.y1p,
-                               t.y2p, halves);
-        hipLaunchKernelGGL((k_tri_reduce<NWV, 16>), dim3((unsigned)(Npad / 16)), dim3(256), 0, s, t.y1p, t.y2p,
-                           t.rtab, Npad, 0, ypart);
+    const dim3 grid((unsigned)t.nitems);
+    const bool deep = pd == 3;
+    if (quarter >= 0) {
+        if (deep) launch_t16<NWV, 2, 1, SP>(s, grid, S, Npad, pb, in, t, quarter, ypart);
+        else launch_t16<NWV, 1, 1, SP>(s, grid, S, Npad, pb, in, t, quarter, ypart);
+    } else if (halves == 3) {
+        if (deep) launch_t16<NWV, 2, 4, SP>(s, grid, S, Npad, pb, in, t, 0, ypart);
+        else launch_t16<NWV, 1, 4, SP>(s, grid, S, Npad, pb, in, t, 0, ypart);
     } else {
-        if (pd == 3)
-            hipLaunchKernelGGL((k_gemm_x3t<NWV, 2, false, SP>), grid, block, 0, s, S, Npad, pb, in, t.items, t.y1p,
-                               t.y2p, halves);
-        else
-            hipLaunchKernelGGL((k_gemm_x3t<NWV, 1, false, SP>), grid, block, 0, s, S, Npad, pb, in, t.items, t.y1p,
-                               t.y2p, halves);
-        hipLaunchKernelGGL((k_tri_reduce<NWV, 8>), dim3((unsigned)(Npad / 32)), dim3(256), 0, s, t.y1p, t.y2p,
-                           t.rtab, Npad, halves == 2 ? 32 : 0, ypart);
+        const int q0 = halves == 2 ? 2 : 0;
+        if (deep) launch_t16<NWV, 2, 2, SP>(s, grid, S, Npad, pb, in, t, q0, ypart);
+        else launch_t16<NWV, 1, 2, SP>(s, grid, S, Npad, pb, in, t, q0, ypart);
     }
     return hipGetLastError();
 }
@@ -1735,7 +1758,7 @@ hipError_t launch_tri(hipStream_t s, const float* S, int64_t Npad, const float* 
 
 hipError_t launch_grm_gemm(hipStream_t s, const float* S, int64_t Npad, int64_t Nnm, const float* in,
                            float* out, SigmaArgs a, double* partial, int64_t* nparts, float* ypart, GemmConfig g,
-                           int halves, const TriPlan* tri, bool s16) {
+                           int halves, const TriPlan* tri, bool s16, int quarter) {
     // Npad is a multiple of 256 = 128 * RT for both RT; a slice holds at least
     // one superchunk when Npad / GKS >= nsplit
     if (!gemm_config_valid(g) || Npad % 256 != 0 || halves < 1 || halves > 3) return hipErrorInvalidValue;
@@ -1743,12 +1766,13 @@ hipError_t launch_grm_gemm(hipStream_t s, const float* S, int64_t Npad, int64_t 
         // the lower-triangle product sums its partials into slice 0
         if (!tri || tri->Npad != Npad || !tri->items) return hipErrorInvalidValue;
         hipError_t e;
+        if (quarter >= 4 || (quarter >= 0 && halves != 1 << (quarter / 2))) return hipErrorInvalidValue;
         if (s16)
-            e = tri->nwv == 8 ? launch_tri<8, 2>(s, S, Npad, in, ypart, g.pd, halves, *tri)
-                              : launch_tri<12, 2>(s, S, Npad, in, ypart, g.pd, halves, *tri);
+            e = tri->nwv == 8 ? launch_tri<8, 2>(s, S, Npad, in, ypart, g.pd, halves, quarter, *tri)
+                              : launch_tri<12, 2>(s, S, Npad, in, ypart, g.pd, halves, quarter, *tri);
         else
-            e = tri->nwv == 8 ? launch_tri<8, 3>(s, S, Npad, in, ypart, g.pd, halves, *tri)
-                              : launch_tri<12, 3>(s, S, Npad, in, ypart, g.pd, halves, *tri);
+            e = tri->nwv == 8 ? launch_tri<8, 3>(s, S, Npad, in, ypart, g.pd, halves, quarter, *tri)
+                              : launch_tri<12, 3>(s, S, Npad, in, ypart, g.pd, halves, quarter, *tri);
         if (e != hipSuccess) return e;
         const int64_t nb = Npad / EROWS;
         hipLaunchKernelGGL(k_gemm_epi, dim3((unsigned)nb), dim3(256), 0, s, ypart, 1, Npad, Nnm, in, out, a,

@@ -395,10 +395,17 @@ int engine_step(saige_ctx* c, Engine& e, int n_iter, uint8_t* done_out, int32_t*
     SAIGE_HIP_TRY(hipStreamSynchronize(st));
     // 32-column halves of the block that hold an occupied slot; the GEMM skips
     // the others (a done slot that is not yet taken is recomputed unchanged)
-    int halves = 0;
+    int halves = 0, quarters = 0;
     for (int k = 0; k < e.kpad; k++)
-        if (e.h_live[k]) halves |= 1 << (k / 32);
+        if (e.h_live[k]) {
+            halves |= 1 << (k / 32);
+            quarters |= 1 << (k / 16);
+        }
     if (halves == 0) halves = 1;
+    // the lower-triangle product can also run one 16-column quarter alone
+    int quarter = -1;
+    if (e.kpad != 1 && c->gemm_cfg.mode == GEMM_BF16X3_TRI && quarters != 0 && (quarters & (quarters - 1)) == 0)
+        quarter = __builtin_ctz((unsigned)quarters);
     for (int it = 0; it < n_iter; it++) {
         bool any = false;
         for (int k = 0; k < e.kpad; k++) any |= (e.h_live[k] && !e.h_done[k]);
@@ -409,7 +416,10 @@ int engine_step(saige_ctx* c, Engine& e, int n_iter, uint8_t* done_out, int32_t*
             SAIGE_HIP_TRY(launch_grm_gemv(st, g.S, Np, N, e.P, e.AP, a, e.partial, &np, tv));
         else
             SAIGE_HIP_TRY(launch_grm_gemm(st, g.S, Np, N, e.P, e.AP, a, e.partial, &np, e.ypart,
-                                          halves == 3 ? c->gemm_cfg : c->gemm_cfg_half, halves, &c->tri, g.s16));
+                                          quarter >= 0 ? c->gemm_cfg_quarter
+                                          : halves == 3 ? c->gemm_cfg
+                                                        : c->gemm_cfg_half,
+                                          halves, &c->tri, g.s16, quarter));
         SAIGE_HIP_TRY(hipEventRecord(c->ev1, st));
         SAIGE_HIP_TRY(launch_alpha(st, e.kpad, e.partial, np, e.rz, e.done, e.live, e.kind, e.alpha));
         int64_t np2 = 0;
@@ -422,7 +432,10 @@ int engine_step(saige_ctx* c, Engine& e, int n_iter, uint8_t* done_out, int32_t*
         SAIGE_HIP_TRY(hipStreamSynchronize(st));
         float ms = 0.f;
         SAIGE_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        const int kind = e.kpad == 1 ? 0 : (halves == 3 ? 2 : 1);
+        const int kind = e.kpad == 1 ? SAIGE_LAUNCH_GEMV
+                         : quarter >= 0 ? SAIGE_LAUNCH_GEMM_QUARTER
+                         : halves == 3  ? SAIGE_LAUNCH_GEMM_FULL
+                                        : SAIGE_LAUNCH_GEMM_HALF;
         e.gemm_seconds[kind] += ms * 1e-3;
         e.gemm_launches[kind]++;
     }
@@ -1216,9 +1229,13 @@ static int grm_matvec(saige_ctx* c, int loco, const float* B, int64_t k, float* 
         const int kb = (int)std::min<int64_t>(EW, k - k0);
         e = hipMemcpyAsync(d_in, B + k0 * N, 4 * N * kb, hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = launch_scatter_cols(st, d_in, kb, N, Np, EW, d_blk);
+        // the lower-triangle product runs as narrow as the block: 16, 32 or
+        // 64 columns
+        const bool q = kb <= 16 && c->gemm_cfg.mode == GEMM_BF16X3_TRI;
         if (e == hipSuccess)
             e = launch_grm_gemm(st, g.S, Np, N, d_blk, d_out, a, nullptr, nullptr, c->d_mv_ypart,
-                                kb <= 32 ? c->gemm_cfg_half : c->gemm_cfg, kb <= 32 ? 1 : 3, &c->tri, g.s16);
+                                q ? c->gemm_cfg_quarter : kb <= 32 ? c->gemm_cfg_half : c->gemm_cfg,
+                                kb <= 32 ? 1 : 3, &c->tri, g.s16, q ? 0 : -1);
         if (e == hipSuccess) e = launch_gather_cols(st, d_out, kb, N, EW, d_in);
         if (e == hipSuccess) e = hipMemcpyAsync(Y + k0 * N, d_in, 4 * N * kb, hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -1873,13 +1890,13 @@ int saige_autotune(saige_ctx* c, int reps, double* ms_table) {
     a.mode = 0;
     // time one shape: one warm-up launch, then reps launches between events
     const bool s16 = grm_view(c, 0).s16;
-    auto time_shape = [&](GemmConfig g, int halves, double* out_ms) -> hipError_t {
+    auto time_shape = [&](GemmConfig g, int halves, double* out_ms, int quarter = -1) -> hipError_t {
         hipError_t e = launch_grm_gemm(st, c->d_grm, Np, N, c->d_mv_blk, c->d_mv_out, a, nullptr, nullptr,
-                                       c->d_mv_ypart, g, halves, &c->tri, s16);
+                                       c->d_mv_ypart, g, halves, &c->tri, s16, quarter);
         if (e == hipSuccess) e = hipEventRecord(c->ev0, st);
         for (int r = 0; r < reps && e == hipSuccess; r++)
             e = launch_grm_gemm(st, c->d_grm, Np, N, c->d_mv_blk, c->d_mv_out, a, nullptr, nullptr, c->d_mv_ypart,
-                                g, halves, &c->tri, s16);
+                                g, halves, &c->tri, s16, quarter);
         if (e == hipSuccess) e = hipEventRecord(c->ev1, st);
         if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
         float ms = 0.f;
@@ -1916,17 +1933,17 @@ int saige_autotune(saige_ctx* c, int reps, double* ms_table) {
     c->gemm_cfg = best;
     if (int rc = ensure_tri(c)) return rc;
     // one-half launches: rt and pd only, the slice count stays the full shape's
-    GemmConfig best_h = best;
+    GemmConfig best_h = best, best_q = best;
     double best_h_ms = -1.0;
     if (tri) {
-        // rt fixes the summation order, so full and one-half launches share
-        // it: per rt the fastest pd of each, and the rt of the least
-        // full + 2 x half time (a fit round runs about twice as many
-        // one-half launches as full ones)
+        // rt fixes the summation order, so the 64-, 32- and 16-column
+        // launches share it: per rt the fastest pd of each, and the rt of the
+        // least full + half/2 + 16-column time (a fit round runs about as
+        // many 16-column launches as full ones, fewer one-half ones)
         double score = -1.0;
         for (int rt = 1; rt <= 2; rt++) {
-            GemmConfig bf{}, bh{};
-            double mf = -1.0, mh = -1.0;
+            GemmConfig bf{}, bh{}, bq{};
+            double mf = -1.0, mh = -1.0, mq = -1.0;
             for (int i = 0; i < kNumGemmCandidates; i++)
                 if (kGemmCandidates[i].rt == rt && tfull[i] >= 0.0) {
                     const double t = tfull[i];
@@ -1947,13 +1964,19 @@ int saige_autotune(saige_ctx* c, int reps, double* ms_table) {
                     bh = g;
                     mh = per;
                 }
+                SAIGE_HIP_TRY(time_shape(g, 1, &per, 0));
+                if (mq < 0.0 || per < mq) {
+                    bq = g;
+                    mq = per;
+                }
             }
-            if (mf >= 0.0 && (score < 0.0 || mf + 2.0 * mh < score)) {
-                score = mf + 2.0 * mh;
+            if (mf >= 0.0 && (score < 0.0 || mf + 0.5 * mh + mq < score)) {
+                score = mf + 0.5 * mh + mq;
                 best = GemmConfig{rt, 8, bf.pd, mode};
                 best_ms = mf;
                 best_h = bh;
                 best_h_ms = mh;
+                best_q = bq;
             }
         }
         c->gemm_cfg = best;
@@ -1974,6 +1997,7 @@ int saige_autotune(saige_ctx* c, int reps, double* ms_table) {
     c->gemm_tuned_ms = best_ms;
     c->gemm_cfg_half = best_h;
     c->gemm_tuned_ms_half = best_h_ms;
+    c->gemm_cfg_quarter = tri ? best_q : best_h;
     return SAIGE_OK;
 }
 
@@ -1998,6 +2022,7 @@ int saige_set_gemm_config(saige_ctx* c, int rt, int pd, int nsplit) {
     c->gemm_cfg = g;
     c->gemm_tuned_ms = 0.0;
     c->gemm_cfg_half = g;
+    c->gemm_cfg_quarter = g;
     c->gemm_tuned_ms_half = 0.0;
     return SAIGE_OK;
 }
@@ -2014,6 +2039,7 @@ int saige_set_gemm_mode(saige_ctx* c, int mode) {
         // timings of the other mode do not carry over
         c->gemm_cfg.mode = mode;
         c->gemm_cfg_half.mode = mode;
+        c->gemm_cfg_quarter.mode = mode;
         c->gemm_tuned_ms = 0.0;
         c->gemm_tuned_ms_half = 0.0;
     }
@@ -2077,7 +2103,7 @@ int saige_engine_stats(const saige_ctx* c, int kind, double* secs, int64_t* laun
         set_error("null context");
         return SAIGE_E_ARG;
     }
-    if (kind < 0 || kind > 2) {
+    if (kind < 0 || kind > 3) {
         set_error("unknown launch kind %d", kind);
         return SAIGE_E_ARG;
     }

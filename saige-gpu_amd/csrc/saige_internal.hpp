@@ -135,8 +135,8 @@ struct Engine {
     std::vector<uint8_t> h_live;
     // device time of the GRM product launches by kind: [0] GEMV (width 1),
     // [1] GEMM over one 32-column half, [2] GEMM over both halves
-    double gemm_seconds[3] = {0.0, 0.0, 0.0};
-    int64_t gemm_launches[3] = {0, 0, 0};
+    double gemm_seconds[4] = {0.0, 0.0, 0.0, 0.0};   // by SAIGE_LAUNCH_* kind
+    int64_t gemm_launches[4] = {0, 0, 0, 0};
 };
 
 }  // namespace saige
@@ -180,6 +180,7 @@ struct saige_ctx {
     // shape for launches over one 32-column half: same nsplit (the summation
     // order, so a column's bits do not depend on the other half), own rt/pd
     saige::GemmConfig gemm_cfg_half;
+    saige::GemmConfig gemm_cfg_quarter;     // 16-column launches (lower-triangle mode)
     saige::TriPlan tri;                          // lower-triangle product (mode 2), built on first use
     int gemv_mode = SAIGE_GEMV_TRI;              // single-column products: lower triangle or all of S
     saige::GemvPlan triv;                        // the lower-triangle GEMV's plan, built on first use
@@ -318,9 +319,12 @@ bool gemm_config_valid(GemmConfig g);
 // s16: every value of S has at most 16 significant bits (the split GRM's
 // storage); the lower-triangle mode then splits S into two BF16 pieces
 // instead of three, with the same bits.
+// quarter >= 0 (lower-triangle mode): only columns 16*quarter + [0, 16) are
+// computed (halves must be the half that holds them).
 hipError_t launch_grm_gemm(hipStream_t s, const float* S, int64_t Npad, int64_t Nnm, const float* in,
                            float* out, SigmaArgs a, double* partial, int64_t* nparts, float* ypart,
-                           GemmConfig g, int halves = 3, const TriPlan* tri = nullptr, bool s16 = false);
+                           GemmConfig g, int halves = 3, const TriPlan* tri = nullptr, bool s16 = false,
+                           int quarter = -1);
 int64_t gemm_ypart_floats(int64_t Npad);
 int64_t gemv_max_parts(int64_t Npad);
 int64_t gemm_max_parts(int64_t Npad);

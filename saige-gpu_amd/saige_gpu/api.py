@@ -488,7 +488,7 @@ class Session:
         check(self._L.saige_engine_take(self.ctx, int(slot), ptr(x)))
         return x
 
-    LAUNCH_KINDS = ("gemv", "gemm_half", "gemm_full")
+    LAUNCH_KINDS = ("gemv", "gemm_half", "gemm_full", "gemm_quarter")
 
     def fit_null_batch(self, jobs, opt=None, max_concurrent=16):
         """Null fits through the library's C++ drivers (saige_fit_null_batch):
@@ -515,7 +515,8 @@ class Session:
 
     def engine_stats(self, kind=None):
         """Device seconds and launch count of the GRM product launches: for one
-        kind ("gemv", "gemm_half", "gemm_full") or summed over all."""
+        kind ("gemv", "gemm_half", "gemm_full", "gemm_quarter") or summed over
+        all."""
         kinds = self.LAUNCH_KINDS if kind is None else (kind,)
         tot_s, tot_n = 0.0, 0
         for k in kinds:

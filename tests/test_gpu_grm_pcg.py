@@ -342,8 +342,8 @@ def test_gemm_bf16x3_tri_mode(session, grm1000):
     S_IJ P_J and, transposed on the MFMA, S_IJ^T P_I): exact on unit vectors
     (the transpose and every partial carry one exact value), within 2^-21 of
     s*b for one nonzero entry, K B within the FP32 product tolerance, the
-    same bits for both prefetch depths at one wave count and for one-half vs
-    full launches, PCG solves as in FP32 mode, and an engine solve with the
+    same bits for both prefetch depths at one wave count and for 16-column,
+    one-half and full launches, PCG solves as in FP32 mode, and an engine solve with the
     same bits alone and in a full mix.  N = 1000 pads to 1024: the last
     row set is partial (waves past the padded size read a clamped row and
     store nothing)."""
@@ -386,8 +386,10 @@ def test_gemm_bf16x3_tri_mode(session, grm1000):
             if rt in by_rt:
                 assert np.array_equal(KB, by_rt[rt]), (rt, pd)
             by_rt[rt] = KB
-            narrow = session.grm_matvec(B[:, :5])
+            narrow = session.grm_matvec(B[:, :5])           # a 16-column launch
             assert np.array_equal(narrow.view(np.uint32), KB[:, :5].view(np.uint32))
+            half = session.grm_matvec(B[:, :20])            # a 32-column launch
+            assert np.array_equal(half.view(np.uint32), KB[:, :20].view(np.uint32))
             X, _ = session.pcg_block(w, tau, B[:, :3], 500, 1e-5)
             assert np.max(np.abs(X - X64)) <= 2e-3 * np.max(np.abs(X64)), (rt, pd)
         session.set_gemm_config(1, 1, 8)
@@ -424,7 +426,7 @@ def test_gemm_tri_mode_many_items(session):
     """The lower-triangle product where row sets span several items (N =
     4700: 13 row sets of 384 rows, up to 144 chunks left of the diagonal
     cut into items of at most 96), against the FP64 product of the GPU's own
-    FP32 GRM, for both wave counts, full and one-half launches."""
+    FP32 GRM, for both wave counts, full, one-half and 16-column launches."""
     N, M = 4700, 300
     print("case: GEMM lower-triangle mode, N=4700", flush=True)
     body, G, mm = random_bed(N, M, 0.01, seed=47, maf_lo=0.05, maf_hi=0.5)
@@ -449,6 +451,8 @@ def test_gemm_tri_mode_many_items(session):
             assert err <= 2e-5 * np.max(np.abs(RB)), rt
             half = session.grm_matvec(B[:, :20])
             assert np.array_equal(half.view(np.uint32), KB[:, :20].view(np.uint32)), rt
+            quarter = session.grm_matvec(B[:, :12])
+            assert np.array_equal(quarter.view(np.uint32), KB[:, :12].view(np.uint32)), rt
     finally:
         session.set_gemm_mode("fp32")
         session.set_gemm_config(1, 1, 4)
@@ -479,9 +483,9 @@ def test_split_grm_two_piece_gemm(session, monkeypatch):
         for rt in (1, 2):
             session.set_gemm_config(rt, 3, 8)
             monkeypatch.delenv("SAIGE_GEMM_S_PIECES", raising=False)
-            two = session.grm_matvec(B), session.grm_matvec(B[:, :20])
+            two = session.grm_matvec(B), session.grm_matvec(B[:, :20]), session.grm_matvec(B[:, :9])
             monkeypatch.setenv("SAIGE_GEMM_S_PIECES", "3")
-            three = session.grm_matvec(B), session.grm_matvec(B[:, :20])
+            three = session.grm_matvec(B), session.grm_matvec(B[:, :20]), session.grm_matvec(B[:, :9])
             for a, b in zip(two, three):
                 assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), rt
             assert np.max(np.abs(two[0] - RB)) <= 2e-5 * np.max(np.abs(RB)), rt
@@ -498,12 +502,12 @@ def test_split_grm_two_piece_gemm(session, monkeypatch):
                 monkeypatch.setenv("SAIGE_GEMM_S_PIECES", pieces)
             else:
                 monkeypatch.delenv("SAIGE_GEMM_S_PIECES", raising=False)
-            for k in (64, 20):
+            for k in (64, 20, 9):
                 session.engine_add_products(list(range(k)), [0] * k, B[:, :k])
                 done, _ = session.engine_step(1)
                 assert all(done[:k])
                 out[pieces, k] = session.engine_take_batch(list(range(k)))
-        for k in (64, 20):
+        for k in (64, 20, 9):
             assert np.array_equal(out[None, k].view(np.uint32), out["3", k].view(np.uint32)), k
         session.engine_use_loco(False)
     finally:
