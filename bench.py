@@ -39,10 +39,12 @@ FP32_MFMA_PEAK = 157.3
 # (mode, N, rt, pd, nsplit).  Sources: PROFILE_FILES
 PROFILED_GEMM_TRAFFIC = {("fp32", 100_000, 1, 3, 8): (42.10 + 0.205 + 0.108 + 0.026) * 1e9,
                          ("bf16x3", 100_000, 1, 3, 8): (0.064 + 41.579 + 0.152) * 1e9,
-                         # lower triangle, eight waves, on the split GRM (two pieces of S):
-                         # all four kernels of a launch (pd 3 spills)
-                         ("bf16x3_tri", 100_000, 2, 1, 8): (26.65 + 5.54) * 1e9,
-                         ("bf16x3_tri", 100_000, 2, 3, 8): (31.15 + 5.92) * 1e9}
+                         # lower triangle on 16x16 tiles, on the split GRM (two pieces of
+                         # S): all four kernels of a launch (pd 3 spills)
+                         ("bf16x3_tri", 100_000, 1, 1, 8): (25.01 + 4.08) * 1e9,
+                         ("bf16x3_tri", 100_000, 2, 1, 8): (26.63 + 5.54) * 1e9,
+                         ("bf16x3_tri", 100_000, 1, 3, 8): (46.77 + 6.12) * 1e9,
+                         ("bf16x3_tri", 100_000, 2, 3, 8): (29.99 + 9.17) * 1e9}
 PROFILE_FILES = {"fp32": "profiles/kernels_N100k_M150k_bf16.md", "bf16x3": "profiles/gemm_bf16x3_N100k_M150k.md",
                  "bf16x3_tri": "profiles/kernels_N100k_M150k_split_grm.md"}
 BF16_MFMA_PEAK = 2500.0
@@ -766,8 +768,8 @@ def main():
         # on the split GRM (stored at 16 significant bits) S has two pieces:
         # five piece products per FP32 product instead of six
         npc = 5 if sess.grm_precision() == "bf16_split" else 6
-        roofline["kernel"] = ("k_split_p + k_gemm_x3t + k_tri_reduce + k_gemm_epi (block-PCG GRM x 64 RHS from "
-                              "the lower triangle, P as three BF16 pieces, S as " +
+        roofline["kernel"] = ("k_split_p + k_gemm_t16 + k_tri_reduce + k_gemm_epi (block-PCG GRM x 64 RHS from "
+                              "the lower triangle on 16x16x32 MFMA tiles, P as three BF16 pieces, S as " +
                               ("two (the split GRM's 16-bit values)" if npc == 5 else "three") + ")")
         roofline["bytes"] = f"2*N^2 = {2.0 * N * N:.3e} B of S's lower triangle per launch"
         roofline["hbm_GBs_of_triangle"] = 2.0 * N * N / gemm_avg / 1e9
