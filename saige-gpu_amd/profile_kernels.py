@@ -11,13 +11,14 @@ Kernels exercised: decode/QC (k_decode_stats), repack, diag, the GRM SYRK
 (2-bit input k_syrk_pk256 for --grm bf16; k_syrk_pk256_split, hi.hi + hi.lo
 + lo.hi from two tables, and k_mirror_diag for bf16_split; --grm takes a
 comma-separated list, built in turn, the last one resident), the
-autotuner's sixteen block-GEMM
-shapes, the tuned 64-column block GEMM (k_split_p + k_gemm_x3t + k_tri_reduce
-+ k_gemm_epi in the default --gemm-mode bf16x3_tri, k_split_p + k_gemm_x3w/x3
-+ k_gemm_epi in bf16x3, k_gemm_part + k_gemm_epi in fp32, through
-saige_grm_matvec) and its one-half (16-column) form, the single-column GEMV
-(k_gemv), one LOCO range build (copy + range SYRK in subtract mode over
---loco-frac of the markers), and the runtime's copy kernel on COPY_BYTES.
+autotuner's block-GEMM shapes, the tuned 64-column block GEMM (k_split_p +
+k_gemm_t16 + k_tri_reduce + k_gemm_epi in the default --gemm-mode
+bf16x3_tri, k_split_p + k_gemm_x3w/x3 + k_gemm_epi in bf16x3, k_gemm_part +
+k_gemm_epi in fp32, through saige_grm_matvec) and its 16-column form (in
+bf16x3_tri a 16-column launch, else a one-half launch), the single-column
+GEMV (k_gemv_tri), one LOCO range build (copy + range SYRK in subtract mode
+over --loco-frac of the markers), and the runtime's copy kernel on
+COPY_BYTES.
 """
 import argparse
 import os
