@@ -777,6 +777,9 @@ def main():
                                            "frac": npc * gemm_flop / gemm_avg / 1e12 / BF16_MFMA_PEAK,
                                            "note": f"{npc} BF16 piece products per FP32 product (the emulation's "
                                                    "cost)"}
+        roofline["note"] = ("FP32-accurate products from BF16 piece products on the BF16 MFMA: the FP32-flop rate "
+                            "may exceed the FP32 MFMA peak (frac > 1); the matrix unit the kernel runs on is the "
+                            "BF16 one (bf16_piece_products)")
     else:
         roofline["kernel"] = ("k_split_p + k_gemm_x3w (rt 1) or k_gemm_x3 (rt 2) + k_gemm_epi (block-PCG GRM x 64 "
                               "RHS, FP32 operands as three BF16 pieces on the BF16 MFMA)")
