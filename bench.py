@@ -435,7 +435,9 @@ def c4_leg(args):
            "fit_seconds": t1 - t0, "theta": list(map(float, fit["theta"])), "vr_seconds": t2 - t1,
            "variance_ratios": [row[0] for row in vr["table"]], "vr_markers": [len(r) for r in vr["ratios"]],
            "gemm_avg_ms": gemm_avg * 1e3, "gemm_tflops": 2.0 * N * N * 64 / gemm_avg / 1e12,
-           "gemm_hbm_GBs": 4.0 * N * N / gemm_avg / 1e9, "gemm_mode": args.gemm_mode, "gemm_autotune": tune}
+           # bytes of S a product must read: its lower triangle in that mode
+           "gemm_hbm_GBs": (2.0 if args.gemm_mode == "bf16x3_tri" else 4.0) * N * N / gemm_avg / 1e9,
+           "gemm_mode": args.gemm_mode, "gemm_autotune": tune}
     log(f"C4: fit {t1 - t0:.1f}s theta {fit['theta']}, VR {t2 - t1:.1f}s ratios {out['variance_ratios']}")
     print(json.dumps(out))
 
