@@ -79,21 +79,52 @@ void dfree(T*& p) {
     p = nullptr;
 }
 
+// the per-slot state block: lists [2][kpad] and slot systems, maxiter [kpad]
+// int32, tol [kpad] double, live and kind [kpad] bytes, take lists [3][kpad]
+// int32
+int64_t slot_state_bytes(int kpad) { return 40 * (int64_t)kpad; }
+
+void set_slot_state_views(Engine& e) {
+    const int64_t k = e.kpad;
+    uint8_t* d = e.slot_state;
+    uint8_t* h = e.h_state;
+    e.d_list = reinterpret_cast<int32_t*>(d);
+    e.h_list = reinterpret_cast<int32_t*>(h);
+    e.slot_sys = reinterpret_cast<int32_t*>(d + 8 * k);
+    e.h_slot_sys = reinterpret_cast<int32_t*>(h + 8 * k);
+    e.maxiter = reinterpret_cast<int32_t*>(d + 12 * k);
+    e.h_maxiter = reinterpret_cast<int32_t*>(h + 12 * k);
+    e.tol = reinterpret_cast<double*>(d + 16 * k);
+    e.h_tol = reinterpret_cast<double*>(h + 16 * k);
+    e.live = d + 24 * k;
+    e.h_live = h + 24 * k;
+    e.kind = d + 25 * k;
+    e.h_kind = h + 25 * k;
+    e.d_take = reinterpret_cast<int32_t*>(d + 28 * k);
+    e.h_take = reinterpret_cast<int32_t*>(h + 28 * k);
+}
+
+// uploads the host mirror of the per-slot state (asynchronous on the
+// context's stream: the mirror is left unchanged until the stream is synced)
+hipError_t push_slot_state(saige_ctx* c, Engine& e) {
+    return hipMemcpyAsync(e.slot_state, e.h_state, slot_state_bytes(e.kpad), hipMemcpyHostToDevice, c->stream);
+}
+
 void free_engine(Engine& e) {
     dfree(e.X); dfree(e.R); dfree(e.Z); dfree(e.P); dfree(e.AP);
     dfree(e.winv); dfree(e.minv); dfree(e.mask); dfree(e.tau);
-    dfree(e.slot_sys); dfree(e.rz); dfree(e.sumr2); dfree(e.pap); dfree(e.alpha); dfree(e.beta);
-    dfree(e.iter); dfree(e.maxiter); dfree(e.tol); dfree(e.live); dfree(e.done); dfree(e.kind);
-    e.h_kind.clear();
-    dfree(e.partial); dfree(e.stage); dfree(e.stageB); dfree(e.d_list); dfree(e.ypart);
-    e.h_maxiter.clear();
-    e.h_tol.clear();
+    dfree(e.rz); dfree(e.sumr2); dfree(e.pap); dfree(e.alpha); dfree(e.beta);
+    dfree(e.iter); dfree(e.done); dfree(e.slot_state);
+    dfree(e.partial); dfree(e.stage); dfree(e.stageB); dfree(e.ypart);
     if (e.h_done) (void)hipHostFree(e.h_done);
     if (e.h_iter) (void)hipHostFree(e.h_iter);
+    if (e.h_state) (void)hipHostFree(e.h_state);
     e.h_done = nullptr;
     e.h_iter = nullptr;
-    e.h_slot_sys.clear();
-    e.h_live.clear();
+    e.h_state = nullptr;
+    e.d_list = e.h_list = e.slot_sys = e.h_slot_sys = e.maxiter = e.h_maxiter = e.d_take = e.h_take = nullptr;
+    e.tol = e.h_tol = nullptr;
+    e.live = e.h_live = e.kind = e.h_kind = nullptr;
     e.width = e.nsys = e.kpad = 0;
 }
 
@@ -108,7 +139,7 @@ void free_geno(saige_ctx* c) {
     dfree(c->d_stats); dfree(c->d_packed); dfree(c->d_packed_vr); dfree(c->d_lut); dfree(c->d_lut_vr);
     dfree(c->d_diag); dfree(c->d_grm); dfree(c->d_vec); dfree(c->d_grm_loco); dfree(c->d_diag_loco);
     dfree(c->d_mv_in); dfree(c->d_mv_blk); dfree(c->d_mv_out); dfree(c->d_mv_ypart);
-    dfree(c->d_dots_partial); dfree(c->d_dots_out); dfree(c->d_take_lists);
+    dfree(c->d_dots_partial); dfree(c->d_dots_out);
     tri_plan_free(c->tri);
     gemv_plan_free(c->triv);
     free_engine(c->eng);
@@ -153,21 +184,17 @@ int engine_reserve(saige_ctx* c, Engine& e, int kpad, int nsys) {
     SAIGE_HIP_TRY(dalloc(&e.minv, Np * nsys));
     SAIGE_HIP_TRY(dalloc(&e.mask, Np * nsys));
     SAIGE_HIP_TRY(dalloc(&e.tau, 2 * nsys));
-    SAIGE_HIP_TRY(dalloc(&e.slot_sys, kpad));
+    SAIGE_HIP_TRY(dalloc(&e.slot_state, slot_state_bytes(kpad)));
+    SAIGE_HIP_TRY(hipHostMalloc((void**)&e.h_state, slot_state_bytes(kpad), hipHostMallocDefault));
     SAIGE_HIP_TRY(dalloc(&e.rz, kpad));
     SAIGE_HIP_TRY(dalloc(&e.sumr2, kpad));
     SAIGE_HIP_TRY(dalloc(&e.pap, kpad));
     SAIGE_HIP_TRY(dalloc(&e.alpha, kpad));
     SAIGE_HIP_TRY(dalloc(&e.beta, kpad));
     SAIGE_HIP_TRY(dalloc(&e.iter, kpad));
-    SAIGE_HIP_TRY(dalloc(&e.maxiter, kpad));
-    SAIGE_HIP_TRY(dalloc(&e.tol, kpad));
-    SAIGE_HIP_TRY(dalloc(&e.live, kpad));
     SAIGE_HIP_TRY(dalloc(&e.done, kpad));
-    SAIGE_HIP_TRY(dalloc(&e.kind, kpad));
     SAIGE_HIP_TRY(dalloc(&e.stage, Np));
     SAIGE_HIP_TRY(dalloc(&e.stageB, Np * kpad));
-    SAIGE_HIP_TRY(dalloc(&e.d_list, 2 * kpad));
     if (kpad == EW) SAIGE_HIP_TRY(dalloc(&e.ypart, gemm_ypart_floats(Np)));
     int64_t np = std::max<int64_t>(gemv_max_parts(Np), gemm_max_parts(Np) * kpad);
     np = std::max<int64_t>(np, vec_max_parts(Np, kpad) * kpad);
@@ -178,11 +205,8 @@ int engine_reserve(saige_ctx* c, Engine& e, int kpad, int nsys) {
     e.kpad = kpad;
     e.width = kpad;
     e.nsys = nsys;
-    e.h_slot_sys.assign(kpad, 0);
-    e.h_live.assign(kpad, 0);
-    e.h_maxiter.assign(kpad, 0);
-    e.h_kind.assign(kpad, 0);
-    e.h_tol.assign(kpad, 0.0);
+    set_slot_state_views(e);
+    std::memset(e.h_state, 0, (size_t)slot_state_bytes(kpad));
     // zero all column blocks so padded rows and idle columns read as zero
     hipStream_t s = c->stream;
     SAIGE_HIP_TRY(launch_fill_zero(s, e.X, Np * kpad * 4));
@@ -194,14 +218,9 @@ int engine_reserve(saige_ctx* c, Engine& e, int kpad, int nsys) {
     SAIGE_HIP_TRY(launch_fill_zero(s, e.minv, Np * nsys * 4));
     SAIGE_HIP_TRY(launch_fill_zero(s, e.mask, round_up(Np * nsys, 4)));
     SAIGE_HIP_TRY(launch_fill_zero(s, e.stage, Np * 4));
-    std::vector<uint8_t> zeros(kpad, 0);
-    std::vector<int32_t> zi(kpad, 0);
-    SAIGE_HIP_TRY(hipMemcpyAsync(e.live, zeros.data(), kpad, hipMemcpyHostToDevice, s));
+    const std::vector<uint8_t> zeros(kpad, 0);
     SAIGE_HIP_TRY(hipMemcpyAsync(e.done, zeros.data(), kpad, hipMemcpyHostToDevice, s));
-    SAIGE_HIP_TRY(hipMemcpyAsync(e.kind, zeros.data(), kpad, hipMemcpyHostToDevice, s));
-    SAIGE_HIP_TRY(hipMemcpyAsync(e.slot_sys, zi.data(), 4 * kpad, hipMemcpyHostToDevice, s));
-    SAIGE_HIP_TRY(hipMemcpyAsync(e.maxiter, zi.data(), 4 * kpad, hipMemcpyHostToDevice, s));
-    SAIGE_HIP_TRY(hipMemcpyAsync(e.tol, e.h_tol.data(), 8 * kpad, hipMemcpyHostToDevice, s));
+    SAIGE_HIP_TRY(push_slot_state(c, e));
     SAIGE_HIP_TRY(hipStreamSynchronize(s));
     return SAIGE_OK;
 }
@@ -260,14 +279,11 @@ int engine_set_system(saige_ctx* c, Engine& e, int s, const float* w_host, const
     return SAIGE_OK;
 }
 
-int engine_sync_host_arrays(saige_ctx* c, Engine& e, const std::vector<int32_t>* maxiter,
-                            const std::vector<double>* tol) {
-    hipStream_t s = c->stream;
-    SAIGE_HIP_TRY(hipMemcpyAsync(e.live, e.h_live.data(), e.kpad, hipMemcpyHostToDevice, s));
-    SAIGE_HIP_TRY(hipMemcpyAsync(e.slot_sys, e.h_slot_sys.data(), 4 * e.kpad, hipMemcpyHostToDevice, s));
-    if (maxiter) SAIGE_HIP_TRY(hipMemcpyAsync(e.maxiter, maxiter->data(), 4 * e.kpad, hipMemcpyHostToDevice, s));
-    if (tol) SAIGE_HIP_TRY(hipMemcpyAsync(e.tol, tol->data(), 8 * e.kpad, hipMemcpyHostToDevice, s));
-    SAIGE_HIP_TRY(hipStreamSynchronize(s));
+// uploads the per-slot state after the host changed it (live slots and
+// their systems)
+int engine_sync_host_arrays(saige_ctx* c, Engine& e) {
+    SAIGE_HIP_TRY(push_slot_state(c, e));
+    SAIGE_HIP_TRY(hipStreamSynchronize(c->stream));
     return SAIGE_OK;
 }
 
@@ -309,15 +325,10 @@ int engine_add_batch(saige_ctx* c, Engine& e, int k, const int32_t* slots, const
     }
     hipStream_t st = c->stream;
     const int64_t Np = c->Npad, N = c->Nnm;
-    std::vector<int32_t> lists(2 * e.kpad, 0);
-    std::copy(slots, slots + k, lists.begin());
-    std::copy(systems, systems + k, lists.begin() + e.kpad);
-    SAIGE_HIP_TRY(hipMemcpyAsync(e.d_list, lists.data(), 8 * e.kpad, hipMemcpyHostToDevice, st));
-    SAIGE_HIP_TRY(hipMemcpyAsync(e.live, e.h_live.data(), e.kpad, hipMemcpyHostToDevice, st));
-    SAIGE_HIP_TRY(hipMemcpyAsync(e.slot_sys, e.h_slot_sys.data(), 4 * e.kpad, hipMemcpyHostToDevice, st));
-    SAIGE_HIP_TRY(hipMemcpyAsync(e.maxiter, e.h_maxiter.data(), 4 * e.kpad, hipMemcpyHostToDevice, st));
-    SAIGE_HIP_TRY(hipMemcpyAsync(e.tol, e.h_tol.data(), 8 * e.kpad, hipMemcpyHostToDevice, st));
-    SAIGE_HIP_TRY(hipMemcpyAsync(e.kind, e.h_kind.data(), e.kpad, hipMemcpyHostToDevice, st));
+    std::fill(e.h_list, e.h_list + 2 * e.kpad, 0);
+    std::copy(slots, slots + k, e.h_list);
+    std::copy(systems, systems + k, e.h_list + e.kpad);
+    SAIGE_HIP_TRY(push_slot_state(c, e));                 // lists, live, systems, maxiter, tol, kind
     // right-hand sides: column-major N x k, uploaded, or read in place on the device
     const float* B = B_host;
     if (!b_on_device) {
@@ -462,15 +473,16 @@ int engine_take_batch(saige_ctx* c, Engine& e, int k, const int32_t* slots, floa
             return SAIGE_E_ARG;
         }
     hipStream_t st = c->stream;
+    // the slots' list and their freed state in one upload (the gather does
+    // not read the live flags)
+    std::fill(e.h_list, e.h_list + 2 * e.kpad, 0);
+    std::copy(slots, slots + k, e.h_list);
+    for (int j = 0; j < k; j++) e.h_live[slots[j]] = 0;
+    SAIGE_HIP_TRY(push_slot_state(c, e));
     if (X_host) {
-        std::vector<int32_t> lists(2 * e.kpad, 0);
-        std::copy(slots, slots + k, lists.begin());
-        SAIGE_HIP_TRY(hipMemcpyAsync(e.d_list, lists.data(), 8 * e.kpad, hipMemcpyHostToDevice, st));
         SAIGE_HIP_TRY(launch_cols_gather(st, e.X, k, e.d_list, e.kpad, c->Nnm, e.stageB));
         SAIGE_HIP_TRY(hipMemcpyAsync(X_host, e.stageB, sizeof(float) * c->Nnm * k, hipMemcpyDeviceToHost, st));
     }
-    for (int j = 0; j < k; j++) e.h_live[slots[j]] = 0;
-    SAIGE_HIP_TRY(hipMemcpyAsync(e.live, e.h_live.data(), e.kpad, hipMemcpyHostToDevice, st));
     SAIGE_HIP_TRY(hipStreamSynchronize(st));
     return SAIGE_OK;
 }
@@ -478,10 +490,9 @@ int engine_take_batch(saige_ctx* c, Engine& e, int k, const int32_t* slots, floa
 int ensure_dots_scratch(saige_ctx* c) {
     if (c->d_dots_partial) return SAIGE_OK;
     if (dalloc(&c->d_dots_partial, col_dots_partial_doubles(c->Npad, EW)) ||
-        dalloc(&c->d_dots_out, (int64_t)EW * col_dots_max()) || dalloc(&c->d_take_lists, 3 * EW)) {
+        dalloc(&c->d_dots_out, (int64_t)EW * col_dots_max())) {
         dfree(c->d_dots_partial);
         dfree(c->d_dots_out);
-        dfree(c->d_take_lists);
         set_error("device allocation failed (dot scratch)");
         return SAIGE_E_NOMEM;
     }
@@ -542,14 +553,17 @@ int engine_take_batch_ex(saige_ctx* c, Engine& e, int k, const int32_t* slots, f
     }
     if (int rc = ensure_dots_scratch(c)) return rc;
     hipStream_t st = c->stream;
-    std::vector<int32_t> lists(3 * EW, 0);
+    // the take's lists and the slots' freed state in one upload (the dots
+    // and gathers do not read the live flags)
+    std::fill(e.h_take, e.h_take + 3 * EW, 0);
     for (int j = 0; j < k; j++) {
-        lists[j] = slots[j];
-        lists[EW + j] = xcols ? xcols[j] : j;
-        lists[2 * EW + j] = pcols ? pcols[j] : j;
+        e.h_take[j] = slots[j];
+        e.h_take[EW + j] = xcols ? xcols[j] : j;
+        e.h_take[2 * EW + j] = pcols ? pcols[j] : j;
+        e.h_live[slots[j]] = 0;
     }
-    int32_t* d_slots = c->d_take_lists;
-    SAIGE_HIP_TRY(hipMemcpyAsync(d_slots, lists.data(), sizeof(int32_t) * 3 * EW, hipMemcpyHostToDevice, st));
+    int32_t* d_slots = e.d_take;
+    SAIGE_HIP_TRY(push_slot_state(c, e));
     if (int rc = column_dots(c, e.X, 0, e.kpad, d_slots, k, P1, P2, ldp, d_slots + 2 * EW, Q, m, ldq, dots_host))
         return rc;
     if (X_dev) SAIGE_HIP_TRY(launch_cols_gather_to(st, e.X, k, d_slots, d_slots + EW, e.kpad, c->Nnm, ldx, X_dev));
@@ -557,8 +571,6 @@ int engine_take_batch_ex(saige_ctx* c, Engine& e, int k, const int32_t* slots, f
         SAIGE_HIP_TRY(launch_cols_gather(st, e.X, k, d_slots, e.kpad, c->Nnm, e.stageB));
         SAIGE_HIP_TRY(hipMemcpyAsync(X_host, e.stageB, sizeof(float) * c->Nnm * k, hipMemcpyDeviceToHost, st));
     }
-    for (int j = 0; j < k; j++) e.h_live[slots[j]] = 0;
-    SAIGE_HIP_TRY(hipMemcpyAsync(e.live, e.h_live.data(), e.kpad, hipMemcpyHostToDevice, st));
     SAIGE_HIP_TRY(hipStreamSynchronize(st));
     return SAIGE_OK;
 }
@@ -1302,7 +1314,7 @@ static int get_crossprod(saige_ctx* c, int loco, const float* b, const float* w,
     const int64_t Np = c->Npad, N = c->Nnm;
     e.h_slot_sys[0] = 0;
     e.h_live[0] = 1;
-    rc = engine_sync_host_arrays(c, e, nullptr, nullptr);
+    rc = engine_sync_host_arrays(c, e);
     if (rc) return rc;
     SAIGE_HIP_TRY(hipMemcpyAsync(e.stage, b, 4 * N, hipMemcpyHostToDevice, st));
     SigmaArgs a;
@@ -1324,7 +1336,7 @@ static int get_crossprod(saige_ctx* c, int loco, const float* b, const float* w,
     SAIGE_HIP_TRY(hipMemcpyAsync(out, e.AP, 4 * N, hipMemcpyDeviceToHost, st));
     SAIGE_HIP_TRY(hipStreamSynchronize(st));
     e.h_live[0] = 0;
-    return engine_sync_host_arrays(c, e, nullptr, nullptr);
+    return engine_sync_host_arrays(c, e);
 }
 
 int saige_get_crossprod(saige_ctx* c, const float* b, const float* w, const float* tau, float* out) {

@@ -106,7 +106,13 @@ struct Engine {
     // per system [nsys][Npad]
     float* winv = nullptr; float* minv = nullptr; uint8_t* mask = nullptr;
     float* tau = nullptr;         // [nsys][2]
-    // per slot state
+    // per slot state.  What the host sets per slot lives in one device block
+    // (slot_state, slot_state_bytes(kpad)) mirrored in page-locked host memory
+    // (h_state), so a call that changes it uploads it with one copy:
+    // d_list, slot_sys, maxiter, tol, live, kind and d_take point into the
+    // device block, h_list .. h_take at the same offsets into the mirror
+    uint8_t* slot_state = nullptr;
+    uint8_t* h_state = nullptr;
     int32_t* slot_sys = nullptr;  // device [kpad]
     double* rz = nullptr;         // z^T r
     double* sumr2 = nullptr;      // r^T r
@@ -119,20 +125,23 @@ struct Engine {
     uint8_t* live = nullptr;      // slot is solving
     uint8_t* done = nullptr;      // slot stopped (device)
     uint8_t* kind = nullptr;      // 0 = PCG solve, 1 = GRM product (one GEMM pass)
-    std::vector<uint8_t> h_kind;
+    uint8_t* h_kind = nullptr;
     double* partial = nullptr;    // [nparts][kpad][2] partial dot products
     int64_t nparts = 0;
     float* stage = nullptr;       // [Npad] staging for one column
     float* stageB = nullptr;      // [kpad][Npad] staging for a block of columns
     float* ypart = nullptr;       // split-K partial products of the block GEMM
     int32_t* d_list = nullptr;    // [2][kpad] slot / system lists of a batch call
-    std::vector<int32_t> h_maxiter;
-    std::vector<double> h_tol;
+    int32_t* d_take = nullptr;    // [3][kpad] slots, destination and partner columns of a take
+    int32_t* h_list = nullptr;
+    int32_t* h_take = nullptr;
+    int32_t* h_maxiter = nullptr;
+    double* h_tol = nullptr;
     // pinned host mirrors
     uint8_t* h_done = nullptr;
     int32_t* h_iter = nullptr;
-    std::vector<int> h_slot_sys;
-    std::vector<uint8_t> h_live;
+    int32_t* h_slot_sys = nullptr;
+    uint8_t* h_live = nullptr;
     // device time of the GRM product launches by kind: [0] GEMV (width 1),
     // [1] GEMM over one 32-column half, [2] GEMM over both halves
     double gemm_seconds[4] = {0.0, 0.0, 0.0, 0.0};   // by SAIGE_LAUNCH_* kind
@@ -192,7 +201,6 @@ struct saige_ctx {
     std::mutex park_mu;
     std::vector<std::pair<int64_t, void*>> parked;
     double* d_dots_out = nullptr;
-    int32_t* d_take_lists = nullptr;             // [3][EW]: slots, destination columns, partner columns
     int grm_precision = -1;
     double grm_seconds = 0.0;
     // scratch vectors
