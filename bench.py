@@ -523,7 +523,9 @@ def main():
     log(f"autotune block GEMM ({args.gemm_mode}): " + ", ".join(f"rt{a}/pd{b}/s{c} {t:.2f}ms" for a, b, c, t in tune["table"]) +
         f" -> rt{tune['rt']}/pd{tune['pd']}/s{tune['nsplit']}; one half: " +
         ", ".join(f"rt{a}/pd{b} {t:.2f}ms" for a, b, _, t in tune["half_table"]) +
-        f" -> rt{tune['half']['rt']}/pd{tune['half']['pd']}")
+        f" -> rt{tune['half']['rt']}/pd{tune['half']['pd']}" +
+        ("; 16 columns: " + ", ".join(f"rt{a}/pd{b} {t:.2f}ms" for a, b, _, t in tune["quarter_table"]) +
+         f" -> rt{tune['quarter']['rt']}/pd{tune['quarter']['pd']}" if tune["quarter_table"] else ""))
 
     opt = D.Options(nrun=args.nrun)
     jobs = [{"trait": "binary", "y": y, "X": X, "tag": f"prev{p}"} for y, p in zip(phenos, prevalences)]

@@ -318,11 +318,12 @@ int saige_engine_use_loco(saige_ctx* ctx, int on);
  * for launches over one 32-column half of the block, it tunes rt and pd
  * again at the chosen nsplit (nsplit fixes the summation order, so a
  * column's bits must not depend on whether the other half is live).
- * ms_table (may be NULL) receives 20 per-launch times in ms: the 16 full
+ * ms_table (may be NULL) receives 24 per-launch times in ms: the 16 full
  * shapes in the order of saige_autotune_candidates, then the one-half
- * shapes (rt, pd) = (1,1), (2,1), (1,3), (2,3).  Call it once before
- * solving; results are reproducible for a fixed shape.  SAIGE_E_STATE if the
- * engine has live slots. */
+ * shapes (rt, pd) = (1,1), (2,1), (1,3), (2,3), then the 16-column shapes of
+ * the lower-triangle mode in the same order (-1 in other modes).  Call it
+ * once before solving; results are reproducible for a fixed shape.
+ * SAIGE_E_STATE if the engine has live slots. */
 int saige_autotune(saige_ctx* ctx, int reps, double* ms_table);
 /* The candidate shapes (rt[i], pd[i], nsplit[i]); returns their count (16). */
 int saige_autotune_candidates(int* rt, int* pd, int* nsplit);
@@ -333,6 +334,8 @@ int saige_set_gemm_config(saige_ctx* ctx, int rt, int pd, int nsplit);
 int saige_get_gemm_config(const saige_ctx* ctx, int* rt, int* pd, int* nsplit, double* tuned_ms);
 /* Shape of one-half launches (nsplit as the full shape). */
 int saige_get_gemm_config_half(const saige_ctx* ctx, int* rt, int* pd, double* tuned_ms);
+/* Shape of 16-column launches (lower-triangle mode; rt as the full shape). */
+int saige_get_gemm_config_quarter(const saige_ctx* ctx, int* rt, int* pd, double* tuned_ms);
 /* Arithmetic of the block GEMM (no reference counterpart; the reference's
  * product is an FP32 SGEMV).  SAIGE_GEMM_FP32: FP32 MFMA 32x32x2.
  * SAIGE_GEMM_BF16X3: every FP32 operand split exactly into three BF16
@@ -343,11 +346,14 @@ int saige_get_gemm_config_half(const saige_ctx* ctx, int* rt, int* pd, double* t
  * shape parameters mean the same in both, except that in SAIGE_GEMM_BF16X3
  * rt = 1 runs workgroups of twelve one-tile waves (384 rows share each
  * staged chunk of P) whose registers hold at most two chunks ahead, so pd = 3
- * means two there.  SAIGE_GEMM_BF16X3_TRI: the same six piece products,
+ * means two there.  SAIGE_GEMM_BF16X3_TRI: the same six piece products (five
+ * on the split GRM, whose values need two pieces), on the 16x16x32 MFMA,
  * read from the lower triangle of the GRM only (half the bytes): each 32x32
- * block left of the diagonal gives both S_IJ P_J and S_IJ^T P_I.  rt selects
- * the workgroup (1: twelve waves, 2: eight), pd 1 or 3 the prefetch depth
- * (one or two chunks ahead); nsplit is unused.  It needs device scratch for
+ * block left of the diagonal gives both S_IJ P_J and S_IJ^T P_I.  A launch
+ * covers 64, 32 or 16 columns, as few as hold live slots, with the same
+ * per-column arithmetic (SAIGE_LAUNCH_GEMM_QUARTER counts the 16-column
+ * ones).  rt selects the workgroup (1: twelve waves, 2: eight), pd 1 or 3
+ * the prefetch depth (one or two chunks ahead); nsplit is unused.  It needs device scratch for
  * its partial products, about 128 N^2 / (32 * waves) bytes (3.3 GB at
  * N = 100k with twelve waves), allocated on first use.  Changing the mode
  * clears the tuned times (re-run saige_autotune).  Engine idle. */

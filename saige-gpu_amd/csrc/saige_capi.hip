@@ -1946,7 +1946,9 @@ int saige_autotune(saige_ctx* c, int reps, double* ms_table) {
     if (int rc = ensure_tri(c)) return rc;
     // one-half launches: rt and pd only, the slice count stays the full shape's
     GemmConfig best_h = best, best_q = best;
-    double best_h_ms = -1.0;
+    double best_h_ms = -1.0, best_q_ms = -1.0;
+    if (ms_table)
+        for (int i = 0; i < kNumHalfCandidates; i++) ms_table[kNumGemmCandidates + kNumHalfCandidates + i] = -1.0;
     if (tri) {
         // rt fixes the summation order, so the 64-, 32- and 16-column
         // launches share it: per rt the fastest pd of each, and the rt of the
@@ -1977,6 +1979,7 @@ int saige_autotune(saige_ctx* c, int reps, double* ms_table) {
                     mh = per;
                 }
                 SAIGE_HIP_TRY(time_shape(g, 1, &per, 0));
+                if (ms_table) ms_table[kNumGemmCandidates + kNumHalfCandidates + i] = per;
                 if (mq < 0.0 || per < mq) {
                     bq = g;
                     mq = per;
@@ -1989,6 +1992,7 @@ int saige_autotune(saige_ctx* c, int reps, double* ms_table) {
                 best_h = bh;
                 best_h_ms = mh;
                 best_q = bq;
+                best_q_ms = mq;
             }
         }
         c->gemm_cfg = best;
@@ -2010,6 +2014,7 @@ int saige_autotune(saige_ctx* c, int reps, double* ms_table) {
     c->gemm_cfg_half = best_h;
     c->gemm_tuned_ms_half = best_h_ms;
     c->gemm_cfg_quarter = tri ? best_q : best_h;
+    c->gemm_tuned_ms_quarter = tri ? best_q_ms : best_h_ms;
     return SAIGE_OK;
 }
 
@@ -2036,6 +2041,7 @@ int saige_set_gemm_config(saige_ctx* c, int rt, int pd, int nsplit) {
     c->gemm_cfg_half = g;
     c->gemm_cfg_quarter = g;
     c->gemm_tuned_ms_half = 0.0;
+    c->gemm_tuned_ms_quarter = 0.0;
     return SAIGE_OK;
 }
 
@@ -2054,6 +2060,7 @@ int saige_set_gemm_mode(saige_ctx* c, int mode) {
         c->gemm_cfg_quarter.mode = mode;
         c->gemm_tuned_ms = 0.0;
         c->gemm_tuned_ms_half = 0.0;
+        c->gemm_tuned_ms_quarter = 0.0;
     }
     return SAIGE_OK;
 }
@@ -2095,6 +2102,17 @@ int saige_get_gemm_config_half(const saige_ctx* c, int* rt, int* pd, double* tun
     if (rt) *rt = c->gemm_cfg_half.rt;
     if (pd) *pd = c->gemm_cfg_half.pd;
     if (tuned_ms) *tuned_ms = c->gemm_tuned_ms_half;
+    return SAIGE_OK;
+}
+
+int saige_get_gemm_config_quarter(const saige_ctx* c, int* rt, int* pd, double* tuned_ms) {
+    if (!c) {
+        set_error("null context");
+        return SAIGE_E_ARG;
+    }
+    if (rt) *rt = c->gemm_cfg_quarter.rt;
+    if (pd) *pd = c->gemm_cfg_quarter.pd;
+    if (tuned_ms) *tuned_ms = c->gemm_tuned_ms_quarter;
     return SAIGE_OK;
 }
 

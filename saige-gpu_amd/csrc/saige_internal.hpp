@@ -194,6 +194,7 @@ struct saige_ctx {
     int gemv_mode = SAIGE_GEMV_TRI;              // single-column products: lower triangle or all of S
     saige::GemvPlan triv;                        // the lower-triangle GEMV's plan, built on first use
     double gemm_tuned_ms_half = 0.0;
+    double gemm_tuned_ms_quarter = 0.0;
     // FP64 column-dot scratch (saige_engine_take_batch_ex, saige_dots_device)
     double* d_dots_partial = nullptr;
     double* d_noalloc = nullptr;            // this device's address of the allocation-free scratch

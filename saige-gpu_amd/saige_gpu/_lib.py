@@ -138,6 +138,7 @@ SIGNATURES = {
     "saige_get_gemm_config": (_int, [_c_p, ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.POINTER(_int),
                                       ctypes.POINTER(_f64)]),
     "saige_get_gemm_config_half": (_int, [_c_p, ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.POINTER(_f64)]),
+    "saige_get_gemm_config_quarter": (_int, [_c_p, ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.POINTER(_f64)]),
     "saige_set_gemm_mode": (_int, [_c_p, _int]),
     "saige_get_gemm_mode": (_int, [_c_p, ctypes.POINTER(_int)]),
     "saige_set_gemv_mode": (_int, [_c_p, _int]),

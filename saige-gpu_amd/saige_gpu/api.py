@@ -289,13 +289,14 @@ class Session:
         pd = np.zeros(n, np.int32)
         ns = np.zeros(n, np.int32)
         self._L.saige_autotune_candidates(ptr(rt), ptr(pd), ptr(ns))
-        ms = np.zeros(n + 4, np.float64)
+        ms = np.zeros(n + 8, np.float64)
         check(self._L.saige_autotune(self.ctx, int(reps), ptr(ms)))
         cfg = self.gemm_config()
         # shapes a mode does not use read -1 and are left out
+        shapes = [(1, 1), (2, 1), (1, 3), (2, 3)]
         cfg["table"] = [(int(a), int(b), int(c), float(t)) for a, b, c, t in zip(rt, pd, ns, ms[:n]) if t >= 0]
-        cfg["half_table"] = [(a, b, cfg["nsplit"], float(t))
-                             for (a, b), t in zip([(1, 1), (2, 1), (1, 3), (2, 3)], ms[n:]) if t >= 0]
+        cfg["half_table"] = [(a, b, cfg["nsplit"], float(t)) for (a, b), t in zip(shapes, ms[n:n + 4]) if t >= 0]
+        cfg["quarter_table"] = [(a, b, cfg["nsplit"], float(t)) for (a, b), t in zip(shapes, ms[n + 4:]) if t >= 0]
         return cfg
 
     def set_gemm_config(self, rt, pd, nsplit):
@@ -335,9 +336,12 @@ class Session:
                                             ctypes.byref(ms)))
         hr, hp, hm = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_double(0.0)
         check(self._L.saige_get_gemm_config_half(self.ctx, ctypes.byref(hr), ctypes.byref(hp), ctypes.byref(hm)))
+        qr, qp, qm = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_double(0.0)
+        check(self._L.saige_get_gemm_config_quarter(self.ctx, ctypes.byref(qr), ctypes.byref(qp), ctypes.byref(qm)))
         return {"rt": int(rt.value), "pd": int(pd.value), "nsplit": int(ns.value), "ms": float(ms.value),
                 "mode": self.gemm_mode(),
-                "half": {"rt": int(hr.value), "pd": int(hp.value), "ms": float(hm.value)}}
+                "half": {"rt": int(hr.value), "pd": int(hp.value), "ms": float(hm.value)},
+                "quarter": {"rt": int(qr.value), "pd": int(qp.value), "ms": float(qm.value)}}
 
     def engine_use_loco(self, on):
         """Point the block engine at the LOCO GRM (on) or the full one."""

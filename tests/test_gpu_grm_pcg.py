@@ -453,6 +453,16 @@ def test_gemm_tri_mode_many_items(session):
             assert np.array_equal(half.view(np.uint32), KB[:, :20].view(np.uint32)), rt
             quarter = session.grm_matvec(B[:, :12])
             assert np.array_equal(quarter.view(np.uint32), KB[:, :12].view(np.uint32)), rt
+        # the autotuner in this mode: one wave count for all three widths,
+        # the fastest pd of each at that count
+        res = session.autotune(reps=1)
+        assert len(res["table"]) == 4 and len(res["half_table"]) == 4 and len(res["quarter_table"]) == 4
+        assert res["half"]["rt"] == res["rt"] == res["quarter"]["rt"]
+        for key, tab in (("half", res["half_table"]), ("quarter", res["quarter_table"])):
+            assert res[key]["ms"] == min(t[3] for t in tab if t[0] == res["rt"]), key
+        KBt = session.grm_matvec(B)
+        assert np.array_equal(session.grm_matvec(B[:, :7]).view(np.uint32), KBt[:, :7].view(np.uint32))
+        assert np.array_equal(session.grm_matvec(B[:, :30]).view(np.uint32), KBt[:, :30].view(np.uint32))
     finally:
         session.set_gemm_mode("fp32")
         session.set_gemm_config(1, 1, 4)
@@ -607,7 +617,8 @@ def test_autotune_and_every_gemm_shape(session, grm1000):
     for rt, pd, ns, _ in table:
         session.set_gemm_config(rt, pd, ns)
         assert session.gemm_config() == {"rt": rt, "pd": pd, "nsplit": ns, "ms": 0.0, "mode": "fp32",
-                                         "half": {"rt": rt, "pd": pd, "ms": 0.0}}
+                                         "half": {"rt": rt, "pd": pd, "ms": 0.0},
+                                         "quarter": {"rt": rt, "pd": pd, "ms": 0.0}}
         KB = session.grm_matvec(B)
         assert np.max(np.abs(KB - RB)) <= 2e-5 * np.max(np.abs(RB)), (rt, pd, ns)
         X, _ = session.pcg_block(w, tau, B[:, :3], 500, 1e-5)
