@@ -165,12 +165,51 @@ __global__ __launch_bounds__(256) void k_decode_stats(const uint8_t* __restrict_
 
 // Repack listed markers into subSampleInGeno order with imputation
 // (FG.cpp:551-576).  One workgroup per marker; each thread builds whole
-// output bytes.  subidx == nullptr means the identity order with Nnm == N.
+// output bytes.  subidx == nullptr means the identity order with Nnm == N:
+// then every output byte is a function of its input byte and the marker's
+// fill genotype alone, taken from a 256-entry table per fill (0, 1, 2) built
+// once per workgroup in LDS, four consecutive bytes per thread; the last
+// byte keeps only the bits of samples below Nnm.
 __global__ __launch_bounds__(256) void k_repack(const uint8_t* __restrict__ bed, int64_t bpm,
                                                 const int32_t* __restrict__ markers, int64_t nmark,
                                                 const MarkerStat* __restrict__ stats,
                                                 const int32_t* __restrict__ subidx, int64_t Nnm,
                                                 int64_t bpm_nm, uint8_t* __restrict__ out) {
+    if (subidx == nullptr) {
+        __shared__ uint8_t tab[3][256];
+        for (int e = threadIdx.x; e < 3 * 256; e += blockDim.x) {
+            const int fill = e / 256, in = e % 256;
+            unsigned byte = 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                int g = code_geno((unsigned)(in >> (2 * j)) & 3u);
+                if (g == 3) g = fill;
+                byte |= internal_code(g) << (2 * j);
+            }
+            tab[fill][in] = (uint8_t)byte;
+        }
+        __syncthreads();
+        const int tail = (int)(Nnm % 4);                      // samples in a partial last byte
+        const unsigned tail_mask = tail ? (1u << (2 * tail)) - 1u : 0xFFu;
+        for (int64_t mi = blockIdx.x; mi < nmark; mi += gridDim.x) {
+            const int64_t m = markers[mi];
+            const uint8_t* row = bed + m * bpm;
+            const uint8_t* t = tab[min(max(stats[m].fill, 0), 2)];   // fill = round(2p) is 0, 1 or 2
+            uint8_t* orow = out + mi * bpm_nm;
+            for (int64_t o = 4 * (int64_t)threadIdx.x; o < bpm_nm; o += 4 * (int64_t)blockDim.x) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int64_t oj = o + j;
+                    if (oj < bpm_nm) {
+                        unsigned v = t[row[oj]];
+                        if (oj == bpm_nm - 1) v &= tail_mask;
+                        orow[oj] = (uint8_t)v;
+                    }
+                }
+            }
+        }
+        return;
+    }
     for (int64_t mi = blockIdx.x; mi < nmark; mi += gridDim.x) {
         const int64_t m = markers[mi];
         const uint8_t* row = bed + m * bpm;
@@ -178,25 +217,14 @@ __global__ __launch_bounds__(256) void k_repack(const uint8_t* __restrict__ bed,
         uint8_t* orow = out + mi * bpm_nm;
         for (int64_t o = threadIdx.x; o < bpm_nm; o += blockDim.x) {
             unsigned byte = 0;
-            if (subidx == nullptr) {
-                unsigned in = row[o];
 #pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    if (o * 4 + j >= Nnm) break;
-                    int g = code_geno((in >> (2 * j)) & 3u);
-                    if (g == 3) g = fill;
-                    byte |= internal_code(g) << (2 * j);
-                }
-            } else {
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const int64_t indx = o * 4 + j;
-                    if (indx >= Nnm) break;
-                    const int64_t src = (int64_t)subidx[indx] - 1;
-                    int g = code_geno((row[src >> 2] >> ((src & 3) * 2)) & 3u);
-                    if (g == 3) g = fill;
-                    byte |= internal_code(g) << (2 * j);
-                }
+            for (int j = 0; j < 4; j++) {
+                const int64_t indx = o * 4 + j;
+                if (indx >= Nnm) break;
+                const int64_t src = (int64_t)subidx[indx] - 1;
+                int g = code_geno((row[src >> 2] >> ((src & 3) * 2)) & 3u);
+                if (g == 3) g = fill;
+                byte |= internal_code(g) << (2 * j);
             }
             orow[o] = (uint8_t)byte;
         }
@@ -226,8 +254,10 @@ __device__ __forceinline__ float lut_pick(const float4 L, int g) {
 }
 
 // Get_Diagof_StdGeno (FG.cpp:665-704): one thread per sample, markers summed
-// strictly in order in FP32 (z*z rounded, then added), with 8 marker loads
-// in flight per iteration.
+// strictly in order in FP32 (z*z rounded, then added), with DU marker loads
+// in flight per iteration (32 ran no faster: a wave's load covers 16 bytes of
+// a marker row, and that request rate, not the latency, bounds it).
+constexpr int DU = 8;
 __global__ __launch_bounds__(256) void k_diag_std(const uint8_t* __restrict__ packed,
                                                   const float4* __restrict__ lut, int64_t Mqc,
                                                   int64_t Nnm, int64_t bpm_nm,
@@ -238,12 +268,12 @@ __global__ __launch_bounds__(256) void k_diag_std(const uint8_t* __restrict__ pa
     const int sh = (int)(i & 3) * 2;
     float d = 0.0f;
     int64_t j = 0;
-    for (; j + 8 <= Mqc; j += 8) {
-        unsigned b[8];
+    for (; j + DU <= Mqc; j += DU) {
+        unsigned b[DU];
 #pragma unroll
-        for (int u = 0; u < 8; u++) b[u] = packed[(j + u) * bpm_nm + byte_i];
+        for (int u = 0; u < DU; u++) b[u] = packed[(j + u) * bpm_nm + byte_i];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
+        for (int u = 0; u < DU; u++) {
             float z = lut_pick(lut[j + u], internal_geno((b[u] >> sh) & 3u));
             d = __fadd_rn(d, __fmul_rn(z, z));
         }
