@@ -719,7 +719,11 @@ __global__ __launch_bounds__(768) void k_gemm_x3w(const float* __restrict__ S, i
 // row-set order: a fixed order, so results are reproducible, and a column's
 // arithmetic is the same whatever NQ, so a launch can be as narrow as its
 // live columns (a 16-column launch does a quarter of a full one's MFMA work)
-// and a column's bits still do not depend on its co-tenants.
+// and a column's bits still do not depend on its co-tenants.  A 16-column
+// launch takes any 16 block columns (a map of launch column -> block column),
+// so at most 16 live columns anywhere in the block run in one.  The
+// arithmetic of an output column depends only on its own column of P, in
+// whatever position of the MFMA operands it sits.
 // Eight waves have registers for a software-pipelined loop: chunk c's pieces
 // are split during chunk c-1, beside that chunk's MFMAs.  Twelve waves split
 // each chunk at its own turn, PD chunks of raw data ahead.
@@ -792,11 +796,21 @@ __device__ __forceinline__ void t16_transpose(const u32x4 (&a)[2], const u32x4 (
     }
 }
 
+// block column of launch column k in a 16-column launch: byte k of the map,
+// picked with selects (no indexing into the kernel argument)
+__device__ __forceinline__ int map_col(const ColMap16& m, int k) {
+    const uint32_t w = (k & 8) ? ((k & 4) ? m.w[3] : m.w[2]) : ((k & 4) ? m.w[1] : m.w[0]);
+    return (int)((w >> (8 * (k & 3))) & 0xFFu);
+}
+
+// q0: the launch's first 16-column tile (NQ 2 and 4); cm: the block columns
+// of a 16-column launch (NQ 1), which holds its P pieces, partials and LDS
+// tiles at columns [0, 16)
 template <int NWV, int PD, int NQ, int SP>
 __global__ __launch_bounds__(64 * NWV) void k_gemm_t16(const float* __restrict__ S, int64_t ld,
                                                        const u32x4* __restrict__ pb, const float* __restrict__ P,
                                                        const int4* __restrict__ items, float* __restrict__ y1p,
-                                                       float* __restrict__ y2p, int q0) {
+                                                       float* __restrict__ y2p, int q0, ColMap16 cm) {
     constexpr int GM = 32 * NWV;
     constexpr bool PIPE = NWV == 8;                  // software-pipelined chunk loop (register room)
     constexpr int RING = PD + 1;                     // raw chunks held
@@ -813,8 +827,10 @@ __global__ __launch_bounds__(64 * NWV) void k_gemm_t16(const float* __restrict__
     const int64_t rw = R * GM + wv * 32;             // the wave's first row
     const bool valid = rw < ld;                      // ld = Npad, a multiple of 32
     const float* srow = S + ((valid ? rw : ld - 32) + ln) * ld + 8 * g;
-    const int c0 = 16 * q0;                          // the launch's first column
-    // staging: element k of a chunk's NSTG pieces is column c0 + k % (16*NQ)
+    const int c0 = NQ == 1 ? 0 : 16 * q0;           // the launch's first column in LDS and partials
+    // the block column of launch column k (of 16 * NQ)
+    auto gcol = [&](int k) { return NQ == 1 ? map_col(cm, k) : c0 + k; };
+    // staging: element k of a chunk's NSTG pieces is launch column k % (16*NQ)
     // of (group, piece) k / (16*NQ); threads past the count repeat an
     // element of the first round
     const u32x4* bsrc[STG];
@@ -822,9 +838,9 @@ __global__ __launch_bounds__(64 * NWV) void k_gemm_t16(const float* __restrict__
 #pragma unroll
     for (int s = 0; s < STG; s++) {
         const int k = tid + s * NTH < NSTG ? tid + s * NTH : tid % NSTG;
-        const int gp = k / (16 * NQ), n = c0 + k % (16 * NQ);
-        bsrc[s] = pb + ((int64_t)(gp % 3) * G + gp / 3) * EW + n;
-        bdi[s] = gp * 64 + n;
+        const int gp = k / (16 * NQ), kk = k % (16 * NQ);
+        bsrc[s] = pb + ((int64_t)(gp % 3) * G + gp / 3) * EW + gcol(kk);
+        bdi[s] = gp * 64 + c0 + kk;
     }
     // E_jg as B operand: lane (ln, g) holds K = columns 8g + [0, 8) of column
     // ln, E_jg[k][n] = 1 when k == 16*jg + n
@@ -856,13 +872,13 @@ __global__ __launch_bounds__(64 * NWV) void k_gemm_t16(const float* __restrict__
     auto run = [&](auto u2tag) {
         constexpr bool U2 = decltype(u2tag)::value;
         // the A operand of use 2: P rows rw + 4g + tt (tt < 4) and
-        // rw + 16 + 4g + tt - 4 (tt >= 4), column c0 + 16t + ln, as three
+        // rw + 16 + 4g + tt - 4 (tt >= 4), launch column 16t + ln, as three
         // pieces (zero for rows past ld)
         u32x4 pah[NQ], pam[NQ], pal[NQ];
         if (U2) {
 #pragma unroll
             for (int t = 0; t < NQ; t++) {
-                const int col = c0 + 16 * t + ln;
+                const int col = gcol(16 * t + ln);
                 float v[8];
 #pragma unroll
                 for (int tt = 0; tt < 8; tt++)
@@ -1057,11 +1073,12 @@ __global__ __launch_bounds__(64 * NWV) void k_gemm_t16(const float* __restrict__
 // k_tri_reduce: Y[row][q] = the row set's use-1 partials (rtab[R] = {first
 // slot, count}) in slot order, then the chunk's use-2 partials in row-set
 // order; float4 per thread, QW threads per row (16: all 64 columns, 8: the
-// half at column q0).
+// half at column q0, 4: the 16 partial columns [0, 16) of a 16-column
+// launch, stored to the block columns of cm).
 template <int NWV, int QW>
 __global__ __launch_bounds__(256) void k_tri_reduce(const float* __restrict__ y1p, const float* __restrict__ y2p,
                                                     const int2* __restrict__ rtab, int64_t ld, int q0,
-                                                    float* __restrict__ y) {
+                                                    ColMap16 cm, float* __restrict__ y) {
     constexpr int GM = 32 * NWV;
     const int64_t row = (int64_t)blockIdx.x * (256 / QW) + threadIdx.x / QW;
     const int q = q0 + 4 * (threadIdx.x % QW);
@@ -1088,7 +1105,15 @@ __global__ __launch_bounds__(256) void k_tri_reduce(const float* __restrict__ y1
         s.z += v.z;
         s.w += v.w;
     }
-    *reinterpret_cast<float4*>(y + row * EW + q) = s;
+    if constexpr (QW == 4) {
+        float* yr = y + row * EW;
+        yr[map_col(cm, q)] = s.x;
+        yr[map_col(cm, q + 1)] = s.y;
+        yr[map_col(cm, q + 2)] = s.z;
+        yr[map_col(cm, q + 3)] = s.w;
+    } else {
+        *reinterpret_cast<float4*>(y + row * EW + q) = s;
+    }
 }
 
 constexpr int EROWS = 256;   // rows per workgroup of the epilogue
@@ -1724,33 +1749,35 @@ hipError_t tri_plan_build(TriPlan& t, int64_t Npad, int nwv, int ch) {
 namespace {
 template <int NWV, int PD, int NQ, int SP>
 void launch_t16(hipStream_t s, dim3 grid, const float* S, int64_t Npad, const u32x4* pb, const float* in,
-                const TriPlan& t, int q0, float* ypart) {
+                const TriPlan& t, int q0, ColMap16 cm, float* ypart) {
     hipLaunchKernelGGL((k_gemm_t16<NWV, PD, NQ, SP>), grid, dim3(64 * NWV), 0, s, S, Npad, pb, in, t.items, t.y1p,
-                       t.y2p, q0);
+                       t.y2p, q0, cm);
     hipLaunchKernelGGL((k_tri_reduce<NWV, 4 * NQ>), dim3((unsigned)(Npad * 4 * NQ / 256)), dim3(256), 0, s, t.y1p,
-                       t.y2p, t.rtab, Npad, 16 * q0, ypart);
+                       t.y2p, t.rtab, Npad, NQ == 1 ? 0 : 16 * q0, cm, ypart);
 }
 
 // the lower-triangle product over the launch's 64, 32 (halves 1 or 2) or 16
-// (quarter >= 0: columns 16*quarter + [0, 16)) columns
+// (quarter: the block columns it lists) columns
 template <int NWV, int SP>
 hipError_t launch_tri(hipStream_t s, const float* S, int64_t Npad, const float* in, float* ypart, int pd,
-                      int halves, int quarter, const TriPlan& t) {
+                      int halves, const QuarterCols* quarter, const TriPlan& t) {
     const int64_t G = Npad / 8;
     u32x4* pb = reinterpret_cast<u32x4*>(ypart + (int64_t)GSPLIT_MAX * Npad * EW);
     hipLaunchKernelGGL(k_split_p, dim3((unsigned)(G * EW / 256)), dim3(256), 0, s, in, G, pb);
     const dim3 grid((unsigned)t.nitems);
     const bool deep = pd == 3;
-    if (quarter >= 0) {
-        if (deep) launch_t16<NWV, 2, 1, SP>(s, grid, S, Npad, pb, in, t, quarter, ypart);
-        else launch_t16<NWV, 1, 1, SP>(s, grid, S, Npad, pb, in, t, quarter, ypart);
+    ColMap16 cm{};
+    if (quarter) {
+        for (int k = 0; k < 16; k++) cm.w[k / 4] |= (uint32_t)quarter->col[k] << (8 * (k % 4));
+        if (deep) launch_t16<NWV, 2, 1, SP>(s, grid, S, Npad, pb, in, t, 0, cm, ypart);
+        else launch_t16<NWV, 1, 1, SP>(s, grid, S, Npad, pb, in, t, 0, cm, ypart);
     } else if (halves == 3) {
-        if (deep) launch_t16<NWV, 2, 4, SP>(s, grid, S, Npad, pb, in, t, 0, ypart);
-        else launch_t16<NWV, 1, 4, SP>(s, grid, S, Npad, pb, in, t, 0, ypart);
+        if (deep) launch_t16<NWV, 2, 4, SP>(s, grid, S, Npad, pb, in, t, 0, cm, ypart);
+        else launch_t16<NWV, 1, 4, SP>(s, grid, S, Npad, pb, in, t, 0, cm, ypart);
     } else {
         const int q0 = halves == 2 ? 2 : 0;
-        if (deep) launch_t16<NWV, 2, 2, SP>(s, grid, S, Npad, pb, in, t, q0, ypart);
-        else launch_t16<NWV, 1, 2, SP>(s, grid, S, Npad, pb, in, t, q0, ypart);
+        if (deep) launch_t16<NWV, 2, 2, SP>(s, grid, S, Npad, pb, in, t, q0, cm, ypart);
+        else launch_t16<NWV, 1, 2, SP>(s, grid, S, Npad, pb, in, t, q0, cm, ypart);
     }
     return hipGetLastError();
 }
@@ -1758,7 +1785,7 @@ hipError_t launch_tri(hipStream_t s, const float* S, int64_t Npad, const float* 
 
 hipError_t launch_grm_gemm(hipStream_t s, const float* S, int64_t Npad, int64_t Nnm, const float* in,
                            float* out, SigmaArgs a, double* partial, int64_t* nparts, float* ypart, GemmConfig g,
-                           int halves, const TriPlan* tri, bool s16, int quarter) {
+                           int halves, const TriPlan* tri, bool s16, const QuarterCols* quarter) {
     // Npad is a multiple of 256 = 128 * RT for both RT; a slice holds at least
     // one superchunk when Npad / GKS >= nsplit
     if (!gemm_config_valid(g) || Npad % 256 != 0 || halves < 1 || halves > 3) return hipErrorInvalidValue;
@@ -1766,7 +1793,15 @@ hipError_t launch_grm_gemm(hipStream_t s, const float* S, int64_t Npad, int64_t 
         // the lower-triangle product sums its partials into slice 0
         if (!tri || tri->Npad != Npad || !tri->items) return hipErrorInvalidValue;
         hipError_t e;
-        if (quarter >= 4 || (quarter >= 0 && halves != 1 << (quarter / 2))) return hipErrorInvalidValue;
+        if (quarter) {
+            // sixteen distinct block columns
+            uint64_t seen = 0;
+            for (int k = 0; k < 16; k++) {
+                const int col = quarter->col[k];
+                if (col >= EW || (seen >> col & 1)) return hipErrorInvalidValue;
+                seen |= 1ull << col;
+            }
+        }
         if (s16)
             e = tri->nwv == 8 ? launch_tri<8, 2>(s, S, Npad, in, ypart, g.pd, halves, quarter, *tri)
                               : launch_tri<12, 2>(s, S, Npad, in, ypart, g.pd, halves, quarter, *tri);

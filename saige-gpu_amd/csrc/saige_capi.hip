@@ -406,17 +406,26 @@ int engine_step(saige_ctx* c, Engine& e, int n_iter, uint8_t* done_out, int32_t*
     SAIGE_HIP_TRY(hipStreamSynchronize(st));
     // 32-column halves of the block that hold an occupied slot; the GEMM skips
     // the others (a done slot that is not yet taken is recomputed unchanged)
-    int halves = 0, quarters = 0;
+    int halves = 0, nlive = 0;
     for (int k = 0; k < e.kpad; k++)
         if (e.h_live[k]) {
             halves |= 1 << (k / 32);
-            quarters |= 1 << (k / 16);
+            nlive++;
         }
     if (halves == 0) halves = 1;
-    // the lower-triangle product can also run one 16-column quarter alone
-    int quarter = -1;
-    if (e.kpad != 1 && c->gemm_cfg.mode == GEMM_BF16X3_TRI && quarters != 0 && (quarters & (quarters - 1)) == 0)
-        quarter = __builtin_ctz((unsigned)quarters);
+    // the lower-triangle product runs at most 16 occupied slots, wherever
+    // they sit, as one 16-column launch: the occupied slots in order, then
+    // free ones (computed, not used)
+    QuarterCols qc;
+    bool quarter = false;
+    if (e.kpad == EW && c->gemm_cfg.mode == GEMM_BF16X3_TRI && nlive > 0 && nlive <= 16) {
+        int n = 0;
+        for (int k = 0; k < e.kpad; k++)
+            if (e.h_live[k]) qc.col[n++] = (uint8_t)k;
+        for (int k = 0; k < e.kpad && n < 16; k++)
+            if (!e.h_live[k]) qc.col[n++] = (uint8_t)k;
+        quarter = true;
+    }
     for (int it = 0; it < n_iter; it++) {
         bool any = false;
         for (int k = 0; k < e.kpad; k++) any |= (e.h_live[k] && !e.h_done[k]);
@@ -427,10 +436,10 @@ int engine_step(saige_ctx* c, Engine& e, int n_iter, uint8_t* done_out, int32_t*
             SAIGE_HIP_TRY(launch_grm_gemv(st, g.S, Np, N, e.P, e.AP, a, e.partial, &np, tv));
         else
             SAIGE_HIP_TRY(launch_grm_gemm(st, g.S, Np, N, e.P, e.AP, a, e.partial, &np, e.ypart,
-                                          quarter >= 0 ? c->gemm_cfg_quarter
+                                          quarter       ? c->gemm_cfg_quarter
                                           : halves == 3 ? c->gemm_cfg
                                                         : c->gemm_cfg_half,
-                                          halves, &c->tri, g.s16, quarter));
+                                          halves, &c->tri, g.s16, quarter ? &qc : nullptr));
         SAIGE_HIP_TRY(hipEventRecord(c->ev1, st));
         SAIGE_HIP_TRY(launch_alpha(st, e.kpad, e.partial, np, e.rz, e.done, e.live, e.kind, e.alpha));
         int64_t np2 = 0;
@@ -444,7 +453,7 @@ int engine_step(saige_ctx* c, Engine& e, int n_iter, uint8_t* done_out, int32_t*
         float ms = 0.f;
         SAIGE_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
         const int kind = e.kpad == 1 ? SAIGE_LAUNCH_GEMV
-                         : quarter >= 0 ? SAIGE_LAUNCH_GEMM_QUARTER
+                         : quarter      ? SAIGE_LAUNCH_GEMM_QUARTER
                          : halves == 3  ? SAIGE_LAUNCH_GEMM_FULL
                                         : SAIGE_LAUNCH_GEMM_HALF;
         e.gemm_seconds[kind] += ms * 1e-3;
@@ -1244,10 +1253,11 @@ static int grm_matvec(saige_ctx* c, int loco, const float* B, int64_t k, float* 
         // the lower-triangle product runs as narrow as the block: 16, 32 or
         // 64 columns
         const bool q = kb <= 16 && c->gemm_cfg.mode == GEMM_BF16X3_TRI;
+        const QuarterCols q0 = QuarterCols::tile(0);
         if (e == hipSuccess)
             e = launch_grm_gemm(st, g.S, Np, N, d_blk, d_out, a, nullptr, nullptr, c->d_mv_ypart,
                                 q ? c->gemm_cfg_quarter : kb <= 32 ? c->gemm_cfg_half : c->gemm_cfg,
-                                kb <= 32 ? 1 : 3, &c->tri, g.s16, q ? 0 : -1);
+                                kb <= 32 ? 1 : 3, &c->tri, g.s16, q ? &q0 : nullptr);
         if (e == hipSuccess) e = launch_gather_cols(st, d_out, kb, N, EW, d_in);
         if (e == hipSuccess) e = hipMemcpyAsync(Y + k0 * N, d_in, 4 * N * kb, hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -1902,7 +1912,9 @@ int saige_autotune(saige_ctx* c, int reps, double* ms_table) {
     a.mode = 0;
     // time one shape: one warm-up launch, then reps launches between events
     const bool s16 = grm_view(c, 0).s16;
-    auto time_shape = [&](GemmConfig g, int halves, double* out_ms, int quarter = -1) -> hipError_t {
+    const QuarterCols q0 = QuarterCols::tile(0);
+    auto time_shape = [&](GemmConfig g, int halves, double* out_ms, bool q = false) -> hipError_t {
+        const QuarterCols* quarter = q ? &q0 : nullptr;
         hipError_t e = launch_grm_gemm(st, c->d_grm, Np, N, c->d_mv_blk, c->d_mv_out, a, nullptr, nullptr,
                                        c->d_mv_ypart, g, halves, &c->tri, s16, quarter);
         if (e == hipSuccess) e = hipEventRecord(c->ev0, st);
@@ -1978,7 +1990,7 @@ int saige_autotune(saige_ctx* c, int reps, double* ms_table) {
                     bh = g;
                     mh = per;
                 }
-                SAIGE_HIP_TRY(time_shape(g, 1, &per, 0));
+                SAIGE_HIP_TRY(time_shape(g, 1, &per, true));
                 if (ms_table) ms_table[kNumGemmCandidates + kNumHalfCandidates + i] = per;
                 if (mq < 0.0 || per < mq) {
                     bq = g;

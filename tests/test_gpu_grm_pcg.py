@@ -422,6 +422,68 @@ def test_gemm_bf16x3_tri_mode(session, grm1000):
         session.set_gemm_config(1, 1, 4)
 
 
+def test_scattered_slots_run_16_columns(session, grm1000):
+    """At most 16 occupied slots anywhere in the block run as one 16-column
+    launch of the lower-triangle product (launch column k -> block column
+    of the k-th occupied slot, free slots after them): four columns in
+    slots of four different 16-column tiles take only 16-column launches
+    and give the bits each gives alone in slot 0 (launch column 0, block
+    column 0); 20 occupied slots spread over the block (done slots stay
+    occupied until taken) run 64-column launches, with the same bits
+    again."""
+    og, S = grm1000
+    print("case: scattered slots in 16-column launches", flush=True)
+    session.build_grm("fp32")
+    rng = np.random.default_rng(77)
+    B = rng.standard_normal((1000, 20)).astype(np.float32)
+    w = rng.uniform(0.05, 0.25, 1000).astype(np.float32)
+    tau = np.array([1.0, 0.5], np.float32)
+    try:
+        session.set_gemm_mode("bf16x3_tri")
+        session.set_gemm_config(1, 1, 8)
+        session.engine_reserve(64, 1)
+        session.engine_set_system(0, w, tau)
+        alone = []
+        for j in range(4):
+            session.engine_add(0, 0, B[:, j], 500, 1e-5)
+            while not session.engine_step(50)[0][0]:
+                pass
+            alone.append(session.engine_take(0))
+        slots = [5, 21, 38, 60]
+        n_full0 = session.engine_stats("gemm_full")[1] + session.engine_stats("gemm_half")[1]
+        n_q0 = session.engine_stats("gemm_quarter")[1]
+        for j, sl in enumerate(slots):
+            session.engine_add(sl, 0, B[:, j], 500, 1e-5)
+        steps = 0
+        while True:
+            done, _ = session.engine_step(1)
+            steps += 1
+            if all(done[sl] for sl in slots):
+                break
+        assert session.engine_stats("gemm_quarter")[1] - n_q0 == steps
+        assert session.engine_stats("gemm_full")[1] + session.engine_stats("gemm_half")[1] == n_full0
+        for j, sl in enumerate(slots):
+            got = session.engine_take(sl)
+            assert np.array_equal(got.view(np.uint32), alone[j].view(np.uint32)), sl
+        # 20 occupied slots spread over the block: 64-column launches first
+        many = list(range(1, 64, 3))[:20]
+        n_full0 = session.engine_stats("gemm_full")[1]
+        for j, sl in enumerate(many):
+            session.engine_add(sl, 0, B[:, j], 500, 1e-5)
+        for _ in range(2000):
+            done, _ = session.engine_step(1)
+            if all(done[sl] for sl in many):
+                break
+        assert session.engine_stats("gemm_full")[1] > n_full0
+        for j, sl in enumerate(many):
+            got = session.engine_take(sl)
+            if j < 4:
+                assert np.array_equal(got.view(np.uint32), alone[j].view(np.uint32)), sl
+    finally:
+        session.set_gemm_mode("fp32")
+        session.set_gemm_config(1, 1, 4)
+
+
 def test_gemm_tri_mode_many_items(session):
     """The lower-triangle product where row sets span several items (N =
     4700: 13 row sets of 384 rows, up to 144 chunks left of the diagonal
