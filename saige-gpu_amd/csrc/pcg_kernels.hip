@@ -813,7 +813,6 @@ __global__ __launch_bounds__(64 * NWV) void k_gemm_t16(const float* __restrict__
                                                        float* __restrict__ y2p, int q0, ColMap16 cm) {
     constexpr int GM = 32 * NWV;
     constexpr bool PIPE = NWV == 8;                  // software-pipelined chunk loop (register room)
-    constexpr bool GRP = NWV == 12 && NQ == 1;       // chunks in groups of four (16 columns)
     constexpr int RING = PD + 1;                     // raw chunks held
     constexpr int NTH = 64 * NWV;
     constexpr int NSTG = 192 * NQ;                   // P pieces a chunk needs: 4 groups x 3 pieces x 16*NQ columns
@@ -988,108 +987,7 @@ __global__ __launch_bounds__(64 * NWV) void k_gemm_t16(const float* __restrict__
             }
             chunk_mfma(par, c, ah, am, al);
         };
-        if constexpr (GRP) {
-            // a 16-column launch of twelve waves: chunks in groups of QG = 4.
-            // A group's P pieces are staged together (chunk g0 + sub at LDS
-            // columns 16*sub + [0, 16)) and its use-2 tiles summed together
-            // (rhs 16*sub + [0, 16) of the LDS tile), so a group takes two
-            // barriers instead of eight; the S ring, the MFMAs and every sum
-            // are those of the chunk loop below.  The 4 x 192 pieces of a
-            // group are one per thread.
-            constexpr int QG = 4;
-            static_assert(QG % RING == 0 && NTH == QG * NSTG, "group layout");
-            const int sub_t = tid / NSTG, kk_t = tid % NSTG, gp_t = kk_t / 16;
-            const u32x4* gsrc = pb + ((int64_t)(gp_t % 3) * G + gp_t / 3) * EW + gcol(kk_t % 16);
-            const int gdi = gp_t * 64 + 16 * sub_t + kk_t % 16;
-            auto load_group = [&](int64_t g0) { return gsrc[4 * (g0 + sub_t < cb ? g0 + sub_t : cb - 1) * EW]; };
-            auto issue_s = [&](float4 (&rs)[2][2], int64_t ch) {
-#pragma unroll
-                for (int r = 0; r < 2; r++)
-#pragma unroll
-                    for (int h = 0; h < 2; h++)
-                        rs[r][h] = *reinterpret_cast<const float4*>(srow + (int64_t)r * 16 * ld + ch * 32 + 4 * h);
-            };
-            auto group_mfma = [&](int par, int sub, const u32x4 (&ah)[2], const u32x4 (&am)[2],
-                                  const u32x4 (&al)[2]) {
-                const u32x4* sb = (par ? sB1 : sB0) + g * 3 * 64 + 16 * sub + ln;
-#pragma unroll
-                for (int r = 0; r < 2; r++) {
-                    if (SP == 3)
-                        acc1[r][0] = mfma16_x3(ah[r], am[r], al[r], sb[0], sb[64], sb[128], acc1[r][0]);
-                    else
-                        acc1[r][0] = mfma16_a2(ah[r], am[r], sb[0], sb[64], sb[128], acc1[r][0]);
-                }
-                if (U2) {
-                    u32x4 th[2], tm[2], tl[2];
-                    t16_transpose(ah, esel, th);
-                    t16_transpose(am, esel, tm);
-                    if (SP == 3) t16_transpose(al, esel, tl);
-                    float* dst = sR[0] + wv * 32 * 64;
-#pragma unroll
-                    for (int jg = 0; jg < 2; jg++) {
-                        f32x4 c2 = {0.0f, 0.0f, 0.0f, 0.0f};
-                        if (SP == 3)
-                            c2 = mfma16_x3(pah[0], pam[0], pal[0], th[jg], tm[jg], tl[jg], c2);
-                        else
-                            c2 = mfma16_b2(pah[0], pam[0], pal[0], th[jg], tm[jg], c2);
-                        *reinterpret_cast<float4*>(dst + tri_sr_index(16 * jg + ln, 16 * sub + 4 * g)) =
-                            make_float4(c2[0], c2[1], c2[2], c2[3]);
-                    }
-                }
-            };
-            // the group's use-2 partials: waves [0, 8) sum one (column, rhs)
-            // entry of each chunk over the waves, in wave order
-            auto reduce_group = [&](int64_t g0) {
-                if (wv < 8) {
-                    const int o = wv * 64 + lane, j = o / 16, ql = o % 16;
-#pragma unroll
-                    for (int sub = 0; sub < QG; sub++) {
-                        const int64_t cc = g0 + sub;
-                        if (cc < cb) {
-                            const float* src = sR[0] + tri_sr_index(j, 16 * sub + ql);
-                            float sum = src[0];
-#pragma unroll
-                            for (int w = 1; w < NWV; w++) sum += src[w * 32 * 64];
-                            y2p[(tri_y2_first<NWV>(cc, NR) + (R - cc / NWV - 1)) * 2048 + j * 64 + ql] = sum;
-                        }
-                    }
-                }
-            };
-#pragma unroll
-            for (int d = 0; d < PD; d++) issue_s(ring[d], ca + d < cb ? ca + d : cb - 1);
-            u32x4 bnext = load_group(ca);
-            for (int64_t g0 = ca; g0 < cb; g0 += QG) {
-                // group g0's pieces go to buffer par, last read in the group
-                // before the previous one, before the previous staging barrier
-                const int par = (int)(((g0 - ca) / QG) & 1);
-                (par ? sB1 : sB0)[gdi] = bnext;
-                lds_barrier();
-                bnext = load_group(g0 + QG);
-#pragma unroll
-                for (int sub = 0; sub < QG; sub++) {
-                    const int64_t c = g0 + sub;
-                    if (c < cb) {
-                        const int u = sub % RING;
-                        u32x4 ah[2], am[2], al[2];
-#pragma unroll
-                        for (int r = 0; r < 2; r++) {
-                            if (SP == 3)
-                                split3_8(ring[u][r][0], ring[u][r][1], ah[r], am[r], al[r]);
-                            else
-                                split2_8(ring[u][r][0], ring[u][r][1], ah[r], am[r]);
-                        }
-                        issue_s(ring[(sub + PD) % RING], c + PD < cb ? c + PD : cb - 1);
-                        group_mfma(par, sub, ah, am, al);
-                    }
-                }
-                // every wave's tiles are in LDS; the sums are read before the
-                // next group's staging barrier, after which tiles are written
-                if (U2) {
-                    lds_barrier();
-                    reduce_group(g0);
-                }
-            }
-        } else if (PIPE) {
+        if (PIPE) {
             // software-pipelined: chunk c's pieces were split during chunk
             // c-1, so this chunk's MFMAs run beside the next chunk's split.
             // Raw chunks c+1 .. c+RING are resident or in flight (slot (c-ca)
@@ -1871,7 +1769,7 @@ hipError_t launch_tri(hipStream_t s, const float* S, int64_t Npad, const float* 
     ColMap16 cm{};
     if (quarter) {
         for (int k = 0; k < 16; k++) cm.w[k / 4] |= (uint32_t)quarter->col[k] << (8 * (k % 4));
-        if (deep) launch_t16<NWV, NWV == 12 ? 3 : 2, 1, SP>(s, grid, S, Npad, pb, in, t, 0, cm, ypart);
+        if (deep) launch_t16<NWV, 2, 1, SP>(s, grid, S, Npad, pb, in, t, 0, cm, ypart);
         else launch_t16<NWV, 1, 1, SP>(s, grid, S, Npad, pb, in, t, 0, cm, ypart);
     } else if (halves == 3) {
         if (deep) launch_t16<NWV, 2, 4, SP>(s, grid, S, Npad, pb, in, t, 0, cm, ypart);
