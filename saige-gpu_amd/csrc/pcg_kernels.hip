@@ -817,6 +817,7 @@ __global__ __launch_bounds__(64 * NWV) void k_gemm_t16(const float* __restrict__
     constexpr int NTH = 64 * NWV;
     constexpr int NSTG = 192 * NQ;                   // P pieces a chunk needs: 4 groups x 3 pieces x 16*NQ columns
     constexpr int STG = (NSTG + NTH - 1) / NTH;      // P pieces staged per thread and chunk
+    constexpr int YS = 16 * NQ;                      // row stride of the partials: the launch's columns
     __shared__ u32x4 sB0[12 * 64], sB1[12 * 64];     // a chunk's P pieces, [(g*3 + piece)*64 + n]
     constexpr int SRB = NWV == 8 ? 2 : 1;            // use-2 tile buffers
     __shared__ float sR[SRB][NWV * 32 * 64];         // use-2 results, [wave][column j][rhs - 16*q0]
@@ -895,7 +896,7 @@ __global__ __launch_bounds__(64 * NWV) void k_gemm_t16(const float* __restrict__
                 const int o = (wv * 64 + lane) * NQ;
                 const int j = o / (16 * NQ), ql = o % (16 * NQ);
                 const float* src = sr + tri_sr_index(j, ql);
-                float* dst = y2p + (tri_y2_first<NWV>(cc, NR) + (R - cc / NWV - 1)) * 2048 + j * 64 + c0 + ql;
+                float* dst = y2p + (tri_y2_first<NWV>(cc, NR) + (R - cc / NWV - 1)) * (32 * YS) + j * YS + ql;
                 if constexpr (NQ == 4) {
                     float4 sum = *reinterpret_cast<const float4*>(src);
 #pragma unroll
@@ -1061,45 +1062,46 @@ __global__ __launch_bounds__(64 * NWV) void k_gemm_t16(const float* __restrict__
     else
         run(std::integral_constant<bool, true>());
     if (!valid) return;
-    float* yp = y1p + slot * GM * EW;
+    float* yp = y1p + slot * GM * YS;
 #pragma unroll
     for (int r = 0; r < 2; r++)
 #pragma unroll
         for (int t = 0; t < NQ; t++)
 #pragma unroll
-            for (int a = 0; a < 4; a++) yp[(int64_t)(wv * 32 + 16 * r + 4 * g + a) * EW + c0 + 16 * t + ln] = acc1[r][t][a];
+            for (int a = 0; a < 4; a++) yp[(int64_t)(wv * 32 + 16 * r + 4 * g + a) * YS + 16 * t + ln] = acc1[r][t][a];
 }
 
 // k_tri_reduce: Y[row][q] = the row set's use-1 partials (rtab[R] = {first
 // slot, count}) in slot order, then the chunk's use-2 partials in row-set
 // order; float4 per thread, QW threads per row (16: all 64 columns, 8: the
-// half at column q0, 4: the 16 partial columns [0, 16) of a 16-column
-// launch, stored to the block columns of cm).
+// half at column q0, 4: the 16 columns of a 16-column launch, stored to the
+// block columns of cm).  The partials hold the launch's 4 * QW columns only
+// (row stride 4 * QW).
 template <int NWV, int QW>
 __global__ __launch_bounds__(256) void k_tri_reduce(const float* __restrict__ y1p, const float* __restrict__ y2p,
                                                     const int2* __restrict__ rtab, int64_t ld, int q0,
                                                     ColMap16 cm, float* __restrict__ y) {
-    constexpr int GM = 32 * NWV;
+    constexpr int GM = 32 * NWV, YS = 4 * QW;
     const int64_t row = (int64_t)blockIdx.x * (256 / QW) + threadIdx.x / QW;
-    const int q = q0 + 4 * (threadIdx.x % QW);
+    const int q = 4 * (threadIdx.x % QW);               // the launch's column
     const int64_t nch = ld / 32, NR = (nch + NWV - 1) / NWV;
     const int64_t c = row / 32, R = c / NWV;
     const int2 rt = rtab[R];
     const int64_t rr = row - R * GM;
-    const float* p1 = y1p + ((int64_t)rt.x * GM + rr) * EW + q;
+    const float* p1 = y1p + ((int64_t)rt.x * GM + rr) * YS + q;
     float4 s = *reinterpret_cast<const float4*>(p1);
     for (int k = 1; k < rt.y; k++) {
-        const float4 v = *reinterpret_cast<const float4*>(p1 + (int64_t)k * GM * EW);
+        const float4 v = *reinterpret_cast<const float4*>(p1 + (int64_t)k * GM * YS);
         s.x += v.x;
         s.y += v.y;
         s.z += v.z;
         s.w += v.w;
     }
-    const float* p2 = y2p + tri_y2_first<NWV>(c, NR) * 2048 + (row % 32) * 64 + q;
+    const float* p2 = y2p + tri_y2_first<NWV>(c, NR) * (32 * YS) + (row % 32) * YS + q;
     const int64_t n2 = NR - 1 - R;
 #pragma unroll 8
     for (int64_t k = 0; k < n2; k++) {
-        const float4 v = *reinterpret_cast<const float4*>(p2 + k * 2048);
+        const float4 v = *reinterpret_cast<const float4*>(p2 + k * (32 * YS));
         s.x += v.x;
         s.y += v.y;
         s.z += v.z;
@@ -1112,7 +1114,7 @@ __global__ __launch_bounds__(256) void k_tri_reduce(const float* __restrict__ y1
         yr[map_col(cm, q + 2)] = s.z;
         yr[map_col(cm, q + 3)] = s.w;
     } else {
-        *reinterpret_cast<float4*>(y + row * EW + q) = s;
+        *reinterpret_cast<float4*>(y + row * EW + q0 + q) = s;
     }
 }
 

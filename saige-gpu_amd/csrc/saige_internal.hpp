@@ -81,8 +81,8 @@ struct TriPlan {
     int64_t nitems = 0, nr = 0, y1_slots = 0, y2_slots = 0;
     int4* items = nullptr;       // device [nitems], longest first
     int2* rtab = nullptr;        // device [nr]
-    float* y1p = nullptr;        // [y1_slots][32*nwv][EW]
-    float* y2p = nullptr;        // [y2_slots][32][EW]
+    float* y1p = nullptr;        // [y1_slots][32*nwv][W], W = the launch's width (<= EW)
+    float* y2p = nullptr;        // [y2_slots][32][W]
 };
 // builds (or keeps, when Npad/nwv/ch match) the plan; frees on failure
 hipError_t tri_plan_build(TriPlan& t, int64_t Npad, int nwv, int ch);
