@@ -350,10 +350,10 @@ int saige_get_gemm_config_quarter(const saige_ctx* ctx, int* rt, int* pd, double
  * on the split GRM, whose values need two pieces), on the 16x16x32 MFMA,
  * read from the lower triangle of the GRM only (half the bytes): each 32x32
  * block left of the diagonal gives both S_IJ P_J and S_IJ^T P_I.  A launch
- * covers 64, 32 or 16 columns, as few as hold live slots (at most 16
- * occupied slots run as one 16-column launch wherever they sit; up to 32 in
- * one half as a 32-column one), with the same per-column arithmetic
- * (SAIGE_LAUNCH_GEMM_QUARTER counts the 16-column ones).  rt selects the workgroup (1: twelve waves, 2: eight), pd 1 or 3
+ * covers 64, 32 or 16 columns, as few as hold live slots (at most 16 or 32
+ * occupied slots run as one 16- or 32-column launch wherever they sit), with
+ * the same per-column arithmetic (SAIGE_LAUNCH_GEMM_QUARTER counts the
+ * 16-column ones, SAIGE_LAUNCH_GEMM_HALF the 32-column ones).  rt selects the workgroup (1: twelve waves, 2: eight), pd 1 or 3
  * the prefetch depth (one or two chunks ahead); nsplit is unused.  It needs device scratch for
  * its partial products, about 128 N^2 / (32 * waves) bytes (3.3 GB at
  * N = 100k with twelve waves), allocated on first use.  Changing the mode

@@ -681,8 +681,8 @@ __global__ __launch_bounds__(768) void k_gemm_x3w(const float* __restrict__ S, i
 // ------------------------------------------------- lower-triangle product
 // k_gemm_t16<NWV, PD, NQ, SP>: the three-piece product read from the lower
 // triangle of S only (half the bytes), on the 16x16x32 BF16 MFMA, over NQ
-// tiles of 16 right-hand sides (columns 16*q0 + [0, 16*NQ)): NQ = 4 for 64
-// columns, 2 for 32, 1 for 16.  S is symmetric, so a 32x32 block S_IJ left
+// tiles of 16 right-hand sides: NQ = 4 for all 64 columns, 2 for 32 and 1
+// for 16 block columns that a map names.  S is symmetric, so a 32x32 block S_IJ left
 // of the diagonal serves two products:
 //   use 1: Y_I += S_IJ P_J
 //   use 2: Y_J += S_IJ^T P_I
@@ -719,11 +719,11 @@ __global__ __launch_bounds__(768) void k_gemm_x3w(const float* __restrict__ S, i
 // row-set order: a fixed order, so results are reproducible, and a column's
 // arithmetic is the same whatever NQ, so a launch can be as narrow as its
 // live columns (a 16-column launch does a quarter of a full one's MFMA work)
-// and a column's bits still do not depend on its co-tenants.  A 16-column
-// launch takes any 16 block columns (a map of launch column -> block column),
-// so at most 16 live columns anywhere in the block run in one.  The
-// arithmetic of an output column depends only on its own column of P, in
-// whatever position of the MFMA operands it sits.
+// and a column's bits still do not depend on its co-tenants.  A 16- or
+// 32-column launch takes any 16 or 32 block columns (a map of launch column
+// -> block column), so at most 16 or 32 live columns anywhere in the block
+// run in one.  The arithmetic of an output column depends only on its own
+// column of P, in whatever position of the MFMA operands it sits.
 // Eight waves have registers for a software-pipelined loop: chunk c's pieces
 // are split during chunk c-1, beside that chunk's MFMAs.  Twelve waves split
 // each chunk at its own turn, PD chunks of raw data ahead.
@@ -796,21 +796,21 @@ __device__ __forceinline__ void t16_transpose(const u32x4 (&a)[2], const u32x4 (
     }
 }
 
-// block column of launch column k in a 16-column launch: byte k of the map,
-// picked with selects (no indexing into the kernel argument)
-__device__ __forceinline__ int map_col(const ColMap16& m, int k) {
-    const uint32_t w = (k & 8) ? ((k & 4) ? m.w[3] : m.w[2]) : ((k & 4) ? m.w[1] : m.w[0]);
-    return (int)((w >> (8 * (k & 3))) & 0xFFu);
+// block column of launch column k (< 32) of a narrow launch: byte k of the
+// map, picked with selects (no indexing into the kernel argument)
+__device__ __forceinline__ int map_col(const ColMap32& m, int k) {
+    const uint32_t lo = (k & 8) ? ((k & 4) ? m.w[3] : m.w[2]) : ((k & 4) ? m.w[1] : m.w[0]);
+    const uint32_t hi = (k & 8) ? ((k & 4) ? m.w[7] : m.w[6]) : ((k & 4) ? m.w[5] : m.w[4]);
+    return (int)((((k & 16) ? hi : lo) >> (8 * (k & 3))) & 0xFFu);
 }
 
-// q0: the launch's first 16-column tile (NQ 2 and 4); cm: the block columns
-// of a 16-column launch (NQ 1), which holds its P pieces, partials and LDS
-// tiles at columns [0, 16)
+// cm: the block columns of a 16- or 32-column launch (NQ 1 or 2), which
+// holds its P pieces, partials and LDS tiles at launch columns [0, 16*NQ)
 template <int NWV, int PD, int NQ, int SP>
 __global__ __launch_bounds__(64 * NWV) void k_gemm_t16(const float* __restrict__ S, int64_t ld,
                                                        const u32x4* __restrict__ pb, const float* __restrict__ P,
                                                        const int4* __restrict__ items, float* __restrict__ y1p,
-                                                       float* __restrict__ y2p, int q0, ColMap16 cm) {
+                                                       float* __restrict__ y2p, ColMap32 cm) {
     constexpr int GM = 32 * NWV;
     constexpr bool PIPE = NWV == 8;                  // software-pipelined chunk loop (register room)
     constexpr int RING = PD + 1;                     // raw chunks held
@@ -820,7 +820,7 @@ __global__ __launch_bounds__(64 * NWV) void k_gemm_t16(const float* __restrict__
     constexpr int YS = 16 * NQ;                      // row stride of the partials: the launch's columns
     __shared__ u32x4 sB0[12 * 64], sB1[12 * 64];     // a chunk's P pieces, [(g*3 + piece)*64 + n]
     constexpr int SRB = NWV == 8 ? 2 : 1;            // use-2 tile buffers
-    __shared__ float sR[SRB][NWV * 32 * 64];         // use-2 results, [wave][column j][rhs - 16*q0]
+    __shared__ float sR[SRB][NWV * 32 * 64];         // use-2 results, [wave][column j][launch column]
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, g = lane >> 4, ln = lane & 15;
     const int4 item = items[blockIdx.x];
     const int64_t R = item.x, ca = item.y, cb = item.z, slot = item.w;
@@ -828,9 +828,9 @@ __global__ __launch_bounds__(64 * NWV) void k_gemm_t16(const float* __restrict__
     const int64_t rw = R * GM + wv * 32;             // the wave's first row
     const bool valid = rw < ld;                      // ld = Npad, a multiple of 32
     const float* srow = S + ((valid ? rw : ld - 32) + ln) * ld + 8 * g;
-    const int c0 = NQ == 1 ? 0 : 16 * q0;           // the launch's first column in LDS and partials
+    constexpr int c0 = 0;                            // the launch's first column in LDS and partials
     // the block column of launch column k (of 16 * NQ)
-    auto gcol = [&](int k) { return NQ == 1 ? map_col(cm, k) : c0 + k; };
+    auto gcol = [&](int k) { return NQ < 4 ? map_col(cm, k) : k; };
     // staging: element k of a chunk's NSTG pieces is launch column k % (16*NQ)
     // of (group, piece) k / (16*NQ); threads past the count repeat an
     // element of the first round
@@ -1073,14 +1073,14 @@ __global__ __launch_bounds__(64 * NWV) void k_gemm_t16(const float* __restrict__
 
 // k_tri_reduce: Y[row][q] = the row set's use-1 partials (rtab[R] = {first
 // slot, count}) in slot order, then the chunk's use-2 partials in row-set
-// order; float4 per thread, QW threads per row (16: all 64 columns, 8: the
-// half at column q0, 4: the 16 columns of a 16-column launch, stored to the
-// block columns of cm).  The partials hold the launch's 4 * QW columns only
-// (row stride 4 * QW).
+// order; float4 per thread, QW threads per row (16: all 64 columns; 8 and 4:
+// the 32 or 16 columns of a narrow launch, stored to the block columns of
+// cm).  The partials hold the launch's 4 * QW columns only (row stride
+// 4 * QW).
 template <int NWV, int QW>
 __global__ __launch_bounds__(256) void k_tri_reduce(const float* __restrict__ y1p, const float* __restrict__ y2p,
-                                                    const int2* __restrict__ rtab, int64_t ld, int q0,
-                                                    ColMap16 cm, float* __restrict__ y) {
+                                                    const int2* __restrict__ rtab, int64_t ld, ColMap32 cm,
+                                                    float* __restrict__ y) {
     constexpr int GM = 32 * NWV, YS = 4 * QW;
     const int64_t row = (int64_t)blockIdx.x * (256 / QW) + threadIdx.x / QW;
     const int q = 4 * (threadIdx.x % QW);               // the launch's column
@@ -1107,14 +1107,14 @@ __global__ __launch_bounds__(256) void k_tri_reduce(const float* __restrict__ y1
         s.z += v.z;
         s.w += v.w;
     }
-    if constexpr (QW == 4) {
+    if constexpr (QW < 16) {
         float* yr = y + row * EW;
         yr[map_col(cm, q)] = s.x;
         yr[map_col(cm, q + 1)] = s.y;
         yr[map_col(cm, q + 2)] = s.z;
         yr[map_col(cm, q + 3)] = s.w;
     } else {
-        *reinterpret_cast<float4*>(y + row * EW + q0 + q) = s;
+        *reinterpret_cast<float4*>(y + row * EW + q) = s;
     }
 }
 
@@ -1751,35 +1751,37 @@ hipError_t tri_plan_build(TriPlan& t, int64_t Npad, int nwv, int ch) {
 namespace {
 template <int NWV, int PD, int NQ, int SP>
 void launch_t16(hipStream_t s, dim3 grid, const float* S, int64_t Npad, const u32x4* pb, const float* in,
-                const TriPlan& t, int q0, ColMap16 cm, float* ypart) {
+                const TriPlan& t, ColMap32 cm, float* ypart) {
     hipLaunchKernelGGL((k_gemm_t16<NWV, PD, NQ, SP>), grid, dim3(64 * NWV), 0, s, S, Npad, pb, in, t.items, t.y1p,
-                       t.y2p, q0, cm);
+                       t.y2p, cm);
     hipLaunchKernelGGL((k_tri_reduce<NWV, 4 * NQ>), dim3((unsigned)(Npad * 4 * NQ / 256)), dim3(256), 0, s, t.y1p,
-                       t.y2p, t.rtab, Npad, NQ == 1 ? 0 : 16 * q0, cm, ypart);
+                       t.y2p, t.rtab, Npad, cm, ypart);
 }
 
-// the lower-triangle product over the launch's 64, 32 (halves 1 or 2) or 16
-// (quarter: the block columns it lists) columns
+// the lower-triangle product over the launch's 64 columns, or the 16 or 32
+// block columns narrow lists (nullptr and halves 1 or 2: the half's 32)
 template <int NWV, int SP>
 hipError_t launch_tri(hipStream_t s, const float* S, int64_t Npad, const float* in, float* ypart, int pd,
-                      int halves, const QuarterCols* quarter, const TriPlan& t) {
+                      int halves, const NarrowCols* narrow, const TriPlan& t) {
     const int64_t G = Npad / 8;
     u32x4* pb = reinterpret_cast<u32x4*>(ypart + (int64_t)GSPLIT_MAX * Npad * EW);
     hipLaunchKernelGGL(k_split_p, dim3((unsigned)(G * EW / 256)), dim3(256), 0, s, in, G, pb);
     const dim3 grid((unsigned)t.nitems);
     const bool deep = pd == 3;
-    ColMap16 cm{};
-    if (quarter) {
-        for (int k = 0; k < 16; k++) cm.w[k / 4] |= (uint32_t)quarter->col[k] << (8 * (k % 4));
-        if (deep) launch_t16<NWV, 2, 1, SP>(s, grid, S, Npad, pb, in, t, 0, cm, ypart);
-        else launch_t16<NWV, 1, 1, SP>(s, grid, S, Npad, pb, in, t, 0, cm, ypart);
-    } else if (halves == 3) {
-        if (deep) launch_t16<NWV, 2, 4, SP>(s, grid, S, Npad, pb, in, t, 0, cm, ypart);
-        else launch_t16<NWV, 1, 4, SP>(s, grid, S, Npad, pb, in, t, 0, cm, ypart);
+    const NarrowCols half = NarrowCols::tile(32, halves == 2 ? 32 : 0);
+    if (!narrow && halves != 3) narrow = &half;
+    ColMap32 cm{};
+    if (narrow)
+        for (int k = 0; k < narrow->width; k++) cm.w[k / 4] |= (uint32_t)narrow->col[k] << (8 * (k % 4));
+    if (narrow && narrow->width == 16) {
+        if (deep) launch_t16<NWV, 2, 1, SP>(s, grid, S, Npad, pb, in, t, cm, ypart);
+        else launch_t16<NWV, 1, 1, SP>(s, grid, S, Npad, pb, in, t, cm, ypart);
+    } else if (narrow) {
+        if (deep) launch_t16<NWV, 2, 2, SP>(s, grid, S, Npad, pb, in, t, cm, ypart);
+        else launch_t16<NWV, 1, 2, SP>(s, grid, S, Npad, pb, in, t, cm, ypart);
     } else {
-        const int q0 = halves == 2 ? 2 : 0;
-        if (deep) launch_t16<NWV, 2, 2, SP>(s, grid, S, Npad, pb, in, t, q0, cm, ypart);
-        else launch_t16<NWV, 1, 2, SP>(s, grid, S, Npad, pb, in, t, q0, cm, ypart);
+        if (deep) launch_t16<NWV, 2, 4, SP>(s, grid, S, Npad, pb, in, t, cm, ypart);
+        else launch_t16<NWV, 1, 4, SP>(s, grid, S, Npad, pb, in, t, cm, ypart);
     }
     return hipGetLastError();
 }
@@ -1787,7 +1789,7 @@ hipError_t launch_tri(hipStream_t s, const float* S, int64_t Npad, const float* 
 
 hipError_t launch_grm_gemm(hipStream_t s, const float* S, int64_t Npad, int64_t Nnm, const float* in,
                            float* out, SigmaArgs a, double* partial, int64_t* nparts, float* ypart, GemmConfig g,
-                           int halves, const TriPlan* tri, bool s16, const QuarterCols* quarter) {
+                           int halves, const TriPlan* tri, bool s16, const NarrowCols* narrow) {
     // Npad is a multiple of 256 = 128 * RT for both RT; a slice holds at least
     // one superchunk when Npad / GKS >= nsplit
     if (!gemm_config_valid(g) || Npad % 256 != 0 || halves < 1 || halves > 3) return hipErrorInvalidValue;
@@ -1795,21 +1797,22 @@ hipError_t launch_grm_gemm(hipStream_t s, const float* S, int64_t Npad, int64_t 
         // the lower-triangle product sums its partials into slice 0
         if (!tri || tri->Npad != Npad || !tri->items) return hipErrorInvalidValue;
         hipError_t e;
-        if (quarter) {
-            // sixteen distinct block columns
+        if (narrow) {
+            // 16 or 32 distinct block columns
+            if (narrow->width != 16 && narrow->width != 32) return hipErrorInvalidValue;
             uint64_t seen = 0;
-            for (int k = 0; k < 16; k++) {
-                const int col = quarter->col[k];
+            for (int k = 0; k < narrow->width; k++) {
+                const int col = narrow->col[k];
                 if (col >= EW || (seen >> col & 1)) return hipErrorInvalidValue;
                 seen |= 1ull << col;
             }
         }
         if (s16)
-            e = tri->nwv == 8 ? launch_tri<8, 2>(s, S, Npad, in, ypart, g.pd, halves, quarter, *tri)
-                              : launch_tri<12, 2>(s, S, Npad, in, ypart, g.pd, halves, quarter, *tri);
+            e = tri->nwv == 8 ? launch_tri<8, 2>(s, S, Npad, in, ypart, g.pd, halves, narrow, *tri)
+                              : launch_tri<12, 2>(s, S, Npad, in, ypart, g.pd, halves, narrow, *tri);
         else
-            e = tri->nwv == 8 ? launch_tri<8, 3>(s, S, Npad, in, ypart, g.pd, halves, quarter, *tri)
-                              : launch_tri<12, 3>(s, S, Npad, in, ypart, g.pd, halves, quarter, *tri);
+            e = tri->nwv == 8 ? launch_tri<8, 3>(s, S, Npad, in, ypart, g.pd, halves, narrow, *tri)
+                              : launch_tri<12, 3>(s, S, Npad, in, ypart, g.pd, halves, narrow, *tri);
         if (e != hipSuccess) return e;
         const int64_t nb = Npad / EROWS;
         hipLaunchKernelGGL(k_gemm_epi, dim3((unsigned)nb), dim3(256), 0, s, ypart, 1, Npad, Nnm, in, out, a,

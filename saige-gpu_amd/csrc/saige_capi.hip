@@ -413,19 +413,22 @@ int engine_step(saige_ctx* c, Engine& e, int n_iter, uint8_t* done_out, int32_t*
             nlive++;
         }
     if (halves == 0) halves = 1;
-    // the lower-triangle product runs at most 16 occupied slots, wherever
-    // they sit, as one 16-column launch: the occupied slots in order, then
-    // free ones (computed, not used)
-    QuarterCols qc;
-    bool quarter = false;
-    if (e.kpad == EW && c->gemm_cfg.mode == GEMM_BF16X3_TRI && nlive > 0 && nlive <= 16) {
+    // the lower-triangle product runs at most 16 (32) occupied slots,
+    // wherever they sit, as one 16- (32-) column launch: the occupied slots
+    // in order, then free ones (computed, not used)
+    NarrowCols nc;
+    bool narrow = false;
+    if (e.kpad == EW && c->gemm_cfg.mode == GEMM_BF16X3_TRI && nlive > 0 && nlive <= 32) {
+        nc.width = nlive <= 16 ? 16 : 32;
         int n = 0;
         for (int k = 0; k < e.kpad; k++)
-            if (e.h_live[k]) qc.col[n++] = (uint8_t)k;
-        for (int k = 0; k < e.kpad && n < 16; k++)
-            if (!e.h_live[k]) qc.col[n++] = (uint8_t)k;
-        quarter = true;
+            if (e.h_live[k]) nc.col[n++] = (uint8_t)k;
+        for (int k = 0; k < e.kpad && n < nc.width; k++)
+            if (!e.h_live[k]) nc.col[n++] = (uint8_t)k;
+        narrow = true;
     }
+    const bool quarter = narrow && nc.width == 16;
+    if (narrow && !quarter) halves = 1;   // for the shape and the statistics below
     for (int it = 0; it < n_iter; it++) {
         bool any = false;
         for (int k = 0; k < e.kpad; k++) any |= (e.h_live[k] && !e.h_done[k]);
@@ -439,7 +442,7 @@ int engine_step(saige_ctx* c, Engine& e, int n_iter, uint8_t* done_out, int32_t*
                                           quarter       ? c->gemm_cfg_quarter
                                           : halves == 3 ? c->gemm_cfg
                                                         : c->gemm_cfg_half,
-                                          halves, &c->tri, g.s16, quarter ? &qc : nullptr));
+                                          halves, &c->tri, g.s16, narrow ? &nc : nullptr));
         SAIGE_HIP_TRY(hipEventRecord(c->ev1, st));
         SAIGE_HIP_TRY(launch_alpha(st, e.kpad, e.partial, np, e.rz, e.done, e.live, e.kind, e.alpha));
         int64_t np2 = 0;
@@ -1253,7 +1256,7 @@ static int grm_matvec(saige_ctx* c, int loco, const float* B, int64_t k, float* 
         // the lower-triangle product runs as narrow as the block: 16, 32 or
         // 64 columns
         const bool q = kb <= 16 && c->gemm_cfg.mode == GEMM_BF16X3_TRI;
-        const QuarterCols q0 = QuarterCols::tile(0);
+        const NarrowCols q0 = NarrowCols::tile(16, 0);
         if (e == hipSuccess)
             e = launch_grm_gemm(st, g.S, Np, N, d_blk, d_out, a, nullptr, nullptr, c->d_mv_ypart,
                                 q ? c->gemm_cfg_quarter : kb <= 32 ? c->gemm_cfg_half : c->gemm_cfg,
@@ -1912,9 +1915,9 @@ int saige_autotune(saige_ctx* c, int reps, double* ms_table) {
     a.mode = 0;
     // time one shape: one warm-up launch, then reps launches between events
     const bool s16 = grm_view(c, 0).s16;
-    const QuarterCols q0 = QuarterCols::tile(0);
+    const NarrowCols q0 = NarrowCols::tile(16, 0);
     auto time_shape = [&](GemmConfig g, int halves, double* out_ms, bool q = false) -> hipError_t {
-        const QuarterCols* quarter = q ? &q0 : nullptr;
+        const NarrowCols* quarter = q ? &q0 : nullptr;
         hipError_t e = launch_grm_gemm(st, c->d_grm, Np, N, c->d_mv_blk, c->d_mv_out, a, nullptr, nullptr,
                                        c->d_mv_ypart, g, halves, &c->tri, s16, quarter);
         if (e == hipSuccess) e = hipEventRecord(c->ev0, st);

@@ -59,19 +59,22 @@ enum GemmMode { GEMM_FP32 = 0, GEMM_BF16X3 = 1, GEMM_BF16X3_TRI = 2 };
 // one Npad.  Row set R = rows 32*nwv*R + [0, 32*nwv); item = (R, chunk range
 // [ca, cb) of 32 columns, use-1 slot); row set R's use-1 partials are the
 // slots rtab[R].x + [0, rtab[R].y).
-// the block columns of a 16-column launch of the lower-triangle GEMM: launch
-// column k computes block column col[k]; sixteen distinct columns below EW
-struct QuarterCols {
-    uint8_t col[16];
-    static QuarterCols tile(int q) {   // columns 16q + [0, 16)
-        QuarterCols c;
-        for (int k = 0; k < 16; k++) c.col[k] = (uint8_t)(16 * q + k);
+// the block columns of a 16- or 32-column launch of the lower-triangle
+// GEMM: launch column k < width computes block column col[k]; distinct
+// columns below EW
+struct NarrowCols {
+    int width = 16;
+    uint8_t col[32] = {};
+    static NarrowCols tile(int width, int first) {   // columns first + [0, width)
+        NarrowCols c;
+        c.width = width;
+        for (int k = 0; k < width; k++) c.col[k] = (uint8_t)(first + k);
         return c;
     }
 };
 // the same map as a kernel argument: byte k of w[k / 4] is col[k]
-struct ColMap16 {
-    uint32_t w[4];
+struct ColMap32 {
+    uint32_t w[8];
 };
 
 struct TriPlan {
@@ -343,12 +346,12 @@ bool gemm_config_valid(GemmConfig g);
 // s16: every value of S has at most 16 significant bits (the split GRM's
 // storage); the lower-triangle mode then splits S into two BF16 pieces
 // instead of three, with the same bits.
-// quarter (lower-triangle mode): only the 16 block columns it lists are
-// computed (halves is then unused); nullptr: the halves.
+// narrow (lower-triangle mode): only the 16 or 32 block columns it lists
+// are computed (halves is then unused); nullptr: the halves.
 hipError_t launch_grm_gemm(hipStream_t s, const float* S, int64_t Npad, int64_t Nnm, const float* in,
                            float* out, SigmaArgs a, double* partial, int64_t* nparts, float* ypart,
                            GemmConfig g, int halves = 3, const TriPlan* tri = nullptr, bool s16 = false,
-                           const QuarterCols* quarter = nullptr);
+                           const NarrowCols* narrow = nullptr);
 int64_t gemm_ypart_floats(int64_t Npad);
 int64_t gemv_max_parts(int64_t Npad);
 int64_t gemm_max_parts(int64_t Npad);

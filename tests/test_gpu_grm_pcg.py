@@ -422,15 +422,15 @@ def test_gemm_bf16x3_tri_mode(session, grm1000):
         session.set_gemm_config(1, 1, 4)
 
 
-def test_scattered_slots_run_16_columns(session, grm1000):
-    """At most 16 occupied slots anywhere in the block run as one 16-column
-    launch of the lower-triangle product (launch column k -> block column
-    of the k-th occupied slot, free slots after them): four columns in
-    slots of four different 16-column tiles take only 16-column launches
-    and give the bits each gives alone in slot 0 (launch column 0, block
-    column 0); 20 occupied slots spread over the block (done slots stay
-    occupied until taken) run 64-column launches, with the same bits
-    again."""
+def test_scattered_slots_run_narrow_launches(session, grm1000):
+    """At most 16 (32) occupied slots anywhere in the block run as one 16-
+    (32-) column launch of the lower-triangle product (launch column k ->
+    block column of the k-th occupied slot, free slots after them): four
+    columns in slots of four different 16-column tiles take only 16-column
+    launches and give the bits each gives alone in slot 0 (launch column 0,
+    block column 0); 20 occupied slots spread over both halves (done slots
+    stay occupied until taken) take only 32-column launches, and 40 only
+    64-column ones, with the same bits again."""
     og, S = grm1000
     print("case: scattered slots in 16-column launches", flush=True)
     session.build_grm("fp32")
@@ -465,20 +465,25 @@ def test_scattered_slots_run_16_columns(session, grm1000):
         for j, sl in enumerate(slots):
             got = session.engine_take(sl)
             assert np.array_equal(got.view(np.uint32), alone[j].view(np.uint32)), sl
-        # 20 occupied slots spread over the block: 64-column launches first
-        many = list(range(1, 64, 3))[:20]
-        n_full0 = session.engine_stats("gemm_full")[1]
-        for j, sl in enumerate(many):
-            session.engine_add(sl, 0, B[:, j], 500, 1e-5)
-        for _ in range(2000):
-            done, _ = session.engine_step(1)
-            if all(done[sl] for sl in many):
-                break
-        assert session.engine_stats("gemm_full")[1] > n_full0
-        for j, sl in enumerate(many):
-            got = session.engine_take(sl)
-            if j < 4:
-                assert np.array_equal(got.view(np.uint32), alone[j].view(np.uint32)), sl
+        # 20 occupied slots spread over both halves: 32-column launches; 40:
+        # 64-column ones
+        for many, kind in ((list(range(1, 64, 3))[:20], "gemm_half"), (list(range(0, 64))[3::3][:20] +
+                                                                        list(range(1, 64, 3))[:20], "gemm_full")):
+            counts0 = {k: session.engine_stats(k)[1] for k in ("gemm_quarter", "gemm_half", "gemm_full")}
+            for j, sl in enumerate(many):
+                session.engine_add(sl, 0, B[:, j % 20], 500, 1e-5)
+            steps = 0
+            for _ in range(2000):
+                done, _ = session.engine_step(1)
+                steps += 1
+                if all(done[sl] for sl in many):
+                    break
+            counts = {k: session.engine_stats(k)[1] - counts0[k] for k in counts0}
+            assert counts[kind] == steps and sum(counts.values()) == steps, (kind, counts, steps)
+            for j, sl in enumerate(many):
+                got = session.engine_take(sl)
+                if j < 4:
+                    assert np.array_equal(got.view(np.uint32), alone[j].view(np.uint32)), sl
     finally:
         session.set_gemm_mode("fp32")
         session.set_gemm_config(1, 1, 4)
