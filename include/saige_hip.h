@@ -383,7 +383,8 @@ int64_t saige_engine_system_bytes(const saige_ctx* ctx);
 /* Seconds of device time spent in the engines' GRM product launches of one
  * kind, and their count (HIP events around each launch). */
 #define SAIGE_LAUNCH_GEMV 0       /* width-1 engine (saige_pcg1): k_gemv          */
-#define SAIGE_LAUNCH_GEMM_HALF 1  /* block GEMM over one 32-column half            */
+#define SAIGE_LAUNCH_GEMM_HALF 1  /* block GEMM over 32 columns: one half, or (lower
+                                     triangle) at most 32 occupied slots anywhere */
 #define SAIGE_LAUNCH_GEMM_FULL 2  /* block GEMM over all 64 columns                */
 #define SAIGE_LAUNCH_GEMM_QUARTER 3 /* lower-triangle block GEMM over 16 columns
                                      (at most 16 occupied slots, anywhere)       */
