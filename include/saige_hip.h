@@ -353,8 +353,9 @@ int saige_get_gemm_config_quarter(const saige_ctx* ctx, int* rt, int* pd, double
  * covers 64, 32 or 16 columns, as few as hold live slots (at most 16 or 32
  * occupied slots run as one 16- or 32-column launch wherever they sit), with
  * the same per-column arithmetic (SAIGE_LAUNCH_GEMM_QUARTER counts the
- * 16-column ones, SAIGE_LAUNCH_GEMM_HALF the 32-column ones).  rt selects the workgroup (1: twelve waves, 2: eight), pd 1 or 3
- * the prefetch depth (one or two chunks ahead); nsplit is unused.  It needs device scratch for
+ * 16-column ones, SAIGE_LAUNCH_GEMM_HALF the 32-column ones).  rt selects
+ * the workgroup (1: twelve waves, 2: eight), pd 1 or 3 the prefetch depth
+ * (one or two chunks ahead); nsplit is unused.  It needs device scratch for
  * its partial products, about 128 N^2 / (32 * waves) bytes (3.3 GB at
  * N = 100k with twelve waves), allocated on first use.  Changing the mode
  * clears the tuned times (re-run saige_autotune).  Engine idle. */
